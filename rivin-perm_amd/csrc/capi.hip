@@ -1,0 +1,198 @@
+// extern "C" entry points of libpermhip.so (declared in include/permhip.h).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "device_context.h"
+#include "permhip.h"
+#include "phip_arith.h"
+#include "residues.h"
+
+namespace phip {
+int mod3_from_columns(DeviceContext* ctx, const uint64_t* mags, const uint64_t* sgns, int n);
+}
+
+using phip::DeviceContext;
+
+namespace {
+
+// Validation shared by the binary entry points, mirroring the reference's
+// native checks (perm_binary_omp.c:422-452, perm_binary_cuda.cu:314-340).
+bool validate_binary(const uint8_t* A, int n, const int64_t* primes, int k) {
+    if (n < 0 || n > 63) {
+        phip::set_error("native binary backend supports order at most 63");
+        return false;
+    }
+    if (n > 0 && !A) {
+        phip::set_error("matrix pointer is null");
+        return false;
+    }
+    if (k < 0 || (k > 0 && !primes)) {
+        phip::set_error("invalid prime list");
+        return false;
+    }
+    for (int i = 0; i < k; ++i) {
+        const int64_t p = primes[i];
+        if (p <= 1) {
+            phip::set_error("p must be greater than 1");
+            return false;
+        }
+        if (p == 2) {
+            phip::set_error("p must be odd");
+            return false;
+        }
+        if (p <= n) {
+            phip::set_error("p must be greater than the matrix order");
+            return false;
+        }
+        if (!phip::is_prime_u64((uint64_t)p)) {
+            phip::set_error("p must be prime");
+            return false;
+        }
+    }
+    for (int i = 0; i < n * n; ++i) {
+        if (A[i] > 1) {
+            phip::set_error("matrix entries must be 0 or 1");
+            return false;
+        }
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int phip_available(void) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess) return 0;
+    return count > 0 ? 1 : 0;
+}
+
+int phip_last_error(char* buf, size_t len) {
+    const std::string& e = phip::last_error();
+    if (buf && len > 0) {
+        const size_t m = e.size() < len - 1 ? e.size() : len - 1;
+        memcpy(buf, e.data(), m);
+        buf[m] = '\0';
+    }
+    return (int)e.size();
+}
+
+int phip_device_info(char* buf, size_t len) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        phip::set_error("no HIP device available");
+        return -1;
+    }
+    std::string s = std::string(prop.name) + " " + prop.gcnArchName + " CUs=" +
+                    std::to_string(prop.multiProcessorCount) + " clock_kHz=" + std::to_string(prop.clockRate);
+    if (buf && len > 0) {
+        const size_t m = s.size() < len - 1 ? s.size() : len - 1;
+        memcpy(buf, s.data(), m);
+        buf[m] = '\0';
+    }
+    return 0;
+}
+
+int phip_permanent_residues(const int64_t* A, int64_t prime_stride, int n, const int64_t* moduli, int k,
+                            int64_t* out) {
+    DeviceContext* ctx = DeviceContext::current();
+    if (!ctx) return -1;
+    std::lock_guard<std::mutex> lock(ctx->mutex());
+    return phip::compute_residues(ctx, A, prime_stride, n, moduli, k, out);
+}
+
+int64_t phip_permanent_modp(const int64_t* A, int n, int64_t p) {
+    int64_t r = -1;
+    if (phip_permanent_residues(A, 0, n, &p, 1, &r) != 0) return -1;
+    return r;
+}
+
+int64_t phip_schur_permanent_modp(int n, int64_t omega, int64_t p) {
+    if (n < 0 || n >= 64) {
+        phip::set_error("n must satisfy 0 <= n < 64");
+        return -1;
+    }
+    if (p <= 0) {
+        phip::set_error("p must be a positive signed 64-bit integer");
+        return -1;
+    }
+    if (n <= 1) return 1 % p;
+    if (p == 1) return 0;
+    const uint64_t P = (uint64_t)p;
+    const uint64_t w = (uint64_t)phip::host_floor_mod(omega, p);
+    // Cofactor identity for odd n when omega^n == 1 (perm_c_omp.c:964-1025):
+    // every permanental cofactor of the all-ones first row is equal.
+    const bool cofactor = n >= 3 && (n & 1) && (p & 1) && phip::host_powmod(w, (uint64_t)n, P) == 1 % P;
+    const int m = cofactor ? n - 1 : n;
+    const uint64_t first = cofactor ? w : 1 % P;  // omega^(row index) of the first row used
+    std::vector<int64_t> M((size_t)m * m);
+    uint64_t wj = first;
+    for (int r = 0; r < m; ++r) {
+        uint64_t pw = cofactor ? wj : 1 % P;
+        for (int c = 0; c < m; ++c) {
+            M[(size_t)r * m + c] = (int64_t)pw;
+            pw = phip::host_mulmod(pw, wj, P);
+        }
+        wj = phip::host_mulmod(wj, w, P);
+    }
+    int64_t r = -1;
+    if (phip_permanent_residues(M.data(), 0, m, &p, 1, &r) != 0) return -1;
+    if (cofactor) r = (int64_t)phip::host_mulmod((uint64_t)r, (uint64_t)n % P, P);
+    return r;
+}
+
+int phip_binary_permanent_residues(const uint8_t* A, int n, const int64_t* primes, int k, int64_t* out) {
+    if (!validate_binary(A, n, primes, k)) return -1;
+    if (k > 0 && !out) {
+        phip::set_error("output pointer is null");
+        return -1;
+    }
+    std::vector<int64_t> M((size_t)n * n);
+    for (int i = 0; i < n * n; ++i) M[i] = A[i];
+    return phip_permanent_residues(n ? M.data() : nullptr, 0, n, primes, k, out);
+}
+
+int64_t phip_binary_permanent_modp(const uint8_t* A, int n, int64_t p) {
+    int64_t r = -1;
+    if (phip_binary_permanent_residues(A, n, &p, 1, &r) != 0) return -1;
+    return r;
+}
+
+int phip_permanent_mod3(const uint64_t* mags, const uint64_t* sgns, int n) {
+    DeviceContext* ctx = DeviceContext::current();
+    if (!ctx) return -1;
+    std::lock_guard<std::mutex> lock(ctx->mutex());
+    return phip::mod3_from_columns(ctx, mags, sgns, n);
+}
+
+int phip_permanent_mod3_matrix(const int* A, int n) {
+    if (n < 0 || n >= 64) {
+        phip::set_error("matrix size must be less than 64");
+        return -1;
+    }
+    if (n == 0) return 1;
+    if (!A) {
+        phip::set_error("matrix pointer is null");
+        return -1;
+    }
+    std::vector<uint64_t> mags(n, 0), sgns(n, 0);
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < n; ++j) {
+            const int v = ((A[(size_t)i * n + j] % 3) + 3) % 3;
+            if (v) {
+                mags[j] |= 1ull << i;
+                if (v == 2) sgns[j] |= 1ull << i;
+            }
+        }
+    }
+    return phip_permanent_mod3(mags.data(), sgns.data(), n);
+}
+
+}  // extern "C"

@@ -1,0 +1,27 @@
+// Internal residue engine shared by the C-ABI entry points.
+#pragma once
+#include <stdint.h>
+
+#include <vector>
+
+#include "device_context.h"
+
+namespace phip {
+
+// Tunables (set through phip_set_tuning): target number of lanes per launch
+// for the register-resident Glynn kernel; minimum chunk bits.
+struct Tuning {
+    int lanes_log2 = 20;
+    int min_cbits = 4;
+};
+Tuning& tuning();
+
+// perm(A_i) mod moduli[i] for i < k, where A_i = A + i * stride (row-major
+// n x n int64).  Any 0 < m < 2^63.  Returns 0 or -1 (error set).
+int compute_residues(DeviceContext* ctx, const int64_t* A, int64_t stride, int n,
+                     const int64_t* moduli, int k, int64_t* out);
+
+// Deterministic Miller-Rabin for 64-bit values.
+bool is_prime_u64(uint64_t v);
+
+}  // namespace phip

@@ -1,0 +1,360 @@
+// Residue engine: splits each modulus m = 2^e * q into an odd part served by
+// the Glynn kernels and a power-of-two part served by the wrapping Ryser
+// kernel, prepares the per-modulus tables on the host, runs every odd
+// modulus of one arithmetic class in a single launch, and reassembles the
+// residues (CRT for even moduli).
+//
+// Host preparation follows the reference's normalisation of entries before
+// fixed-width arithmetic (perm_c_omp.c:912-916, normalize_mod_i64).
+#include "residues.h"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <string>
+
+#include "generic_kernels.h"
+#include "launchers.h"
+#include "phip_arith.h"
+
+namespace phip {
+
+Tuning& tuning() {
+    static Tuning t;
+    return t;
+}
+
+namespace {
+
+int bit_length(uint64_t v) { return v ? 64 - __builtin_clzll(v) : 0; }
+
+// Largest group size whose exact products fit the kernel's bounds given the
+// maximum absolute row sum S (S^G < 2^31, S^(G-1) < 2^23); 0 = wide rows.
+int choose_group(uint64_t S) {
+    const int b = bit_length(S);
+    if (b <= 7) return 4;
+    if (b <= 10) return 3;
+    if (b <= 15) return 2;
+    return 0;
+}
+
+int ceil_log2(uint64_t v) {
+    int l = 0;
+    while ((1ull << l) < v) ++l;
+    return l;
+}
+
+uint64_t pow2_mod(int e, uint64_t q) { return host_powmod(2, (uint64_t)e, q); }
+
+struct OddJob {
+    int index;   // position in the caller's modulus list
+    uint64_t q;  // odd part, > 1
+};
+
+const int64_t* matrix_of(const int64_t* A, int64_t stride, int index) { return A + (int64_t)index * stride; }
+
+// ---------------------------------------------------------------------
+// 31-bit Glynn (register-resident rows).
+// ---------------------------------------------------------------------
+int run_glynn31(DeviceContext* ctx, const int64_t* A, int64_t stride, int n, int group,
+                const std::vector<OddJob>& jobs, std::vector<uint64_t>& odd_residue) {
+    const int kk = (int)jobs.size();
+    const int NR = n;
+    std::vector<int32_t> rs0((size_t)kk * NR), deltas((size_t)kk * (n - 1) * NR), primes(kk);
+    std::vector<uint32_t> pinvs(kk);
+    std::vector<uint64_t> moduli(kk);
+    for (int s = 0; s < kk; ++s) {
+        const uint32_t q = (uint32_t)jobs[s].q;
+        const int64_t* M = matrix_of(A, stride, jobs[s].index);
+        primes[s] = (int32_t)q;
+        pinvs[s] = inverse_mod_2_32(q);
+        moduli[s] = q;
+        for (int i = 0; i < n; ++i) {
+            int64_t row = 0;
+            for (int j = 0; j < n; ++j) {
+                const int64_t r = host_floor_mod(M[(size_t)i * n + j], (int64_t)q);
+                int64_t v;
+                if (group == 0) {
+                    v = r;
+                    row = (row + r) % (int64_t)q;
+                } else {
+                    v = (r > (int64_t)(q / 2)) ? r - (int64_t)q : r;
+                    row += v;
+                }
+                if (j < n - 1) {
+                    const int64_t d = (group == 0) ? (2 * r) % (int64_t)q : 2 * v;
+                    deltas[((size_t)s * (n - 1) + j) * NR + i] = (int32_t)d;
+                }
+            }
+            rs0[(size_t)s * NR + i] = (int32_t)row;
+        }
+    }
+
+    int cbits = (n - 1) + ceil_log2((uint64_t)kk) - tuning().lanes_log2;
+    cbits = std::max(cbits, std::max(4, tuning().min_cbits));
+    cbits = std::min(cbits, std::min(n - 1, 30));
+    const uint64_t lanes = 1ull << (n - 1 - cbits);
+    const uint64_t gx = (lanes + kGlynnBlock - 1) / kGlynnBlock;
+    if (gx > 0x7fffffffull) {
+        set_error("internal: grid too large");
+        return -1;
+    }
+
+    hipStream_t st = ctx->stream();
+    auto* d_rs0 = (int32_t*)ctx->buffer(0, rs0.size() * 4);
+    auto* d_del = (int32_t*)ctx->buffer(1, deltas.size() * 4);
+    auto* d_pr = (int32_t*)ctx->buffer(2, primes.size() * 4);
+    auto* d_pi = (uint32_t*)ctx->buffer(3, pinvs.size() * 4);
+    auto* d_part = (uint64_t*)ctx->buffer(4, (size_t)kk * gx * 8);
+    auto* d_mod = (uint64_t*)ctx->buffer(5, moduli.size() * 8);
+    auto* d_sum = (uint64_t*)ctx->buffer(6, moduli.size() * 8);
+    auto* h_sum = (uint64_t*)ctx->host_buffer(0, moduli.size() * 8);
+    if (!d_rs0 || !d_del || !d_pr || !d_pi || !d_part || !d_mod || !d_sum || !h_sum) return -1;
+
+    PHIP_HIP_TRY(hipMemcpyAsync(d_rs0, rs0.data(), rs0.size() * 4, hipMemcpyHostToDevice, st));
+    PHIP_HIP_TRY(hipMemcpyAsync(d_del, deltas.data(), deltas.size() * 4, hipMemcpyHostToDevice, st));
+    PHIP_HIP_TRY(hipMemcpyAsync(d_pr, primes.data(), primes.size() * 4, hipMemcpyHostToDevice, st));
+    PHIP_HIP_TRY(hipMemcpyAsync(d_pi, pinvs.data(), pinvs.size() * 4, hipMemcpyHostToDevice, st));
+    PHIP_HIP_TRY(hipMemcpyAsync(d_mod, moduli.data(), moduli.size() * 8, hipMemcpyHostToDevice, st));
+
+    Glynn31Args args{d_rs0, d_del, d_pr, d_pi, n, cbits, lanes, d_part};
+    PHIP_HIP_TRY(launch_glynn31(group, NR, dim3((unsigned)gx, (unsigned)kk), st, args));
+    hipLaunchKernelGGL(sum_mod_kernel, dim3(kk), dim3(256), 0, st, d_part, gx, d_mod, d_sum);
+    PHIP_HIP_TRY(hipGetLastError());
+    PHIP_HIP_TRY(hipMemcpyAsync(h_sum, d_sum, (size_t)kk * 8, hipMemcpyDeviceToHost, st));
+    PHIP_HIP_TRY(hipStreamSynchronize(st));
+
+    const int groups = (group == 0) ? NR : (NR + group - 1) / group;
+    for (int s = 0; s < kk; ++s) {
+        const uint64_t q = jobs[s].q;
+        const uint64_t r_mod_q = ((1ull << 32) % q);
+        const uint64_t inv2 = (q + 1) / 2;
+        const uint64_t scale = host_mulmod(host_powmod(r_mod_q, (uint64_t)(groups - 1), q),
+                                           host_powmod(inv2, (uint64_t)(n - 1), q), q);
+        odd_residue[jobs[s].index] = host_mulmod(h_sum[s] % q, scale, q);
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------
+// Generic 64-bit Montgomery Glynn (any odd q < 2^63, any order >= 2).
+// ---------------------------------------------------------------------
+int run_glynn63(DeviceContext* ctx, const int64_t* A, int64_t stride, int n,
+                const std::vector<OddJob>& jobs, std::vector<uint64_t>& odd_residue) {
+    const int kk = (int)jobs.size();
+    std::vector<uint64_t> rs0((size_t)kk * n), deltas((size_t)kk * (n - 1) * n), moduli(kk), nprimes(kk);
+    for (int s = 0; s < kk; ++s) {
+        const uint64_t q = jobs[s].q;
+        const int64_t* M = matrix_of(A, stride, jobs[s].index);
+        moduli[s] = q;
+        nprimes[s] = 0 - inverse_mod_2_64(q);
+        for (int i = 0; i < n; ++i) {
+            uint64_t row = 0;
+            for (int j = 0; j < n; ++j) {
+                const uint64_t r = (uint64_t)host_floor_mod(M[(size_t)i * n + j], (int64_t)q);
+                row = add_mod_u64(row, r, q);
+                if (j < n - 1) deltas[((size_t)s * (n - 1) + j) * n + i] = add_mod_u64(r, r, q);
+            }
+            rs0[(size_t)s * n + i] = row;
+        }
+    }
+    const int cbits = std::max(0, (n - 1) - 14);
+    const uint64_t lanes = 1ull << (n - 1 - cbits);
+    const uint64_t gx = (lanes + kGenericBlock - 1) / kGenericBlock;
+
+    hipStream_t st = ctx->stream();
+    auto* d_rs0 = (uint64_t*)ctx->buffer(7, rs0.size() * 8);
+    auto* d_del = (uint64_t*)ctx->buffer(8, deltas.size() * 8);
+    auto* d_mod = (uint64_t*)ctx->buffer(9, moduli.size() * 8);
+    auto* d_np = (uint64_t*)ctx->buffer(10, nprimes.size() * 8);
+    auto* d_part = (uint64_t*)ctx->buffer(11, (size_t)kk * lanes * 8);
+    auto* d_sum = (uint64_t*)ctx->buffer(12, (size_t)kk * 8);
+    auto* h_sum = (uint64_t*)ctx->host_buffer(1, (size_t)kk * 8);
+    if (!d_rs0 || !d_del || !d_mod || !d_np || !d_part || !d_sum || !h_sum) return -1;
+    PHIP_HIP_TRY(hipMemcpyAsync(d_rs0, rs0.data(), rs0.size() * 8, hipMemcpyHostToDevice, st));
+    PHIP_HIP_TRY(hipMemcpyAsync(d_del, deltas.data(), deltas.size() * 8, hipMemcpyHostToDevice, st));
+    PHIP_HIP_TRY(hipMemcpyAsync(d_mod, moduli.data(), moduli.size() * 8, hipMemcpyHostToDevice, st));
+    PHIP_HIP_TRY(hipMemcpyAsync(d_np, nprimes.data(), nprimes.size() * 8, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(glynn63_kernel, dim3((unsigned)gx, (unsigned)kk), dim3(kGenericBlock), 0, st, d_rs0,
+                       d_del, d_mod, d_np, n, cbits, lanes, d_part);
+    PHIP_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(sum_mod_kernel, dim3(kk), dim3(256), 0, st, d_part, lanes, d_mod, d_sum);
+    PHIP_HIP_TRY(hipGetLastError());
+    PHIP_HIP_TRY(hipMemcpyAsync(h_sum, d_sum, (size_t)kk * 8, hipMemcpyDeviceToHost, st));
+    PHIP_HIP_TRY(hipStreamSynchronize(st));
+    for (int s = 0; s < kk; ++s) {
+        const uint64_t q = jobs[s].q;
+        const uint64_t r_mod_q = (uint64_t)((((unsigned __int128)1) << 64) % q);
+        const uint64_t inv2 = (q + 1) / 2;
+        const uint64_t scale = host_mulmod(host_powmod(r_mod_q, (uint64_t)(n - 1), q),
+                                           host_powmod(inv2, (uint64_t)(n - 1), q), q);
+        odd_residue[jobs[s].index] = host_mulmod(h_sum[s], scale, q);
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------
+// Ryser modulo 2^64.
+// ---------------------------------------------------------------------
+int run_ryser_u64(DeviceContext* ctx, const int64_t* M, int n, uint64_t* result) {
+    std::vector<uint64_t> cols((size_t)n * n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) cols[(size_t)j * n + i] = (uint64_t)M[(size_t)i * n + j];
+    const int cbits = std::max(0, n - 14);
+    const uint64_t lanes = 1ull << (n - cbits);
+    const uint64_t gx = (lanes + kGenericBlock - 1) / kGenericBlock;
+    hipStream_t st = ctx->stream();
+    auto* d_a = (uint64_t*)ctx->buffer(13, cols.size() * 8);
+    auto* d_part = (uint64_t*)ctx->buffer(14, lanes * 8);
+    auto* d_sum = (uint64_t*)ctx->buffer(15, 16);
+    auto* h_sum = (uint64_t*)ctx->host_buffer(2, 16);
+    if (!d_a || !d_part || !d_sum || !h_sum) return -1;
+    const uint64_t zero = 0;
+    PHIP_HIP_TRY(hipMemcpyAsync(d_a, cols.data(), cols.size() * 8, hipMemcpyHostToDevice, st));
+    PHIP_HIP_TRY(hipMemcpyAsync(d_sum + 1, &zero, 8, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(ryser_u64_kernel, dim3((unsigned)gx), dim3(kGenericBlock), 0, st, d_a, n, cbits,
+                       lanes, d_part);
+    PHIP_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(sum_mod_kernel, dim3(1), dim3(256), 0, st, d_part, lanes, d_sum + 1, d_sum);
+    PHIP_HIP_TRY(hipGetLastError());
+    PHIP_HIP_TRY(hipMemcpyAsync(h_sum, d_sum, 8, hipMemcpyDeviceToHost, st));
+    PHIP_HIP_TRY(hipStreamSynchronize(st));
+    // Ryser's (-1)^n factor (perm_c_omp.c:767-770).
+    *result = (n & 1) ? (uint64_t)0 - h_sum[0] : h_sum[0];
+    return 0;
+}
+
+}  // namespace
+
+bool is_prime_u64(uint64_t v) {
+    static const uint64_t small[] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+    if (v < 2) return false;
+    for (uint64_t p : small) {
+        if (v == p) return true;
+        if (v % p == 0) return false;
+    }
+    uint64_t d = v - 1;
+    int r = 0;
+    while (!(d & 1)) {
+        d >>= 1;
+        ++r;
+    }
+    // Bases proven sufficient for every 64-bit integer (Sinclair's set).
+    static const uint64_t bases[] = {2, 325, 9375, 28178, 450775, 9780504, 1795265022};
+    for (uint64_t a : bases) {
+        a %= v;
+        if (a == 0) continue;
+        uint64_t x = host_powmod(a, d, v);
+        if (x == 1 || x == v - 1) continue;
+        bool composite = true;
+        for (int i = 1; i < r; ++i) {
+            x = host_mulmod(x, x, v);
+            if (x == v - 1) {
+                composite = false;
+                break;
+            }
+        }
+        if (composite) return false;
+    }
+    return true;
+}
+
+int compute_residues(DeviceContext* ctx, const int64_t* A, int64_t stride, int n, const int64_t* moduli,
+                     int k, int64_t* out) {
+    if (n < 0 || n >= 64) {
+        set_error("matrix dimension must satisfy 0 <= n < 64");
+        return -1;
+    }
+    if (k < 0 || (k > 0 && (!moduli || !out))) {
+        set_error("invalid modulus list");
+        return -1;
+    }
+    if (n > 0 && !A) {
+        set_error("matrix pointer is null");
+        return -1;
+    }
+    for (int i = 0; i < k; ++i) {
+        if (moduli[i] <= 0) {
+            set_error("moduli must be positive signed 64-bit integers");
+            return -1;
+        }
+    }
+    if (k == 0) return 0;
+    if (n == 0) {
+        for (int i = 0; i < k; ++i) out[i] = 1 % moduli[i];
+        return 0;
+    }
+    if (n == 1) {
+        for (int i = 0; i < k; ++i) out[i] = host_floor_mod(matrix_of(A, stride, i)[0], moduli[i]);
+        return 0;
+    }
+
+    std::vector<uint64_t> odd_residue(k, 0);
+    std::map<int, std::vector<OddJob>> by_group;  // 31-bit class, keyed by group size
+    std::vector<OddJob> wide;                     // 63-bit class
+    for (int i = 0; i < k; ++i) {
+        const uint64_t m = (uint64_t)moduli[i];
+        const int e = __builtin_ctzll(m);
+        const uint64_t q = m >> e;
+        if (q <= 1) continue;
+        if (q < (1ull << 31) && n >= kGlynnMinOrder && n <= kGlynnMaxOrder) {
+            // Maximum absolute row sum of the centred residues picks the
+            // row-sum representation.
+            const int64_t* M = matrix_of(A, stride, i);
+            uint64_t S = 0;
+            for (int r = 0; r < n; ++r) {
+                uint64_t row = 0;
+                for (int c = 0; c < n; ++c) {
+                    int64_t v = host_floor_mod(M[(size_t)r * n + c], (int64_t)q);
+                    if (v > (int64_t)(q / 2)) v -= (int64_t)q;
+                    row += (uint64_t)(v < 0 ? -v : v);
+                }
+                S = std::max(S, row);
+            }
+            by_group[choose_group(S)].push_back({i, q});
+        } else {
+            wide.push_back({i, q});
+        }
+    }
+    for (auto& kv : by_group) {
+        if (run_glynn31(ctx, A, stride, n, kv.first, kv.second, odd_residue) != 0) return -1;
+    }
+    if (!wide.empty() && run_glynn63(ctx, A, stride, n, wide, odd_residue) != 0) return -1;
+
+    bool have_two = false;
+    uint64_t two_residue = 0;
+    for (int i = 0; i < k; ++i) {
+        const uint64_t m = (uint64_t)moduli[i];
+        const int e = __builtin_ctzll(m);
+        const uint64_t q = m >> e;
+        if (e == 0) {
+            out[i] = (int64_t)odd_residue[i];
+            continue;
+        }
+        uint64_t r2;
+        if (stride == 0) {
+            if (!have_two) {
+                if (run_ryser_u64(ctx, A, n, &two_residue) != 0) return -1;
+                have_two = true;
+            }
+            r2 = two_residue;
+        } else {
+            if (run_ryser_u64(ctx, matrix_of(A, stride, i), n, &r2) != 0) return -1;
+        }
+        const uint64_t pow2 = 1ull << e;  // e <= 62
+        r2 &= pow2 - 1;
+        if (q == 1) {
+            out[i] = (int64_t)r2;
+            continue;
+        }
+        // x = r2 + 2^e * t with t = (r_q - r2) * 2^-e mod q.
+        const uint64_t rq = odd_residue[i];
+        const uint64_t diff = (rq + q - (r2 % q)) % q;
+        const uint64_t inv = host_powmod((q + 1) / 2, (uint64_t)e, q);
+        const uint64_t t = host_mulmod(diff, inv, q);
+        out[i] = (int64_t)(r2 + pow2 * t);
+    }
+    return 0;
+}
+
+}  // namespace phip

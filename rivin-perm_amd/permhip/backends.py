@@ -1,0 +1,254 @@
+"""Backend objects shaped like the reference's native extensions.
+
+The reference selects its residue backend through module globals
+(``perm.HAS_C_EXT`` / ``perm.perm_c_ext``, perm.py:130-136;
+``perm_binary._binary_ext``, perm_binary.py:23-32;
+``perm_mod3.perm_mod3_omp``, perm_mod3.py:24-30).  The classes below expose
+the same method names, argument meaning, validation order and exception
+types as those CPython extensions, but compute every residue on the GPU
+through ``libpermhip.so``.  They can be assigned to the reference's globals
+as a drop-in, and the mirror modules in this package use them directly.
+
+There is no CPU fallback: without a HIP device every residue call raises.
+"""
+
+from __future__ import annotations
+
+import operator
+from numbers import Integral
+
+import numpy as np
+
+from . import _native
+
+_INT64_MIN = -(1 << 63)
+_INT64_MAX = (1 << 63) - 1
+_INT32_MAX = (1 << 31) - 1
+
+
+def _parse_long_long(value, what: str = "argument") -> int:
+    """CPython 'L' conversion: __index__ required, int64 range enforced."""
+    if isinstance(value, float):
+        raise TypeError(f"{what} must be an integer, not float")
+    try:
+        v = operator.index(value)
+    except TypeError as exc:
+        raise TypeError(f"{what} must be an integer") from exc
+    if not _INT64_MIN <= v <= _INT64_MAX:
+        raise OverflowError("Python int too large to convert to C long long")
+    return int(v)
+
+
+def _parse_int(value, what: str = "argument") -> int:
+    """CPython 'i' conversion (32-bit signed)."""
+    if isinstance(value, float):
+        raise TypeError(f"{what} must be an integer, not float")
+    try:
+        v = operator.index(value)
+    except TypeError as exc:
+        raise TypeError(f"{what} must be an integer") from exc
+    if v > _INT32_MAX:
+        raise OverflowError("signed integer is greater than maximum")
+    if v < -_INT32_MAX - 1:
+        raise OverflowError("signed integer is less than minimum")
+    return int(v)
+
+
+class HipPermanentExtension:
+    """GPU counterpart of ``perm_c_ext`` (perm_c_omp.c:1103-1122)."""
+
+    __name__ = "permhip.perm_c_ext"
+
+    def _matrix(self, matrix, p: int) -> np.ndarray:
+        # Mirrors py_matrix_permanent_mod_p's checks (perm_c_omp.c:858-916).
+        if not isinstance(matrix, list):
+            raise TypeError("matrix must be a list of lists")
+        if p <= 0:
+            raise ValueError("p must be a positive signed 64-bit integer")
+        n = len(matrix)
+        if n >= 64:
+            raise ValueError("matrix dimension must be less than 64")
+        out = np.empty((n, n), dtype=np.int64)
+        for i, row in enumerate(matrix):
+            if not isinstance(row, list) or len(row) != n:
+                raise ValueError("matrix must be square")
+            for j, item in enumerate(row):
+                out[i, j] = _parse_long_long(item, "matrix entry")
+        return out
+
+    def _residue(self, args, what: str) -> int:
+        if len(args) < 2 or len(args) > 3:
+            raise TypeError(f"{what}() takes 2 or 3 arguments")
+        matrix = args[0]
+        p = _parse_long_long(args[1], "p")
+        if len(args) == 3:
+            _parse_int(args[2], "num_threads")
+        a = self._matrix(matrix, p)
+        if a.shape[0] == 0:
+            return 1 % p
+        return _native.permanent_residues(a, [p])[0]
+
+    def permanent_mod_p(self, *args) -> int:
+        """perm(matrix) mod p for 0 < p < 2^63; num_threads is accepted and
+        ignored (the GPU uses the whole device)."""
+        return self._residue(args, "permanent_mod_p")
+
+    def ryser_permanent_mod_p(self, *args) -> int:
+        """Same residue as permanent_mod_p (the algorithm is internal)."""
+        return self._residue(args, "ryser_permanent_mod_p")
+
+    def schur_permanent_mod_p(self, *args) -> int:
+        """perm(S_n) mod p for S_n[j][k] = omega^(jk) (perm_c_omp.c:942-1060)."""
+        if len(args) < 3 or len(args) > 4:
+            raise TypeError("schur_permanent_mod_p() takes 3 or 4 arguments")
+        n = _parse_int(args[0], "n")
+        omega = _parse_long_long(args[1], "omega")
+        p = _parse_long_long(args[2], "p")
+        if len(args) == 4:
+            _parse_int(args[3], "num_threads")
+        if n < 0 or n >= 64:
+            raise ValueError("n must satisfy 0 <= n < 64")
+        if p <= 0:
+            raise ValueError("p must be a positive signed 64-bit integer")
+        if n <= 1:
+            return 1 % p
+        if p == 1:
+            return 0
+        return _native.schur_permanent_modp(n, omega, p)
+
+    def get_max_threads(self) -> int:
+        """Parallel width reported to thread-count heuristics: the device's
+        compute units, so callers never fan residues out over processes."""
+        info = _native.device_info()
+        for token in info.split():
+            if token.startswith("CUs="):
+                return max(1, int(token[4:]))
+        return 1
+
+    def set_num_threads(self, num_threads) -> None:
+        n = _parse_int(num_threads, "num_threads")
+        if n <= 0:
+            raise ValueError("num_threads must be positive")
+
+
+def _index_long_long(obj, name: str) -> int:
+    # parse_index_long_long (perm_binary_omp.c:367-384).
+    if isinstance(obj, (bool, np.bool_)):
+        raise TypeError(f"{name} must be an integer")
+    try:
+        value = operator.index(obj)
+    except TypeError as exc:
+        raise TypeError(f"{name} must be an integer") from exc
+    if not _INT64_MIN <= value <= _INT64_MAX:
+        raise OverflowError("Python int too large to convert to C long long")
+    return int(value)
+
+
+class HipBinaryExtension:
+    """GPU counterpart of ``perm_binary_ext`` (perm_binary_omp.c:386-534)."""
+
+    __name__ = "permhip.perm_binary_ext"
+
+    def binary_permanent_mod_p(self, matrix, p, num_threads=None) -> int:
+        p = _index_long_long(p, "p")
+        if num_threads is not None:
+            threads = _index_long_long(num_threads, "num_threads")
+            if threads > _INT32_MAX:
+                raise ValueError("num_threads is too large")
+            if threads < 0:
+                raise ValueError("num_threads must be non-negative")
+        try:
+            rows = list(matrix)
+        except TypeError as exc:
+            raise TypeError("matrix must be a sequence of rows") from exc
+        n = len(rows)
+        if n > 63:
+            raise ValueError("native binary backend supports order at most 63")
+        if p <= 1:
+            raise ValueError("p must be greater than 1")
+        if p == 2:
+            raise ValueError("p must be odd")
+        if p <= n:
+            raise ValueError("p must be greater than the matrix order")
+        if not _is_prime_u64(p):
+            raise ValueError("p must be prime")
+        a = np.zeros((n, n), dtype=np.uint8)
+        for i, row in enumerate(rows):
+            try:
+                items = list(row)
+            except TypeError as exc:
+                raise TypeError("matrix rows must be sequences") from exc
+            if len(items) != n:
+                raise ValueError("matrix must be square")
+            for j, entry in enumerate(items):
+                value = _parse_long_long(entry, "matrix entry")
+                if value not in (0, 1):
+                    raise ValueError("matrix entries must be 0 or 1")
+                a[i, j] = value
+        if n == 0:
+            return 1
+        if n == 1:
+            return int(a[0, 0])
+        if not a.any(axis=0).all() or not a.any(axis=1).all():
+            return 0
+        return _native.binary_permanent_residues(a, [p])[0]
+
+    def get_max_threads(self) -> int:
+        return HipPermanentExtension().get_max_threads()
+
+
+def _is_prime_u64(value: int) -> bool:
+    """Deterministic Miller--Rabin for 64-bit values (bases of
+    perm_binary.py:764-793)."""
+    if value < 2:
+        return False
+    for small in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+        if value == small:
+            return True
+        if value % small == 0:
+            return False
+    d, r = value - 1, 0
+    while d % 2 == 0:
+        d //= 2
+        r += 1
+    for base in (2, 325, 9375, 28178, 450775, 9780504, 1795265022):
+        base %= value
+        if base == 0:
+            continue
+        x = pow(base, d, value)
+        if x in (1, value - 1):
+            continue
+        for _ in range(r - 1):
+            x = x * x % value
+            if x == value - 1:
+                break
+        else:
+            return False
+    return True
+
+
+class HipMod3Extension:
+    """GPU counterpart of ``perm_mod3_omp`` (perm_mod3_omp.c:220-313)."""
+
+    __name__ = "permhip.perm_mod3_omp"
+
+    def permanent_mod3(self, matrix, num_threads=-1) -> int:
+        if not isinstance(matrix, np.ndarray):
+            raise TypeError("argument 1 must be numpy.ndarray")
+        _parse_int(num_threads, "num_threads")
+        if matrix.ndim != 2:
+            raise ValueError("Matrix must be 2-dimensional")
+        if matrix.shape[0] != matrix.shape[1]:
+            raise ValueError("Matrix must be square")
+        if matrix.shape[0] >= 64:
+            raise ValueError("Matrix size must be less than 64")
+        n = int(matrix.shape[0])
+        if n == 0:
+            return 1
+        reduced = np.asarray(matrix, dtype=np.int64) % 3
+        return _native.permanent_mod3(reduced)
+
+
+perm_c_ext = HipPermanentExtension()
+perm_binary_ext = HipBinaryExtension()
+perm_mod3_omp = HipMod3Extension()

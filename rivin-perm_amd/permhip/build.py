@@ -1,0 +1,93 @@
+"""Build recipe for libpermhip.so (HIP kernels for gfx950 plus the C ABI).
+
+Compiles every translation unit under ``csrc/`` with hipcc into
+``_build/`` and links ``permhip/libpermhip.so`` in-tree, so the library
+travels with the source tree.  The 31-bit Glynn instantiations are split
+into one object per row-sum group size so they compile in parallel.
+
+Usage: ``python -m permhip.build`` from ``rivin-perm_amd/`` or
+``permhip.build.build()``.
+"""
+
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+PACKAGE_DIR = Path(__file__).resolve().parent
+ROOT = PACKAGE_DIR.parent
+CSRC = ROOT / "csrc"
+INCLUDE = ROOT.parent / "include"
+BUILD_DIR = ROOT / "_build"
+LIBRARY = PACKAGE_DIR / "libpermhip.so"
+
+ARCH = "gfx950"
+GLYNN_GROUPS = (0, 2, 3, 4)
+PLAIN_SOURCES = ("device_context.hip", "residues.hip", "mod3.hip", "capi.hip")
+
+
+def _hipcc() -> str:
+    explicit = os.environ.get("HIPCC")
+    if explicit:
+        return explicit
+    for candidate in ("/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if candidate and Path(candidate).exists():
+            return candidate
+    raise RuntimeError("hipcc not found; set HIPCC")
+
+
+def _flags() -> list:
+    return [
+        f"--offload-arch={ARCH}",
+        "-O3",
+        "-fPIC",
+        "-std=c++17",
+        # Packed FP32 instructions are kept out of every code object.
+        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        "-I", str(CSRC),
+        "-I", str(INCLUDE),
+    ]
+
+
+def _compile(args):
+    source, obj, defines = args
+    cmd = [_hipcc(), *_flags(), *defines, "-c", str(source), "-o", str(obj)]
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(
+            f"hipcc failed for {source.name}:\n{' '.join(cmd)}\n{proc.stdout}\n{proc.stderr}"
+        )
+    return obj
+
+
+def build(jobs: int | None = None, verbose: bool = False) -> Path:
+    """Compile all sources and link ``libpermhip.so``; returns its path."""
+    BUILD_DIR.mkdir(parents=True, exist_ok=True)
+    units = []
+    for group in GLYNN_GROUPS:
+        units.append(
+            (CSRC / "glynn31_inst.hip", BUILD_DIR / f"glynn31_g{group}.o", [f"-DPHIP_GROUP={group}"])
+        )
+    for name in PLAIN_SOURCES:
+        units.append((CSRC / name, BUILD_DIR / (Path(name).stem + ".o"), []))
+    workers = jobs or min(len(units), max(1, min(16, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        objects = list(pool.map(_compile, units))
+    tmp = LIBRARY.with_suffix(".so.tmp")
+    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objects), "-o", str(tmp)]
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{proc.stdout}\n{proc.stderr}")
+    os.replace(tmp, LIBRARY)
+    if verbose:
+        print(f"built {LIBRARY}")
+    return LIBRARY
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

@@ -1,0 +1,138 @@
+"""Device residues through the C ABI against the oracle (bit-exact).
+
+Every case names its seed, order and moduli so a failure identifies itself.
+"""
+
+import random
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+INT64_MIN = -(1 << 63)
+INT64_MAX = (1 << 63) - 1
+
+PRIMES_31 = [1073741827, 1073741831, 1073741833, 2147483647, 2147483629]
+ODD_SMALL = [3, 5, 7, 101, 65537]
+ODD_COMPOSITE = [9, 15, 45, 3 * 5 * 7 * 11 * 13 * 17]
+EVEN = [2, 4, 6, 12, 1 << 20, (1 << 40) * 3, 1 << 62]
+WIDE = [(1 << 61) - 1, 3_037_000_499, 3_037_000_501, INT64_MAX - 24, INT64_MAX]
+ALL_MODULI = PRIMES_31 + ODD_SMALL + ODD_COMPOSITE + EVEN + WIDE + [1]
+
+
+def _random_matrix(rng, n, lo, hi):
+    return [[rng.randrange(lo, hi + 1) for _ in range(n)] for _ in range(n)]
+
+
+@pytest.mark.parametrize("n", list(range(0, 15)))
+def test_all_moduli_small_orders(native, oracle_c, n):
+    rng = random.Random(20261203 + n)
+    A = _random_matrix(rng, n, -10, 10)
+    got = native.permanent_residues(np.array(A, dtype=np.int64).reshape(n, n), ALL_MODULI)
+    for m, r in zip(ALL_MODULI, got):
+        assert r == oracle_c.permanent_mod_p(A, m, 4), (n, m)
+
+
+@pytest.mark.parametrize(
+    "n,lo,hi",
+    [
+        (16, -1, 1),      # row sums < 2^7: groups of 4
+        (17, -10, 10),    # groups of 3
+        (18, -100, 100),  # groups of 2
+        (19, -(10 ** 6), 10 ** 6),  # wide rows
+        (20, -10, 10),
+        (21, 0, 1),
+        (22, -50, 50),
+    ],
+)
+def test_row_sum_representations(native, oracle_c, n, lo, hi):
+    rng = random.Random(1000 + n)
+    A = _random_matrix(rng, n, lo, hi)
+    moduli = PRIMES_31 + [101, 9]
+    got = native.permanent_residues(np.array(A, dtype=np.int64), moduli)
+    for m, r in zip(moduli, got):
+        assert r == oracle_c.permanent_mod_p(A, m, 8), (n, lo, hi, m)
+
+
+@pytest.mark.parametrize("n", [23, 24, 25])
+def test_larger_orders_against_openmp_oracle(native, oracle_c, n):
+    rng = random.Random(77 + n)
+    A = _random_matrix(rng, n, -10, 10)
+    moduli = [1073741827, 2147483647]
+    got = native.permanent_residues(np.array(A, dtype=np.int64), moduli)
+    for m, r in zip(moduli, got):
+        assert r == oracle_c.permanent_mod_p(A, m, 8), (n, m)
+
+
+def test_int64_extremes(native, oracle_c):
+    A = [[INT64_MIN, INT64_MAX, -1, 5, 7], [INT64_MAX, INT64_MIN, 1, 0, 3],
+         [0, -1, INT64_MAX, 2, 2], [1, 1, 1, 1, 1], [INT64_MIN, 0, 0, 0, 9]]
+    got = native.permanent_residues(np.array(A, dtype=np.int64), ALL_MODULI)
+    for m, r in zip(ALL_MODULI, got):
+        assert r == oracle_c.permanent_mod_p(A, m, 2), m
+
+
+def test_per_modulus_matrices_stride(native, oracle_c):
+    """k x n x n input: each modulus gets its own (already reduced) matrix."""
+    rng = random.Random(5)
+    n = 12
+    moduli = [1073741827, 101, 2147483647]
+    mats = [_random_matrix(rng, n, 0, m - 1) for m in moduli]
+    got = native.permanent_residues(np.array(mats, dtype=np.int64), moduli)
+    for M, m, r in zip(mats, moduli, got):
+        assert r == oracle_c.permanent_mod_p(M, m, 4)
+
+
+def test_schur_entry_point(native, oracle_c):
+    from permhip import planner
+
+    for n, p in [(3, 7), (5, 11), (7, 29), (11, 23), (13, 53), (9, 19), (15, 31), (17, 103)]:
+        omega = planner.nth_root_of_unity_mod_p(n, p)
+        S = planner.build_schur_matrix_mod_p(n, omega, p)
+        assert native.schur_permanent_modp(n, omega, p) == oracle_c.permanent_mod_p(S, p, 2), (n, p)
+    # omega that is not an n-th root of unity takes the full-matrix path.
+    S = planner.build_schur_matrix_mod_p(6, 5, 101)
+    assert native.schur_permanent_modp(6, 5, 101) == oracle_c.permanent_mod_p(S, 101, 2)
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 8, 13, 20, 24])
+def test_binary_residues(native, oracle_c, n):
+    rng = np.random.default_rng(20260815 + n)
+    B = rng.integers(0, 2, size=(n, n), dtype=np.uint8)
+    primes = [101, 2147483647, 2147483629, 2147483659, 9223372036854775783]
+    primes = [p for p in primes if p > n]
+    got = native.binary_permanent_residues(B, primes)
+    for p, r in zip(primes, got):
+        assert r == oracle_c.binary_mod_p(B, p, 8), (n, p)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 8, 16, 21, 24, 31, 32, 33])
+def test_mod3(native, oracle_c, n):
+    rng = np.random.default_rng(3 + n)
+    A = rng.integers(-5, 6, size=(n, n), dtype=np.int64)
+    assert native.permanent_mod3(A % 3) == oracle_c.mod3(A, 8), n
+
+
+def test_abi_rejections_do_not_poison_state(native):
+    import ctypes
+
+    lib = native.load()
+    eye = np.eye(5, dtype=np.uint8)
+    ptr = eye.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+    null = ctypes.POINTER(ctypes.c_uint8)()
+    assert lib.phip_binary_permanent_modp(ptr, 5, 9) == -1
+    assert lib.phip_binary_permanent_modp(ptr, 5, 5) == -1
+    assert lib.phip_binary_permanent_modp(ptr, -1, 101) == -1
+    assert lib.phip_binary_permanent_modp(ptr, 64, 101) == -1
+    assert lib.phip_binary_permanent_modp(null, 5, 101) == -1
+    bad = eye.copy()
+    bad[0, 0] = 2
+    assert lib.phip_binary_permanent_modp(bad.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), 5, 101) == -1
+    assert lib.phip_binary_permanent_modp(null, 0, 3) == 1
+    assert lib.phip_binary_permanent_modp(ptr, 5, 101) == 1
+    a = np.eye(6, dtype=np.int64)
+    iptr = a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    assert lib.phip_permanent_modp(iptr, 6, 0) == -1
+    assert lib.phip_permanent_modp(iptr, 64, 101) == -1
+    assert lib.phip_permanent_modp(iptr, 6, 101) == 1
