@@ -1,0 +1,319 @@
+#!/usr/bin/env python3
+"""Benchmark of the exact modular permanent path on one MI355X.
+
+Workload (one step): the exact permanent of the 32 x 32 integer matrix
+A = default_rng(20261203).integers(-10, 11), i.e. 8 Glynn residues modulo the
+ascending 31-bit primes from 2^30 (2^31 Gray states each) plus host CRT.  The
+matrix and its tables are resident in HBM (a device plan created before the
+timed region); a step replays the captured launch and reconstructs the
+integer.
+
+Printed as one JSON line (rank 0 only; this build runs on one GPU):
+  value        algorithmic Gray-code mulmods per second, n 2^(n-1) per
+               residue (the reference's n-factor row product per state)
+  roofline     the dominant kernel against the VALU issue bound of its own
+               instruction mix (rates measured live by a microbenchmark)
+  cpu_baseline the reference's own OpenMP kernel (perm_c_omp.c, compiled
+               from its sources into oracle/_ref) on a bounded sample
+  plain / batch / serve   the per-matrix plain launch, the 4096-matrix
+               uneven batch in one persistent launch, and the served stream
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "rivin-perm_amd"))
+
+METRIC = "Gray-code mulmods/s & exact perms/s at n=32 vs plain launch; batch p99 latency"
+HEADLINE_SEED = 20261203
+HEADLINE_N = 32
+BATCH_SEED = 4
+
+
+def headline_matrix() -> np.ndarray:
+    return np.random.default_rng(HEADLINE_SEED).integers(-10, 11, (HEADLINE_N, HEADLINE_N)).astype(np.int64)
+
+
+def batch_matrices(count: int = 4096):
+    rng = np.random.default_rng(BATCH_SEED)
+    mats = []
+    for _ in range(count):
+        n = int(rng.integers(12, 29))
+        mats.append(rng.integers(-100, 101, (n, n)).astype(np.int64))
+    return mats
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def instruction_mix(n: int, group: int) -> dict:
+    """Minimal VALU instructions per Gray state of the Glynn kernel."""
+    groups = n if group == 0 else -(-n // group)
+    mix = {"v_add_u32": 3 + (3 * n if group == 0 else n)}
+    if group:
+        mix["v_mul_i32_i24"] = n - groups
+    mix["v_mad_i64_i32"] = 2 * (groups - 1)
+    mix["v_mul_lo_u32"] = groups - 1
+    return mix
+
+
+def roofline(n: int, group: int, rates: dict, mulmods_per_s: float) -> dict:
+    """Mulmod roofline of the kernel's own instruction mix at measured rates."""
+    mix = instruction_mix(n, group)
+    seconds_per_state_lane = sum(count / rates[name] for name, count in mix.items())
+    peak_states = 1.0 / seconds_per_state_lane  # Gray states per second, whole device
+    peak = n * peak_states
+    return {
+        "bound": "valu",
+        "achieved": mulmods_per_s / 1e12,
+        "peak": peak / 1e12,
+        "unit": "Tmulmod/s",
+        "frac": mulmods_per_s / peak,
+        "traffic": traffic_from_profiles(),
+        "instructions_per_state": mix,
+        # Every mulmod as one signed Montgomery product plus one row-sum add.
+        "general_montgomery_peak": 1.0 / (1.0 / rates["mont31_product"] + 1.0 / rates["v_add_u32"]) / 1e12,
+    }
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the headline kernel from the committed rocprof
+    counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), else None."""
+    path = ROOT / "profiles" / "glynn_headline_n32_pmc.txt"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("hbm_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1])
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(matrix: np.ndarray, primes, threads: int) -> dict:
+    """Reference OpenMP Glynn (perm_c_ext.permanent_mod_p) on the leading
+    30 x 30 block of the headline matrix for one prime: 2^29 states."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle  # checker/baseline only
+
+    exts = oracle.reference_extensions()
+    sub = 30
+    p = primes[0]
+    block = [[int(v) % p for v in row[:sub]] for row in matrix[:sub]]
+    if "perm_c_ext" in exts:
+        ext = exts["perm_c_ext"]
+        kind = "reference"
+        fn = lambda: ext.permanent_mod_p(block, p, threads)  # noqa: E731
+    else:
+        kind = "port"
+        fn = lambda: oracle.C.permanent_mod_p(block, p, threads)  # noqa: E731
+    t0 = time.perf_counter()
+    fn()
+    dt = time.perf_counter() - t0
+    mulmods = sub * 2.0 ** (sub - 1)
+    rate = mulmods / dt
+    full = HEADLINE_N * 2.0 ** (HEADLINE_N - 1) * len(primes)
+    return {
+        "value": rate,
+        "unit": "mulmod/s",
+        "cores": threads,
+        "kind": kind,
+        "sample": (
+            f"perm_c_ext.permanent_mod_p on the leading {sub}x{sub} block of the headline matrix, "
+            f"one prime ({p}), 2^{sub - 1} Glynn states, {dt:.2f} s"
+        ),
+        "extrapolated_exact_n32_s": full / rate,
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 4096-matrix batch leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--kernel-only", action="store_true", help="headline launches only (for profilers)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(
+            "bench.py runs on one GPU only: this build has no multi-GPU path "
+            "(each residue would shard by prime or Gray range; see DESIGN.md)",
+            file=sys.stderr,
+        )
+        return 2
+    rank = int(os.environ.get("RANK", "0"))
+
+    from permhip import _native, planner
+    from permhip.batch import PermanentBatch
+
+    info = _native.device_info()
+    log(f"device: {info}")
+    A = headline_matrix()
+    bound = planner.matrix_permanent_bound(A.tolist())
+    primes, product = planner.generic_crt_primes(bound)
+    mulmods_per_perm = HEADLINE_N * 2.0 ** (HEADLINE_N - 1) * len(primes)
+
+    plan = PermanentBatch([A])
+    log(f"headline plan: n={HEADLINE_N} primes={len(primes)} stats={plan.stats()}")
+    exact = None
+    for _ in range(args.warmup):
+        exact = plan.run()[0]
+    if args.kernel_only:
+        for _ in range(args.steps):
+            plan.run()
+        log("kernel-only run done")
+        return 0
+
+    # Timed region: K exact permanents from resident tables.
+    _native.load()
+    step_times = []
+    t_all0 = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        value = plan.run()[0]
+        step_times.append(time.perf_counter() - t0)
+        if exact is not None and value != exact:
+            raise RuntimeError("headline permanent changed between steps")
+        exact = value
+    total = time.perf_counter() - t_all0
+    ms_per_step = 1e3 * total / args.steps
+    mulmods_per_s = mulmods_per_perm * args.steps / total
+    log(f"headline: {ms_per_step:.2f} ms/step, {mulmods_per_s:.3e} mulmod/s, perm={exact}")
+
+    # Kernel-only time of the persistent launch (events around it, no graph).
+    plan.plan.set_option(_native.BATCH_USE_GRAPH, 0)
+    kernel_ms = []
+    for _ in range(5):
+        plan.run()
+        kernel_ms.append(plan.stats()["last_ms"])
+    plan.plan.set_option(_native.BATCH_USE_GRAPH, 1)
+    k_ms = statistics.median(kernel_ms)
+    kernel_rate = mulmods_per_perm / (k_ms * 1e-3)
+
+    rates = _native.valu_microbench()
+    log("VALU lane-ops/s: " + ", ".join(f"{k}={v:.3e}" for k, v in rates.items()))
+    # Row sums of the headline matrix fit exact groups of three.
+    S = int(np.abs(A).sum(axis=1).max())
+    group = 4 if S < 2 ** 7 else 3 if S < 2 ** 10 else 2 if S < 2 ** 15 else 0
+    roof = roofline(HEADLINE_N, group, rates, kernel_rate)
+    roof["kernel"] = f"batch_glynn_kernel<{group}> (n={HEADLINE_N}, {len(primes)} primes per launch)"
+    roof["kernel_ms"] = k_ms
+
+    # Plain launch: one synchronous C-ABI call per matrix (host tables, copy,
+    # launch, reduce, copy back), as the reference wrappers do.
+    plain = []
+    for _ in range(max(3, args.steps // 2)):
+        t0 = time.perf_counter()
+        res = _native.permanent_residues(A, primes)
+        plain.append(time.perf_counter() - t0)
+    plain_exact = planner.signed_lift(planner.chinese_remainder_theorem(res, primes), bound, product)
+    if plain_exact != exact:
+        raise RuntimeError("plain launch and resident plan disagree")
+    plain_s = statistics.median(plain)
+    result = {
+        "metric": METRIC,
+        "value": mulmods_per_s,
+        "unit": "mulmod/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded integer matrices)",
+        "config": {
+            "workload": (
+                "exact permanent of a 32x32 integer matrix, entries U[-10,10] "
+                f"(numpy default_rng({HEADLINE_SEED})), 8 CRT primes from 2^30, resident device plan"
+            ),
+            "n": HEADLINE_N,
+            "primes": len(primes),
+            "parallelism": "single GPU",
+        },
+        "perms_per_s": args.steps / total,
+        "exact_permanent": str(exact),
+        "plain_launch": {"s_per_perm": plain_s, "perms_per_s": 1.0 / plain_s},
+        "roofline": roof,
+        "device": info,
+    }
+
+    if not args.no_batch:
+        mats = batch_matrices()
+        t0 = time.perf_counter()
+        batch = PermanentBatch(mats)
+        t_create = time.perf_counter() - t0
+        warm = batch.run()
+        runs = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            got = batch.run()
+            runs.append(time.perf_counter() - t0)
+            if got != warm:
+                raise RuntimeError("batch results changed between runs")
+        b_s = statistics.median(runs)
+        st = batch.stats()
+        # Plain per-matrix launches of the same matrices.
+        t0 = time.perf_counter()
+        plain_res = []
+        for M in mats:
+            b = planner.matrix_permanent_bound(M.tolist())
+            ps, prod = planner.generic_crt_primes(b)
+            r = _native.permanent_residues(M, ps)
+            plain_res.append(planner.signed_lift(planner.chinese_remainder_theorem(r, ps), b, prod))
+        t_plain = time.perf_counter() - t0
+        if plain_res != warm:
+            raise RuntimeError("batch and plain per-matrix results disagree")
+        result["batch"] = {
+            "workload": "4096 integer matrices, n ~ U[12,28], entries U[-100,100], default_rng(4)",
+            "matrices_per_s": len(mats) / b_s,
+            "s_per_batch": b_s,
+            "plan_create_s": t_create,
+            "mulmod_per_s": st.get("mulmods", 0) / b_s,
+            "jobs": st.get("jobs"),
+            "items": st.get("items"),
+            "plain_matrices_per_s": len(mats) / t_plain,
+            "speedup_vs_plain": t_plain / b_s,
+            "bit_exact_vs_plain": True,
+        }
+        log(f"batch: {result['batch']}")
+        batch.close()
+
+    if not args.no_serve:
+        try:
+            from permhip import serve
+        except ImportError:
+            serve = None
+        if serve is not None:
+            result["serve"] = serve.benchmark(seconds=args.serve_seconds)
+            log(f"serve: {result['serve']}")
+
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        result["cpu_baseline"] = cpu_baseline(A, primes, threads)
+        log(f"cpu baseline: {result['cpu_baseline']}")
+
+    if rank == 0:
+        print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -84,6 +84,60 @@ int phip_permanent_mod3_matrix(const int *A, int n);
  * Replaces permanent_mod3_cuda (perm_mod3_cuda.cu:164-166). */
 int phip_permanent_mod3(const uint64_t *mags, const uint64_t *sgns, int n);
 
+/* ---------------------------------------------------------------------- */
+/* Persistent batch (no reference counterpart: the reference issues one    */
+/* synchronous call per (matrix, prime), perm_cuda.py:196-198).            */
+/* ---------------------------------------------------------------------- */
+
+/* Build a device-resident plan for `count` matrices: entries holds the
+ * row-major int64 matrices back to back (sum of orders[i]^2 values),
+ * moduli holds moduli_counts[i] odd moduli in [3, 2^31) per matrix.  Orders
+ * must lie in [5, 40].  *out receives an opaque handle.  Returns 0 or -1. */
+int phip_batch_create(const int64_t *entries, const int32_t *orders, int count, const int64_t *moduli,
+                      const int32_t *moduli_counts, void **out);
+
+/* Run the whole plan (one persistent launch per row-sum class, replayed
+ * from a captured hipGraph).  residues_out, when not NULL, receives
+ * perm(A_i) mod each of A_i's moduli in input order.  Returns 0 or -1. */
+int phip_batch_run(void *plan, int64_t *residues_out);
+
+#define PHIP_BATCH_BLOCKS_PER_CU 1 /* persistent workgroups per CU (1..64) */
+#define PHIP_BATCH_USE_GRAPH 2     /* 0: plain stream launches, else hipGraph replay */
+int phip_batch_set_option(void *plan, int option, int64_t value);
+
+/* out[0] last run ms (persistent kernels only without a graph; whole graph
+ * otherwise), [1] Gray states, [2] algorithmic mulmods (n 2^(n-1) per job),
+ * [3] jobs, [4] work items, [5] persistent launches, [6] persistent
+ * wavefronts.  Returns the number of values written or -1. */
+int phip_batch_stats(void *plan, double *out, int len);
+
+int phip_batch_destroy(void *plan);
+
+/* ---------------------------------------------------------------------- */
+/* Tuning and measurement (no reference counterpart).                      */
+/* ---------------------------------------------------------------------- */
+
+#define PHIP_TUNE_LANES_LOG2 1       /* single-matrix launch: target lanes (log2) */
+#define PHIP_TUNE_MIN_CBITS 2        /* single-matrix launch: minimum chunk bits */
+#define PHIP_TUNE_BATCH_ITEMS_LOG2 3 /* batch: target work items (log2), read at create */
+#define PHIP_TUNE_BATCH_BLOCKS 4     /* batch: default persistent workgroups per CU */
+int phip_set_tuning(int key, int64_t value);
+int64_t phip_get_tuning(int key);
+
+/* Event timing of the single-matrix Glynn kernel launches: enable (and
+ * reset) with phip_profile_enable(1); phip_profile_read returns the summed
+ * kernel milliseconds and the launch count. */
+int phip_profile_enable(int on);
+int phip_profile_read(double *kernel_ms, uint64_t *launches);
+
+/* Whole-device throughput (lane operations per second) of v_add_u32,
+ * v_mul_i32_i24, v_mul_lo_u32, v_mul_hi_i32, v_mad_i64_i32, v_mad_u64_u32
+ * and one signed Montgomery product, in that order.  Returns the count. */
+int phip_valu_microbench(double *lane_ops_per_s, int len);
+
+/* `reps` launches of a plain copy kernel over `bytes` bytes. */
+int phip_copy_calibration(uint64_t bytes, int reps);
+
 #ifdef __cplusplus
 }
 #endif

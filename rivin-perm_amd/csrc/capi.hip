@@ -100,6 +100,69 @@ int phip_device_info(char* buf, size_t len) {
     return 0;
 }
 
+int phip_set_tuning(int key, int64_t value) {
+    phip::Tuning& t = phip::tuning();
+    switch (key) {
+        case PHIP_TUNE_LANES_LOG2:
+            if (value < 8 || value > 26) break;
+            t.lanes_log2 = (int)value;
+            return 0;
+        case PHIP_TUNE_MIN_CBITS:
+            if (value < 4 || value > 30) break;
+            t.min_cbits = (int)value;
+            return 0;
+        case PHIP_TUNE_BATCH_ITEMS_LOG2:
+            if (value < 6 || value > 24) break;
+            t.batch_items_log2 = (int)value;
+            return 0;
+        case PHIP_TUNE_BATCH_BLOCKS:
+            if (value < 1 || value > 64) break;
+            t.batch_blocks_per_cu = (int)value;
+            return 0;
+        default:
+            phip::set_error("unknown tuning key");
+            return -1;
+    }
+    phip::set_error("tuning value out of range");
+    return -1;
+}
+
+int64_t phip_get_tuning(int key) {
+    const phip::Tuning& t = phip::tuning();
+    switch (key) {
+        case PHIP_TUNE_LANES_LOG2: return t.lanes_log2;
+        case PHIP_TUNE_MIN_CBITS: return t.min_cbits;
+        case PHIP_TUNE_BATCH_ITEMS_LOG2: return t.batch_items_log2;
+        case PHIP_TUNE_BATCH_BLOCKS: return t.batch_blocks_per_cu;
+        default:
+            phip::set_error("unknown tuning key");
+            return -1;
+    }
+}
+
+int phip_profile_enable(int on) {
+    DeviceContext* ctx = DeviceContext::current();
+    if (!ctx) return -1;
+    std::lock_guard<std::mutex> lock(ctx->mutex());
+    if (on && !ctx->prof_start) {
+        PHIP_HIP_TRY(hipEventCreate(&ctx->prof_start));
+        PHIP_HIP_TRY(hipEventCreate(&ctx->prof_stop));
+    }
+    ctx->profiling = on != 0;
+    ctx->profiled_ms = 0;
+    ctx->profiled_launches = 0;
+    return 0;
+}
+
+int phip_profile_read(double* kernel_ms, uint64_t* launches) {
+    DeviceContext* ctx = DeviceContext::current();
+    if (!ctx) return -1;
+    std::lock_guard<std::mutex> lock(ctx->mutex());
+    if (kernel_ms) *kernel_ms = ctx->profiled_ms;
+    if (launches) *launches = ctx->profiled_launches;
+    return 0;
+}
+
 int phip_permanent_residues(const int64_t* A, int64_t prime_stride, int n, const int64_t* moduli, int k,
                             int64_t* out) {
     DeviceContext* ctx = DeviceContext::current();

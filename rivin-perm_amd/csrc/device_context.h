@@ -41,6 +41,12 @@ class DeviceContext {
     // Pinned host staging buffer for a slot.
     void* host_buffer(int slot, size_t bytes);
 
+    // Optional event timing of the Glynn kernel launches.
+    bool profiling = false;
+    double profiled_ms = 0;
+    uint64_t profiled_launches = 0;
+    hipEvent_t prof_start = nullptr, prof_stop = nullptr;
+
    private:
     explicit DeviceContext(int device);
     int device_ = -1;

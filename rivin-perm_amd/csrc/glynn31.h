@@ -148,35 +148,26 @@ struct GlynnRows {
     }
 };
 
-// Per-prime slice layout (all int32):
-//   rs0    [k][NR]       row sums of state 0 (all column signs +1):
-//                        exact centred sums (small) or residues (G = 0);
-//                        padding rows hold 1.
-//   deltas [k][n-1][NR]  2a_ij for every Gray column j (centred, small) or
-//                        2a_ij mod p (G = 0); padding rows hold 0.
-//   partials[k][gridDim.x] receives each block's sum of lane residues.
+// One lane's chunk: the 2^cbits Gray states starting at `base` (a multiple
+// of 2^cbits).  rs0 holds the NR row sums of state 0 (all column signs +1:
+// exact centred sums when G > 0, residues when G = 0; padding rows hold 1),
+// D the flip deltas of the n-1 Gray columns, column-major with stride NR
+// (2a_ij centred, or 2a_ij mod p; padding rows hold 0).  Returns the lane's
+// signed sum of terms reduced into [0, p); it still carries the factor
+// R^-(groups-1) and lacks 2^-(n-1).
 template <int NR, int G>
-__global__ __launch_bounds__(kGlynnBlock) void glynn31_kernel(
-    const int32_t* __restrict__ rs0, const int32_t* __restrict__ deltas,
-    const int32_t* __restrict__ primes, const uint32_t* __restrict__ pinvs, int n, int cbits,
-    uint64_t lanes_per_prime, uint64_t* __restrict__ partials) {
+__device__ __forceinline__ uint64_t glynn_chunk(const int32_t* __restrict__ rs0,
+                                                const int32_t* __restrict__ D, int n, int cbits,
+                                                uint64_t base, int32_t p, uint32_t pinv) {
     using Rows = GlynnRows<NR, G>;
-    const int q = blockIdx.y;
-    const int32_t p = primes[q];
     const int32_t negp = -p;
-    const uint32_t pinv = pinvs[q];
-    const int32_t* __restrict__ D = deltas + (size_t)q * (n - 1) * NR;
-
-    const uint64_t lane = (uint64_t)blockIdx.x * kGlynnBlock + threadIdx.x;
-    const bool active = lane < lanes_per_prime;
-    const uint64_t base = (active ? lane : 0) << cbits;
     const uint64_t gray = base ^ (base >> 1);
 
     // Row sums of the chunk's first state: subtract 2a for every set Gray
     // bit (perm_c_omp.c:582-600).  Bits below c-1 of gray(base) are zero.
     int32_t rs[NR];
 #pragma unroll
-    for (int i = 0; i < NR; ++i) rs[i] = rs0[(size_t)q * NR + i];
+    for (int i = 0; i < NR; ++i) rs[i] = rs0[i];
     for (int j = cbits - 1; j < n - 1; ++j) {
         const int32_t mask = -(int32_t)((gray >> j) & 1u);
         const int32_t* __restrict__ col = D + j * NR;
@@ -232,8 +223,24 @@ __global__ __launch_bounds__(kGlynnBlock) void glynn31_kernel(
     if (__popcll(gray) & 1) acc = -acc;
     int64_t r = acc % (int64_t)p;
     if (r < 0) r += p;
-    const uint64_t mine = active ? (uint64_t)r : 0u;
-    const uint64_t block_sum = block_reduce_sum_u64<kGlynnBlock>(mine);
+    return (uint64_t)r;
+}
+
+// Per-prime slice layout (all int32):
+//   rs0    [k][NR]       row sums of state 0 (see glynn_chunk)
+//   deltas [k][n-1][NR]  flip deltas (see glynn_chunk)
+//   partials[k][gridDim.x] receives each block's sum of lane residues.
+template <int NR, int G>
+__global__ __launch_bounds__(kGlynnBlock) void glynn31_kernel(
+    const int32_t* __restrict__ rs0, const int32_t* __restrict__ deltas,
+    const int32_t* __restrict__ primes, const uint32_t* __restrict__ pinvs, int n, int cbits,
+    uint64_t lanes_per_prime, uint64_t* __restrict__ partials) {
+    const int q = blockIdx.y;
+    const uint64_t lane = (uint64_t)blockIdx.x * kGlynnBlock + threadIdx.x;
+    const bool active = lane < lanes_per_prime;
+    const uint64_t r = glynn_chunk<NR, G>(rs0 + (size_t)q * NR, deltas + (size_t)q * (n - 1) * NR, n, cbits,
+                                          (active ? lane : 0) << cbits, primes[q], pinvs[q]);
+    const uint64_t block_sum = block_reduce_sum_u64<kGlynnBlock>(active ? r : 0u);
     if (threadIdx.x == 0) partials[(size_t)q * gridDim.x + blockIdx.x] = block_sum;
 }
 

@@ -1,0 +1,143 @@
+// Measurement helpers used by the benchmark to pin its roofline:
+//
+//   phip_valu_microbench  throughput of the VALU instructions the Glynn
+//                         kernels issue (v_add_u32, v_mul_i32_i24,
+//                         v_mul_lo_u32, v_mul_hi_i32, v_mad_i64_i32,
+//                         v_mad_u64_u32) and of one complete signed
+//                         Montgomery product, each as lane-operations per
+//                         second over the whole device.  Eight independent
+//                         dependency chains per lane keep the pipeline full.
+//   phip_copy_calibration a plain device-to-device copy kernel of a known
+//                         size, to calibrate HBM byte counters.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "device_context.h"
+#include "permhip.h"
+#include "phip_arith.h"
+
+namespace phip {
+
+namespace {
+
+constexpr int kChains = 8;
+constexpr int kUnroll = 8;
+
+__device__ __forceinline__ int32_t mont31_mb(int32_t a, int32_t b, int32_t negp, uint32_t pinv) {
+    int64_t t = (int64_t)a * (int64_t)b;
+    int32_t m = (int32_t)((uint32_t)t * pinv);
+    int64_t u = t + (int64_t)m * (int64_t)negp;
+    return (int32_t)(u >> 32);
+}
+
+template <int OP>
+__global__ __launch_bounds__(256) void valu_rate_kernel(uint32_t* out, int iters, uint32_t c) {
+    uint32_t x[kChains];
+    uint64_t y[kChains];
+#pragma unroll
+    for (int k = 0; k < kChains; ++k) {
+        x[k] = threadIdx.x * 2654435761u + k + c;
+        y[k] = x[k];
+    }
+    const int32_t p = 2147483647;
+    const uint32_t pinv = inverse_mod_2_32((uint32_t)p);
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+#pragma unroll
+            for (int k = 0; k < kChains; ++k) {
+                if constexpr (OP == 0) {
+                    asm volatile("v_add_u32 %0, %0, %1" : "+v"(x[k]) : "v"(c));
+                } else if constexpr (OP == 1) {
+                    asm volatile("v_mul_i32_i24 %0, %0, %1" : "+v"(x[k]) : "v"(c));
+                } else if constexpr (OP == 2) {
+                    asm volatile("v_mul_lo_u32 %0, %0, %1" : "+v"(x[k]) : "v"(c));
+                } else if constexpr (OP == 3) {
+                    asm volatile("v_mul_hi_i32 %0, %0, %1" : "+v"(x[k]) : "v"(c));
+                } else if constexpr (OP == 4) {
+                    asm volatile("v_mad_i64_i32 %0, vcc, %1, %2, %0" : "+v"(y[k]) : "v"(x[k]), "v"(c) : "vcc");
+                } else if constexpr (OP == 5) {
+                    asm volatile("v_mad_u64_u32 %0, vcc, %1, %2, %0" : "+v"(y[k]) : "v"(x[k]), "v"(c) : "vcc");
+                } else {
+                    x[k] = (uint32_t)mont31_mb((int32_t)x[k], (int32_t)c, -p, pinv);
+                }
+            }
+        }
+    }
+    uint32_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < kChains; ++k) acc ^= x[k] ^ (uint32_t)y[k] ^ (uint32_t)(y[k] >> 32);
+    out[blockIdx.x * blockDim.x + threadIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                   uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
+}  // namespace
+
+}  // namespace phip
+
+extern "C" {
+
+int phip_valu_microbench(double* lane_ops_per_s, int len) {
+    phip::DeviceContext* ctx = phip::DeviceContext::current();
+    if (!ctx) return -1;
+    std::lock_guard<std::mutex> lock(ctx->mutex());
+    const int cus = ctx->compute_units() > 0 ? ctx->compute_units() : 256;
+    const int blocks = cus * 8;  // 32 waves per CU
+    const int iters = 4096;
+    hipStream_t st = ctx->stream();
+    auto* out = (uint32_t*)ctx->buffer(0, (size_t)blocks * 256 * 4);
+    if (!out) return -1;
+    hipEvent_t e0, e1;
+    PHIP_HIP_TRY(hipEventCreate(&e0));
+    PHIP_HIP_TRY(hipEventCreate(&e1));
+    using Kernel = void (*)(uint32_t*, int, uint32_t);
+    const Kernel kernels[] = {phip::valu_rate_kernel<0>, phip::valu_rate_kernel<1>, phip::valu_rate_kernel<2>,
+                              phip::valu_rate_kernel<3>, phip::valu_rate_kernel<4>, phip::valu_rate_kernel<5>,
+                              phip::valu_rate_kernel<6>};
+    const int nk = (int)(sizeof(kernels) / sizeof(kernels[0]));
+    int written = 0;
+    for (int k = 0; k < nk && k < len; ++k) {
+        // Warm-up, then one timed launch.
+        hipLaunchKernelGGL(kernels[k], dim3(blocks), dim3(256), 0, st, out, 64, 12345u);
+        PHIP_HIP_TRY(hipGetLastError());
+        PHIP_HIP_TRY(hipEventRecord(e0, st));
+        hipLaunchKernelGGL(kernels[k], dim3(blocks), dim3(256), 0, st, out, iters, 12345u);
+        PHIP_HIP_TRY(hipGetLastError());
+        PHIP_HIP_TRY(hipEventRecord(e1, st));
+        PHIP_HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0;
+        PHIP_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        const double ops = (double)blocks * 256 * iters * phip::kUnroll * phip::kChains;
+        lane_ops_per_s[k] = ops / (ms * 1e-3);
+        ++written;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return written;
+}
+
+int phip_copy_calibration(uint64_t bytes, int reps) {
+    phip::DeviceContext* ctx = phip::DeviceContext::current();
+    if (!ctx) return -1;
+    std::lock_guard<std::mutex> lock(ctx->mutex());
+    const uint64_t n = (bytes + 15) / 16;
+    auto* src = (uint4*)ctx->buffer(0, n * 16);
+    auto* dst = (uint4*)ctx->buffer(1, n * 16);
+    if (!src || !dst) return -1;
+    hipStream_t st = ctx->stream();
+    for (int r = 0; r < reps; ++r) {
+        hipLaunchKernelGGL(phip::copy_kernel, dim3(2048), dim3(256), 0, st, src, dst, n);
+        PHIP_HIP_TRY(hipGetLastError());
+    }
+    PHIP_HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+}  // extern "C"

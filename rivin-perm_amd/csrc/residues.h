@@ -13,6 +13,8 @@ namespace phip {
 struct Tuning {
     int lanes_log2 = 20;
     int min_cbits = 4;
+    int batch_items_log2 = 15;
+    int batch_blocks_per_cu = 2;
 };
 Tuning& tuning();
 

@@ -118,11 +118,19 @@ int run_glynn31(DeviceContext* ctx, const int64_t* A, int64_t stride, int n, int
     PHIP_HIP_TRY(hipMemcpyAsync(d_mod, moduli.data(), moduli.size() * 8, hipMemcpyHostToDevice, st));
 
     Glynn31Args args{d_rs0, d_del, d_pr, d_pi, n, cbits, lanes, d_part};
+    if (ctx->profiling) PHIP_HIP_TRY(hipEventRecord(ctx->prof_start, st));
     PHIP_HIP_TRY(launch_glynn31(group, NR, dim3((unsigned)gx, (unsigned)kk), st, args));
+    if (ctx->profiling) PHIP_HIP_TRY(hipEventRecord(ctx->prof_stop, st));
     hipLaunchKernelGGL(sum_mod_kernel, dim3(kk), dim3(256), 0, st, d_part, gx, d_mod, d_sum);
     PHIP_HIP_TRY(hipGetLastError());
     PHIP_HIP_TRY(hipMemcpyAsync(h_sum, d_sum, (size_t)kk * 8, hipMemcpyDeviceToHost, st));
     PHIP_HIP_TRY(hipStreamSynchronize(st));
+    if (ctx->profiling) {
+        float ms = 0;
+        PHIP_HIP_TRY(hipEventElapsedTime(&ms, ctx->prof_start, ctx->prof_stop));
+        ctx->profiled_ms += ms;
+        ctx->profiled_launches += 1;
+    }
 
     const int groups = (group == 0) ? NR : (NR + group - 1) / group;
     for (int s = 0; s < kk; ++s) {

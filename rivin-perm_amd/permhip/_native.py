@@ -25,6 +25,8 @@ _I64P = ctypes.POINTER(ctypes.c_int64)
 _U64P = ctypes.POINTER(ctypes.c_uint64)
 _U8P = ctypes.POINTER(ctypes.c_uint8)
 _INTP = ctypes.POINTER(ctypes.c_int)
+_I32P = ctypes.POINTER(ctypes.c_int32)
+_F64P = ctypes.POINTER(ctypes.c_double)
 
 # name -> (restype, argtypes)
 _SIGNATURES = {
@@ -44,7 +46,37 @@ _SIGNATURES = {
     "phip_binary_permanent_modp": (ctypes.c_int64, [_U8P, ctypes.c_int, ctypes.c_int64]),
     "phip_permanent_mod3_matrix": (ctypes.c_int, [_INTP, ctypes.c_int]),
     "phip_permanent_mod3": (ctypes.c_int, [_U64P, _U64P, ctypes.c_int]),
+    "phip_batch_create": (
+        ctypes.c_int,
+        [_I64P, _I32P, ctypes.c_int, _I64P, _I32P, ctypes.POINTER(ctypes.c_void_p)],
+    ),
+    "phip_batch_run": (ctypes.c_int, [ctypes.c_void_p, _I64P]),
+    "phip_batch_set_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64]),
+    "phip_batch_stats": (ctypes.c_int, [ctypes.c_void_p, _F64P, ctypes.c_int]),
+    "phip_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "phip_set_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64]),
+    "phip_get_tuning": (ctypes.c_int64, [ctypes.c_int]),
+    "phip_profile_enable": (ctypes.c_int, [ctypes.c_int]),
+    "phip_profile_read": (ctypes.c_int, [_F64P, _U64P]),
+    "phip_valu_microbench": (ctypes.c_int, [_F64P, ctypes.c_int]),
+    "phip_copy_calibration": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int]),
 }
+
+TUNE_LANES_LOG2 = 1
+TUNE_MIN_CBITS = 2
+TUNE_BATCH_ITEMS_LOG2 = 3
+TUNE_BATCH_BLOCKS = 4
+BATCH_BLOCKS_PER_CU = 1
+BATCH_USE_GRAPH = 2
+MICROBENCH_NAMES = (
+    "v_add_u32",
+    "v_mul_i32_i24",
+    "v_mul_lo_u32",
+    "v_mul_hi_i32",
+    "v_mad_i64_i32",
+    "v_mad_u64_u32",
+    "mont31_product",
+)
 
 
 def exported_symbols():
@@ -166,3 +198,99 @@ def permanent_mod3(matrix: np.ndarray) -> int:
     result = lib.phip_permanent_mod3_matrix(a.ctypes.data_as(_INTP), n)
     _check(result, "phip_permanent_mod3_matrix")
     return int(result)
+
+
+def set_tuning(key: int, value: int) -> None:
+    _check(load().phip_set_tuning(int(key), int(value)), "phip_set_tuning")
+
+
+def get_tuning(key: int) -> int:
+    value = load().phip_get_tuning(int(key))
+    _check(value, "phip_get_tuning")
+    return int(value)
+
+
+def profile_enable(on: bool) -> None:
+    _check(require_device().phip_profile_enable(1 if on else 0), "phip_profile_enable")
+
+
+def profile_read():
+    """(summed kernel milliseconds, launches) of the single-matrix kernel."""
+    ms = ctypes.c_double()
+    launches = ctypes.c_uint64()
+    _check(require_device().phip_profile_read(ctypes.byref(ms), ctypes.byref(launches)), "phip_profile_read")
+    return ms.value, launches.value
+
+
+def valu_microbench() -> dict:
+    """{instruction: lane-operations per second} over the whole device."""
+    out = (ctypes.c_double * len(MICROBENCH_NAMES))()
+    count = require_device().phip_valu_microbench(out, len(out))
+    _check(count, "phip_valu_microbench")
+    return {name: out[i] for i, name in enumerate(MICROBENCH_NAMES[:count])}
+
+
+def copy_calibration(nbytes: int, reps: int = 1) -> None:
+    _check(require_device().phip_copy_calibration(int(nbytes), int(reps)), "phip_copy_calibration")
+
+
+class Batch:
+    """Device-resident batch plan (phip_batch_*)."""
+
+    def __init__(self, matrices, moduli_lists):
+        lib = require_device()
+        mats = [np.ascontiguousarray(m, dtype=np.int64) for m in matrices]
+        if len(mats) != len(moduli_lists):
+            raise ValueError("one modulus list per matrix is required")
+        orders = np.array([m.shape[0] for m in mats], dtype=np.int32)
+        entries = (
+            np.concatenate([m.ravel() for m in mats]) if mats else np.zeros(1, dtype=np.int64)
+        ).astype(np.int64)
+        counts = np.array([len(ms) for ms in moduli_lists], dtype=np.int32)
+        flat = [int(q) for ms in moduli_lists for q in ms]
+        moduli = np.array(flat if flat else [0], dtype=np.int64)
+        self._keep = (entries, orders, counts, moduli)
+        self.counts = [int(c) for c in counts]
+        self.total = len(flat)
+        handle = ctypes.c_void_p()
+        status = lib.phip_batch_create(
+            entries.ctypes.data_as(_I64P),
+            orders.ctypes.data_as(_I32P),
+            len(mats),
+            moduli.ctypes.data_as(_I64P),
+            counts.ctypes.data_as(_I32P),
+            ctypes.byref(handle),
+        )
+        _check(status, "phip_batch_create")
+        self._lib = lib
+        self._handle = handle
+        self._out = np.zeros(max(1, self.total), dtype=np.int64)
+
+    def set_option(self, option: int, value: int) -> None:
+        _check(self._lib.phip_batch_set_option(self._handle, option, int(value)), "phip_batch_set_option")
+
+    def run(self, fetch: bool = True):
+        """Run the plan; returns the flat residue list (input order) or None."""
+        out = self._out.ctypes.data_as(_I64P) if fetch else ctypes.cast(None, _I64P)
+        _check(self._lib.phip_batch_run(self._handle, out), "phip_batch_run")
+        if not fetch:
+            return None
+        return [int(v) for v in self._out[: self.total]]
+
+    def stats(self) -> dict:
+        vals = (ctypes.c_double * 7)()
+        count = self._lib.phip_batch_stats(self._handle, vals, 7)
+        _check(count, "phip_batch_stats")
+        keys = ("last_ms", "states", "mulmods", "jobs", "items", "launches", "waves")
+        return {k: vals[i] for i, k in enumerate(keys[:count])}
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None):
+            self._lib.phip_batch_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -27,7 +27,15 @@ LIBRARY = PACKAGE_DIR / "libpermhip.so"
 
 ARCH = "gfx950"
 GLYNN_GROUPS = (0, 2, 3, 4)
-PLAIN_SOURCES = ("device_context.hip", "residues.hip", "mod3.hip", "capi.hip")
+BATCH_GROUPS = (0, 2, 3, 4)
+PLAIN_SOURCES = (
+    "device_context.hip",
+    "residues.hip",
+    "mod3.hip",
+    "capi.hip",
+    "batch_plan.hip",
+    "microbench.hip",
+)
 
 
 def _hipcc() -> str:
@@ -71,6 +79,10 @@ def build(jobs: int | None = None, verbose: bool = False) -> Path:
     for group in GLYNN_GROUPS:
         units.append(
             (CSRC / "glynn31_inst.hip", BUILD_DIR / f"glynn31_g{group}.o", [f"-DPHIP_GROUP={group}"])
+        )
+    for group in BATCH_GROUPS:
+        units.append(
+            (CSRC / "batch.hip", BUILD_DIR / f"batch_g{group}.o", [f"-DPHIP_BATCH_GROUP={group}"])
         )
     for name in PLAIN_SOURCES:
         units.append((CSRC / name, BUILD_DIR / (Path(name).stem + ".o"), []))
