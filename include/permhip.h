@@ -101,6 +101,13 @@ int phip_batch_create(const int64_t *entries, const int32_t *orders, int count, 
  * perm(A_i) mod each of A_i's moduli in input order.  Returns 0 or -1. */
 int phip_batch_run(void *plan, int64_t *residues_out);
 
+/* Asynchronous form: phip_batch_launch enqueues a run and returns at once;
+ * phip_batch_query returns 1 once it has finished (copying the residues to
+ * residues_out when not NULL), 0 while it runs (block == 0), or -1.  Runs of
+ * different plans proceed concurrently on their own streams. */
+int phip_batch_launch(void *plan);
+int phip_batch_query(void *plan, int64_t *residues_out, int block);
+
 #define PHIP_BATCH_BLOCKS_PER_CU 1 /* persistent workgroups per CU (1..64) */
 #define PHIP_BATCH_USE_GRAPH 2     /* 0: plain stream launches, else hipGraph replay */
 int phip_batch_set_option(void *plan, int option, int64_t value);

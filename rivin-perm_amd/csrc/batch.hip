@@ -39,10 +39,14 @@ __global__ __launch_bounds__(256) void batch_glynn_kernel(const BatchJob* __rest
                                                          const int32_t* __restrict__ blob,
                                                          uint64_t* __restrict__ item_partials) {
     const int lane_id = threadIdx.x & 63;
+    const uint32_t claim = lane_id == 0 ? 1u : 0u;
     for (;;) {
-        uint32_t idx = 0;
-        if (lane_id == 0) idx = atomicAdd(counter, 1u);
-        idx = (uint32_t)__builtin_amdgcn_readfirstlane((int)idx);
+        // Every lane executes the same atomic (lane 0 adds 1, the others 0)
+        // so no lane-dependent branch sits in front of the broadcast: with a
+        // lane-0-only branch here the compiler split the loop so that lanes
+        // 1..63 re-read a stale index without lane 0 and never left it.
+        const uint32_t old = atomicAdd(counter, claim);
+        const uint32_t idx = (uint32_t)__builtin_amdgcn_readfirstlane((int)old);
         if (idx >= n_items) break;
         const uint2 it = items[idx];
         const BatchJob jb = jobs[it.x];

@@ -133,8 +133,10 @@ struct BatchPlan {
         return 0;
     }
 
-    int run(int64_t* out) {
-        std::lock_guard<std::mutex> lock(mu);
+    bool launched = false;
+
+    // Enqueue one run on the plan's stream without waiting.
+    int launch_locked() {
         PHIP_HIP_TRY(hipSetDevice(device));
         if (use_graph) {
             if (!exec) {
@@ -154,10 +156,34 @@ struct BatchPlan {
         } else {
             if (enqueue(true) != 0) return -1;
         }
-        PHIP_HIP_TRY(hipStreamSynchronize(stream));
+        launched = true;
+        return 0;
+    }
+
+    // 1: finished (residues copied to out when not null), 0: still running.
+    int finish_locked(int64_t* out, bool block) {
+        if (!launched) {
+            set_error("batch plan has not been launched");
+            return -1;
+        }
+        if (block) {
+            PHIP_HIP_TRY(hipStreamSynchronize(stream));
+        } else {
+            const hipError_t e = hipStreamQuery(stream);
+            if (e == hipErrorNotReady) return 0;
+            PHIP_HIP_TRY(e);
+        }
+        launched = false;
         PHIP_HIP_TRY(hipEventElapsedTime(&last_ms, ev0, ev1));
         if (out) memcpy(out, h_res, jobs.size() * sizeof(int64_t));
-        return 0;
+        return 1;
+    }
+
+    int run(int64_t* out) {
+        std::lock_guard<std::mutex> lock(mu);
+        if (launched && finish_locked(nullptr, true) < 0) return -1;
+        if (launch_locked() != 0) return -1;
+        return finish_locked(out, true) < 0 ? -1 : 0;
     }
 };
 
@@ -350,6 +376,32 @@ int phip_batch_run(void* handle, int64_t* residues_out) {
     return plan->run(residues_out);
 }
 
+int phip_batch_launch(void* handle) {
+    if (!handle) {
+        phip::set_error("null batch handle");
+        return -1;
+    }
+    auto* plan = static_cast<BatchPlan*>(handle);
+    if (plan->jobs.empty()) return 0;
+    std::lock_guard<std::mutex> lock(plan->mu);
+    if (plan->launched) {
+        phip::set_error("batch plan is already running");
+        return -1;
+    }
+    return plan->launch_locked();
+}
+
+int phip_batch_query(void* handle, int64_t* residues_out, int block) {
+    if (!handle) {
+        phip::set_error("null batch handle");
+        return -1;
+    }
+    auto* plan = static_cast<BatchPlan*>(handle);
+    if (plan->jobs.empty()) return 1;
+    std::lock_guard<std::mutex> lock(plan->mu);
+    return plan->finish_locked(residues_out, block != 0);
+}
+
 int phip_batch_set_option(void* handle, int option, int64_t value) {
     if (!handle) {
         phip::set_error("null batch handle");
@@ -357,6 +409,10 @@ int phip_batch_set_option(void* handle, int option, int64_t value) {
     }
     auto* plan = static_cast<BatchPlan*>(handle);
     std::lock_guard<std::mutex> lock(plan->mu);
+    if (plan->launched) {
+        phip::set_error("cannot change options while the plan is running");
+        return -1;
+    }
     switch (option) {
         case PHIP_BATCH_BLOCKS_PER_CU:
             if (value < 1 || value > 64) {

@@ -51,6 +51,8 @@ _SIGNATURES = {
         [_I64P, _I32P, ctypes.c_int, _I64P, _I32P, ctypes.POINTER(ctypes.c_void_p)],
     ),
     "phip_batch_run": (ctypes.c_int, [ctypes.c_void_p, _I64P]),
+    "phip_batch_launch": (ctypes.c_int, [ctypes.c_void_p]),
+    "phip_batch_query": (ctypes.c_int, [ctypes.c_void_p, _I64P, ctypes.c_int]),
     "phip_batch_set_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64]),
     "phip_batch_stats": (ctypes.c_int, [ctypes.c_void_p, _F64P, ctypes.c_int]),
     "phip_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
@@ -274,6 +276,19 @@ class Batch:
         out = self._out.ctypes.data_as(_I64P) if fetch else ctypes.cast(None, _I64P)
         _check(self._lib.phip_batch_run(self._handle, out), "phip_batch_run")
         if not fetch:
+            return None
+        return [int(v) for v in self._out[: self.total]]
+
+    def launch(self) -> None:
+        """Start a run without waiting (see poll)."""
+        _check(self._lib.phip_batch_launch(self._handle), "phip_batch_launch")
+
+    def poll(self, block: bool = False):
+        """Residue list once the launched run has finished, else None."""
+        out = self._out.ctypes.data_as(_I64P)
+        status = self._lib.phip_batch_query(self._handle, out, 1 if block else 0)
+        _check(status, "phip_batch_query")
+        if status == 0:
             return None
         return [int(v) for v in self._out[: self.total]]
 

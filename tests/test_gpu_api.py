@@ -1,0 +1,213 @@
+"""The drop-in modules (perm, perm_binary, perm_mod3) on the device against
+the oracle, closed forms and the reference's golden values."""
+
+import json
+import math
+import random
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = Path(__file__).resolve().parent / "golden" / "reference_residues.json"
+
+OEIS_A003112 = {
+    1: 1, 3: -3, 5: -5, 7: -105, 9: 81, 11: 6765, 13: 175747, 15: 30375,
+    17: 25219857, 19: 142901109, 21: 4548104883, 23: -31152650265,
+    25: -5198937484375, 27: 65230244418933, 29: -1300425712598285,
+    31: 126691467546591,
+}
+
+
+@pytest.mark.parametrize("n", sorted(OEIS_A003112))
+def test_schur_permanents_match_oeis(native, n):
+    import perm
+
+    assert perm.permanent(n) == OEIS_A003112[n]
+    assert perm.schur_permanent(n) == OEIS_A003112[n]
+
+
+def test_even_and_trivial_schur_orders(native):
+    import perm
+
+    assert [perm.permanent(n) for n in (0, 1, 2, 4, 10)] == [1, 1, 0, 0, 0]
+
+
+def test_matrix_permanent_random_against_subset_dp(native):
+    import perm
+
+    rng = random.Random(20261203)
+    for n in range(0, 13):
+        for lo, hi in ((-10, 10), (0, 1), (-(10 ** 12), 10 ** 12)):
+            A = [[rng.randrange(lo, hi + 1) for _ in range(n)] for _ in range(n)]
+            assert perm.matrix_permanent(A) == oracle.subset_permanent(A), (n, lo)
+
+
+def test_matrix_permanent_numpy_and_bigint_inputs(native):
+    import perm
+
+    A = np.array([[2 ** 62, -3], [5, 2 ** 62]], dtype=np.int64)
+    assert perm.matrix_permanent(A) == 2 ** 124 - 15
+    U = np.array([[2 ** 63, 1], [1, 2 ** 63]], dtype=np.uint64)
+    assert perm.matrix_permanent(U) == 2 ** 126 + 1
+    big = [[1 << 80, 3, 0], [2, 1 << 70, 5], [7, 0, -(1 << 90)]]
+    assert perm.matrix_permanent(big) == oracle.definition_permanent(big)
+    assert type(perm.matrix_permanent([[3]])) is int
+
+
+def test_closed_forms(native):
+    import perm
+
+    for n in (5, 9, 14, 20):
+        assert perm.matrix_permanent([[1] * n for _ in range(n)]) == math.factorial(n)
+        u = list(range(1, n + 1))
+        v = [(-1) ** i * (i % 5 + 1) for i in range(n)]
+        R = [[a * b for b in v] for a in u]
+        assert perm.matrix_permanent(R) == math.factorial(n) * math.prod(u) * math.prod(v)
+        s = 7
+        M = [[(s if i == j else 0) + 1 for j in range(n)] for i in range(n)]
+        assert perm.matrix_permanent(M) == sum(
+            math.comb(n, k) * math.factorial(n - k) * s ** k for k in range(n + 1)
+        )
+
+
+def test_low_rank_method_matches_generic(native):
+    import perm
+
+    n = 12
+    u = [[(i * 7 + 3 * j) % 11 - 5 for j in range(2)] for i in range(n)]
+    w = [[(i * 5 + j) % 7 - 3 for i in range(n)] for j in range(2)]
+    A = [[sum(u[i][k] * w[k][j] for k in range(2)) for j in range(n)] for i in range(n)]
+    assert perm.matrix_permanent(A, method="low_rank") == perm.matrix_permanent(A, method="generic")
+    with pytest.raises(perm.LowRankLimitExceeded):
+        perm.matrix_permanent([[random.Random(1).randrange(9) for _ in range(8)] for _ in range(8)],
+                              method="low_rank")
+
+
+def test_golden_reference_residues(native):
+    """Residues written by the reference's own compiled kernels."""
+    if not GOLDEN.exists():
+        pytest.skip("golden fixtures not generated")
+    import perm
+    from permhip import _native, workloads
+
+    data = json.loads(GOLDEN.read_text())
+    cases = {"config1": workloads.config1(), "headline": workloads.headline()}
+    mats = workloads.batch4096()
+    for key in data:
+        if key.startswith("batch4096["):
+            cases[key] = mats[int(key[len("batch4096["):-1])]
+    for key, A in cases.items():
+        if key not in data:
+            continue
+        case = data[key]
+        assert _native.permanent_residues(A, case["moduli"]) == case["residues"], key
+        assert perm.matrix_permanent(A.tolist()) == int(case["exact"]), key
+    if "schur31" in data:
+        case = data["schur31"]
+        for p, r in zip(case["moduli"], case["residues"]):
+            assert perm.schur_permanent_mod_p_fast(31, p) == r
+    if "binary36" in data:
+        import perm_binary
+        import perm_mod3
+
+        case = data["binary36"]
+        B = workloads.binary36()
+        assert _native.binary_permanent_residues(B, case["moduli"]) == case["residues"]
+        assert perm_mod3.permanent_mod3_fast(B) == case["mod3"]
+        assert perm_binary.binary_permanent(B.tolist()) % 3 == case["mod3"]
+
+
+def test_binary_known_counts(native):
+    import perm_binary
+    from permhip import grids
+
+    for shape, expect in [((2, 2), 2), ((2, 4), 5), ((4, 4), 36), ((4, 6), 281), ((2, 2, 2), 9),
+                          ((2, 2, 3), 32), ((2, 2, 4), 121), ((2, 3, 4), 1845)]:
+        assert perm_binary.binary_permanent(grids.grid_biadjacency(shape)) == expect, shape
+    assert perm_binary.binary_permanent(grids.grid_biadjacency((4, 4), periodic_axes=(0, 1))) == 272
+    assert perm_binary.binary_permanent(grids.grid_biadjacency((4, 4, 2), periodic_axes=(0, 1))) == 589185
+    assert perm_binary.binary_permanent([[1] * 13 for _ in range(13)]) == math.factorial(13)
+    for d, k in ((2, 5), (3, 4), (5, 3)):
+        n = d * k
+        B = [[1 if i // d == j // d else 0 for j in range(n)] for i in range(n)]
+        assert perm_binary.binary_permanent(B) == math.factorial(d) ** k
+        assert perm_binary.binary_permanent_bound(B) == math.factorial(d) ** k
+    der = [1, 0]
+    for n in range(2, 12):
+        der.append((n - 1) * (der[-1] + der[-2]))
+    for n in range(2, 12):
+        D = [[0 if i == j else 1 for j in range(n)] for i in range(n)]
+        assert perm_binary.binary_permanent(D) == der[n]
+
+
+def test_periodic_4x4x5_grid_exact_count(native):
+    """2,551,276,235,535,120 with residues 1,238,389,710 and 1,259,774,250
+    modulo 2^31-1 and 2147483629 (BINARY_APPLICATIONS.md:155-162)."""
+    import perm_binary
+    from permhip import _native, grids
+
+    G = grids.grid_biadjacency((4, 4, 5), periodic_axes=(0, 1))
+    assert len(G) == 40
+    assert perm_binary._crt_primes_above_bound(perm_binary.binary_permanent_bound(G)) == (2147483647, 2147483629)
+    assert _native.binary_permanent_residues(np.array(G, dtype=np.uint8), [2147483647, 2147483629]) == [
+        1238389710, 1259774250]
+    assert perm_binary.binary_permanent(G) == 2551276235535120
+
+
+def test_binary_random_and_mod_p(native):
+    import perm_binary
+
+    rng = random.Random(314159)
+    for n in range(2, 12):
+        for _ in range(3):
+            B = [[rng.randrange(2) for _ in range(n)] for _ in range(n)]
+            d = oracle.subset_permanent(B)
+            assert perm_binary.binary_permanent(B) == d
+            assert perm_binary.binary_permanent_mod_p(B, 101) == d % 101
+            assert perm_binary.binary_permanent(B, strict=False) == d
+
+
+def test_mod3_module(native):
+    import perm_mod3
+
+    rng = np.random.default_rng(123)
+    for n in (0, 1, 2, 5, 10, 15, 20, 26):
+        A = rng.integers(-10, 11, (n, n))
+        assert perm_mod3.permanent_mod3_fast(A) == oracle.C.mod3(A, 8), n
+    U = np.array([[2 ** 64 - 1, 1], [1, 1]], dtype=np.uint64)
+    assert perm_mod3.permanent_mod3_fast(U) == (2 ** 64 - 1 + 1) % 3
+
+
+def test_adapter_objects_as_reference_seams(native):
+    from permhip.backends import perm_binary_ext, perm_c_ext, perm_mod3_omp
+
+    assert perm_c_ext.permanent_mod_p([[2, -1], [3, 4]], 101, 0) == 5
+    assert perm_c_ext.ryser_permanent_mod_p([[2, -1], [3, 4]], 101) == 5
+    assert perm_c_ext.get_max_threads() >= 1
+    assert perm_binary_ext.binary_permanent_mod_p([[1] * 8 for _ in range(8)], 2147483647, (1 << 31) - 1) == 40320
+    assert perm_mod3_omp.permanent_mod3(np.array([[1, 1], [1, 1]])) == 2
+
+
+def test_served_results_equal_alone(native):
+    from permhip.serve import PermanentServer
+
+    rng = np.random.default_rng(55)
+    reqs = []
+    for i in range(40):
+        if i % 2:
+            n = int(rng.integers(3, 16))
+            reqs.append(("integer", rng.integers(-50, 51, (n, n))))
+        else:
+            n = int(rng.integers(3, 18))
+            reqs.append(("binary", (rng.random((n, n)) < 0.5).astype(np.int64)))
+    server = PermanentServer(max_inflight=2, plan_cost=1e6)
+    futures = [server.submit(m, k) for k, m in reqs]
+    mixed = [f.result(timeout=60) for f in futures]
+    server.close()
+    for (kind, m), v in zip(reqs, mixed):
+        assert v == oracle.subset_permanent(m.tolist()), kind
