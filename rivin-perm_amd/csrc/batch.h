@@ -7,7 +7,7 @@ namespace phip {
 
 // One (matrix, modulus) job of a batch plan; 48 bytes, read with scalar loads.
 struct BatchJob {
-    uint64_t data_off;  // int32 offset of rs0 (n values) followed by D ((n-1) x n)
+    uint64_t data_off;  // int32 offset of the job's glynn_chunk tables (n * glynn_stride words)
     uint64_t lanes;     // lanes of this job (2^(n-1-cbits))
     int32_t n;
     int32_t cbits;

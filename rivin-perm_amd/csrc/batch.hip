@@ -53,13 +53,12 @@ __global__ __launch_bounds__(256) void batch_glynn_kernel(const BatchJob* __rest
         const uint64_t lane = (uint64_t)it.y * 64 + lane_id;
         const bool active = lane < jb.lanes;
         const uint64_t base = (active ? lane : 0) << jb.cbits;
-        const int32_t* __restrict__ rs0 = blob + jb.data_off;
-        const int32_t* __restrict__ D = rs0 + jb.n;
+        const int32_t* __restrict__ tables = blob + jb.data_off;
         uint64_t r = 0;
         switch (jb.n) {
 #define PHIP_BATCH_CASE(N) \
     case N:                \
-        r = glynn_chunk<N, G>(rs0, D, N, jb.cbits, base, jb.p, jb.pinv); \
+        r = glynn_chunk<N, G>(tables, N, jb.cbits, base, jb.p, jb.pinv); \
         break;
             PHIP_BATCH_CASE(5) PHIP_BATCH_CASE(6) PHIP_BATCH_CASE(7) PHIP_BATCH_CASE(8)
             PHIP_BATCH_CASE(9) PHIP_BATCH_CASE(10) PHIP_BATCH_CASE(11) PHIP_BATCH_CASE(12)

@@ -17,6 +17,7 @@
 #include "phip_arith.h"
 #include "phip_reduce.h"
 #include "residues.h"
+#include "tables.h"
 
 namespace phip {
 
@@ -47,14 +48,6 @@ BatchLaunchFn launcher_for(int group) {
         case 4: return launch_batch_g4;
         default: return nullptr;
     }
-}
-
-int choose_group_for(uint64_t S) {
-    const int b = S ? 64 - __builtin_clzll(S) : 0;
-    if (b <= 7) return 4;
-    if (b <= 10) return 3;
-    if (b <= 15) return 2;
-    return 0;
 }
 
 struct Class {
@@ -222,18 +215,8 @@ int build_plan(BatchPlan& plan, const int64_t* entries, const int32_t* orders, i
                 set_error("batch moduli must be odd and lie in [3, 2^31)");
                 return -1;
             }
-            const int64_t* M = entries + entry_off;
-            uint64_t S = 0;
-            for (int r = 0; r < n; ++r) {
-                uint64_t row = 0;
-                for (int c = 0; c < n; ++c) {
-                    int64_t v = host_floor_mod(M[(size_t)r * n + c], q);
-                    if (v > q / 2) v -= q;
-                    row += (uint64_t)(v < 0 ? -v : v);
-                }
-                S = std::max(S, row);
-            }
-            pend.push_back({m, entry_off, n, (uint32_t)q, choose_group_for(S)});
+            const uint64_t S = centred_row_bound(entries + entry_off, n, q);
+            pend.push_back({m, entry_off, n, (uint32_t)q, choose_group(S)});
         }
         entry_off += (uint64_t)n * n;
         mod_off += moduli_counts[m];
@@ -268,28 +251,9 @@ int build_plan(BatchPlan& plan, const int64_t* entries, const int32_t* orders, i
         const uint64_t q = pd.q;
         jb.scale = (uint32_t)host_mulmod(host_powmod((1ull << 32) % q, (uint64_t)(groups - 1), q),
                                          host_powmod((q + 1) / 2, (uint64_t)L, q), q);
-        const int64_t* M = entries + pd.entry_off;
-        blob.resize(blob.size() + (size_t)n * n);
-        int32_t* rs0 = blob.data() + jb.data_off;
-        int32_t* D = rs0 + n;
-        for (int i = 0; i < n; ++i) {
-            int64_t row = 0;
-            for (int jj = 0; jj < n; ++jj) {
-                const int64_t r = host_floor_mod(M[(size_t)i * n + jj], (int64_t)q);
-                int64_t v, d;
-                if (pd.group == 0) {
-                    v = r;
-                    row = (row + r) % (int64_t)q;
-                    d = (2 * r) % (int64_t)q;
-                } else {
-                    v = r > (int64_t)(q / 2) ? r - (int64_t)q : r;
-                    row += v;
-                    d = 2 * v;
-                }
-                if (jj < n - 1) D[(size_t)jj * n + i] = (int32_t)d;
-            }
-            rs0[i] = (int32_t)row;
-        }
+        const size_t words = (size_t)n * glynn_stride(n, pd.group);
+        blob.resize(blob.size() + words);
+        build_glynn_tables(entries + pd.entry_off, n, pd.q, pd.group, blob.data() + jb.data_off);
         plan.jobs[j] = jb;
         plan.total_states += std::ldexp(1.0, L);
         plan.total_mulmods += (double)n * std::ldexp(1.0, L);

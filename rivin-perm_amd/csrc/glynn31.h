@@ -17,20 +17,25 @@
 // prime at once: blockIdx.y selects the prime slice.
 //
 // Row-sum representations (template parameter G):
-//   G = 0     row sums are residues in [0, p); a flip adds or subtracts
-//             d = 2a mod p and reduces with one unsigned min.
+//   G = 0     row sums are residues in [0, p), one per VGPR; a flip adds or
+//             subtracts d = 2a mod p and reduces with one unsigned min.
 //   G = 2..4  "small" matrices whose centred residues have maximum absolute
-//             row sum S with S^G < 2^31 and S^(G-1) < 2^23: row sums are the
-//             exact integers (one add per row, no reduction) and each group
-//             of G row sums is multiplied exactly with 24-bit multiplies
-//             before one Montgomery step, so a state costs ceil(n/G)-1
-//             Montgomery products instead of n-1.
+//             row sum S with S^G < 2^31 and S^(G-1) < 2^23 (so S < 2^15):
+//             row sums are the exact integers, two per VGPR as 16-bit
+//             halves, so a flip is one v_pk_add_u16 / v_pk_sub_i16 per two
+//             rows (wrapping 16-bit arithmetic is exact because every true
+//             row sum lies in (-2^15, 2^15)).  Each group of G row sums is
+//             multiplied exactly with v_mul_i32_i24 reading the halves
+//             through SDWA word selects, before one Montgomery step, so a
+//             state costs ceil(n/G)-1 Montgomery products instead of n-1.
 // Products are combined with the signed Montgomery multiply in a balanced
 // tree; the accumulated sum therefore carries R^-(groups-1), which the
 // finalisation step removes together with 2^-(n-1).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <utility>
 
 #include "phip_arith.h"
 #include "launchers.h"
@@ -65,157 +70,244 @@ __device__ __forceinline__ const int32_t* opaque_ptr(const int32_t* p) {
     return p;
 }
 
-template <int NR, int G>
-struct GlynnRows {
-    static constexpr bool kWide = (G == 0);
-    static constexpr int kGroups = kWide ? NR : (NR + G - 1) / G;
+typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
 
-    // Product of the row sums times R^-(kGroups-1); |result| < 2^31.
-    __device__ __forceinline__ static int32_t term(const int32_t (&rs)[NR], int32_t negp,
-                                                   uint32_t pinv) {
-        int32_t v[kGroups];
-#pragma unroll
-        for (int g = 0; g < kGroups; ++g) {
-            if constexpr (kWide) {
-                v[g] = rs[g];
-            } else {
-                int32_t gp = rs[g * G];
-#pragma unroll
-                for (int e = 1; e < G; ++e) {
-                    if (g * G + e < NR) gp = mul_i24(gp, rs[g * G + e]);
-                }
-                v[g] = gp;
-            }
-        }
-#pragma unroll
-        for (int w = 1; w < kGroups; w *= 2) {
-#pragma unroll
-            for (int i = 0; i + w < kGroups; i += 2 * w) {
-                v[i] = mont31(v[i], v[i + w], negp, pinv);
-            }
-        }
-        return v[0];
+// v_mul_i32_i24 on sign-extended 16-bit halves (S = 0: low, 1: high) or on a
+// full dword (S = 2), through SDWA source selects.
+template <int S0, int S1>
+__device__ __forceinline__ int32_t mul_sel(uint32_t a, uint32_t b);
+
+#define PHIP_MUL_SEL(S0, S1, OPS, SELS)                                                             \
+    template <>                                                                                     \
+    __device__ __forceinline__ int32_t mul_sel<S0, S1>(uint32_t a, uint32_t b) {                    \
+        int32_t r;                                                                                  \
+        asm("v_mul_i32_i24_sdwa %0, " OPS " dst_sel:DWORD dst_unused:UNUSED_PAD " SELS              \
+            : "=v"(r)                                                                               \
+            : "v"(a), "v"(b));                                                                      \
+        return r;                                                                                   \
     }
+PHIP_MUL_SEL(0, 0, "sext(%1), sext(%2)", "src0_sel:WORD_0 src1_sel:WORD_0")
+PHIP_MUL_SEL(0, 1, "sext(%1), sext(%2)", "src0_sel:WORD_0 src1_sel:WORD_1")
+PHIP_MUL_SEL(1, 0, "sext(%1), sext(%2)", "src0_sel:WORD_1 src1_sel:WORD_0")
+PHIP_MUL_SEL(1, 1, "sext(%1), sext(%2)", "src0_sel:WORD_1 src1_sel:WORD_1")
+PHIP_MUL_SEL(2, 0, "%1, sext(%2)", "src0_sel:DWORD src1_sel:WORD_0")
+PHIP_MUL_SEL(2, 1, "%1, sext(%2)", "src0_sel:DWORD src1_sel:WORD_1")
+#undef PHIP_MUL_SEL
 
-    __device__ __forceinline__ static int32_t add_row(int32_t r, int32_t d, int32_t p) {
-        if constexpr (kWide) {
-            uint32_t x = (uint32_t)r + (uint32_t)d;
-            uint32_t y = x - (uint32_t)p;
-            return (int32_t)(x < y ? x : y);
-        } else {
-            return r + d;
-        }
+__device__ __forceinline__ uint32_t as_u32(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ u16x2 as_u16x2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
+
+// Row sums as residues in [0, p), one per register (G = 0).
+template <int NR>
+struct WideRows {
+    static constexpr int kStride = NR;  // table words per column
+    static constexpr int kGroups = NR;
+    using Word = int32_t;
+    int32_t r[NR];
+
+    __device__ __forceinline__ static int32_t add(int32_t x, int32_t d, int32_t p) {
+        uint32_t a = (uint32_t)x + (uint32_t)d;
+        uint32_t b = a - (uint32_t)p;
+        return (int32_t)(a < b ? a : b);
     }
-
-    __device__ __forceinline__ static int32_t sub_row(int32_t r, int32_t d, int32_t p) {
-        if constexpr (kWide) {
-            uint32_t x = (uint32_t)r - (uint32_t)d;
-            uint32_t y = x + (uint32_t)p;
-            return (int32_t)(x < y ? x : y);
-        } else {
-            return r - d;
-        }
+    __device__ __forceinline__ static int32_t sub(int32_t x, int32_t d, int32_t p) {
+        uint32_t a = (uint32_t)x - (uint32_t)d;
+        uint32_t b = a + (uint32_t)p;
+        return (int32_t)(a < b ? a : b);
     }
-
+    __device__ __forceinline__ void load(const int32_t* __restrict__ t) {
+#pragma unroll
+        for (int i = 0; i < NR; ++i) r[i] = t[i];
+    }
+    __device__ __forceinline__ void sub_masked(const int32_t* __restrict__ col, int32_t mask, int32_t p) {
+#pragma unroll
+        for (int i = 0; i < NR; ++i) r[i] = sub(r[i], col[i] & mask, p);
+    }
     template <bool SUB>
-    __device__ __forceinline__ static void flip(int32_t (&rs)[NR], const int32_t (&d)[NR], int32_t p) {
+    __device__ __forceinline__ void flip(const Word (&d)[kStride], int32_t p) {
 #pragma unroll
-        for (int i = 0; i < NR; ++i) rs[i] = SUB ? sub_row(rs[i], d[i], p) : add_row(rs[i], d[i], p);
+        for (int i = 0; i < NR; ++i) r[i] = SUB ? sub(r[i], d[i], p) : add(r[i], d[i], p);
     }
-
     template <bool SUB>
-    __device__ __forceinline__ static void flip_mem(int32_t (&rs)[NR], const int32_t* __restrict__ col,
-                                                    int32_t p) {
+    __device__ __forceinline__ void flip_mem(const int32_t* __restrict__ col, int32_t p) {
         const int32_t* c = opaque_ptr(col);
 #pragma unroll
-        for (int i = 0; i < NR; ++i) rs[i] = SUB ? sub_row(rs[i], c[i], p) : add_row(rs[i], c[i], p);
+        for (int i = 0; i < NR; ++i) r[i] = SUB ? sub(r[i], c[i], p) : add(r[i], c[i], p);
     }
-
-    // Flip whose direction may differ between lanes.
-    __device__ __forceinline__ static void flip_lane(int32_t (&rs)[NR], const int32_t* __restrict__ col,
-                                                     bool sub, int32_t p) {
+    __device__ __forceinline__ void flip_lane(const int32_t* __restrict__ col, bool sub_dir, int32_t p) {
         const int32_t* c = opaque_ptr(col);
 #pragma unroll
         for (int i = 0; i < NR; ++i) {
             const int32_t d = c[i];
-            if constexpr (kWide) {
-                const int32_t e = sub ? p - d : d;  // p - 0 = p still reduces to rs
-                rs[i] = add_row(rs[i], e, p);
-            } else {
-                rs[i] += sub ? -d : d;
-            }
+            r[i] = add(r[i], sub_dir ? p - d : d, p);  // p - 0 = p still reduces to r
         }
+    }
+    // Product of the row sums times R^-(NR-1); |result| < 2^31.
+    __device__ __forceinline__ int32_t term(int32_t negp, uint32_t pinv) const {
+        int32_t v[NR];
+#pragma unroll
+        for (int i = 0; i < NR; ++i) v[i] = r[i];
+#pragma unroll
+        for (int w = 1; w < NR; w *= 2) {
+#pragma unroll
+            for (int i = 0; i + w < NR; i += 2 * w) v[i] = mont31(v[i], v[i + w], negp, pinv);
+        }
+        return v[0];
     }
 };
 
-// One lane's chunk: the 2^cbits Gray states starting at `base` (a multiple
-// of 2^cbits).  rs0 holds the NR row sums of state 0 (all column signs +1:
-// exact centred sums when G > 0, residues when G = 0; padding rows hold 1),
-// D the flip deltas of the n-1 Gray columns, column-major with stride NR
-// (2a_ij centred, or 2a_ij mod p; padding rows hold 0).  Returns the lane's
-// signed sum of terms reduced into [0, p); it still carries the factor
-// R^-(groups-1) and lacks 2^-(n-1).
+// Exact small row sums, two per register as 16-bit halves (G = 2..4).
 template <int NR, int G>
-__device__ __forceinline__ uint64_t glynn_chunk(const int32_t* __restrict__ rs0,
-                                                const int32_t* __restrict__ D, int n, int cbits,
+struct PackedRows {
+    static constexpr int kStride = (NR + 1) / 2;
+    static constexpr int kGroups = (NR + G - 1) / G;
+    using Word = u16x2;
+    u16x2 r[kStride];
+
+    __device__ __forceinline__ void load(const int32_t* __restrict__ t) {
+#pragma unroll
+        for (int k = 0; k < kStride; ++k) r[k] = as_u16x2((uint32_t)t[k]);
+    }
+    __device__ __forceinline__ void sub_masked(const int32_t* __restrict__ col, int32_t mask, int32_t) {
+#pragma unroll
+        for (int k = 0; k < kStride; ++k) r[k] = r[k] - as_u16x2((uint32_t)(col[k] & mask));
+    }
+    template <bool SUB>
+    __device__ __forceinline__ void flip(const Word (&d)[kStride], int32_t) {
+#pragma unroll
+        for (int k = 0; k < kStride; ++k) r[k] = SUB ? r[k] - d[k] : r[k] + d[k];
+    }
+    template <bool SUB>
+    __device__ __forceinline__ void flip_mem(const int32_t* __restrict__ col, int32_t) {
+        const int32_t* c = opaque_ptr(col);
+#pragma unroll
+        for (int k = 0; k < kStride; ++k) {
+            const u16x2 d = as_u16x2((uint32_t)c[k]);
+            r[k] = SUB ? r[k] - d : r[k] + d;
+        }
+    }
+    __device__ __forceinline__ void flip_lane(const int32_t* __restrict__ col, bool sub_dir, int32_t) {
+        const int32_t* c = opaque_ptr(col);
+#pragma unroll
+        for (int k = 0; k < kStride; ++k) {
+            const u16x2 d = as_u16x2((uint32_t)c[k]);
+            r[k] = sub_dir ? r[k] - d : r[k] + d;
+        }
+    }
+
+    template <int ROW>
+    __device__ __forceinline__ uint32_t word() const {
+        return as_u32(r[ROW / 2]);
+    }
+
+    // Exact product of the group's row sums (|result| < 2^31).
+    template <int GI>
+    __device__ __forceinline__ int32_t group_product() const {
+        constexpr int first = GI * G;
+        constexpr int count = (NR - first < G) ? NR - first : G;
+        if constexpr (count == 1) {
+            const uint32_t w = word<first>();
+            return (first & 1) ? ((int32_t)w >> 16) : (int32_t)(int16_t)(w & 0xffffu);
+        } else {
+            int32_t gp = mul_sel<first & 1, (first + 1) & 1>(word<first>(), word<first + 1>());
+            if constexpr (count >= 3) gp = mul_sel<2, (first + 2) & 1>((uint32_t)gp, word<first + 2>());
+            if constexpr (count >= 4) gp = mul_sel<2, (first + 3) & 1>((uint32_t)gp, word<first + 3>());
+            return gp;
+        }
+    }
+
+    template <int... GI>
+    __device__ __forceinline__ void groups(int32_t (&v)[kGroups], std::integer_sequence<int, GI...>) const {
+        ((v[GI] = group_product<GI>()), ...);
+    }
+
+    // Product of the row sums times R^-(kGroups-1); |result| < 2^31.
+    __device__ __forceinline__ int32_t term(int32_t negp, uint32_t pinv) const {
+        int32_t v[kGroups];
+        groups(v, std::make_integer_sequence<int, kGroups>{});
+#pragma unroll
+        for (int w = 1; w < kGroups; w *= 2) {
+#pragma unroll
+            for (int i = 0; i + w < kGroups; i += 2 * w) v[i] = mont31(v[i], v[i + w], negp, pinv);
+        }
+        return v[0];
+    }
+};
+
+template <int NR, int G>
+using GlynnRows = typename std::conditional<G == 0, WideRows<NR>, PackedRows<NR, G>>::type;
+
+// One lane's chunk: the 2^cbits Gray states starting at `base` (a multiple
+// of 2^cbits).  `tables` holds the row sums of state 0 (all column signs +1:
+// exact centred sums for G > 0, packed two per word, or residues for G = 0)
+// followed by the flip deltas of the n-1 Gray columns, column-major with
+// glynn_stride(NR, G) words per column (2a_ij centred and packed, or
+// 2a_ij mod p).  Returns the lane's signed sum of terms reduced into
+// [0, p); it still carries R^-(groups-1) and lacks 2^-(n-1).
+template <int NR, int G>
+__device__ __forceinline__ uint64_t glynn_chunk(const int32_t* __restrict__ tables, int n, int cbits,
                                                 uint64_t base, int32_t p, uint32_t pinv) {
     using Rows = GlynnRows<NR, G>;
+    constexpr int W = Rows::kStride;
+    const int32_t* __restrict__ D = tables + W;
     const int32_t negp = -p;
     const uint64_t gray = base ^ (base >> 1);
 
     // Row sums of the chunk's first state: subtract 2a for every set Gray
     // bit (perm_c_omp.c:582-600).  Bits below c-1 of gray(base) are zero.
-    int32_t rs[NR];
-#pragma unroll
-    for (int i = 0; i < NR; ++i) rs[i] = rs0[i];
+    Rows rs;
+    rs.load(tables);
     for (int j = cbits - 1; j < n - 1; ++j) {
-        const int32_t mask = -(int32_t)((gray >> j) & 1u);
-        const int32_t* __restrict__ col = D + j * NR;
-#pragma unroll
-        for (int i = 0; i < NR; ++i) rs[i] = Rows::sub_row(rs[i], col[i] & mask, p);
+        rs.sub_masked(D + j * W, -(int32_t)((gray >> j) & 1u), p);
     }
 
     // Column 0 flips every other state; keep it resident.
-    int32_t c0[NR];
+    typename Rows::Word c0[W];
 #pragma unroll
-    for (int i = 0; i < NR; ++i) c0[i] = D[i];
-    const int32_t* __restrict__ d1 = D + NR;
-    const int32_t* __restrict__ d2 = D + 2 * NR;
+    for (int k = 0; k < W; ++k) {
+        if constexpr (G == 0) {
+            c0[k] = D[k];
+        } else {
+            c0[k] = as_u16x2((uint32_t)D[k]);
+        }
+    }
+    const int32_t* __restrict__ d1 = D + W;
+    const int32_t* __restrict__ d2 = D + 2 * W;
 
     // Terms at even local index carry the chunk's starting sign, odd ones
     // the opposite sign.
-    int64_t acc_e = Rows::term(rs, negp, pinv);
+    int64_t acc_e = rs.term(negp, pinv);
     int64_t acc_o = 0;
     const uint32_t T = 1u << cbits;
     for (uint32_t t0 = 0; t0 < T; t0 += kGlynnUnroll) {
         // Directions for s = 1..15 follow from bit j+1 of t0+s, which lies
         // inside the unrolled block except for s = 8 (bit 4 of base+t0).
-        Rows::template flip<true>(rs, c0, p);         acc_o += Rows::term(rs, negp, pinv);  // s=1
-        Rows::template flip_mem<true>(rs, d1, p);     acc_e += Rows::term(rs, negp, pinv);  // s=2
-        Rows::template flip<false>(rs, c0, p);        acc_o += Rows::term(rs, negp, pinv);  // s=3
-        Rows::template flip_mem<true>(rs, d2, p);     acc_e += Rows::term(rs, negp, pinv);  // s=4
-        Rows::template flip<true>(rs, c0, p);         acc_o += Rows::term(rs, negp, pinv);  // s=5
-        Rows::template flip_mem<false>(rs, d1, p);    acc_e += Rows::term(rs, negp, pinv);  // s=6
-        Rows::template flip<false>(rs, c0, p);        acc_o += Rows::term(rs, negp, pinv);  // s=7
+        rs.template flip<true>(c0, p);         acc_o += rs.term(negp, pinv);  // s=1
+        rs.template flip_mem<true>(d1, p);     acc_e += rs.term(negp, pinv);  // s=2
+        rs.template flip<false>(c0, p);        acc_o += rs.term(negp, pinv);  // s=3
+        rs.template flip_mem<true>(d2, p);     acc_e += rs.term(negp, pinv);  // s=4
+        rs.template flip<true>(c0, p);         acc_o += rs.term(negp, pinv);  // s=5
+        rs.template flip_mem<false>(d1, p);    acc_e += rs.term(negp, pinv);  // s=6
+        rs.template flip<false>(c0, p);        acc_o += rs.term(negp, pinv);  // s=7
         {
             // s=8 flips column 3; its new Gray bit is 1 ^ bit 4 of (base+t).
             const uint64_t t = base + t0 + 8;
-            Rows::flip_lane(rs, D + 3 * NR, !((t >> 4) & 1u), p);
-            acc_e += Rows::term(rs, negp, pinv);
+            rs.flip_lane(D + 3 * W, !((t >> 4) & 1u), p);
+            acc_e += rs.term(negp, pinv);
         }
-        Rows::template flip<true>(rs, c0, p);         acc_o += Rows::term(rs, negp, pinv);  // s=9
-        Rows::template flip_mem<true>(rs, d1, p);     acc_e += Rows::term(rs, negp, pinv);  // s=10
-        Rows::template flip<false>(rs, c0, p);        acc_o += Rows::term(rs, negp, pinv);  // s=11
-        Rows::template flip_mem<false>(rs, d2, p);    acc_e += Rows::term(rs, negp, pinv);  // s=12
-        Rows::template flip<true>(rs, c0, p);         acc_o += Rows::term(rs, negp, pinv);  // s=13
-        Rows::template flip_mem<false>(rs, d1, p);    acc_e += Rows::term(rs, negp, pinv);  // s=14
-        Rows::template flip<false>(rs, c0, p);        acc_o += Rows::term(rs, negp, pinv);  // s=15
+        rs.template flip<true>(c0, p);         acc_o += rs.term(negp, pinv);  // s=9
+        rs.template flip_mem<true>(d1, p);     acc_e += rs.term(negp, pinv);  // s=10
+        rs.template flip<false>(c0, p);        acc_o += rs.term(negp, pinv);  // s=11
+        rs.template flip_mem<false>(d2, p);    acc_e += rs.term(negp, pinv);  // s=12
+        rs.template flip<true>(c0, p);         acc_o += rs.term(negp, pinv);  // s=13
+        rs.template flip_mem<false>(d1, p);    acc_e += rs.term(negp, pinv);  // s=14
+        rs.template flip<false>(c0, p);        acc_o += rs.term(negp, pinv);  // s=15
         const uint32_t tn = t0 + kGlynnUnroll;
         if (tn < T) {
             const int j = __builtin_ctz(tn);  // 4 <= j <= cbits-1, uniform
             const uint64_t t = base + tn;
-            Rows::flip_lane(rs, D + j * NR, !((t >> (j + 1)) & 1u), p);
-            acc_e += Rows::term(rs, negp, pinv);
+            rs.flip_lane(D + j * W, !((t >> (j + 1)) & 1u), p);
+            acc_e += rs.term(negp, pinv);
         }
     }
 
@@ -226,19 +318,17 @@ __device__ __forceinline__ uint64_t glynn_chunk(const int32_t* __restrict__ rs0,
     return (uint64_t)r;
 }
 
-// Per-prime slice layout (all int32):
-//   rs0    [k][NR]       row sums of state 0 (see glynn_chunk)
-//   deltas [k][n-1][NR]  flip deltas (see glynn_chunk)
-//   partials[k][gridDim.x] receives each block's sum of lane residues.
+// tables[k] holds one prime's glynn_chunk tables (n * stride words);
+// partials[k][gridDim.x] receives each block's sum of lane residues.
 template <int NR, int G>
 __global__ __launch_bounds__(kGlynnBlock) void glynn31_kernel(
-    const int32_t* __restrict__ rs0, const int32_t* __restrict__ deltas,
-    const int32_t* __restrict__ primes, const uint32_t* __restrict__ pinvs, int n, int cbits,
-    uint64_t lanes_per_prime, uint64_t* __restrict__ partials) {
+    const int32_t* __restrict__ tables, const int32_t* __restrict__ primes,
+    const uint32_t* __restrict__ pinvs, int n, int cbits, uint64_t lanes_per_prime,
+    uint64_t* __restrict__ partials) {
     const int q = blockIdx.y;
     const uint64_t lane = (uint64_t)blockIdx.x * kGlynnBlock + threadIdx.x;
     const bool active = lane < lanes_per_prime;
-    const uint64_t r = glynn_chunk<NR, G>(rs0 + (size_t)q * NR, deltas + (size_t)q * (n - 1) * NR, n, cbits,
+    const uint64_t r = glynn_chunk<NR, G>(tables + (size_t)q * n * GlynnRows<NR, G>::kStride, n, cbits,
                                           (active ? lane : 0) << cbits, primes[q], pinvs[q]);
     const uint64_t block_sum = block_reduce_sum_u64<kGlynnBlock>(active ? r : 0u);
     if (threadIdx.x == 0) partials[(size_t)q * gridDim.x + blockIdx.x] = block_sum;

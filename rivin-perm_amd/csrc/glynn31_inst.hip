@@ -18,8 +18,8 @@ hipError_t launch_nr(int nr, dim3 grid, hipStream_t stream, const Glynn31Args& a
         return hipErrorInvalidValue;
     } else {
         if (nr != NR) return launch_nr<NR + 1>(nr, grid, stream, a);
-        hipLaunchKernelGGL((glynn31_kernel<NR, PHIP_GROUP>), grid, dim3(kGlynnBlock), 0, stream, a.rs0,
-                           a.deltas, a.primes, a.pinvs, a.n, a.cbits, a.lanes_per_prime, a.partials);
+        hipLaunchKernelGGL((glynn31_kernel<NR, PHIP_GROUP>), grid, dim3(kGlynnBlock), 0, stream, a.tables,
+                           a.primes, a.pinvs, a.n, a.cbits, a.lanes_per_prime, a.partials);
         return hipGetLastError();
     }
 }

@@ -12,9 +12,12 @@ constexpr int kGlynnMinOrder = 5;
 constexpr int kGlynnMaxOrder = 40;
 constexpr int kGlynnBlock = 256;  // threads per workgroup of glynn31_kernel
 
+// Words per table column: residues one per word (group 0) or exact row sums
+// packed two 16-bit halves per word (groups 2..4).
+__host__ __device__ constexpr int glynn_stride(int n, int group) { return group == 0 ? n : (n + 1) / 2; }
+
 struct Glynn31Args {
-    const int32_t* rs0;
-    const int32_t* deltas;
+    const int32_t* tables;  // per prime: n * glynn_stride words (state-0 row sums, then n-1 delta columns)
     const int32_t* primes;
     const uint32_t* pinvs;
     int n;

@@ -16,6 +16,7 @@
 #include "generic_kernels.h"
 #include "launchers.h"
 #include "phip_arith.h"
+#include "tables.h"
 
 namespace phip {
 
@@ -27,16 +28,6 @@ Tuning& tuning() {
 namespace {
 
 int bit_length(uint64_t v) { return v ? 64 - __builtin_clzll(v) : 0; }
-
-// Largest group size whose exact products fit the kernel's bounds given the
-// maximum absolute row sum S (S^G < 2^31, S^(G-1) < 2^23); 0 = wide rows.
-int choose_group(uint64_t S) {
-    const int b = bit_length(S);
-    if (b <= 7) return 4;
-    if (b <= 10) return 3;
-    if (b <= 15) return 2;
-    return 0;
-}
 
 int ceil_log2(uint64_t v) {
     int l = 0;
@@ -60,34 +51,16 @@ int run_glynn31(DeviceContext* ctx, const int64_t* A, int64_t stride, int n, int
                 const std::vector<OddJob>& jobs, std::vector<uint64_t>& odd_residue) {
     const int kk = (int)jobs.size();
     const int NR = n;
-    std::vector<int32_t> rs0((size_t)kk * NR), deltas((size_t)kk * (n - 1) * NR), primes(kk);
+    const size_t words = (size_t)n * glynn_stride(n, group);
+    std::vector<int32_t> tables((size_t)kk * words), primes(kk);
     std::vector<uint32_t> pinvs(kk);
     std::vector<uint64_t> moduli(kk);
     for (int s = 0; s < kk; ++s) {
         const uint32_t q = (uint32_t)jobs[s].q;
-        const int64_t* M = matrix_of(A, stride, jobs[s].index);
         primes[s] = (int32_t)q;
         pinvs[s] = inverse_mod_2_32(q);
         moduli[s] = q;
-        for (int i = 0; i < n; ++i) {
-            int64_t row = 0;
-            for (int j = 0; j < n; ++j) {
-                const int64_t r = host_floor_mod(M[(size_t)i * n + j], (int64_t)q);
-                int64_t v;
-                if (group == 0) {
-                    v = r;
-                    row = (row + r) % (int64_t)q;
-                } else {
-                    v = (r > (int64_t)(q / 2)) ? r - (int64_t)q : r;
-                    row += v;
-                }
-                if (j < n - 1) {
-                    const int64_t d = (group == 0) ? (2 * r) % (int64_t)q : 2 * v;
-                    deltas[((size_t)s * (n - 1) + j) * NR + i] = (int32_t)d;
-                }
-            }
-            rs0[(size_t)s * NR + i] = (int32_t)row;
-        }
+        build_glynn_tables(matrix_of(A, stride, jobs[s].index), n, q, group, tables.data() + (size_t)s * words);
     }
 
     int cbits = (n - 1) + ceil_log2((uint64_t)kk) - tuning().lanes_log2;
@@ -101,23 +74,21 @@ int run_glynn31(DeviceContext* ctx, const int64_t* A, int64_t stride, int n, int
     }
 
     hipStream_t st = ctx->stream();
-    auto* d_rs0 = (int32_t*)ctx->buffer(0, rs0.size() * 4);
-    auto* d_del = (int32_t*)ctx->buffer(1, deltas.size() * 4);
+    auto* d_tab = (int32_t*)ctx->buffer(0, tables.size() * 4);
     auto* d_pr = (int32_t*)ctx->buffer(2, primes.size() * 4);
     auto* d_pi = (uint32_t*)ctx->buffer(3, pinvs.size() * 4);
     auto* d_part = (uint64_t*)ctx->buffer(4, (size_t)kk * gx * 8);
     auto* d_mod = (uint64_t*)ctx->buffer(5, moduli.size() * 8);
     auto* d_sum = (uint64_t*)ctx->buffer(6, moduli.size() * 8);
     auto* h_sum = (uint64_t*)ctx->host_buffer(0, moduli.size() * 8);
-    if (!d_rs0 || !d_del || !d_pr || !d_pi || !d_part || !d_mod || !d_sum || !h_sum) return -1;
+    if (!d_tab || !d_pr || !d_pi || !d_part || !d_mod || !d_sum || !h_sum) return -1;
 
-    PHIP_HIP_TRY(hipMemcpyAsync(d_rs0, rs0.data(), rs0.size() * 4, hipMemcpyHostToDevice, st));
-    PHIP_HIP_TRY(hipMemcpyAsync(d_del, deltas.data(), deltas.size() * 4, hipMemcpyHostToDevice, st));
+    PHIP_HIP_TRY(hipMemcpyAsync(d_tab, tables.data(), tables.size() * 4, hipMemcpyHostToDevice, st));
     PHIP_HIP_TRY(hipMemcpyAsync(d_pr, primes.data(), primes.size() * 4, hipMemcpyHostToDevice, st));
     PHIP_HIP_TRY(hipMemcpyAsync(d_pi, pinvs.data(), pinvs.size() * 4, hipMemcpyHostToDevice, st));
     PHIP_HIP_TRY(hipMemcpyAsync(d_mod, moduli.data(), moduli.size() * 8, hipMemcpyHostToDevice, st));
 
-    Glynn31Args args{d_rs0, d_del, d_pr, d_pi, n, cbits, lanes, d_part};
+    Glynn31Args args{d_tab, d_pr, d_pi, n, cbits, lanes, d_part};
     if (ctx->profiling) PHIP_HIP_TRY(hipEventRecord(ctx->prof_start, st));
     PHIP_HIP_TRY(launch_glynn31(group, NR, dim3((unsigned)gx, (unsigned)kk), st, args));
     if (ctx->profiling) PHIP_HIP_TRY(hipEventRecord(ctx->prof_stop, st));
@@ -308,17 +279,7 @@ int compute_residues(DeviceContext* ctx, const int64_t* A, int64_t stride, int n
         if (q < (1ull << 31) && n >= kGlynnMinOrder && n <= kGlynnMaxOrder) {
             // Maximum absolute row sum of the centred residues picks the
             // row-sum representation.
-            const int64_t* M = matrix_of(A, stride, i);
-            uint64_t S = 0;
-            for (int r = 0; r < n; ++r) {
-                uint64_t row = 0;
-                for (int c = 0; c < n; ++c) {
-                    int64_t v = host_floor_mod(M[(size_t)r * n + c], (int64_t)q);
-                    if (v > (int64_t)(q / 2)) v -= (int64_t)q;
-                    row += (uint64_t)(v < 0 ? -v : v);
-                }
-                S = std::max(S, row);
-            }
+            const uint64_t S = centred_row_bound(matrix_of(A, stride, i), n, (int64_t)q);
             by_group[choose_group(S)].push_back({i, q});
         } else {
             wide.push_back({i, q});

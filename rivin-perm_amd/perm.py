@@ -227,6 +227,8 @@ def _schur_residues(n: int, primes: List[int]) -> List[int]:
     gives n equal permanental cofactors (perm_c_omp.c:964-975), so each
     residue is n * per(minor) with minor entries omega^((r+1)(c+1)).
     """
+    if not primes:
+        return []
     minors = []
     for p in primes:
         omega = nth_root_of_unity_mod_p(n, p)

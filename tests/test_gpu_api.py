@@ -84,8 +84,8 @@ def test_low_rank_method_matches_generic(native):
     A = [[sum(u[i][k] * w[k][j] for k in range(2)) for j in range(n)] for i in range(n)]
     assert perm.matrix_permanent(A, method="low_rank") == perm.matrix_permanent(A, method="generic")
     with pytest.raises(perm.LowRankLimitExceeded):
-        perm.matrix_permanent([[random.Random(1).randrange(9) for _ in range(8)] for _ in range(8)],
-                              method="low_rank")
+        rng = random.Random(1)
+        perm.matrix_permanent([[rng.randrange(9) for _ in range(8)] for _ in range(8)], method="low_rank")
 
 
 def test_golden_reference_residues(native):
