@@ -59,22 +59,30 @@ def log(msg: str) -> None:
 
 
 def instruction_mix(n: int, group: int) -> dict:
-    """Minimal VALU instructions per Gray state of the Glynn kernel."""
+    """VALU instructions per Gray state of glynn_chunk<n, group> (the
+    unrolled 16-state step, amortised): row-sum flips, exact group products,
+    the Montgomery tree (two v_mad_i64_i32 and one v_mul_lo_u32 per product)
+    and the signed accumulation."""
     groups = n if group == 0 else -(-n // group)
-    mix = {"v_add_u32": 3 + (3 * n if group == 0 else n)}
-    if group:
-        mix["v_mul_i32_i24"] = n - groups
+    mix = {}
+    if group == 0:
+        mix["v_add_u32"] = 3 * n + 3  # add/sub, wrap, min per row; accumulate
+    else:
+        mix["v_pk_add_u16"] = -(-n // 2)
+        mix["v_mul_i32_i24_sdwa"] = n - groups
+        mix["v_add_u32"] = 3 + (1 if n % group == 1 else 0)
     mix["v_mad_i64_i32"] = 2 * (groups - 1)
     mix["v_mul_lo_u32"] = groups - 1
     return mix
 
 
 def roofline(n: int, group: int, rates: dict, mulmods_per_s: float) -> dict:
-    """Mulmod roofline of the kernel's own instruction mix at measured rates."""
+    """Mulmod roofline of the kernel's own instruction mix at the measured
+    instruction rates (the kernel is VALU-issue bound; HBM traffic is a few
+    MB per launch)."""
     mix = instruction_mix(n, group)
     seconds_per_state_lane = sum(count / rates[name] for name, count in mix.items())
-    peak_states = 1.0 / seconds_per_state_lane  # Gray states per second, whole device
-    peak = n * peak_states
+    peak = n / seconds_per_state_lane  # mulmods per second, whole device
     return {
         "bound": "valu",
         "achieved": mulmods_per_s / 1e12,
@@ -82,6 +90,7 @@ def roofline(n: int, group: int, rates: dict, mulmods_per_s: float) -> dict:
         "unit": "Tmulmod/s",
         "frac": mulmods_per_s / peak,
         "traffic": traffic_from_profiles(),
+        "group": group,
         "instructions_per_state": mix,
         # Every mulmod as one signed Montgomery product plus one row-sum add.
         "general_montgomery_peak": 1.0 / (1.0 / rates["mont31_product"] + 1.0 / rates["v_add_u32"]) / 1e12,
@@ -149,6 +158,8 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--kernel-only", action="store_true", help="headline launches only (for profilers)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="run one plain device copy of BYTES (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -161,6 +172,11 @@ def main() -> int:
 
     from permhip import _native, planner
     from permhip.batch import PermanentBatch
+
+    if args.copy_calibration:
+        _native.copy_calibration(args.copy_calibration, 1)
+        log(f"copied {args.copy_calibration} bytes")
+        return 0
 
     info = _native.device_info()
     log(f"device: {info}")
@@ -208,19 +224,18 @@ def main() -> int:
 
     rates = _native.valu_microbench()
     log("VALU lane-ops/s: " + ", ".join(f"{k}={v:.3e}" for k, v in rates.items()))
-    # Row sums of the headline matrix fit exact groups of three.
-    S = int(np.abs(A).sum(axis=1).max())
-    group = 4 if S < 2 ** 7 else 3 if S < 2 ** 10 else 2 if S < 2 ** 15 else 0
+    group = int(plan.stats()["group"])
     roof = roofline(HEADLINE_N, group, rates, kernel_rate)
     roof["kernel"] = f"batch_glynn_kernel<{group}> (n={HEADLINE_N}, {len(primes)} primes per launch)"
     roof["kernel_ms"] = k_ms
 
-    # Plain launch: one synchronous C-ABI call per matrix (host tables, copy,
-    # launch, reduce, copy back), as the reference wrappers do.
+    # Plain launch: one synchronous C-ABI call per (matrix, prime) -- host
+    # tables, copy, launch, reduce, copy back -- as the reference wrappers
+    # call their device code (perm_cuda.py:196-198, perm_binary_cuda.py:133-139).
     plain = []
-    for _ in range(max(3, args.steps // 2)):
+    for _ in range(max(3, args.steps // 4)):
         t0 = time.perf_counter()
-        res = _native.permanent_residues(A, primes)
+        res = [_native.permanent_residues(A, [p])[0] for p in primes]
         plain.append(time.perf_counter() - t0)
     plain_exact = planner.signed_lift(planner.chinese_remainder_theorem(res, primes), bound, product)
     if plain_exact != exact:
@@ -270,17 +285,25 @@ def main() -> int:
                 raise RuntimeError("batch results changed between runs")
         b_s = statistics.median(runs)
         st = batch.stats()
-        # Plain per-matrix launches of the same matrices.
-        t0 = time.perf_counter()
-        plain_res = []
+        # Plain launches of the same matrices: one call per (matrix, prime),
+        # and one call per matrix carrying all of its primes.
+        plans = []
         for M in mats:
             b = planner.matrix_permanent_bound(M.tolist())
             ps, prod = planner.generic_crt_primes(b)
-            r = _native.permanent_residues(M, ps)
+            plans.append((M, b, ps, prod))
+        t0 = time.perf_counter()
+        plain_res = []
+        for M, b, ps, prod in plans:
+            r = [_native.permanent_residues(M, [p])[0] for p in ps]
             plain_res.append(planner.signed_lift(planner.chinese_remainder_theorem(r, ps), b, prod))
         t_plain = time.perf_counter() - t0
         if plain_res != warm:
-            raise RuntimeError("batch and plain per-matrix results disagree")
+            raise RuntimeError("batch and plain per-(matrix, prime) results disagree")
+        t0 = time.perf_counter()
+        for M, b, ps, prod in plans:
+            _native.permanent_residues(M, ps)
+        t_plain_matrix = time.perf_counter() - t0
         result["batch"] = {
             "workload": "4096 integer matrices, n ~ U[12,28], entries U[-100,100], default_rng(4)",
             "matrices_per_s": len(mats) / b_s,
@@ -291,6 +314,8 @@ def main() -> int:
             "items": st.get("items"),
             "plain_matrices_per_s": len(mats) / t_plain,
             "speedup_vs_plain": t_plain / b_s,
+            "plain_all_primes_per_call_matrices_per_s": len(mats) / t_plain_matrix,
+            "speedup_vs_plain_all_primes_per_call": t_plain_matrix / b_s,
             "bit_exact_vs_plain": True,
         }
         log(f"batch: {result['batch']}")

@@ -34,5 +34,7 @@ hipError_t launch_batch_g3(dim3, hipStream_t, const BatchJob*, const uint2*, uin
                            uint64_t*);
 hipError_t launch_batch_g4(dim3, hipStream_t, const BatchJob*, const uint2*, uint32_t, uint32_t*, const int32_t*,
                            uint64_t*);
+hipError_t launch_batch_g6(dim3, hipStream_t, const BatchJob*, const uint2*, uint32_t, uint32_t*, const int32_t*,
+                           uint64_t*);
 
 }  // namespace phip

@@ -46,6 +46,7 @@ BatchLaunchFn launcher_for(int group) {
         case 2: return launch_batch_g2;
         case 3: return launch_batch_g3;
         case 4: return launch_batch_g4;
+        case 6: return launch_batch_g6;
         default: return nullptr;
     }
 }
@@ -195,6 +196,7 @@ int build_plan(BatchPlan& plan, const int64_t* entries, const int32_t* orders, i
         int n;
         uint32_t q;
         int group;
+        std::vector<int> order;
     };
     std::vector<Pending> pend;
     uint64_t entry_off = 0;
@@ -215,8 +217,9 @@ int build_plan(BatchPlan& plan, const int64_t* entries, const int32_t* orders, i
                 set_error("batch moduli must be odd and lie in [3, 2^31)");
                 return -1;
             }
-            const uint64_t S = centred_row_bound(entries + entry_off, n, q);
-            pend.push_back({m, entry_off, n, (uint32_t)q, choose_group(S)});
+            std::vector<int> order;
+            const int group = plan_rows(centred_row_bounds(entries + entry_off, n, q), order);
+            pend.push_back({m, entry_off, n, (uint32_t)q, group, std::move(order)});
         }
         entry_off += (uint64_t)n * n;
         mod_off += moduli_counts[m];
@@ -253,7 +256,7 @@ int build_plan(BatchPlan& plan, const int64_t* entries, const int32_t* orders, i
                                          host_powmod((q + 1) / 2, (uint64_t)L, q), q);
         const size_t words = (size_t)n * glynn_stride(n, pd.group);
         blob.resize(blob.size() + words);
-        build_glynn_tables(entries + pd.entry_off, n, pd.q, pd.group, blob.data() + jb.data_off);
+        build_glynn_tables(entries + pd.entry_off, n, pd.q, pd.group, pd.order, blob.data() + jb.data_off);
         plan.jobs[j] = jb;
         plan.total_states += std::ldexp(1.0, L);
         plan.total_mulmods += (double)n * std::ldexp(1.0, L);
@@ -261,7 +264,7 @@ int build_plan(BatchPlan& plan, const int64_t* entries, const int32_t* orders, i
     // Items: per class, jobs by descending cost, each job's items contiguous.
     std::vector<uint2> items;
     plan.classes.clear();
-    for (int group : {0, 2, 3, 4}) {
+    for (int group : {0, 2, 3, 4, 6}) {
         std::vector<uint32_t> order;
         for (uint32_t j = 0; j < plan.jobs.size(); ++j)
             if (plan.jobs[j].group == group) order.push_back(j);
@@ -415,7 +418,8 @@ int phip_batch_stats(void* handle, double* out, int len) {
                         (double)plan->jobs.size(),
                         (double)plan->n_items,
                         (double)plan->classes.size(),
-                        (double)(plan->cus * plan->blocks_per_cu * 4)};
+                        (double)(plan->cus * plan->blocks_per_cu * 4),
+                        plan->classes.empty() ? -1.0 : (double)plan->classes[0].group};
     const int m = std::min<int>(len, (int)(sizeof(v) / sizeof(v[0])));
     for (int i = 0; i < m; ++i) out[i] = v[i];
     return m;

@@ -19,8 +19,11 @@
 // Row-sum representations (template parameter G):
 //   G = 0     row sums are residues in [0, p), one per VGPR; a flip adds or
 //             subtracts d = 2a mod p and reduces with one unsigned min.
-//   G = 2..4  "small" matrices whose centred residues have maximum absolute
-//             row sum S with S^G < 2^31 and S^(G-1) < 2^23 (so S < 2^15):
+//   G = 2..6  "small" matrices whose centred residues have row bounds
+//             S_i = sum_j |a_ij| < 2^15 and, after the host balances rows
+//             into groups (row order does not change the permanent), every
+//             group product bound below 2^31 and every partial product that
+//             feeds a 24-bit multiply below 2^23:
 //             row sums are the exact integers, two per VGPR as 16-bit
 //             halves, so a flip is one v_pk_add_u16 / v_pk_sub_i16 per two
 //             rows (wrapping 16-bit arithmetic is exact because every true
@@ -200,20 +203,36 @@ struct PackedRows {
         return as_u32(r[ROW / 2]);
     }
 
-    // Exact product of the group's row sums (|result| < 2^31).
+    template <int ROW>
+    __device__ __forceinline__ int32_t row_value() const {
+        const uint32_t w = word<ROW>();
+        return (ROW & 1) ? ((int32_t)w >> 16) : (int32_t)(int16_t)(w & 0xffffu);
+    }
+
+    // Exact product of `C` consecutive row sums starting at FIRST, as
+    //   1: r0   2: r0 r1   3: (r0 r1) r2   4: (r0 r1)(r2 r3)
+    //   5: ((r0 r1) r2)(r3 r4)   6: ((r0 r1) r2)((r3 r4) r5)
+    // with every 24-bit multiply operand below 2^23 in magnitude and the
+    // result below 2^31 (the host checks these bounds when it picks G).
+    template <int FIRST, int C>
+    __device__ __forceinline__ int32_t run_product() const {
+        if constexpr (C == 1) {
+            return row_value<FIRST>();
+        } else if constexpr (C == 2) {
+            return mul_sel<FIRST & 1, (FIRST + 1) & 1>(word<FIRST>(), word<FIRST + 1>());
+        } else if constexpr (C == 3) {
+            return mul_sel<2, (FIRST + 2) & 1>((uint32_t)run_product<FIRST, 2>(), word<FIRST + 2>());
+        } else {
+            constexpr int A = (C + 1) / 2;
+            return mul_i24(run_product<FIRST, A>(), run_product<FIRST + A, C - A>());
+        }
+    }
+
     template <int GI>
     __device__ __forceinline__ int32_t group_product() const {
         constexpr int first = GI * G;
         constexpr int count = (NR - first < G) ? NR - first : G;
-        if constexpr (count == 1) {
-            const uint32_t w = word<first>();
-            return (first & 1) ? ((int32_t)w >> 16) : (int32_t)(int16_t)(w & 0xffffu);
-        } else {
-            int32_t gp = mul_sel<first & 1, (first + 1) & 1>(word<first>(), word<first + 1>());
-            if constexpr (count >= 3) gp = mul_sel<2, (first + 2) & 1>((uint32_t)gp, word<first + 2>());
-            if constexpr (count >= 4) gp = mul_sel<2, (first + 3) & 1>((uint32_t)gp, word<first + 3>());
-            return gp;
-        }
+        return run_product<first, count>();
     }
 
     template <int... GI>

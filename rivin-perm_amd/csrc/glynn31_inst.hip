@@ -1,12 +1,12 @@
 // Instantiates glynn31_kernel<NR, PHIP_GROUP> for every register-resident
 // order NR in [kGlynnMinOrder, kGlynnMaxOrder].  The build compiles this file
-// once per group size (0, 2, 3, 4) so the instantiations spread over
+// once per group size (0, 2, 3, 4, 6) so the instantiations spread over
 // several translation units.
 #include "launchers.h"
 #include "glynn31.h"
 
 #ifndef PHIP_GROUP
-#error "PHIP_GROUP must be defined (0, 2, 3 or 4)"
+#error "PHIP_GROUP must be defined (0, 2, 3, 4 or 6)"
 #endif
 
 namespace phip {

@@ -30,6 +30,7 @@ hipError_t launch_glynn31_g0(int nr, dim3 grid, hipStream_t stream, const Glynn3
 hipError_t launch_glynn31_g2(int nr, dim3 grid, hipStream_t stream, const Glynn31Args& a);
 hipError_t launch_glynn31_g3(int nr, dim3 grid, hipStream_t stream, const Glynn31Args& a);
 hipError_t launch_glynn31_g4(int nr, dim3 grid, hipStream_t stream, const Glynn31Args& a);
+hipError_t launch_glynn31_g6(int nr, dim3 grid, hipStream_t stream, const Glynn31Args& a);
 
 inline hipError_t launch_glynn31(int group, int nr, dim3 grid, hipStream_t stream,
                                  const Glynn31Args& a) {
@@ -38,6 +39,7 @@ inline hipError_t launch_glynn31(int group, int nr, dim3 grid, hipStream_t strea
         case 2: return launch_glynn31_g2(nr, grid, stream, a);
         case 3: return launch_glynn31_g3(nr, grid, stream, a);
         case 4: return launch_glynn31_g4(nr, grid, stream, a);
+        case 6: return launch_glynn31_g6(nr, grid, stream, a);
         default: return hipErrorInvalidValue;
     }
 }

@@ -3,8 +3,9 @@
 //   phip_valu_microbench  throughput of the VALU instructions the Glynn
 //                         kernels issue (v_add_u32, v_mul_i32_i24,
 //                         v_mul_lo_u32, v_mul_hi_i32, v_mad_i64_i32,
-//                         v_mad_u64_u32) and of one complete signed
-//                         Montgomery product, each as lane-operations per
+//                         v_mad_u64_u32, v_pk_add_u16, SDWA v_mul_i32_i24)
+//                         and of one complete signed Montgomery product,
+//                         each as lane-operations per
 //                         second over the whole device.  Eight independent
 //                         dependency chains per lane keep the pipeline full.
 //   phip_copy_calibration a plain device-to-device copy kernel of a known
@@ -60,8 +61,16 @@ __global__ __launch_bounds__(256) void valu_rate_kernel(uint32_t* out, int iters
                     asm volatile("v_mad_i64_i32 %0, vcc, %1, %2, %0" : "+v"(y[k]) : "v"(x[k]), "v"(c) : "vcc");
                 } else if constexpr (OP == 5) {
                     asm volatile("v_mad_u64_u32 %0, vcc, %1, %2, %0" : "+v"(y[k]) : "v"(x[k]), "v"(c) : "vcc");
-                } else {
+                } else if constexpr (OP == 6) {
                     x[k] = (uint32_t)mont31_mb((int32_t)x[k], (int32_t)c, -p, pinv);
+                } else if constexpr (OP == 7) {
+                    asm volatile("v_pk_add_u16 %0, %0, %1" : "+v"(x[k]) : "v"(c));
+                } else {
+                    asm volatile(
+                        "v_mul_i32_i24_sdwa %0, sext(%0), sext(%1) dst_sel:DWORD dst_unused:UNUSED_PAD "
+                        "src0_sel:WORD_1 src1_sel:WORD_0"
+                        : "+v"(x[k])
+                        : "v"(c));
                 }
             }
         }
@@ -100,7 +109,7 @@ int phip_valu_microbench(double* lane_ops_per_s, int len) {
     using Kernel = void (*)(uint32_t*, int, uint32_t);
     const Kernel kernels[] = {phip::valu_rate_kernel<0>, phip::valu_rate_kernel<1>, phip::valu_rate_kernel<2>,
                               phip::valu_rate_kernel<3>, phip::valu_rate_kernel<4>, phip::valu_rate_kernel<5>,
-                              phip::valu_rate_kernel<6>};
+                              phip::valu_rate_kernel<6>, phip::valu_rate_kernel<7>, phip::valu_rate_kernel<8>};
     const int nk = (int)(sizeof(kernels) / sizeof(kernels[0]));
     int written = 0;
     for (int k = 0; k < nk && k < len; ++k) {

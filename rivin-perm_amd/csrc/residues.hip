@@ -40,6 +40,7 @@ uint64_t pow2_mod(int e, uint64_t q) { return host_powmod(2, (uint64_t)e, q); }
 struct OddJob {
     int index;   // position in the caller's modulus list
     uint64_t q;  // odd part, > 1
+    std::vector<int> order = {};  // kernel row order (packed groups)
 };
 
 const int64_t* matrix_of(const int64_t* A, int64_t stride, int index) { return A + (int64_t)index * stride; }
@@ -60,7 +61,8 @@ int run_glynn31(DeviceContext* ctx, const int64_t* A, int64_t stride, int n, int
         primes[s] = (int32_t)q;
         pinvs[s] = inverse_mod_2_32(q);
         moduli[s] = q;
-        build_glynn_tables(matrix_of(A, stride, jobs[s].index), n, q, group, tables.data() + (size_t)s * words);
+        build_glynn_tables(matrix_of(A, stride, jobs[s].index), n, q, group, jobs[s].order,
+                           tables.data() + (size_t)s * words);
     }
 
     int cbits = (n - 1) + ceil_log2((uint64_t)kk) - tuning().lanes_log2;
@@ -279,8 +281,9 @@ int compute_residues(DeviceContext* ctx, const int64_t* A, int64_t stride, int n
         if (q < (1ull << 31) && n >= kGlynnMinOrder && n <= kGlynnMaxOrder) {
             // Maximum absolute row sum of the centred residues picks the
             // row-sum representation.
-            const uint64_t S = centred_row_bound(matrix_of(A, stride, i), n, (int64_t)q);
-            by_group[choose_group(S)].push_back({i, q});
+            std::vector<int> order;
+            const int group = plan_rows(centred_row_bounds(matrix_of(A, stride, i), n, (int64_t)q), order);
+            by_group[group].push_back({i, q, order});
         } else {
             wide.push_back({i, q});
         }

@@ -2,6 +2,8 @@
 #pragma once
 #include <stdint.h>
 
+#include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "launchers.h"
@@ -9,41 +11,114 @@
 
 namespace phip {
 
-// Maximum absolute row sum of the centred residues of M modulo q.
-inline uint64_t centred_row_bound(const int64_t* M, int n, int64_t q) {
-    uint64_t S = 0;
-    for (int r = 0; r < n; ++r) {
-        uint64_t row = 0;
+// Row bounds S_i = sum_j |centred a_ij mod q|.
+inline std::vector<uint64_t> centred_row_bounds(const int64_t* M, int n, int64_t q) {
+    std::vector<uint64_t> S(n, 0);
+    for (int r = 0; r < n; ++r)
         for (int c = 0; c < n; ++c) {
             int64_t v = host_floor_mod(M[(size_t)r * n + c], q);
             if (v > q / 2) v -= q;
-            row += (uint64_t)(v < 0 ? -v : v);
+            S[r] += (uint64_t)(v < 0 ? -v : v);
         }
-        if (row > S) S = row;
-    }
     return S;
 }
 
-// Largest group size whose exact products fit the kernel's bounds given the
-// maximum absolute row sum S (S^G < 2^31, S^(G-1) < 2^23); 0 = wide rows.
-inline int choose_group(uint64_t S) {
-    const int b = S ? 64 - __builtin_clzll(S) : 0;
-    if (b <= 7) return 4;
-    if (b <= 10) return 3;
-    if (b <= 15) return 2;
+// Saturating product of row bounds (values above 2^62 collapse to 2^62).
+inline uint64_t bound_product(const uint64_t* b, int c) {
+    unsigned __int128 p = 1;
+    for (int i = 0; i < c; ++i) {
+        p *= b[i];
+        if (p > ((unsigned __int128)1 << 62)) return (uint64_t)1 << 62;
+    }
+    return (uint64_t)p;
+}
+
+// Whether `c` row bounds in this order satisfy the operand limits of the
+// kernel's product of a group of c rows (glynn31.h, PackedRows::run_product):
+// every partial product that feeds a 24-bit multiply below 2^23, the group
+// product below 2^31.
+inline bool group_fits(const uint64_t* b, int c) {
+    const uint64_t k23 = 1ull << 23, k31 = 1ull << 31;
+    if (bound_product(b, c) >= k31) return false;
+    switch (c) {
+        case 1:
+        case 2: return true;
+        case 3: return bound_product(b, 2) < k23;
+        case 4: return bound_product(b, 2) < k23 && bound_product(b + 2, 2) < k23;
+        case 5: return bound_product(b, 3) < k23 && bound_product(b + 3, 2) < k23;
+        case 6: return bound_product(b, 3) < k23 && bound_product(b + 3, 3) < k23;
+        default: return false;
+    }
+}
+
+// Row order and group size for the packed kernels: rows are dealt largest
+// bound first into the group with the smallest bound product so far (all
+// groups hold G rows except the last), then each group is arranged so its
+// partial products fit.  Row order does not change the permanent.  Returns
+// the largest feasible G in {6, 4, 3, 2}, or 0 (wide residue rows).
+inline int plan_rows(const std::vector<uint64_t>& S, std::vector<int>& order) {
+    const int n = (int)S.size();
+    order.resize(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    for (uint64_t s : S)
+        if (s >= (1ull << 15)) return 0;
+    std::vector<int> by_size(order);
+    std::stable_sort(by_size.begin(), by_size.end(), [&](int a, int b) { return S[a] > S[b]; });
+    for (int G : {6, 4, 3, 2}) {
+        const int ng = (n + G - 1) / G;
+        std::vector<std::vector<int>> groups(ng);
+        std::vector<double> logp(ng, 0.0);
+        for (int r : by_size) {
+            int best = -1;
+            for (int g = 0; g < ng; ++g) {
+                const int cap = (g == ng - 1) ? n - (ng - 1) * G : G;
+                if ((int)groups[g].size() >= cap) continue;
+                if (best < 0 || logp[g] < logp[best]) best = g;
+            }
+            groups[best].push_back(r);
+            logp[best] += std::log2((double)S[r] + 1.0);
+        }
+        bool ok = true;
+        std::vector<int> out;
+        for (auto& grp : groups) {
+            std::sort(grp.begin(), grp.end());
+            bool found = false;
+            do {
+                uint64_t b[6];
+                for (size_t i = 0; i < grp.size(); ++i) b[i] = S[grp[i]];
+                if (group_fits(b, (int)grp.size())) {
+                    found = true;
+                    break;
+                }
+            } while (std::next_permutation(grp.begin(), grp.end()));
+            if (!found) {
+                ok = false;
+                break;
+            }
+            out.insert(out.end(), grp.begin(), grp.end());
+        }
+        if (ok) {
+            order = out;
+            return G;
+        }
+    }
     return 0;
 }
 
 // Writes n * glynn_stride(n, group) int32 words: the row sums of state 0
-// (every column sign +1), then the flip deltas 2a_ij of the n-1 Gray columns.
-// group 0: residues in [0, q) one per word.  group > 0: exact centred values
-// as 16-bit two's-complement halves, row 2k in the low half of word k.
-inline void build_glynn_tables(const int64_t* M, int n, uint32_t q, int group, int32_t* out) {
+// (every column sign +1), then the flip deltas 2a_ij of the n-1 Gray columns,
+// with kernel row i taken from matrix row order[i] (order may be empty for
+// the identity).  group 0: residues in [0, q) one per word.  group > 0: exact
+// centred values as 16-bit two's-complement halves, row 2k in the low half of
+// word k.
+inline void build_glynn_tables(const int64_t* M, int n, uint32_t q, int group, const std::vector<int>& order,
+                               int32_t* out) {
     const int W = glynn_stride(n, group);
     std::vector<int64_t> v((size_t)n * n);
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j) {
-            const int64_t r = host_floor_mod(M[(size_t)i * n + j], (int64_t)q);
+            const int src = order.empty() ? i : order[i];
+            const int64_t r = host_floor_mod(M[(size_t)src * n + j], (int64_t)q);
             v[(size_t)i * n + j] = (group == 0 || r <= (int64_t)(q / 2)) ? r : r - (int64_t)q;
         }
     auto put = [&](int col, int row, int64_t value) {

@@ -78,6 +78,8 @@ MICROBENCH_NAMES = (
     "v_mad_i64_i32",
     "v_mad_u64_u32",
     "mont31_product",
+    "v_pk_add_u16",
+    "v_mul_i32_i24_sdwa",
 )
 
 
@@ -293,10 +295,10 @@ class Batch:
         return [int(v) for v in self._out[: self.total]]
 
     def stats(self) -> dict:
-        vals = (ctypes.c_double * 7)()
-        count = self._lib.phip_batch_stats(self._handle, vals, 7)
+        vals = (ctypes.c_double * 8)()
+        count = self._lib.phip_batch_stats(self._handle, vals, 8)
         _check(count, "phip_batch_stats")
-        keys = ("last_ms", "states", "mulmods", "jobs", "items", "launches", "waves")
+        keys = ("last_ms", "states", "mulmods", "jobs", "items", "launches", "waves", "group")
         return {k: vals[i] for i, k in enumerate(keys[:count])}
 
     def close(self) -> None:

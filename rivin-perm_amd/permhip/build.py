@@ -26,8 +26,8 @@ BUILD_DIR = ROOT / "_build"
 LIBRARY = PACKAGE_DIR / "libpermhip.so"
 
 ARCH = "gfx950"
-GLYNN_GROUPS = (0, 2, 3, 4)
-BATCH_GROUPS = (0, 2, 3, 4)
+GLYNN_GROUPS = (0, 2, 3, 4, 6)
+BATCH_GROUPS = (0, 2, 3, 4, 6)
 PLAIN_SOURCES = (
     "device_context.hip",
     "residues.hip",

@@ -37,9 +37,11 @@ def test_all_moduli_small_orders(native, oracle_c, n):
 @pytest.mark.parametrize(
     "n,lo,hi",
     [
-        (16, -1, 1),      # row sums < 2^7: groups of 4
-        (17, -10, 10),    # groups of 3
-        (18, -100, 100),  # groups of 2
+        (16, -1, 1),      # small row bounds: groups of 6
+        (17, -10, 10),    # groups of 4
+        (18, -100, 100),  # groups of 3
+        (14, -1000, 1000),  # groups of 2
+        (23, -3, 3),      # groups of 6, odd order (last group partial)
         (19, -(10 ** 6), 10 ** 6),  # wide rows
         (20, -10, 10),
         (21, 0, 1),
