@@ -58,31 +58,44 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def instruction_mix(n: int, group: int) -> dict:
-    """VALU instructions per Gray state of glynn_chunk<n, group> (the
-    unrolled 16-state step, amortised): row-sum flips, exact group products,
-    the Montgomery tree (two v_mad_i64_i32 and one v_mul_lo_u32 per product)
-    and the signed accumulation."""
+def instruction_mix(n: int, group: int, k: int = 1) -> dict:
+    """VALU instructions per Gray state of one walk of gray_walk<n, group>
+    serving a set of k primes (the unrolled 16-state step, amortised).
+
+    Shared by the set: row-sum flips (one packed op per two rows) and the
+    exact group products; for k > 1 also the exact 64-bit products of
+    adjacent groups.  Per prime: the Montgomery tree (two v_mad_i64_i32 and
+    one v_mul_lo_u32 per product; one of each per reduced pair product) and
+    the signed 64-bit accumulation (v_ashrrev_i32 + v_lshl_add_u64)."""
     groups = n if group == 0 else -(-n // group)
-    mix = {}
+    mix = {"v_mad_i64_i32": 0, "v_mul_lo_u32": 0}
     if group == 0:
-        mix["v_add_u32"] = 3 * n + 3  # add/sub, wrap, min per row; accumulate
+        mix["v_add_u32"] = 3 * n  # add/sub, wrap, min per row
     else:
         mix["v_pk_add_u16"] = -(-n // 2)
         mix["v_mul_i32_i24_sdwa"] = n - groups
-        mix["v_add_u32"] = 3 + (1 if n % group == 1 else 0)
-    mix["v_mad_i64_i32"] = 2 * (groups - 1)
-    mix["v_mul_lo_u32"] = groups - 1
+        mix["v_add_u32"] = 1 if n % group == 1 else 0
+    if k == 1:
+        mix["v_mad_i64_i32"] += 2 * (groups - 1)
+        mix["v_mul_lo_u32"] += groups - 1
+    else:
+        pairs = groups // 2
+        leaves = pairs + groups % 2
+        mix["v_mad_i64_i32"] += pairs + k * (pairs + 2 * (leaves - 1))
+        mix["v_mul_lo_u32"] += k * (pairs + leaves - 1)
+    mix["v_ashrrev_i32"] = k
+    mix["v_lshl_add_u64"] = k
     return mix
 
 
-def roofline(n: int, group: int, rates: dict, mulmods_per_s: float) -> dict:
+def roofline(n: int, group: int, k: int, rates: dict, mulmods_per_s: float) -> dict:
     """Mulmod roofline of the kernel's own instruction mix at the measured
     instruction rates (the kernel is VALU-issue bound; HBM traffic is a few
-    MB per launch)."""
-    mix = instruction_mix(n, group)
+    MB per launch).  One walk of a k-prime set does k n algorithmic
+    mulmods per state."""
+    mix = instruction_mix(n, group, k)
     seconds_per_state_lane = sum(count / rates[name] for name, count in mix.items())
-    peak = n / seconds_per_state_lane  # mulmods per second, whole device
+    peak = k * n / seconds_per_state_lane  # mulmods per second, whole device
     return {
         "bound": "valu",
         "achieved": mulmods_per_s / 1e12,
@@ -91,6 +104,7 @@ def roofline(n: int, group: int, rates: dict, mulmods_per_s: float) -> dict:
         "frac": mulmods_per_s / peak,
         "traffic": traffic_from_profiles(),
         "group": group,
+        "prime_set": k,
         "instructions_per_state": mix,
         # Every mulmod as one signed Montgomery product plus one row-sum add.
         "general_montgomery_peak": 1.0 / (1.0 / rates["mont31_product"] + 1.0 / rates["v_add_u32"]) / 1e12,
@@ -224,9 +238,13 @@ def main() -> int:
 
     rates = _native.valu_microbench()
     log("VALU lane-ops/s: " + ", ".join(f"{k}={v:.3e}" for k, v in rates.items()))
-    group = int(plan.stats()["group"])
-    roof = roofline(HEADLINE_N, group, rates, kernel_rate)
-    roof["kernel"] = f"batch_glynn_kernel<{group}> (n={HEADLINE_N}, {len(primes)} primes per launch)"
+    st = plan.stats()
+    group, kset = int(st["group"]), int(st["set"])
+    roof = roofline(HEADLINE_N, group, kset, rates, kernel_rate)
+    roof["kernel"] = (
+        f"batch_glynn_kernel<{group}, {kset}> (n={HEADLINE_N}, {len(primes)} primes in "
+        f"{int(st['jobs'])} walk(s) of the Gray code)"
+    )
     roof["kernel_ms"] = k_ms
 
     # Plain launch: one synchronous C-ABI call per (matrix, prime) -- host

@@ -113,10 +113,11 @@ int phip_batch_query(void *plan, int64_t *residues_out, int block);
 int phip_batch_set_option(void *plan, int option, int64_t value);
 
 /* out[0] last run ms (persistent kernels only without a graph; whole graph
- * otherwise), [1] Gray states, [2] algorithmic mulmods (n 2^(n-1) per job),
- * [3] jobs, [4] work items, [5] persistent launches, [6] persistent
- * wavefronts, [7] row-sum group size of the first launch.  Returns the
- * number of values written or -1. */
+ * otherwise), [1] Gray states and [2] algorithmic mulmods (2^(n-1) and
+ * n 2^(n-1) per residue), [3] jobs (one table with a set of primes that
+ * share it), [4] work items, [5] persistent launches, [6] persistent
+ * wavefronts, [7] row-sum group size and [9] prime-set size of the first
+ * launch, [8] residues.  Returns the number of values written or -1. */
 int phip_batch_stats(void *plan, double *out, int len);
 
 int phip_batch_destroy(void *plan);
@@ -140,8 +141,9 @@ int phip_profile_read(double *kernel_ms, uint64_t *launches);
 
 /* Whole-device throughput (lane operations per second) of v_add_u32,
  * v_mul_i32_i24, v_mul_lo_u32, v_mul_hi_i32, v_mad_i64_i32, v_mad_u64_u32,
- * one signed Montgomery product, v_pk_add_u16 and v_mul_i32_i24 with SDWA
- * word selects, in that order.  Returns the count. */
+ * one signed Montgomery product, v_pk_add_u16, v_mul_i32_i24 with SDWA
+ * word selects, v_lshl_add_u64 and v_ashrrev_i32, in that order.  Returns
+ * the count. */
 int phip_valu_microbench(double *lane_ops_per_s, int len);
 
 /* `reps` launches of a plain copy kernel over `bytes` bytes. */

@@ -1,40 +1,130 @@
-// Shared declarations of the persistent batch (see batch.hip, batch_plan.hip).
+// Shared declarations of the persistent Glynn engine (batch.hip,
+// batch_plan.hip).  Every 31-bit residue of the library runs through it: a
+// batch plan keeps its tables resident on the device, the single-matrix
+// entry points build a one-matrix plan in the device context's buffers.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace phip {
 
-// One (matrix, modulus) job of a batch plan; 48 bytes, read with scalar loads.
-struct BatchJob {
-    uint64_t data_off;  // int32 offset of the job's glynn_chunk tables (n * glynn_stride words)
-    uint64_t lanes;     // lanes of this job (2^(n-1-cbits))
-    int32_t n;
-    int32_t cbits;
-    int32_t p;
-    uint32_t pinv;
-    uint32_t first_item;
-    uint32_t n_items;
-    uint32_t scale;  // R^(groups-1) * 2^-(n-1) mod p
-    int32_t group;
-};
+class DeviceContext;
 
 constexpr int kBatchMinOrder = 5;
 constexpr int kBatchMaxOrder = 40;
+// Most primes one job evaluates together, and the stride of the per-item
+// partial sums.
+constexpr int kMaxSet = 8;
+
+// One job: one table (a matrix under one row order) and a set of K odd
+// moduli below 2^31 that share it (K = 1 for residue rows).  48 bytes,
+// read with scalar loads.
+struct BatchJob {
+    uint64_t data_off;   // int32 offset of the job's tables (n * glynn_stride words)
+    uint64_t lanes;      // lanes of this job (2^(n-1-cbits))
+    int32_t n;
+    int32_t cbits;
+    uint32_t prime_off;  // first of the job's K entries in the prime arrays
+    uint32_t first_item;
+    uint32_t n_items;
+    int32_t group;
+    int32_t nprimes;     // real primes (<= K); padded entries repeat the last one
+    int32_t set;         // K of the job's launch class
+};
+
+// One requested residue: prime slot `k` of job `job`.
+struct BatchSlot {
+    uint32_t job;
+    uint32_t k;
+    uint32_t p;
+    uint32_t scale;  // R^(groups-1) * 2^-(n-1) mod p
+};
+
+// One persistent launch: the jobs of one (row-sum group, set size) pair.
+struct BatchClass {
+    int group;
+    int set;
+    uint32_t first_item;
+    uint32_t n_items;
+};
+
+// One requested residue before planning: a row-major n x n matrix and an odd
+// modulus in [3, 2^31).  Inputs with equal `key` may share a job when their
+// tables are identical.
+struct PlanInput {
+    const int64_t* a;
+    int n;
+    uint32_t q;
+    int key;
+};
+
+// Host image of a plan.
+struct PlanHost {
+    std::vector<BatchJob> jobs;
+    std::vector<uint2> items;  // (job, wavefront index), grouped by class
+    std::vector<int32_t> blob;
+    std::vector<int32_t> set_p;
+    std::vector<uint32_t> set_pinv;
+    std::vector<BatchSlot> slots;  // input order
+    std::vector<BatchClass> classes;
+    double total_states = 0;   // Gray states over all residues
+    double total_mulmods = 0;  // n 2^(n-1) per residue
+};
+
+// Device image of a plan (pointers into device memory).
+struct PlanDevice {
+    BatchJob* jobs;
+    uint2* items;
+    int32_t* blob;
+    int32_t* set_p;
+    uint32_t* set_pinv;
+    BatchSlot* slots;
+    uint64_t* partials;  // kMaxSet per item
+    uint32_t* counters;  // one per class
+    int64_t* residues;   // one per slot
+};
+
+// Validates the inputs and builds the plan: tables per distinct (matrix,
+// row order, representation), prime sets, cost-sorted items of one
+// wavefront each, about 2^items_log2 items in all.  Returns 0 or -1.
+int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cbits, PlanHost& out);
+
+// Enqueues one run: counter reset, one persistent launch per class (between
+// ev0 and ev1 when given), finalisation into d.residues.  Returns 0 or -1.
+int enqueue_plan(const PlanHost& h, const PlanDevice& d, hipStream_t stream, unsigned grid, hipEvent_t ev0,
+                 hipEvent_t ev1);
+
+// Builds and runs a plan for `in` in the device context's stream and
+// buffers (the single-matrix entry points); out[i] receives the residue of
+// in[i].  Chunk sizes follow the context tuning.  Returns 0 or -1.
+int run_plan_in_context(DeviceContext* ctx, const std::vector<PlanInput>& in, int64_t* out);
 
 using BatchLaunchFn = hipError_t (*)(dim3 grid, hipStream_t stream, const BatchJob* jobs, const uint2* items,
                                      uint32_t n_items, uint32_t* counter, const int32_t* blob,
-                                     uint64_t* item_partials);
+                                     const int32_t* set_p, const uint32_t* set_pinv, uint64_t* item_partials);
 
-hipError_t launch_batch_g0(dim3, hipStream_t, const BatchJob*, const uint2*, uint32_t, uint32_t*, const int32_t*,
-                           uint64_t*);
-hipError_t launch_batch_g2(dim3, hipStream_t, const BatchJob*, const uint2*, uint32_t, uint32_t*, const int32_t*,
-                           uint64_t*);
-hipError_t launch_batch_g3(dim3, hipStream_t, const BatchJob*, const uint2*, uint32_t, uint32_t*, const int32_t*,
-                           uint64_t*);
-hipError_t launch_batch_g4(dim3, hipStream_t, const BatchJob*, const uint2*, uint32_t, uint32_t*, const int32_t*,
-                           uint64_t*);
-hipError_t launch_batch_g6(dim3, hipStream_t, const BatchJob*, const uint2*, uint32_t, uint32_t*, const int32_t*,
-                           uint64_t*);
+#define PHIP_DECLARE_BATCH_LAUNCH(G, K)                                                                    \
+    hipError_t launch_batch_g##G##_k##K(dim3, hipStream_t, const BatchJob*, const uint2*, uint32_t, uint32_t*, \
+                                        const int32_t*, const int32_t*, const uint32_t*, uint64_t*);
+PHIP_DECLARE_BATCH_LAUNCH(0, 1)
+PHIP_DECLARE_BATCH_LAUNCH(2, 1)
+PHIP_DECLARE_BATCH_LAUNCH(2, 2)
+PHIP_DECLARE_BATCH_LAUNCH(2, 4)
+PHIP_DECLARE_BATCH_LAUNCH(2, 8)
+PHIP_DECLARE_BATCH_LAUNCH(3, 1)
+PHIP_DECLARE_BATCH_LAUNCH(3, 2)
+PHIP_DECLARE_BATCH_LAUNCH(3, 4)
+PHIP_DECLARE_BATCH_LAUNCH(3, 8)
+PHIP_DECLARE_BATCH_LAUNCH(4, 1)
+PHIP_DECLARE_BATCH_LAUNCH(4, 2)
+PHIP_DECLARE_BATCH_LAUNCH(4, 4)
+PHIP_DECLARE_BATCH_LAUNCH(4, 8)
+PHIP_DECLARE_BATCH_LAUNCH(6, 1)
+PHIP_DECLARE_BATCH_LAUNCH(6, 2)
+PHIP_DECLARE_BATCH_LAUNCH(6, 4)
+PHIP_DECLARE_BATCH_LAUNCH(6, 8)
+#undef PHIP_DECLARE_BATCH_LAUNCH
 
 }  // namespace phip

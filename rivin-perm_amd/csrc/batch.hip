@@ -1,22 +1,19 @@
-// Persistent, work-stealing batch of exact-permanent residues (gfx950).
+// Persistent, work-stealing Glynn kernel for 31-bit odd moduli (gfx950).
 //
-// A plan holds many matrices of uneven order, each with its own list of odd
-// moduli below 2^31.  Every (matrix, modulus) pair is a job; every job is cut
-// into items of one wavefront each (64 lanes x 2^cbits Gray states, the same
-// lane chunk as glynn31_kernel).  Items are sorted by cost, largest first,
-// and grouped by row-sum representation (group size G); each group runs as
-// one persistent launch whose wavefronts take items from a global counter
-// until the list is exhausted.  Each item's wavefront sum goes to its own
-// slot, and a finalisation kernel folds each job's contiguous slots into a
-// residue, so no atomics touch the arithmetic.  The whole sequence (counter
-// reset, persistent launches, finalisation, residue copy) is captured once
-// into a hipGraph and replayed.
+// A plan holds jobs of uneven order; each job is one table with a set of K
+// moduli that share it.  Every job is cut into items of one wavefront each
+// (64 lanes x 2^cbits Gray states, see gray_walk).  Items are sorted by cost,
+// largest first, and grouped by row-sum representation G and set size K;
+// each (G, K) class runs as one persistent launch whose wavefronts take
+// items from a global counter until the list is exhausted.  An item's
+// wavefront sums (one per prime of the set) go to its own slots, and a
+// finalisation kernel folds each job's contiguous slots into residues, so
+// no atomics touch the arithmetic.
 //
-// The matrices and tables stay resident in device memory for the lifetime
-// of the plan, so a replay moves only the residues back to the host.
+// This file is compiled once per (G, K) class: PHIP_BATCH_GROUP and
+// PHIP_BATCH_SET select it.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
-#include <string.h>
 
 #include "batch.h"
 #include "glynn31.h"
@@ -24,19 +21,33 @@
 #ifndef PHIP_BATCH_GROUP
 #error "PHIP_BATCH_GROUP must be defined"
 #endif
+#ifndef PHIP_BATCH_SET
+#error "PHIP_BATCH_SET must be defined"
+#endif
 
 namespace phip {
 
-#define PHIP_CAT2(a, b) a##b
-#define PHIP_CAT(a, b) PHIP_CAT2(a, b)
-
 namespace {
 
-template <int G>
-__global__ __launch_bounds__(256) void batch_glynn_kernel(const BatchJob* __restrict__ jobs,
+// One wavefront's chunk of job `jb` for every prime of its set; returns the
+// lane residues in r.
+template <int N, int G, int K>
+__device__ __forceinline__ void run_item(const int32_t* __restrict__ tables, const BatchJob& jb, uint64_t base,
+                                         const int32_t (&p)[K], const uint32_t (&pinv)[K], uint64_t (&r)[K]) {
+    if constexpr (K == 1) {
+        r[0] = glynn_chunk<N, G>(tables, N, jb.cbits, base, p[0], pinv[0]);
+    } else {
+        glynn_chunk_set<N, G, K>(tables, N, jb.cbits, base, p, pinv, r);
+    }
+}
+
+template <int G, int K>
+__global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __restrict__ jobs,
                                                          const uint2* __restrict__ items, uint32_t n_items,
                                                          uint32_t* __restrict__ counter,
                                                          const int32_t* __restrict__ blob,
+                                                         const int32_t* __restrict__ set_p,
+                                                         const uint32_t* __restrict__ set_pinv,
                                                          uint64_t* __restrict__ item_partials) {
     const int lane_id = threadIdx.x & 63;
     const uint32_t claim = lane_id == 0 ? 1u : 0u;
@@ -54,11 +65,20 @@ __global__ __launch_bounds__(256) void batch_glynn_kernel(const BatchJob* __rest
         const bool active = lane < jb.lanes;
         const uint64_t base = (active ? lane : 0) << jb.cbits;
         const int32_t* __restrict__ tables = blob + jb.data_off;
-        uint64_t r = 0;
+        int32_t p[K];
+        uint32_t pinv[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            p[k] = __builtin_amdgcn_readfirstlane(set_p[jb.prime_off + k]);
+            pinv[k] = (uint32_t)__builtin_amdgcn_readfirstlane((int)set_pinv[jb.prime_off + k]);
+        }
+        uint64_t r[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) r[k] = 0;
         switch (jb.n) {
 #define PHIP_BATCH_CASE(N) \
     case N:                \
-        r = glynn_chunk<N, G>(tables, N, jb.cbits, base, jb.p, jb.pinv); \
+        run_item<N, G, K>(tables, jb, base, p, pinv, r); \
         break;
             PHIP_BATCH_CASE(5) PHIP_BATCH_CASE(6) PHIP_BATCH_CASE(7) PHIP_BATCH_CASE(8)
             PHIP_BATCH_CASE(9) PHIP_BATCH_CASE(10) PHIP_BATCH_CASE(11) PHIP_BATCH_CASE(12)
@@ -73,18 +93,26 @@ __global__ __launch_bounds__(256) void batch_glynn_kernel(const BatchJob* __rest
             default:
                 break;
         }
-        const uint64_t s = wave_reduce_sum_u64(active ? r : 0u);
-        if (lane_id == 0) item_partials[idx] = s;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint64_t s = wave_reduce_sum_u64(active ? r[k] : 0u);
+            if (lane_id == 0) item_partials[(size_t)idx * kMaxSet + k] = s;
+        }
     }
 }
 
 }  // namespace
 
-hipError_t PHIP_CAT(launch_batch_g, PHIP_BATCH_GROUP)(dim3 grid, hipStream_t stream, const BatchJob* jobs,
-                                                      const uint2* items, uint32_t n_items, uint32_t* counter,
-                                                      const int32_t* blob, uint64_t* item_partials) {
-    hipLaunchKernelGGL(batch_glynn_kernel<PHIP_BATCH_GROUP>, grid, dim3(256), 0, stream, jobs, items, n_items,
-                       counter, blob, item_partials);
+#define PHIP_CAT4(a, b, c, d) a##b##c##d
+#define PHIP_LAUNCH_NAME(G, K) PHIP_CAT4(launch_batch_g, G, _k, K)
+
+hipError_t PHIP_LAUNCH_NAME(PHIP_BATCH_GROUP, PHIP_BATCH_SET)(dim3 grid, hipStream_t stream, const BatchJob* jobs,
+                                                              const uint2* items, uint32_t n_items,
+                                                              uint32_t* counter, const int32_t* blob,
+                                                              const int32_t* set_p, const uint32_t* set_pinv,
+                                                              uint64_t* item_partials) {
+    hipLaunchKernelGGL((batch_glynn_kernel<PHIP_BATCH_GROUP, PHIP_BATCH_SET>), grid, dim3(256), 0, stream, jobs,
+                       items, n_items, counter, blob, set_p, set_pinv, item_partials);
     return hipGetLastError();
 }
 

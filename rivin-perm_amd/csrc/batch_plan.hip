@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -28,34 +29,57 @@ __global__ void batch_reset_kernel(uint32_t* counters, int n) {
     if (i < n) counters[i] = 0;
 }
 
-// One workgroup per job: fold the job's contiguous item sums, reduce, scale.
+// One workgroup per requested residue: fold the item sums of the slot's job
+// for its prime, reduce, scale.
 __global__ __launch_bounds__(256) void batch_finalize_kernel(const BatchJob* __restrict__ jobs,
+                                                             const BatchSlot* __restrict__ slots,
                                                              const uint64_t* __restrict__ item_partials,
                                                              int64_t* __restrict__ residues) {
-    const BatchJob jb = jobs[blockIdx.x];
-    const uint64_t p = (uint64_t)jb.p;
+    const BatchSlot sl = slots[blockIdx.x];
+    const BatchJob jb = jobs[sl.job];
+    const uint64_t p = sl.p;
     uint64_t s = 0;
-    for (uint32_t i = threadIdx.x; i < jb.n_items; i += 256) s += item_partials[jb.first_item + i] % p;
+    for (uint32_t i = threadIdx.x; i < jb.n_items; i += 256)
+        s += item_partials[(size_t)(jb.first_item + i) * kMaxSet + sl.k] % p;
     s = block_reduce_sum_u64<256>(s);
-    if (threadIdx.x == 0) residues[blockIdx.x] = (int64_t)(((s % p) * (uint64_t)jb.scale) % p);
+    if (threadIdx.x == 0) residues[blockIdx.x] = (int64_t)(((s % p) * (uint64_t)sl.scale) % p);
 }
 
-BatchLaunchFn launcher_for(int group) {
-    switch (group) {
-        case 0: return launch_batch_g0;
-        case 2: return launch_batch_g2;
-        case 3: return launch_batch_g3;
-        case 4: return launch_batch_g4;
-        case 6: return launch_batch_g6;
-        default: return nullptr;
+BatchLaunchFn launcher_for(int group, int set) {
+#define PHIP_LAUNCHER(G, K) \
+    if (group == G && set == K) return launch_batch_g##G##_k##K;
+    PHIP_LAUNCHER(0, 1)
+    PHIP_LAUNCHER(2, 1) PHIP_LAUNCHER(2, 2) PHIP_LAUNCHER(2, 4) PHIP_LAUNCHER(2, 8)
+    PHIP_LAUNCHER(3, 1) PHIP_LAUNCHER(3, 2) PHIP_LAUNCHER(3, 4) PHIP_LAUNCHER(3, 8)
+    PHIP_LAUNCHER(4, 1) PHIP_LAUNCHER(4, 2) PHIP_LAUNCHER(4, 4) PHIP_LAUNCHER(4, 8)
+    PHIP_LAUNCHER(6, 1) PHIP_LAUNCHER(6, 2) PHIP_LAUNCHER(6, 4) PHIP_LAUNCHER(6, 8)
+#undef PHIP_LAUNCHER
+    return nullptr;
+}
+
+// Set sizes a table's r moduli are split into.  A set of K primes costs one
+// shared walk (flips, exact group products) plus K per-prime Montgomery
+// trees; the shared part is worth about two trees, so five primes run
+// cheaper as 4 + 1 than as 8 with three padded slots, and six or seven as
+// one padded set of 8.
+std::vector<int> split_set(int r) {
+    std::vector<int> out;
+    while (r > 0) {
+        int k;
+        if (r >= 6) {
+            k = 8;
+        } else if (r == 5 || r == 4 || r == 3) {
+            k = 4;
+        } else {
+            k = r;  // 2 or 1
+        }
+        out.push_back(k);
+        r -= std::min(r, k);
     }
+    return out;
 }
 
-struct Class {
-    int group;
-    uint32_t first_item;
-    uint32_t n_items;
-};
+int groups_of(int n, int group) { return group == 0 ? n : (n + group - 1) / group; }
 
 template <typename T>
 struct DevBuf {
@@ -76,17 +100,173 @@ struct DevBuf {
 
 }  // namespace
 
+int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cbits, PlanHost& out) {
+    out = PlanHost();
+    // Pass 1: validate; one table per distinct (key, order, representation,
+    // table contents); the inputs that share it form a set.
+    struct Set {
+        int key;
+        int n;
+        int group;
+        uint64_t data_off;
+        std::vector<size_t> inputs;
+    };
+    std::vector<Set> sets;
+    std::map<int, std::vector<size_t>> sets_of_key;
+    std::vector<int32_t> table;
+    for (size_t i = 0; i < in.size(); ++i) {
+        const PlanInput& x = in[i];
+        if (x.n < kBatchMinOrder || x.n > kBatchMaxOrder) {
+            set_error("batch orders must lie in [5, 40]; use the single-matrix entry point otherwise");
+            return -1;
+        }
+        if (x.q < 3 || x.q >= (1u << 31) || !(x.q & 1)) {
+            set_error("batch moduli must be odd and lie in [3, 2^31)");
+            return -1;
+        }
+        std::vector<int> order;
+        const int group = plan_rows(centred_row_bounds(x.a, x.n, (int64_t)x.q), order);
+        const size_t words = (size_t)x.n * glynn_stride(x.n, group);
+        table.assign(words, 0);
+        build_glynn_tables(x.a, x.n, x.q, group, order, table.data());
+        size_t found = sets.size();
+        if (group > 0) {
+            for (size_t si : sets_of_key[x.key]) {
+                const Set& st = sets[si];
+                if (st.n == x.n && st.group == group &&
+                    memcmp(out.blob.data() + st.data_off, table.data(), words * sizeof(int32_t)) == 0) {
+                    found = si;
+                    break;
+                }
+            }
+        }
+        if (found == sets.size()) {
+            sets.push_back({x.key, x.n, group, (uint64_t)out.blob.size(), {}});
+            sets_of_key[x.key].push_back(found);
+            out.blob.insert(out.blob.end(), table.begin(), table.end());
+        }
+        sets[found].inputs.push_back(i);
+    }
+
+    // Pass 2: jobs (one per set chunk) and slots.
+    out.slots.resize(in.size());
+    double job_states = 0;
+    for (const Set& st : sets) {
+        size_t next = 0;
+        for (int K : st.group == 0 ? std::vector<int>(st.inputs.size(), 1) : split_set((int)st.inputs.size())) {
+            const int take = (int)std::min<size_t>((size_t)K, st.inputs.size() - next);
+            BatchJob jb{};
+            jb.data_off = st.data_off;
+            jb.n = st.n;
+            jb.group = st.group;
+            jb.set = K;
+            jb.nprimes = take;
+            jb.prime_off = (uint32_t)out.set_p.size();
+            const uint32_t job = (uint32_t)out.jobs.size();
+            const int L = st.n - 1;
+            const int groups = groups_of(st.n, st.group);
+            for (int k = 0; k < K; ++k) {
+                const size_t idx = st.inputs[next + std::min(k, take - 1)];
+                const uint32_t q = in[idx].q;
+                out.set_p.push_back((int32_t)q);
+                out.set_pinv.push_back(inverse_mod_2_32(q));
+                if (k < take) {
+                    BatchSlot sl;
+                    sl.job = job;
+                    sl.k = (uint32_t)k;
+                    sl.p = q;
+                    sl.scale = (uint32_t)host_mulmod(host_powmod((1ull << 32) % q, (uint64_t)(groups - 1), q),
+                                                     host_powmod((q + 1) / 2, (uint64_t)L, q), q);
+                    out.slots[idx] = sl;
+                    out.total_states += std::ldexp(1.0, L);
+                    out.total_mulmods += (double)st.n * std::ldexp(1.0, L);
+                }
+            }
+            next += take;
+            job_states += std::ldexp(1.0, L);
+            out.jobs.push_back(jb);
+        }
+    }
+
+    // Chunk bits: spread the job states over about 2^items_log2 items.
+    const double target = std::ldexp(1.0, items_log2);
+    int s = (int)std::lround(std::log2(std::max(1.0, job_states / target)));
+    s = std::min(std::max(s, 10), 30);
+    for (BatchJob& jb : out.jobs) {
+        const int L = jb.n - 1;
+        int c = (L >= 10) ? std::max(4, std::min(s - 6, L - 6)) : 4;
+        c = std::min(std::max(c, min_cbits), L);
+        jb.cbits = c;
+        jb.lanes = 1ull << (L - c);
+    }
+
+    // Items: per (group, set) class, jobs by descending cost, each job's
+    // items contiguous.
+    static const int kClasses[][2] = {{0, 1}, {2, 8}, {2, 4}, {2, 2}, {2, 1}, {3, 8}, {3, 4}, {3, 2}, {3, 1},
+                                      {4, 8}, {4, 4}, {4, 2}, {4, 1}, {6, 8}, {6, 4}, {6, 2}, {6, 1}};
+    for (const auto& gk : kClasses) {
+        std::vector<uint32_t> order;
+        for (uint32_t j = 0; j < out.jobs.size(); ++j)
+            if (out.jobs[j].group == gk[0] && out.jobs[j].set == gk[1]) order.push_back(j);
+        if (order.empty()) continue;
+        auto cost = [&](uint32_t j) {
+            const BatchJob& b = out.jobs[j];
+            return std::ldexp(1.0, b.cbits) * (double)std::min<uint64_t>(b.lanes, 64) *
+                   (b.n + 3.0 * groups_of(b.n, b.group));
+        };
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost(a) > cost(b); });
+        BatchClass cl{gk[0], gk[1], (uint32_t)out.items.size(), 0};
+        for (uint32_t j : order) {
+            BatchJob& b = out.jobs[j];
+            const uint64_t waves = (b.lanes + 63) / 64;
+            if (out.items.size() + waves >= 0xffffffffull) {
+                set_error("batch has too many work items");
+                return -1;
+            }
+            b.first_item = (uint32_t)out.items.size();
+            b.n_items = (uint32_t)waves;
+            for (uint64_t w = 0; w < waves; ++w) out.items.push_back(make_uint2(j, (uint32_t)w));
+        }
+        cl.n_items = (uint32_t)out.items.size() - cl.first_item;
+        out.classes.push_back(cl);
+    }
+    return 0;
+}
+
+int enqueue_plan(const PlanHost& h, const PlanDevice& d, hipStream_t stream, unsigned grid, hipEvent_t ev0,
+                 hipEvent_t ev1) {
+    if (h.slots.empty()) return 0;
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, stream, d.counters, (int)h.classes.size());
+    PHIP_HIP_TRY(hipGetLastError());
+    if (ev0) PHIP_HIP_TRY(hipEventRecord(ev0, stream));
+    for (size_t c = 0; c < h.classes.size(); ++c) {
+        const BatchClass& cl = h.classes[c];
+        const BatchLaunchFn fn = launcher_for(cl.group, cl.set);
+        if (!fn) {
+            set_error("internal: no kernel for a batch class");
+            return -1;
+        }
+        PHIP_HIP_TRY(fn(dim3(grid), stream, d.jobs, d.items + cl.first_item, cl.n_items, d.counters + c, d.blob,
+                        d.set_p, d.set_pinv, d.partials + (size_t)cl.first_item * kMaxSet));
+    }
+    if (ev1) PHIP_HIP_TRY(hipEventRecord(ev1, stream));
+    hipLaunchKernelGGL(batch_finalize_kernel, dim3((unsigned)h.slots.size()), dim3(256), 0, stream, d.jobs,
+                       d.slots, d.partials, d.residues);
+    PHIP_HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 struct BatchPlan {
     int device = 0;
     hipStream_t stream = nullptr;
     std::mutex mu;
-    std::vector<BatchJob> jobs;  // in input (matrix, modulus) order
-    std::vector<Class> classes;
-    uint32_t n_items = 0;
-    double total_states = 0, total_mulmods = 0;
+    PlanHost host;
     DevBuf<BatchJob> d_jobs;
     DevBuf<uint2> d_items;
     DevBuf<int32_t> d_blob;
+    DevBuf<int32_t> d_set_p;
+    DevBuf<uint32_t> d_set_pinv;
+    DevBuf<BatchSlot> d_slots;
     DevBuf<uint64_t> d_partials;
     DevBuf<uint32_t> d_counters;
     DevBuf<int64_t> d_res;
@@ -98,6 +278,7 @@ struct BatchPlan {
     hipGraph_t graph = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0;
+    bool launched = false;
 
     ~BatchPlan() {
         if (stream) hipStreamSynchronize(stream);
@@ -109,25 +290,38 @@ struct BatchPlan {
         if (stream) hipStreamDestroy(stream);
     }
 
-    int enqueue(bool timed) {
-        hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, stream, d_counters.ptr, (int)classes.size());
-        PHIP_HIP_TRY(hipGetLastError());
-        if (timed) PHIP_HIP_TRY(hipEventRecord(ev0, stream));
-        const dim3 grid((unsigned)(cus * blocks_per_cu));
-        for (size_t c = 0; c < classes.size(); ++c) {
-            const Class& cl = classes[c];
-            PHIP_HIP_TRY(launcher_for(cl.group)(grid, stream, d_jobs.ptr, d_items.ptr + cl.first_item, cl.n_items,
-                                                d_counters.ptr + c, d_blob.ptr, d_partials.ptr + cl.first_item));
-        }
-        if (timed) PHIP_HIP_TRY(hipEventRecord(ev1, stream));
-        hipLaunchKernelGGL(batch_finalize_kernel, dim3((unsigned)jobs.size()), dim3(256), 0, stream, d_jobs.ptr,
-                           d_partials.ptr, d_res.ptr);
-        PHIP_HIP_TRY(hipGetLastError());
-        PHIP_HIP_TRY(hipMemcpyAsync(h_res, d_res.ptr, jobs.size() * sizeof(int64_t), hipMemcpyDeviceToHost, stream));
+    PlanDevice device_image() const {
+        return PlanDevice{d_jobs.ptr,  d_items.ptr,    d_blob.ptr,     d_set_p.ptr, d_set_pinv.ptr,
+                          d_slots.ptr, d_partials.ptr, d_counters.ptr, d_res.ptr};
+    }
+
+    int upload() {
+        const PlanHost& h = host;
+        if (d_jobs.alloc(h.jobs.size()) || d_items.alloc(h.items.size()) || d_blob.alloc(h.blob.size()) ||
+            d_set_p.alloc(h.set_p.size()) || d_set_pinv.alloc(h.set_pinv.size()) || d_slots.alloc(h.slots.size()) ||
+            d_partials.alloc(h.items.size() * kMaxSet) || d_counters.alloc(32) || d_res.alloc(h.slots.size()))
+            return -1;
+        PHIP_HIP_TRY(hipHostMalloc((void**)&h_res, std::max<size_t>(h.slots.size(), 1) * sizeof(int64_t),
+                                   hipHostMallocDefault));
+        PHIP_HIP_TRY(hipMemcpy(d_jobs.ptr, h.jobs.data(), h.jobs.size() * sizeof(BatchJob), hipMemcpyHostToDevice));
+        PHIP_HIP_TRY(hipMemcpy(d_items.ptr, h.items.data(), h.items.size() * sizeof(uint2), hipMemcpyHostToDevice));
+        PHIP_HIP_TRY(hipMemcpy(d_blob.ptr, h.blob.data(), h.blob.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        PHIP_HIP_TRY(hipMemcpy(d_set_p.ptr, h.set_p.data(), h.set_p.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        PHIP_HIP_TRY(hipMemcpy(d_set_pinv.ptr, h.set_pinv.data(), h.set_pinv.size() * sizeof(uint32_t),
+                               hipMemcpyHostToDevice));
+        PHIP_HIP_TRY(
+            hipMemcpy(d_slots.ptr, h.slots.data(), h.slots.size() * sizeof(BatchSlot), hipMemcpyHostToDevice));
         return 0;
     }
 
-    bool launched = false;
+    int enqueue(bool timed) {
+        if (enqueue_plan(host, device_image(), stream, (unsigned)(cus * blocks_per_cu), timed ? ev0 : nullptr,
+                         timed ? ev1 : nullptr) != 0)
+            return -1;
+        PHIP_HIP_TRY(hipMemcpyAsync(h_res, d_res.ptr, host.slots.size() * sizeof(int64_t), hipMemcpyDeviceToHost,
+                                    stream));
+        return 0;
+    }
 
     // Enqueue one run on the plan's stream without waiting.
     int launch_locked() {
@@ -169,7 +363,7 @@ struct BatchPlan {
         }
         launched = false;
         PHIP_HIP_TRY(hipEventElapsedTime(&last_ms, ev0, ev1));
-        if (out) memcpy(out, h_res, jobs.size() * sizeof(int64_t));
+        if (out) memcpy(out, h_res, host.slots.size() * sizeof(int64_t));
         return 1;
     }
 
@@ -181,6 +375,53 @@ struct BatchPlan {
     }
 };
 
+int run_plan_in_context(DeviceContext* ctx, const std::vector<PlanInput>& in, int64_t* out) {
+    PlanHost h;
+    const Tuning& t = tuning();
+    if (build_plan_host(in, std::max(0, t.lanes_log2 - 6), t.min_cbits, h) != 0) return -1;
+    if (h.slots.empty()) return 0;
+    // Device buffers of the context, slots 16..24.
+    PlanDevice d;
+    d.jobs = (BatchJob*)ctx->buffer(16, h.jobs.size() * sizeof(BatchJob));
+    d.items = (uint2*)ctx->buffer(17, h.items.size() * sizeof(uint2));
+    d.blob = (int32_t*)ctx->buffer(18, h.blob.size() * sizeof(int32_t));
+    d.set_p = (int32_t*)ctx->buffer(19, h.set_p.size() * sizeof(int32_t));
+    d.set_pinv = (uint32_t*)ctx->buffer(20, h.set_pinv.size() * sizeof(uint32_t));
+    d.slots = (BatchSlot*)ctx->buffer(21, h.slots.size() * sizeof(BatchSlot));
+    d.partials = (uint64_t*)ctx->buffer(22, h.items.size() * kMaxSet * sizeof(uint64_t));
+    d.counters = (uint32_t*)ctx->buffer(23, 32 * sizeof(uint32_t));
+    d.residues = (int64_t*)ctx->buffer(24, h.slots.size() * sizeof(int64_t));
+    auto* h_res = (int64_t*)ctx->host_buffer(4, h.slots.size() * sizeof(int64_t));
+    if (!d.jobs || !d.items || !d.blob || !d.set_p || !d.set_pinv || !d.slots || !d.partials || !d.counters ||
+        !d.residues || !h_res)
+        return -1;
+    hipStream_t st = ctx->stream();
+    PHIP_HIP_TRY(hipMemcpyAsync(d.jobs, h.jobs.data(), h.jobs.size() * sizeof(BatchJob), hipMemcpyHostToDevice, st));
+    PHIP_HIP_TRY(hipMemcpyAsync(d.items, h.items.data(), h.items.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
+    PHIP_HIP_TRY(hipMemcpyAsync(d.blob, h.blob.data(), h.blob.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    PHIP_HIP_TRY(
+        hipMemcpyAsync(d.set_p, h.set_p.data(), h.set_p.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    PHIP_HIP_TRY(hipMemcpyAsync(d.set_pinv, h.set_pinv.data(), h.set_pinv.size() * sizeof(uint32_t),
+                                hipMemcpyHostToDevice, st));
+    PHIP_HIP_TRY(
+        hipMemcpyAsync(d.slots, h.slots.data(), h.slots.size() * sizeof(BatchSlot), hipMemcpyHostToDevice, st));
+    const int cus = ctx->compute_units() > 0 ? ctx->compute_units() : 256;
+    const unsigned grid = (unsigned)(cus * t.batch_blocks_per_cu);
+    if (enqueue_plan(h, d, st, grid, ctx->profiling ? ctx->prof_start : nullptr,
+                     ctx->profiling ? ctx->prof_stop : nullptr) != 0)
+        return -1;
+    PHIP_HIP_TRY(hipMemcpyAsync(h_res, d.residues, h.slots.size() * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    PHIP_HIP_TRY(hipStreamSynchronize(st));
+    if (ctx->profiling) {
+        float ms = 0;
+        PHIP_HIP_TRY(hipEventElapsedTime(&ms, ctx->prof_start, ctx->prof_stop));
+        ctx->profiled_ms += ms;
+        ctx->profiled_launches += h.classes.size();
+    }
+    memcpy(out, h_res, h.slots.size() * sizeof(int64_t));
+    return 0;
+}
+
 namespace {
 
 int build_plan(BatchPlan& plan, const int64_t* entries, const int32_t* orders, int count, const int64_t* moduli,
@@ -189,16 +430,7 @@ int build_plan(BatchPlan& plan, const int64_t* entries, const int32_t* orders, i
         set_error("invalid batch description");
         return -1;
     }
-    // Pass 1: validate, pick the row-sum representation of every job.
-    struct Pending {
-        int matrix;
-        uint64_t entry_off;
-        int n;
-        uint32_t q;
-        int group;
-        std::vector<int> order;
-    };
-    std::vector<Pending> pend;
+    std::vector<PlanInput> in;
     uint64_t entry_off = 0;
     size_t mod_off = 0;
     for (int m = 0; m < count; ++m) {
@@ -217,91 +449,13 @@ int build_plan(BatchPlan& plan, const int64_t* entries, const int32_t* orders, i
                 set_error("batch moduli must be odd and lie in [3, 2^31)");
                 return -1;
             }
-            std::vector<int> order;
-            const int group = plan_rows(centred_row_bounds(entries + entry_off, n, q), order);
-            pend.push_back({m, entry_off, n, (uint32_t)q, group, std::move(order)});
+            in.push_back({entries + entry_off, n, (uint32_t)q, m});
         }
         entry_off += (uint64_t)n * n;
         mod_off += moduli_counts[m];
     }
-    // Item size: total states spread over the target item count.
-    double total_states = 0;
-    for (const auto& p : pend) total_states += std::ldexp(1.0, p.n - 1);
-    const double target = std::ldexp(1.0, tuning().batch_items_log2);
-    int s = (int)std::lround(std::log2(std::max(1.0, total_states / target)));
-    s = std::min(std::max(s, 10), 30);
-
-    // Pass 2: tables, jobs.
-    std::vector<int32_t> blob;
-    plan.jobs.resize(pend.size());
-    plan.total_states = 0;
-    plan.total_mulmods = 0;
-    for (size_t j = 0; j < pend.size(); ++j) {
-        const Pending& pd = pend[j];
-        const int n = pd.n;
-        const int L = n - 1;
-        int c = (L >= 10) ? std::max(4, std::min(s - 6, L - 6)) : 4;
-        c = std::min(c, L);
-        BatchJob jb;
-        jb.data_off = blob.size();
-        jb.n = n;
-        jb.cbits = c;
-        jb.lanes = 1ull << (L - c);
-        jb.p = (int32_t)pd.q;
-        jb.pinv = inverse_mod_2_32(pd.q);
-        jb.group = pd.group;
-        const int groups = pd.group == 0 ? n : (n + pd.group - 1) / pd.group;
-        const uint64_t q = pd.q;
-        jb.scale = (uint32_t)host_mulmod(host_powmod((1ull << 32) % q, (uint64_t)(groups - 1), q),
-                                         host_powmod((q + 1) / 2, (uint64_t)L, q), q);
-        const size_t words = (size_t)n * glynn_stride(n, pd.group);
-        blob.resize(blob.size() + words);
-        build_glynn_tables(entries + pd.entry_off, n, pd.q, pd.group, pd.order, blob.data() + jb.data_off);
-        plan.jobs[j] = jb;
-        plan.total_states += std::ldexp(1.0, L);
-        plan.total_mulmods += (double)n * std::ldexp(1.0, L);
-    }
-    // Items: per class, jobs by descending cost, each job's items contiguous.
-    std::vector<uint2> items;
-    plan.classes.clear();
-    for (int group : {0, 2, 3, 4, 6}) {
-        std::vector<uint32_t> order;
-        for (uint32_t j = 0; j < plan.jobs.size(); ++j)
-            if (plan.jobs[j].group == group) order.push_back(j);
-        if (order.empty()) continue;
-        auto cost = [&](uint32_t j) {
-            const BatchJob& b = plan.jobs[j];
-            const int groups = b.group == 0 ? b.n : (b.n + b.group - 1) / b.group;
-            return std::ldexp(1.0, b.cbits) * (double)std::min<uint64_t>(b.lanes, 64) * (b.n + 3.0 * groups);
-        };
-        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost(a) > cost(b); });
-        Class cl{group, (uint32_t)items.size(), 0};
-        for (uint32_t j : order) {
-            BatchJob& b = plan.jobs[j];
-            const uint64_t waves = (b.lanes + 63) / 64;
-            b.first_item = (uint32_t)items.size();
-            b.n_items = (uint32_t)waves;
-            for (uint64_t w = 0; w < waves; ++w) items.push_back(make_uint2(j, (uint32_t)w));
-        }
-        cl.n_items = (uint32_t)items.size() - cl.first_item;
-        plan.classes.push_back(cl);
-    }
-    if (items.size() >= 0xffffffffull) {
-        set_error("batch has too many work items");
-        return -1;
-    }
-    plan.n_items = (uint32_t)items.size();
-
-    if (plan.d_jobs.alloc(plan.jobs.size()) || plan.d_items.alloc(items.size()) || plan.d_blob.alloc(blob.size()) ||
-        plan.d_partials.alloc(items.size()) || plan.d_counters.alloc(8) || plan.d_res.alloc(plan.jobs.size()))
-        return -1;
-    PHIP_HIP_TRY(hipHostMalloc((void**)&plan.h_res, std::max<size_t>(plan.jobs.size(), 1) * sizeof(int64_t),
-                               hipHostMallocDefault));
-    PHIP_HIP_TRY(hipMemcpy(plan.d_jobs.ptr, plan.jobs.data(), plan.jobs.size() * sizeof(BatchJob),
-                           hipMemcpyHostToDevice));
-    PHIP_HIP_TRY(hipMemcpy(plan.d_items.ptr, items.data(), items.size() * sizeof(uint2), hipMemcpyHostToDevice));
-    PHIP_HIP_TRY(hipMemcpy(plan.d_blob.ptr, blob.data(), blob.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-    return 0;
+    if (build_plan_host(in, tuning().batch_items_log2, 4, plan.host) != 0) return -1;
+    return plan.upload();
 }
 
 }  // namespace
@@ -339,7 +493,7 @@ int phip_batch_run(void* handle, int64_t* residues_out) {
         return -1;
     }
     auto* plan = static_cast<BatchPlan*>(handle);
-    if (plan->jobs.empty()) return 0;
+    if (plan->host.slots.empty()) return 0;
     return plan->run(residues_out);
 }
 
@@ -349,7 +503,7 @@ int phip_batch_launch(void* handle) {
         return -1;
     }
     auto* plan = static_cast<BatchPlan*>(handle);
-    if (plan->jobs.empty()) return 0;
+    if (plan->host.slots.empty()) return 0;
     std::lock_guard<std::mutex> lock(plan->mu);
     if (plan->launched) {
         phip::set_error("batch plan is already running");
@@ -364,7 +518,7 @@ int phip_batch_query(void* handle, int64_t* residues_out, int block) {
         return -1;
     }
     auto* plan = static_cast<BatchPlan*>(handle);
-    if (plan->jobs.empty()) return 1;
+    if (plan->host.slots.empty()) return 1;
     std::lock_guard<std::mutex> lock(plan->mu);
     return plan->finish_locked(residues_out, block != 0);
 }
@@ -412,14 +566,17 @@ int phip_batch_stats(void* handle, double* out, int len) {
         return -1;
     }
     auto* plan = static_cast<BatchPlan*>(handle);
+    const phip::PlanHost& h = plan->host;
     const double v[] = {(double)plan->last_ms,
-                        plan->total_states,
-                        plan->total_mulmods,
-                        (double)plan->jobs.size(),
-                        (double)plan->n_items,
-                        (double)plan->classes.size(),
+                        h.total_states,
+                        h.total_mulmods,
+                        (double)h.jobs.size(),
+                        (double)h.items.size(),
+                        (double)h.classes.size(),
                         (double)(plan->cus * plan->blocks_per_cu * 4),
-                        plan->classes.empty() ? -1.0 : (double)plan->classes[0].group};
+                        h.classes.empty() ? -1.0 : (double)h.classes[0].group,
+                        (double)h.slots.size(),
+                        h.classes.empty() ? -1.0 : (double)h.classes[0].set};
     const int m = std::min<int>(len, (int)(sizeof(v) / sizeof(v[0])));
     for (int i = 0; i < m; ++i) out[i] = v[i];
     return m;

@@ -13,7 +13,7 @@
 //                    partials modulo that modulus (tree in LDS, no atomics).
 //
 // These paths exist for completeness of the modular contract (tiny orders,
-// wide or even moduli); the hot path is glynn31_kernel.
+// wide or even moduli); the hot path is the persistent kernel of batch.hip.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>

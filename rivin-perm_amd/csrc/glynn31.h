@@ -13,8 +13,7 @@
 // for the local index t, identical for every lane of the wavefront, so the
 // column deltas are read with scalar loads and the n running row sums live
 // in VGPRs.  The only lane-dependent flip direction is the one at
-// t = 2^(c-1), which reads bit c of the lane's base.  A launch covers every
-// prime at once: blockIdx.y selects the prime slice.
+// t = 2^(c-1), which reads bit c of the lane's base.
 //
 // Row-sum representations (template parameter G):
 //   G = 0     row sums are residues in [0, p), one per VGPR; a flip adds or
@@ -34,6 +33,11 @@
 // Products are combined with the signed Montgomery multiply in a balanced
 // tree; the accumulated sum therefore carries R^-(groups-1), which the
 // finalisation step removes together with 2^-(n-1).
+//
+// Prime sets.  Packed row sums are exact integers, so every prime whose
+// centred table is the same (all primes above twice the largest entry) sees
+// the same row sums and group products.  glynn_chunk_set walks the states
+// once for K such primes and runs only the Montgomery tree per prime.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -256,20 +260,98 @@ struct PackedRows {
 template <int NR, int G>
 using GlynnRows = typename std::conditional<G == 0, WideRows<NR>, PackedRows<NR, G>>::type;
 
-// One lane's chunk: the 2^cbits Gray states starting at `base` (a multiple
-// of 2^cbits).  `tables` holds the row sums of state 0 (all column signs +1:
-// exact centred sums for G > 0, packed two per word, or residues for G = 0)
-// followed by the flip deltas of the n-1 Gray columns, column-major with
-// glynn_stride(NR, G) words per column (2a_ij centred and packed, or
-// 2a_ij mod p).  Returns the lane's signed sum of terms reduced into
-// [0, p); it still carries R^-(groups-1) and lacks 2^-(n-1).
+// Per-state consumers of the Gray walk.  state<ODD>(rows) receives the row
+// sums of every state of the chunk; ODD marks odd local indices, whose
+// terms carry the opposite sign of the chunk's first state.
+//
+// One prime: the product of the row sums through the balanced Montgomery
+// tree, summed separately for even and odd local indices.
 template <int NR, int G>
-__device__ __forceinline__ uint64_t glynn_chunk(const int32_t* __restrict__ tables, int n, int cbits,
-                                                uint64_t base, int32_t p, uint32_t pinv) {
+struct OnePrimeSink {
+    using Rows = GlynnRows<NR, G>;
+    int32_t negp;
+    uint32_t pinv;
+    int64_t acc_e = 0, acc_o = 0;
+    template <bool ODD>
+    __device__ __forceinline__ void state(const Rows& rs) {
+        if (ODD) {
+            acc_o += rs.term(negp, pinv);
+        } else {
+            acc_e += rs.term(negp, pinv);
+        }
+    }
+};
+
+// Montgomery reduction of an exact signed product |t| < 2^62:
+// t * R^-1 mod p with |result| < 2^31 (the second half of mont31).
+__device__ __forceinline__ int32_t redc31(int64_t t, int32_t negp, uint32_t pinv) {
+    int32_t m = (int32_t)((uint32_t)t * pinv);
+    int64_t u = t + (int64_t)m * (int64_t)negp;
+    return (int32_t)(u >> 32);
+}
+
+// K primes sharing one packed table.  The exact row sums, the exact group
+// products and the exact 64-bit products of adjacent groups (|g| < 2^31, so
+// |g g'| < 2^62) do not depend on the prime and are formed once per state;
+// each prime then reduces the pair products with one Montgomery reduction
+// apiece (the first level of the same balanced tree, bit-identical to
+// mont31 on the two groups) and combines the leaves with mont31.  The term
+// therefore carries R^-(groups-1) exactly as in OnePrimeSink.
+template <int NR, int G, int K>
+struct PrimeSetSink {
+    static_assert(G > 0, "prime sets need exact packed row sums");
+    using Rows = PackedRows<NR, G>;
+    static constexpr int kGroups = Rows::kGroups;
+    static constexpr int kPairs = kGroups / 2;
+    static constexpr int kLeaves = kPairs + (kGroups & 1);
+    int32_t negp[K];
+    uint32_t pinv[K];
+    int64_t acc[K];
+
+    template <bool ODD>
+    __device__ __forceinline__ void state(const Rows& rs) {
+        int32_t g[kGroups];
+        rs.groups(g, std::make_integer_sequence<int, kGroups>{});
+        int64_t P[kPairs > 0 ? kPairs : 1];
+#pragma unroll
+        for (int j = 0; j < kPairs; ++j) P[j] = (int64_t)g[2 * j] * (int64_t)g[2 * j + 1];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            int32_t v[kLeaves];
+#pragma unroll
+            for (int j = 0; j < kPairs; ++j) v[j] = redc31(P[j], negp[k], pinv[k]);
+            if constexpr (kGroups & 1) v[kLeaves - 1] = g[kGroups - 1];
+#pragma unroll
+            for (int w = 1; w < kLeaves; w *= 2) {
+#pragma unroll
+                for (int i = 0; i + w < kLeaves; i += 2 * w) v[i] = mont31(v[i], v[i + w], negp[k], pinv[k]);
+            }
+            if (ODD) {
+                acc[k] -= v[0];
+            } else {
+                acc[k] += v[0];
+            }
+            // Keeps the scheduler from interleaving the trees of all primes,
+            // whose live leaves would otherwise exhaust the register file.
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+};
+
+// Walks one lane's chunk: the 2^cbits Gray states starting at `base` (a
+// multiple of 2^cbits), handing the row sums of each state to the sink.
+// `tables` holds the row sums of state 0 (all column signs +1: exact centred
+// sums for G > 0, packed two per word, or residues for G = 0) followed by
+// the flip deltas of the n-1 Gray columns, column-major with
+// glynn_stride(NR, G) words per column (2a_ij centred and packed, or
+// 2a_ij mod p; p is read only for G = 0).  Returns the parity of gray(base),
+// the sign of the chunk's first state.
+template <int NR, int G, class Sink>
+__device__ __forceinline__ bool gray_walk(const int32_t* __restrict__ tables, int n, int cbits, uint64_t base,
+                                          int32_t p, Sink& sink) {
     using Rows = GlynnRows<NR, G>;
     constexpr int W = Rows::kStride;
     const int32_t* __restrict__ D = tables + W;
-    const int32_t negp = -p;
     const uint64_t gray = base ^ (base >> 1);
 
     // Row sums of the chunk's first state: subtract 2a for every set Gray
@@ -293,64 +375,77 @@ __device__ __forceinline__ uint64_t glynn_chunk(const int32_t* __restrict__ tabl
     const int32_t* __restrict__ d1 = D + W;
     const int32_t* __restrict__ d2 = D + 2 * W;
 
-    // Terms at even local index carry the chunk's starting sign, odd ones
-    // the opposite sign.
-    int64_t acc_e = rs.term(negp, pinv);
-    int64_t acc_o = 0;
+    sink.template state<false>(rs);
     const uint32_t T = 1u << cbits;
     for (uint32_t t0 = 0; t0 < T; t0 += kGlynnUnroll) {
         // Directions for s = 1..15 follow from bit j+1 of t0+s, which lies
         // inside the unrolled block except for s = 8 (bit 4 of base+t0).
-        rs.template flip<true>(c0, p);         acc_o += rs.term(negp, pinv);  // s=1
-        rs.template flip_mem<true>(d1, p);     acc_e += rs.term(negp, pinv);  // s=2
-        rs.template flip<false>(c0, p);        acc_o += rs.term(negp, pinv);  // s=3
-        rs.template flip_mem<true>(d2, p);     acc_e += rs.term(negp, pinv);  // s=4
-        rs.template flip<true>(c0, p);         acc_o += rs.term(negp, pinv);  // s=5
-        rs.template flip_mem<false>(d1, p);    acc_e += rs.term(negp, pinv);  // s=6
-        rs.template flip<false>(c0, p);        acc_o += rs.term(negp, pinv);  // s=7
+        rs.template flip<true>(c0, p);         sink.template state<true>(rs);   // s=1
+        rs.template flip_mem<true>(d1, p);     sink.template state<false>(rs);  // s=2
+        rs.template flip<false>(c0, p);        sink.template state<true>(rs);   // s=3
+        rs.template flip_mem<true>(d2, p);     sink.template state<false>(rs);  // s=4
+        rs.template flip<true>(c0, p);         sink.template state<true>(rs);   // s=5
+        rs.template flip_mem<false>(d1, p);    sink.template state<false>(rs);  // s=6
+        rs.template flip<false>(c0, p);        sink.template state<true>(rs);   // s=7
         {
             // s=8 flips column 3; its new Gray bit is 1 ^ bit 4 of (base+t).
             const uint64_t t = base + t0 + 8;
             rs.flip_lane(D + 3 * W, !((t >> 4) & 1u), p);
-            acc_e += rs.term(negp, pinv);
+            sink.template state<false>(rs);
         }
-        rs.template flip<true>(c0, p);         acc_o += rs.term(negp, pinv);  // s=9
-        rs.template flip_mem<true>(d1, p);     acc_e += rs.term(negp, pinv);  // s=10
-        rs.template flip<false>(c0, p);        acc_o += rs.term(negp, pinv);  // s=11
-        rs.template flip_mem<false>(d2, p);    acc_e += rs.term(negp, pinv);  // s=12
-        rs.template flip<true>(c0, p);         acc_o += rs.term(negp, pinv);  // s=13
-        rs.template flip_mem<false>(d1, p);    acc_e += rs.term(negp, pinv);  // s=14
-        rs.template flip<false>(c0, p);        acc_o += rs.term(negp, pinv);  // s=15
+        rs.template flip<true>(c0, p);         sink.template state<true>(rs);   // s=9
+        rs.template flip_mem<true>(d1, p);     sink.template state<false>(rs);  // s=10
+        rs.template flip<false>(c0, p);        sink.template state<true>(rs);   // s=11
+        rs.template flip_mem<false>(d2, p);    sink.template state<false>(rs);  // s=12
+        rs.template flip<true>(c0, p);         sink.template state<true>(rs);   // s=13
+        rs.template flip_mem<false>(d1, p);    sink.template state<false>(rs);  // s=14
+        rs.template flip<false>(c0, p);        sink.template state<true>(rs);   // s=15
         const uint32_t tn = t0 + kGlynnUnroll;
         if (tn < T) {
             const int j = __builtin_ctz(tn);  // 4 <= j <= cbits-1, uniform
             const uint64_t t = base + tn;
             rs.flip_lane(D + j * W, !((t >> (j + 1)) & 1u), p);
-            acc_e += rs.term(negp, pinv);
+            sink.template state<false>(rs);
         }
     }
+    return __popcll(gray) & 1;
+}
 
-    int64_t acc = acc_e - acc_o;
-    if (__popcll(gray) & 1) acc = -acc;
+// Signed lane sum reduced into [0, p).
+__device__ __forceinline__ uint64_t lane_residue(int64_t acc, bool negate, int32_t p) {
+    if (negate) acc = -acc;
     int64_t r = acc % (int64_t)p;
     if (r < 0) r += p;
     return (uint64_t)r;
 }
 
-// tables[k] holds one prime's glynn_chunk tables (n * stride words);
-// partials[k][gridDim.x] receives each block's sum of lane residues.
+// One prime: the lane's signed sum of terms reduced into [0, p); it still
+// carries R^-(groups-1) and lacks 2^-(n-1).
 template <int NR, int G>
-__global__ __launch_bounds__(kGlynnBlock) void glynn31_kernel(
-    const int32_t* __restrict__ tables, const int32_t* __restrict__ primes,
-    const uint32_t* __restrict__ pinvs, int n, int cbits, uint64_t lanes_per_prime,
-    uint64_t* __restrict__ partials) {
-    const int q = blockIdx.y;
-    const uint64_t lane = (uint64_t)blockIdx.x * kGlynnBlock + threadIdx.x;
-    const bool active = lane < lanes_per_prime;
-    const uint64_t r = glynn_chunk<NR, G>(tables + (size_t)q * n * GlynnRows<NR, G>::kStride, n, cbits,
-                                          (active ? lane : 0) << cbits, primes[q], pinvs[q]);
-    const uint64_t block_sum = block_reduce_sum_u64<kGlynnBlock>(active ? r : 0u);
-    if (threadIdx.x == 0) partials[(size_t)q * gridDim.x + blockIdx.x] = block_sum;
+__device__ __forceinline__ uint64_t glynn_chunk(const int32_t* __restrict__ tables, int n, int cbits,
+                                                uint64_t base, int32_t p, uint32_t pinv) {
+    OnePrimeSink<NR, G> sink;
+    sink.negp = -p;
+    sink.pinv = pinv;
+    const bool neg = gray_walk<NR, G>(tables, n, cbits, base, p, sink);
+    return lane_residue(sink.acc_e - sink.acc_o, neg, p);
+}
+
+// K primes sharing one packed table: out[k] as glynn_chunk for primes[k].
+template <int NR, int G, int K>
+__device__ __forceinline__ void glynn_chunk_set(const int32_t* __restrict__ tables, int n, int cbits,
+                                                uint64_t base, const int32_t (&primes)[K],
+                                                const uint32_t (&pinvs)[K], uint64_t (&out)[K]) {
+    PrimeSetSink<NR, G, K> sink;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        sink.negp[k] = -primes[k];
+        sink.pinv[k] = pinvs[k];
+        sink.acc[k] = 0;
+    }
+    const bool neg = gray_walk<NR, G>(tables, n, cbits, base, 0, sink);
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = lane_residue(sink.acc[k], neg, primes[k]);
 }
 
 }  // namespace phip

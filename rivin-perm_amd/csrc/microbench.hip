@@ -3,7 +3,8 @@
 //   phip_valu_microbench  throughput of the VALU instructions the Glynn
 //                         kernels issue (v_add_u32, v_mul_i32_i24,
 //                         v_mul_lo_u32, v_mul_hi_i32, v_mad_i64_i32,
-//                         v_mad_u64_u32, v_pk_add_u16, SDWA v_mul_i32_i24)
+//                         v_mad_u64_u32, v_pk_add_u16, SDWA v_mul_i32_i24,
+//                         v_lshl_add_u64, v_ashrrev_i32)
 //                         and of one complete signed Montgomery product,
 //                         each as lane-operations per
 //                         second over the whole device.  Eight independent
@@ -65,6 +66,10 @@ __global__ __launch_bounds__(256) void valu_rate_kernel(uint32_t* out, int iters
                     x[k] = (uint32_t)mont31_mb((int32_t)x[k], (int32_t)c, -p, pinv);
                 } else if constexpr (OP == 7) {
                     asm volatile("v_pk_add_u16 %0, %0, %1" : "+v"(x[k]) : "v"(c));
+                } else if constexpr (OP == 9) {
+                    asm volatile("v_lshl_add_u64 %0, %0, 0, %1" : "+v"(y[k]) : "v"((uint64_t)c));
+                } else if constexpr (OP == 10) {
+                    asm volatile("v_ashrrev_i32 %0, 1, %0" : "+v"(x[k]));
                 } else {
                     asm volatile(
                         "v_mul_i32_i24_sdwa %0, sext(%0), sext(%1) dst_sel:DWORD dst_unused:UNUSED_PAD "
@@ -109,7 +114,8 @@ int phip_valu_microbench(double* lane_ops_per_s, int len) {
     using Kernel = void (*)(uint32_t*, int, uint32_t);
     const Kernel kernels[] = {phip::valu_rate_kernel<0>, phip::valu_rate_kernel<1>, phip::valu_rate_kernel<2>,
                               phip::valu_rate_kernel<3>, phip::valu_rate_kernel<4>, phip::valu_rate_kernel<5>,
-                              phip::valu_rate_kernel<6>, phip::valu_rate_kernel<7>, phip::valu_rate_kernel<8>};
+                              phip::valu_rate_kernel<6>, phip::valu_rate_kernel<7>, phip::valu_rate_kernel<8>,
+                              phip::valu_rate_kernel<9>, phip::valu_rate_kernel<10>};
     const int nk = (int)(sizeof(kernels) / sizeof(kernels[0]));
     int written = 0;
     for (int k = 0; k < nk && k < len; ++k) {

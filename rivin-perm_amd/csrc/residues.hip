@@ -1,8 +1,7 @@
 // Residue engine: splits each modulus m = 2^e * q into an odd part served by
 // the Glynn kernels and a power-of-two part served by the wrapping Ryser
-// kernel, prepares the per-modulus tables on the host, runs every odd
-// modulus of one arithmetic class in a single launch, and reassembles the
-// residues (CRT for even moduli).
+// kernel, runs every odd part of one arithmetic class together, and
+// reassembles the residues (CRT for even moduli).
 //
 // Host preparation follows the reference's normalisation of entries before
 // fixed-width arithmetic (perm_c_omp.c:912-916, normalize_mod_i64).
@@ -13,6 +12,7 @@
 #include <map>
 #include <string>
 
+#include "batch.h"
 #include "generic_kernels.h"
 #include "launchers.h"
 #include "phip_arith.h"
@@ -40,82 +40,9 @@ uint64_t pow2_mod(int e, uint64_t q) { return host_powmod(2, (uint64_t)e, q); }
 struct OddJob {
     int index;   // position in the caller's modulus list
     uint64_t q;  // odd part, > 1
-    std::vector<int> order = {};  // kernel row order (packed groups)
 };
 
 const int64_t* matrix_of(const int64_t* A, int64_t stride, int index) { return A + (int64_t)index * stride; }
-
-// ---------------------------------------------------------------------
-// 31-bit Glynn (register-resident rows).
-// ---------------------------------------------------------------------
-int run_glynn31(DeviceContext* ctx, const int64_t* A, int64_t stride, int n, int group,
-                const std::vector<OddJob>& jobs, std::vector<uint64_t>& odd_residue) {
-    const int kk = (int)jobs.size();
-    const int NR = n;
-    const size_t words = (size_t)n * glynn_stride(n, group);
-    std::vector<int32_t> tables((size_t)kk * words), primes(kk);
-    std::vector<uint32_t> pinvs(kk);
-    std::vector<uint64_t> moduli(kk);
-    for (int s = 0; s < kk; ++s) {
-        const uint32_t q = (uint32_t)jobs[s].q;
-        primes[s] = (int32_t)q;
-        pinvs[s] = inverse_mod_2_32(q);
-        moduli[s] = q;
-        build_glynn_tables(matrix_of(A, stride, jobs[s].index), n, q, group, jobs[s].order,
-                           tables.data() + (size_t)s * words);
-    }
-
-    int cbits = (n - 1) + ceil_log2((uint64_t)kk) - tuning().lanes_log2;
-    cbits = std::max(cbits, std::max(4, tuning().min_cbits));
-    cbits = std::min(cbits, std::min(n - 1, 30));
-    const uint64_t lanes = 1ull << (n - 1 - cbits);
-    const uint64_t gx = (lanes + kGlynnBlock - 1) / kGlynnBlock;
-    if (gx > 0x7fffffffull) {
-        set_error("internal: grid too large");
-        return -1;
-    }
-
-    hipStream_t st = ctx->stream();
-    auto* d_tab = (int32_t*)ctx->buffer(0, tables.size() * 4);
-    auto* d_pr = (int32_t*)ctx->buffer(2, primes.size() * 4);
-    auto* d_pi = (uint32_t*)ctx->buffer(3, pinvs.size() * 4);
-    auto* d_part = (uint64_t*)ctx->buffer(4, (size_t)kk * gx * 8);
-    auto* d_mod = (uint64_t*)ctx->buffer(5, moduli.size() * 8);
-    auto* d_sum = (uint64_t*)ctx->buffer(6, moduli.size() * 8);
-    auto* h_sum = (uint64_t*)ctx->host_buffer(0, moduli.size() * 8);
-    if (!d_tab || !d_pr || !d_pi || !d_part || !d_mod || !d_sum || !h_sum) return -1;
-
-    PHIP_HIP_TRY(hipMemcpyAsync(d_tab, tables.data(), tables.size() * 4, hipMemcpyHostToDevice, st));
-    PHIP_HIP_TRY(hipMemcpyAsync(d_pr, primes.data(), primes.size() * 4, hipMemcpyHostToDevice, st));
-    PHIP_HIP_TRY(hipMemcpyAsync(d_pi, pinvs.data(), pinvs.size() * 4, hipMemcpyHostToDevice, st));
-    PHIP_HIP_TRY(hipMemcpyAsync(d_mod, moduli.data(), moduli.size() * 8, hipMemcpyHostToDevice, st));
-
-    Glynn31Args args{d_tab, d_pr, d_pi, n, cbits, lanes, d_part};
-    if (ctx->profiling) PHIP_HIP_TRY(hipEventRecord(ctx->prof_start, st));
-    PHIP_HIP_TRY(launch_glynn31(group, NR, dim3((unsigned)gx, (unsigned)kk), st, args));
-    if (ctx->profiling) PHIP_HIP_TRY(hipEventRecord(ctx->prof_stop, st));
-    hipLaunchKernelGGL(sum_mod_kernel, dim3(kk), dim3(256), 0, st, d_part, gx, d_mod, d_sum);
-    PHIP_HIP_TRY(hipGetLastError());
-    PHIP_HIP_TRY(hipMemcpyAsync(h_sum, d_sum, (size_t)kk * 8, hipMemcpyDeviceToHost, st));
-    PHIP_HIP_TRY(hipStreamSynchronize(st));
-    if (ctx->profiling) {
-        float ms = 0;
-        PHIP_HIP_TRY(hipEventElapsedTime(&ms, ctx->prof_start, ctx->prof_stop));
-        ctx->profiled_ms += ms;
-        ctx->profiled_launches += 1;
-    }
-
-    const int groups = (group == 0) ? NR : (NR + group - 1) / group;
-    for (int s = 0; s < kk; ++s) {
-        const uint64_t q = jobs[s].q;
-        const uint64_t r_mod_q = ((1ull << 32) % q);
-        const uint64_t inv2 = (q + 1) / 2;
-        const uint64_t scale = host_mulmod(host_powmod(r_mod_q, (uint64_t)(groups - 1), q),
-                                           host_powmod(inv2, (uint64_t)(n - 1), q), q);
-        odd_residue[jobs[s].index] = host_mulmod(h_sum[s] % q, scale, q);
-    }
-    return 0;
-}
 
 // ---------------------------------------------------------------------
 // Generic 64-bit Montgomery Glynn (any odd q < 2^63, any order >= 2).
@@ -271,25 +198,28 @@ int compute_residues(DeviceContext* ctx, const int64_t* A, int64_t stride, int n
     }
 
     std::vector<uint64_t> odd_residue(k, 0);
-    std::map<int, std::vector<OddJob>> by_group;  // 31-bit class, keyed by group size
-    std::vector<OddJob> wide;                     // 63-bit class
+    // Odd parts below 2^31 at orders [5, 40] run on the persistent Glynn
+    // engine as a one-matrix plan; moduli whose tables coincide share one
+    // Gray walk.  The rest use the 64-bit Montgomery kernel.
+    std::vector<PlanInput> plan_in;
+    std::vector<int> plan_index;
+    std::vector<OddJob> wide;
     for (int i = 0; i < k; ++i) {
         const uint64_t m = (uint64_t)moduli[i];
         const int e = __builtin_ctzll(m);
         const uint64_t q = m >> e;
         if (q <= 1) continue;
         if (q < (1ull << 31) && n >= kGlynnMinOrder && n <= kGlynnMaxOrder) {
-            // Maximum absolute row sum of the centred residues picks the
-            // row-sum representation.
-            std::vector<int> order;
-            const int group = plan_rows(centred_row_bounds(matrix_of(A, stride, i), n, (int64_t)q), order);
-            by_group[group].push_back({i, q, order});
+            plan_in.push_back({matrix_of(A, stride, i), n, (uint32_t)q, 0});
+            plan_index.push_back(i);
         } else {
             wide.push_back({i, q});
         }
     }
-    for (auto& kv : by_group) {
-        if (run_glynn31(ctx, A, stride, n, kv.first, kv.second, odd_residue) != 0) return -1;
+    if (!plan_in.empty()) {
+        std::vector<int64_t> res(plan_in.size());
+        if (run_plan_in_context(ctx, plan_in, res.data()) != 0) return -1;
+        for (size_t j = 0; j < plan_in.size(); ++j) odd_residue[plan_index[j]] = (uint64_t)res[j];
     }
     if (!wide.empty() && run_glynn63(ctx, A, stride, n, wide, odd_residue) != 0) return -1;
 

@@ -80,6 +80,8 @@ MICROBENCH_NAMES = (
     "mont31_product",
     "v_pk_add_u16",
     "v_mul_i32_i24_sdwa",
+    "v_lshl_add_u64",
+    "v_ashrrev_i32",
 )
 
 
@@ -295,10 +297,10 @@ class Batch:
         return [int(v) for v in self._out[: self.total]]
 
     def stats(self) -> dict:
-        vals = (ctypes.c_double * 8)()
-        count = self._lib.phip_batch_stats(self._handle, vals, 8)
+        keys = ("last_ms", "states", "mulmods", "jobs", "items", "launches", "waves", "group", "residues", "set")
+        vals = (ctypes.c_double * len(keys))()
+        count = self._lib.phip_batch_stats(self._handle, vals, len(keys))
         _check(count, "phip_batch_stats")
-        keys = ("last_ms", "states", "mulmods", "jobs", "items", "launches", "waves", "group")
         return {k: vals[i] for i, k in enumerate(keys[:count])}
 
     def close(self) -> None:

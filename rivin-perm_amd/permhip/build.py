@@ -2,8 +2,8 @@
 
 Compiles every translation unit under ``csrc/`` with hipcc into
 ``_build/`` and links ``permhip/libpermhip.so`` in-tree, so the library
-travels with the source tree.  The 31-bit Glynn instantiations are split
-into one object per row-sum group size so they compile in parallel.
+travels with the source tree.  The persistent Glynn kernel is compiled once
+per (row-sum group, prime-set size) class so the classes build in parallel.
 
 Usage: ``python -m permhip.build`` from ``rivin-perm_amd/`` or
 ``permhip.build.build()``.
@@ -26,8 +26,8 @@ BUILD_DIR = ROOT / "_build"
 LIBRARY = PACKAGE_DIR / "libpermhip.so"
 
 ARCH = "gfx950"
-GLYNN_GROUPS = (0, 2, 3, 4, 6)
-BATCH_GROUPS = (0, 2, 3, 4, 6)
+# (row-sum group, prime-set size) classes of the persistent Glynn kernel.
+BATCH_CLASSES = ((0, 1),) + tuple((g, k) for g in (2, 3, 4, 6) for k in (1, 2, 4, 8))
 PLAIN_SOURCES = (
     "device_context.hip",
     "residues.hip",
@@ -76,13 +76,14 @@ def build(jobs: int | None = None, verbose: bool = False) -> Path:
     """Compile all sources and link ``libpermhip.so``; returns its path."""
     BUILD_DIR.mkdir(parents=True, exist_ok=True)
     units = []
-    for group in GLYNN_GROUPS:
+    # Largest prime sets first: they take longest to compile.
+    for group, k in sorted(BATCH_CLASSES, key=lambda gk: -gk[1]):
         units.append(
-            (CSRC / "glynn31_inst.hip", BUILD_DIR / f"glynn31_g{group}.o", [f"-DPHIP_GROUP={group}"])
-        )
-    for group in BATCH_GROUPS:
-        units.append(
-            (CSRC / "batch.hip", BUILD_DIR / f"batch_g{group}.o", [f"-DPHIP_BATCH_GROUP={group}"])
+            (
+                CSRC / "batch.hip",
+                BUILD_DIR / f"batch_g{group}_k{k}.o",
+                [f"-DPHIP_BATCH_GROUP={group}", f"-DPHIP_BATCH_SET={k}"],
+            )
         )
     for name in PLAIN_SOURCES:
         units.append((CSRC / name, BUILD_DIR / (Path(name).stem + ".o"), []))

@@ -67,6 +67,54 @@ def test_larger_orders_against_openmp_oracle(native, oracle_c, n):
         assert r == oracle_c.permanent_mod_p(A, m, 8), (n, m)
 
 
+def _primes_from(start, count):
+    from permhip import planner
+
+    out, q = [], start | 1
+    while len(out) < count:
+        if planner.is_prime(q):
+            out.append(q)
+        q += 2
+    return out
+
+
+@pytest.mark.parametrize("k", [2, 3, 4, 5, 6, 7, 8, 9, 12])
+@pytest.mark.parametrize(
+    "n,lo,hi",
+    [
+        (14, -10, 10),    # groups of 4
+        (17, -3, 3),      # groups of 6, partial last group
+        (16, 0, 1),       # groups of 6
+        (13, -100, 100),  # groups of 3
+        (12, -1000, 1000),  # groups of 2
+        (5, -2, 2),       # a single group
+        (7, -10, 10),
+    ],
+)
+def test_prime_sets_against_oracle(native, oracle_c, n, lo, hi, k):
+    """Primes whose centred tables coincide share one Gray walk (sets of 2, 4
+    or 8, padded); every residue must equal the oracle's."""
+    rng = random.Random(4000 + 31 * n + k)
+    A = _random_matrix(rng, n, lo, hi)
+    primes = _primes_from(1 << 30, k)
+    got = native.permanent_residues(np.array(A, dtype=np.int64), primes)
+    for p, r in zip(primes, got):
+        assert r == oracle_c.permanent_mod_p(A, p, 8), (n, lo, hi, k, p)
+
+
+def test_prime_sets_mixed_tables(native, oracle_c):
+    """Moduli below twice the largest entry get their own tables; the rest
+    share one.  Order of the answers follows the caller's modulus order."""
+    rng = random.Random(77)
+    n = 15
+    A = _random_matrix(rng, n, -60, 60)
+    big = _primes_from(1 << 30, 6)
+    moduli = [big[0], 101, big[1], 103, big[2], 9, big[3], 2147483647, big[4], 65537, big[5]]
+    got = native.permanent_residues(np.array(A, dtype=np.int64), moduli)
+    for m, r in zip(moduli, got):
+        assert r == oracle_c.permanent_mod_p(A, m, 8), m
+
+
 def test_int64_extremes(native, oracle_c):
     A = [[INT64_MIN, INT64_MAX, -1, 5, 7], [INT64_MAX, INT64_MIN, 1, 0, 3],
          [0, -1, INT64_MAX, 2, 2], [1, 1, 1, 1, 1], [INT64_MIN, 0, 0, 0, 9]]
