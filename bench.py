@@ -12,7 +12,8 @@ Printed as one JSON line (rank 0 only; this build runs on one GPU):
   value        algorithmic Gray-code mulmods per second, n 2^(n-1) per
                residue (the reference's n-factor row product per state)
   roofline     the dominant kernel against the VALU issue bound of its own
-               instruction mix (rates measured live by a microbenchmark)
+               instruction mix (a synthetic kernel issuing the same
+               per-state instruction multiset, measured live)
   cpu_baseline the reference's own OpenMP kernel (perm_c_omp.c, compiled
                from its sources into oracle/_ref) on a bounded sample
   plain / batch / serve   the per-matrix plain launch, the 4096-matrix
@@ -60,42 +61,48 @@ def log(msg: str) -> None:
 
 def instruction_mix(n: int, group: int, k: int = 1) -> dict:
     """VALU instructions per Gray state of one walk of gray_walk<n, group>
-    serving a set of k primes (the unrolled 16-state step, amortised).
+    serving a set of k primes (packed rows; the unrolled 16-state step
+    amortised, as counted in the compiled loop).
 
-    Shared by the set: row-sum flips (one packed op per two rows) and the
-    exact group products; for k > 1 also the exact 64-bit products of
-    adjacent groups.  Per prime: the Montgomery tree (two v_mad_i64_i32 and
-    one v_mul_lo_u32 per product; one of each per reduced pair product) and
-    the signed 64-bit accumulation (v_ashrrev_i32 + v_lshl_add_u64)."""
-    groups = n if group == 0 else -(-n // group)
-    mix = {"v_mad_i64_i32": 0, "v_mul_lo_u32": 0}
+    Shared by the set: row-sum flips (one packed op per two rows, plus the
+    two lane-dependent flips per 16 states that issue both directions and a
+    select), the exact group products, and for k > 1 the exact 64-bit
+    products of adjacent groups.  Per prime: the Montgomery tree (two
+    v_mad_i64_i32 and one v_mul_lo_u32 per product; one of each per reduced
+    pair product) and the in-order signed accumulation (one v_mad_i64_i32)."""
     if group == 0:
-        mix["v_add_u32"] = 3 * n  # add/sub, wrap, min per row
-    else:
-        mix["v_pk_add_u16"] = -(-n // 2)
-        mix["v_mul_i32_i24_sdwa"] = n - groups
-        mix["v_add_u32"] = 1 if n % group == 1 else 0
+        raise ValueError("the roofline mix covers packed row sums")
+    groups = -(-n // group)
+    words = -(-n // 2)
+    mix = {
+        "v_pk_add_u16": words + words / 8,
+        "v_mul_i32_i24_sdwa": n - groups,
+        "v_add_u32": words / 8 + 2,  # selects of the lane-dependent flips, register moves
+    }
     if k == 1:
-        mix["v_mad_i64_i32"] += 2 * (groups - 1)
-        mix["v_mul_lo_u32"] += groups - 1
+        mix["v_mad_i64_i32"] = 2 * (groups - 1) + 1
+        mix["v_mul_lo_u32"] = groups - 1
     else:
         pairs = groups // 2
         leaves = pairs + groups % 2
-        mix["v_mad_i64_i32"] += pairs + k * (pairs + 2 * (leaves - 1))
-        mix["v_mul_lo_u32"] += k * (pairs + leaves - 1)
-    mix["v_ashrrev_i32"] = k
-    mix["v_lshl_add_u64"] = k
+        mix["v_mad_i64_i32"] = pairs + k * (pairs + 2 * (leaves - 1) + 1)
+        mix["v_mul_lo_u32"] = k * (pairs + leaves - 1)
     return mix
 
 
-def roofline(n: int, group: int, k: int, rates: dict, mulmods_per_s: float) -> dict:
-    """Mulmod roofline of the kernel's own instruction mix at the measured
-    instruction rates (the kernel is VALU-issue bound; HBM traffic is a few
-    MB per launch).  One walk of a k-prime set does k n algorithmic
-    mulmods per state."""
+def roofline(n: int, group: int, k: int, mulmods_per_s: float) -> dict:
+    """The dominant kernel against the issue bound of its own instruction
+    stream: a synthetic kernel issues the same per-state instruction
+    multiset (8 states per iteration, every type on independent chains, no
+    memory, no branches) at full occupancy, and its state rate times the
+    k n algorithmic mulmods per state is the peak.  HBM traffic is a few MB
+    per launch, so the path is VALU-issue bound."""
+    from permhip import _native
+
     mix = instruction_mix(n, group, k)
-    seconds_per_state_lane = sum(count / rates[name] for name, count in mix.items())
-    peak = k * n / seconds_per_state_lane  # mulmods per second, whole device
+    rounds = [round(mix[t]) for t in _native.MIX_TYPES]  # 8 states x count / 8 chains
+    states_per_s = 8.0 * _native.mix_microbench(rounds, iters=128)
+    peak = states_per_s * k * n
     return {
         "bound": "valu",
         "achieved": mulmods_per_s / 1e12,
@@ -106,8 +113,7 @@ def roofline(n: int, group: int, k: int, rates: dict, mulmods_per_s: float) -> d
         "group": group,
         "prime_set": k,
         "instructions_per_state": mix,
-        # Every mulmod as one signed Montgomery product plus one row-sum add.
-        "general_montgomery_peak": 1.0 / (1.0 / rates["mont31_product"] + 1.0 / rates["v_add_u32"]) / 1e12,
+        "mix_rounds_per_8_states": dict(zip(_native.MIX_TYPES, rounds)),
     }
 
 
@@ -162,6 +168,37 @@ def cpu_baseline(matrix: np.ndarray, primes, threads: int) -> dict:
     }
 
 
+def published_workloads() -> dict:
+    """End-to-end times (host planning, device residues, CRT) of workloads
+    the reference publishes timings for, next to its numbers
+    (BINARY_APPLICATIONS.md:155-172, README.md:594-596)."""
+    import perm
+    import perm_binary
+    from permhip import grids
+
+    out = {}
+    G = grids.grid_biadjacency((4, 4, 5), periodic_axes=(0, 1))
+    t0 = time.perf_counter()
+    count = perm_binary.binary_permanent(G)
+    dt = time.perf_counter() - t0
+    if count != 2551276235535120:
+        raise RuntimeError("40x40 periodic grid count is wrong")
+    out["binary_40x40_periodic_grid_exact"] = {
+        "s": dt, "value": str(count),
+        "reference_s": {"GH200 CUDA end-to-end": 22.73, "GH200 CPU 64 threads": 242.38},
+    }
+    # perm(S_37) and perm(S_41) as checked in by the reference (schur_permanents.txt).
+    known = {37: -339696021338069660245, 41: 1617562409111471411255401}
+    for n, ref in ((37, {"64-core CPU C/OpenMP": 18.43}), (41, {"GH200 CUDA 40x40-minor necklace": 47.39})):
+        t0 = time.perf_counter()
+        value = perm.permanent(n)
+        dt = time.perf_counter() - t0
+        if value != known[n]:
+            raise RuntimeError(f"perm(S_{n}) is wrong")
+        out[f"schur_{n}_exact"] = {"s": dt, "value": str(value), "reference_s": ref}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,6 +207,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 4096-matrix batch leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
+    ap.add_argument("--no-published", action="store_true",
+                    help="skip the workloads the reference publishes timings for")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--kernel-only", action="store_true", help="headline launches only (for profilers)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -240,7 +279,12 @@ def main() -> int:
     log("VALU lane-ops/s: " + ", ".join(f"{k}={v:.3e}" for k, v in rates.items()))
     st = plan.stats()
     group, kset = int(st["group"]), int(st["set"])
-    roof = roofline(HEADLINE_N, group, kset, rates, kernel_rate)
+    roof = roofline(HEADLINE_N, group, kset, kernel_rate)
+    # Every mulmod as one signed Montgomery product plus one row-sum add: the
+    # ceiling of a kernel that multiplies each row sum in modular arithmetic.
+    roof["one_montgomery_per_mulmod_peak"] = (
+        1.0 / (1.0 / rates["mont31_product"] + 1.0 / rates["v_add_u32"]) / 1e12)
+    roof["instruction_rates"] = rates
     roof["kernel"] = (
         f"batch_glynn_kernel<{group}, {kset}> (n={HEADLINE_N}, {len(primes)} primes in "
         f"{int(st['jobs'])} walk(s) of the Gray code)"
@@ -347,6 +391,10 @@ def main() -> int:
         if serve is not None:
             result["serve"] = serve.benchmark(seconds=args.serve_seconds)
             log(f"serve: {result['serve']}")
+
+    if not args.no_published:
+        result["published_workloads"] = published_workloads()
+        log(f"published workloads: {result['published_workloads']}")
 
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

@@ -146,6 +146,13 @@ int phip_profile_read(double *kernel_ms, uint64_t *launches);
  * the count. */
 int phip_valu_microbench(double *lane_ops_per_s, int len);
 
+/* Issue rate of an instruction mix: each iteration of every lane issues
+ * rounds[t] rounds of 8 independent instructions of type t = v_mad_i64_i32,
+ * v_mul_lo_u32, v_pk_add_u16, SDWA v_mul_i32_i24, v_add_u32 (each in
+ * [0, 4096]).  *lane_iters_per_s receives lane-iterations per second over
+ * the whole device.  Returns 0 or -1. */
+int phip_mix_microbench(const int32_t *rounds, int iters, double *lane_iters_per_s);
+
 /* `reps` launches of a plain copy kernel over `bytes` bytes. */
 int phip_copy_calibration(uint64_t bytes, int reps);
 

@@ -39,7 +39,7 @@ struct BatchSlot {
     uint32_t job;
     uint32_t k;
     uint32_t p;
-    uint32_t scale;  // R^(groups-1) * 2^-(n-1) mod p
+    uint32_t scale;  // R^groups * 2^-(n-1) mod p
 };
 
 // One persistent launch: the jobs of one (row-sum group, set size) pair.

@@ -6,9 +6,10 @@
 // largest first, and grouped by row-sum representation G and set size K;
 // each (G, K) class runs as one persistent launch whose wavefronts take
 // items from a global counter until the list is exhausted.  An item's
-// wavefront sums (one per prime of the set) go to its own slots, and a
-// finalisation kernel folds each job's contiguous slots into residues, so
-// no atomics touch the arithmetic.
+// wavefront sums (one per prime of the set; signed 64-bit sums of the lanes'
+// Montgomery residues) go to its own slots, and a finalisation kernel folds
+// each job's contiguous slots into residues, so no atomics touch the
+// arithmetic.
 //
 // This file is compiled once per (G, K) class: PHIP_BATCH_GROUP and
 // PHIP_BATCH_SET select it.
@@ -30,10 +31,10 @@ namespace phip {
 namespace {
 
 // One wavefront's chunk of job `jb` for every prime of its set; returns the
-// lane residues in r.
+// lane values (signed Montgomery residues) in r.
 template <int N, int G, int K>
 __device__ __forceinline__ void run_item(const int32_t* __restrict__ tables, const BatchJob& jb, uint64_t base,
-                                         const int32_t (&p)[K], const uint32_t (&pinv)[K], uint64_t (&r)[K]) {
+                                         const int32_t (&p)[K], const uint32_t (&pinv)[K], int32_t (&r)[K]) {
     if constexpr (K == 1) {
         r[0] = glynn_chunk<N, G>(tables, N, jb.cbits, base, p[0], pinv[0]);
     } else {
@@ -72,7 +73,7 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
             p[k] = __builtin_amdgcn_readfirstlane(set_p[jb.prime_off + k]);
             pinv[k] = (uint32_t)__builtin_amdgcn_readfirstlane((int)set_pinv[jb.prime_off + k]);
         }
-        uint64_t r[K];
+        int32_t r[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) r[k] = 0;
         switch (jb.n) {
@@ -95,7 +96,8 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
         }
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const uint64_t s = wave_reduce_sum_u64(active ? r[k] : 0u);
+            // Signed lane values summed in two's complement: |sum| < 2^37.
+            const uint64_t s = wave_reduce_sum_u64(active ? (uint64_t)(int64_t)r[k] : 0u);
             if (lane_id == 0) item_partials[(size_t)idx * kMaxSet + k] = s;
         }
     }

@@ -38,9 +38,12 @@ __global__ __launch_bounds__(256) void batch_finalize_kernel(const BatchJob* __r
     const BatchSlot sl = slots[blockIdx.x];
     const BatchJob jb = jobs[sl.job];
     const uint64_t p = sl.p;
+    // Item sums are signed (two's complement) sums of lane residues.
     uint64_t s = 0;
-    for (uint32_t i = threadIdx.x; i < jb.n_items; i += 256)
-        s += item_partials[(size_t)(jb.first_item + i) * kMaxSet + sl.k] % p;
+    for (uint32_t i = threadIdx.x; i < jb.n_items; i += 256) {
+        const int64_t v = (int64_t)item_partials[(size_t)(jb.first_item + i) * kMaxSet + sl.k] % (int64_t)p;
+        s += (uint64_t)(v < 0 ? v + (int64_t)p : v);
+    }
     s = block_reduce_sum_u64<256>(s);
     if (threadIdx.x == 0) residues[blockIdx.x] = (int64_t)(((s % p) * (uint64_t)sl.scale) % p);
 }
@@ -175,7 +178,9 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
                     sl.job = job;
                     sl.k = (uint32_t)k;
                     sl.p = q;
-                    sl.scale = (uint32_t)host_mulmod(host_powmod((1ull << 32) % q, (uint64_t)(groups - 1), q),
+                    // The tree leaves R^-(groups-1) and the lane reduction one
+                    // more R^-1.
+                    sl.scale = (uint32_t)host_mulmod(host_powmod((1ull << 32) % q, (uint64_t)groups, q),
                                                      host_powmod((q + 1) / 2, (uint64_t)L, q), q);
                     out.slots[idx] = sl;
                     out.total_states += std::ldexp(1.0, L);

@@ -70,11 +70,14 @@ __device__ __forceinline__ int32_t mont31(int32_t a, int32_t b, int32_t negp, ui
     return (int32_t)(u >> 32);
 }
 
-// Hides a pointer from loop-invariant code motion so each flip issues its
-// own scalar loads instead of pinning every column in SGPRs.
-__device__ __forceinline__ const int32_t* opaque_ptr(const int32_t* p) {
+// Column pointer for the flips: the constant address space lets the
+// compiler read the wave-uniform deltas with scalar loads into SGPRs, and
+// the empty asm hides the pointer from loop-invariant code motion so each
+// flip issues its own loads instead of pinning every column in registers.
+typedef const __attribute__((address_space(4))) int32_t* ConstCol;
+__device__ __forceinline__ ConstCol opaque_ptr(const int32_t* p) {
     asm volatile("" : "+s"(p));
-    return p;
+    return (ConstCol)p;
 }
 
 typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
@@ -137,12 +140,12 @@ struct WideRows {
     }
     template <bool SUB>
     __device__ __forceinline__ void flip_mem(const int32_t* __restrict__ col, int32_t p) {
-        const int32_t* c = opaque_ptr(col);
+        const ConstCol c = opaque_ptr(col);
 #pragma unroll
         for (int i = 0; i < NR; ++i) r[i] = SUB ? sub(r[i], c[i], p) : add(r[i], c[i], p);
     }
     __device__ __forceinline__ void flip_lane(const int32_t* __restrict__ col, bool sub_dir, int32_t p) {
-        const int32_t* c = opaque_ptr(col);
+        const ConstCol c = opaque_ptr(col);
 #pragma unroll
         for (int i = 0; i < NR; ++i) {
             const int32_t d = c[i];
@@ -186,7 +189,7 @@ struct PackedRows {
     }
     template <bool SUB>
     __device__ __forceinline__ void flip_mem(const int32_t* __restrict__ col, int32_t) {
-        const int32_t* c = opaque_ptr(col);
+        const ConstCol c = opaque_ptr(col);
 #pragma unroll
         for (int k = 0; k < kStride; ++k) {
             const u16x2 d = as_u16x2((uint32_t)c[k]);
@@ -194,7 +197,7 @@ struct PackedRows {
         }
     }
     __device__ __forceinline__ void flip_lane(const int32_t* __restrict__ col, bool sub_dir, int32_t) {
-        const int32_t* c = opaque_ptr(col);
+        const ConstCol c = opaque_ptr(col);
 #pragma unroll
         for (int k = 0; k < kStride; ++k) {
             const u16x2 d = as_u16x2((uint32_t)c[k]);
@@ -260,25 +263,38 @@ struct PackedRows {
 template <int NR, int G>
 using GlynnRows = typename std::conditional<G == 0, WideRows<NR>, PackedRows<NR, G>>::type;
 
+// acc + S * v (S = +1 or -1) as one v_mad_i64_i32 with an inline constant.
+// Written as asm so the accumulation stays in state order: left to itself
+// the compiler reassociates the unrolled states' additions into a tree and
+// keeps every term of every prime live at once, which exhausts the register
+// file.
+template <int S>
+__device__ __forceinline__ int64_t mac_sign(int64_t acc, int32_t v) {
+    int64_t r;
+    uint64_t carry;
+    if constexpr (S > 0) {
+        asm("v_mad_i64_i32 %0, %1, %2, 1, %3" : "=v"(r), "=s"(carry) : "v"(v), "v"(acc));
+    } else {
+        asm("v_mad_i64_i32 %0, %1, %2, -1, %3" : "=v"(r), "=s"(carry) : "v"(v), "v"(acc));
+    }
+    return r;
+}
+
 // Per-state consumers of the Gray walk.  state<ODD>(rows) receives the row
 // sums of every state of the chunk; ODD marks odd local indices, whose
 // terms carry the opposite sign of the chunk's first state.
 //
 // One prime: the product of the row sums through the balanced Montgomery
-// tree, summed separately for even and odd local indices.
+// tree, added for even and subtracted for odd local indices.
 template <int NR, int G>
 struct OnePrimeSink {
     using Rows = GlynnRows<NR, G>;
     int32_t negp;
     uint32_t pinv;
-    int64_t acc_e = 0, acc_o = 0;
+    int64_t acc = 0;
     template <bool ODD>
     __device__ __forceinline__ void state(const Rows& rs) {
-        if (ODD) {
-            acc_o += rs.term(negp, pinv);
-        } else {
-            acc_e += rs.term(negp, pinv);
-        }
+        acc = mac_sign<ODD ? -1 : 1>(acc, rs.term(negp, pinv));
     }
 };
 
@@ -326,14 +342,7 @@ struct PrimeSetSink {
 #pragma unroll
                 for (int i = 0; i + w < kLeaves; i += 2 * w) v[i] = mont31(v[i], v[i + w], negp[k], pinv[k]);
             }
-            if (ODD) {
-                acc[k] -= v[0];
-            } else {
-                acc[k] += v[0];
-            }
-            // Keeps the scheduler from interleaving the trees of all primes,
-            // whose live leaves would otherwise exhaust the register file.
-            __builtin_amdgcn_sched_barrier(0);
+            acc[k] = mac_sign<ODD ? -1 : 1>(acc[k], v[0]);
         }
     }
 };
@@ -411,31 +420,31 @@ __device__ __forceinline__ bool gray_walk(const int32_t* __restrict__ tables, in
     return __popcll(gray) & 1;
 }
 
-// Signed lane sum reduced into [0, p).
-__device__ __forceinline__ uint64_t lane_residue(int64_t acc, bool negate, int32_t p) {
-    if (negate) acc = -acc;
-    int64_t r = acc % (int64_t)p;
-    if (r < 0) r += p;
-    return (uint64_t)r;
+// The lane's signed sum of terms as one Montgomery residue: |acc| <
+// 2^(31+cbits) <= 2^61, so redc31 applies and the result is congruent to
+// acc * R^-1 with |result| < 2^31 (no 64-bit division in the epilogue).
+__device__ __forceinline__ int32_t lane_value(int64_t acc, bool negate, int32_t negp, uint32_t pinv) {
+    const int32_t r = redc31(acc, negp, pinv);
+    return negate ? -r : r;
 }
 
-// One prime: the lane's signed sum of terms reduced into [0, p); it still
-// carries R^-(groups-1) and lacks 2^-(n-1).
+// One prime: the lane's value, congruent to its signed sum of terms times
+// R^-groups; the finalisation multiplies by R^groups 2^-(n-1).
 template <int NR, int G>
-__device__ __forceinline__ uint64_t glynn_chunk(const int32_t* __restrict__ tables, int n, int cbits,
-                                                uint64_t base, int32_t p, uint32_t pinv) {
+__device__ __forceinline__ int32_t glynn_chunk(const int32_t* __restrict__ tables, int n, int cbits, uint64_t base,
+                                               int32_t p, uint32_t pinv) {
     OnePrimeSink<NR, G> sink;
     sink.negp = -p;
     sink.pinv = pinv;
     const bool neg = gray_walk<NR, G>(tables, n, cbits, base, p, sink);
-    return lane_residue(sink.acc_e - sink.acc_o, neg, p);
+    return lane_value(sink.acc, neg, -p, pinv);
 }
 
 // K primes sharing one packed table: out[k] as glynn_chunk for primes[k].
 template <int NR, int G, int K>
 __device__ __forceinline__ void glynn_chunk_set(const int32_t* __restrict__ tables, int n, int cbits,
                                                 uint64_t base, const int32_t (&primes)[K],
-                                                const uint32_t (&pinvs)[K], uint64_t (&out)[K]) {
+                                                const uint32_t (&pinvs)[K], int32_t (&out)[K]) {
     PrimeSetSink<NR, G, K> sink;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -445,7 +454,7 @@ __device__ __forceinline__ void glynn_chunk_set(const int32_t* __restrict__ tabl
     }
     const bool neg = gray_walk<NR, G>(tables, n, cbits, base, 0, sink);
 #pragma unroll
-    for (int k = 0; k < K; ++k) out[k] = lane_residue(sink.acc[k], neg, primes[k]);
+    for (int k = 0; k < K; ++k) out[k] = lane_value(sink.acc[k], neg, sink.negp[k], sink.pinv[k]);
 }
 
 }  // namespace phip

@@ -9,6 +9,9 @@
 //                         each as lane-operations per
 //                         second over the whole device.  Eight independent
 //                         dependency chains per lane keep the pipeline full.
+//   phip_mix_microbench   issue rate of a whole instruction mix (rounds of 8
+//                         independent instructions per type and iteration),
+//                         the roofline of a kernel's own instruction stream.
 //   phip_copy_calibration a plain device-to-device copy kernel of a known
 //                         size, to calibrate HBM byte counters.
 #include <hip/hip_runtime.h>
@@ -86,6 +89,55 @@ __global__ __launch_bounds__(256) void valu_rate_kernel(uint32_t* out, int iters
     out[blockIdx.x * blockDim.x + threadIdx.x] = acc;
 }
 
+// Issue bound of an instruction mix: every iteration issues rounds[t]
+// rounds of 8 independent instructions of type t (v_mad_i64_i32,
+// v_mul_lo_u32, v_pk_add_u16, SDWA v_mul_i32_i24, v_add_u32), each type on
+// its own chains so nothing but issue limits the rate.
+__global__ __launch_bounds__(256) void mix_kernel(uint32_t* out, int iters, int r_mad, int r_mul, int r_pk,
+                                                  int r_m24, int r_add, uint32_t c) {
+    uint64_t y[kChains];
+    uint32_t a[kChains], b[kChains], d[kChains], e[kChains];
+#pragma unroll
+    for (int k = 0; k < kChains; ++k) {
+        y[k] = threadIdx.x + k;
+        a[k] = b[k] = d[k] = e[k] = threadIdx.x * 2654435761u + k + c;
+    }
+    for (int it = 0; it < iters; ++it) {
+        for (int r = 0; r < r_mad; ++r) {
+#pragma unroll
+            for (int k = 0; k < kChains; ++k) {
+                uint64_t carry;
+                asm volatile("v_mad_i64_i32 %0, %1, %2, %3, %0" : "+v"(y[k]), "=s"(carry) : "v"(c + k), "v"(c));
+            }
+        }
+        for (int r = 0; r < r_mul; ++r) {
+#pragma unroll
+            for (int k = 0; k < kChains; ++k) asm volatile("v_mul_lo_u32 %0, %0, %1" : "+v"(a[k]) : "v"(c));
+        }
+        for (int r = 0; r < r_pk; ++r) {
+#pragma unroll
+            for (int k = 0; k < kChains; ++k) asm volatile("v_pk_add_u16 %0, %0, %1" : "+v"(b[k]) : "v"(c));
+        }
+        for (int r = 0; r < r_m24; ++r) {
+#pragma unroll
+            for (int k = 0; k < kChains; ++k)
+                asm volatile(
+                    "v_mul_i32_i24_sdwa %0, sext(%0), sext(%1) dst_sel:DWORD dst_unused:UNUSED_PAD "
+                    "src0_sel:WORD_1 src1_sel:WORD_0"
+                    : "+v"(d[k])
+                    : "v"(c));
+        }
+        for (int r = 0; r < r_add; ++r) {
+#pragma unroll
+            for (int k = 0; k < kChains; ++k) asm volatile("v_add_u32 %0, %0, %1" : "+v"(e[k]) : "v"(c));
+        }
+    }
+    uint32_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < kChains; ++k) acc ^= (uint32_t)y[k] ^ (uint32_t)(y[k] >> 32) ^ a[k] ^ b[k] ^ d[k] ^ e[k];
+    out[blockIdx.x * blockDim.x + threadIdx.x] = acc;
+}
+
 __global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
                                                    uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
@@ -136,6 +188,45 @@ int phip_valu_microbench(double* lane_ops_per_s, int len) {
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     return written;
+}
+
+int phip_mix_microbench(const int32_t* rounds, int iters, double* lane_iters_per_s) {
+    if (!rounds || !lane_iters_per_s || iters < 1 || iters > (1 << 20)) {
+        phip::set_error("invalid mix microbenchmark arguments");
+        return -1;
+    }
+    for (int t = 0; t < 5; ++t) {
+        if (rounds[t] < 0 || rounds[t] > 4096) {
+            phip::set_error("mix rounds must lie in [0, 4096]");
+            return -1;
+        }
+    }
+    phip::DeviceContext* ctx = phip::DeviceContext::current();
+    if (!ctx) return -1;
+    std::lock_guard<std::mutex> lock(ctx->mutex());
+    const int cus = ctx->compute_units() > 0 ? ctx->compute_units() : 256;
+    const int blocks = cus * 8;  // 32 waves per CU
+    hipStream_t st = ctx->stream();
+    auto* out = (uint32_t*)ctx->buffer(0, (size_t)blocks * 256 * 4);
+    if (!out) return -1;
+    hipEvent_t e0, e1;
+    PHIP_HIP_TRY(hipEventCreate(&e0));
+    PHIP_HIP_TRY(hipEventCreate(&e1));
+    hipLaunchKernelGGL(phip::mix_kernel, dim3(blocks), dim3(256), 0, st, out, 1, rounds[0], rounds[1], rounds[2],
+                       rounds[3], rounds[4], 12345u);
+    PHIP_HIP_TRY(hipGetLastError());
+    PHIP_HIP_TRY(hipEventRecord(e0, st));
+    hipLaunchKernelGGL(phip::mix_kernel, dim3(blocks), dim3(256), 0, st, out, iters, rounds[0], rounds[1],
+                       rounds[2], rounds[3], rounds[4], 12345u);
+    PHIP_HIP_TRY(hipGetLastError());
+    PHIP_HIP_TRY(hipEventRecord(e1, st));
+    PHIP_HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0;
+    PHIP_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *lane_iters_per_s = (double)blocks * 256 * iters / (ms * 1e-3);
+    return 0;
 }
 
 int phip_copy_calibration(uint64_t bytes, int reps) {

@@ -136,11 +136,16 @@ def binary_permanent(matrix, *, backend: str = "auto", parallel: bool = True,
     return result
 
 
+# Private helper names of the reference module (perm_binary.py:173-227,
+# 538-814), used by perm_binary_cuda as they are in the reference.
 _crt_primes_above_bound = _prep.crt_primes_above_bound
 _is_prime = _prep.is_prime_u64
 _matching_blocks = _prep.matching_blocks
+_normalize_binary_matrix = _prep.normalize_binary_matrix
 _normalized_bound = _prep.normalized_bound
 _row_masks = _prep.row_masks
+_unpack_masks = _prep.unpack_masks
+_validate_residue_prime = _prep.validate_residue_prime
 
 __all__ = [
     "HAS_BINARY_EXT",

@@ -61,6 +61,7 @@ _SIGNATURES = {
     "phip_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "phip_profile_read": (ctypes.c_int, [_F64P, _U64P]),
     "phip_valu_microbench": (ctypes.c_int, [_F64P, ctypes.c_int]),
+    "phip_mix_microbench": (ctypes.c_int, [_I32P, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "phip_copy_calibration": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int]),
 }
 
@@ -104,7 +105,12 @@ def load(path: Path | None = None):
             )
         lib = ctypes.CDLL(str(target))
         for name, (restype, argtypes) in _SIGNATURES.items():
-            fn = getattr(lib, name)
+            try:
+                fn = getattr(lib, name)
+            except AttributeError as exc:
+                raise RuntimeError(
+                    f"{target} lacks {name}: the library is outdated; rebuild it with `python -m permhip.build`"
+                ) from exc
             fn.restype = restype
             fn.argtypes = argtypes
         if path is None:
@@ -234,6 +240,18 @@ def valu_microbench() -> dict:
     count = require_device().phip_valu_microbench(out, len(out))
     _check(count, "phip_valu_microbench")
     return {name: out[i] for i, name in enumerate(MICROBENCH_NAMES[:count])}
+
+
+MIX_TYPES = ("v_mad_i64_i32", "v_mul_lo_u32", "v_pk_add_u16", "v_mul_i32_i24_sdwa", "v_add_u32")
+
+
+def mix_microbench(rounds, iters: int = 64) -> float:
+    """Lane-iterations per second of a synthetic kernel issuing rounds[t]
+    rounds of 8 independent instructions of each type in MIX_TYPES."""
+    arr = (ctypes.c_int32 * 5)(*[int(r) for r in rounds])
+    out = ctypes.c_double()
+    _check(require_device().phip_mix_microbench(arr, int(iters), ctypes.byref(out)), "phip_mix_microbench")
+    return out.value
 
 
 def copy_calibration(nbytes: int, reps: int = 1) -> None:

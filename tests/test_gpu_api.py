@@ -31,6 +31,15 @@ def test_schur_permanents_match_oeis(native, n):
     assert perm.schur_permanent(n) == OEIS_A003112[n]
 
 
+def test_reference_schur_residue_order_33(native):
+    """Known answer of the reference's device tests
+    (tests/test_cuda_device_schur_glynn.py:141-146).  The prime is above 2^31,
+    so the 32 x 32 cofactor minor runs on the 64-bit Montgomery kernel."""
+    p, omega = 4_294_967_161, 2_662_262
+    assert pow(omega, 33, p) == 1
+    assert native.schur_permanent_modp(33, omega, p) == 1_428_739_342
+
+
 def test_even_and_trivial_schur_orders(native):
     import perm
 
@@ -211,3 +220,54 @@ def test_served_results_equal_alone(native):
     server.close()
     for (kind, m), v in zip(reqs, mixed):
         assert v == oracle.subset_permanent(m.tolist()), kind
+
+
+# --- the reference's GPU-side modules (perm_cuda, perm_binary_cuda) ----------
+
+SCHUR_KNOWN = {3: -3, 5: -5, 7: -105, 9: 81, 11: 6765, 13: 175747, 15: 30375, 17: 25219857,
+               19: 142901109, 21: 4548104883, 33: 868088125376401545}
+
+
+@pytest.mark.parametrize("n", sorted(SCHUR_KNOWN))
+def test_perm_cuda_schur_permanents(native, n):
+    """Device CRT plan (descending 32-bit primes) end to end against
+    OEIS A003112 and the reference's checked-in n = 33 value."""
+    import perm_cuda
+
+    assert perm_cuda.is_cuda_available()
+    assert perm_cuda.permanent_cuda(n, verbose=False) == SCHUR_KNOWN[n]
+
+
+def test_perm_cuda_residue_functions(native, oracle_c):
+    import perm_cuda
+
+    rng = np.random.default_rng(99)
+    for n in (2, 3, 7, 12, 16):
+        A = rng.integers(-(10 ** 12), 10 ** 12, (n, n))
+        for p in (101, 1073741827, 4294967291, (1 << 61) - 1):
+            assert perm_cuda.permanent_modp_cuda(A, p) == oracle_c.permanent_mod_p(A.tolist(), p, 8), (n, p)
+        assert perm_cuda.permanent_mod3_cuda(A) == oracle_c.mod3(A, 8), n
+    for n, p in ((5, 11), (9, 19), (15, 31)):
+        from permhip import planner
+
+        w = planner.nth_root_of_unity_mod_p(n, p)
+        S = planner.build_schur_matrix_mod_p(n, w, p)
+        assert perm_cuda.schur_permanent_modp_cuda(n, w, p) == oracle_c.permanent_mod_p(S, p, 8)
+
+
+def test_perm_binary_cuda_counts(native):
+    import perm_binary_cuda
+    from permhip import grids
+
+    assert perm_binary_cuda.is_binary_cuda_available()
+    assert perm_binary_cuda.binary_permanent_cuda(grids.grid_biadjacency((4, 6))) == 281
+    assert perm_binary_cuda.binary_permanent_cuda(grids.grid_biadjacency((2, 3, 4))) == 1845
+    assert perm_binary_cuda.binary_permanent_cuda(
+        grids.grid_biadjacency((4, 4, 2), periodic_axes=(0, 1))) == 589185
+    assert perm_binary_cuda.binary_permanent_cuda([[1] * 13 for _ in range(13)]) == math.factorial(13)
+    rng = random.Random(2718)
+    for n in range(2, 14):
+        B = [[rng.randrange(2) for _ in range(n)] for _ in range(n)]
+        d = oracle.subset_permanent(B)
+        assert perm_binary_cuda.binary_permanent_cuda(B) == d
+        assert perm_binary_cuda.binary_permanent_mod_p_cuda(B, 2147483629) == d % 2147483629
