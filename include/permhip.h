@@ -148,9 +148,12 @@ int phip_valu_microbench(double *lane_ops_per_s, int len);
 
 /* Issue rate of an instruction mix: each iteration of every lane issues
  * rounds[t] rounds of 8 independent instructions of type t = v_mad_i64_i32,
- * v_mul_lo_u32, v_pk_add_u16, SDWA v_mul_i32_i24, v_add_u32 (each in
- * [0, 4096]).  *lane_iters_per_s receives lane-iterations per second over
- * the whole device.  Returns 0 or -1. */
+ * v_mul_lo_u32, v_pk_add_u16, SDWA v_mul_i32_i24, v_add_u32, interleaved
+ * evenly.  Only the mixes of compiled Glynn classes are available (per 8
+ * Gray states of n = 32, groups of 4: {92, 56, 18, 24, 4} for a set of 8
+ * primes, {15, 7, 18, 24, 4} for one prime); others return -1.
+ * *lane_iters_per_s receives lane-iterations per second over the whole
+ * device.  Returns 0 or -1. */
 int phip_mix_microbench(const int32_t *rounds, int iters, double *lane_iters_per_s);
 
 /* `reps` launches of a plain copy kernel over `bytes` bytes. */

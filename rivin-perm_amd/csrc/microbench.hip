@@ -10,12 +10,14 @@
 //                         second over the whole device.  Eight independent
 //                         dependency chains per lane keep the pipeline full.
 //   phip_mix_microbench   issue rate of a whole instruction mix (rounds of 8
-//                         independent instructions per type and iteration),
-//                         the roofline of a kernel's own instruction stream.
+//                         independent instructions per type and iteration,
+//                         interleaved evenly), the roofline of a Glynn
+//                         kernel's own instruction stream.
 //   phip_copy_calibration a plain device-to-device copy kernel of a known
 //                         size, to calibrate HBM byte counters.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
 
 #include <string>
 
@@ -89,54 +91,96 @@ __global__ __launch_bounds__(256) void valu_rate_kernel(uint32_t* out, int iters
     out[blockIdx.x * blockDim.x + threadIdx.x] = acc;
 }
 
-// Issue bound of an instruction mix: every iteration issues rounds[t]
-// rounds of 8 independent instructions of type t (v_mad_i64_i32,
-// v_mul_lo_u32, v_pk_add_u16, SDWA v_mul_i32_i24, v_add_u32), each type on
-// its own chains so nothing but issue limits the rate.
-__global__ __launch_bounds__(256) void mix_kernel(uint32_t* out, int iters, int r_mad, int r_mul, int r_pk,
-                                                  int r_m24, int r_add, uint32_t c) {
-    uint64_t y[kChains];
-    uint32_t a[kChains], b[kChains], d[kChains], e[kChains];
+// Issue bound of an instruction mix: every iteration issues C_t rounds of
+// 8 independent instructions of type t (v_mad_i64_i32, v_mul_lo_u32,
+// v_pk_add_u16, SDWA v_mul_i32_i24, v_add_u32), each type on its own
+// chains with register operands, interleaved evenly over R = max C_t steps
+// (type t issues in step r when floor((r+1) C_t / R) > floor(r C_t / R)),
+// as the instructions of one Gray state interleave in the Glynn kernels.
+// The interleaving is fixed at compile time, so only the mixes of the
+// compiled Glynn classes the benchmark reports are instantiated.
+typedef uint16_t mb_u16x2 __attribute__((ext_vector_type(2)));
+
+struct MixRegs {
+    int64_t y[kChains];
+    int32_t x[kChains];
+    uint32_t a[4][kChains];
+};
+
+// One round of type T on 8 chains.  Plain arithmetic lets the compiler emit
+// the instructions (and none of the hazard padding it adds around inline
+// asm); `c` is re-hidden every step, so nothing folds.
+template <int T>
+__device__ __forceinline__ void mix_issue(MixRegs& m, uint32_t c) {
 #pragma unroll
     for (int k = 0; k < kChains; ++k) {
-        y[k] = threadIdx.x + k;
-        a[k] = b[k] = d[k] = e[k] = threadIdx.x * 2654435761u + k + c;
-    }
-    for (int it = 0; it < iters; ++it) {
-        for (int r = 0; r < r_mad; ++r) {
-#pragma unroll
-            for (int k = 0; k < kChains; ++k) {
-                uint64_t carry;
-                asm volatile("v_mad_i64_i32 %0, %1, %2, %3, %0" : "+v"(y[k]), "=s"(carry) : "v"(c + k), "v"(c));
-            }
-        }
-        for (int r = 0; r < r_mul; ++r) {
-#pragma unroll
-            for (int k = 0; k < kChains; ++k) asm volatile("v_mul_lo_u32 %0, %0, %1" : "+v"(a[k]) : "v"(c));
-        }
-        for (int r = 0; r < r_pk; ++r) {
-#pragma unroll
-            for (int k = 0; k < kChains; ++k) asm volatile("v_pk_add_u16 %0, %0, %1" : "+v"(b[k]) : "v"(c));
-        }
-        for (int r = 0; r < r_m24; ++r) {
-#pragma unroll
-            for (int k = 0; k < kChains; ++k)
-                asm volatile(
-                    "v_mul_i32_i24_sdwa %0, sext(%0), sext(%1) dst_sel:DWORD dst_unused:UNUSED_PAD "
-                    "src0_sel:WORD_1 src1_sel:WORD_0"
-                    : "+v"(d[k])
-                    : "v"(c));
-        }
-        for (int r = 0; r < r_add; ++r) {
-#pragma unroll
-            for (int k = 0; k < kChains; ++k) asm volatile("v_add_u32 %0, %0, %1" : "+v"(e[k]) : "v"(c));
+        if constexpr (T == 0) {
+            m.y[k] += (int64_t)m.x[k] * (int64_t)(int32_t)c;
+        } else if constexpr (T == 1) {
+            m.a[0][k] *= c;
+        } else if constexpr (T == 2) {
+            m.a[1][k] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(mb_u16x2, m.a[1][k]) +
+                                                         __builtin_bit_cast(mb_u16x2, c));
+        } else if constexpr (T == 3) {
+            asm volatile(
+                "v_mul_i32_i24_sdwa %0, sext(%0), sext(%1) dst_sel:DWORD dst_unused:UNUSED_PAD "
+                "src0_sel:WORD_1 src1_sel:WORD_0"
+                : "+v"(m.a[2][k])
+                : "v"(c));
+        } else {
+            m.a[3][k] += c;
         }
     }
+}
+
+constexpr bool mix_step(int count, int R, int r) { return (r + 1) * count / R > r * count / R; }
+
+template <int C0, int C1, int C2, int C3, int C4, int R, int r>
+__device__ __forceinline__ void mix_rounds(MixRegs& m, uint32_t& c) {
+    if constexpr (r < R) {
+        asm volatile("" : "+v"(c));
+        if constexpr (mix_step(C0, R, r)) mix_issue<0>(m, c);
+        if constexpr (mix_step(C1, R, r)) mix_issue<1>(m, c);
+        if constexpr (mix_step(C2, R, r)) mix_issue<2>(m, c);
+        if constexpr (mix_step(C3, R, r)) mix_issue<3>(m, c);
+        if constexpr (mix_step(C4, R, r)) mix_issue<4>(m, c);
+        mix_rounds<C0, C1, C2, C3, C4, R, r + 1>(m, c);
+    }
+}
+
+constexpr int mix_max(int a, int b) { return a > b ? a : b; }
+
+template <int C0, int C1, int C2, int C3, int C4>
+__global__ __launch_bounds__(256) void mix_kernel(uint32_t* out, int iters, uint32_t c0) {
+    constexpr int R = mix_max(mix_max(mix_max(C0, C1), mix_max(C2, C3)), C4);
+    MixRegs m;
+#pragma unroll
+    for (int k = 0; k < kChains; ++k) {
+        m.y[k] = threadIdx.x + k;
+        m.x[k] = (int32_t)(threadIdx.x * 2654435761u + k + c0);
+        for (int t = 0; t < 4; ++t) m.a[t][k] = (uint32_t)m.x[k] + t;
+    }
+    uint32_t c = c0 + threadIdx.x;
+    for (int it = 0; it < iters; ++it) mix_rounds<C0, C1, C2, C3, C4, R, 0>(m, c);
     uint32_t acc = 0;
 #pragma unroll
-    for (int k = 0; k < kChains; ++k) acc ^= (uint32_t)y[k] ^ (uint32_t)(y[k] >> 32) ^ a[k] ^ b[k] ^ d[k] ^ e[k];
+    for (int k = 0; k < kChains; ++k)
+        acc ^= (uint32_t)m.y[k] ^ (uint32_t)((uint64_t)m.y[k] >> 32) ^ m.a[0][k] ^ m.a[1][k] ^ m.a[2][k] ^ m.a[3][k];
     out[blockIdx.x * blockDim.x + threadIdx.x] = acc;
 }
+
+using MixKernel = void (*)(uint32_t*, int, uint32_t);
+
+// Mixes of the compiled Glynn classes, per Gray state of n = 32 with groups
+// of 4: the 8-prime set and the single prime.
+struct MixEntry {
+    int32_t rounds[5];
+    MixKernel kernel;
+};
+const MixEntry kMixes[] = {
+    {{92, 56, 18, 24, 4}, mix_kernel<92, 56, 18, 24, 4>},
+    {{15, 7, 18, 24, 4}, mix_kernel<15, 7, 18, 24, 4>},
+};
 
 __global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
                                                    uint64_t n) {
@@ -195,11 +239,13 @@ int phip_mix_microbench(const int32_t* rounds, int iters, double* lane_iters_per
         phip::set_error("invalid mix microbenchmark arguments");
         return -1;
     }
-    for (int t = 0; t < 5; ++t) {
-        if (rounds[t] < 0 || rounds[t] > 4096) {
-            phip::set_error("mix rounds must lie in [0, 4096]");
-            return -1;
-        }
+    phip::MixKernel kernel = nullptr;
+    for (const auto& e : phip::kMixes) {
+        if (memcmp(e.rounds, rounds, sizeof(e.rounds)) == 0) kernel = e.kernel;
+    }
+    if (!kernel) {
+        phip::set_error("no compiled instruction mix matches these rounds");
+        return -1;
     }
     phip::DeviceContext* ctx = phip::DeviceContext::current();
     if (!ctx) return -1;
@@ -212,19 +258,17 @@ int phip_mix_microbench(const int32_t* rounds, int iters, double* lane_iters_per
     hipEvent_t e0, e1;
     PHIP_HIP_TRY(hipEventCreate(&e0));
     PHIP_HIP_TRY(hipEventCreate(&e1));
-    hipLaunchKernelGGL(phip::mix_kernel, dim3(blocks), dim3(256), 0, st, out, 1, rounds[0], rounds[1], rounds[2],
-                       rounds[3], rounds[4], 12345u);
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(256), 0, st, out, 1, 12345u);
     PHIP_HIP_TRY(hipGetLastError());
     PHIP_HIP_TRY(hipEventRecord(e0, st));
-    hipLaunchKernelGGL(phip::mix_kernel, dim3(blocks), dim3(256), 0, st, out, iters, rounds[0], rounds[1],
-                       rounds[2], rounds[3], rounds[4], 12345u);
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(256), 0, st, out, iters, 12345u);
     PHIP_HIP_TRY(hipGetLastError());
     PHIP_HIP_TRY(hipEventRecord(e1, st));
     PHIP_HIP_TRY(hipEventSynchronize(e1));
     float ms = 0;
     PHIP_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     *lane_iters_per_s = (double)blocks * 256 * iters / (ms * 1e-3);
     return 0;
 }
