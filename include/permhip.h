@@ -59,8 +59,11 @@ int64_t phip_permanent_modp(const int64_t *A, int n, int64_t p);
 /* perm(S_n) mod p with S_n[j][k] = omega^(jk); n == 0 or 1 gives 1 mod p.
  * Odd n >= 3, odd p and omega^n == 1 use the cofactor identity
  * per(S_n) = n * per(minor_(n-1)) (perm_c_omp.c:971-975); otherwise the
- * full matrix is used.  Replaces schur_permanent_modp_cuda
- * (perm_modp_cuda.cu:1266) and perm_c_ext.schur_permanent_mod_p. */
+ * full matrix is used.  For a prime order n in [necklace min order, 47] and
+ * an odd prime p < 2^31 the minor is evaluated by necklace orbits of its
+ * circulant form (perm_c_omp.c:79-507), else by Glynn.  Replaces
+ * schur_permanent_modp_cuda (perm_modp_cuda.cu:1266) and
+ * perm_c_ext.schur_permanent_mod_p. */
 int64_t phip_schur_permanent_modp(int n, int64_t omega, int64_t p);
 
 /* Permanent of a 0/1 matrix (row-major bytes) modulo k odd primes p with
@@ -130,6 +133,8 @@ int phip_batch_destroy(void *plan);
 #define PHIP_TUNE_MIN_CBITS 2        /* single-matrix launch: minimum chunk bits */
 #define PHIP_TUNE_BATCH_ITEMS_LOG2 3 /* batch: target work items (log2), read at create */
 #define PHIP_TUNE_BATCH_BLOCKS 4     /* batch: default persistent workgroups per CU */
+#define PHIP_TUNE_NECKLACE_MIN_ORDER 5 /* Schur: smallest prime order using necklace orbits (5..64) */
+#define PHIP_TUNE_NECKLACE_DEPTH 6   /* Schur necklaces: largest FKM prefix depth of the tasks (2..30) */
 int phip_set_tuning(int key, int64_t value);
 int64_t phip_get_tuning(int key);
 

@@ -10,6 +10,7 @@
 #include "device_context.h"
 #include "permhip.h"
 #include "phip_arith.h"
+#include "necklace.h"
 #include "residues.h"
 
 namespace phip {
@@ -119,6 +120,14 @@ int phip_set_tuning(int key, int64_t value) {
             if (value < 1 || value > 64) break;
             t.batch_blocks_per_cu = (int)value;
             return 0;
+        case PHIP_TUNE_NECKLACE_MIN_ORDER:
+            if (value < 5 || value > 64) break;
+            t.necklace_min_order = (int)value;
+            return 0;
+        case PHIP_TUNE_NECKLACE_DEPTH:
+            if (value < 2 || value > 30) break;
+            t.necklace_depth = (int)value;
+            return 0;
         default:
             phip::set_error("unknown tuning key");
             return -1;
@@ -134,6 +143,8 @@ int64_t phip_get_tuning(int key) {
         case PHIP_TUNE_MIN_CBITS: return t.min_cbits;
         case PHIP_TUNE_BATCH_ITEMS_LOG2: return t.batch_items_log2;
         case PHIP_TUNE_BATCH_BLOCKS: return t.batch_blocks_per_cu;
+        case PHIP_TUNE_NECKLACE_MIN_ORDER: return t.necklace_min_order;
+        case PHIP_TUNE_NECKLACE_DEPTH: return t.necklace_depth;
         default:
             phip::set_error("unknown tuning key");
             return -1;
@@ -193,6 +204,15 @@ int64_t phip_schur_permanent_modp(int n, int64_t omega, int64_t p) {
     // Cofactor identity for odd n when omega^n == 1 (perm_c_omp.c:964-1025):
     // every permanental cofactor of the all-ones first row is equal.
     const bool cofactor = n >= 3 && (n & 1) && (p & 1) && phip::host_powmod(w, (uint64_t)n, P) == 1 % P;
+    if (cofactor && n >= phip::tuning().necklace_min_order && phip::necklace_supported(n, P) &&
+        phip::is_prime_u64(P)) {
+        DeviceContext* ctx = DeviceContext::current();
+        if (!ctx) return -1;
+        std::lock_guard<std::mutex> lock(ctx->mutex());
+        int64_t minor = -1;
+        if (phip::necklace_minor_modp(ctx, n, w, P, &minor) != 0) return -1;
+        return (int64_t)phip::host_mulmod((uint64_t)minor, (uint64_t)n % P, P);
+    }
     const int m = cofactor ? n - 1 : n;
     const uint64_t first = cofactor ? w : 1 % P;  // omega^(row index) of the first row used
     std::vector<int64_t> M((size_t)m * m);

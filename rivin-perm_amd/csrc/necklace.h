@@ -1,0 +1,19 @@
+// Necklace-orbit evaluation of Schur cofactor minors (necklace.hip).
+#pragma once
+#include <stdint.h>
+
+#include "device_context.h"
+
+namespace phip {
+
+// Largest Schur order with a compiled necklace kernel (minor order 46).
+constexpr int kNecklaceMaxOrder = 47;
+
+// True for a prime order 5 <= n <= kNecklaceMaxOrder and an odd p < 2^31.
+bool necklace_supported(int n, uint64_t p);
+
+// perm of the (0,0) minor of S_n (entries omega^(jk), 1 <= j, k < n) mod p,
+// for omega^n == 1 mod p, by necklace orbits.  Returns 0 or -1.
+int necklace_minor_modp(DeviceContext* ctx, int n, uint64_t omega, uint64_t p, int64_t* minor_out);
+
+}  // namespace phip

@@ -8,13 +8,17 @@
 
 namespace phip {
 
-// Tunables (set through phip_set_tuning): target number of lanes per launch
-// for the register-resident Glynn kernel; minimum chunk bits.
+// Tunables (set through phip_set_tuning): target number of lanes per
+// single-matrix launch; minimum chunk bits; batch item count and workgroups
+// per CU; the smallest prime Schur order and the task depth of the necklace
+// kernel.
 struct Tuning {
     int lanes_log2 = 20;
     int min_cbits = 4;
     int batch_items_log2 = 15;
     int batch_blocks_per_cu = 2;
+    int necklace_min_order = 23;  // smallest prime Schur order sent to the necklace kernel
+    int necklace_depth = 28;      // largest FKM prefix depth of the necklace tasks
 };
 Tuning& tuning();
 

@@ -229,6 +229,12 @@ def _schur_residues(n: int, primes: List[int]) -> List[int]:
     """
     if not primes:
         return []
+    if (_planner.is_prime(n) and _native.get_tuning(_native.TUNE_NECKLACE_MIN_ORDER) <= n <= 47
+            and all(p < (1 << 31) for p in primes)):
+        # Prime orders: the minor is circulant after reindexing by a primitive
+        # root, and the device sums Glynn terms over necklace orbits
+        # (perm_c_omp.c:979-1006), about 2(n-1) times fewer terms.
+        return [_native.schur_permanent_modp(n, nth_root_of_unity_mod_p(n, p), p) for p in primes]
     minors = []
     for p in primes:
         omega = nth_root_of_unity_mod_p(n, p)

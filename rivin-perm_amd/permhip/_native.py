@@ -69,6 +69,8 @@ TUNE_LANES_LOG2 = 1
 TUNE_MIN_CBITS = 2
 TUNE_BATCH_ITEMS_LOG2 = 3
 TUNE_BATCH_BLOCKS = 4
+TUNE_NECKLACE_MIN_ORDER = 5
+TUNE_NECKLACE_DEPTH = 6
 BATCH_BLOCKS_PER_CU = 1
 BATCH_USE_GRAPH = 2
 MICROBENCH_NAMES = (

@@ -35,6 +35,7 @@ PLAIN_SOURCES = (
     "capi.hip",
     "batch_plan.hip",
     "microbench.hip",
+    "necklace.hip",
 )
 
 

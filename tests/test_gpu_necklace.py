@@ -1,0 +1,66 @@
+"""Necklace-orbit evaluation of Schur cofactor minors on the device against
+the Glynn path, the oracle and known perm(S_n) (bit-exact residues)."""
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# OEIS A003112 and the reference's checked-in values (schur_permanents.txt).
+KNOWN = {5: -5, 7: -105, 11: 6765, 13: 175747, 17: 25219857, 19: 142901109,
+         23: -31152650265, 29: -1300425712598285, 31: 126691467546591,
+         37: -339696021338069660245, 41: 1617562409111471411255401}
+
+
+def _with_tuning(native, key, value, fn):
+    old = native.get_tuning(key)
+    native.set_tuning(key, value)
+    try:
+        return fn()
+    finally:
+        native.set_tuning(key, old)
+
+
+@pytest.mark.parametrize("n", [5, 7, 11, 13, 17, 19, 23, 29, 31])
+def test_necklace_matches_glynn_and_known_values(native, n):
+    from permhip import planner
+
+    for p in planner.find_primes_congruent_to_1_mod_n(n, 2, 1 << 30) + [
+            planner.find_primes_congruent_to_1_mod_n(n, 1, 1000)[0]]:
+        w = planner.nth_root_of_unity_mod_p(n, p)
+        neck = _with_tuning(native, native.TUNE_NECKLACE_MIN_ORDER, 5,
+                            lambda: native.schur_permanent_modp(n, w, p))
+        glynn = _with_tuning(native, native.TUNE_NECKLACE_MIN_ORDER, 64,
+                             lambda: native.schur_permanent_modp(n, w, p))
+        assert neck == glynn == KNOWN[n] % p, (n, p)
+
+
+def test_necklace_against_oracle_small_orders(native, oracle_c):
+    from permhip import planner
+
+    for n in (5, 7, 11, 13):
+        for p in planner.find_primes_congruent_to_1_mod_n(n, 3, 200):
+            w = planner.nth_root_of_unity_mod_p(n, p)
+            S = planner.build_schur_matrix_mod_p(n, w, p)
+            got = _with_tuning(native, native.TUNE_NECKLACE_MIN_ORDER, 5,
+                               lambda: native.schur_permanent_modp(n, w, p))
+            assert got == oracle_c.permanent_mod_p(S, p, 8), (n, p)
+
+
+@pytest.mark.parametrize("depth", [2, 19, 20, 28])
+def test_task_depth_does_not_change_residues(native, depth):
+    """The task depth is clamped to [L/2, L - 14]; for n = 37 that is 18..22."""
+    from permhip import planner
+
+    n = 37
+    p = planner.find_primes_congruent_to_1_mod_n(n, 1, 1 << 30)[0]
+    w = planner.nth_root_of_unity_mod_p(n, p)
+    got = _with_tuning(native, native.TUNE_NECKLACE_DEPTH, depth,
+                       lambda: native.schur_permanent_modp(n, w, p))
+    assert got == KNOWN[n] % p
+
+
+@pytest.mark.parametrize("n", [37, 41])
+def test_exact_schur_permanents_through_necklaces(native, n):
+    import perm
+
+    assert perm.permanent(n) == KNOWN[n]
