@@ -213,6 +213,9 @@ def main() -> int:
     ap.add_argument("--kernel-only", action="store_true", help="headline launches only (for profilers)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="run one plain device copy of BYTES (HBM counter calibration) and exit")
+    ap.add_argument("--schur-residue", type=int, default=0, metavar="N",
+                    help="compute perm(S_N) modulo the first prime = 1 (mod N) above 2^30 and exit "
+                         "(one necklace or Glynn launch, for profilers)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -229,6 +232,13 @@ def main() -> int:
     if args.copy_calibration:
         _native.copy_calibration(args.copy_calibration, 1)
         log(f"copied {args.copy_calibration} bytes")
+        return 0
+    if args.schur_residue:
+        n = args.schur_residue
+        p = planner.find_primes_congruent_to_1_mod_n(n, 1, 1 << 30)[0]
+        t0 = time.perf_counter()
+        r = _native.schur_permanent_modp(n, planner.nth_root_of_unity_mod_p(n, p), p)
+        log(f"perm(S_{n}) mod {p} = {r} in {time.perf_counter() - t0:.2f} s")
         return 0
 
     info = _native.device_info()
