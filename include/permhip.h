@@ -95,7 +95,7 @@ int phip_permanent_mod3(const uint64_t *mags, const uint64_t *sgns, int n);
 /* Build a device-resident plan for `count` matrices: entries holds the
  * row-major int64 matrices back to back (sum of orders[i]^2 values),
  * moduli holds moduli_counts[i] odd moduli in [3, 2^31) per matrix.  Orders
- * must lie in [5, 40].  *out receives an opaque handle.  Returns 0 or -1. */
+ * must lie in [5, 46].  *out receives an opaque handle.  Returns 0 or -1. */
 int phip_batch_create(const int64_t *entries, const int32_t *orders, int count, const int64_t *moduli,
                       const int32_t *moduli_counts, void **out);
 

@@ -13,7 +13,7 @@ namespace phip {
 class DeviceContext;
 
 constexpr int kBatchMinOrder = 5;
-constexpr int kBatchMaxOrder = 40;
+constexpr int kBatchMaxOrder = 46;
 // Most primes one job evaluates together, and the stride of the per-item
 // partial sums.
 constexpr int kMaxSet = 8;

@@ -90,6 +90,8 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
             PHIP_BATCH_CASE(29) PHIP_BATCH_CASE(30) PHIP_BATCH_CASE(31) PHIP_BATCH_CASE(32)
             PHIP_BATCH_CASE(33) PHIP_BATCH_CASE(34) PHIP_BATCH_CASE(35) PHIP_BATCH_CASE(36)
             PHIP_BATCH_CASE(37) PHIP_BATCH_CASE(38) PHIP_BATCH_CASE(39) PHIP_BATCH_CASE(40)
+            PHIP_BATCH_CASE(41) PHIP_BATCH_CASE(42) PHIP_BATCH_CASE(43) PHIP_BATCH_CASE(44)
+            PHIP_BATCH_CASE(45) PHIP_BATCH_CASE(46)
 #undef PHIP_BATCH_CASE
             default:
                 break;

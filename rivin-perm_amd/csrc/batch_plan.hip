@@ -120,7 +120,7 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
     for (size_t i = 0; i < in.size(); ++i) {
         const PlanInput& x = in[i];
         if (x.n < kBatchMinOrder || x.n > kBatchMaxOrder) {
-            set_error("batch orders must lie in [5, 40]; use the single-matrix entry point otherwise");
+            set_error("batch orders must lie in [5, 46]; use the single-matrix entry point otherwise");
             return -1;
         }
         if (x.q < 3 || x.q >= (1u << 31) || !(x.q & 1)) {
@@ -441,7 +441,7 @@ int build_plan(BatchPlan& plan, const int64_t* entries, const int32_t* orders, i
     for (int m = 0; m < count; ++m) {
         const int n = orders[m];
         if (n < kBatchMinOrder || n > kBatchMaxOrder) {
-            set_error("batch orders must lie in [5, 40]; use the single-matrix entry point otherwise");
+            set_error("batch orders must lie in [5, 46]; use the single-matrix entry point otherwise");
             return -1;
         }
         if (moduli_counts[m] < 0) {

@@ -9,7 +9,7 @@ namespace phip {
 // orders (too few states for a 16-state unrolled chunk) and larger ones use
 // the generic kernels.
 constexpr int kGlynnMinOrder = 5;
-constexpr int kGlynnMaxOrder = 40;
+constexpr int kGlynnMaxOrder = 46;
 
 // Words per table column: residues one per word (group 0) or exact row sums
 // packed two 16-bit halves per word (groups 2..4).

@@ -198,7 +198,7 @@ int compute_residues(DeviceContext* ctx, const int64_t* A, int64_t stride, int n
     }
 
     std::vector<uint64_t> odd_residue(k, 0);
-    // Odd parts below 2^31 at orders [5, 40] run on the persistent Glynn
+    // Odd parts below 2^31 at orders [5, 46] run on the persistent Glynn
     // engine as a one-matrix plan; moduli whose tables coincide share one
     // Gray walk.  The rest use the 64-bit Montgomery kernel.
     std::vector<PlanInput> plan_in;

@@ -4,7 +4,7 @@
 matrix API does (row-sum bound, ascending CRT primes from 2^30, signed lift;
 perm.py:939-953, 1115-1128, 1207-1210), uploads all (matrix, prime) jobs
 once, and replays the captured persistent launch for each run.  Matrices the
-persistent kernel does not serve (order outside [5, 40] or entries beyond
+persistent kernel does not serve (order outside [5, 46] or entries beyond
 int64) go through the single-matrix device path.
 """
 
@@ -17,7 +17,7 @@ import numpy as np
 from . import _native, planner
 
 BATCH_MIN_ORDER = 5
-BATCH_MAX_ORDER = 40
+BATCH_MAX_ORDER = 46
 
 
 def _int64_rows(matrix):

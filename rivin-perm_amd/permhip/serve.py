@@ -171,7 +171,7 @@ class PermanentServer:
         self.completed.append(req)
 
     def _run_outside_plans(self, req: _Request) -> None:
-        """Parts the persistent kernel does not serve (order < 5 or > 40) go
+        """Parts the persistent kernel does not serve (order < 5 or > 46) go
         through the single-matrix device launch."""
         for part in req.parts:
             n = part.matrix.shape[0]

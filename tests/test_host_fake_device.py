@@ -23,8 +23,8 @@ class FakeBatch:
         self.options = {}
         self._pending = None
         for m in self.mats:
-            if not 5 <= m.shape[0] <= 40:
-                raise RuntimeError("phip_batch_create failed: batch orders must lie in [5, 40]")
+            if not 5 <= m.shape[0] <= 46:
+                raise RuntimeError("phip_batch_create failed: batch orders must lie in [5, 46]")
 
     def _compute(self):
         out = []
