@@ -199,6 +199,25 @@ def published_workloads() -> dict:
     return out
 
 
+def _launch_shape(plan) -> dict:
+    """Chunk size and workgroups per CU a plan runs with, and how the
+    autotuner chose them."""
+    st = plan.stats()
+    out = {
+        "items_log2": int(st["items_log2"]),
+        "chunk_bits": [int(st["chunk_bits_min"]), int(st["chunk_bits_max"])],
+        "blocks_per_cu": int(st["blocks_per_cu"]),
+        "waves_per_cu": 4 * int(st["blocks_per_cu"]),
+        "chosen_by": "defaults",
+    }
+    if plan.tuning is not None:
+        t = plan.tuning.as_dict()
+        out["chosen_by"] = "autotune"
+        out["autotune"] = {k: t[k] for k in ("seconds", "default_seconds", "speedup_vs_default", "trials",
+                                              "tuning_seconds")}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -213,6 +232,12 @@ def main() -> int:
     ap.add_argument("--kernel-only", action="store_true", help="headline launches only (for profilers)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="run one plain device copy of BYTES (HBM counter calibration) and exit")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default chunk size and workgroups per CU (no tuning trials)")
+    ap.add_argument("--items-log2", type=int, default=None,
+                    help="headline plan: fixed work-item target (log2), no tuning")
+    ap.add_argument("--blocks-per-cu", type=int, default=None,
+                    help="headline plan: fixed workgroups per CU, no tuning")
     ap.add_argument("--schur-residue", type=int, default=0, metavar="N",
                     help="compute perm(S_N) modulo the first prime = 1 (mod N) above 2^30 and exit "
                          "(one necklace or Glynn launch, for profilers)")
@@ -248,7 +273,14 @@ def main() -> int:
     primes, product = planner.generic_crt_primes(bound)
     mulmods_per_perm = HEADLINE_N * 2.0 ** (HEADLINE_N - 1) * len(primes)
 
-    plan = PermanentBatch([A])
+    fixed = args.items_log2 is not None or args.blocks_per_cu is not None
+    plan = PermanentBatch([A], autotune=False if (args.no_autotune or fixed) else {"reps": 3, "max_seconds": 20.0})
+    if args.items_log2 is not None:
+        plan.plan.set_option(_native.BATCH_ITEMS_LOG2, args.items_log2)
+    if args.blocks_per_cu is not None:
+        plan.plan.set_option(_native.BATCH_BLOCKS_PER_CU, args.blocks_per_cu)
+    if plan.tuning is not None:
+        log(f"headline autotune: {plan.tuning.as_dict()}")
     log(f"headline plan: n={HEADLINE_N} primes={len(primes)} stats={plan.stats()}")
     exact = None
     for _ in range(args.warmup):
@@ -339,6 +371,7 @@ def main() -> int:
         "exact_permanent": str(exact),
         "plain_launch": {"s_per_perm": plain_s, "perms_per_s": 1.0 / plain_s},
         "roofline": roof,
+        "launch_shape": _launch_shape(plan),
         "device": info,
     }
 
@@ -347,6 +380,10 @@ def main() -> int:
         t0 = time.perf_counter()
         batch = PermanentBatch(mats)
         t_create = time.perf_counter() - t0
+        if not args.no_autotune:
+            from permhip import autotune
+
+            batch.tuning = autotune.tune_plan(batch.plan, reps=1, max_seconds=25.0)
         warm = batch.run()
         runs = []
         for _ in range(3):
@@ -389,6 +426,7 @@ def main() -> int:
             "plain_all_primes_per_call_matrices_per_s": len(mats) / t_plain_matrix,
             "speedup_vs_plain_all_primes_per_call": t_plain_matrix / b_s,
             "bit_exact_vs_plain": True,
+            "launch_shape": _launch_shape(batch),
         }
         log(f"batch: {result['batch']}")
         batch.close()

@@ -113,6 +113,7 @@ int phip_batch_query(void *plan, int64_t *residues_out, int block);
 
 #define PHIP_BATCH_BLOCKS_PER_CU 1 /* persistent workgroups per CU (1..64) */
 #define PHIP_BATCH_USE_GRAPH 2     /* 0: plain stream launches, else hipGraph replay */
+#define PHIP_BATCH_ITEMS_LOG2 3    /* re-size the chunks for about 2^value work items (0..30) */
 int phip_batch_set_option(void *plan, int option, int64_t value);
 
 /* out[0] last run ms (persistent kernels only without a graph; whole graph
@@ -120,7 +121,10 @@ int phip_batch_set_option(void *plan, int option, int64_t value);
  * n 2^(n-1) per residue), [3] jobs (one table with a set of primes that
  * share it), [4] work items, [5] persistent launches, [6] persistent
  * wavefronts, [7] row-sum group size and [9] prime-set size of the first
- * launch, [8] residues.  Returns the number of values written or -1. */
+ * launch, [8] residues, [10] and [11] smallest and largest chunk (log2 of
+ * the Gray states per lane), [12] work-item target (log2) the chunks were
+ * sized for, [13] workgroups per CU.  Returns the number of values written
+ * or -1. */
 int phip_batch_stats(void *plan, double *out, int len);
 
 int phip_batch_destroy(void *plan);

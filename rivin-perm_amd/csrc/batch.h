@@ -71,6 +71,8 @@ struct PlanHost {
     std::vector<BatchClass> classes;
     double total_states = 0;   // Gray states over all residues
     double total_mulmods = 0;  // n 2^(n-1) per residue
+    int items_log2 = 0;        // work-item target the chunks were sized for
+    int min_cbits = 0;
 };
 
 // Device image of a plan (pointers into device memory).
@@ -90,6 +92,11 @@ struct PlanDevice {
 // row order, representation), prime sets, cost-sorted items of one
 // wavefront each, about 2^items_log2 items in all.  Returns 0 or -1.
 int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cbits, PlanHost& out);
+
+// Re-sizes the chunks of an existing plan's jobs for about 2^items_log2
+// work items (chunk bits, lanes, items and classes; the tables, prime sets
+// and slots stay).  Returns 0 or -1.
+int assign_items(PlanHost& out, int items_log2, int min_cbits);
 
 // Enqueues one run: counter reset, one persistent launch per class (between
 // ev0 and ev1 when given), finalisation into d.residues.  Returns 0 or -1.

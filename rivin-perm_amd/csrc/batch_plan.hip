@@ -91,6 +91,10 @@ struct DevBuf {
         if (ptr) hipFree(ptr);
     }
     int alloc(size_t count) {
+        if (ptr) {
+            hipFree(ptr);
+            ptr = nullptr;
+        }
         hipError_t e = hipMalloc(&ptr, std::max<size_t>(count, 1) * sizeof(T));
         if (e != hipSuccess) {
             set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
@@ -153,7 +157,6 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
 
     // Pass 2: jobs (one per set chunk) and slots.
     out.slots.resize(in.size());
-    double job_states = 0;
     for (const Set& st : sets) {
         size_t next = 0;
         for (int K : st.group == 0 ? std::vector<int>(st.inputs.size(), 1) : split_set((int)st.inputs.size())) {
@@ -188,12 +191,21 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
                 }
             }
             next += take;
-            job_states += std::ldexp(1.0, L);
             out.jobs.push_back(jb);
         }
     }
 
+    return assign_items(out, items_log2, min_cbits);
+}
+
+int assign_items(PlanHost& out, int items_log2, int min_cbits) {
+    out.items.clear();
+    out.classes.clear();
+    out.items_log2 = items_log2;
+    out.min_cbits = min_cbits;
     // Chunk bits: spread the job states over about 2^items_log2 items.
+    double job_states = 0;
+    for (const BatchJob& jb : out.jobs) job_states += std::ldexp(1.0, jb.n - 1);
     const double target = std::ldexp(1.0, items_log2);
     int s = (int)std::lround(std::log2(std::max(1.0, job_states / target)));
     s = std::min(std::max(s, 10), 30);
@@ -302,21 +314,44 @@ struct BatchPlan {
 
     int upload() {
         const PlanHost& h = host;
-        if (d_jobs.alloc(h.jobs.size()) || d_items.alloc(h.items.size()) || d_blob.alloc(h.blob.size()) ||
-            d_set_p.alloc(h.set_p.size()) || d_set_pinv.alloc(h.set_pinv.size()) || d_slots.alloc(h.slots.size()) ||
-            d_partials.alloc(h.items.size() * kMaxSet) || d_counters.alloc(32) || d_res.alloc(h.slots.size()))
+        if (d_blob.alloc(h.blob.size()) || d_set_p.alloc(h.set_p.size()) || d_set_pinv.alloc(h.set_pinv.size()) ||
+            d_slots.alloc(h.slots.size()) || d_counters.alloc(32) || d_res.alloc(h.slots.size()))
             return -1;
         PHIP_HIP_TRY(hipHostMalloc((void**)&h_res, std::max<size_t>(h.slots.size(), 1) * sizeof(int64_t),
                                    hipHostMallocDefault));
-        PHIP_HIP_TRY(hipMemcpy(d_jobs.ptr, h.jobs.data(), h.jobs.size() * sizeof(BatchJob), hipMemcpyHostToDevice));
-        PHIP_HIP_TRY(hipMemcpy(d_items.ptr, h.items.data(), h.items.size() * sizeof(uint2), hipMemcpyHostToDevice));
         PHIP_HIP_TRY(hipMemcpy(d_blob.ptr, h.blob.data(), h.blob.size() * sizeof(int32_t), hipMemcpyHostToDevice));
         PHIP_HIP_TRY(hipMemcpy(d_set_p.ptr, h.set_p.data(), h.set_p.size() * sizeof(int32_t), hipMemcpyHostToDevice));
         PHIP_HIP_TRY(hipMemcpy(d_set_pinv.ptr, h.set_pinv.data(), h.set_pinv.size() * sizeof(uint32_t),
                                hipMemcpyHostToDevice));
         PHIP_HIP_TRY(
             hipMemcpy(d_slots.ptr, h.slots.data(), h.slots.size() * sizeof(BatchSlot), hipMemcpyHostToDevice));
+        return upload_items();
+    }
+
+    // Jobs, items and partials: the parts that depend on the chunk sizes.
+    int upload_items() {
+        const PlanHost& h = host;
+        if (d_jobs.alloc(h.jobs.size()) || d_items.alloc(h.items.size()) || d_partials.alloc(h.items.size() * kMaxSet))
+            return -1;
+        PHIP_HIP_TRY(hipMemcpy(d_jobs.ptr, h.jobs.data(), h.jobs.size() * sizeof(BatchJob), hipMemcpyHostToDevice));
+        PHIP_HIP_TRY(hipMemcpy(d_items.ptr, h.items.data(), h.items.size() * sizeof(uint2), hipMemcpyHostToDevice));
         return 0;
+    }
+
+    // New chunk sizes for about 2^items_log2 work items.
+    int resize_items(int items_log2) {
+        PHIP_HIP_TRY(hipSetDevice(device));
+        PHIP_HIP_TRY(hipStreamSynchronize(stream));
+        const int previous = host.items_log2;
+        if (assign_items(host, items_log2, host.min_cbits) != 0) {
+            // The previous sizing succeeded before; restore it so the plan
+            // stays runnable.
+            const std::string why = last_error();
+            assign_items(host, previous, host.min_cbits);
+            set_error(why);
+            return -1;
+        }
+        return upload_items();
     }
 
     int enqueue(bool timed) {
@@ -330,6 +365,10 @@ struct BatchPlan {
 
     // Enqueue one run on the plan's stream without waiting.
     int launch_locked() {
+        if (!d_jobs.ptr || !d_items.ptr || !d_partials.ptr) {
+            set_error("batch plan has no device work list (a failed re-sizing); create it again");
+            return -1;
+        }
         PHIP_HIP_TRY(hipSetDevice(device));
         if (use_graph) {
             if (!exec) {
@@ -547,6 +586,13 @@ int phip_batch_set_option(void* handle, int option, int64_t value) {
             }
             plan->blocks_per_cu = (int)value;
             break;
+        case PHIP_BATCH_ITEMS_LOG2:
+            if (value < 0 || value > 30) {
+                phip::set_error("work items (log2) must lie in [0, 30]");
+                return -1;
+            }
+            if (plan->resize_items((int)value) != 0) return -1;
+            break;
         case PHIP_BATCH_USE_GRAPH:
             plan->use_graph = value != 0;
             break;
@@ -572,6 +618,11 @@ int phip_batch_stats(void* handle, double* out, int len) {
     }
     auto* plan = static_cast<BatchPlan*>(handle);
     const phip::PlanHost& h = plan->host;
+    int cbits_min = h.jobs.empty() ? -1 : 64, cbits_max = -1;
+    for (const phip::BatchJob& jb : h.jobs) {
+        cbits_min = std::min(cbits_min, jb.cbits);
+        cbits_max = std::max(cbits_max, jb.cbits);
+    }
     const double v[] = {(double)plan->last_ms,
                         h.total_states,
                         h.total_mulmods,
@@ -581,7 +632,11 @@ int phip_batch_stats(void* handle, double* out, int len) {
                         (double)(plan->cus * plan->blocks_per_cu * 4),
                         h.classes.empty() ? -1.0 : (double)h.classes[0].group,
                         (double)h.slots.size(),
-                        h.classes.empty() ? -1.0 : (double)h.classes[0].set};
+                        h.classes.empty() ? -1.0 : (double)h.classes[0].set,
+                        (double)cbits_min,
+                        (double)cbits_max,
+                        (double)h.items_log2,
+                        (double)plan->blocks_per_cu};
     const int m = std::min<int>(len, (int)(sizeof(v) / sizeof(v[0])));
     for (int i = 0; i < m; ++i) out[i] = v[i];
     return m;

@@ -73,6 +73,7 @@ TUNE_NECKLACE_MIN_ORDER = 5
 TUNE_NECKLACE_DEPTH = 6
 BATCH_BLOCKS_PER_CU = 1
 BATCH_USE_GRAPH = 2
+BATCH_ITEMS_LOG2 = 3
 MICROBENCH_NAMES = (
     "v_add_u32",
     "v_mul_i32_i24",
@@ -317,7 +318,8 @@ class Batch:
         return [int(v) for v in self._out[: self.total]]
 
     def stats(self) -> dict:
-        keys = ("last_ms", "states", "mulmods", "jobs", "items", "launches", "waves", "group", "residues", "set")
+        keys = ("last_ms", "states", "mulmods", "jobs", "items", "launches", "waves", "group", "residues", "set",
+                "chunk_bits_min", "chunk_bits_max", "items_log2", "blocks_per_cu")
         vals = (ctypes.c_double * len(keys))()
         count = self._lib.phip_batch_stats(self._handle, vals, len(keys))
         _check(count, "phip_batch_stats")

@@ -34,7 +34,17 @@ def _int64_rows(matrix):
 class PermanentBatch:
     """Plan once, run many times; results are exact Python integers."""
 
-    def __init__(self, matrices: Sequence, *, use_graph: bool = True, blocks_per_cu: int | None = None):
+    def __init__(
+        self,
+        matrices: Sequence,
+        *,
+        use_graph: bool = True,
+        blocks_per_cu: int | None = None,
+        autotune: bool | dict = False,
+    ):
+        """``autotune`` True (or a dict of ``autotune.tune_plan`` keyword
+        arguments) times candidate chunk sizes and workgroups per CU on this
+        plan and keeps the fastest; the choice is in ``self.tuning``."""
         self.rows: List[List[List[int]]] = []
         self.bounds: List[int] = []
         self.primes: List[List[int]] = []
@@ -61,6 +71,13 @@ class PermanentBatch:
             self.plan.set_option(_native.BATCH_USE_GRAPH, 1 if use_graph else 0)
             if blocks_per_cu is not None:
                 self.plan.set_option(_native.BATCH_BLOCKS_PER_CU, blocks_per_cu)
+        self.tuning = None
+        if autotune and self.plan is not None:
+            from . import autotune as _autotune
+
+            kwargs = dict(autotune) if isinstance(autotune, dict) else {}
+            kwargs.setdefault("cache", _autotune.default_cache_path())
+            self.tuning = _autotune.tune_plan(self.plan, **kwargs)
 
     @property
     def mulmods(self) -> float:

@@ -106,3 +106,60 @@ def test_permanent_batch_exact_integers(native):
     got = PermanentBatch(mats).run()
     for M, v in zip(mats, got):
         assert v == oracle.subset_permanent([[int(x) for x in r] for r in M])
+
+
+def test_item_resizing_of_a_live_plan(native):
+    """Re-sizing the chunks of an uploaded plan keeps every residue."""
+    mats, moduli = _mixed_batch(17, 24)
+    batch = native.Batch(mats, moduli)
+    flat = batch.run()
+    seen = set()
+    for items_log2 in (0, 9, 14, 20, 30, 15):
+        batch.set_option(native.BATCH_ITEMS_LOG2, items_log2)
+        st = batch.stats()
+        assert st["items_log2"] == items_log2
+        assert 4 <= st["chunk_bits_min"] <= st["chunk_bits_max"] <= 45
+        seen.add((st["chunk_bits_min"], st["chunk_bits_max"], st["items"]))
+        assert batch.run() == flat, items_log2
+    assert len(seen) > 1
+    with pytest.raises(RuntimeError, match="work items"):
+        batch.set_option(native.BATCH_ITEMS_LOG2, 31)
+    assert batch.run() == flat
+    batch.close()
+
+
+def test_autotuned_batch_matches_default(native):
+    from permhip import autotune
+    from permhip.batch import PermanentBatch
+
+    rng = np.random.default_rng(31)
+    mats = [rng.integers(-50, 51, (n, n)).astype(np.int64) for n in rng.integers(14, 23, size=40)]
+    plain = PermanentBatch(mats)
+    expect = plain.run()
+    plain.close()
+    tuned = PermanentBatch(mats, autotune={"reps": 1, "max_seconds": 20.0})
+    r = tuned.tuning
+    assert r is not None and len(r.trials) >= 2
+    st = tuned.stats()
+    assert (st["items_log2"], st["blocks_per_cu"]) == (r.items_log2, r.blocks_per_cu)
+    assert st["waves"] % (autotune.WAVES_PER_BLOCK * st["blocks_per_cu"]) == 0
+    assert tuned.run() == expect
+    tuned.close()
+
+
+def test_autotuned_single_matrix_path(native):
+    import perm
+    from permhip import autotune
+
+    old = native.get_tuning(native.TUNE_LANES_LOG2), native.get_tuning(native.TUNE_BATCH_BLOCKS)
+    rng = np.random.default_rng(41)
+    A = rng.integers(-10, 11, (22, 22)).astype(np.int64)
+    before = perm.matrix_permanent(A)
+    try:
+        r = autotune.tune_single(22, reps=1, max_seconds=10.0)
+        assert native.get_tuning(native.TUNE_LANES_LOG2) == r.items_log2 + 6
+        assert native.get_tuning(native.TUNE_BATCH_BLOCKS) == r.blocks_per_cu
+        assert perm.matrix_permanent(A) == before
+    finally:
+        native.set_tuning(native.TUNE_LANES_LOG2, old[0])
+        native.set_tuning(native.TUNE_BATCH_BLOCKS, old[1])

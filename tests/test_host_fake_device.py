@@ -161,6 +161,12 @@ def test_permanent_batch_host(fake_device):
     got = PermanentBatch(mats).run()
     assert got == [oracle.subset_permanent(m.tolist()) for m in mats]
 
+    tuned = PermanentBatch(mats, autotune={"reps": 1, "items": (14, 15), "blocks": (1, 2)})
+    assert tuned.tuning is not None
+    assert tuned.plan.options[fake_device.BATCH_ITEMS_LOG2] == tuned.tuning.items_log2
+    assert tuned.plan.options[fake_device.BATCH_BLOCKS_PER_CU] == tuned.tuning.blocks_per_cu
+    assert tuned.run() == got
+
 
 def test_serve_scheduler(fake_device):
     from permhip.serve import PermanentServer
