@@ -47,6 +47,10 @@ int phip_device_info(char *buf, size_t len);
  * every modulus; n*n: a k x n x n array of matrices already reduced by the
  * caller, the reference wrappers' normalise-before-narrowing rule,
  * perm_cuda.py:104-122).  Any 0 < m < 2^63 is accepted, 0 <= n < 64.
+ * Odd parts below 2^31 at orders 5..46 run on the persistent 31-bit
+ * engine; other odd parts on the wide Glynn kernels (32-bit Montgomery
+ * below 2^32, 61-bit pseudo-Mersenne for 2^61 - c with odd c < 2^28,
+ * 64-bit Montgomery otherwise); even parts add a Ryser pass modulo 2^64.
  * out[i] receives perm(A) mod moduli[i] in [0, m).  Returns 0 or -1.
  * Multi-modulus form of permanent_modp_cuda (perm_modp_cuda.cu:740-742). */
 int phip_permanent_residues(const int64_t *A, int64_t prime_stride, int n, const int64_t *moduli,
@@ -60,7 +64,7 @@ int64_t phip_permanent_modp(const int64_t *A, int n, int64_t p);
  * Odd n >= 3, odd p and omega^n == 1 use the cofactor identity
  * per(S_n) = n * per(minor_(n-1)) (perm_c_omp.c:971-975); otherwise the
  * full matrix is used.  For a prime order n in [necklace min order, 47] and
- * an odd prime p < 2^31 the minor is evaluated by necklace orbits of its
+ * an odd prime p < 2^32 the minor is evaluated by necklace orbits of its
  * circulant form (perm_c_omp.c:79-507), else by Glynn.  Replaces
  * schur_permanent_modp_cuda (perm_modp_cuda.cu:1266) and
  * perm_c_ext.schur_permanent_mod_p. */
@@ -139,6 +143,7 @@ int phip_batch_destroy(void *plan);
 #define PHIP_TUNE_BATCH_BLOCKS 4     /* batch: default persistent workgroups per CU */
 #define PHIP_TUNE_NECKLACE_MIN_ORDER 5 /* Schur: smallest prime order using necklace orbits (5..64) */
 #define PHIP_TUNE_NECKLACE_DEPTH 6   /* Schur necklaces: largest FKM prefix depth of the tasks (2..30) */
+#define PHIP_TUNE_WIDE_MIN_TIER 7    /* wide-modulus Glynn: smallest row-register tier (16, 32, 48, 64) */
 int phip_set_tuning(int key, int64_t value);
 int64_t phip_get_tuning(int key);
 

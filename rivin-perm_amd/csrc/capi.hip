@@ -128,6 +128,10 @@ int phip_set_tuning(int key, int64_t value) {
             if (value < 2 || value > 30) break;
             t.necklace_depth = (int)value;
             return 0;
+        case PHIP_TUNE_WIDE_MIN_TIER:
+            if (value != 16 && value != 32 && value != 48 && value != 64) break;
+            t.wide_min_tier = (int)value;
+            return 0;
         default:
             phip::set_error("unknown tuning key");
             return -1;
@@ -145,6 +149,7 @@ int64_t phip_get_tuning(int key) {
         case PHIP_TUNE_BATCH_BLOCKS: return t.batch_blocks_per_cu;
         case PHIP_TUNE_NECKLACE_MIN_ORDER: return t.necklace_min_order;
         case PHIP_TUNE_NECKLACE_DEPTH: return t.necklace_depth;
+        case PHIP_TUNE_WIDE_MIN_TIER: return t.wide_min_tier;
         default:
             phip::set_error("unknown tuning key");
             return -1;

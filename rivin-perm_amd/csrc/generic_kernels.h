@@ -1,9 +1,8 @@
-// Kernels for the moduli the 31-bit Glynn kernels do not cover, so that the
+// Kernels for the even moduli the Glynn kernels cannot serve, so that the
 // modular entry point accepts every modulus the reference's perm_c_ext does
-// (any 0 < m < 2^63, perm_c_omp.c:876-880):
+// (any 0 < m < 2^63, perm_c_omp.c:876-880); odd moduli of any width run on
+// the Glynn kernels of batch.hip and wide.hip:
 //
-//   glynn63_kernel   fixed-sign Glynn for any odd modulus q < 2^63 with the
-//                    64-bit unsigned Montgomery product; runtime order n.
 //   ryser_u64_kernel Ryser's formula in wrapping 64-bit arithmetic, i.e. the
 //                    permanent modulo 2^64 (perm_c_omp.c:666-770 restated
 //                    over Z/2^64).  An even modulus 2^e * q is served by this
@@ -12,8 +11,8 @@
 //   sum_mod_kernel   one workgroup per modulus sums per-lane or per-block
 //                    partials modulo that modulus (tree in LDS, no atomics).
 //
-// These paths exist for completeness of the modular contract (tiny orders,
-// wide or even moduli); the hot path is the persistent kernel of batch.hip.
+// These paths exist for completeness of the modular contract (even moduli);
+// the hot path is the persistent kernel of batch.hip.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -24,50 +23,6 @@ namespace phip {
 
 constexpr int kGenericBlock = 256;
 constexpr int kMaxOrder = 63;
-
-// rs0[k][n]: row sums of state 0 modulo q; deltas[k][n-1][n]: 2a mod q,
-// column-major.  partials[k][lanes] gets each lane's sum modulo q.  The sum
-// carries R^-(n-1) with R = 2^64.
-__global__ __launch_bounds__(kGenericBlock) void glynn63_kernel(
-    const uint64_t* __restrict__ rs0, const uint64_t* __restrict__ deltas,
-    const uint64_t* __restrict__ moduli, const uint64_t* __restrict__ nprimes, int n, int cbits,
-    uint64_t lanes_per_prime, uint64_t* __restrict__ partials) {
-    const int q = blockIdx.y;
-    const uint64_t p = moduli[q];
-    const uint64_t np = nprimes[q];
-    const uint64_t lane = (uint64_t)blockIdx.x * kGenericBlock + threadIdx.x;
-    if (lane >= lanes_per_prime) return;
-    const uint64_t* __restrict__ D = deltas + (size_t)q * (n - 1) * n;
-    const uint64_t base = lane << cbits;
-    uint64_t gray = base ^ (base >> 1);
-
-    uint64_t rs[kMaxOrder];
-    for (int i = 0; i < n; ++i) rs[i] = rs0[(size_t)q * n + i];
-    for (int j = 0; j < n - 1; ++j) {
-        if ((gray >> j) & 1u) {
-            for (int i = 0; i < n; ++i) rs[i] = sub_mod_u64(rs[i], D[j * n + i], p);
-        }
-    }
-    bool negative = __popcll(gray) & 1;
-    uint64_t acc = 0;
-    const uint64_t states = 1ull << cbits;
-    for (uint64_t t = 0; t < states; ++t) {
-        if (t != 0) {
-            const int j = __builtin_ctzll(t);
-            const uint64_t s = base + t;
-            gray = s ^ (s >> 1);
-            const bool sub = (gray >> j) & 1u;
-            for (int i = 0; i < n; ++i) {
-                rs[i] = sub ? sub_mod_u64(rs[i], D[j * n + i], p) : add_mod_u64(rs[i], D[j * n + i], p);
-            }
-            negative = !negative;
-        }
-        uint64_t prod = rs[0];
-        for (int i = 1; i < n; ++i) prod = mont63_mul(prod, rs[i], p, np);
-        acc = negative ? sub_mod_u64(acc, prod, p) : add_mod_u64(acc, prod, p);
-    }
-    partials[(size_t)q * lanes_per_prime + lane] = acc;
-}
 
 // A[n][n] column-major, entries as two's-complement 64-bit words.  Each lane
 // covers 2^cbits Ryser subsets; partials[lane] = sum of (-1)^|S| prod rs

@@ -27,11 +27,14 @@
 // Row sums.  Each lane keeps the exact integer row sums of its last emitted
 // word (centred entries, |sum| < L p / 2 < 2^37) and updates them by the
 // bits that differ from the next one.  The lanes' changed columns are
-// merged into one wave-uniform set, so each column's deltas 2 w[(c - r) mod
+// merged into one wave-uniform set, so each column's entries w[(c - r) mod
 // L] come from scalar loads and each row update is one v_mad_i64_i32 with a
-// per-lane sign in {-1, 0, +1}.  The product reduces each row once (signed
-// Montgomery, R = 2^32) and multiplies the rows in a balanced tree, so a
+// per-lane factor in {-2, 0, +2}.  The product reduces each row once
+// (Montgomery, R = 2^32) and multiplies the rows in a balanced tree, so a
 // term carries R^-(2L-1); folding a lane's accumulator adds one more R^-1.
+// Two arithmetics share the kernel: signed Montgomery for odd p < 2^31 and
+// unsigned Montgomery for odd p < 2^32 (the reference's 32-bit primes
+// q = 1 mod n, perm_cuda.py:289-302, perm_modp_cuda.cu:487-701).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -44,6 +47,7 @@
 #include "phip_arith.h"
 #include "phip_reduce.h"
 #include "residues.h"
+#include "wide_arith.h"
 
 namespace phip {
 
@@ -93,20 +97,52 @@ __device__ __forceinline__ uint64_t wave_or_u64(uint64_t v) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// Signed Montgomery, odd p < 2^31: values in (-2^31, 2^31).
+struct NeckSigned31 {
+    typedef int32_t T;
+    int32_t p;
+    int32_t negp;
+    uint32_t pinv;
+    __device__ __forceinline__ T row(int64_t s) const { return redc31(s, negp, pinv); }
+    __device__ __forceinline__ T mul(T a, T b) const { return mont31(a, b, negp, pinv); }
+    __device__ __forceinline__ int32_t centred(T v) const { return v; }
+    __device__ __forceinline__ int64_t fold(int64_t acc) const { return redc31(acc, negp, pinv); }
+};
+
+// Unsigned Montgomery, odd p < 2^32: values in [0, p).  Row sums (|s| <
+// 23 p) and accumulators (|acc| < 2^60) are made non-negative by adding a
+// multiple of p that keeps them below p 2^32, then reduced by REDC.
+struct NeckUnsigned32 {
+    typedef uint32_t T;
+    Mont32 m;
+    int64_t row_bias;  // 32 p
+    int64_t acc_bias;  // 2^30 p >= 2^61
+    __device__ __forceinline__ T redc(uint64_t x) const {
+        const uint32_t k = (uint32_t)x * m.np;
+        const uint64_t s = x + (uint64_t)k * m.p;
+        const uint64_t r = (s >> 32) | ((uint64_t)(s < x) << 32);
+        return (T)(r >= m.p ? r - m.p : r);
+    }
+    __device__ __forceinline__ T row(int64_t s) const { return redc((uint64_t)(s + row_bias)); }
+    __device__ __forceinline__ T mul(T a, T b) const { return m.mul(a, b); }
+    __device__ __forceinline__ int32_t centred(T v) const { return (int32_t)(v > (m.p >> 1) ? v - m.p : v); }
+    __device__ __forceinline__ int64_t fold(int64_t acc) const { return (int64_t)redc((uint64_t)(acc + acc_bias)); }
+};
+
 // tasks[t] = prefix bits (depth bits, a[1] on top) | period << 56.
-// E[j] = 2 w[(-j) mod L] centred, 0 <= j < 2L, so row r of column c reads
+// E[j] = w[(-j) mod L] centred, 0 <= j < 2L, so row r of column c reads
 // E[L - c + r].  block_partials[b] receives the block's sum of lane
 // residues in [0, p).
-template <int L>
+template <int L, class Ar>
 __global__ __launch_bounds__(kNecklaceBlock) void necklace_kernel(const uint64_t* __restrict__ tasks,
                                                                   uint32_t n_tasks, int depth,
                                                                   const int32_t* __restrict__ E, int64_t base_sum,
-                                                                  int32_t p, uint32_t pinv,
+                                                                  Ar ar, int64_t p,
                                                                   uint32_t* __restrict__ counter,
                                                                   uint64_t* __restrict__ block_partials) {
+    typedef typename Ar::T T;
     constexpr uint64_t kMask = (1ull << L) - 1;
     constexpr uint64_t kDivisors = divisor_mask(L);
-    const int32_t negp = -p;
     int64_t rs[L];
 #pragma unroll
     for (int r = 0; r < L; ++r) rs[r] = base_sum;
@@ -169,7 +205,7 @@ __global__ __launch_bounds__(kNecklaceBlock) void necklace_kernel(const uint64_t
             const int b = __builtin_ctzll(u);
             u &= u - 1;
             const int col = L - 1 - b;
-            const int32_t sgn = ((diff >> b) & 1u) ? (((w >> b) & 1u) ? -1 : 1) : 0;
+            const int32_t sgn = ((diff >> b) & 1u) ? (((w >> b) & 1u) ? -2 : 2) : 0;
             const ConstCol e = opaque_ptr(E + (L - col));
 #pragma unroll
             for (int r = 0; r < L; ++r) rs[r] += (int64_t)e[r] * (int64_t)sgn;
@@ -177,27 +213,27 @@ __global__ __launch_bounds__(kNecklaceBlock) void necklace_kernel(const uint64_t
         if (!done) prev = w;
 
         // Term: reduce each row, multiply in a balanced tree.
-        int32_t v[L];
+        T v[L];
 #pragma unroll
-        for (int r = 0; r < L; ++r) v[r] = redc31(rs[r], negp, pinv);
+        for (int r = 0; r < L; ++r) v[r] = ar.row(rs[r]);
 #pragma unroll
         for (int s = 1; s < L; s *= 2) {
 #pragma unroll
-            for (int r = 0; r + s < L; r += 2 * s) v[r] = mont31(v[r], v[r + s], negp, pinv);
+            for (int r = 0; r + s < L; r += 2 * s) v[r] = ar.mul(v[r], v[r + s]);
         }
         const int wt = __popcll(w);
         const int orbit = (2 * wt < L) ? 2 * period : period;
         const int32_t weight = done ? 0 : ((wt & 1) ? -orbit : orbit);
-        acc += (int64_t)v[0] * (int64_t)weight;
+        acc += (int64_t)ar.centred(v[0]) * (int64_t)weight;
         if (++reps == kFoldReps) {
-            total += redc31(acc, negp, pinv);
+            total += ar.fold(acc);
             acc = 0;
             reps = 0;
         }
         valid = valid && !done;
     }
-    total += redc31(acc, negp, pinv);
-    int64_t r = total % (int64_t)p;
+    total += ar.fold(acc);
+    int64_t r = total % p;
     if (r < 0) r += p;
     const uint64_t s = block_reduce_sum_u64<kNecklaceBlock>((uint64_t)r);
     if (threadIdx.x == 0) block_partials[blockIdx.x] = s;
@@ -239,13 +275,15 @@ void collect_tasks(int L, int depth, std::vector<uint64_t>& out) {
     }
 }
 
-using NecklaceKernel = void (*)(const uint64_t*, uint32_t, int, const int32_t*, int64_t, int32_t, uint32_t,
-                                uint32_t*, uint64_t*);
+template <class Ar>
+using NecklaceKernel = void (*)(const uint64_t*, uint32_t, int, const int32_t*, int64_t, Ar, int64_t, uint32_t*,
+                                uint64_t*);
 
-NecklaceKernel kernel_for(int L) {
+template <class Ar>
+NecklaceKernel<Ar> kernel_for(int L) {
     switch (L) {
 #define PHIP_NECKLACE_CASE(X) \
-    case X: return necklace_kernel<X>;
+    case X: return necklace_kernel<X, Ar>;
         PHIP_NECKLACE_CASE(4) PHIP_NECKLACE_CASE(6) PHIP_NECKLACE_CASE(10) PHIP_NECKLACE_CASE(12)
         PHIP_NECKLACE_CASE(16) PHIP_NECKLACE_CASE(18) PHIP_NECKLACE_CASE(22) PHIP_NECKLACE_CASE(28)
         PHIP_NECKLACE_CASE(30) PHIP_NECKLACE_CASE(36) PHIP_NECKLACE_CASE(40) PHIP_NECKLACE_CASE(42)
@@ -277,12 +315,12 @@ int smallest_primitive_root(int n) {
 
 bool necklace_supported(int n, uint64_t p) {
     return n >= 5 && n <= kNecklaceMaxOrder && is_prime_u64((uint64_t)n) && (p & 1) && p >= 3 &&
-           p < (1ull << 31) && kernel_for(n - 1) != nullptr;
+           p < (1ull << 32) && kernel_for<NeckSigned31>(n - 1) != nullptr;
 }
 
 int necklace_minor_modp(DeviceContext* ctx, int n, uint64_t omega, uint64_t p, int64_t* minor_out) {
     if (!necklace_supported(n, p)) {
-        set_error("necklace path needs a prime order n <= 47 and an odd prime p < 2^31");
+        set_error("necklace path needs a prime order n <= 47 and an odd prime p < 2^32");
         return -1;
     }
     const int L = n - 1;
@@ -303,7 +341,7 @@ int necklace_minor_modp(DeviceContext* ctx, int n, uint64_t omega, uint64_t p, i
         e = e * (uint64_t)g % (uint64_t)n;
     }
     std::vector<int32_t> E(2 * L);
-    for (int j = 0; j < 2 * L; ++j) E[j] = (int32_t)(2 * first[((-j) % L + L) % L]);
+    for (int j = 0; j < 2 * L; ++j) E[j] = (int32_t)first[((-j) % L + L) % L];
 
     // Task depth: at least L/2 (one-step prefix extension in the kernel),
     // 14 levels above the leaves where the task list allows (fewer
@@ -330,13 +368,20 @@ int necklace_minor_modp(DeviceContext* ctx, int n, uint64_t omega, uint64_t p, i
     PHIP_HIP_TRY(hipMemcpyAsync(d_E, E.data(), E.size() * 4, hipMemcpyHostToDevice, st));
     PHIP_HIP_TRY(hipMemcpyAsync(d_cnt, &zero, 4, hipMemcpyHostToDevice, st));
     if (ctx->profiling) PHIP_HIP_TRY(hipEventRecord(ctx->prof_start, st));
-    hipLaunchKernelGGL(kernel_for(L), dim3(grid), dim3(kNecklaceBlock), 0, st, d_tasks, (uint32_t)tasks.size(), depth,
-                       d_E, base_sum, (int32_t)p, inverse_mod_2_32((uint32_t)p), d_cnt, d_part);
+    if (p < (1ull << 31)) {
+        const NeckSigned31 ar{(int32_t)p, -(int32_t)p, inverse_mod_2_32((uint32_t)p)};
+        hipLaunchKernelGGL(kernel_for<NeckSigned31>(L), dim3(grid), dim3(kNecklaceBlock), 0, st, d_tasks,
+                           (uint32_t)tasks.size(), depth, d_E, base_sum, ar, (int64_t)p, d_cnt, d_part);
+    } else {
+        const NeckUnsigned32 ar{make_mont32(p), (int64_t)p << 5, (int64_t)p << 30};
+        hipLaunchKernelGGL(kernel_for<NeckUnsigned32>(L), dim3(grid), dim3(kNecklaceBlock), 0, st, d_tasks,
+                           (uint32_t)tasks.size(), depth, d_E, base_sum, ar, (int64_t)p, d_cnt, d_part);
+    }
     PHIP_HIP_TRY(hipGetLastError());
     if (ctx->profiling) PHIP_HIP_TRY(hipEventRecord(ctx->prof_stop, st));
     PHIP_HIP_TRY(hipMemcpyAsync(h_part, d_part, (size_t)grid * 8, hipMemcpyDeviceToHost, st));
     PHIP_HIP_TRY(hipStreamSynchronize(st));
-    uint64_t sum = 0;  // block sums are below 256 p < 2^39
+    uint64_t sum = 0;  // block sums are below 256 p < 2^40
     for (unsigned b = 0; b < grid; ++b) sum = (sum + h_part[b] % p) % p;
     if (ctx->profiling) {
         float ms = 0;

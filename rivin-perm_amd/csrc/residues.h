@@ -11,7 +11,7 @@ namespace phip {
 // Tunables (set through phip_set_tuning): target number of lanes per
 // single-matrix launch; minimum chunk bits; batch item count and workgroups
 // per CU; the smallest prime Schur order and the task depth of the necklace
-// kernel.
+// kernel; the smallest row-register tier of the wide Glynn kernels.
 struct Tuning {
     int lanes_log2 = 20;
     int min_cbits = 4;
@@ -19,6 +19,7 @@ struct Tuning {
     int batch_blocks_per_cu = 2;
     int necklace_min_order = 23;  // smallest prime Schur order sent to the necklace kernel
     int necklace_depth = 28;      // largest FKM prefix depth of the necklace tasks
+    int wide_min_tier = 16;       // smallest register tier (16, 32, 48, 64) of the wide Glynn kernels
 };
 Tuning& tuning();
 

@@ -17,6 +17,7 @@
 #include "launchers.h"
 #include "phip_arith.h"
 #include "tables.h"
+#include "wide.h"
 
 namespace phip {
 
@@ -37,69 +38,7 @@ int ceil_log2(uint64_t v) {
 
 uint64_t pow2_mod(int e, uint64_t q) { return host_powmod(2, (uint64_t)e, q); }
 
-struct OddJob {
-    int index;   // position in the caller's modulus list
-    uint64_t q;  // odd part, > 1
-};
-
 const int64_t* matrix_of(const int64_t* A, int64_t stride, int index) { return A + (int64_t)index * stride; }
-
-// ---------------------------------------------------------------------
-// Generic 64-bit Montgomery Glynn (any odd q < 2^63, any order >= 2).
-// ---------------------------------------------------------------------
-int run_glynn63(DeviceContext* ctx, const int64_t* A, int64_t stride, int n,
-                const std::vector<OddJob>& jobs, std::vector<uint64_t>& odd_residue) {
-    const int kk = (int)jobs.size();
-    std::vector<uint64_t> rs0((size_t)kk * n), deltas((size_t)kk * (n - 1) * n), moduli(kk), nprimes(kk);
-    for (int s = 0; s < kk; ++s) {
-        const uint64_t q = jobs[s].q;
-        const int64_t* M = matrix_of(A, stride, jobs[s].index);
-        moduli[s] = q;
-        nprimes[s] = 0 - inverse_mod_2_64(q);
-        for (int i = 0; i < n; ++i) {
-            uint64_t row = 0;
-            for (int j = 0; j < n; ++j) {
-                const uint64_t r = (uint64_t)host_floor_mod(M[(size_t)i * n + j], (int64_t)q);
-                row = add_mod_u64(row, r, q);
-                if (j < n - 1) deltas[((size_t)s * (n - 1) + j) * n + i] = add_mod_u64(r, r, q);
-            }
-            rs0[(size_t)s * n + i] = row;
-        }
-    }
-    const int cbits = std::max(0, (n - 1) - 14);
-    const uint64_t lanes = 1ull << (n - 1 - cbits);
-    const uint64_t gx = (lanes + kGenericBlock - 1) / kGenericBlock;
-
-    hipStream_t st = ctx->stream();
-    auto* d_rs0 = (uint64_t*)ctx->buffer(7, rs0.size() * 8);
-    auto* d_del = (uint64_t*)ctx->buffer(8, deltas.size() * 8);
-    auto* d_mod = (uint64_t*)ctx->buffer(9, moduli.size() * 8);
-    auto* d_np = (uint64_t*)ctx->buffer(10, nprimes.size() * 8);
-    auto* d_part = (uint64_t*)ctx->buffer(11, (size_t)kk * lanes * 8);
-    auto* d_sum = (uint64_t*)ctx->buffer(12, (size_t)kk * 8);
-    auto* h_sum = (uint64_t*)ctx->host_buffer(1, (size_t)kk * 8);
-    if (!d_rs0 || !d_del || !d_mod || !d_np || !d_part || !d_sum || !h_sum) return -1;
-    PHIP_HIP_TRY(hipMemcpyAsync(d_rs0, rs0.data(), rs0.size() * 8, hipMemcpyHostToDevice, st));
-    PHIP_HIP_TRY(hipMemcpyAsync(d_del, deltas.data(), deltas.size() * 8, hipMemcpyHostToDevice, st));
-    PHIP_HIP_TRY(hipMemcpyAsync(d_mod, moduli.data(), moduli.size() * 8, hipMemcpyHostToDevice, st));
-    PHIP_HIP_TRY(hipMemcpyAsync(d_np, nprimes.data(), nprimes.size() * 8, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(glynn63_kernel, dim3((unsigned)gx, (unsigned)kk), dim3(kGenericBlock), 0, st, d_rs0,
-                       d_del, d_mod, d_np, n, cbits, lanes, d_part);
-    PHIP_HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(sum_mod_kernel, dim3(kk), dim3(256), 0, st, d_part, lanes, d_mod, d_sum);
-    PHIP_HIP_TRY(hipGetLastError());
-    PHIP_HIP_TRY(hipMemcpyAsync(h_sum, d_sum, (size_t)kk * 8, hipMemcpyDeviceToHost, st));
-    PHIP_HIP_TRY(hipStreamSynchronize(st));
-    for (int s = 0; s < kk; ++s) {
-        const uint64_t q = jobs[s].q;
-        const uint64_t r_mod_q = (uint64_t)((((unsigned __int128)1) << 64) % q);
-        const uint64_t inv2 = (q + 1) / 2;
-        const uint64_t scale = host_mulmod(host_powmod(r_mod_q, (uint64_t)(n - 1), q),
-                                           host_powmod(inv2, (uint64_t)(n - 1), q), q);
-        odd_residue[jobs[s].index] = host_mulmod(h_sum[s], scale, q);
-    }
-    return 0;
-}
 
 // ---------------------------------------------------------------------
 // Ryser modulo 2^64.
@@ -200,10 +139,11 @@ int compute_residues(DeviceContext* ctx, const int64_t* A, int64_t stride, int n
     std::vector<uint64_t> odd_residue(k, 0);
     // Odd parts below 2^31 at orders [5, 46] run on the persistent Glynn
     // engine as a one-matrix plan; moduli whose tables coincide share one
-    // Gray walk.  The rest use the 64-bit Montgomery kernel.
+    // Gray walk.  The rest run on the wide Glynn kernels (32-bit Montgomery,
+    // 61-bit pseudo-Mersenne or 64-bit Montgomery by modulus).
     std::vector<PlanInput> plan_in;
     std::vector<int> plan_index;
-    std::vector<OddJob> wide;
+    std::vector<WideJob> wide;
     for (int i = 0; i < k; ++i) {
         const uint64_t m = (uint64_t)moduli[i];
         const int e = __builtin_ctzll(m);
@@ -221,7 +161,7 @@ int compute_residues(DeviceContext* ctx, const int64_t* A, int64_t stride, int n
         if (run_plan_in_context(ctx, plan_in, res.data()) != 0) return -1;
         for (size_t j = 0; j < plan_in.size(); ++j) odd_residue[plan_index[j]] = (uint64_t)res[j];
     }
-    if (!wide.empty() && run_glynn63(ctx, A, stride, n, wide, odd_residue) != 0) return -1;
+    if (!wide.empty() && run_wide_glynn(ctx, A, stride, n, wide, odd_residue) != 0) return -1;
 
     bool have_two = false;
     uint64_t two_residue = 0;

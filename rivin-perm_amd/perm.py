@@ -230,7 +230,7 @@ def _schur_residues(n: int, primes: List[int]) -> List[int]:
     if not primes:
         return []
     if (_planner.is_prime(n) and _native.get_tuning(_native.TUNE_NECKLACE_MIN_ORDER) <= n <= 47
-            and all(p < (1 << 31) for p in primes)):
+            and all(p < (1 << 32) for p in primes)):
         # Prime orders: the minor is circulant after reindexing by a primitive
         # root, and the device sums Glynn terms over necklace orbits
         # (perm_c_omp.c:979-1006), about 2(n-1) times fewer terms.

@@ -71,6 +71,7 @@ TUNE_BATCH_ITEMS_LOG2 = 3
 TUNE_BATCH_BLOCKS = 4
 TUNE_NECKLACE_MIN_ORDER = 5
 TUNE_NECKLACE_DEPTH = 6
+TUNE_WIDE_MIN_TIER = 7
 BATCH_BLOCKS_PER_CU = 1
 BATCH_USE_GRAPH = 2
 BATCH_ITEMS_LOG2 = 3

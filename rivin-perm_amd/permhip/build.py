@@ -36,6 +36,7 @@ PLAIN_SOURCES = (
     "batch_plan.hip",
     "microbench.hip",
     "necklace.hip",
+    "wide.hip",
 )
 
 

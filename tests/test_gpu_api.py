@@ -33,8 +33,9 @@ def test_schur_permanents_match_oeis(native, n):
 
 def test_reference_schur_residue_order_33(native):
     """Known answer of the reference's device tests
-    (tests/test_cuda_device_schur_glynn.py:141-146).  The prime is above 2^31,
-    so the 32 x 32 cofactor minor runs on the 64-bit Montgomery kernel."""
+    (tests/test_cuda_device_schur_glynn.py:141-146).  The prime is above 2^31
+    and 33 is not prime, so the 32 x 32 cofactor minor runs on the wide Glynn
+    kernel in 32-bit unsigned Montgomery arithmetic."""
     p, omega = 4_294_967_161, 2_662_262
     assert pow(omega, 33, p) == 1
     assert native.schur_permanent_modp(33, omega, p) == 1_428_739_342

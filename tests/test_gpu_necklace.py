@@ -64,3 +64,34 @@ def test_exact_schur_permanents_through_necklaces(native, n):
     import perm
 
     assert perm.permanent(n) == KNOWN[n]
+
+
+@pytest.mark.parametrize("n", [5, 7, 11, 13, 17, 19, 23, 29, 31])
+def test_necklace_with_32_bit_primes(native, n):
+    """Primes in [2^31, 2^32) take the unsigned Montgomery kernel: the two
+    smallest = 1 (mod n) above 2^31 and the largest below 2^32 (the
+    reference's device planner, perm_cuda.py:289-302)."""
+    import perm_cuda
+    from permhip import planner
+
+    primes = planner.find_primes_congruent_to_1_mod_n(n, 2, 1 << 31) + perm_cuda._find_primes_for_schur(n, 1)
+    for p in primes:
+        assert (1 << 31) < p < (1 << 32) and p % n == 1
+        w = planner.nth_root_of_unity_mod_p(n, p)
+        neck = _with_tuning(native, native.TUNE_NECKLACE_MIN_ORDER, 5,
+                            lambda: native.schur_permanent_modp(n, w, p))
+        glynn = _with_tuning(native, native.TUNE_NECKLACE_MIN_ORDER, 64,
+                             lambda: native.schur_permanent_modp(n, w, p))
+        assert neck == glynn == KNOWN[n] % p, (n, p)
+
+
+@pytest.mark.parametrize("n,omega,p,expected", [
+    (41, 782_766_686, 4_294_966_153, 3_889_040_436),
+    (47, 1_002_938_914, 4_294_967_161, 4_173_217_525),
+])
+def test_reference_device_golden_residues(native, n, omega, p, expected):
+    """Known answers of the reference's device tests
+    (tests/test_cuda_device_schur_glynn.py:147-169): 32-bit primes at the
+    two orders its necklace kernel serves."""
+    assert pow(omega, n, p) == 1
+    assert native.schur_permanent_modp(n, omega, p) == expected

@@ -186,3 +186,53 @@ def test_abi_rejections_do_not_poison_state(native):
     assert lib.phip_permanent_modp(iptr, 6, 0) == -1
     assert lib.phip_permanent_modp(iptr, 64, 101) == -1
     assert lib.phip_permanent_modp(iptr, 6, 101) == 1
+
+
+# One modulus of each wide arithmetic class and edge: 32-bit Montgomery (the
+# reference's 32-bit device primes among them), 61-bit pseudo-Mersenne
+# (Mersenne 2^61 - 1 and the largest complement taken) and 64-bit
+# Montgomery.
+WIDE_CLASSES = [
+    2_147_483_659, 4_294_967_291, 4_294_966_153, 4_294_967_161,
+    (1 << 61) - 1, (1 << 61) - 31, (1 << 61) - ((1 << 28) - 1),
+    (1 << 61) - ((1 << 28) + 1), (1 << 61) + 1, INT64_MAX - 24, INT64_MAX,
+]
+
+
+@pytest.mark.parametrize("tier", [16, 32, 48, 64])
+def test_wide_register_tiers(native, oracle_c, tier):
+    """Every row-register tier of the wide kernels, forced at small orders."""
+    old = native.get_tuning(native.TUNE_WIDE_MIN_TIER)
+    try:
+        native.set_tuning(native.TUNE_WIDE_MIN_TIER, tier)
+        for n in (2, 3, 4, 5, 9, 13, 16):
+            rng = random.Random(4000 + 17 * tier + n)
+            A = _random_matrix(rng, n, -(10 ** 12), 10 ** 12)
+            got = native.permanent_residues(np.array(A, dtype=np.int64), WIDE_CLASSES)
+            for m, r in zip(WIDE_CLASSES, got):
+                assert r == oracle_c.permanent_mod_p(A, m, 4), (tier, n, m)
+    finally:
+        native.set_tuning(native.TUNE_WIDE_MIN_TIER, old)
+
+
+@pytest.mark.parametrize("n", [17, 22])
+def test_wide_moduli_against_oracle(native, oracle_c, n):
+    rng = random.Random(5000 + n)
+    A = _random_matrix(rng, n, -(10 ** 15), 10 ** 15)
+    got = native.permanent_residues(np.array(A, dtype=np.int64), WIDE_CLASSES)
+    for m, r in zip(WIDE_CLASSES, got):
+        assert r == oracle_c.permanent_mod_p(A, m, 16), (n, m)
+
+
+@pytest.mark.parametrize("n", [27, 33])
+def test_wide_moduli_agree_with_exact_permanent(native, n):
+    """At orders the oracle cannot reach in seconds: every wide residue is
+    the exact permanent (from the 31-bit engine and CRT) reduced."""
+    import perm
+
+    rng = random.Random(6000 + n)
+    A = _random_matrix(rng, n, -10, 10)
+    exact = perm.matrix_permanent(A)
+    moduli = [4_294_966_153, (1 << 61) - 1, (1 << 61) - 31, INT64_MAX - 24]
+    got = native.permanent_residues(np.array(A, dtype=np.int64), moduli)
+    assert got == [exact % m for m in moduli], n

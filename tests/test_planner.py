@@ -140,3 +140,18 @@ def test_is_prime_against_trial_division():
         assert planner.is_prime(v) == trial
     assert not planner.is_prime(341_550_071_728_321)
     assert planner.is_prime((1 << 61) - 1)
+
+
+def test_reference_device_golden_residues_match_known_permanents():
+    """The reference's device known answers (tests/test_cuda_device_schur_glynn.py:
+    141-169) are the pinned exact permanents reduced: they check each other,
+    and the omegas are roots of unity of the stated order."""
+    exact = {**OEIS_A003112, **PROJECT_EXTENSIONS}
+    for n, omega, p, expected in [
+        (33, 2_662_262, 4_294_967_161, 1_428_739_342),
+        (41, 782_766_686, 4_294_966_153, 3_889_040_436),
+        (47, 1_002_938_914, 4_294_967_161, 4_173_217_525),
+    ]:
+        assert planner.is_prime(p) and p % n == 1
+        assert pow(omega, n, p) == 1 and omega != 1
+        assert exact[n] % p == expected, n
