@@ -23,6 +23,7 @@ Printed as one JSON line (rank 0 only; this build runs on one GPU):
 from __future__ import annotations
 
 import argparse
+import re
 import json
 import math
 import os
@@ -118,19 +119,52 @@ def roofline(n: int, group: int, k: int, mulmods_per_s: float) -> dict:
     }
 
 
-def traffic_from_profiles():
-    """HBM bytes per launch of the headline kernel from the committed rocprof
-    counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), else None."""
+def _profile_value(key: str):
+    """A figure of the committed rocprof counter summary of the headline
+    kernel, or None."""
     path = ROOT / "profiles" / "glynn_headline_n32_pmc.txt"
     if not path.exists():
         return None
     for line in path.read_text().splitlines():
-        if line.startswith("hbm_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             try:
                 return float(line.split(":", 1)[1])
             except ValueError:
                 return None
     return None
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the headline kernel (FETCH_SIZE + WRITE_SIZE,
+    copy-calibrated), else None."""
+    return _profile_value("hbm_bytes_per_launch")
+
+
+def valu_int32_utilisation(kernel_ms: float, compute_units: int, rates: dict) -> dict:
+    """Achieved VALU lane operations per second of the headline kernel
+    (SQ_INSTS_VALU per launch from the committed counter profile, 64 lanes
+    per wavefront instruction, over this run's kernel time) against two
+    INT32 peaks of the chip: the nominal one (4 SIMDs per CU, each completing
+    a 64-lane instruction every 4 cycles at the 2.4 GHz peak engine clock)
+    and the fastest 32-bit VALU instruction measured here (v_add_u32 and
+    v_ashrrev_i32 issue above the nominal rate on gfx950).  The kernel's
+    stream is mostly 64-bit and 32-bit multiplies, which issue at about half
+    the add rate, so its tight bound is the instruction-mix roofline."""
+    insts = _profile_value("SQ_INSTS_VALU")
+    if insts is None:
+        return None
+    achieved = insts * 64 / (kernel_ms * 1e-3)
+    nominal = compute_units * 4 * 16 * 2.4e9
+    measured = max(rates.get("v_add_u32", 0.0), rates.get("v_ashrrev_i32", 0.0))
+    return {
+        "achieved_lane_ops_per_s": achieved,
+        "peak_nominal_lane_ops_per_s": nominal,
+        "frac_of_nominal": achieved / nominal,
+        "peak_measured_int32_lane_ops_per_s": measured,
+        "frac_of_measured": achieved / measured if measured else None,
+        "valu_wave_instructions_per_launch": insts,
+        "source": "profiles/glynn_headline_n32_pmc.txt (SQ_INSTS_VALU)",
+    }
 
 
 def cpu_baseline(matrix: np.ndarray, primes, threads: int) -> dict:
@@ -197,6 +231,55 @@ def published_workloads() -> dict:
         if value != known[n]:
             raise RuntimeError(f"perm(S_{n}) is wrong")
         out[f"schur_{n}_exact"] = {"s": dt, "value": str(value), "reference_s": ref}
+    # The reference's device entry point at order 41 plans descending 32-bit
+    # primes = 1 (mod 41) (perm_cuda.py:466-538); the mirror runs the same
+    # plan on the unsigned-Montgomery necklace kernel.
+    import perm_cuda
+
+    t0 = time.perf_counter()
+    value = perm_cuda.permanent_cuda(41, verbose=False)
+    dt = time.perf_counter() - t0
+    if value != known[41]:
+        raise RuntimeError("perm_cuda.permanent_cuda(41) is wrong")
+    out["perm_cuda_permanent_cuda_41"] = {
+        "s": dt, "value": str(value),
+        "primes": perm_cuda._plan_schur_crt_primes(41, perm_cuda.schur_permanent_bound(41)),
+        "reference_s": {"GH200 CUDA 40x40-minor necklace": 47.39},
+    }
+    return out
+
+
+def wide_moduli() -> dict:
+    """Kernel throughput of the wide-modulus Glynn kernels, one modulus of
+    each arithmetic, on a 30x30 matrix (2^29 Gray states, 30 mulmods each);
+    every residue is checked against the exact permanent reduced."""
+    import perm
+    from permhip import _native
+
+    n = 30
+    A = np.random.default_rng(7).integers(-(10 ** 6), 10 ** 6 + 1, (n, n)).astype(np.int64)
+    exact = perm.matrix_permanent(A)
+    cases = {
+        "mont32 q=2^32-5": (1 << 32) - 5,
+        "m61 q=2^61-1": (1 << 61) - 1,
+        "pm61 q=2^61-31": (1 << 61) - 31,
+        "mont64 q=2^61+1": (1 << 61) + 1,
+        "mont64 q=2^63-25": (1 << 63) - 25,
+    }
+    mulmods = n * 2.0 ** (n - 1)
+    out = {}
+    for name, q in cases.items():
+        best = None
+        for _ in range(3):
+            _native.profile_enable(True)
+            ms0, _ = _native.profile_read()
+            got = _native.permanent_residues(A, [q])[0]
+            ms1, _ = _native.profile_read()
+            _native.profile_enable(False)
+            if got != exact % q:
+                raise RuntimeError(f"wide residue mod {q} is wrong")
+            best = ms1 - ms0 if best is None else min(best, ms1 - ms0)
+        out[name] = {"kernel_ms": best, "mulmod_per_s": mulmods / (best * 1e-3)}
     return out
 
 
@@ -274,7 +357,9 @@ def main() -> int:
     primes, product = planner.generic_crt_primes(bound)
     mulmods_per_perm = HEADLINE_N * 2.0 ** (HEADLINE_N - 1) * len(primes)
 
-    fixed = args.items_log2 is not None or args.blocks_per_cu is not None
+    # Profiler runs (--kernel-only) launch only the measured kernel: no tuning
+    # trials, the setting comes from the flags.
+    fixed = args.items_log2 is not None or args.blocks_per_cu is not None or args.kernel_only
     plan = PermanentBatch([A], autotune=False if (args.no_autotune or fixed) else {"reps": 3, "max_seconds": 20.0})
     if args.items_log2 is not None:
         plan.plan.set_option(_native.BATCH_ITEMS_LOG2, args.items_log2)
@@ -333,6 +418,8 @@ def main() -> int:
         f"{int(st['jobs'])} walk(s) of the Gray code)"
     )
     roof["kernel_ms"] = k_ms
+    cus = int(re.search(r"CUs=(\d+)", info).group(1)) if re.search(r"CUs=(\d+)", info) else 256
+    roof["valu_int32"] = valu_int32_utilisation(k_ms, cus, rates)
 
     # Plain launch: one synchronous C-ABI call per (matrix, prime) -- host
     # tables, copy, launch, reduce, copy back -- as the reference wrappers
@@ -442,6 +529,8 @@ def main() -> int:
             log(f"serve: {result['serve']}")
 
     if not args.no_published:
+        result["wide_moduli"] = wide_moduli()
+        log(f"wide moduli: {result['wide_moduli']}")
         result["published_workloads"] = published_workloads()
         log(f"published workloads: {result['published_workloads']}")
 

@@ -4,7 +4,7 @@
 // 2^63, perm_c_omp.c:876-880) and of the reference's device Glynn kernels
 // over 32-bit primes (perm_modp_cuda.cu:328-485).
 //
-// One launch per arithmetic class (wide_arith.h: Mont32, PM61, Mont64) and
+// One launch per arithmetic class (wide_arith.h: Mont32, M61, PM61, Mont64) and
 // order tier; blockIdx.y selects the modulus.  Each lane walks 2^cbits
 // consecutive Gray states with its n row sums in registers (the tier NMAX
 // bounds the unrolled row loops; rows at or beyond n are skipped by a
@@ -137,6 +137,10 @@ Mont32 make_arith<Mont32>(uint64_t q) {
     return make_mont32(q);
 }
 template <>
+M61 make_arith<M61>(uint64_t q) {
+    return make_m61(q);
+}
+template <>
 PM61 make_arith<PM61>(uint64_t q) {
     return make_pm61(q);
 }
@@ -263,7 +267,7 @@ int run_wide_glynn(DeviceContext* ctx, const int64_t* A, int64_t stride, int n, 
         set_error("wide Glynn orders must lie in [2, 64)");
         return -1;
     }
-    std::vector<WideJob> by_class[3];
+    std::vector<WideJob> by_class[4];
     for (const WideJob& j : jobs) {
         if (!(j.q & 1) || j.q < 3 || j.q >= (1ull << 63)) {
             set_error("wide Glynn moduli must be odd and lie in [3, 2^63)");
@@ -272,6 +276,7 @@ int run_wide_glynn(DeviceContext* ctx, const int64_t* A, int64_t stride, int n, 
         by_class[wide_class(j.q)].push_back(j);
     }
     if (run_class<Mont32>(ctx, A, stride, n, by_class[kWideMont32], odd_residue) != 0) return -1;
+    if (run_class<M61>(ctx, A, stride, n, by_class[kWideM61], odd_residue) != 0) return -1;
     if (run_class<PM61>(ctx, A, stride, n, by_class[kWidePM61], odd_residue) != 0) return -1;
     if (run_class<Mont64>(ctx, A, stride, n, by_class[kWideMont64], odd_residue) != 0) return -1;
     return 0;

@@ -5,8 +5,10 @@
 //   Mont32  odd q < 2^32, unsigned Montgomery with R = 2^32.  The 32-bit
 //           primes of the reference's device planner (q = 1 mod n below
 //           2^32, perm_cuda.py:289-302) land here.
-//   PM61    q = 2^61 - c with odd 1 <= c < 2^28: pseudo-Mersenne folding,
-//           no Montgomery factor.  2^61 - 1 (c = 1) is the Mersenne prime.
+//   M61     q = 2^61 - 1, the Mersenne prime: t = H 2^61 + L == H + L, no
+//           multiplication beyond the 122-bit product.
+//   PM61    q = 2^61 - c with odd 3 <= c < 2^28: pseudo-Mersenne folding
+//           (two folds through c), no Montgomery factor.
 //   Mont64  any other odd q < 2^63, unsigned Montgomery with R = 2^64.
 //
 // Every policy keeps values in [0, q); `add` and `mul` take and return
@@ -51,6 +53,35 @@ struct Mont32 {
     }
 };
 
+// The exact 122-bit product of a, b < 2^61 as (hi, lo) 64-bit words.
+PHIP_HD void mul_61x61(uint64_t a, uint64_t b, uint64_t& hi, uint64_t& lo) {
+    const uint64_t a0 = (uint32_t)a, a1 = a >> 32, b0 = (uint32_t)b, b1 = b >> 32;
+    const uint64_t p00 = a0 * b0;
+    const uint64_t mid = a0 * b1 + a1 * b0;  // < 2^62
+    lo = p00 + (mid << 32);
+    hi = a1 * b1 + (mid >> 32) + (lo < p00);
+}
+
+struct M61 {
+    typedef uint64_t T;
+    static constexpr int kRBits = 0;
+    static constexpr uint64_t kMask = (1ull << 61) - 1;
+    uint64_t p;  // 2^61 - 1
+
+    PHIP_HDM T add(T a, T b) const {
+        const T s = a + b;  // < 2^62
+        return s >= p ? s - p : s;
+    }
+    PHIP_HDM T neg(T a) const { return a ? p - a : 0; }
+    PHIP_HDM T mul(T a, T b) const {
+        uint64_t hi, lo;
+        mul_61x61(a, b, hi, lo);
+        uint64_t r = (lo & kMask) + ((hi << 3) | (lo >> 61));  // < 2^62
+        r = (r & kMask) + (r >> 61);                          // <= 2^61
+        return r >= p ? r - p : r;
+    }
+};
+
 struct PM61 {
     typedef uint64_t T;
     static constexpr int kRBits = 0;
@@ -67,11 +98,8 @@ struct PM61 {
     // a b = H 2^61 + L == H c + L (mod p); H c = H2 2^61 + L2 again, with
     // H2 < 2^28; one last fold of the at most two carried bits.
     PHIP_HDM T mul(T a, T b) const {
-        const uint64_t a0 = (uint32_t)a, a1 = a >> 32, b0 = (uint32_t)b, b1 = b >> 32;
-        const uint64_t p00 = a0 * b0;
-        const uint64_t mid = a0 * b1 + a1 * b0;  // < 2^62
-        const uint64_t lo = p00 + (mid << 32);
-        const uint64_t hi = a1 * b1 + (mid >> 32) + (lo < p00);  // product < 2^122
+        uint64_t hi, lo;
+        mul_61x61(a, b, hi, lo);  // product < 2^122
         const uint64_t L = lo & kMask;
         const uint64_t H = (hi << 3) | (lo >> 61);  // < 2^61
         const uint64_t h0c = (uint64_t)(uint32_t)H * c;
@@ -98,7 +126,7 @@ struct Mont64 {
 };
 
 // Arithmetic class of an odd modulus q >= 3 (q < 2^63).
-enum WideClass { kWideMont32 = 0, kWidePM61 = 1, kWideMont64 = 2 };
+enum WideClass { kWideMont32 = 0, kWidePM61 = 1, kWideMont64 = 2, kWideM61 = 3 };
 
 PHIP_HD bool is_pm61(uint64_t q) {
     const uint64_t c = (1ull << 61) - q;
@@ -107,11 +135,13 @@ PHIP_HD bool is_pm61(uint64_t q) {
 
 PHIP_HD int wide_class(uint64_t q) {
     if (q < (1ull << 32)) return kWideMont32;
+    if (q == (1ull << 61) - 1) return kWideM61;
     if (is_pm61(q)) return kWidePM61;
     return kWideMont64;
 }
 
 static inline Mont32 make_mont32(uint64_t q) { return Mont32{(uint32_t)q, 0u - inverse_mod_2_32((uint32_t)q)}; }
+static inline M61 make_m61(uint64_t q) { return M61{q}; }
 static inline PM61 make_pm61(uint64_t q) { return PM61{q, (uint32_t)((1ull << 61) - q), 0u}; }
 static inline Mont64 make_mont64(uint64_t q) { return Mont64{q, 0 - inverse_mod_2_64(q)}; }
 

@@ -3,8 +3,8 @@
 The header's products are written in 32x32 -> 64 steps so that its host
 build runs the device arithmetic; this test compiles it with g++ and checks
 every policy against Python integers on edge values and random operands:
-Mont32 (odd q < 2^32, R = 2^32), PM61 (q = 2^61 - c, odd c < 2^28) and
-Mont64 (odd q < 2^63, R = 2^64)."""
+Mont32 (odd q < 2^32, R = 2^32), M61 (q = 2^61 - 1), PM61 (q = 2^61 - c,
+odd c < 2^28) and Mont64 (odd q < 2^63, R = 2^64)."""
 
 import random
 import shutil
@@ -33,6 +33,9 @@ int main() {
         if (kind[0] == '3') {
             Mont32 ar = make_mont32(q);
             m = ar.mul((uint32_t)a, (uint32_t)b); s = ar.add((uint32_t)a, (uint32_t)b); ng = ar.neg((uint32_t)a);
+        } else if (kind[0] == 'm') {
+            M61 ar = make_m61(q);
+            m = ar.mul(a, b); s = ar.add(a, b); ng = ar.neg(a);
         } else if (kind[0] == 'p') {
             PM61 ar = make_pm61(q);
             m = ar.mul(a, b); s = ar.add(a, b); ng = ar.neg(a);
@@ -88,6 +91,10 @@ def test_mont32(harness):
         _check(harness, "3", q, 32, rng)
 
 
+def test_m61(harness):
+    _check(harness, "m", (1 << 61) - 1, 0, random.Random(4))
+
+
 def test_pm61(harness):
     rng = random.Random(2)
     for c in (1, 3, 31, 1 << 10 | 1, (1 << 28) - 1, 12345679):
@@ -103,7 +110,7 @@ def test_mont64(harness):
 def test_classes(harness):
     cases = {
         3: 0, 2 ** 31 + 1: 0, 2 ** 32 - 1: 0,
-        2 ** 61 - 1: 1, 2 ** 61 - (2 ** 28 - 1): 1,
+        2 ** 61 - 1: 3, 2 ** 61 - 3: 1, 2 ** 61 - (2 ** 28 - 1): 1,
         2 ** 61 - (2 ** 28 + 1): 2, 2 ** 61 + 1: 2, 2 ** 61 - 2: 2, 2 ** 63 - 25: 2, 2 ** 32 + 1: 2,
     }
     rows = harness([f"c {q} 0 0" for q in cases])
