@@ -56,6 +56,18 @@ int phip_device_info(char *buf, size_t len);
 int phip_permanent_residues(const int64_t *A, int64_t prime_stride, int n, const int64_t *moduli,
                             int k, int64_t *out);
 
+/* Coefficients of per(C + x L) modulo each of k odd moduli 3 <= m < 2^63
+ * (not necessarily prime), 0 <= n <= 32.  C and L are row-major int64;
+ * prime_stride as in phip_permanent_residues (0: one pair for every modulus;
+ * n*n: k pairs already reduced by the caller).  out[i*(n+1) + d] receives
+ * the coefficient of x^d modulo moduli[i] in [0, m).  Returns 0 or -1.
+ * Device form of perm_polynomial_ext.affine_permanent_mod
+ * (perm_polynomial_omp.c:1207-1260; perm_polynomial.py:402-443), all
+ * moduli in one pass.  The reference accepts orders up to 63; orders above
+ * 32 (2^32 Gray states of degree-n products) are rejected here. */
+int phip_affine_permanent_residues(const int64_t *C, const int64_t *L, int64_t prime_stride, int n,
+                                   const int64_t *moduli, int k, int64_t *out);
+
 /* perm(A) mod p for one modulus; -1 on failure.
  * Replaces permanent_modp_cuda (perm_modp_cuda.cu:740-742). */
 int64_t phip_permanent_modp(const int64_t *A, int n, int64_t p);
@@ -144,6 +156,7 @@ int phip_batch_destroy(void *plan);
 #define PHIP_TUNE_NECKLACE_MIN_ORDER 5 /* Schur: smallest prime order using necklace orbits (5..64) */
 #define PHIP_TUNE_NECKLACE_DEPTH 6   /* Schur necklaces: largest FKM prefix depth of the tasks (2..30) */
 #define PHIP_TUNE_WIDE_MIN_TIER 7    /* wide-modulus Glynn: smallest row-register tier (16, 32, 48, 64) */
+#define PHIP_TUNE_AFFINE_MIN_TIER 8  /* affine Glynn: smallest row-register tier (8, 16, 24, 32) */
 int phip_set_tuning(int key, int64_t value);
 int64_t phip_get_tuning(int key);
 

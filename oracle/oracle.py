@@ -61,6 +61,8 @@ class _C:
             lib.oracle_mod3.restype = ctypes.c_int
             lib.oracle_mod3.argtypes = [_I64P, ctypes.c_int, ctypes.c_int]
             lib.oracle_max_threads.restype = ctypes.c_int
+            lib.oracle_affine_mod_p.restype = None
+            lib.oracle_affine_mod_p.argtypes = [_I64P, _I64P, ctypes.c_int, ctypes.c_int64, ctypes.c_int, _I64P]
             self._lib = lib
         return self._lib
 
@@ -105,6 +107,18 @@ class _C:
     def mod3(self, matrix, threads: int = 0) -> int:
         a = np.ascontiguousarray(np.asarray(matrix, dtype=np.int64))
         return int(self.lib.oracle_mod3(a.ctypes.data_as(_I64P), a.shape[0], threads))
+
+    def affine_mod_p(self, constant, linear, p: int, threads: int = 0) -> tuple:
+        """Coefficients of per(C + x L) mod odd p, ascending degree."""
+        n = len(constant)
+        if n == 0:
+            return (1 % p,)
+        c = self._reduced(constant, p)
+        l = self._reduced(linear, p)
+        out = np.zeros(n + 1, dtype=np.int64)
+        self.lib.oracle_affine_mod_p(c.ctypes.data_as(_I64P), l.ctypes.data_as(_I64P), n, p, threads,
+                                     out.ctypes.data_as(_I64P))
+        return tuple(int(v) for v in out)
 
     def max_threads(self) -> int:
         return int(self.lib.oracle_max_threads())
@@ -173,7 +187,7 @@ def exact_glynn(matrix) -> int:
 def reference_extensions():
     """{name: module} for the reference's OpenMP extensions in _ref/, or {}."""
     found = {}
-    for name in ("perm_c_ext", "perm_binary_ext", "perm_mod3_omp"):
+    for name in ("perm_c_ext", "perm_binary_ext", "perm_mod3_omp", "perm_polynomial_ext"):
         path = REF_DIR / f"{name}.so"
         if not path.exists():
             continue

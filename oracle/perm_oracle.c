@@ -452,4 +452,86 @@ int oracle_mod3(const int64_t *A, int n, int threads) {
 }
 
 /* Number of OpenMP threads a default parallel region would use. */
+/* Coefficients of per(C + x L) mod p (odd p < 2^63), C and L row-major
+ * n x n already reduced into [0, p): the coefficient-valued fixed-sign Glynn
+ * sum of perm_polynomial.py:268-330 (row sums of both matrices, one affine
+ * factor per row multiplied into a coefficient vector, signed accumulation,
+ * division by 2^(n-1)), split over OpenMP threads by Gray state.  out
+ * receives n + 1 coefficients in ascending degree. */
+void oracle_affine_mod_p(const int64_t *C, const int64_t *L, int n, int64_t p, int threads, int64_t *out) {
+    if (n <= 0) {
+        out[0] = 1 % p;
+        return;
+    }
+    if (n == 1) {
+        out[0] = C[0] % p;
+        out[1] = L[0] % p;
+        return;
+    }
+    const uint64_t states = UINT64_C(1) << (n - 1);
+    int nt = thread_count(threads);
+    if ((uint64_t)nt > states) nt = (int)states;
+    for (int d = 0; d <= n; ++d) out[d] = 0;
+#pragma omp parallel num_threads(nt)
+    {
+        int tid = 0, team = 1;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+        team = omp_get_num_threads();
+#endif
+        uint64_t start, end;
+        split(states, tid, team, &start, &end);
+        int64_t cs[ORACLE_MAX_ORDER + 1], ls[ORACLE_MAX_ORDER + 1];
+        int64_t prod[ORACLE_MAX_ORDER + 2], local[ORACLE_MAX_ORDER + 2];
+        for (int d = 0; d <= n; ++d) local[d] = 0;
+        if (start < end) {
+            for (int i = 0; i < n; ++i) {
+                int64_t a = 0, b = 0;
+                for (int j = 0; j < n; ++j) {
+                    a = addm(a, C[i * n + j], p);
+                    b = addm(b, L[i * n + j], p);
+                }
+                cs[i] = a;
+                ls[i] = b;
+            }
+            uint64_t g = start ^ (start >> 1);
+            for (int j = 0; j < n - 1; ++j) {
+                if ((g >> j) & 1u) {
+                    for (int i = 0; i < n; ++i) {
+                        cs[i] = subm(cs[i], addm(C[i * n + j], C[i * n + j], p), p);
+                        ls[i] = subm(ls[i], addm(L[i * n + j], L[i * n + j], p), p);
+                    }
+                }
+            }
+            int negative = __builtin_popcountll(g) & 1;
+            for (uint64_t s = start; s < end; ++s) {
+                if (s != start) {
+                    const int j = __builtin_ctzll(s);
+                    g = s ^ (s >> 1);
+                    const int sub = (int)((g >> j) & 1u);
+                    for (int i = 0; i < n; ++i) {
+                        const int64_t dc = addm(C[i * n + j], C[i * n + j], p);
+                        const int64_t dl = addm(L[i * n + j], L[i * n + j], p);
+                        cs[i] = sub ? subm(cs[i], dc, p) : addm(cs[i], dc, p);
+                        ls[i] = sub ? subm(ls[i], dl, p) : addm(ls[i], dl, p);
+                    }
+                    negative = !negative;
+                }
+                /* prod = product over rows of (cs[i] + x ls[i]). */
+                prod[0] = 1 % p;
+                for (int i = 0; i < n; ++i) {
+                    prod[i + 1] = mulm(prod[i], ls[i], p);
+                    for (int d = i; d >= 1; --d) prod[d] = addm(mulm(prod[d], cs[i], p), mulm(prod[d - 1], ls[i], p), p);
+                    prod[0] = mulm(prod[0], cs[i], p);
+                }
+                for (int d = 0; d <= n; ++d) local[d] = negative ? subm(local[d], prod[d], p) : addm(local[d], prod[d], p);
+            }
+        }
+#pragma omp critical(oracle_affine)
+        for (int d = 0; d <= n; ++d) out[d] = addm(out[d], local[d], p);
+    }
+    const int64_t inv = powm((p >> 1) + 1, (uint64_t)(n - 1), p);
+    for (int d = 0; d <= n; ++d) out[d] = mulm(out[d], inv, p);
+}
+
 int oracle_max_threads(void) { return thread_count(0); }

@@ -1,0 +1,384 @@
+// Coefficient-valued fixed-sign Glynn: the coefficients of per(C + x L)
+// modulo odd moduli (the reference's affine_permanent_mod contract,
+// perm_polynomial.py:268-330 and the native perm_polynomial_omp.c:681-790,
+// 1207-1260).  Glynn's sum runs over the same 2^(n-1) Gray states as the
+// integer path, with two row sums per row (the constant and the linear
+// part); each state's term is the polynomial prod_i (c_i + x l_i) of degree
+// n, built in registers factor by factor with one fma2 (a b + c d, one
+// Montgomery factor) per coefficient, and added to the lane's n + 1
+// accumulators with the state's sign.
+//
+// One launch per residue arithmetic (wide_arith.h: Mont31L below 2^31 with
+// one lazy reduction per coefficient, Mont32, M61, PM61, Mont64) and
+// register tier (8, 16, 24 or 32 rows); blockIdx.y selects the modulus.
+// As in wide.hip the flipped column and its direction are wave-uniform
+// except at a chunk's middle state, so deltas and negated deltas come by
+// scalar loads.  Each wavefront folds its lanes' accumulators with
+// cross-lane swizzles (modular adds), writes one partial per coefficient,
+// and affine_sum_kernel folds the wavefront partials per (modulus,
+// coefficient).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "affine.h"
+#include "device_context.h"
+#include "phip_arith.h"
+#include "wide_arith.h"
+
+namespace phip {
+
+namespace {
+
+constexpr int kAffineBlock = 256;
+constexpr int kAffineLanesLog2 = 18;
+
+template <class T>
+__device__ __forceinline__ T xor_lane(T v, int mask) {
+    if constexpr (sizeof(T) == 4) {
+        return (T)__shfl_xor((int)v, mask, 64);
+    } else {
+        const int lo = __shfl_xor((int)(uint32_t)v, mask, 64);
+        const int hi = __shfl_xor((int)(uint32_t)((uint64_t)v >> 32), mask, 64);
+        return (T)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+    }
+}
+
+// c0, l0: [q][n] state-0 row sums; dc, ndc, dl, ndl: [q][n-1][n] deltas
+// (2 C, 2 L by column) and their negations; partials: [q][waves][n + 1].
+template <int NMAX, class A>
+__global__ __launch_bounds__(kAffineBlock) void affine_glynn_kernel(
+    const typename A::T* __restrict__ c0, const typename A::T* __restrict__ l0,
+    const typename A::T* __restrict__ dc, const typename A::T* __restrict__ ndc,
+    const typename A::T* __restrict__ dl, const typename A::T* __restrict__ ndl, const A* __restrict__ arith, int n,
+    int cbits, uint64_t lanes, uint64_t waves, uint64_t* __restrict__ partials) {
+    typedef typename A::T T;
+    const int q = blockIdx.y;
+    const A ar = arith[q];
+    const uint64_t lane = (uint64_t)blockIdx.x * kAffineBlock + threadIdx.x;
+    const bool active = lane < lanes;
+    const size_t table = (size_t)q * (n - 1) * n;
+
+    T acc[NMAX + 1];
+#pragma unroll
+    for (int d = 0; d <= NMAX; ++d) acc[d] = 0;
+
+    if (active) {
+        T c[NMAX], l[NMAX];
+        {
+            const ConstPtr<T> pc = scalar_ptr(c0 + (size_t)q * n);
+            const ConstPtr<T> pl = scalar_ptr(l0 + (size_t)q * n);
+#pragma unroll
+            for (int i = 0; i < NMAX; ++i) {
+                c[i] = i < n ? pc[i] : T(0);
+                l[i] = i < n ? pl[i] : T(0);
+            }
+        }
+        const uint64_t base = lane << cbits;
+        const uint64_t gray = base ^ (base >> 1);
+        for (int j = 0; j < n - 1; ++j) {
+            const bool on = (gray >> j) & 1u;
+            const ConstPtr<T> nc = scalar_ptr(ndc + table + (size_t)j * n);
+            const ConstPtr<T> nl = scalar_ptr(ndl + table + (size_t)j * n);
+#pragma unroll
+            for (int i = 0; i < NMAX; ++i) {
+                if (i < n) {
+                    c[i] = ar.add(c[i], on ? nc[i] : T(0));
+                    l[i] = ar.add(l[i], on ? nl[i] : T(0));
+                }
+            }
+        }
+        bool negative = __popcll(gray) & 1;
+        const bool lane_odd = lane & 1u;
+        const uint64_t states = 1ull << cbits;
+        for (uint64_t t = 0; t < states; ++t) {
+            if (t != 0) {
+                const int j = __builtin_ctzll(t);
+                if (j == cbits - 1) {
+                    // Bit j of gray(base + t) is 1 xor the lane's lowest bit.
+                    const bool sub = !lane_odd;
+                    const ConstPtr<T> pc = scalar_ptr(dc + table + (size_t)j * n);
+                    const ConstPtr<T> nc = scalar_ptr(ndc + table + (size_t)j * n);
+                    const ConstPtr<T> pl = scalar_ptr(dl + table + (size_t)j * n);
+                    const ConstPtr<T> nl = scalar_ptr(ndl + table + (size_t)j * n);
+#pragma unroll
+                    for (int i = 0; i < NMAX; ++i) {
+                        if (i < n) {
+                            c[i] = ar.add(c[i], sub ? nc[i] : pc[i]);
+                            l[i] = ar.add(l[i], sub ? nl[i] : pl[i]);
+                        }
+                    }
+                } else {
+                    const bool sub = ((t ^ (t >> 1)) >> j) & 1u;
+                    const ConstPtr<T> pc = scalar_ptr((sub ? ndc : dc) + table + (size_t)j * n);
+                    const ConstPtr<T> pl = scalar_ptr((sub ? ndl : dl) + table + (size_t)j * n);
+#pragma unroll
+                    for (int i = 0; i < NMAX; ++i) {
+                        if (i < n) {
+                            c[i] = ar.add(c[i], pc[i]);
+                            l[i] = ar.add(l[i], pl[i]);
+                        }
+                    }
+                }
+                negative = !negative;
+            }
+            // prod_i (c_i + x l_i), one factor at a time, highest degree first.
+            T P[NMAX + 1];
+            P[0] = c[0];
+            P[1] = l[0];
+#pragma unroll
+            for (int i = 1; i < NMAX; ++i) {
+                if (i < n) {
+                    P[i + 1] = ar.mul(P[i], l[i]);
+#pragma unroll
+                    for (int d = i; d >= 1; --d) P[d] = ar.fma2(P[d], c[i], P[d - 1], l[i]);
+                    P[0] = ar.mul(P[0], c[i]);
+                }
+            }
+#pragma unroll
+            for (int d = 0; d <= NMAX; ++d) {
+                if (d <= n) acc[d] = ar.add(acc[d], negative ? ar.neg(P[d]) : P[d]);
+            }
+        }
+    }
+
+    // Wavefront fold of every coefficient, then one partial per wavefront.
+    const uint64_t wave = lane >> 6;
+#pragma unroll
+    for (int d = 0; d <= NMAX; ++d) {
+        if (d <= n) {
+            T v = acc[d];
+#pragma unroll
+            for (int m = 32; m >= 1; m >>= 1) v = ar.add(v, xor_lane(v, m));
+            if ((threadIdx.x & 63) == 0 && wave < waves) partials[((size_t)q * waves + wave) * (n + 1) + d] = (uint64_t)v;
+        }
+    }
+}
+
+// out[q][d] = sum over wavefronts of partials[q][w][d] modulo moduli[q].
+__global__ __launch_bounds__(256) void affine_sum_kernel(const uint64_t* __restrict__ partials, uint64_t waves,
+                                                         int width, const uint64_t* __restrict__ moduli,
+                                                         uint64_t* __restrict__ out) {
+    __shared__ uint64_t tree[256];
+    const int q = blockIdx.y;
+    const int d = blockIdx.x;
+    const uint64_t p = moduli[q];
+    uint64_t s = 0;
+    for (uint64_t w = threadIdx.x; w < waves; w += 256)
+        s = add_mod_u64(s, partials[((size_t)q * waves + w) * width + d], p);
+    tree[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) tree[threadIdx.x] = add_mod_u64(tree[threadIdx.x], tree[threadIdx.x + k], p);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[(size_t)q * width + d] = tree[0];
+}
+
+template <class A>
+A make_affine_arith(uint64_t q);
+template <>
+Mont31L make_affine_arith<Mont31L>(uint64_t q) {
+    return make_mont31l(q);
+}
+template <>
+Mont32 make_affine_arith<Mont32>(uint64_t q) {
+    return make_mont32(q);
+}
+template <>
+M61 make_affine_arith<M61>(uint64_t q) {
+    return make_m61(q);
+}
+template <>
+PM61 make_affine_arith<PM61>(uint64_t q) {
+    return make_pm61(q);
+}
+template <>
+Mont64 make_affine_arith<Mont64>(uint64_t q) {
+    return make_mont64(q);
+}
+
+template <class A>
+int launch_affine(int tier, dim3 grid, hipStream_t st, const typename A::T* c0, const typename A::T* l0,
+                  const typename A::T* dc, const typename A::T* ndc, const typename A::T* dl,
+                  const typename A::T* ndl, const A* ar, int n, int cbits, uint64_t lanes, uint64_t waves,
+                  uint64_t* part) {
+#define PHIP_AFFINE_LAUNCH(NM)                                                                                  \
+    hipLaunchKernelGGL((affine_glynn_kernel<NM, A>), grid, dim3(kAffineBlock), 0, st, c0, l0, dc, ndc, dl, ndl, \
+                       ar, n, cbits, lanes, waves, part)
+    switch (tier) {
+        case 8: PHIP_AFFINE_LAUNCH(8); break;
+        case 16: PHIP_AFFINE_LAUNCH(16); break;
+        case 24: PHIP_AFFINE_LAUNCH(24); break;
+        default: PHIP_AFFINE_LAUNCH(32); break;
+    }
+#undef PHIP_AFFINE_LAUNCH
+    PHIP_HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// One arithmetic class.  Device buffers 36..39 of the context plus
+// host buffer 7; the class completes before the next reuses them.
+template <class A>
+int run_affine_class(DeviceContext* ctx, const int64_t* C, const int64_t* L, int64_t stride, int n,
+                     const std::vector<AffineJob>& jobs, int min_tier, int64_t* out) {
+    typedef typename A::T T;
+    const int kk = (int)jobs.size();
+    if (kk == 0) return 0;
+    const size_t cols = (size_t)(n - 1) * n;
+    // One host image: c0, l0 ([kk][n] each), then dc, ndc, dl, ndl ([kk][n-1][n] each).
+    std::vector<T> img((size_t)kk * (2 * n + 4 * cols));
+    T* h_c0 = img.data();
+    T* h_l0 = h_c0 + (size_t)kk * n;
+    T* h_dc = h_l0 + (size_t)kk * n;
+    T* h_ndc = h_dc + (size_t)kk * cols;
+    T* h_dl = h_ndc + (size_t)kk * cols;
+    T* h_ndl = h_dl + (size_t)kk * cols;
+    std::vector<A> ar(kk);
+    std::vector<uint64_t> moduli(kk);
+    for (int s = 0; s < kk; ++s) {
+        const uint64_t q = jobs[s].q;
+        const int64_t* Cs = C + (int64_t)jobs[s].index * stride;
+        const int64_t* Ls = L + (int64_t)jobs[s].index * stride;
+        ar[s] = make_affine_arith<A>(q);
+        moduli[s] = q;
+        for (int i = 0; i < n; ++i) {
+            uint64_t rc = 0, rl = 0;
+            for (int j = 0; j < n; ++j) {
+                const uint64_t vc = (uint64_t)host_floor_mod(Cs[(size_t)i * n + j], (int64_t)q);
+                const uint64_t vl = (uint64_t)host_floor_mod(Ls[(size_t)i * n + j], (int64_t)q);
+                rc = add_mod_u64(rc, vc, q);
+                rl = add_mod_u64(rl, vl, q);
+                if (j < n - 1) {
+                    const size_t at = (size_t)s * cols + (size_t)j * n + i;
+                    const uint64_t d2c = add_mod_u64(vc, vc, q), d2l = add_mod_u64(vl, vl, q);
+                    h_dc[at] = (T)d2c;
+                    h_ndc[at] = (T)(d2c ? q - d2c : 0);
+                    h_dl[at] = (T)d2l;
+                    h_ndl[at] = (T)(d2l ? q - d2l : 0);
+                }
+            }
+            h_c0[(size_t)s * n + i] = (T)rc;
+            h_l0[(size_t)s * n + i] = (T)rl;
+        }
+    }
+    const int Lbits = n - 1;
+    const int lanes_log2 = std::min(Lbits, kAffineLanesLog2);
+    const int cbits = Lbits - lanes_log2;
+    const uint64_t lanes = 1ull << lanes_log2;
+    const uint64_t gx = (lanes + kAffineBlock - 1) / kAffineBlock;
+    const uint64_t waves = gx * (kAffineBlock / 64);
+    const int fit = n <= 8 ? 8 : n <= 16 ? 16 : n <= 24 ? 24 : 32;
+    const int tier = std::max(fit, min_tier);
+    const int width = n + 1;
+
+    hipStream_t st = ctx->stream();
+    auto* d_img = (T*)ctx->buffer(36, img.size() * sizeof(T));
+    auto* d_ar = (A*)ctx->buffer(37, ar.size() * sizeof(A));
+    auto* d_mod = (uint64_t*)ctx->buffer(38, moduli.size() * 8);
+    auto* d_part = (uint64_t*)ctx->buffer(39, (size_t)kk * waves * width * 8 + (size_t)kk * width * 8);
+    auto* h_out = (uint64_t*)ctx->host_buffer(7, (size_t)kk * width * 8);
+    if (!d_img || !d_ar || !d_mod || !d_part || !h_out) return -1;
+    uint64_t* d_sum = d_part + (size_t)kk * waves * width;
+    PHIP_HIP_TRY(hipMemcpyAsync(d_img, img.data(), img.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    PHIP_HIP_TRY(hipMemcpyAsync(d_ar, ar.data(), ar.size() * sizeof(A), hipMemcpyHostToDevice, st));
+    PHIP_HIP_TRY(hipMemcpyAsync(d_mod, moduli.data(), moduli.size() * 8, hipMemcpyHostToDevice, st));
+    const T* d_c0 = d_img;
+    const T* d_l0 = d_c0 + (size_t)kk * n;
+    const T* d_dc = d_l0 + (size_t)kk * n;
+    const T* d_ndc = d_dc + (size_t)kk * cols;
+    const T* d_dl = d_ndc + (size_t)kk * cols;
+    const T* d_ndl = d_dl + (size_t)kk * cols;
+    const bool prof = ctx->profiling;
+    if (prof) PHIP_HIP_TRY(hipEventRecord(ctx->prof_start, st));
+    if (launch_affine<A>(tier, dim3((unsigned)gx, (unsigned)kk), st, d_c0, d_l0, d_dc, d_ndc, d_dl, d_ndl, d_ar, n,
+                         cbits, lanes, waves, d_part) != 0)
+        return -1;
+    if (prof) PHIP_HIP_TRY(hipEventRecord(ctx->prof_stop, st));
+    hipLaunchKernelGGL(affine_sum_kernel, dim3((unsigned)width, (unsigned)kk), dim3(256), 0, st, d_part, waves,
+                       width, d_mod, d_sum);
+    PHIP_HIP_TRY(hipGetLastError());
+    PHIP_HIP_TRY(hipMemcpyAsync(h_out, d_sum, (size_t)kk * width * 8, hipMemcpyDeviceToHost, st));
+    PHIP_HIP_TRY(hipStreamSynchronize(st));
+    if (prof) {
+        float ms = 0;
+        PHIP_HIP_TRY(hipEventElapsedTime(&ms, ctx->prof_start, ctx->prof_stop));
+        ctx->profiled_ms += ms;
+        ctx->profiled_launches += 1;
+    }
+    for (int s = 0; s < kk; ++s) {
+        const uint64_t q = jobs[s].q;
+        // n - 1 Montgomery factors per term; Glynn's sum carries 2^(n-1).
+        uint64_t scale = host_powmod((q + 1) / 2, (uint64_t)Lbits, q);
+        if (A::kRBits > 0) {
+            const uint64_t r_mod_q = (uint64_t)((((unsigned __int128)1) << A::kRBits) % q);
+            scale = host_mulmod(scale, host_powmod(r_mod_q, (uint64_t)Lbits, q), q);
+        }
+        for (int d = 0; d < width; ++d)
+            out[(size_t)jobs[s].index * width + d] = (int64_t)host_mulmod(h_out[(size_t)s * width + d], scale, q);
+    }
+    return 0;
+}
+
+}  // namespace
+
+int affine_permanent_residues(DeviceContext* ctx, const int64_t* C, const int64_t* L, int64_t stride, int n,
+                              const int64_t* moduli, int k, int min_tier, int64_t* out) {
+    if (n < 0 || n > kAffineMaxOrder) {
+        set_error("affine permanent orders must lie in [0, 32] on the device");
+        return -1;
+    }
+    if (k < 0 || (k > 0 && (!moduli || !out))) {
+        set_error("invalid modulus list");
+        return -1;
+    }
+    if (n > 0 && (!C || !L)) {
+        set_error("matrix pointer is null");
+        return -1;
+    }
+    for (int i = 0; i < k; ++i) {
+        const int64_t m = moduli[i];
+        if (m < 3 || !(m & 1)) {
+            set_error("affine moduli must be odd and at least 3");
+            return -1;
+        }
+    }
+    const int width = n + 1;
+    if (n == 0) {
+        for (int i = 0; i < k; ++i) out[i] = 1;
+        return 0;
+    }
+    if (n == 1) {
+        for (int i = 0; i < k; ++i) {
+            out[(size_t)i * 2] = host_floor_mod(C[(int64_t)i * stride], moduli[i]);
+            out[(size_t)i * 2 + 1] = host_floor_mod(L[(int64_t)i * stride], moduli[i]);
+        }
+        return 0;
+    }
+    std::vector<AffineJob> by_class[5];
+    for (int i = 0; i < k; ++i) {
+        const uint64_t q = (uint64_t)moduli[i];
+        int cls;
+        if (q < (1ull << 31)) {
+            cls = 0;
+        } else {
+            switch (wide_class(q)) {
+                case kWideMont32: cls = 1; break;
+                case kWideM61: cls = 2; break;
+                case kWidePM61: cls = 3; break;
+                default: cls = 4; break;
+            }
+        }
+        by_class[cls].push_back({i, q});
+    }
+    (void)width;
+    if (run_affine_class<Mont31L>(ctx, C, L, stride, n, by_class[0], min_tier, out) != 0) return -1;
+    if (run_affine_class<Mont32>(ctx, C, L, stride, n, by_class[1], min_tier, out) != 0) return -1;
+    if (run_affine_class<M61>(ctx, C, L, stride, n, by_class[2], min_tier, out) != 0) return -1;
+    if (run_affine_class<PM61>(ctx, C, L, stride, n, by_class[3], min_tier, out) != 0) return -1;
+    if (run_affine_class<Mont64>(ctx, C, L, stride, n, by_class[4], min_tier, out) != 0) return -1;
+    return 0;
+}
+
+}  // namespace phip

@@ -11,6 +11,7 @@
 #include "permhip.h"
 #include "phip_arith.h"
 #include "necklace.h"
+#include "affine.h"
 #include "residues.h"
 
 namespace phip {
@@ -132,6 +133,10 @@ int phip_set_tuning(int key, int64_t value) {
             if (value != 16 && value != 32 && value != 48 && value != 64) break;
             t.wide_min_tier = (int)value;
             return 0;
+        case PHIP_TUNE_AFFINE_MIN_TIER:
+            if (value != 8 && value != 16 && value != 24 && value != 32) break;
+            t.affine_min_tier = (int)value;
+            return 0;
         default:
             phip::set_error("unknown tuning key");
             return -1;
@@ -150,6 +155,7 @@ int64_t phip_get_tuning(int key) {
         case PHIP_TUNE_NECKLACE_MIN_ORDER: return t.necklace_min_order;
         case PHIP_TUNE_NECKLACE_DEPTH: return t.necklace_depth;
         case PHIP_TUNE_WIDE_MIN_TIER: return t.wide_min_tier;
+        case PHIP_TUNE_AFFINE_MIN_TIER: return t.affine_min_tier;
         default:
             phip::set_error("unknown tuning key");
             return -1;
@@ -185,6 +191,15 @@ int phip_permanent_residues(const int64_t* A, int64_t prime_stride, int n, const
     if (!ctx) return -1;
     std::lock_guard<std::mutex> lock(ctx->mutex());
     return phip::compute_residues(ctx, A, prime_stride, n, moduli, k, out);
+}
+
+int phip_affine_permanent_residues(const int64_t* C, const int64_t* L, int64_t prime_stride, int n,
+                                   const int64_t* moduli, int k, int64_t* out) {
+    DeviceContext* ctx = DeviceContext::current();
+    if (!ctx) return -1;
+    std::lock_guard<std::mutex> lock(ctx->mutex());
+    return phip::affine_permanent_residues(ctx, C, L, prime_stride, n, moduli, k, phip::tuning().affine_min_tier,
+                                           out);
 }
 
 int64_t phip_permanent_modp(const int64_t* A, int n, int64_t p) {

@@ -31,17 +31,6 @@ namespace {
 constexpr int kWideBlock = 256;
 constexpr int kWideLanesLog2 = 20;
 
-template <class T>
-using ConstPtr = const __attribute__((address_space(4))) T*;
-
-// Hides the pointer's provenance so loads through it stay scalar loads of a
-// read-only table.
-template <class T>
-__device__ __forceinline__ ConstPtr<T> scalar_ptr(const T* p) {
-    asm volatile("" : "+s"(p));
-    return (ConstPtr<T>)p;
-}
-
 template <int NMAX, class A>
 __global__ __launch_bounds__(kWideBlock) void wide_glynn_kernel(const typename A::T* __restrict__ rs0,
                                                                 const typename A::T* __restrict__ deltas,

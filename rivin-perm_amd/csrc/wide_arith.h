@@ -2,6 +2,8 @@
 // take (q >= 2^31).  Three policies share one interface so the wide Glynn
 // kernel is written once:
 //
+//   Mont31L odd q < 2^31, unsigned Montgomery with R = 2^32 and one lazy
+//           reduction of a sum of two products (polynomial kernels).
 //   Mont32  odd q < 2^32, unsigned Montgomery with R = 2^32.  The 32-bit
 //           primes of the reference's device planner (q = 1 mod n below
 //           2^32, perm_cuda.py:289-302) land here.
@@ -31,6 +33,32 @@
 
 namespace phip {
 
+// Every policy also has fma2(a, b, c, d) = a b + c d with one Montgomery
+// factor, the step of a polynomial product; Mont31L reduces the exact sum
+// once, the others reduce each product.
+
+// Unsigned Montgomery, R = 2^32, odd p < 2^31: a b + c d < 2 p^2 < p 2^32,
+// so the sum of two products takes a single REDC.
+struct Mont31L {
+    typedef uint32_t T;
+    static constexpr int kRBits = 32;
+    uint32_t p;
+    uint32_t np;  // -p^-1 mod 2^32
+
+    PHIP_HDM T add(T a, T b) const {
+        const T s = a + b;  // < 2^32
+        return s >= p ? s - p : s;
+    }
+    PHIP_HDM T neg(T a) const { return a ? p - a : 0; }
+    PHIP_HDM T redc(uint64_t t) const {  // t < p 2^32 < 2^63: no carry
+        const uint32_t m = (uint32_t)t * np;
+        const uint64_t r = (t + (uint64_t)m * p) >> 32;
+        return (T)(r >= p ? r - p : r);
+    }
+    PHIP_HDM T mul(T a, T b) const { return redc((uint64_t)a * b); }
+    PHIP_HDM T fma2(T a, T b, T c, T d) const { return redc((uint64_t)a * b + (uint64_t)c * d); }
+};
+
 struct Mont32 {
     typedef uint32_t T;
     static constexpr int kRBits = 32;
@@ -51,6 +79,7 @@ struct Mont32 {
         const uint64_t r = (s >> 32) | ((uint64_t)(s < t) << 32);
         return (T)(r >= p ? r - p : r);
     }
+    PHIP_HDM T fma2(T a, T b, T c, T d) const { return add(mul(a, b), mul(c, d)); }
 };
 
 // The exact 122-bit product of a, b < 2^61 as (hi, lo) 64-bit words.
@@ -80,6 +109,7 @@ struct M61 {
         r = (r & kMask) + (r >> 61);                          // <= 2^61
         return r >= p ? r - p : r;
     }
+    PHIP_HDM T fma2(T a, T b, T c, T d) const { return add(mul(a, b), mul(c, d)); }
 };
 
 struct PM61 {
@@ -112,6 +142,7 @@ struct PM61 {
         r = (r & kMask) + (r >> 61) * c;                // < 2^61 + 3 c < 2 p
         return r >= p ? r - p : r;
     }
+    PHIP_HDM T fma2(T a, T b, T c, T d) const { return add(mul(a, b), mul(c, d)); }
 };
 
 struct Mont64 {
@@ -123,6 +154,7 @@ struct Mont64 {
     PHIP_HDM T add(T a, T b) const { return add_mod_u64(a, b, p); }
     PHIP_HDM T neg(T a) const { return a ? p - a : 0; }
     PHIP_HDM T mul(T a, T b) const { return mont63_mul(a, b, p, np); }
+    PHIP_HDM T fma2(T a, T b, T c, T d) const { return add(mul(a, b), mul(c, d)); }
 };
 
 // Arithmetic class of an odd modulus q >= 3 (q < 2^63).
@@ -140,9 +172,26 @@ PHIP_HD int wide_class(uint64_t q) {
     return kWideMont64;
 }
 
+static inline Mont31L make_mont31l(uint64_t q) {
+    return Mont31L{(uint32_t)q, 0u - inverse_mod_2_32((uint32_t)q)};
+}
 static inline Mont32 make_mont32(uint64_t q) { return Mont32{(uint32_t)q, 0u - inverse_mod_2_32((uint32_t)q)}; }
 static inline M61 make_m61(uint64_t q) { return M61{q}; }
 static inline PM61 make_pm61(uint64_t q) { return PM61{q, (uint32_t)((1ull << 61) - q), 0u}; }
 static inline Mont64 make_mont64(uint64_t q) { return Mont64{q, 0 - inverse_mod_2_64(q)}; }
+
+#if defined(__HIPCC__)
+// Read-only tables read through the constant address space: a uniform
+// address then compiles to scalar loads.  The empty asm hides the pointer's
+// provenance so the compiler keeps them scalar.
+template <class T>
+using ConstPtr = const __attribute__((address_space(4))) T*;
+
+template <class T>
+__device__ __forceinline__ ConstPtr<T> scalar_ptr(const T* p) {
+    asm volatile("" : "+s"(p));
+    return (ConstPtr<T>)p;
+}
+#endif
 
 }  // namespace phip

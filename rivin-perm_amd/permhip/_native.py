@@ -38,6 +38,10 @@ _SIGNATURES = {
         [_I64P, ctypes.c_int64, ctypes.c_int, _I64P, ctypes.c_int, _I64P],
     ),
     "phip_permanent_modp": (ctypes.c_int64, [_I64P, ctypes.c_int, ctypes.c_int64]),
+    "phip_affine_permanent_residues": (
+        ctypes.c_int,
+        [_I64P, _I64P, ctypes.c_int64, ctypes.c_int, _I64P, ctypes.c_int, _I64P],
+    ),
     "phip_schur_permanent_modp": (ctypes.c_int64, [ctypes.c_int, ctypes.c_int64, ctypes.c_int64]),
     "phip_binary_permanent_residues": (
         ctypes.c_int,
@@ -72,6 +76,7 @@ TUNE_BATCH_BLOCKS = 4
 TUNE_NECKLACE_MIN_ORDER = 5
 TUNE_NECKLACE_DEPTH = 6
 TUNE_WIDE_MIN_TIER = 7
+TUNE_AFFINE_MIN_TIER = 8
 BATCH_BLOCKS_PER_CU = 1
 BATCH_USE_GRAPH = 2
 BATCH_ITEMS_LOG2 = 3
@@ -185,6 +190,35 @@ def permanent_residues(matrix: np.ndarray, moduli) -> list:
     status = lib.phip_permanent_residues(_i64ptr(a), stride, n, _i64ptr(mods), k, _i64ptr(out))
     _check(status, "phip_permanent_residues")
     return [int(v) for v in out]
+
+
+def affine_permanent_residues(constant: np.ndarray, linear: np.ndarray, moduli) -> list:
+    """Coefficient tuples of per(C + x L) modulo each modulus, one device
+    pass.  C and L are (n, n) int64 arrays used for every modulus, or
+    (k, n, n) arrays holding one already-reduced pair per modulus."""
+    lib = require_device()
+    c = np.ascontiguousarray(constant, dtype=np.int64)
+    l = np.ascontiguousarray(linear, dtype=np.int64)
+    if c.shape != l.shape:
+        raise ValueError("constant and linear matrices must have the same shape")
+    mods = np.ascontiguousarray(np.asarray(list(moduli), dtype=np.int64))
+    k = int(mods.shape[0])
+    if c.ndim == 2:
+        n = int(c.shape[0])
+        stride = 0
+    elif c.ndim == 3:
+        if c.shape[0] != k:
+            raise ValueError("one matrix pair per modulus is required")
+        n = int(c.shape[1])
+        stride = n * n
+    else:
+        raise ValueError("matrices must be 2- or 3-dimensional")
+    if n == 0:
+        c = l = np.zeros(1, dtype=np.int64)
+    out = np.zeros(max(1, k * (n + 1)), dtype=np.int64)
+    status = lib.phip_affine_permanent_residues(_i64ptr(c), _i64ptr(l), stride, n, _i64ptr(mods), k, _i64ptr(out))
+    _check(status, "phip_affine_permanent_residues")
+    return [tuple(int(v) for v in out[i * (n + 1):(i + 1) * (n + 1)]) for i in range(k)]
 
 
 def schur_permanent_modp(n: int, omega: int, p: int) -> int:

@@ -249,6 +249,115 @@ class HipMod3Extension:
         return _native.permanent_mod3(reduced)
 
 
+def _poly_matrix(obj, name: str, expected_n: int = -1):
+    # parse_matrix (perm_polynomial_omp.c:165-240): a sequence of square rows
+    # of int64 integers, order below 64.
+    if isinstance(obj, (str, bytes)) or not hasattr(obj, "__len__") or not hasattr(obj, "__getitem__"):
+        raise TypeError(f"{name} must be a sequence of rows")
+    rows = list(obj)
+    n = len(rows)
+    if n >= 64:
+        raise ValueError(f"{name} order must be less than 64")
+    if expected_n >= 0 and n != expected_n:
+        raise ValueError("constant_matrix and linear_matrix must have the same order")
+    out = np.zeros((n, n), dtype=np.int64)
+    for i, row in enumerate(rows):
+        if isinstance(row, (str, bytes)) or not hasattr(row, "__len__") or not hasattr(row, "__getitem__"):
+            raise TypeError("matrix rows must be sequences")
+        items = list(row)
+        if len(items) != n:
+            raise ValueError(f"{name} must be square")
+        for j, entry in enumerate(items):
+            try:
+                value = operator.index(entry)
+            except TypeError as exc:
+                raise TypeError("matrix entries must be integers") from exc
+            if not _INT64_MIN <= value <= _INT64_MAX:
+                raise OverflowError("Python int too large to convert to C long long")
+            out[i, j] = value
+    return out
+
+
+def _poly_threads(num_threads) -> None:
+    if num_threads is None:
+        return
+    if isinstance(num_threads, (bool, np.bool_)):
+        raise TypeError("num_threads must be an integer")
+    value = _index_long_long(num_threads, "num_threads")
+    if value < 0:
+        raise ValueError("num_threads must be non-negative")
+    if value > _INT32_MAX:
+        raise ValueError("num_threads is too large")
+
+
+_ROOK_MAX_MODULI = 16
+_WEIGHTED_ROOK_MAX_ENTRY = (1 << 16) - 1
+
+
+class HipPolynomialExtension:
+    """GPU counterpart of ``perm_polynomial_ext`` (perm_polynomial_omp.c:
+    1207-1260, 1706-1836): coefficient vectors of per(C + x L) modulo odd
+    moduli, and rook / weighted-rook residues, from the coefficient-valued
+    Glynn kernel (phip_affine_permanent_residues)."""
+
+    __name__ = "permhip.perm_polynomial_ext"
+
+    def affine_permanent_mod(self, constant_matrix, linear_matrix, modulus, num_threads=None):
+        m = _index_long_long(modulus, "modulus")
+        if isinstance(modulus, (bool, np.bool_)):
+            raise TypeError("modulus must be an integer")
+        if m <= 1 or m % 2 == 0:
+            raise ValueError("modulus must be an odd signed 64-bit integer greater than 1")
+        _poly_threads(num_threads)
+        C = _poly_matrix(constant_matrix, "constant_matrix")
+        L = _poly_matrix(linear_matrix, "linear_matrix", C.shape[0])
+        n = C.shape[0]
+        if n == 0:
+            return (1 % m,)
+        if n == 1:
+            return (int(C[0, 0]) % m, int(L[0, 0]) % m)
+        return _native.affine_permanent_residues(C, L, [m])[0]
+
+    def _rook_many(self, matrix, moduli, num_threads, binary: bool):
+        _poly_threads(num_threads)
+        A = _poly_matrix(matrix, "matrix")
+        n = A.shape[0]
+        if binary and not np.isin(A, (0, 1)).all():
+            raise ValueError("matrix entries must be 0 or 1")
+        if not binary and ((A < 0).any() or (A > _WEIGHTED_ROOK_MAX_ENTRY).any()):
+            raise ValueError(f"matrix entries must be between 0 and {_WEIGHTED_ROOK_MAX_ENTRY}")
+        try:
+            mods = [_index_long_long(q, "modulus") for q in moduli]
+        except TypeError as exc:
+            raise TypeError("moduli must be a sequence of integers") from exc
+        if not 1 <= len(mods) <= _ROOK_MAX_MODULI:
+            raise ValueError(f"moduli must hold between 1 and {_ROOK_MAX_MODULI} primes")
+        for q in mods:
+            if q <= max(n, 2) or q % 2 == 0 or not _is_prime_u64(q):
+                raise ValueError("moduli must be odd primes greater than the matrix order")
+        if n == 0:
+            return tuple((1,) for _ in mods)
+        if n == 1:
+            return tuple((1, int(A[0, 0]) % q) for q in mods)
+        # r_k = [x^k] per(J + x A) / (n - k)!
+        J = np.ones((n, n), dtype=np.int64)
+        rows = _native.affine_permanent_residues(J, A, mods)
+        out = []
+        for q, row in zip(mods, rows):
+            fact = [1] * (n + 1)
+            for i in range(1, n + 1):
+                fact[i] = fact[i - 1] * i % q
+            out.append(tuple(c * pow(fact[n - k], -1, q) % q for k, c in enumerate(row)))
+        return tuple(out)
+
+    def rook_polynomial_mod_many(self, matrix, moduli, num_threads=None):
+        return self._rook_many(matrix, moduli, num_threads, True)
+
+    def weighted_rook_polynomial_mod_many(self, matrix, moduli, num_threads=None):
+        return self._rook_many(matrix, moduli, num_threads, False)
+
+
 perm_c_ext = HipPermanentExtension()
 perm_binary_ext = HipBinaryExtension()
 perm_mod3_omp = HipMod3Extension()
+perm_polynomial_ext = HipPolynomialExtension()

@@ -37,6 +37,7 @@ PLAIN_SOURCES = (
     "microbench.hip",
     "necklace.hip",
     "wide.hip",
+    "affine.hip",
 )
 
 
