@@ -203,6 +203,69 @@ def cpu_baseline(matrix: np.ndarray, primes, threads: int) -> dict:
     }
 
 
+POLY_N = 26
+POLY_SEED = 9
+
+
+def polynomial_board() -> list:
+    rng = np.random.default_rng(POLY_SEED)
+    return rng.integers(0, 2, (POLY_N, POLY_N)).tolist()
+
+
+def polynomial_leg() -> dict:
+    """Exact rook polynomial of a seeded 26x26 0/1 board through the
+    perm_polynomial mirror, and the coefficient-valued Glynn kernel's rate
+    (n - 1)(n + 2) modular products per Gray state per prime: each of the
+    n - 1 affine factors after the first costs one fma2 (two products) per
+    lower coefficient plus two plain products."""
+    import perm_polynomial
+    from permhip import _native
+
+    A = polynomial_board()
+    t0 = time.perf_counter()
+    rook = perm_polynomial.rook_polynomial(A)
+    dt = time.perf_counter() - t0
+    n = POLY_N
+    ones = np.ones((n, n), dtype=np.int64)
+    primes = [2_147_483_647, 2_147_483_629, 2_147_483_587, 2_147_483_579]
+    _native.profile_enable(True)
+    _native.affine_permanent_residues(ones, np.array(A, dtype=np.int64), primes)
+    ms, _ = _native.profile_read()
+    _native.profile_enable(False)
+    products = (n - 1) * (n + 2) * 2.0 ** (n - 1) * len(primes)
+    return {
+        "workload": f"rook polynomial of a {n}x{n} 0/1 board, default_rng({POLY_SEED}).integers(0, 2)",
+        "s_exact": dt,
+        "r_n_permanent": str(rook[n]),
+        "kernel_ms_4_primes": ms,
+        "products_per_s": products / (ms * 1e-3),
+    }
+
+
+def polynomial_cpu_baseline(threads: int) -> dict:
+    """The reference's own perm_polynomial_ext.affine_permanent_mod (built
+    from its source into oracle/_ref) on per(J + x A) of the same board,
+    one prime, or the oracle's restatement when that is not built."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle  # checker/baseline only
+
+    A = polynomial_board()
+    n = POLY_N
+    ones = [[1] * n for _ in range(n)]
+    p = 2_147_483_647
+    ext = oracle.reference_extensions().get("perm_polynomial_ext")
+    if ext is not None:
+        kind, fn = "reference", (lambda: ext.affine_permanent_mod(ones, A, p, threads))
+    else:
+        kind, fn = "port", (lambda: oracle.C.affine_mod_p(ones, A, p, threads))
+    t0 = time.perf_counter()
+    fn()
+    dt = time.perf_counter() - t0
+    products = (n - 1) * (n + 2) * 2.0 ** (n - 1)
+    return {"value": products / dt, "unit": "modular products/s", "cores": threads, "kind": kind,
+            "sample": f"affine_permanent_mod(J, A, {p}) on the {n}x{n} board, one prime, {dt:.2f} s"}
+
+
 def published_workloads() -> dict:
     """End-to-end times (host planning, device residues, CRT) of workloads
     the reference publishes timings for, next to its numbers
@@ -529,6 +592,8 @@ def main() -> int:
             log(f"serve: {result['serve']}")
 
     if not args.no_published:
+        result["polynomial"] = polynomial_leg()
+        log(f"polynomial: {result['polynomial']}")
         result["wide_moduli"] = wide_moduli()
         log(f"wide moduli: {result['wide_moduli']}")
         result["published_workloads"] = published_workloads()
@@ -538,6 +603,9 @@ def main() -> int:
         threads = min(16, os.cpu_count() or 1)
         result["cpu_baseline"] = cpu_baseline(A, primes, threads)
         log(f"cpu baseline: {result['cpu_baseline']}")
+        if "polynomial" in result:
+            result["polynomial"]["cpu_baseline"] = polynomial_cpu_baseline(threads)
+            log(f"polynomial cpu baseline: {result['polynomial']['cpu_baseline']}")
 
     if rank == 0:
         print(json.dumps(result), flush=True)
