@@ -68,6 +68,16 @@ int phip_permanent_residues(const int64_t *A, int64_t prime_stride, int n, const
 int phip_affine_permanent_residues(const int64_t *C, const int64_t *L, int64_t prime_stride, int n,
                                    const int64_t *moduli, int k, int64_t *out);
 
+/* Host preprocessing (no device work): a perfect matching of the 0/1 matrix
+ * given as n <= 63 row masks (bit j = column j) and the irreducible blocks
+ * of its matching digraph, in the order and local column order of
+ * perm_binary.py's _matching_blocks (perm_binary.py:709-752).  Returns 1
+ * with *block_count blocks (block_sizes[b] rows each; block_masks holds
+ * their rows' local masks, block after block, n words in all), 0 when no
+ * perfect matching exists, -1 on invalid input. */
+int phip_binary_blocks(const uint64_t *masks, int n, int *block_count, int *block_sizes,
+                       uint64_t *block_masks);
+
 /* perm(A) mod p for one modulus; -1 on failure.
  * Replaces permanent_modp_cuda (perm_modp_cuda.cu:740-742). */
 int64_t phip_permanent_modp(const int64_t *A, int n, int64_t p);

@@ -30,7 +30,8 @@ from typing import Sequence
 
 import numpy as np
 
-from permhip import _native, planner
+from permhip import _native
+from permhip import binary_prep as _prep
 from permhip.backends import perm_polynomial_ext as _polynomial_ext
 
 _INT64_MAX = (1 << 63) - 1
@@ -220,15 +221,13 @@ def affine_permanent_bounds(constant_matrix, linear_matrix) -> tuple:
 def _descending_primes(target: int, minimum: int = 3) -> list:
     """Descending primes from 2^31 - 1 until their product exceeds target
     (perm_polynomial.py:929-945), all above `minimum`."""
-    primes, product, candidate = [], 1, _CRT_PRIME_START
+    primes, product = [], 1
     while product <= target:
-        while candidate > minimum and not planner.is_prime(candidate):
-            candidate -= 2
-        if candidate <= minimum:
+        p = _prep.descending_prime(len(primes))
+        if p <= minimum:
             raise RuntimeError("exhausted the deterministic CRT prime range")
-        primes.append(candidate)
-        product *= candidate
-        candidate -= 2
+        primes.append(p)
+        product *= p
     return primes
 
 

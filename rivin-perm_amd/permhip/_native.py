@@ -38,6 +38,11 @@ _SIGNATURES = {
         [_I64P, ctypes.c_int64, ctypes.c_int, _I64P, ctypes.c_int, _I64P],
     ),
     "phip_permanent_modp": (ctypes.c_int64, [_I64P, ctypes.c_int, ctypes.c_int64]),
+    "phip_binary_blocks": (
+        ctypes.c_int,
+        [ctypes.POINTER(ctypes.c_uint64), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint64)],
+    ),
     "phip_affine_permanent_residues": (
         ctypes.c_int,
         [_I64P, _I64P, ctypes.c_int64, ctypes.c_int, _I64P, ctypes.c_int, _I64P],
@@ -190,6 +195,27 @@ def permanent_residues(matrix: np.ndarray, moduli) -> list:
     status = lib.phip_permanent_residues(_i64ptr(a), stride, n, _i64ptr(mods), k, _i64ptr(out))
     _check(status, "phip_permanent_residues")
     return [int(v) for v in out]
+
+
+def binary_blocks(masks):
+    """(feasible, blocks as tuples of local row masks) of a 0/1 matrix given
+    as row masks: host code in the library, no device needed."""
+    lib = load()
+    n = len(masks)
+    arr = (ctypes.c_uint64 * max(1, n))(*[int(m) for m in masks])
+    count = ctypes.c_int()
+    sizes = (ctypes.c_int * max(1, n))()
+    out = (ctypes.c_uint64 * max(1, n))()
+    status = lib.phip_binary_blocks(arr, n, ctypes.byref(count), sizes, out)
+    _check(status, "phip_binary_blocks")
+    if status == 0:
+        return False, ()
+    blocks, pos = [], 0
+    for b in range(count.value):
+        k = sizes[b]
+        blocks.append(tuple(int(out[pos + i]) for i in range(k)))
+        pos += k
+    return True, tuple(blocks)
 
 
 def affine_permanent_residues(constant: np.ndarray, linear: np.ndarray, moduli) -> list:

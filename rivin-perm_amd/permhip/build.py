@@ -38,6 +38,7 @@ PLAIN_SOURCES = (
     "necklace.hip",
     "wide.hip",
     "affine.hip",
+    "host_prep.cpp",
 )
 
 

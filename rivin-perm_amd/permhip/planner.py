@@ -9,8 +9,11 @@ lift.  Each function cites the reference function it restates.
 from __future__ import annotations
 
 import math
+import threading
 from numbers import Integral
 from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
 
 INT64_MAX = (1 << 63) - 1
 GENERIC_PRIME_FLOOR = 1 << 30  # perm.py:1121
@@ -180,10 +183,36 @@ def matrix_permanent_bound(rows: Sequence[Sequence[int]]) -> int:
     """prod_i max(1, sum_j |a_ij|) (perm.py:939-953)."""
     if len(rows) == 0:
         return 1
+    if isinstance(rows, np.ndarray) and rows.dtype == np.int64 and rows.ndim == 2:
+        # Row sums of |a| stay exact in int64 while n * max|a| < 2^63.
+        top = int(np.abs(rows.astype(np.float64)).max(initial=0.0))
+        if top < (1 << 62) // max(1, rows.shape[1]):
+            bound = 1
+            for v in np.abs(rows).sum(axis=1).tolist():
+                bound *= max(1, int(v))
+            return bound
     bound = 1
     for row in rows:
         bound *= max(1, sum(abs(int(x)) for x in row))
     return bound
+
+
+_ASCENDING_PRIMES: List[int] = []
+_ASCENDING_PRIMES_LOCK = threading.Lock()
+
+
+def _ascending_prime(index: int) -> int:
+    """The index-th prime at or above 2^30 (the sequence is fixed, so it
+    is found once per process and reused by every plan)."""
+    if index < len(_ASCENDING_PRIMES):
+        return _ASCENDING_PRIMES[index]
+    with _ASCENDING_PRIMES_LOCK:
+        while len(_ASCENDING_PRIMES) <= index:
+            candidate = _ASCENDING_PRIMES[-1] + 1 if _ASCENDING_PRIMES else GENERIC_PRIME_FLOOR
+            while not is_prime(candidate):
+                candidate += 1
+            _ASCENDING_PRIMES.append(candidate)
+        return _ASCENDING_PRIMES[index]
 
 
 def generic_crt_primes(bound: int) -> Tuple[List[int], int]:
@@ -191,12 +220,10 @@ def generic_crt_primes(bound: int) -> Tuple[List[int], int]:
     (perm.py:1115-1128).  Returns (primes, product)."""
     target = 2 * bound + 1
     primes, product = [], 1
-    candidate = GENERIC_PRIME_FLOOR
     while product < target:
-        if is_prime(candidate):
-            primes.append(candidate)
-            product *= candidate
-        candidate += 1
+        p = _ascending_prime(len(primes))
+        primes.append(p)
+        product *= p
     return primes, product
 
 

@@ -72,9 +72,9 @@ def _plan_integer(matrix) -> tuple:
         return [], 1
     if n == 1:
         return [], rows[0][0]
-    bound = planner.matrix_permanent_bound(rows)
-    primes, _ = planner.generic_crt_primes(bound)
     arr = np.array(rows, dtype=np.int64)
+    bound = planner.matrix_permanent_bound(arr)
+    primes, _ = planner.generic_crt_primes(bound)
     return [_Part(arr, primes, "signed", bound)], 1
 
 
@@ -117,7 +117,7 @@ class PermanentServer:
     """Background scheduler; ``submit`` returns a Future of the exact integer."""
 
     def __init__(self, *, max_inflight: int = 3, plan_cost: float = 4e11,
-                 plan_memory: int = 1 << 30, blocks_per_cu: int = 2):
+                 plan_memory: int = 1 << 30, blocks_per_cu: int = 3):
         _native.require_device()
         self.max_inflight = max_inflight
         self.plan_cost = plan_cost
@@ -251,6 +251,39 @@ class PermanentServer:
                 time.sleep(0.0002)
 
 
+def latency_floor(count: int = 40, seed: int = 6) -> dict:
+    """Median end-to-end latency of small requests submitted one at a time
+    to an idle server (integer 12x12 entries U[-50,50], 0/1 20x20 density
+    0.5), next to the median host planning time of the same requests
+    (bound, primes, and for 0/1 matrices the matching and block split)."""
+    import statistics
+
+    rng = np.random.default_rng(seed)
+    server = PermanentServer(max_inflight=1)
+    out = {}
+    try:
+        for kind, n in (("integer", 12), ("binary", 20)):
+            lat, plan = [], []
+            for _ in range(count):
+                if kind == "integer":
+                    M = rng.integers(-50, 51, (n, n)).astype(np.int64)
+                    planner_fn = _plan_integer
+                else:
+                    M = (rng.random((n, n)) < 0.5).astype(np.int64)
+                    planner_fn = _plan_binary
+                t0 = time.perf_counter()
+                planner_fn(M)
+                plan.append(time.perf_counter() - t0)
+                t0 = time.perf_counter()
+                server.submit(M, kind).result()
+                lat.append(time.perf_counter() - t0)
+            out[f"{kind}_n{n}"] = {"p50_ms": 1e3 * statistics.median(lat),
+                                   "host_planning_p50_ms": 1e3 * statistics.median(plan)}
+    finally:
+        server.close()
+    return out
+
+
 def benchmark(seconds: float = 2.0, rate: float = 200.0, seed: int = 5, verify: int = 24) -> dict:
     """Replay the seeded Poisson stream against a live server; report
     latency percentiles, throughput and backlog, and check a sample of
@@ -305,4 +338,5 @@ def benchmark(seconds: float = 2.0, rate: float = 200.0, seed: int = 5, verify: 
         "plans_launched": server.plans_launched,
         "verified_alone_vs_mix": int(len(idx)),
         "offered_gray_mulmods_per_prime": mulmods,
+        "small_request_latency": latency_floor(),
     }
