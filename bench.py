@@ -60,7 +60,7 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def instruction_mix(n: int, group: int, k: int = 1) -> dict:
+def instruction_mix(n: int, group: int, k: int = 1, paired: bool = False) -> dict:
     """VALU instructions per Gray state of one walk of gray_walk<n, group>
     serving a set of k primes (packed rows; the unrolled 16-state step
     amortised, as counted in the compiled loop).
@@ -70,7 +70,10 @@ def instruction_mix(n: int, group: int, k: int = 1) -> dict:
     select), the exact group products, and for k > 1 the exact 64-bit
     products of adjacent groups.  Per prime: the Montgomery tree (two
     v_mad_i64_i32 and one v_mul_lo_u32 per product; one of each per reduced
-    pair product) and the in-order signed accumulation (one v_mad_i64_i32)."""
+    pair product) and the in-order signed accumulation (one v_mad_i64_i32).
+    Paired sets replace, per two states, the tree's two last products and two
+    accumulations by two exact products, one reduction and one accumulation,
+    and negate one group per even state."""
     if group == 0:
         raise ValueError("the roofline mix covers packed row sums")
     groups = -(-n // group)
@@ -86,12 +89,17 @@ def instruction_mix(n: int, group: int, k: int = 1) -> dict:
     else:
         pairs = groups // 2
         leaves = pairs + groups % 2
-        mix["v_mad_i64_i32"] = pairs + k * (pairs + 2 * (leaves - 1) + 1)
-        mix["v_mul_lo_u32"] = k * (pairs + leaves - 1)
+        if paired:
+            mix["v_mad_i64_i32"] = pairs + k * (pairs + 2 * (leaves - 2) + 2)
+            mix["v_mul_lo_u32"] = k * (pairs + leaves - 2 + 0.5)
+            mix["v_add_u32"] += 0.5
+        else:
+            mix["v_mad_i64_i32"] = pairs + k * (pairs + 2 * (leaves - 1) + 1)
+            mix["v_mul_lo_u32"] = k * (pairs + leaves - 1)
     return mix
 
 
-def roofline(n: int, group: int, k: int, mulmods_per_s: float) -> dict:
+def roofline(n: int, group: int, k: int, mulmods_per_s: float, paired: bool = False) -> dict:
     """The dominant kernel against the issue bound of its own instruction
     stream: a synthetic kernel issues the same per-state instruction
     multiset (8 states per iteration, every type on independent chains, no
@@ -100,7 +108,7 @@ def roofline(n: int, group: int, k: int, mulmods_per_s: float) -> dict:
     per launch, so the path is VALU-issue bound."""
     from permhip import _native
 
-    mix = instruction_mix(n, group, k)
+    mix = instruction_mix(n, group, k, paired)
     rounds = [round(mix[t]) for t in _native.MIX_TYPES]  # 8 states x count / 8 chains
     # Best of three timed launches: the bound is the mix's top issue rate.
     states_per_s = 8.0 * max(_native.mix_microbench(rounds, iters=256) for _ in range(3))
@@ -114,6 +122,7 @@ def roofline(n: int, group: int, k: int, mulmods_per_s: float) -> dict:
         "traffic": traffic_from_profiles(),
         "group": group,
         "prime_set": k,
+        "paired_states": paired,
         "instructions_per_state": mix,
         "mix_rounds_per_8_states": dict(zip(_native.MIX_TYPES, rounds)),
     }
@@ -469,15 +478,15 @@ def main() -> int:
     rates = _native.valu_microbench()
     log("VALU lane-ops/s: " + ", ".join(f"{k}={v:.3e}" for k, v in rates.items()))
     st = plan.stats()
-    group, kset = int(st["group"]), int(st["set"])
-    roof = roofline(HEADLINE_N, group, kset, kernel_rate)
+    group, kset, paired = int(st["group"]), int(st["set"]), bool(st.get("paired", 0))
+    roof = roofline(HEADLINE_N, group, kset, kernel_rate, paired)
     # Every mulmod as one signed Montgomery product plus one row-sum add: the
     # ceiling of a kernel that multiplies each row sum in modular arithmetic.
     roof["one_montgomery_per_mulmod_peak"] = (
         1.0 / (1.0 / rates["mont31_product"] + 1.0 / rates["v_add_u32"]) / 1e12)
     roof["instruction_rates"] = rates
     roof["kernel"] = (
-        f"batch_glynn_kernel<{group}, {kset}> (n={HEADLINE_N}, {len(primes)} primes in "
+        f"batch_glynn_kernel<{group}, {kset}, {str(paired).lower()}> (n={HEADLINE_N}, {len(primes)} primes in "
         f"{int(st['jobs'])} walk(s) of the Gray code)"
     )
     roof["kernel_ms"] = k_ms

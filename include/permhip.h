@@ -149,8 +149,9 @@ int phip_batch_set_option(void *plan, int option, int64_t value);
  * wavefronts, [7] row-sum group size and [9] prime-set size of the first
  * launch, [8] residues, [10] and [11] smallest and largest chunk (log2 of
  * the Gray states per lane), [12] work-item target (log2) the chunks were
- * sized for, [13] workgroups per CU.  Returns the number of values written
- * or -1. */
+ * sized for, [13] workgroups per CU, [14] 1 when the first launch pairs
+ * consecutive states (prime sets below 1.25 * 2^30).  Returns the number of
+ * values written or -1. */
 int phip_batch_stats(void *plan, double *out, int len);
 
 int phip_batch_destroy(void *plan);

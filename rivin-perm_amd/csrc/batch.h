@@ -19,7 +19,7 @@ constexpr int kBatchMaxOrder = 46;
 constexpr int kMaxSet = 8;
 
 // One job: one table (a matrix under one row order) and a set of K odd
-// moduli below 2^31 that share it (K = 1 for residue rows).  48 bytes,
+// moduli below 2^31 that share it (K = 1 for residue rows).  56 bytes,
 // read with scalar loads.
 struct BatchJob {
     uint64_t data_off;   // int32 offset of the job's tables (n * glynn_stride words)
@@ -32,6 +32,8 @@ struct BatchJob {
     int32_t group;
     int32_t nprimes;     // real primes (<= K); padded entries repeat the last one
     int32_t set;         // K of the job's launch class
+    int32_t paired;      // 1: the class pairs consecutive states (primes below 1.25 * 2^30)
+    int32_t pad;
 };
 
 // One requested residue: prime slot `k` of job `job`.
@@ -42,10 +44,12 @@ struct BatchSlot {
     uint32_t scale;  // R^groups * 2^-(n-1) mod p
 };
 
-// One persistent launch: the jobs of one (row-sum group, set size) pair.
+// One persistent launch: the jobs of one (row-sum group, set size,
+// paired) class.
 struct BatchClass {
     int group;
     int set;
+    int paired;
     uint32_t first_item;
     uint32_t n_items;
 };
@@ -133,5 +137,19 @@ PHIP_DECLARE_BATCH_LAUNCH(6, 2)
 PHIP_DECLARE_BATCH_LAUNCH(6, 4)
 PHIP_DECLARE_BATCH_LAUNCH(6, 8)
 #undef PHIP_DECLARE_BATCH_LAUNCH
+// Paired-state variants (every prime of the set below kPairedPrimeLimit).
+#define PHIP_DECLARE_BATCH_LAUNCH_PAIRED(G, K)                                                                   \
+    hipError_t launch_batch_g##G##_k##K##_paired(dim3, hipStream_t, const BatchJob*, const uint2*, uint32_t,   \
+                                               uint32_t*, const int32_t*, const int32_t*, const uint32_t*,   \
+                                               uint64_t*);
+PHIP_DECLARE_BATCH_LAUNCH_PAIRED(2, 4)
+PHIP_DECLARE_BATCH_LAUNCH_PAIRED(2, 8)
+PHIP_DECLARE_BATCH_LAUNCH_PAIRED(3, 4)
+PHIP_DECLARE_BATCH_LAUNCH_PAIRED(3, 8)
+PHIP_DECLARE_BATCH_LAUNCH_PAIRED(4, 4)
+PHIP_DECLARE_BATCH_LAUNCH_PAIRED(4, 8)
+PHIP_DECLARE_BATCH_LAUNCH_PAIRED(6, 4)
+PHIP_DECLARE_BATCH_LAUNCH_PAIRED(6, 8)
+#undef PHIP_DECLARE_BATCH_LAUNCH_PAIRED
 
 }  // namespace phip

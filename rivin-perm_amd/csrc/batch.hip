@@ -12,7 +12,8 @@
 // arithmetic.
 //
 // This file is compiled once per (G, K) class: PHIP_BATCH_GROUP and
-// PHIP_BATCH_SET select it.
+// PHIP_BATCH_SET select it, and PHIP_BATCH_PAIRED=1 the variant for prime
+// sets below kPairedPrimeLimit that pairs consecutive states.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -25,6 +26,9 @@
 #ifndef PHIP_BATCH_SET
 #error "PHIP_BATCH_SET must be defined"
 #endif
+#ifndef PHIP_BATCH_PAIRED
+#define PHIP_BATCH_PAIRED 0
+#endif
 
 namespace phip {
 
@@ -32,17 +36,17 @@ namespace {
 
 // One wavefront's chunk of job `jb` for every prime of its set; returns the
 // lane values (signed Montgomery residues) in r.
-template <int N, int G, int K>
+template <int N, int G, int K, bool P>
 __device__ __forceinline__ void run_item(const int32_t* __restrict__ tables, const BatchJob& jb, uint64_t base,
                                          const int32_t (&p)[K], const uint32_t (&pinv)[K], int32_t (&r)[K]) {
     if constexpr (K == 1) {
         r[0] = glynn_chunk<N, G>(tables, N, jb.cbits, base, p[0], pinv[0]);
     } else {
-        glynn_chunk_set<N, G, K>(tables, N, jb.cbits, base, p, pinv, r);
+        glynn_chunk_set<N, G, K, P>(tables, N, jb.cbits, base, p, pinv, r);
     }
 }
 
-template <int G, int K>
+template <int G, int K, bool P>
 __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __restrict__ jobs,
                                                          const uint2* __restrict__ items, uint32_t n_items,
                                                          uint32_t* __restrict__ counter,
@@ -79,7 +83,7 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
         switch (jb.n) {
 #define PHIP_BATCH_CASE(N) \
     case N:                \
-        run_item<N, G, K>(tables, jb, base, p, pinv, r); \
+        run_item<N, G, K, P>(tables, jb, base, p, pinv, r); \
         break;
             PHIP_BATCH_CASE(5) PHIP_BATCH_CASE(6) PHIP_BATCH_CASE(7) PHIP_BATCH_CASE(8)
             PHIP_BATCH_CASE(9) PHIP_BATCH_CASE(10) PHIP_BATCH_CASE(11) PHIP_BATCH_CASE(12)
@@ -107,15 +111,20 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
 
 }  // namespace
 
-#define PHIP_CAT4(a, b, c, d) a##b##c##d
-#define PHIP_LAUNCH_NAME(G, K) PHIP_CAT4(launch_batch_g, G, _k, K)
+#define PHIP_CAT5(a, b, c, d, e) a##b##c##d##e
+#define PHIP_LAUNCH_NAME(G, K, P) PHIP_CAT5(launch_batch_g, G, _k, K, P)
+#if PHIP_BATCH_PAIRED
+#define PHIP_PAIRED_SUFFIX _paired
+#else
+#define PHIP_PAIRED_SUFFIX
+#endif
 
-hipError_t PHIP_LAUNCH_NAME(PHIP_BATCH_GROUP, PHIP_BATCH_SET)(dim3 grid, hipStream_t stream, const BatchJob* jobs,
+hipError_t PHIP_LAUNCH_NAME(PHIP_BATCH_GROUP, PHIP_BATCH_SET, PHIP_PAIRED_SUFFIX)(dim3 grid, hipStream_t stream, const BatchJob* jobs,
                                                               const uint2* items, uint32_t n_items,
                                                               uint32_t* counter, const int32_t* blob,
                                                               const int32_t* set_p, const uint32_t* set_pinv,
                                                               uint64_t* item_partials) {
-    hipLaunchKernelGGL((batch_glynn_kernel<PHIP_BATCH_GROUP, PHIP_BATCH_SET>), grid, dim3(256), 0, stream, jobs,
+    hipLaunchKernelGGL((batch_glynn_kernel<PHIP_BATCH_GROUP, PHIP_BATCH_SET, PHIP_BATCH_PAIRED != 0>), grid, dim3(256), 0, stream, jobs,
                        items, n_items, counter, blob, set_p, set_pinv, item_partials);
     return hipGetLastError();
 }

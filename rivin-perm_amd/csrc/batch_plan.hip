@@ -14,6 +14,7 @@
 
 #include "batch.h"
 #include "device_context.h"
+#include "launchers.h"
 #include "permhip.h"
 #include "phip_arith.h"
 #include "phip_reduce.h"
@@ -48,16 +49,30 @@ __global__ __launch_bounds__(256) void batch_finalize_kernel(const BatchJob* __r
     if (threadIdx.x == 0) residues[blockIdx.x] = (int64_t)(((s % p) * (uint64_t)sl.scale) % p);
 }
 
-BatchLaunchFn launcher_for(int group, int set) {
+BatchLaunchFn launcher_for(int group, int set, int paired) {
 #define PHIP_LAUNCHER(G, K) \
-    if (group == G && set == K) return launch_batch_g##G##_k##K;
+    if (group == G && set == K && !paired) return launch_batch_g##G##_k##K;
+#define PHIP_LAUNCHER_PAIRED(G, K) \
+    if (group == G && set == K && paired) return launch_batch_g##G##_k##K##_paired;
     PHIP_LAUNCHER(0, 1)
     PHIP_LAUNCHER(2, 1) PHIP_LAUNCHER(2, 2) PHIP_LAUNCHER(2, 4) PHIP_LAUNCHER(2, 8)
     PHIP_LAUNCHER(3, 1) PHIP_LAUNCHER(3, 2) PHIP_LAUNCHER(3, 4) PHIP_LAUNCHER(3, 8)
     PHIP_LAUNCHER(4, 1) PHIP_LAUNCHER(4, 2) PHIP_LAUNCHER(4, 4) PHIP_LAUNCHER(4, 8)
     PHIP_LAUNCHER(6, 1) PHIP_LAUNCHER(6, 2) PHIP_LAUNCHER(6, 4) PHIP_LAUNCHER(6, 8)
+    PHIP_LAUNCHER_PAIRED(2, 4) PHIP_LAUNCHER_PAIRED(2, 8) PHIP_LAUNCHER_PAIRED(3, 4) PHIP_LAUNCHER_PAIRED(3, 8)
+    PHIP_LAUNCHER_PAIRED(4, 4) PHIP_LAUNCHER_PAIRED(4, 8) PHIP_LAUNCHER_PAIRED(6, 4) PHIP_LAUNCHER_PAIRED(6, 8)
 #undef PHIP_LAUNCHER
+#undef PHIP_LAUNCHER_PAIRED
     return nullptr;
+}
+
+// Sets of 4 or 8 primes all below kPairedPrimeLimit run on the paired-state
+// kernels.
+bool pairs_states(int set, const uint32_t* primes) {
+    if (set != 4 && set != 8) return false;
+    for (int k = 0; k < set; ++k)
+        if (primes[k] >= kPairedPrimeLimit) return false;
+    return true;
 }
 
 // Set sizes a table's r moduli are split into.  A set of K primes costs one
@@ -191,6 +206,7 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
                 }
             }
             next += take;
+            jb.paired = (st.group > 0 && pairs_states(K, (const uint32_t*)out.set_p.data() + jb.prime_off)) ? 1 : 0;
             out.jobs.push_back(jb);
         }
     }
@@ -219,12 +235,15 @@ int assign_items(PlanHost& out, int items_log2, int min_cbits) {
 
     // Items: per (group, set) class, jobs by descending cost, each job's
     // items contiguous.
-    static const int kClasses[][2] = {{0, 1}, {2, 8}, {2, 4}, {2, 2}, {2, 1}, {3, 8}, {3, 4}, {3, 2}, {3, 1},
-                                      {4, 8}, {4, 4}, {4, 2}, {4, 1}, {6, 8}, {6, 4}, {6, 2}, {6, 1}};
+    static const int kClasses[][3] = {{0, 1, 0}, {2, 8, 1}, {2, 8, 0}, {2, 4, 1}, {2, 4, 0}, {2, 2, 0}, {2, 1, 0},
+                                      {3, 8, 1}, {3, 8, 0}, {3, 4, 1}, {3, 4, 0}, {3, 2, 0}, {3, 1, 0},
+                                      {4, 8, 1}, {4, 8, 0}, {4, 4, 1}, {4, 4, 0}, {4, 2, 0}, {4, 1, 0},
+                                      {6, 8, 1}, {6, 8, 0}, {6, 4, 1}, {6, 4, 0}, {6, 2, 0}, {6, 1, 0}};
     for (const auto& gk : kClasses) {
         std::vector<uint32_t> order;
         for (uint32_t j = 0; j < out.jobs.size(); ++j)
-            if (out.jobs[j].group == gk[0] && out.jobs[j].set == gk[1]) order.push_back(j);
+            if (out.jobs[j].group == gk[0] && out.jobs[j].set == gk[1] && out.jobs[j].paired == gk[2])
+                order.push_back(j);
         if (order.empty()) continue;
         auto cost = [&](uint32_t j) {
             const BatchJob& b = out.jobs[j];
@@ -232,7 +251,7 @@ int assign_items(PlanHost& out, int items_log2, int min_cbits) {
                    (b.n + 3.0 * groups_of(b.n, b.group));
         };
         std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost(a) > cost(b); });
-        BatchClass cl{gk[0], gk[1], (uint32_t)out.items.size(), 0};
+        BatchClass cl{gk[0], gk[1], gk[2], (uint32_t)out.items.size(), 0};
         for (uint32_t j : order) {
             BatchJob& b = out.jobs[j];
             const uint64_t waves = (b.lanes + 63) / 64;
@@ -258,7 +277,7 @@ int enqueue_plan(const PlanHost& h, const PlanDevice& d, hipStream_t stream, uns
     if (ev0) PHIP_HIP_TRY(hipEventRecord(ev0, stream));
     for (size_t c = 0; c < h.classes.size(); ++c) {
         const BatchClass& cl = h.classes[c];
-        const BatchLaunchFn fn = launcher_for(cl.group, cl.set);
+        const BatchLaunchFn fn = launcher_for(cl.group, cl.set, cl.paired);
         if (!fn) {
             set_error("internal: no kernel for a batch class");
             return -1;
@@ -636,7 +655,8 @@ int phip_batch_stats(void* handle, double* out, int len) {
                         (double)cbits_min,
                         (double)cbits_max,
                         (double)h.items_log2,
-                        (double)plan->blocks_per_cu};
+                        (double)plan->blocks_per_cu,
+                        h.classes.empty() ? -1.0 : (double)h.classes[0].paired};
     const int m = std::min<int>(len, (int)(sizeof(v) / sizeof(v[0])));
     for (int i = 0; i < m; ++i) out[i] = v[i];
     return m;

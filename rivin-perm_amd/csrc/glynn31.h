@@ -306,6 +306,14 @@ __device__ __forceinline__ int32_t redc31(int64_t t, int32_t negp, uint32_t pinv
     return (int32_t)(u >> 32);
 }
 
+// acc + a * b as one in-order v_mad_i64_i32 (exact while the sum fits).
+__device__ __forceinline__ int64_t mad_exact(int64_t acc, int32_t a, int32_t b) {
+    int64_t r;
+    uint64_t carry;
+    asm("v_mad_i64_i32 %0, %1, %2, %3, %4" : "=v"(r), "=s"(carry) : "v"(a), "v"(b), "v"(acc));
+    return r;
+}
+
 // K primes sharing one packed table.  The exact row sums, the exact group
 // products and the exact 64-bit products of adjacent groups (|g| < 2^31, so
 // |g g'| < 2^62) do not depend on the prime and are formed once per state;
@@ -313,22 +321,41 @@ __device__ __forceinline__ int32_t redc31(int64_t t, int32_t negp, uint32_t pinv
 // apiece (the first level of the same balanced tree, bit-identical to
 // mont31 on the two groups) and combines the leaves with mont31.  The term
 // therefore carries R^-(groups-1) exactly as in OnePrimeSink.
-template <int NR, int G, int K>
+//
+// kPaired (every prime below kPairedPrimeLimit): consecutive states 2i and
+// 2i+1 carry opposite signs, so the last product of the tree is kept exact
+// for the even state (with the first group negated, once for all primes)
+// and the odd state's exact product is added to it; one Montgomery
+// reduction of that difference replaces two products and two reductions.
+// The term carries the same R^-(groups-1).  Shapes whose last product takes
+// the unreduced odd group directly keep the per-state form.
+template <int NR, int G, int K, bool kPaired = false>
 struct PrimeSetSink {
     static_assert(G > 0, "prime sets need exact packed row sums");
     using Rows = PackedRows<NR, G>;
     static constexpr int kGroups = Rows::kGroups;
     static constexpr int kPairs = kGroups / 2;
     static constexpr int kLeaves = kPairs + (kGroups & 1);
+    static constexpr int top_of(int leaves) {
+        int w = 1;
+        while (2 * w < leaves) w *= 2;
+        return w;
+    }
+    // Index of the last product's right operand.
+    static constexpr int kTop = top_of(kLeaves);
+    static constexpr bool kPairs2 =
+        kPaired && kLeaves >= 2 && !(kTop == kLeaves - 1 && (kGroups & 1));
     int32_t negp[K];
     uint32_t pinv[K];
     int64_t acc[K];
+    int64_t held[kPairs2 ? K : 1];
 
     template <bool ODD>
     __device__ __forceinline__ void state(const Rows& rs) {
         int32_t g[kGroups];
         rs.groups(g, std::make_integer_sequence<int, kGroups>{});
         int64_t P[kPairs > 0 ? kPairs : 1];
+        if constexpr (kPairs2 && !ODD) g[0] = -g[0];
 #pragma unroll
         for (int j = 0; j < kPairs; ++j) P[j] = (int64_t)g[2 * j] * (int64_t)g[2 * j + 1];
 #pragma unroll
@@ -337,12 +364,27 @@ struct PrimeSetSink {
 #pragma unroll
             for (int j = 0; j < kPairs; ++j) v[j] = redc31(P[j], negp[k], pinv[k]);
             if constexpr (kGroups & 1) v[kLeaves - 1] = g[kGroups - 1];
+            if constexpr (kPairs2) {
 #pragma unroll
-            for (int w = 1; w < kLeaves; w *= 2) {
+                for (int w = 1; w < kTop; w *= 2) {
 #pragma unroll
-                for (int i = 0; i + w < kLeaves; i += 2 * w) v[i] = mont31(v[i], v[i + w], negp[k], pinv[k]);
+                    for (int i = 0; i + w < kLeaves; i += 2 * w) v[i] = mont31(v[i], v[i + w], negp[k], pinv[k]);
+                }
+                if constexpr (!ODD) {
+                    held[k] = mad_exact(0, v[0], v[kTop]);
+                } else {
+                    // held = -E(2i); -(E(2i) - E(2i+1)) reduced, subtracted.
+                    const int64_t t = mad_exact(held[k], v[0], v[kTop]);
+                    acc[k] = mac_sign<-1>(acc[k], redc31(t, negp[k], pinv[k]));
+                }
+            } else {
+#pragma unroll
+                for (int w = 1; w < kLeaves; w *= 2) {
+#pragma unroll
+                    for (int i = 0; i + w < kLeaves; i += 2 * w) v[i] = mont31(v[i], v[i + w], negp[k], pinv[k]);
+                }
+                acc[k] = mac_sign<ODD ? -1 : 1>(acc[k], v[0]);
             }
-            acc[k] = mac_sign<ODD ? -1 : 1>(acc[k], v[0]);
         }
     }
 };
@@ -441,11 +483,12 @@ __device__ __forceinline__ int32_t glynn_chunk(const int32_t* __restrict__ table
 }
 
 // K primes sharing one packed table: out[k] as glynn_chunk for primes[k].
-template <int NR, int G, int K>
+// kPaired requires every prime below kPairedPrimeLimit.
+template <int NR, int G, int K, bool kPaired = false>
 __device__ __forceinline__ void glynn_chunk_set(const int32_t* __restrict__ tables, int n, int cbits,
                                                 uint64_t base, const int32_t (&primes)[K],
                                                 const uint32_t (&pinvs)[K], int32_t (&out)[K]) {
-    PrimeSetSink<NR, G, K> sink;
+    PrimeSetSink<NR, G, K, kPaired> sink;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         sink.negp[k] = -primes[k];

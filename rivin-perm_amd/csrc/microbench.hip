@@ -172,13 +172,15 @@ __global__ __launch_bounds__(256) void mix_kernel(uint32_t* out, int iters, uint
 using MixKernel = void (*)(uint32_t*, int, uint32_t);
 
 // Mixes of the compiled Glynn classes, per Gray state of n = 32 with groups
-// of 4: the 8-prime set and the single prime.
+// of 4: the 8-prime set (per state, and with consecutive states paired) and
+// the single prime.
 struct MixEntry {
     int32_t rounds[5];
     MixKernel kernel;
 };
 const MixEntry kMixes[] = {
     {{92, 56, 18, 24, 4}, mix_kernel<92, 56, 18, 24, 4>},
+    {{84, 52, 18, 24, 4}, mix_kernel<84, 52, 18, 24, 4>},
     {{15, 7, 18, 24, 4}, mix_kernel<15, 7, 18, 24, 4>},
 };
 

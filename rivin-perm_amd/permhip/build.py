@@ -26,8 +26,14 @@ BUILD_DIR = ROOT / "_build"
 LIBRARY = PACKAGE_DIR / "libpermhip.so"
 
 ARCH = "gfx950"
-# (row-sum group, prime-set size) classes of the persistent Glynn kernel.
-BATCH_CLASSES = ((0, 1),) + tuple((g, k) for g in (2, 3, 4, 6) for k in (1, 2, 4, 8))
+# (row-sum group, prime-set size, paired states) classes of the persistent
+# Glynn kernel; sets of 4 and 8 primes below 1.25 * 2^30 also have the
+# paired-state variant.
+BATCH_CLASSES = (
+    ((0, 1, 0),)
+    + tuple((g, k, 0) for g in (2, 3, 4, 6) for k in (1, 2, 4, 8))
+    + tuple((g, k, 1) for g in (2, 3, 4, 6) for k in (4, 8))
+)
 PLAIN_SOURCES = (
     "device_context.hip",
     "residues.hip",
@@ -81,12 +87,12 @@ def build(jobs: int | None = None, verbose: bool = False) -> Path:
     BUILD_DIR.mkdir(parents=True, exist_ok=True)
     units = []
     # Largest prime sets first: they take longest to compile.
-    for group, k in sorted(BATCH_CLASSES, key=lambda gk: -gk[1]):
+    for group, k, paired in sorted(BATCH_CLASSES, key=lambda gkp: -gkp[1]):
         units.append(
             (
                 CSRC / "batch.hip",
-                BUILD_DIR / f"batch_g{group}_k{k}.o",
-                [f"-DPHIP_BATCH_GROUP={group}", f"-DPHIP_BATCH_SET={k}"],
+                BUILD_DIR / f"batch_g{group}_k{k}{'_paired' if paired else ''}.o",
+                [f"-DPHIP_BATCH_GROUP={group}", f"-DPHIP_BATCH_SET={k}", f"-DPHIP_BATCH_PAIRED={paired}"],
             )
         )
     for name in PLAIN_SOURCES:

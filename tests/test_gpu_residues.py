@@ -236,3 +236,46 @@ def test_wide_moduli_agree_with_exact_permanent(native, n):
     moduli = [4_294_966_153, (1 << 61) - 1, (1 << 61) - 31, INT64_MAX - 24]
     got = native.permanent_residues(np.array(A, dtype=np.int64), moduli)
     assert got == [exact % m for m in moduli], n
+
+
+PAIRED_LIMIT = 1_342_177_280  # 1.25 * 2^30: sets below it pair consecutive states
+
+
+def _primes_below(limit, count):
+    from permhip import planner
+
+    out, c = [], (limit - 1) | 1
+    while len(out) < count:
+        if c < limit and planner.is_prime(c):
+            out.append(c)
+        c -= 2
+    return out
+
+
+@pytest.mark.parametrize("n,lo,hi", [(16, -1, 1), (20, -10, 10), (23, -100, 100), (21, -3, 3), (25, -10, 10)])
+def test_paired_state_sets_against_oracle(native, oracle_c, n, lo, hi):
+    """Sets of 8 and 4 primes just below the paired bound (paired kernels)
+    and just above it (per-state kernels) agree with the oracle; groups of
+    every row-sum representation and odd group counts are covered."""
+    rng = random.Random(8000 + n)
+    A = _random_matrix(rng, n, lo, hi)
+    below = _primes_below(PAIRED_LIMIT, 8)
+    above = _primes_from(PAIRED_LIMIT, 8)
+    for moduli in (below, above, below[:4], above[:4], below[:4] + above[:4]):
+        got = native.permanent_residues(np.array(A, dtype=np.int64), moduli)
+        for m, r in zip(moduli, got):
+            assert r == oracle_c.permanent_mod_p(A, m, 16), (n, lo, hi, m)
+
+
+def test_paired_classes_are_planned(native):
+    """The plan routes a set of 8 primes from 2^30 to a paired class and a
+    set from 2^31 - 1 down to a per-state class."""
+    rng = np.random.default_rng(3)
+    A = rng.integers(-10, 11, (20, 20)).astype(np.int64)
+    low = native.Batch([A], [_primes_from(1 << 30, 8)])
+    high = native.Batch([A], [_primes_below((1 << 31) - 1, 8)])
+    try:
+        assert low.stats()["paired"] == 1.0 and high.stats()["paired"] == 0.0
+    finally:
+        low.close()
+        high.close()
