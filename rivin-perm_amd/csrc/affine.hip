@@ -8,9 +8,11 @@
 // Montgomery factor) per coefficient, and added to the lane's n + 1
 // accumulators with the state's sign.
 //
-// One launch per residue arithmetic (wide_arith.h: Mont31L below 2^31 with
-// one lazy reduction per coefficient, Mont32, M61, PM61, Mont64) and
-// register tier (8, 16, 24 or 32 rows); blockIdx.y selects the modulus.
+// One launch per residue arithmetic (S30 below 2^30: signed lazy Montgomery
+// with exact int64 accumulators; wide_arith.h: Mont31L below 2^31 with one
+// lazy reduction per coefficient, Mont32, M61, PM61, Mont64) and
+// register tier (8, 16, 24, 28 for 32-bit residues, or 32 rows); blockIdx.y
+// selects the modulus.
 // As in wide.hip the flipped column and its direction are wave-uniform
 // except at a chunk's middle state, so deltas and negated deltas come by
 // scalar loads.  Each wavefront folds its lanes' accumulators with
@@ -45,6 +47,66 @@ __device__ __forceinline__ T xor_lane(T v, int mask) {
     }
 }
 
+// Signed lazy Montgomery for odd p < 2^30 (R = 2^32).  Row sums stay in
+// [0, p) as for the other arithmetics; the product coefficients are int32
+// with |v| < 2^30: a b + c d < 2^61 and the reduction adds |m p| < 2^61, so
+// one REDC of the exact sum returns |r| < 2^30 again with no correction
+// step.  The lane accumulators are exact int64 sums (|acc| < 2^(30 + cbits)),
+// reduced once at the end with one more R^-1.
+struct S30 {
+    typedef uint32_t T;
+    static constexpr int kRBits = 32;
+    uint32_t p;
+    int32_t negp;
+    uint32_t pinv;  // p^-1 mod 2^32
+    __device__ __forceinline__ T add(T a, T b) const {
+        const T s = a + b;  // < 2^31
+        return s >= p ? s - p : s;
+    }
+    __device__ __forceinline__ T neg(T a) const { return a ? p - a : 0; }
+    __device__ __forceinline__ int32_t redc(int64_t t) const {
+        const int32_t m = (int32_t)((uint32_t)t * pinv);
+        return (int32_t)((t + (int64_t)m * (int64_t)negp) >> 32);
+    }
+};
+
+// How a term is built and summed for arithmetic A: by default the product
+// coefficients and the accumulators live in A's residues.
+template <class A>
+struct AffineOps {
+    typedef typename A::T T;
+    typedef typename A::T V;
+    typedef typename A::T Acc;
+    static constexpr int kExtraR = 0;
+    __device__ __forceinline__ static V from_row(T x) { return x; }
+    __device__ __forceinline__ static V mul(const A& ar, V a, T b) { return ar.mul(a, b); }
+    __device__ __forceinline__ static V fma2(const A& ar, V a, T b, V c, T d) { return ar.fma2(a, b, c, d); }
+    __device__ __forceinline__ static Acc acc_add(const A& ar, Acc acc, V v, bool negative) {
+        return ar.add(acc, negative ? ar.neg(v) : v);
+    }
+    __device__ __forceinline__ static T finish(const A&, Acc acc) { return acc; }
+};
+
+template <>
+struct AffineOps<S30> {
+    typedef uint32_t T;
+    typedef int32_t V;
+    typedef int64_t Acc;
+    static constexpr int kExtraR = 1;
+    __device__ __forceinline__ static V from_row(T x) { return (int32_t)x; }
+    __device__ __forceinline__ static V mul(const S30& ar, V a, T b) { return ar.redc((int64_t)a * (int32_t)b); }
+    __device__ __forceinline__ static V fma2(const S30& ar, V a, T b, V c, T d) {
+        return ar.redc((int64_t)a * (int32_t)b + (int64_t)c * (int32_t)d);
+    }
+    __device__ __forceinline__ static Acc acc_add(const S30&, Acc acc, V v, bool negative) {
+        return negative ? acc - v : acc + v;
+    }
+    __device__ __forceinline__ static T finish(const S30& ar, Acc acc) {
+        const int32_t r = ar.redc(acc) % (int32_t)ar.p;
+        return (T)(r < 0 ? r + (int32_t)ar.p : r);
+    }
+};
+
 // c0, l0: [q][n] state-0 row sums; dc, ndc, dl, ndl: [q][n-1][n] deltas
 // (2 C, 2 L by column) and their negations; partials: [q][waves][n + 1].
 template <int NMAX, class A>
@@ -54,13 +116,15 @@ __global__ __launch_bounds__(kAffineBlock) void affine_glynn_kernel(
     const typename A::T* __restrict__ dl, const typename A::T* __restrict__ ndl, const A* __restrict__ arith, int n,
     int cbits, uint64_t lanes, uint64_t waves, uint64_t* __restrict__ partials) {
     typedef typename A::T T;
+    typedef AffineOps<A> Ops;
+    typedef typename Ops::V V;
     const int q = blockIdx.y;
     const A ar = arith[q];
     const uint64_t lane = (uint64_t)blockIdx.x * kAffineBlock + threadIdx.x;
     const bool active = lane < lanes;
     const size_t table = (size_t)q * (n - 1) * n;
 
-    T acc[NMAX + 1];
+    typename Ops::Acc acc[NMAX + 1];
 #pragma unroll
     for (int d = 0; d <= NMAX; ++d) acc[d] = 0;
 
@@ -124,21 +188,21 @@ __global__ __launch_bounds__(kAffineBlock) void affine_glynn_kernel(
                 negative = !negative;
             }
             // prod_i (c_i + x l_i), one factor at a time, highest degree first.
-            T P[NMAX + 1];
-            P[0] = c[0];
-            P[1] = l[0];
+            V P[NMAX + 1];
+            P[0] = Ops::from_row(c[0]);
+            P[1] = Ops::from_row(l[0]);
 #pragma unroll
             for (int i = 1; i < NMAX; ++i) {
                 if (i < n) {
-                    P[i + 1] = ar.mul(P[i], l[i]);
+                    P[i + 1] = Ops::mul(ar, P[i], l[i]);
 #pragma unroll
-                    for (int d = i; d >= 1; --d) P[d] = ar.fma2(P[d], c[i], P[d - 1], l[i]);
-                    P[0] = ar.mul(P[0], c[i]);
+                    for (int d = i; d >= 1; --d) P[d] = Ops::fma2(ar, P[d], c[i], P[d - 1], l[i]);
+                    P[0] = Ops::mul(ar, P[0], c[i]);
                 }
             }
 #pragma unroll
             for (int d = 0; d <= NMAX; ++d) {
-                if (d <= n) acc[d] = ar.add(acc[d], negative ? ar.neg(P[d]) : P[d]);
+                if (d <= n) acc[d] = Ops::acc_add(ar, acc[d], P[d], negative);
             }
         }
     }
@@ -148,7 +212,7 @@ __global__ __launch_bounds__(kAffineBlock) void affine_glynn_kernel(
 #pragma unroll
     for (int d = 0; d <= NMAX; ++d) {
         if (d <= n) {
-            T v = acc[d];
+            T v = Ops::finish(ar, acc[d]);
 #pragma unroll
             for (int m = 32; m >= 1; m >>= 1) v = ar.add(v, xor_lane(v, m));
             if ((threadIdx.x & 63) == 0 && wave < waves) partials[((size_t)q * waves + wave) * (n + 1) + d] = (uint64_t)v;
@@ -178,6 +242,10 @@ __global__ __launch_bounds__(256) void affine_sum_kernel(const uint64_t* __restr
 
 template <class A>
 A make_affine_arith(uint64_t q);
+template <>
+S30 make_affine_arith<S30>(uint64_t q) {
+    return S30{(uint32_t)q, -(int32_t)q, inverse_mod_2_32((uint32_t)q)};
+}
 template <>
 Mont31L make_affine_arith<Mont31L>(uint64_t q) {
     return make_mont31l(q);
@@ -211,6 +279,15 @@ int launch_affine(int tier, dim3 grid, hipStream_t st, const typename A::T* c0, 
         case 8: PHIP_AFFINE_LAUNCH(8); break;
         case 16: PHIP_AFFINE_LAUNCH(16); break;
         case 24: PHIP_AFFINE_LAUNCH(24); break;
+        case 28:
+            // 32-bit residues only: at 28 rows they keep 2 waves per SIMD
+            // where the 32-row tier holds 1.
+            if constexpr (sizeof(typename A::T) == 4) {
+                PHIP_AFFINE_LAUNCH(28);
+            } else {
+                PHIP_AFFINE_LAUNCH(32);
+            }
+            break;
         default: PHIP_AFFINE_LAUNCH(32); break;
     }
 #undef PHIP_AFFINE_LAUNCH
@@ -269,7 +346,7 @@ int run_affine_class(DeviceContext* ctx, const int64_t* C, const int64_t* L, int
     const uint64_t lanes = 1ull << lanes_log2;
     const uint64_t gx = (lanes + kAffineBlock - 1) / kAffineBlock;
     const uint64_t waves = gx * (kAffineBlock / 64);
-    const int fit = n <= 8 ? 8 : n <= 16 ? 16 : n <= 24 ? 24 : 32;
+    const int fit = n <= 8 ? 8 : n <= 16 ? 16 : n <= 24 ? 24 : (n <= 28 && sizeof(T) == 4) ? 28 : 32;
     const int tier = std::max(fit, min_tier);
     const int width = n + 1;
 
@@ -309,11 +386,12 @@ int run_affine_class(DeviceContext* ctx, const int64_t* C, const int64_t* L, int
     }
     for (int s = 0; s < kk; ++s) {
         const uint64_t q = jobs[s].q;
-        // n - 1 Montgomery factors per term; Glynn's sum carries 2^(n-1).
+        // n - 1 Montgomery factors per term (one more for the lane
+        // reduction of exact accumulators); Glynn's sum carries 2^(n-1).
         uint64_t scale = host_powmod((q + 1) / 2, (uint64_t)Lbits, q);
         if (A::kRBits > 0) {
             const uint64_t r_mod_q = (uint64_t)((((unsigned __int128)1) << A::kRBits) % q);
-            scale = host_mulmod(scale, host_powmod(r_mod_q, (uint64_t)Lbits, q), q);
+            scale = host_mulmod(scale, host_powmod(r_mod_q, (uint64_t)(Lbits + AffineOps<A>::kExtraR), q), q);
         }
         for (int d = 0; d < width; ++d)
             out[(size_t)jobs[s].index * width + d] = (int64_t)host_mulmod(h_out[(size_t)s * width + d], scale, q);
@@ -356,11 +434,13 @@ int affine_permanent_residues(DeviceContext* ctx, const int64_t* C, const int64_
         }
         return 0;
     }
-    std::vector<AffineJob> by_class[5];
+    std::vector<AffineJob> by_class[6];
     for (int i = 0; i < k; ++i) {
         const uint64_t q = (uint64_t)moduli[i];
         int cls;
-        if (q < (1ull << 31)) {
+        if (q < (1ull << 30)) {
+            cls = 5;
+        } else if (q < (1ull << 31)) {
             cls = 0;
         } else {
             switch (wide_class(q)) {
@@ -373,6 +453,7 @@ int affine_permanent_residues(DeviceContext* ctx, const int64_t* C, const int64_
         by_class[cls].push_back({i, q});
     }
     (void)width;
+    if (run_affine_class<S30>(ctx, C, L, stride, n, by_class[5], min_tier, out) != 0) return -1;
     if (run_affine_class<Mont31L>(ctx, C, L, stride, n, by_class[0], min_tier, out) != 0) return -1;
     if (run_affine_class<Mont32>(ctx, C, L, stride, n, by_class[1], min_tier, out) != 0) return -1;
     if (run_affine_class<M61>(ctx, C, L, stride, n, by_class[2], min_tier, out) != 0) return -1;

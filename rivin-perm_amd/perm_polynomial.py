@@ -7,7 +7,7 @@ options, validation messages and return values.
 * ``affine_permanent_mod(C, L, m)``: the coefficients of per(C + x L) modulo
   an odd m, ascending degree, ``order + 1`` entries.
 * ``affine_permanent(C, L)``: the exact integer coefficients, from residues
-  modulo descending 31-bit primes (all of them in one device pass) and CRT,
+  modulo descending 30-bit primes (all of them in one device pass) and CRT,
   certified by ``affine_permanent_bounds``.
 * ``principal_permanent_polynomial[_mod](A)``: per(I + x A).
 * ``rook_polynomial(A)`` / ``weighted_rook_polynomial(W)``: matching counts
@@ -25,6 +25,7 @@ device kernel (2^32 Gray states of degree-n products per modulus).
 from __future__ import annotations
 
 import operator
+import threading
 from math import factorial
 from typing import Sequence
 
@@ -36,7 +37,7 @@ from permhip.backends import perm_polynomial_ext as _polynomial_ext
 
 _INT64_MAX = (1 << 63) - 1
 _INT32_MAX = (1 << 31) - 1
-_CRT_PRIME_START = (1 << 31) - 1
+_CRT_PRIME_START = (1 << 30) - 1
 _ROOK_NATIVE_MAX_MODULI = 16
 
 HAS_POLYNOMIAL_EXT = True
@@ -218,12 +219,32 @@ def affine_permanent_bounds(constant_matrix, linear_matrix) -> tuple:
     return _affine_bounds(constant, linear)
 
 
+_PRIMES: list = []
+_PRIMES_LOCK = threading.Lock()
+
+
+def _prime(index: int) -> int:
+    """The index-th prime counting down from 2^30 - 1 (a fixed sequence,
+    found once per process)."""
+    if index < len(_PRIMES):
+        return _PRIMES[index]
+    with _PRIMES_LOCK:
+        while len(_PRIMES) <= index:
+            c = _PRIMES[-1] - 2 if _PRIMES else _CRT_PRIME_START
+            while not _prep.is_prime_u64(c):
+                c -= 2
+            _PRIMES.append(c)
+        return _PRIMES[index]
+
+
 def _descending_primes(target: int, minimum: int = 3) -> list:
-    """Descending primes from 2^31 - 1 until their product exceeds target
-    (perm_polynomial.py:929-945), all above `minimum`."""
+    """Descending primes until their product exceeds target, all above
+    `minimum` (the plan of perm_polynomial.py:929-945, started below 2^30 so
+    that every residue runs on the device's signed lazy Montgomery kernel;
+    the exact coefficients do not depend on the primes)."""
     primes, product = [], 1
     while product <= target:
-        p = _prep.descending_prime(len(primes))
+        p = _prime(len(primes))
         if p <= minimum:
             raise RuntimeError("exhausted the deterministic CRT prime range")
         primes.append(p)
