@@ -401,6 +401,12 @@ def main() -> int:
                     help="headline plan: fixed work-item target (log2), no tuning")
     ap.add_argument("--blocks-per-cu", type=int, default=None,
                     help="headline plan: fixed workgroups per CU, no tuning")
+    ap.add_argument("--rook-residues", type=int, default=0, metavar="N",
+                    help="coefficients of per(J + x A) for the seeded N x N 0/1 board modulo 4 primes below "
+                         "2^30, once, and exit (one affine kernel launch, for profilers)")
+    ap.add_argument("--wide-residue", type=int, default=0, metavar="BITS",
+                    help="perm(A) of the seeded 30x30 wide-moduli matrix modulo 2^61 - 1 (BITS=61) or "
+                         "2^32 - 5 (BITS=32), once, and exit (one wide Glynn launch, for profilers)")
     ap.add_argument("--schur-residue", type=int, default=0, metavar="N",
                     help="compute perm(S_N) modulo the first prime = 1 (mod N) above 2^30 and exit "
                          "(one necklace or Glynn launch, for profilers)")
@@ -420,6 +426,23 @@ def main() -> int:
     if args.copy_calibration:
         _native.copy_calibration(args.copy_calibration, 1)
         log(f"copied {args.copy_calibration} bytes")
+        return 0
+    if args.rook_residues:
+        import perm_polynomial
+
+        n = args.rook_residues
+        board = np.random.default_rng(POLY_SEED).integers(0, 2, (n, n)).astype(np.int64)
+        primes = perm_polynomial._descending_primes(1 << 100)[:4]
+        t0 = time.perf_counter()
+        rows = _native.affine_permanent_residues(np.ones((n, n), dtype=np.int64), board, primes)
+        log(f"per(J + xA), n={n}, primes {primes}: {len(rows)} coefficient vectors in {time.perf_counter() - t0:.3f} s")
+        return 0
+    if args.wide_residue:
+        q = (1 << 61) - 1 if args.wide_residue == 61 else (1 << 32) - 5
+        A = np.random.default_rng(7).integers(-(10 ** 6), 10 ** 6 + 1, (30, 30)).astype(np.int64)
+        t0 = time.perf_counter()
+        r = _native.permanent_residues(A, [q])[0]
+        log(f"perm(A) mod {q} = {r} in {time.perf_counter() - t0:.3f} s")
         return 0
     if args.schur_residue:
         n = args.schur_residue

@@ -16,7 +16,8 @@ struct AffineJob {
 // out[i * (n + 1) + d] = coefficient of x^d in per(C_i + x L_i) mod
 // moduli[i], with C_i = C + i * stride and L_i = L + i * stride (row-major
 // n x n int64).  0 <= n <= 32; odd moduli 3 <= m < 2^63.  min_tier forces a
-// larger register tier (8, 16, 24, 28 or 32).  Returns 0 or -1.
+// larger register tier (8..32, a multiple of 4; 64-bit residues round it up
+// to a multiple of 8).  Returns 0 or -1.
 int affine_permanent_residues(DeviceContext* ctx, const int64_t* C, const int64_t* L, int64_t stride, int n,
                               const int64_t* moduli, int k, int min_tier, int64_t* out);
 

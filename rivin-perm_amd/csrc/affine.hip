@@ -11,8 +11,8 @@
 // One launch per residue arithmetic (S30 below 2^30: signed lazy Montgomery
 // with exact int64 accumulators; wide_arith.h: Mont31L below 2^31 with one
 // lazy reduction per coefficient, Mont32, M61, PM61, Mont64) and
-// register tier (8, 16, 24, 28 for 32-bit residues, or 32 rows); blockIdx.y
-// selects the modulus.
+// register tier (every 4 rows from 8 to 32 for 32-bit residues, every 8
+// for 64-bit ones); blockIdx.y selects the modulus.
 // As in wide.hip the flipped column and its direction are wave-uniform
 // except at a chunk's middle state, so deltas and negated deltas come by
 // scalar loads.  Each wavefront folds its lanes' accumulators with
@@ -107,8 +107,12 @@ struct AffineOps<S30> {
     }
 };
 
-// c0, l0: [q][n] state-0 row sums; dc, ndc, dl, ndl: [q][n-1][n] deltas
-// (2 C, 2 L by column) and their negations; partials: [q][waves][n + 1].
+// Tables hold NMAX rows per column: c0, l0: [q][NMAX] state-0 row sums;
+// dc, ndc, dl, ndl: [q][n-1][NMAX] deltas (2 C, 2 L by column) and their
+// negations.  Rows at or beyond n are the identity factor (constant part
+// the arithmetic's one, linear part 0, deltas 0), so the unrolled loops run
+// without per-row tests and the product is unchanged by them.  partials:
+// [q][waves][n + 1].
 template <int NMAX, class A>
 __global__ __launch_bounds__(kAffineBlock) void affine_glynn_kernel(
     const typename A::T* __restrict__ c0, const typename A::T* __restrict__ l0,
@@ -122,7 +126,7 @@ __global__ __launch_bounds__(kAffineBlock) void affine_glynn_kernel(
     const A ar = arith[q];
     const uint64_t lane = (uint64_t)blockIdx.x * kAffineBlock + threadIdx.x;
     const bool active = lane < lanes;
-    const size_t table = (size_t)q * (n - 1) * n;
+    const size_t table = (size_t)q * (n - 1) * NMAX;
 
     typename Ops::Acc acc[NMAX + 1];
 #pragma unroll
@@ -131,26 +135,24 @@ __global__ __launch_bounds__(kAffineBlock) void affine_glynn_kernel(
     if (active) {
         T c[NMAX], l[NMAX];
         {
-            const ConstPtr<T> pc = scalar_ptr(c0 + (size_t)q * n);
-            const ConstPtr<T> pl = scalar_ptr(l0 + (size_t)q * n);
+            const ConstPtr<T> pc = scalar_ptr(c0 + (size_t)q * NMAX);
+            const ConstPtr<T> pl = scalar_ptr(l0 + (size_t)q * NMAX);
 #pragma unroll
             for (int i = 0; i < NMAX; ++i) {
-                c[i] = i < n ? pc[i] : T(0);
-                l[i] = i < n ? pl[i] : T(0);
+                c[i] = pc[i];
+                l[i] = pl[i];
             }
         }
         const uint64_t base = lane << cbits;
         const uint64_t gray = base ^ (base >> 1);
         for (int j = 0; j < n - 1; ++j) {
             const bool on = (gray >> j) & 1u;
-            const ConstPtr<T> nc = scalar_ptr(ndc + table + (size_t)j * n);
-            const ConstPtr<T> nl = scalar_ptr(ndl + table + (size_t)j * n);
+            const ConstPtr<T> nc = scalar_ptr(ndc + table + (size_t)j * NMAX);
+            const ConstPtr<T> nl = scalar_ptr(ndl + table + (size_t)j * NMAX);
 #pragma unroll
             for (int i = 0; i < NMAX; ++i) {
-                if (i < n) {
-                    c[i] = ar.add(c[i], on ? nc[i] : T(0));
-                    l[i] = ar.add(l[i], on ? nl[i] : T(0));
-                }
+                c[i] = ar.add(c[i], on ? nc[i] : T(0));
+                l[i] = ar.add(l[i], on ? nl[i] : T(0));
             }
         }
         bool negative = __popcll(gray) & 1;
@@ -162,27 +164,23 @@ __global__ __launch_bounds__(kAffineBlock) void affine_glynn_kernel(
                 if (j == cbits - 1) {
                     // Bit j of gray(base + t) is 1 xor the lane's lowest bit.
                     const bool sub = !lane_odd;
-                    const ConstPtr<T> pc = scalar_ptr(dc + table + (size_t)j * n);
-                    const ConstPtr<T> nc = scalar_ptr(ndc + table + (size_t)j * n);
-                    const ConstPtr<T> pl = scalar_ptr(dl + table + (size_t)j * n);
-                    const ConstPtr<T> nl = scalar_ptr(ndl + table + (size_t)j * n);
+                    const ConstPtr<T> pc = scalar_ptr(dc + table + (size_t)j * NMAX);
+                    const ConstPtr<T> nc = scalar_ptr(ndc + table + (size_t)j * NMAX);
+                    const ConstPtr<T> pl = scalar_ptr(dl + table + (size_t)j * NMAX);
+                    const ConstPtr<T> nl = scalar_ptr(ndl + table + (size_t)j * NMAX);
 #pragma unroll
                     for (int i = 0; i < NMAX; ++i) {
-                        if (i < n) {
-                            c[i] = ar.add(c[i], sub ? nc[i] : pc[i]);
-                            l[i] = ar.add(l[i], sub ? nl[i] : pl[i]);
-                        }
+                        c[i] = ar.add(c[i], sub ? nc[i] : pc[i]);
+                        l[i] = ar.add(l[i], sub ? nl[i] : pl[i]);
                     }
                 } else {
                     const bool sub = ((t ^ (t >> 1)) >> j) & 1u;
-                    const ConstPtr<T> pc = scalar_ptr((sub ? ndc : dc) + table + (size_t)j * n);
-                    const ConstPtr<T> pl = scalar_ptr((sub ? ndl : dl) + table + (size_t)j * n);
+                    const ConstPtr<T> pc = scalar_ptr((sub ? ndc : dc) + table + (size_t)j * NMAX);
+                    const ConstPtr<T> pl = scalar_ptr((sub ? ndl : dl) + table + (size_t)j * NMAX);
 #pragma unroll
                     for (int i = 0; i < NMAX; ++i) {
-                        if (i < n) {
-                            c[i] = ar.add(c[i], pc[i]);
-                            l[i] = ar.add(l[i], pl[i]);
-                        }
+                        c[i] = ar.add(c[i], pc[i]);
+                        l[i] = ar.add(l[i], pl[i]);
                     }
                 }
                 negative = !negative;
@@ -193,17 +191,13 @@ __global__ __launch_bounds__(kAffineBlock) void affine_glynn_kernel(
             P[1] = Ops::from_row(l[0]);
 #pragma unroll
             for (int i = 1; i < NMAX; ++i) {
-                if (i < n) {
-                    P[i + 1] = Ops::mul(ar, P[i], l[i]);
+                P[i + 1] = Ops::mul(ar, P[i], l[i]);
 #pragma unroll
-                    for (int d = i; d >= 1; --d) P[d] = Ops::fma2(ar, P[d], c[i], P[d - 1], l[i]);
-                    P[0] = Ops::mul(ar, P[0], c[i]);
-                }
+                for (int d = i; d >= 1; --d) P[d] = Ops::fma2(ar, P[d], c[i], P[d - 1], l[i]);
+                P[0] = Ops::mul(ar, P[0], c[i]);
             }
 #pragma unroll
-            for (int d = 0; d <= NMAX; ++d) {
-                if (d <= n) acc[d] = Ops::acc_add(ar, acc[d], P[d], negative);
-            }
+            for (int d = 0; d <= NMAX; ++d) acc[d] = Ops::acc_add(ar, acc[d], P[d], negative);
         }
     }
 
@@ -275,20 +269,26 @@ int launch_affine(int tier, dim3 grid, hipStream_t st, const typename A::T* c0, 
 #define PHIP_AFFINE_LAUNCH(NM)                                                                                  \
     hipLaunchKernelGGL((affine_glynn_kernel<NM, A>), grid, dim3(kAffineBlock), 0, st, c0, l0, dc, ndc, dl, ndl, \
                        ar, n, cbits, lanes, waves, part)
-    switch (tier) {
-        case 8: PHIP_AFFINE_LAUNCH(8); break;
-        case 16: PHIP_AFFINE_LAUNCH(16); break;
-        case 24: PHIP_AFFINE_LAUNCH(24); break;
-        case 28:
-            // 32-bit residues only: at 28 rows they keep 2 waves per SIMD
-            // where the 32-row tier holds 1.
-            if constexpr (sizeof(typename A::T) == 4) {
-                PHIP_AFFINE_LAUNCH(28);
-            } else {
-                PHIP_AFFINE_LAUNCH(32);
-            }
-            break;
-        default: PHIP_AFFINE_LAUNCH(32); break;
+    // The tables were laid out for exactly `tier` rows per column.
+    if constexpr (sizeof(typename A::T) == 4) {
+        switch (tier) {
+            case 8: PHIP_AFFINE_LAUNCH(8); break;
+            case 12: PHIP_AFFINE_LAUNCH(12); break;
+            case 16: PHIP_AFFINE_LAUNCH(16); break;
+            case 20: PHIP_AFFINE_LAUNCH(20); break;
+            case 24: PHIP_AFFINE_LAUNCH(24); break;
+            case 28: PHIP_AFFINE_LAUNCH(28); break;
+            case 32: PHIP_AFFINE_LAUNCH(32); break;
+            default: set_error("internal: no affine kernel for this register tier"); return -1;
+        }
+    } else {
+        switch (tier) {
+            case 8: PHIP_AFFINE_LAUNCH(8); break;
+            case 16: PHIP_AFFINE_LAUNCH(16); break;
+            case 24: PHIP_AFFINE_LAUNCH(24); break;
+            case 32: PHIP_AFFINE_LAUNCH(32); break;
+            default: set_error("internal: no affine kernel for this register tier"); return -1;
+        }
     }
 #undef PHIP_AFFINE_LAUNCH
     PHIP_HIP_TRY(hipGetLastError());
@@ -303,12 +303,18 @@ int run_affine_class(DeviceContext* ctx, const int64_t* C, const int64_t* L, int
     typedef typename A::T T;
     const int kk = (int)jobs.size();
     if (kk == 0) return 0;
-    const size_t cols = (size_t)(n - 1) * n;
-    // One host image: c0, l0 ([kk][n] each), then dc, ndc, dl, ndl ([kk][n-1][n] each).
-    std::vector<T> img((size_t)kk * (2 * n + 4 * cols));
+    // Register tier: the smallest that holds n rows (32-bit residues have
+    // tiers every 4 rows from 8; 64-bit ones every 8), or the tuning floor.
+    const int step = sizeof(T) == 4 ? 4 : 8;
+    const int rows = std::max({8, n, min_tier});
+    const int W = std::min(32, (rows + step - 1) / step * step);  // rows per table column
+    const size_t cols = (size_t)(n - 1) * W;
+    // One host image: c0, l0 ([kk][W] each), then dc, ndc, dl, ndl
+    // ([kk][n-1][W] each); rows n..W-1 are the identity factor.
+    std::vector<T> img((size_t)kk * (2 * W + 4 * cols), T(0));
     T* h_c0 = img.data();
-    T* h_l0 = h_c0 + (size_t)kk * n;
-    T* h_dc = h_l0 + (size_t)kk * n;
+    T* h_l0 = h_c0 + (size_t)kk * W;
+    T* h_dc = h_l0 + (size_t)kk * W;
     T* h_ndc = h_dc + (size_t)kk * cols;
     T* h_dl = h_ndc + (size_t)kk * cols;
     T* h_ndl = h_dl + (size_t)kk * cols;
@@ -320,7 +326,13 @@ int run_affine_class(DeviceContext* ctx, const int64_t* C, const int64_t* L, int
         const int64_t* Ls = L + (int64_t)jobs[s].index * stride;
         ar[s] = make_affine_arith<A>(q);
         moduli[s] = q;
-        for (int i = 0; i < n; ++i) {
+        // The arithmetic's one: R mod q in Montgomery form, else 1.
+        const uint64_t one = A::kRBits > 0 ? (uint64_t)((((unsigned __int128)1) << A::kRBits) % q) : 1 % q;
+        for (int i = 0; i < W; ++i) {
+            if (i >= n) {
+                h_c0[(size_t)s * W + i] = (T)one;
+                continue;
+            }
             uint64_t rc = 0, rl = 0;
             for (int j = 0; j < n; ++j) {
                 const uint64_t vc = (uint64_t)host_floor_mod(Cs[(size_t)i * n + j], (int64_t)q);
@@ -328,7 +340,7 @@ int run_affine_class(DeviceContext* ctx, const int64_t* C, const int64_t* L, int
                 rc = add_mod_u64(rc, vc, q);
                 rl = add_mod_u64(rl, vl, q);
                 if (j < n - 1) {
-                    const size_t at = (size_t)s * cols + (size_t)j * n + i;
+                    const size_t at = (size_t)s * cols + (size_t)j * W + i;
                     const uint64_t d2c = add_mod_u64(vc, vc, q), d2l = add_mod_u64(vl, vl, q);
                     h_dc[at] = (T)d2c;
                     h_ndc[at] = (T)(d2c ? q - d2c : 0);
@@ -336,8 +348,8 @@ int run_affine_class(DeviceContext* ctx, const int64_t* C, const int64_t* L, int
                     h_ndl[at] = (T)(d2l ? q - d2l : 0);
                 }
             }
-            h_c0[(size_t)s * n + i] = (T)rc;
-            h_l0[(size_t)s * n + i] = (T)rl;
+            h_c0[(size_t)s * W + i] = (T)rc;
+            h_l0[(size_t)s * W + i] = (T)rl;
         }
     }
     const int Lbits = n - 1;
@@ -346,8 +358,7 @@ int run_affine_class(DeviceContext* ctx, const int64_t* C, const int64_t* L, int
     const uint64_t lanes = 1ull << lanes_log2;
     const uint64_t gx = (lanes + kAffineBlock - 1) / kAffineBlock;
     const uint64_t waves = gx * (kAffineBlock / 64);
-    const int fit = n <= 8 ? 8 : n <= 16 ? 16 : n <= 24 ? 24 : (n <= 28 && sizeof(T) == 4) ? 28 : 32;
-    const int tier = std::max(fit, min_tier);
+    const int tier = W;
     const int width = n + 1;
 
     hipStream_t st = ctx->stream();
@@ -362,8 +373,8 @@ int run_affine_class(DeviceContext* ctx, const int64_t* C, const int64_t* L, int
     PHIP_HIP_TRY(hipMemcpyAsync(d_ar, ar.data(), ar.size() * sizeof(A), hipMemcpyHostToDevice, st));
     PHIP_HIP_TRY(hipMemcpyAsync(d_mod, moduli.data(), moduli.size() * 8, hipMemcpyHostToDevice, st));
     const T* d_c0 = d_img;
-    const T* d_l0 = d_c0 + (size_t)kk * n;
-    const T* d_dc = d_l0 + (size_t)kk * n;
+    const T* d_l0 = d_c0 + (size_t)kk * W;
+    const T* d_dc = d_l0 + (size_t)kk * W;
     const T* d_ndc = d_dc + (size_t)kk * cols;
     const T* d_dl = d_ndc + (size_t)kk * cols;
     const T* d_ndl = d_dl + (size_t)kk * cols;

@@ -23,7 +23,7 @@ def _pair(rng, n, lo, hi):
     return C, L
 
 
-@pytest.mark.parametrize("tier", [8, 16, 24, 28, 32])
+@pytest.mark.parametrize("tier", [8, 12, 16, 20, 24, 28, 32])
 def test_affine_register_tiers(native, oracle_c, tier):
     old = native.get_tuning(native.TUNE_AFFINE_MIN_TIER)
     try:
