@@ -166,7 +166,7 @@ int phip_batch_destroy(void *plan);
 #define PHIP_TUNE_BATCH_BLOCKS 4     /* batch: default persistent workgroups per CU */
 #define PHIP_TUNE_NECKLACE_MIN_ORDER 5 /* Schur: smallest prime order using necklace orbits (5..64) */
 #define PHIP_TUNE_NECKLACE_DEPTH 6   /* Schur necklaces: largest FKM prefix depth of the tasks (2..30) */
-#define PHIP_TUNE_WIDE_MIN_TIER 7    /* wide-modulus Glynn: smallest row-register tier (16, 32, 48, 64) */
+#define PHIP_TUNE_WIDE_MIN_TIER 7    /* wide-modulus Glynn: smallest row-register tier (8..64, a multiple of 4) */
 #define PHIP_TUNE_AFFINE_MIN_TIER 8  /* affine Glynn: smallest row-register tier (8..32, a multiple of 4) */
 int phip_set_tuning(int key, int64_t value);
 int64_t phip_get_tuning(int key);

@@ -130,7 +130,7 @@ int phip_set_tuning(int key, int64_t value) {
             t.necklace_depth = (int)value;
             return 0;
         case PHIP_TUNE_WIDE_MIN_TIER:
-            if (value != 16 && value != 32 && value != 48 && value != 64) break;
+            if (value < 8 || value > 64 || value % 4 != 0) break;
             t.wide_min_tier = (int)value;
             return 0;
         case PHIP_TUNE_AFFINE_MIN_TIER:

@@ -19,7 +19,7 @@ struct Tuning {
     int batch_blocks_per_cu = 3;
     int necklace_min_order = 23;  // smallest prime Schur order sent to the necklace kernel
     int necklace_depth = 28;      // largest FKM prefix depth of the necklace tasks
-    int wide_min_tier = 16;       // smallest register tier (16, 32, 48, 64) of the wide Glynn kernels
+    int wide_min_tier = 8;        // smallest register tier (8..64, multiple of 4) of the wide Glynn kernels
     int affine_min_tier = 8;      // smallest register tier (8..32, multiple of 4) of the affine Glynn kernels
 };
 Tuning& tuning();

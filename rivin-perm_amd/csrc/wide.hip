@@ -5,14 +5,15 @@
 // over 32-bit primes (perm_modp_cuda.cu:328-485).
 //
 // One launch per arithmetic class (wide_arith.h: Mont32, M61, PM61, Mont64) and
-// order tier; blockIdx.y selects the modulus.  Each lane walks 2^cbits
-// consecutive Gray states with its n row sums in registers (the tier NMAX
-// bounds the unrolled row loops; rows at or beyond n are skipped by a
-// wave-uniform test).  The flipped column and the direction of its update
+// register tier; blockIdx.y selects the modulus.  Each lane walks 2^cbits
+// consecutive Gray states with its row sums in registers (the tier NMAX
+// rows; rows at or beyond n are the identity factor, so the unrolled loops
+// carry no per-row test).  The flipped column and the direction of its update
 // are the same for every lane except at the middle state of the chunk,
 // where the flipped bit is the chunk's lowest bit, so the deltas arrive by
-// scalar loads and the select of +d or -d is scalar too.  Each lane writes
-// its signed sum to partials; wide_sum_kernel folds them per modulus.
+// scalar loads and the select of +d or -d is scalar too.  Each wavefront
+// folds its lanes' sums by cross-lane swizzles (modular adds) into one
+// partial; wide_sum_kernel folds the partials per modulus.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,73 +32,85 @@ namespace {
 constexpr int kWideBlock = 256;
 constexpr int kWideLanesLog2 = 20;
 
+template <class T>
+__device__ __forceinline__ T wide_xor_lane(T v, int mask) {
+    if constexpr (sizeof(T) == 4) {
+        return (T)__shfl_xor((int)v, mask, 64);
+    } else {
+        const int lo = __shfl_xor((int)(uint32_t)v, mask, 64);
+        const int hi = __shfl_xor((int)(uint32_t)((uint64_t)v >> 32), mask, 64);
+        return (T)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+    }
+}
+
+// Tables hold NMAX rows per column: rs0 [q][NMAX], deltas and neg_deltas
+// [q][n-1][NMAX]; rows at or beyond n are the identity factor (row sum the
+// arithmetic's one, deltas 0), so the unrolled loops carry no per-row test.
+// Each wavefront folds its lanes' sums and writes partials[q][wave].
 template <int NMAX, class A>
 __global__ __launch_bounds__(kWideBlock) void wide_glynn_kernel(const typename A::T* __restrict__ rs0,
                                                                 const typename A::T* __restrict__ deltas,
                                                                 const typename A::T* __restrict__ neg_deltas,
                                                                 const A* __restrict__ arith, int n, int cbits,
-                                                                uint64_t lanes, uint64_t* __restrict__ partials) {
+                                                                uint64_t lanes, uint64_t waves,
+                                                                uint64_t* __restrict__ partials) {
     typedef typename A::T T;
     const int q = blockIdx.y;
     const A ar = arith[q];
     const uint64_t lane = (uint64_t)blockIdx.x * kWideBlock + threadIdx.x;
-    if (lane >= lanes) return;
-    const size_t table = (size_t)q * (n - 1) * n;
+    const size_t table = (size_t)q * (n - 1) * NMAX;
     const T* __restrict__ D = deltas + table;
     const T* __restrict__ ND = neg_deltas + table;
 
-    T rs[NMAX];
-    {
-        const ConstPtr<T> r0 = scalar_ptr(rs0 + (size_t)q * n);
-#pragma unroll
-        for (int i = 0; i < NMAX; ++i) rs[i] = i < n ? r0[i] : T(0);
-    }
-    const uint64_t base = lane << cbits;
-    const uint64_t gray = base ^ (base >> 1);
-    // Start of the chunk: subtract the deltas of the columns set in gray(base).
-    for (int j = 0; j < n - 1; ++j) {
-        const bool on = (gray >> j) & 1u;
-        const ConstPtr<T> nd = scalar_ptr(ND + (size_t)j * n);
-#pragma unroll
-        for (int i = 0; i < NMAX; ++i) {
-            if (i < n) rs[i] = ar.add(rs[i], on ? nd[i] : T(0));
-        }
-    }
-    bool negative = __popcll(gray) & 1;
-    const bool lane_odd = lane & 1u;
-
     T acc = 0;
-    const uint64_t states = 1ull << cbits;
-    for (uint64_t t = 0; t < states; ++t) {
-        if (t != 0) {
-            const int j = __builtin_ctzll(t);
-            if (j == cbits - 1) {
-                // Bit j of gray(base + t) is 1 xor the lane's lowest bit.
-                const bool sub = !lane_odd;
-                const ConstPtr<T> d = scalar_ptr(D + (size_t)j * n);
-                const ConstPtr<T> nd = scalar_ptr(ND + (size_t)j * n);
+    if (lane < lanes) {
+        T rs[NMAX];
+        {
+            const ConstPtr<T> r0 = scalar_ptr(rs0 + (size_t)q * NMAX);
 #pragma unroll
-                for (int i = 0; i < NMAX; ++i) {
-                    if (i < n) rs[i] = ar.add(rs[i], sub ? nd[i] : d[i]);
-                }
-            } else {
-                const bool sub = ((t ^ (t >> 1)) >> j) & 1u;
-                const ConstPtr<T> d = scalar_ptr((sub ? ND : D) + (size_t)j * n);
+            for (int i = 0; i < NMAX; ++i) rs[i] = r0[i];
+        }
+        const uint64_t base = lane << cbits;
+        const uint64_t gray = base ^ (base >> 1);
+        // Start of the chunk: subtract the deltas of the columns set in gray(base).
+        for (int j = 0; j < n - 1; ++j) {
+            const bool on = (gray >> j) & 1u;
+            const ConstPtr<T> nd = scalar_ptr(ND + (size_t)j * NMAX);
 #pragma unroll
-                for (int i = 0; i < NMAX; ++i) {
-                    if (i < n) rs[i] = ar.add(rs[i], d[i]);
+            for (int i = 0; i < NMAX; ++i) rs[i] = ar.add(rs[i], on ? nd[i] : T(0));
+        }
+        bool negative = __popcll(gray) & 1;
+        const bool lane_odd = lane & 1u;
+
+        const uint64_t states = 1ull << cbits;
+        for (uint64_t t = 0; t < states; ++t) {
+            if (t != 0) {
+                const int j = __builtin_ctzll(t);
+                if (j == cbits - 1) {
+                    // Bit j of gray(base + t) is 1 xor the lane's lowest bit.
+                    const bool sub = !lane_odd;
+                    const ConstPtr<T> d = scalar_ptr(D + (size_t)j * NMAX);
+                    const ConstPtr<T> nd = scalar_ptr(ND + (size_t)j * NMAX);
+#pragma unroll
+                    for (int i = 0; i < NMAX; ++i) rs[i] = ar.add(rs[i], sub ? nd[i] : d[i]);
+                } else {
+                    const bool sub = ((t ^ (t >> 1)) >> j) & 1u;
+                    const ConstPtr<T> d = scalar_ptr((sub ? ND : D) + (size_t)j * NMAX);
+#pragma unroll
+                    for (int i = 0; i < NMAX; ++i) rs[i] = ar.add(rs[i], d[i]);
                 }
+                negative = !negative;
             }
-            negative = !negative;
-        }
-        T prod = rs[0];
+            T prod = rs[0];
 #pragma unroll
-        for (int i = 1; i < NMAX; ++i) {
-            if (i < n) prod = ar.mul(prod, rs[i]);
+            for (int i = 1; i < NMAX; ++i) prod = ar.mul(prod, rs[i]);
+            acc = ar.add(acc, negative ? ar.neg(prod) : prod);
         }
-        acc = ar.add(acc, negative ? ar.neg(prod) : prod);
     }
-    partials[(size_t)q * lanes + lane] = (uint64_t)acc;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) acc = ar.add(acc, wide_xor_lane(acc, m));
+    const uint64_t wave = lane >> 6;
+    if ((threadIdx.x & 63) == 0 && wave < waves) partials[(size_t)q * waves + wave] = (uint64_t)acc;
 }
 
 // out[q] = sum of the lanes' partials modulo moduli[q].
@@ -140,27 +153,53 @@ Mont64 make_arith<Mont64>(uint64_t q) {
 
 template <class A>
 int launch_tier(int tier, dim3 grid, hipStream_t st, const typename A::T* rs0, const typename A::T* del,
-                const typename A::T* ndel, const A* ar, int n, int cbits, uint64_t lanes, uint64_t* part) {
-    switch (tier) {
-        case 16:
-            hipLaunchKernelGGL((wide_glynn_kernel<16, A>), grid, dim3(kWideBlock), 0, st, rs0, del, ndel, ar, n,
-                               cbits, lanes, part);
-            break;
-        case 32:
-            hipLaunchKernelGGL((wide_glynn_kernel<32, A>), grid, dim3(kWideBlock), 0, st, rs0, del, ndel, ar, n,
-                               cbits, lanes, part);
-            break;
-        case 48:
-            hipLaunchKernelGGL((wide_glynn_kernel<48, A>), grid, dim3(kWideBlock), 0, st, rs0, del, ndel, ar, n,
-                               cbits, lanes, part);
-            break;
-        default:
-            hipLaunchKernelGGL((wide_glynn_kernel<64, A>), grid, dim3(kWideBlock), 0, st, rs0, del, ndel, ar, n,
-                               cbits, lanes, part);
-            break;
+                const typename A::T* ndel, const A* ar, int n, int cbits, uint64_t lanes, uint64_t waves,
+                uint64_t* part) {
+#define PHIP_WIDE_LAUNCH(NM)                                                                                   \
+    hipLaunchKernelGGL((wide_glynn_kernel<NM, A>), grid, dim3(kWideBlock), 0, st, rs0, del, ndel, ar, n, cbits, \
+                       lanes, waves, part)
+    // The tables were laid out for exactly `tier` rows per column.
+    if constexpr (sizeof(typename A::T) == 4) {
+        switch (tier) {
+            case 8: PHIP_WIDE_LAUNCH(8); break;
+            case 12: PHIP_WIDE_LAUNCH(12); break;
+            case 16: PHIP_WIDE_LAUNCH(16); break;
+            case 20: PHIP_WIDE_LAUNCH(20); break;
+            case 24: PHIP_WIDE_LAUNCH(24); break;
+            case 28: PHIP_WIDE_LAUNCH(28); break;
+            case 32: PHIP_WIDE_LAUNCH(32); break;
+            case 36: PHIP_WIDE_LAUNCH(36); break;
+            case 40: PHIP_WIDE_LAUNCH(40); break;
+            case 44: PHIP_WIDE_LAUNCH(44); break;
+            case 48: PHIP_WIDE_LAUNCH(48); break;
+            case 56: PHIP_WIDE_LAUNCH(56); break;
+            case 64: PHIP_WIDE_LAUNCH(64); break;
+            default: set_error("internal: no wide kernel for this register tier"); return -1;
+        }
+    } else {
+        switch (tier) {
+            case 8: PHIP_WIDE_LAUNCH(8); break;
+            case 16: PHIP_WIDE_LAUNCH(16); break;
+            case 24: PHIP_WIDE_LAUNCH(24); break;
+            case 32: PHIP_WIDE_LAUNCH(32); break;
+            case 40: PHIP_WIDE_LAUNCH(40); break;
+            case 48: PHIP_WIDE_LAUNCH(48); break;
+            case 56: PHIP_WIDE_LAUNCH(56); break;
+            case 64: PHIP_WIDE_LAUNCH(64); break;
+            default: set_error("internal: no wide kernel for this register tier"); return -1;
+        }
     }
+#undef PHIP_WIDE_LAUNCH
     PHIP_HIP_TRY(hipGetLastError());
     return 0;
+}
+
+// Register tier (rows per table column) for order n: 32-bit residues every
+// 4 rows from 8 to 48, then 56 and 64; 64-bit ones every 8.
+int wide_tier(int n, int min_tier, bool narrow) {
+    const int rows = std::max({8, n, min_tier});
+    if (narrow) return rows <= 48 ? (rows + 3) / 4 * 4 : (rows + 7) / 8 * 8;
+    return (rows + 7) / 8 * 8;
 }
 
 // One arithmetic class: tables, launch, fold, scale.  Buffers of the
@@ -172,7 +211,9 @@ int run_class(DeviceContext* ctx, const int64_t* A_, int64_t stride, int n, cons
     typedef typename A::T T;
     const int kk = (int)jobs.size();
     if (kk == 0) return 0;
-    std::vector<T> rs0((size_t)kk * n), del((size_t)kk * (n - 1) * n), ndel(del.size());
+    const int L = n - 1;
+    const int W = wide_tier(n, tuning().wide_min_tier, sizeof(T) == 4);  // rows per table column
+    std::vector<T> rs0((size_t)kk * W, T(0)), del((size_t)kk * (n - 1) * W, T(0)), ndel(del.size(), T(0));
     std::vector<A> ar(kk);
     std::vector<uint64_t> moduli(kk);
     for (int s = 0; s < kk; ++s) {
@@ -180,6 +221,9 @@ int run_class(DeviceContext* ctx, const int64_t* A_, int64_t stride, int n, cons
         const int64_t* M = A_ + (int64_t)jobs[s].index * stride;
         ar[s] = make_arith<A>(q);
         moduli[s] = q;
+        // The arithmetic's one (R mod q in Montgomery form) for the padding rows.
+        const uint64_t one = A::kRBits > 0 ? (uint64_t)((((unsigned __int128)1) << A::kRBits) % q) : 1 % q;
+        for (int i = n; i < W; ++i) rs0[(size_t)s * W + i] = (T)one;
         for (int i = 0; i < n; ++i) {
             uint64_t row = 0;
             for (int j = 0; j < n; ++j) {
@@ -187,20 +231,18 @@ int run_class(DeviceContext* ctx, const int64_t* A_, int64_t stride, int n, cons
                 row = add_mod_u64(row, r, q);
                 if (j < n - 1) {
                     const uint64_t d = add_mod_u64(r, r, q);
-                    const size_t at = ((size_t)s * (n - 1) + j) * n + i;
+                    const size_t at = ((size_t)s * (n - 1) + j) * W + i;
                     del[at] = (T)d;
                     ndel[at] = (T)(d ? q - d : 0);
                 }
             }
-            rs0[(size_t)s * n + i] = (T)row;
+            rs0[(size_t)s * W + i] = (T)row;
         }
     }
-    const int L = n - 1;
     const int cbits = std::min(L, std::max(std::min(L, 4), L - kWideLanesLog2));
     const uint64_t lanes = 1ull << (L - cbits);
     const uint64_t gx = (lanes + kWideBlock - 1) / kWideBlock;
-    const int fit = n <= 16 ? 16 : n <= 32 ? 32 : n <= 48 ? 48 : 64;
-    const int tier = std::max(fit, tuning().wide_min_tier);
+    const uint64_t waves = gx * (kWideBlock / 64);
 
     hipStream_t st = ctx->stream();
     auto* d_rs0 = (T*)ctx->buffer(29, std::max<size_t>(rs0.size(), 1) * sizeof(T));
@@ -208,7 +250,7 @@ int run_class(DeviceContext* ctx, const int64_t* A_, int64_t stride, int n, cons
     auto* d_ndel = (T*)ctx->buffer(31, std::max<size_t>(ndel.size(), 1) * sizeof(T));
     auto* d_ar = (A*)ctx->buffer(32, ar.size() * sizeof(A));
     auto* d_mod = (uint64_t*)ctx->buffer(33, moduli.size() * 8);
-    auto* d_part = (uint64_t*)ctx->buffer(34, (size_t)kk * lanes * 8);
+    auto* d_part = (uint64_t*)ctx->buffer(34, (size_t)kk * waves * 8);
     auto* d_sum = (uint64_t*)ctx->buffer(35, (size_t)kk * 8);
     auto* h_sum = (uint64_t*)ctx->host_buffer(6, (size_t)kk * 8);
     if (!d_rs0 || !d_del || !d_ndel || !d_ar || !d_mod || !d_part || !d_sum || !h_sum) return -1;
@@ -221,11 +263,11 @@ int run_class(DeviceContext* ctx, const int64_t* A_, int64_t stride, int n, cons
     PHIP_HIP_TRY(hipMemcpyAsync(d_mod, moduli.data(), moduli.size() * 8, hipMemcpyHostToDevice, st));
     const bool prof = ctx->profiling;
     if (prof) PHIP_HIP_TRY(hipEventRecord(ctx->prof_start, st));
-    if (launch_tier<A>(tier, dim3((unsigned)gx, (unsigned)kk), st, d_rs0, d_del, d_ndel, d_ar, n, cbits, lanes,
+    if (launch_tier<A>(W, dim3((unsigned)gx, (unsigned)kk), st, d_rs0, d_del, d_ndel, d_ar, n, cbits, lanes, waves,
                        d_part) != 0)
         return -1;
     if (prof) PHIP_HIP_TRY(hipEventRecord(ctx->prof_stop, st));
-    hipLaunchKernelGGL(wide_sum_kernel, dim3(kk), dim3(256), 0, st, d_part, lanes, d_mod, d_sum);
+    hipLaunchKernelGGL(wide_sum_kernel, dim3(kk), dim3(256), 0, st, d_part, waves, d_mod, d_sum);
     PHIP_HIP_TRY(hipGetLastError());
     PHIP_HIP_TRY(hipMemcpyAsync(h_sum, d_sum, (size_t)kk * 8, hipMemcpyDeviceToHost, st));
     PHIP_HIP_TRY(hipStreamSynchronize(st));
