@@ -78,6 +78,31 @@ int phip_affine_permanent_residues(const int64_t *C, const int64_t *L, int64_t p
 int phip_binary_blocks(const uint64_t *masks, int n, int *block_count, int *block_sizes,
                        uint64_t *block_masks);
 
+/* Live-column frontier DP of a 0/1 matrix (perm_binary.py:389-441) on the
+ * device, counts modulo k <= 64 odd moduli below 2^31.  The reference's DP
+ * runs in Python integers with no FFI; this entry replaces its state loop
+ * (perm_binary.py:412-441).  Rows come in processing order as masks over
+ * `slots` <= 64 live-column slots: row_masks[i] has the slots of row i's
+ * columns, expiring[i] (a subset of row_masks[i]) the slots of the columns
+ * whose last row it is; a slot may be reused by a column whose first row
+ * comes after the last row of the column that held it.  Returns 0 with
+ * out[q] = count mod primes[q]; 1 when a layer would hold more than
+ * max_states states (info[0] = rows processed including that one,
+ * info[1] = max_states + 1, as FrontierStateLimitExceeded reports); -1 on
+ * invalid input or a layer with 2^31 or more successor pairs. */
+int phip_binary_frontier_residues(const uint64_t *row_masks, const uint64_t *expiring, int n, int slots,
+                                  const int64_t *primes, int k, int64_t max_states, int64_t *out,
+                                  int64_t *info);
+
+/* Greedy processing order of a 0/1 matrix's rows for the frontier DP
+ * (perm_binary.py:282-386), replacing _frontier_greedy_order.  The matrix
+ * is in CSR form: row r's columns are col_idx[row_ptr[r] .. row_ptr[r+1])
+ * (distinct, in [0, n), any order).  order_out receives n row indices,
+ * *max_width the largest live-column frontier and *area the sum of the
+ * widths.  Host code only.  Returns 0 or -1. */
+int phip_frontier_order(const int32_t *row_ptr, const int32_t *col_idx, int n, int32_t *order_out, int *max_width,
+                        int64_t *area);
+
 /* perm(A) mod p for one modulus; -1 on failure.
  * Replaces permanent_modp_cuda (perm_modp_cuda.cu:740-742). */
 int64_t phip_permanent_modp(const int64_t *A, int n, int64_t p);

@@ -7,6 +7,11 @@ Three layers:
   pure-Python oracles independent of the kernels' algorithms
   (tests/test_generic_native.py:26-48 of the reference use the same two
   definitions).
+* ``frontier_count``: the live-column frontier DP over column masks in
+  Python integers (perm_binary.py:389-441), with its state cap.  Pinned in
+  tests/test_frontier_host.py against ``definition_permanent`` on random
+  small matrices, the closed forms 10!^5 (block diagonal) and 2 (odd
+  cycle), the Hall-obstruction zero and the reference's state-cap fields.
 * ``C``: ctypes binding of ``_build/liboracle.so``, the C + OpenMP
   restatement of the reference's residue kernels (perm_oracle.c).
 * ``reference_extensions()``: the reference's own OpenMP extensions compiled
@@ -178,6 +183,59 @@ def exact_glynn(matrix) -> int:
         total += sign * math.prod(rs)
     assert total % (1 << (n - 1)) == 0
     return total >> (n - 1)
+
+
+class FrontierCapExceeded(Exception):
+    """A layer of frontier_count would hold more than max_states states."""
+
+    def __init__(self, max_states: int, observed: int, rows: int):
+        super().__init__(max_states, observed, rows)
+        self.max_states, self.observed_states, self.processed_rows = max_states, observed, rows
+
+
+def frontier_count(row_masks, row_order, max_states: int = 1 << 62, layer_sizes=None) -> int:
+    """Perfect matchings of a 0/1 matrix by the live-column frontier DP
+    (perm_binary.py:389-441): a state is the set of used columns that some
+    later row still has; each row adds one available column, and the
+    columns whose last row it is must all be used and then leave the state.
+    Raises FrontierCapExceeded(max_states, max_states + 1, row position + 1)
+    when a new state would make a layer larger than max_states.  When
+    layer_sizes is a list, the size of every layer is appended to it."""
+    n = len(row_masks)
+    if n == 0:
+        return 1
+    last = [-1] * n
+    for pos, r in enumerate(row_order):
+        m = int(row_masks[r])
+        for c in range(n):
+            if (m >> c) & 1:
+                last[c] = pos
+    if min(last) < 0:
+        return 0
+    expiring = [0] * n
+    for c, pos in enumerate(last):
+        expiring[pos] |= 1 << c
+    states = {0: 1}
+    for pos, r in enumerate(row_order):
+        nxt = {}
+        for used, count in states.items():
+            for c in range(n):
+                bit = 1 << c
+                if not (int(row_masks[r]) & bit) or (used & bit):
+                    continue
+                target = used | bit
+                if target & expiring[pos] != expiring[pos]:
+                    continue
+                target &= ~expiring[pos]
+                if target not in nxt and len(nxt) >= max_states:
+                    raise FrontierCapExceeded(max_states, max_states + 1, pos + 1)
+                nxt[target] = nxt.get(target, 0) + count
+        if layer_sizes is not None:
+            layer_sizes.append(len(nxt))
+        states = nxt
+        if not states:
+            return 0
+    return states.get(0, 0)
 
 
 # ---------------------------------------------------------------------------

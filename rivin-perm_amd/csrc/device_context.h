@@ -25,7 +25,7 @@ const std::string& last_error();
 
 class DeviceContext {
    public:
-    static constexpr int kSlots = 40;
+    static constexpr int kSlots = 56;
 
     // Context of the calling thread's current device, created on first use.
     // Returns nullptr (with the error set) when no device is usable.

@@ -156,3 +156,133 @@ extern "C" int phip_binary_blocks(const uint64_t* masks, int n, int* block_count
     *block_count = blocks;
     return 1;
 }
+
+// Greedy row order with a small live-column frontier for the frontier DP
+// (perm_binary.py:282-386): a bounded number of starting rows (work budget
+// 2,000,000 / (n^2 * ceil(average degree)), spread evenly over the rows
+// ranked by (degree, index)); each step takes the row minimising (maximum
+// width so far, new width, -closed columns, opened columns, row index); the
+// best start minimises (maximum width, frontier area, order).  The matrix is
+// given in CSR form (row r's columns are col_idx[row_ptr[r] .. row_ptr[r+1]),
+// in any order, no repeats), so any order n works.  Returns 0 and writes the
+// order, or -1.
+extern "C" int phip_frontier_order(const int32_t* row_ptr, const int32_t* col_idx, int n, int32_t* order_out,
+                                   int* max_width, int64_t* area) {
+    if (n < 0) {
+        phip::set_error("frontier order needs n >= 0");
+        return -1;
+    }
+    if (!max_width || !area || (n > 0 && (!row_ptr || !col_idx || !order_out))) {
+        phip::set_error("null argument");
+        return -1;
+    }
+    *max_width = 0;
+    *area = 0;
+    if (n == 0) return 0;
+    if (row_ptr[0] != 0) {
+        phip::set_error("row_ptr must start at 0");
+        return -1;
+    }
+    std::vector<std::vector<int>> cols(n);
+    std::vector<int> degree(n, 0);
+    std::vector<int> last_row(n, -1);
+    int64_t entries = 0;
+    for (int r = 0; r < n; ++r) {
+        if (row_ptr[r + 1] < row_ptr[r]) {
+            phip::set_error("row_ptr must be nondecreasing");
+            return -1;
+        }
+        for (int32_t e = row_ptr[r]; e < row_ptr[r + 1]; ++e) {
+            const int c = col_idx[e];
+            if (c < 0 || c >= n) {
+                phip::set_error("row mask contains a column outside the matrix");
+                return -1;
+            }
+            if (last_row[c] == r) {
+                phip::set_error("repeated column in a row");
+                return -1;
+            }
+            last_row[c] = r;
+            cols[r].push_back(c);
+            ++degree[c];
+            ++entries;
+        }
+    }
+    const int64_t avg_ceil = std::max<int64_t>(1, (entries + n - 1) / n);
+    const int64_t work_per_start = std::max<int64_t>(1, (int64_t)n * n * avg_ceil);
+    const int start_count = (int)std::min<int64_t>(n, std::max<int64_t>(1, 2000000 / work_per_start));
+    std::vector<int> ranked(n);
+    for (int r = 0; r < n; ++r) ranked[r] = r;
+    std::stable_sort(ranked.begin(), ranked.end(), [&](int a, int b) {
+        return cols[a].size() != cols[b].size() ? cols[a].size() < cols[b].size() : a < b;
+    });
+    std::vector<int> starts;
+    if (start_count == n) {
+        starts = ranked;
+    } else if (start_count == 1) {
+        starts.push_back(ranked[0]);
+    } else {
+        for (int i = 0; i < start_count; ++i) starts.push_back(ranked[(int64_t)i * (n - 1) / (start_count - 1)]);
+    }
+    bool have_best = false;
+    int best_width = 0;
+    int64_t best_area = 0;
+    std::vector<int> best_order;
+    for (int start : starts) {
+        std::vector<char> selected(n, 0);
+        std::vector<int> processed(n, 0), order;
+        int width = 0, maximum = 0;
+        int64_t frontier_area = 0;
+        for (int step = 0; step < n; ++step) {
+            bool found = false;
+            int b_max = 0, b_next = 0, b_negclosed = 0, b_opened = 0, b_row = 0;
+            for (int row = (step == 0 ? start : 0); row < (step == 0 ? start + 1 : n); ++row) {
+                if (selected[row]) continue;
+                int opened = 0, closed = 0;
+                for (int c : cols[row]) {
+                    if (processed[c] == 0 && degree[c] > 1) ++opened;
+                    if (processed[c] > 0 && processed[c] == degree[c] - 1) ++closed;
+                }
+                const int next = width + opened - closed;
+                const int s0 = std::max(maximum, next);
+                // Lexicographic (s0, next, -closed, opened, row).
+                const bool better = !found || s0 < b_max ||
+                                    (s0 == b_max && (next < b_next ||
+                                    (next == b_next && (-closed < b_negclosed ||
+                                    (-closed == b_negclosed && (opened < b_opened ||
+                                    (opened == b_opened && row < b_row)))))));
+                if (better) {
+                    found = true;
+                    b_max = s0;
+                    b_next = next;
+                    b_negclosed = -closed;
+                    b_opened = opened;
+                    b_row = row;
+                }
+            }
+            if (!found) {
+                phip::set_error("frontier ordering failed to select a row");
+                return -1;
+            }
+            selected[b_row] = 1;
+            order.push_back(b_row);
+            for (int c : cols[b_row]) ++processed[c];
+            width = b_next;
+            maximum = std::max(maximum, width);
+            frontier_area += width;
+        }
+        const bool better = !have_best || maximum < best_width ||
+                            (maximum == best_width && (frontier_area < best_area ||
+                            (frontier_area == best_area && order < best_order)));
+        if (better) {
+            have_best = true;
+            best_width = maximum;
+            best_area = frontier_area;
+            best_order = order;
+        }
+    }
+    for (int i = 0; i < n; ++i) order_out[i] = best_order[i];
+    *max_width = best_width;
+    *area = best_area;
+    return 0;
+}

@@ -7,10 +7,15 @@ is sent to the device with all of its CRT primes in one launch; bounds, prime
 plans and CRT stay on the host in exact integers.
 
 Backends: 'auto' and 'native' both select the device (``_binary_ext`` is the
-GPU object shaped like the reference's native extension).  The reference's
-portable Python subset DP and sparse frontier DP are not part of this build
-(backend='python' raises), and blocks of order 64 or more are rejected, as
-the reference's native backend does.
+GPU object shaped like the reference's native extension).  As in the
+reference, 'auto' first tries the live-column frontier DP on sparse blocks
+of order >= 20 with a narrow frontier (perm_binary.py:475-516, 975-990) and
+falls back to Glynn when a layer exceeds the state cap; the frontier DP
+runs on the device too (phip_binary_frontier_residues), with counts kept
+modulo the same descending CRT primes.  The reference's portable Python
+subset DP is not part of this build (backend='python' raises), so blocks of
+order 64 or more that the frontier does not take are rejected, as the
+reference's native backend does.
 """
 
 from __future__ import annotations
@@ -47,6 +52,31 @@ def _select_backend(backend) -> str:
     raise ValueError("backend must be one of: 'auto', 'native', 'python'")
 
 
+class FrontierStateLimitExceeded(RuntimeError):
+    """Raised when a frontier layer would exceed its configured state cap
+    (perm_binary.py:42-58)."""
+
+    def __init__(self, max_states: int, observed_states: int, processed_rows: int) -> None:
+        self.max_states = max_states
+        self.observed_states = observed_states
+        self.processed_rows = processed_rows
+        super().__init__(
+            f"frontier state limit {max_states} exceeded while processing row {processed_rows}; "
+            f"at least {observed_states} states would be required"
+        )
+
+
+class _FrontierTooWide(ValueError):
+    """More columns live during one row than the device state mask holds."""
+
+
+_FRONTIER_ORDERING_WORK_BUDGET = _prep.FRONTIER_ORDERING_WORK_BUDGET
+_AUTO_FRONTIER_MIN_ORDER = _prep.AUTO_FRONTIER_MIN_ORDER
+_AUTO_FRONTIER_MAX_AVERAGE_DEGREE = _prep.AUTO_FRONTIER_MAX_AVERAGE_DEGREE
+_AUTO_FRONTIER_MAX_WIDTH = _prep.AUTO_FRONTIER_MAX_WIDTH
+_AUTO_FRONTIER_MAX_STATES = _prep.AUTO_FRONTIER_MAX_STATES
+
+
 def _block_array(block_masks) -> np.ndarray:
     return np.array(_prep.unpack_masks(block_masks), dtype=np.uint8)
 
@@ -60,16 +90,84 @@ def _checked_residues(block: np.ndarray, primes) -> list:
     return residues
 
 
-def _exact_block(block_masks) -> int:
-    """Nonnegative exact permanent of one irreducible block (perm_binary.py:843-863)."""
-    rows = _prep.unpack_masks(block_masks)
-    bound = _prep.normalized_bound(rows)
+def _native_exact_permanent(block_matrix, bound: int, threads: int) -> int:
+    """Nonnegative exact permanent of one irreducible block from device
+    residues modulo descending 31-bit primes (perm_binary.py:843-863).
+    ``threads`` is accepted for the reference's signature; the device
+    decides its own parallelism."""
     primes = _prep.crt_primes_above_bound(bound)
-    residues = _checked_residues(np.array(rows, dtype=np.uint8), primes)
+    residues = _checked_residues(np.array(block_matrix, dtype=np.uint8), primes)
     result = _prep.crt_nonnegative(residues, primes)
     if result > bound:
         raise RuntimeError("native binary-permanent residues reconstructed outside the bound")
     return result
+
+
+def _frontier_permanent_masks(row_masks, row_order, max_states: int) -> int:
+    """Exact count of the live-column frontier DP for validated masks and
+    row order (perm_binary.py:389-441).  The layers run on the device with
+    the counts modulo descending 31-bit primes whose product exceeds the
+    binary bound (the same primes as the Glynn path), CRT on the host; a
+    layer above max_states raises FrontierStateLimitExceeded with the
+    reference's fields.  Raises _FrontierTooWide (a ValueError) when more
+    than 64 columns are live during one row."""
+    order = len(row_masks)
+    if order == 0:
+        return 1
+    layers = _prep.frontier_layers(row_masks, row_order)
+    if layers is None:
+        return 0
+    row_slots, expiring, slots = layers
+    if slots > _prep.FRONTIER_MAX_SLOTS:
+        raise _FrontierTooWide(
+            f"this row order keeps {slots} columns live at once; the device frontier DP holds at most "
+            f"{_prep.FRONTIER_MAX_SLOTS}"
+        )
+    bound = _prep.normalized_bound(_prep.unpack_masks(row_masks))
+    primes = _prep.crt_primes_above_bound(bound)
+    residues = []
+    step = _native.FRONTIER_MAX_PRIMES
+    for start in range(0, len(primes), step):
+        chunk = primes[start:start + step]
+        got, exceeded = _native.binary_frontier_residues(row_slots, expiring, slots, chunk, max_states)
+        if exceeded is not None:
+            raise FrontierStateLimitExceeded(max_states, exceeded[1], exceeded[0])
+        for r, p in zip(got, chunk):
+            if not 0 <= r < p:
+                raise RuntimeError("frontier backend returned an invalid residue")
+        residues.extend(got)
+    result = _prep.crt_nonnegative(residues, primes)
+    if result > bound:
+        raise RuntimeError("frontier residues reconstructed outside the bound")
+    return result
+
+
+def binary_permanent_frontier(matrix, *, row_order=None, max_states: int = 500_000,
+                              strict: bool = True) -> int:
+    """Exact binary permanent by the live-column frontier DP
+    (perm_binary.py:444-472).
+
+    ``row_order`` is None for the deterministic greedy order or an exact
+    permutation of the row indices.  At most ``max_states`` distinct states
+    may be held in any layer; exceeding the cap raises
+    FrontierStateLimitExceeded.  Infeasible matrices return zero before the
+    capped recurrence.  The result is an arbitrary-size Python integer.
+    """
+    max_states = _prep.validate_frontier_max_states(max_states)
+    rows = _normalize_binary_matrix(matrix, strict=strict)
+    order = _prep.validate_frontier_row_order(row_order, len(rows))
+    masks = _row_masks(rows)
+    if _perfect_matching(masks) is None:
+        return 0
+    if order is None:
+        order, _, _ = _frontier_greedy_order(masks)
+    return _frontier_permanent_masks(masks, order, max_states)
+
+
+def _auto_frontier_plan(row_masks):
+    """Orientation and row order for the auto fast path, or None
+    (perm_binary.py:475-516)."""
+    return _prep.auto_frontier_plan(row_masks)
 
 
 def binary_permanent_bound(matrix, *, strict: bool = True) -> int:
@@ -116,9 +214,11 @@ def binary_permanent(matrix, *, backend: str = "auto", parallel: bool = True,
     """Exact nonnegative permanent of a square 0/1 matrix (perm_binary.py:943-1005)."""
     if not isinstance(parallel, bool):
         raise TypeError("parallel must be a boolean")
-    _prep.validate_num_threads(num_threads)
+    num_threads = _prep.validate_num_threads(num_threads)
+    threads = num_threads if parallel else 1
     rows = _prep.normalize_binary_matrix(matrix, strict=strict)
     _select_backend(backend)
+    automatic = backend.lower() == "auto"
     if not rows:
         return 1
     if len(rows) == 1:
@@ -130,28 +230,44 @@ def binary_permanent(matrix, *, backend: str = "auto", parallel: bool = True,
     for block in blocks:
         if len(block) == 1:
             continue
+        if automatic:
+            plan = _auto_frontier_plan(block)
+            if plan is not None:
+                try:
+                    result *= _frontier_permanent_masks(plan[0], plan[1], _AUTO_FRONTIER_MAX_STATES)
+                    continue
+                except (FrontierStateLimitExceeded, _FrontierTooWide):
+                    pass
         if len(block) >= 64:
             raise ValueError("native binary-permanent blocks must have order less than 64")
-        result *= _exact_block(block)
+        block_matrix = _unpack_masks(block)
+        result *= _native_exact_permanent(block_matrix, _normalized_bound(block_matrix), threads)
     return result
 
 
 # Private helper names of the reference module (perm_binary.py:173-227,
 # 538-814), used by perm_binary_cuda as they are in the reference.
 _crt_primes_above_bound = _prep.crt_primes_above_bound
+_frontier_greedy_order = _prep.frontier_greedy_order
 _is_prime = _prep.is_prime_u64
 _matching_blocks = _prep.matching_blocks
 _normalize_binary_matrix = _prep.normalize_binary_matrix
 _normalized_bound = _prep.normalized_bound
+_perfect_matching = _prep.perfect_matching
 _row_masks = _prep.row_masks
+_transpose_row_masks = _prep.transpose_masks
 _unpack_masks = _prep.unpack_masks
 _validate_residue_prime = _prep.validate_residue_prime
+_validate_frontier_max_states = _prep.validate_frontier_max_states
+_validate_frontier_row_order = _prep.validate_frontier_row_order
 
 __all__ = [
+    "FrontierStateLimitExceeded",
     "HAS_BINARY_EXT",
     "available_binary_backends",
     "binary_permanent",
     "binary_permanent_bound",
+    "binary_permanent_frontier",
     "binary_permanent_mod_p",
     "get_binary_backend",
 ]

@@ -38,6 +38,16 @@ _SIGNATURES = {
         [_I64P, ctypes.c_int64, ctypes.c_int, _I64P, ctypes.c_int, _I64P],
     ),
     "phip_permanent_modp": (ctypes.c_int64, [_I64P, ctypes.c_int, ctypes.c_int64]),
+    "phip_binary_frontier_residues": (
+        ctypes.c_int,
+        [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64), ctypes.c_int, ctypes.c_int, _I64P,
+         ctypes.c_int, ctypes.c_int64, _I64P, _I64P],
+    ),
+    "phip_frontier_order": (
+        ctypes.c_int,
+        [ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
+         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int64)],
+    ),
     "phip_binary_blocks": (
         ctypes.c_int,
         [ctypes.POINTER(ctypes.c_uint64), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
@@ -77,6 +87,8 @@ _SIGNATURES = {
 TUNE_LANES_LOG2 = 1
 TUNE_MIN_CBITS = 2
 TUNE_BATCH_ITEMS_LOG2 = 3
+# Moduli per phip_binary_frontier_residues call.
+FRONTIER_MAX_PRIMES = 64
 TUNE_BATCH_BLOCKS = 4
 TUNE_NECKLACE_MIN_ORDER = 5
 TUNE_NECKLACE_DEPTH = 6
@@ -216,6 +228,45 @@ def binary_blocks(masks):
         blocks.append(tuple(int(out[pos + i]) for i in range(k)))
         pos += k
     return True, tuple(blocks)
+
+
+def frontier_order(columns):
+    """(row order, maximum frontier width, frontier area) of the greedy
+    ordering for the frontier DP, for a 0/1 matrix given as the column
+    lists of its rows: host code in the library."""
+    lib = load()
+    n = len(columns)
+    ptr = np.zeros(n + 1, dtype=np.int32)
+    ptr[1:] = np.cumsum([len(c) for c in columns], dtype=np.int64)
+    idx = np.ascontiguousarray(np.fromiter((j for c in columns for j in c), dtype=np.int32, count=int(ptr[-1])))
+    order = np.zeros(max(1, n), dtype=np.int32)
+    width = ctypes.c_int()
+    area = ctypes.c_int64()
+    i32 = ctypes.POINTER(ctypes.c_int32)
+    status = lib.phip_frontier_order(ptr.ctypes.data_as(i32), idx.ctypes.data_as(i32) if idx.size else None, n,
+                                     order.ctypes.data_as(i32), ctypes.byref(width), ctypes.byref(area))
+    _check(status, "phip_frontier_order")
+    return tuple(int(v) for v in order[:n]), int(width.value), int(area.value)
+
+
+def binary_frontier_residues(row_slots, expiring, slots: int, primes, max_states: int):
+    """Frontier-DP counts modulo each prime (at most 64 moduli, odd and
+    below 2^31) for rows given as masks over live-column slots; returns
+    (residues, None), or (None, (rows processed, observed states)) when a
+    layer exceeds max_states."""
+    lib = require_device()
+    n = len(row_slots)
+    rm = (ctypes.c_uint64 * max(1, n))(*[int(m) for m in row_slots])
+    ex = (ctypes.c_uint64 * max(1, n))(*[int(m) for m in expiring])
+    ps = np.ascontiguousarray(np.asarray(list(primes), dtype=np.int64))
+    out = np.zeros(max(1, len(ps)), dtype=np.int64)
+    info = np.zeros(2, dtype=np.int64)
+    status = lib.phip_binary_frontier_residues(rm, ex, n, int(slots), _i64ptr(ps), len(ps), int(max_states),
+                                               _i64ptr(out), _i64ptr(info))
+    _check(status, "phip_binary_frontier_residues")
+    if status == 1:
+        return None, (int(info[0]), int(info[1]))
+    return [int(v) for v in out[: len(ps)]], None
 
 
 def affine_permanent_residues(constant: np.ndarray, linear: np.ndarray, moduli) -> list:

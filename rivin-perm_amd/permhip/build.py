@@ -45,6 +45,7 @@ PLAIN_SOURCES = (
     "wide.hip",
     "affine.hip",
     "host_prep.cpp",
+    "frontier.hip",
 )
 
 

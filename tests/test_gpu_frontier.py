@@ -1,0 +1,272 @@
+"""Live-column frontier DP on the device (perm_binary.py:389-516) against
+the oracle's frontier_count, the permutation definition, closed forms and
+the Glynn path; layer sizes through the state cap; the reference's own
+frontier cases (tests/test_binary_frontier.py)."""
+
+import math
+import random
+
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_masks(rng, n, density, diagonal=True):
+    masks = [sum(1 << c for c in range(n) if rng.random() < density) for _ in range(n)]
+    if diagonal:
+        perm = list(range(n))
+        rng.shuffle(perm)
+        masks = [m | (1 << perm[r]) for r, m in enumerate(masks)]
+    return tuple(masks)
+
+
+def _cycle_adjacency(order):
+    matrix = [[0] * order for _ in range(order)]
+    for v in range(order):
+        u = (v + 1) % order
+        matrix[v][u] = matrix[u][v] = 1
+    return matrix
+
+
+def _block_diagonal_ones(size, count):
+    n = size * count
+    return [[int(r // size == c // size) for c in range(n)] for r in range(n)]
+
+
+def _tridiagonal(n):
+    return [[int(abs(r - c) <= 1) for c in range(n)] for r in range(n)]
+
+
+def _fibonacci(k):
+    a, b = 0, 1
+    for _ in range(k):
+        a, b = b, a + b
+    return a
+
+
+def test_frontier_matches_definition_on_random_small_matrices(native):
+    import perm_binary
+    from permhip import binary_prep as prep
+
+    rng = random.Random(20261203)
+    for n in range(1, 8):
+        for _ in range(6):
+            masks = _random_masks(rng, n, 0.5, diagonal=False)
+            matrix = prep.unpack_masks(masks)
+            expected = oracle.definition_permanent(matrix)
+            print(f"frontier random n={n}", flush=True)
+            assert perm_binary.binary_permanent_frontier(matrix) == expected
+            order = list(range(n))
+            rng.shuffle(order)
+            assert perm_binary.binary_permanent_frontier(matrix, row_order=order) == expected
+
+
+@pytest.mark.parametrize("n,density", [(20, 0.12), (30, 0.08), (40, 0.06), (60, 0.04)])
+def test_frontier_matches_oracle_and_glynn_on_sparse_blocks(native, n, density):
+    import perm_binary
+    from permhip import binary_prep as prep
+
+    rng = random.Random(n * 7919)
+    checked = 0
+    for _ in range(4):
+        masks = _random_masks(rng, n, density)
+        order, _, _ = prep.frontier_greedy_order(masks)
+        try:
+            expected = oracle.frontier_count(masks, order, max_states=200_000)
+        except oracle.FrontierCapExceeded:
+            continue
+        matrix = prep.unpack_masks(masks)
+        print(f"frontier sparse n={n}", flush=True)
+        got = perm_binary.binary_permanent_frontier(matrix, max_states=200_000)
+        assert got == expected
+        if n <= 40:
+            assert perm_binary.binary_permanent(matrix, backend="native") == expected
+        checked += 1
+    assert checked
+
+
+def test_frontier_layer_sizes_match_the_oracle_through_the_cap(native):
+    """The device's layer sizes equal the oracle's: a cap equal to the
+    largest layer passes, one below it stops at that layer's row."""
+    import perm_binary
+    from permhip import binary_prep as prep
+
+    rng = random.Random(4242)
+    for n in (12, 24, 36):
+        masks = _random_masks(rng, n, 0.12)
+        order, _, _ = prep.frontier_greedy_order(masks)
+        sizes = []
+        expected = oracle.frontier_count(masks, order, layer_sizes=sizes)
+        matrix = prep.unpack_masks(masks)
+        widest = max(sizes)
+        print(f"frontier cap n={n} widest={widest}", flush=True)
+        assert perm_binary.binary_permanent_frontier(matrix, row_order=order, max_states=widest) == expected
+        if widest > 1:
+            first = next(i for i, s in enumerate(sizes) if s > widest - 1)
+            with pytest.raises(perm_binary.FrontierStateLimitExceeded) as caught:
+                perm_binary.binary_permanent_frontier(matrix, row_order=order, max_states=widest - 1)
+            assert caught.value.processed_rows == first + 1
+            assert caught.value.observed_states == widest
+
+
+def test_frontier_uses_arbitrary_size_python_integers(native):
+    import perm_binary
+
+    result = perm_binary.binary_permanent_frontier(_block_diagonal_ones(10, 5), max_states=1_000)
+    assert result == math.factorial(10) ** 5
+    assert result.bit_length() > 64 and isinstance(result, int)
+
+
+def test_frontier_chunks_many_primes(native, monkeypatch):
+    """A bound above 64 primes is evaluated in chunks; forcing chunks of
+    two primes must give the same integer."""
+    import perm_binary
+    from permhip import _native
+
+    matrix = _block_diagonal_ones(8, 6)
+    expected = math.factorial(8) ** 6
+    monkeypatch.setattr(_native, "FRONTIER_MAX_PRIMES", 2)
+    assert perm_binary.binary_permanent_frontier(matrix) == expected
+
+
+def test_frontier_accepts_64_moduli_in_one_call(native):
+    from permhip import binary_prep as prep
+
+    masks = prep.row_masks(_tridiagonal(70))
+    row_slots, expiring, slots = prep.frontier_layers(masks, range(70))
+    primes = [prep.descending_prime(i) for i in range(64)]
+    got, exceeded = native.binary_frontier_residues(row_slots, expiring, slots, primes, 1000)
+    assert exceeded is None
+    assert got == [_fibonacci(71) % p for p in primes]
+    with pytest.raises(RuntimeError, match="frontier"):
+        native.binary_frontier_residues(row_slots, expiring, slots, primes + [3], 1000)
+
+
+def test_frontier_rejects_masks_outside_the_slots(native):
+    with pytest.raises(RuntimeError, match="slots"):
+        native.binary_frontier_residues([0b100], [0b100], 2, [2147483647], 10)
+    with pytest.raises(RuntimeError, match="slots"):
+        native.binary_frontier_residues([0b01], [0b10], 2, [2147483647], 10)
+
+
+def test_frontier_state_cap_raises_typed_exception_immediately(native):
+    import perm_binary
+
+    matrix = [[1] * 8 for _ in range(8)]
+    with pytest.raises(perm_binary.FrontierStateLimitExceeded, match="state limit 5 exceeded") as caught:
+        perm_binary.binary_permanent_frontier(matrix, row_order=range(8), max_states=5)
+    assert caught.value.max_states == 5
+    assert caught.value.observed_states == 6
+    assert caught.value.processed_rows == 1
+
+
+def test_frontier_conventions_on_device(native):
+    import perm_binary
+
+    assert perm_binary.binary_permanent_frontier([[1]]) == 1
+    assert perm_binary.binary_permanent_frontier([[1, 1], [1, 1]], row_order=(1, 0)) == 2
+    assert perm_binary.binary_permanent_frontier([[3, 0], [1, 7]], strict=False) == 1
+    assert perm_binary.binary_permanent_frontier(_cycle_adjacency(21)) == 2
+
+
+def test_auto_uses_frontier_for_eligible_sparse_block(native, monkeypatch):
+    import perm_binary
+
+    def fail_native(*args, **kwargs):
+        raise AssertionError("eligible sparse auto input should use frontier")
+
+    monkeypatch.setattr(perm_binary, "_native_exact_permanent", fail_native)
+    assert perm_binary.binary_permanent(_cycle_adjacency(21), backend="auto") == 2
+
+
+def test_auto_frontier_cap_falls_back_to_native(native, monkeypatch):
+    import perm_binary
+
+    calls = []
+
+    def fake_native(argument, bound, threads):
+        calls.append((argument, bound, threads))
+        return 2
+
+    monkeypatch.setattr(perm_binary, "_AUTO_FRONTIER_MAX_STATES", 1)
+    monkeypatch.setattr(perm_binary, "_native_exact_permanent", fake_native)
+    assert perm_binary.binary_permanent(_cycle_adjacency(21), backend="auto") == 2
+    assert len(calls) == 1
+
+
+def test_auto_frontier_cap_fallback_is_exact(native, monkeypatch):
+    import perm_binary
+
+    monkeypatch.setattr(perm_binary, "_AUTO_FRONTIER_MAX_STATES", 1)
+    assert perm_binary.binary_permanent(_cycle_adjacency(21), backend="auto") == 2
+
+
+def test_explicit_native_backend_bypasses_frontier(native, monkeypatch):
+    import perm_binary
+
+    calls = []
+
+    def fail_plan(*args, **kwargs):
+        raise AssertionError("explicit native must not plan a frontier")
+
+    def fake_native(argument, bound, threads):
+        calls.append((argument, bound, threads))
+        return 2
+
+    monkeypatch.setattr(perm_binary, "_auto_frontier_plan", fail_plan)
+    monkeypatch.setattr(perm_binary, "_native_exact_permanent", fake_native)
+    assert perm_binary.binary_permanent(_cycle_adjacency(21), backend="native") == 2
+    assert len(calls) == 1
+
+
+def test_auto_handles_sparse_blocks_of_order_64_and_more(native):
+    """A tridiagonal band of order 120 is one irreducible block with
+    permanent F(121); auto takes it through the frontier, native rejects it
+    as the reference's native backend does."""
+    import perm_binary
+
+    band = _tridiagonal(120)
+    assert perm_binary.binary_permanent(band) == _fibonacci(121)
+    with pytest.raises(ValueError, match="less than 64"):
+        perm_binary.binary_permanent(band, backend="native")
+    assert perm_binary.binary_permanent_frontier(band) == _fibonacci(121)
+
+
+def test_large_order_with_small_blocks(native):
+    """Order 100 in ten all-ones blocks of 10: the block split runs on the
+    host restatement (masks wider than 64 bits), each block on Glynn."""
+    import perm_binary
+
+    matrix = _block_diagonal_ones(10, 10)
+    assert perm_binary.binary_permanent(matrix) == math.factorial(10) ** 10
+    assert perm_binary.binary_permanent_bound(matrix) >= math.factorial(10) ** 10
+
+
+def _kasteleyn(m, n):
+    """Domino tilings of an m x n board (Kasteleyn's product formula)."""
+    value = 1.0
+    for j in range(1, (m + 1) // 2 + 1):
+        for k in range(1, (n + 1) // 2 + 1):
+            value *= 4 * math.cos(math.pi * j / (m + 1)) ** 2 + 4 * math.cos(math.pi * k / (n + 1)) ** 2
+    return round(value)
+
+
+@pytest.mark.parametrize("shape", [(6, 8), (8, 10), (4, 30)])
+def test_grid_domino_tilings_through_the_frontier(native, shape):
+    """Perfect matchings of an open 2D grid (domino tilings, Kasteleyn's
+    closed form): the biadjacency block is sparse (average degree < 4) with
+    a frontier as wide as the short side, so auto plans a frontier."""
+    import perm_binary
+    from permhip import binary_prep as prep
+    from permhip import grids
+
+    matrix = grids.grid_biadjacency(shape)
+    feasible, blocks = prep.matching_blocks(prep.row_masks(matrix))
+    assert feasible and len(blocks) == 1
+    assert prep.auto_frontier_plan(blocks[0]) is not None
+    expected = _kasteleyn(*shape)
+    print(f"frontier grid {shape} -> {expected}", flush=True)
+    assert perm_binary.binary_permanent(matrix) == expected
+    assert perm_binary.binary_permanent_frontier(matrix) == expected
