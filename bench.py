@@ -508,7 +508,7 @@ def main() -> int:
     rates = _native.valu_microbench()
     log("VALU lane-ops/s: " + ", ".join(f"{k}={v:.3e}" for k, v in rates.items()))
     st = plan.stats()
-    group, kset, paired = int(st["group"]), int(st["set"]), bool(st.get("paired", 0))
+    group, kset, paired = int(st["group"]), int(st["set"]), int(st.get("variant", 0)) == 1
     roof = roofline(HEADLINE_N, group, kset, kernel_rate, paired)
     # Every mulmod as one signed Montgomery product plus one row-sum add: the
     # ceiling of a kernel that multiplies each row sum in modular arithmetic.

@@ -174,9 +174,10 @@ int phip_batch_set_option(void *plan, int option, int64_t value);
  * wavefronts, [7] row-sum group size and [9] prime-set size of the first
  * launch, [8] residues, [10] and [11] smallest and largest chunk (log2 of
  * the Gray states per lane), [12] work-item target (log2) the chunks were
- * sized for, [13] workgroups per CU, [14] 1 when the first launch pairs
- * consecutive states (prime sets below 1.25 * 2^30).  Returns the number of
- * values written or -1. */
+ * sized for, [13] workgroups per CU, [14] kernel variant of the first
+ * launch: 0 plain, 1 pairs consecutive states (prime sets below
+ * 1.25 * 2^30), 2 skips zero terms (0/1 tables whose states are mostly
+ * zero).  Returns the number of values written or -1. */
 int phip_batch_stats(void *plan, double *out, int len);
 
 int phip_batch_destroy(void *plan);
@@ -193,6 +194,9 @@ int phip_batch_destroy(void *plan);
 #define PHIP_TUNE_NECKLACE_DEPTH 6   /* Schur necklaces: largest FKM prefix depth of the tasks (2..30) */
 #define PHIP_TUNE_WIDE_MIN_TIER 7    /* wide-modulus Glynn: smallest row-register tier (8..64, a multiple of 4) */
 #define PHIP_TUNE_AFFINE_MIN_TIER 8  /* affine Glynn: smallest row-register tier (8..32, a multiple of 4) */
+#define PHIP_TUNE_SKIP_ZEROS_PERMILLE 9 /* 0/1 tables: largest estimated share of nonzero Gray terms, in
+                                           permille, that runs on the zero-skipping kernels (0..1000, 0 off;
+                                           read when a plan is built) */
 int phip_set_tuning(int key, int64_t value);
 int64_t phip_get_tuning(int key);
 

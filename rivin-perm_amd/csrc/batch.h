@@ -18,6 +18,11 @@ constexpr int kBatchMaxOrder = 46;
 // partial sums.
 constexpr int kMaxSet = 8;
 
+// Kernel variants of a (group, set) class.
+constexpr int kVariantPlain = 0;
+constexpr int kVariantPaired = 1;     // consecutive states paired (primes below kPairedPrimeLimit)
+constexpr int kVariantSkipZeros = 2;  // zero terms skipped through a per-wavefront queue
+
 // One job: one table (a matrix under one row order) and a set of K odd
 // moduli below 2^31 that share it (K = 1 for residue rows).  56 bytes,
 // read with scalar loads.
@@ -32,7 +37,7 @@ struct BatchJob {
     int32_t group;
     int32_t nprimes;     // real primes (<= K); padded entries repeat the last one
     int32_t set;         // K of the job's launch class
-    int32_t paired;      // 1: the class pairs consecutive states (primes below 1.25 * 2^30)
+    int32_t variant;     // kVariantPlain, kVariantPaired or kVariantSkipZeros
     int32_t pad;
 };
 
@@ -45,11 +50,11 @@ struct BatchSlot {
 };
 
 // One persistent launch: the jobs of one (row-sum group, set size,
-// paired) class.
+// variant) class.
 struct BatchClass {
     int group;
     int set;
-    int paired;
+    int variant;
     uint32_t first_item;
     uint32_t n_items;
 };
@@ -151,5 +156,17 @@ PHIP_DECLARE_BATCH_LAUNCH_PAIRED(4, 8)
 PHIP_DECLARE_BATCH_LAUNCH_PAIRED(6, 4)
 PHIP_DECLARE_BATCH_LAUNCH_PAIRED(6, 8)
 #undef PHIP_DECLARE_BATCH_LAUNCH_PAIRED
+// Zero-skipping variants (0/1 tables whose states are mostly zero terms).
+#define PHIP_DECLARE_BATCH_LAUNCH_SKIP(G, K)                                                                   \
+    hipError_t launch_batch_g##G##_k##K##_skip(dim3, hipStream_t, const BatchJob*, const uint2*, uint32_t,   \
+                                             uint32_t*, const int32_t*, const int32_t*, const uint32_t*,   \
+                                             uint64_t*);
+PHIP_DECLARE_BATCH_LAUNCH_SKIP(4, 2)
+PHIP_DECLARE_BATCH_LAUNCH_SKIP(4, 4)
+PHIP_DECLARE_BATCH_LAUNCH_SKIP(4, 8)
+PHIP_DECLARE_BATCH_LAUNCH_SKIP(6, 2)
+PHIP_DECLARE_BATCH_LAUNCH_SKIP(6, 4)
+PHIP_DECLARE_BATCH_LAUNCH_SKIP(6, 8)
+#undef PHIP_DECLARE_BATCH_LAUNCH_SKIP
 
 }  // namespace phip

@@ -12,8 +12,10 @@
 // arithmetic.
 //
 // This file is compiled once per (G, K) class: PHIP_BATCH_GROUP and
-// PHIP_BATCH_SET select it, and PHIP_BATCH_PAIRED=1 the variant for prime
-// sets below kPairedPrimeLimit that pairs consecutive states.
+// PHIP_BATCH_SET select it, PHIP_BATCH_PAIRED=1 the variant for prime
+// sets below kPairedPrimeLimit that pairs consecutive states, and
+// PHIP_BATCH_SKIP_ZEROS=1 the variant for tables whose states are mostly
+// zero terms (ZeroSkipSink), whose item sums cover every lane.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -29,6 +31,9 @@
 #ifndef PHIP_BATCH_PAIRED
 #define PHIP_BATCH_PAIRED 0
 #endif
+#ifndef PHIP_BATCH_SKIP_ZEROS
+#define PHIP_BATCH_SKIP_ZEROS 0
+#endif
 
 namespace phip {
 
@@ -36,17 +41,24 @@ namespace {
 
 // One wavefront's chunk of job `jb` for every prime of its set; returns the
 // lane values (signed Montgomery residues) in r.
-template <int N, int G, int K, bool P>
+template <int N, int G, int K, bool P, bool Z>
 __device__ __forceinline__ void run_item(const int32_t* __restrict__ tables, const BatchJob& jb, uint64_t base,
-                                         const int32_t (&p)[K], const uint32_t (&pinv)[K], int32_t (&r)[K]) {
-    if constexpr (K == 1) {
+                                         bool active, int32_t* queue, const int32_t (&p)[K],
+                                         const uint32_t (&pinv)[K], int32_t (&r)[K]) {
+    if constexpr (Z) {
+        glynn_chunk_set_skip<N, G, K>(tables, N, jb.cbits, base, active, queue, p, pinv, r);
+    } else if constexpr (K == 1) {
         r[0] = glynn_chunk<N, G>(tables, N, jb.cbits, base, p[0], pinv[0]);
     } else {
         glynn_chunk_set<N, G, K, P>(tables, N, jb.cbits, base, p, pinv, r);
     }
 }
 
-template <int G, int K, bool P>
+// Largest group count over the engine's orders, for the zero-skip queue.
+template <int G>
+constexpr int kMaxGroups = G > 0 ? (kBatchMaxOrder + G - 1) / G : 1;
+
+template <int G, int K, bool P, bool Z>
 __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __restrict__ jobs,
                                                          const uint2* __restrict__ items, uint32_t n_items,
                                                          uint32_t* __restrict__ counter,
@@ -56,6 +68,9 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
                                                          uint64_t* __restrict__ item_partials) {
     const int lane_id = threadIdx.x & 63;
     const uint32_t claim = lane_id == 0 ? 1u : 0u;
+    // One zero-skip queue per wavefront (4 per workgroup).
+    __shared__ int32_t queues[Z ? 4 * kMaxGroups<G> * kZeroQueue : 1];
+    int32_t* const queue = queues + (Z ? (threadIdx.x >> 6) * kMaxGroups<G> * kZeroQueue : 0);
     for (;;) {
         // Every lane executes the same atomic (lane 0 adds 1, the others 0)
         // so no lane-dependent branch sits in front of the broadcast: with a
@@ -83,7 +98,7 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
         switch (jb.n) {
 #define PHIP_BATCH_CASE(N) \
     case N:                \
-        run_item<N, G, K, P>(tables, jb, base, p, pinv, r); \
+        run_item<N, G, K, P, Z>(tables, jb, base, active, queue, p, pinv, r); \
         break;
             PHIP_BATCH_CASE(5) PHIP_BATCH_CASE(6) PHIP_BATCH_CASE(7) PHIP_BATCH_CASE(8)
             PHIP_BATCH_CASE(9) PHIP_BATCH_CASE(10) PHIP_BATCH_CASE(11) PHIP_BATCH_CASE(12)
@@ -103,7 +118,9 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             // Signed lane values summed in two's complement: |sum| < 2^37.
-            const uint64_t s = wave_reduce_sum_u64(active ? (uint64_t)(int64_t)r[k] : 0u);
+            // Zero-skip lanes hold terms of any lane's chunk (inactive
+            // lanes queue nothing but may take queued terms).
+            const uint64_t s = wave_reduce_sum_u64((Z || active) ? (uint64_t)(int64_t)r[k] : 0u);
             if (lane_id == 0) item_partials[(size_t)idx * kMaxSet + k] = s;
         }
     }
@@ -113,8 +130,12 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
 
 #define PHIP_CAT5(a, b, c, d, e) a##b##c##d##e
 #define PHIP_LAUNCH_NAME(G, K, P) PHIP_CAT5(launch_batch_g, G, _k, K, P)
-#if PHIP_BATCH_PAIRED
+#if PHIP_BATCH_PAIRED && PHIP_BATCH_SKIP_ZEROS
+#error "a class either pairs states or skips zero terms"
+#elif PHIP_BATCH_PAIRED
 #define PHIP_PAIRED_SUFFIX _paired
+#elif PHIP_BATCH_SKIP_ZEROS
+#define PHIP_PAIRED_SUFFIX _skip
 #else
 #define PHIP_PAIRED_SUFFIX
 #endif
@@ -124,7 +145,9 @@ hipError_t PHIP_LAUNCH_NAME(PHIP_BATCH_GROUP, PHIP_BATCH_SET, PHIP_PAIRED_SUFFIX
                                                               uint32_t* counter, const int32_t* blob,
                                                               const int32_t* set_p, const uint32_t* set_pinv,
                                                               uint64_t* item_partials) {
-    hipLaunchKernelGGL((batch_glynn_kernel<PHIP_BATCH_GROUP, PHIP_BATCH_SET, PHIP_BATCH_PAIRED != 0>), grid, dim3(256), 0, stream, jobs,
+    hipLaunchKernelGGL((batch_glynn_kernel<PHIP_BATCH_GROUP, PHIP_BATCH_SET, PHIP_BATCH_PAIRED != 0,
+                                           PHIP_BATCH_SKIP_ZEROS != 0>),
+                       grid, dim3(256), 0, stream, jobs,
                        items, n_items, counter, blob, set_p, set_pinv, item_partials);
     return hipGetLastError();
 }

@@ -49,11 +49,13 @@ __global__ __launch_bounds__(256) void batch_finalize_kernel(const BatchJob* __r
     if (threadIdx.x == 0) residues[blockIdx.x] = (int64_t)(((s % p) * (uint64_t)sl.scale) % p);
 }
 
-BatchLaunchFn launcher_for(int group, int set, int paired) {
+BatchLaunchFn launcher_for(int group, int set, int variant) {
 #define PHIP_LAUNCHER(G, K) \
-    if (group == G && set == K && !paired) return launch_batch_g##G##_k##K;
+    if (group == G && set == K && variant == kVariantPlain) return launch_batch_g##G##_k##K;
 #define PHIP_LAUNCHER_PAIRED(G, K) \
-    if (group == G && set == K && paired) return launch_batch_g##G##_k##K##_paired;
+    if (group == G && set == K && variant == kVariantPaired) return launch_batch_g##G##_k##K##_paired;
+#define PHIP_LAUNCHER_SKIP(G, K) \
+    if (group == G && set == K && variant == kVariantSkipZeros) return launch_batch_g##G##_k##K##_skip;
     PHIP_LAUNCHER(0, 1)
     PHIP_LAUNCHER(2, 1) PHIP_LAUNCHER(2, 2) PHIP_LAUNCHER(2, 4) PHIP_LAUNCHER(2, 8)
     PHIP_LAUNCHER(3, 1) PHIP_LAUNCHER(3, 2) PHIP_LAUNCHER(3, 4) PHIP_LAUNCHER(3, 8)
@@ -61,8 +63,11 @@ BatchLaunchFn launcher_for(int group, int set, int paired) {
     PHIP_LAUNCHER(6, 1) PHIP_LAUNCHER(6, 2) PHIP_LAUNCHER(6, 4) PHIP_LAUNCHER(6, 8)
     PHIP_LAUNCHER_PAIRED(2, 4) PHIP_LAUNCHER_PAIRED(2, 8) PHIP_LAUNCHER_PAIRED(3, 4) PHIP_LAUNCHER_PAIRED(3, 8)
     PHIP_LAUNCHER_PAIRED(4, 4) PHIP_LAUNCHER_PAIRED(4, 8) PHIP_LAUNCHER_PAIRED(6, 4) PHIP_LAUNCHER_PAIRED(6, 8)
+    PHIP_LAUNCHER_SKIP(4, 2) PHIP_LAUNCHER_SKIP(4, 4) PHIP_LAUNCHER_SKIP(4, 8)
+    PHIP_LAUNCHER_SKIP(6, 2) PHIP_LAUNCHER_SKIP(6, 4) PHIP_LAUNCHER_SKIP(6, 8)
 #undef PHIP_LAUNCHER
 #undef PHIP_LAUNCHER_PAIRED
+#undef PHIP_LAUNCHER_SKIP
     return nullptr;
 }
 
@@ -99,6 +104,30 @@ std::vector<int> split_set(int r) {
 
 int groups_of(int n, int group) { return group == 0 ? n : (n + group - 1) / group; }
 
+// Estimated share of the Gray states whose term is nonzero, for a 0/1
+// matrix: the product over rows of 1 - C(d, d/2) / 2^d, the share of sign
+// vectors that do not zero a row of even degree d (with or without the
+// fixed column in the row; a row of odd degree never sums to zero).  Rows
+// are treated as independent.  1 for any other matrix.
+double nonzero_share(const int64_t* a, int n) {
+    double share = 1.0;
+    for (int i = 0; i < n; ++i) {
+        int d = 0;
+        for (int j = 0; j < n; ++j) {
+            const int64_t v = a[(size_t)i * n + j];
+            if (v != 0 && v != 1) return 1.0;
+            d += (int)v;
+        }
+        if (d % 2 == 0) {
+            double z = 1.0;
+            for (int t = 1; t <= d / 2; ++t) z *= (2.0 * t - 1.0) / (2.0 * t);
+            share *= 1.0 - z;
+        }
+    }
+    return share;
+}
+
+
 template <typename T>
 struct DevBuf {
     T* ptr = nullptr;
@@ -131,6 +160,7 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
         int n;
         int group;
         uint64_t data_off;
+        bool skip_zeros;
         std::vector<size_t> inputs;
     };
     std::vector<Set> sets;
@@ -163,7 +193,12 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
             }
         }
         if (found == sets.size()) {
-            sets.push_back({x.key, x.n, group, (uint64_t)out.blob.size(), {}});
+            // Tables of 0/1 matrices in groups of 4 or 6 with a small share
+            // of nonzero terms run on the zero-skipping kernels.
+            const int permille = tuning().skip_zeros_permille;
+            const bool skip = permille > 0 && (group == 4 || group == 6) &&
+                              nonzero_share(x.a, x.n) * 1000.0 <= (double)permille;
+            sets.push_back({x.key, x.n, group, (uint64_t)out.blob.size(), skip, {}});
             sets_of_key[x.key].push_back(found);
             out.blob.insert(out.blob.end(), table.begin(), table.end());
         }
@@ -174,7 +209,13 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
     out.slots.resize(in.size());
     for (const Set& st : sets) {
         size_t next = 0;
-        for (int K : st.group == 0 ? std::vector<int>(st.inputs.size(), 1) : split_set((int)st.inputs.size())) {
+        std::vector<int> set_sizes =
+            st.group == 0 ? std::vector<int>(st.inputs.size(), 1) : split_set((int)st.inputs.size());
+        // The zero-skipping kernels take sets of 2, 4 or 8 (one prime is
+        // padded to two).
+        if (st.skip_zeros)
+            for (int& K : set_sizes) K = std::max(K, 2);
+        for (int K : set_sizes) {
             const int take = (int)std::min<size_t>((size_t)K, st.inputs.size() - next);
             BatchJob jb{};
             jb.data_off = st.data_off;
@@ -206,7 +247,13 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
                 }
             }
             next += take;
-            jb.paired = (st.group > 0 && pairs_states(K, (const uint32_t*)out.set_p.data() + jb.prime_off)) ? 1 : 0;
+            if (st.skip_zeros) {
+                jb.variant = kVariantSkipZeros;
+            } else if (st.group > 0 && pairs_states(K, (const uint32_t*)out.set_p.data() + jb.prime_off)) {
+                jb.variant = kVariantPaired;
+            } else {
+                jb.variant = kVariantPlain;
+            }
             out.jobs.push_back(jb);
         }
     }
@@ -233,16 +280,19 @@ int assign_items(PlanHost& out, int items_log2, int min_cbits) {
         jb.lanes = 1ull << (L - c);
     }
 
-    // Items: per (group, set) class, jobs by descending cost, each job's
-    // items contiguous.
-    static const int kClasses[][3] = {{0, 1, 0}, {2, 8, 1}, {2, 8, 0}, {2, 4, 1}, {2, 4, 0}, {2, 2, 0}, {2, 1, 0},
+    // Items: per (group, set, variant) class, jobs by descending cost, each
+    // job's items contiguous.
+    static const int kClasses[][3] = {{0, 1, 0},
+                                      {2, 8, 1}, {2, 8, 0}, {2, 4, 1}, {2, 4, 0}, {2, 2, 0}, {2, 1, 0},
                                       {3, 8, 1}, {3, 8, 0}, {3, 4, 1}, {3, 4, 0}, {3, 2, 0}, {3, 1, 0},
-                                      {4, 8, 1}, {4, 8, 0}, {4, 4, 1}, {4, 4, 0}, {4, 2, 0}, {4, 1, 0},
-                                      {6, 8, 1}, {6, 8, 0}, {6, 4, 1}, {6, 4, 0}, {6, 2, 0}, {6, 1, 0}};
+                                      {4, 8, 1}, {4, 8, 2}, {4, 8, 0}, {4, 4, 1}, {4, 4, 2}, {4, 4, 0},
+                                      {4, 2, 2}, {4, 2, 0}, {4, 1, 0},
+                                      {6, 8, 1}, {6, 8, 2}, {6, 8, 0}, {6, 4, 1}, {6, 4, 2}, {6, 4, 0},
+                                      {6, 2, 2}, {6, 2, 0}, {6, 1, 0}};
     for (const auto& gk : kClasses) {
         std::vector<uint32_t> order;
         for (uint32_t j = 0; j < out.jobs.size(); ++j)
-            if (out.jobs[j].group == gk[0] && out.jobs[j].set == gk[1] && out.jobs[j].paired == gk[2])
+            if (out.jobs[j].group == gk[0] && out.jobs[j].set == gk[1] && out.jobs[j].variant == gk[2])
                 order.push_back(j);
         if (order.empty()) continue;
         auto cost = [&](uint32_t j) {
@@ -277,7 +327,7 @@ int enqueue_plan(const PlanHost& h, const PlanDevice& d, hipStream_t stream, uns
     if (ev0) PHIP_HIP_TRY(hipEventRecord(ev0, stream));
     for (size_t c = 0; c < h.classes.size(); ++c) {
         const BatchClass& cl = h.classes[c];
-        const BatchLaunchFn fn = launcher_for(cl.group, cl.set, cl.paired);
+        const BatchLaunchFn fn = launcher_for(cl.group, cl.set, cl.variant);
         if (!fn) {
             set_error("internal: no kernel for a batch class");
             return -1;
@@ -656,7 +706,7 @@ int phip_batch_stats(void* handle, double* out, int len) {
                         (double)cbits_max,
                         (double)h.items_log2,
                         (double)plan->blocks_per_cu,
-                        h.classes.empty() ? -1.0 : (double)h.classes[0].paired};
+                        h.classes.empty() ? -1.0 : (double)h.classes[0].variant};
     const int m = std::min<int>(len, (int)(sizeof(v) / sizeof(v[0])));
     for (int i = 0; i < m; ++i) out[i] = v[i];
     return m;

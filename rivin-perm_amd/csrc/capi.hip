@@ -137,6 +137,10 @@ int phip_set_tuning(int key, int64_t value) {
             if (value < 8 || value > 32 || value % 4 != 0) break;
             t.affine_min_tier = (int)value;
             return 0;
+        case PHIP_TUNE_SKIP_ZEROS_PERMILLE:
+            if (value < 0 || value > 1000) break;
+            t.skip_zeros_permille = (int)value;
+            return 0;
         default:
             phip::set_error("unknown tuning key");
             return -1;
@@ -156,6 +160,7 @@ int64_t phip_get_tuning(int key) {
         case PHIP_TUNE_NECKLACE_DEPTH: return t.necklace_depth;
         case PHIP_TUNE_WIDE_MIN_TIER: return t.wide_min_tier;
         case PHIP_TUNE_AFFINE_MIN_TIER: return t.affine_min_tier;
+        case PHIP_TUNE_SKIP_ZEROS_PERMILLE: return t.skip_zeros_permille;
         default:
             phip::set_error("unknown tuning key");
             return -1;

@@ -389,6 +389,101 @@ struct PrimeSetSink {
     }
 };
 
+// K primes sharing one packed table whose states mostly have a zero row sum
+// (0/1 matrices: a row of even degree d sums to zero in C(d, d/2) / 2^d of
+// the states, and one zero row zeroes the term).  A term is zero exactly
+// when one of the state's exact group products is zero, so only the other
+// states need the per-prime trees.  Each wavefront queues those states in
+// LDS (their group products, with the state's sign folded into the first
+// one) and, once 64 are queued, every lane takes one and runs the trees of
+// PrimeSetSink's per-state form: the trees run on nonzero terms only, at
+// full lane occupancy.  A lane adds the terms of whichever states it takes,
+// so the sign is absolute (the chunk's first-state parity is known before
+// the walk) and the lane's sum is not negated at the end.  The queue holds
+// kZeroQueue entries per group, column-major: between states it holds
+// fewer than 64, and one state adds at most 64.
+constexpr int kZeroQueue = 128;
+
+template <int NR, int G, int K>
+struct ZeroSkipSink {
+    static_assert(G > 0, "prime sets need exact packed row sums");
+    using Rows = PackedRows<NR, G>;
+    static constexpr int kGroups = Rows::kGroups;
+    static constexpr int kPairs = kGroups / 2;
+    static constexpr int kLeaves = kPairs + (kGroups & 1);
+    int32_t negp[K];
+    uint32_t pinv[K];
+    int64_t acc[K];
+    int32_t* queue;   // this wavefront's kGroups * kZeroQueue words of LDS
+    uint32_t count;   // queued states (wave-uniform)
+    uint32_t lane;
+    bool neg;         // the chunk's first state has sign -1
+    bool active;      // the lane's chunk belongs to the job
+
+    // Adds the term of one state given by its signed group products.
+    __device__ __forceinline__ void consume(const int32_t (&g)[kGroups]) {
+        int64_t P[kPairs > 0 ? kPairs : 1];
+#pragma unroll
+        for (int j = 0; j < kPairs; ++j) P[j] = (int64_t)g[2 * j] * (int64_t)g[2 * j + 1];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            int32_t v[kLeaves];
+#pragma unroll
+            for (int j = 0; j < kPairs; ++j) v[j] = redc31(P[j], negp[k], pinv[k]);
+            if constexpr (kGroups & 1) v[kLeaves - 1] = g[kGroups - 1];
+#pragma unroll
+            for (int w = 1; w < kLeaves; w *= 2) {
+#pragma unroll
+                for (int i = 0; i + w < kLeaves; i += 2 * w) v[i] = mont31(v[i], v[i + w], negp[k], pinv[k]);
+            }
+            acc[k] = mac_sign<1>(acc[k], v[0]);
+        }
+    }
+
+    template <bool ODD>
+    __device__ __forceinline__ void state(const Rows& rs) {
+        int32_t g[kGroups];
+        rs.groups(g, std::make_integer_sequence<int, kGroups>{});
+        bool nz = active;
+#pragma unroll
+        for (int i = 0; i < kGroups; ++i) nz &= g[i] != 0;
+        const uint64_t mask = __ballot(nz);
+        if (mask == 0) return;
+        if (nz) {
+            const uint32_t pos =
+                count + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+            queue[pos] = (ODD != neg) ? -g[0] : g[0];
+#pragma unroll
+            for (int i = 1; i < kGroups; ++i) queue[i * kZeroQueue + pos] = g[i];
+        }
+        count += (uint32_t)__popcll(mask);
+        if (count >= 64) {
+            count -= 64;
+            // Other lanes' queue writes above are read below; LDS keeps a
+            // wavefront's accesses in order, and this keeps the compiler
+            // from moving the reads up.
+            __builtin_amdgcn_wave_barrier();
+            int32_t h[kGroups];
+#pragma unroll
+            for (int i = 0; i < kGroups; ++i) h[i] = queue[i * kZeroQueue + count + lane];
+            consume(h);
+        }
+    }
+
+    // Takes the states still queued (fewer than 64), one per lane.
+    __device__ __forceinline__ void flush() {
+        if (count == 0) return;
+        __builtin_amdgcn_wave_barrier();
+        if (lane < count) {
+            int32_t h[kGroups];
+#pragma unroll
+            for (int i = 0; i < kGroups; ++i) h[i] = queue[i * kZeroQueue + lane];
+            consume(h);
+        }
+        count = 0;
+    }
+};
+
 // Walks one lane's chunk: the 2^cbits Gray states starting at `base` (a
 // multiple of 2^cbits), handing the row sums of each state to the sink.
 // `tables` holds the row sums of state 0 (all column signs +1: exact centred
@@ -498,6 +593,36 @@ __device__ __forceinline__ void glynn_chunk_set(const int32_t* __restrict__ tabl
     const bool neg = gray_walk<NR, G>(tables, n, cbits, base, 0, sink);
 #pragma unroll
     for (int k = 0; k < K; ++k) out[k] = lane_value(sink.acc[k], neg, sink.negp[k], sink.pinv[k]);
+}
+
+// glynn_chunk_set through ZeroSkipSink.  Every lane of the wavefront must
+// call it (the queue is shared); lanes with active == false walk but queue
+// nothing.  A lane's value is its sum of the absolutely signed terms it
+// took, which may come from any lane's chunk, so only the wavefront's total
+// is meaningful: it equals the total of glynn_chunk_set's active lanes.
+// Each lane takes at most 2^cbits terms (one per full batch of 64 and one
+// at the flush), so |acc| < 2^(31+cbits) as there.
+template <int NR, int G, int K>
+__device__ __forceinline__ void glynn_chunk_set_skip(const int32_t* __restrict__ tables, int n, int cbits,
+                                                     uint64_t base, bool active, int32_t* queue,
+                                                     const int32_t (&primes)[K], const uint32_t (&pinvs)[K],
+                                                     int32_t (&out)[K]) {
+    ZeroSkipSink<NR, G, K> sink;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        sink.negp[k] = -primes[k];
+        sink.pinv[k] = pinvs[k];
+        sink.acc[k] = 0;
+    }
+    sink.queue = queue;
+    sink.count = 0;
+    sink.lane = __lane_id();
+    sink.neg = __popcll(base ^ (base >> 1)) & 1;
+    sink.active = active;
+    gray_walk<NR, G>(tables, n, cbits, base, 0, sink);
+    sink.flush();
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = lane_value(sink.acc[k], false, sink.negp[k], sink.pinv[k]);
 }
 
 }  // namespace phip

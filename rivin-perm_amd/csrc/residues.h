@@ -11,7 +11,9 @@ namespace phip {
 // Tunables (set through phip_set_tuning): target number of lanes per
 // single-matrix launch; minimum chunk bits; batch item count and workgroups
 // per CU; the smallest prime Schur order and the task depth of the necklace
-// kernel; the smallest row-register tier of the wide Glynn kernels.
+// kernel; the smallest row-register tiers of the wide and affine Glynn
+// kernels; the largest estimated share of nonzero terms (permille) at which
+// a 0/1 table runs on the zero-skipping kernels (0 turns them off).
 struct Tuning {
     int lanes_log2 = 20;
     int min_cbits = 4;
@@ -21,6 +23,7 @@ struct Tuning {
     int necklace_depth = 28;      // largest FKM prefix depth of the necklace tasks
     int wide_min_tier = 8;        // smallest register tier (8..64, multiple of 4) of the wide Glynn kernels
     int affine_min_tier = 8;      // smallest register tier (8..32, multiple of 4) of the affine Glynn kernels
+    int skip_zeros_permille = 250;
 };
 Tuning& tuning();
 

@@ -94,6 +94,7 @@ TUNE_NECKLACE_MIN_ORDER = 5
 TUNE_NECKLACE_DEPTH = 6
 TUNE_WIDE_MIN_TIER = 7
 TUNE_AFFINE_MIN_TIER = 8
+TUNE_SKIP_ZEROS_PERMILLE = 9
 BATCH_BLOCKS_PER_CU = 1
 BATCH_USE_GRAPH = 2
 BATCH_ITEMS_LOG2 = 3
@@ -431,7 +432,7 @@ class Batch:
 
     def stats(self) -> dict:
         keys = ("last_ms", "states", "mulmods", "jobs", "items", "launches", "waves", "group", "residues", "set",
-                "chunk_bits_min", "chunk_bits_max", "items_log2", "blocks_per_cu", "paired")
+                "chunk_bits_min", "chunk_bits_max", "items_log2", "blocks_per_cu", "variant")
         vals = (ctypes.c_double * len(keys))()
         count = self._lib.phip_batch_stats(self._handle, vals, len(keys))
         _check(count, "phip_batch_stats")

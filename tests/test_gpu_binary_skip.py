@@ -1,0 +1,147 @@
+"""Zero-skipping Glynn kernels (0/1 tables whose Gray states are mostly
+zero terms): the planner picks them, and their residues equal the plain
+kernels' and the oracle's, bit for bit, including chunks with inactive
+lanes, padded single primes, zero rows and mixed plans."""
+
+import math
+import random
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from permhip import binary_prep as _prep
+
+# The binary planner's primes (descending from 2^31 - 1) and the integer
+# planner's (ascending from 2^30, which pair consecutive states).
+BINARY_PRIMES = [_prep.descending_prime(i) for i in range(8)]
+LOW_PRIMES = [q for q in range((1 << 30) + 1, (1 << 30) + 2000, 2) if _prep.is_prime_u64(q)][:4]
+
+
+def _with_skip(native, permille, fn):
+    old = native.get_tuning(native.TUNE_SKIP_ZEROS_PERMILLE)
+    native.set_tuning(native.TUNE_SKIP_ZEROS_PERMILLE, permille)
+    try:
+        return fn()
+    finally:
+        native.set_tuning(native.TUNE_SKIP_ZEROS_PERMILLE, old)
+
+
+def _even_degree_matrix(rng, n, density):
+    """A 0/1 matrix whose rows mostly have even degree (many zero terms)."""
+    A = (rng.random((n, n)) < density).astype(np.int64)
+    for i in range(n):
+        if A[i].sum() % 2:
+            j = int(rng.integers(n))
+            A[i, j] ^= 1
+    return A
+
+
+def _variant(native, A, primes):
+    plan = native.Batch([A], [primes])
+    try:
+        return plan.stats()["variant"]
+    finally:
+        plan.close()
+
+
+@pytest.mark.parametrize("n", [5, 6, 8, 11, 14, 17, 20, 23])
+def test_skip_matches_plain_and_oracle_small(native, oracle_c, n):
+    """Small orders (jobs of fewer than 64 lanes leave lanes inactive)."""
+    rng = np.random.default_rng(100 + n)
+    for density in (0.35, 0.6):
+        A = _even_degree_matrix(rng, n, density)
+        for primes in (BINARY_PRIMES[:2], BINARY_PRIMES[:4], BINARY_PRIMES[:8], BINARY_PRIMES[:1]):
+            print(f"skip n={n} density={density} k={len(primes)}", flush=True)
+            skip = native.permanent_residues(A, primes)
+            plain = _with_skip(native, 0, lambda: native.permanent_residues(A, primes))
+            assert skip == plain, (n, density, primes)
+            for p, r in zip(primes, skip):
+                assert r == oracle_c.binary_mod_p(A.astype(np.uint8), p, 16), (n, p)
+
+
+@pytest.mark.parametrize("n", [28, 32])
+def test_skip_matches_plain_large(native, n):
+    rng = np.random.default_rng(7 + n)
+    A = _even_degree_matrix(rng, n, 0.5)
+    assert _variant(native, A, BINARY_PRIMES[:4]) == 2.0
+    print(f"skip large n={n}", flush=True)
+    skip = native.permanent_residues(A, BINARY_PRIMES[:4])
+    plain = _with_skip(native, 0, lambda: native.permanent_residues(A, BINARY_PRIMES[:4]))
+    assert skip == plain
+
+
+def test_planner_picks_the_variant(native):
+    rng = np.random.default_rng(11)
+    A = _even_degree_matrix(rng, 20, 0.5)
+    assert _variant(native, A, BINARY_PRIMES[:4]) == 2.0
+    assert _with_skip(native, 0, lambda: _variant(native, A, BINARY_PRIMES[:4])) == 0.0
+    # Odd degrees everywhere: no row ever sums to zero, so no skipping.
+    odd = np.zeros((20, 20), dtype=np.int64)
+    for i in range(20):
+        for j in range(i, i + 5):
+            odd[i, j % 20] = 1
+    assert _variant(native, odd, BINARY_PRIMES[:4]) == 0.0
+    # Integer tables never skip.
+    B = rng.integers(-3, 4, (20, 20)).astype(np.int64)
+    assert _variant(native, B, BINARY_PRIMES[:4]) != 2.0
+
+
+def test_skip_known_counts(native):
+    """Closed forms through the zero-skipping kernels: all-ones J_n
+    (n!, rows of even degree for even n), J_n - I (derangements), a zero
+    row, and block-diagonal ones."""
+    import perm_binary
+
+    for n in (10, 12, 14):
+        J = np.ones((n, n), dtype=np.int64)
+        assert native.permanent_residues(J, BINARY_PRIMES[:2]) == [math.factorial(n) % p for p in BINARY_PRIMES[:2]]
+    der = [1, 0]
+    for n in range(2, 16):
+        der.append((n - 1) * (der[-1] + der[-2]))
+    for n in (11, 13, 15):  # J - I has rows of degree n - 1, even for odd n
+        D = np.ones((n, n), dtype=np.int64) - np.eye(n, dtype=np.int64)
+        assert _variant(native, D, BINARY_PRIMES[:2]) == 2.0
+        assert native.permanent_residues(D, BINARY_PRIMES[:2]) == [der[n] % p for p in BINARY_PRIMES[:2]]
+    Z = _even_degree_matrix(np.random.default_rng(5), 12, 0.5)
+    Z[3] = 0
+    assert native.permanent_residues(Z, BINARY_PRIMES[:4]) == [0, 0, 0, 0]
+    B = [[1 if i // 4 == j // 4 else 0 for j in range(24)] for i in range(24)]
+    assert perm_binary.binary_permanent(B) == math.factorial(4) ** 6
+
+
+def test_skip_batch_mixes_variants(native, oracle_c):
+    """One plan with zero-skipping, plain and paired jobs: every residue
+    equals the oracle's."""
+    rng = np.random.default_rng(21)
+    mats, mods = [], []
+    for n in (12, 15, 18):
+        mats.append(_even_degree_matrix(rng, n, 0.5))
+        mods.append(BINARY_PRIMES[:4])
+        mats.append(rng.integers(-5, 6, (n, n)).astype(np.int64))
+        mods.append(BINARY_PRIMES[:4])
+        mats.append(rng.integers(-5, 6, (n, n)).astype(np.int64))
+        mods.append(LOW_PRIMES)
+    plan = native.Batch(mats, mods)
+    try:
+        flat = plan.run()
+    finally:
+        plan.close()
+    at = 0
+    for A, ps in zip(mats, mods):
+        for p in ps:
+            assert flat[at] == oracle_c.permanent_mod_p(A.tolist(), p, 16), (A.shape[0], p)
+            at += 1
+
+
+def test_grid_and_binary36_through_skip(native):
+    """The 40x40 periodic grid (exact count) and the binary36 golden
+    residues run on the zero-skipping kernels."""
+    import perm_binary
+    from permhip import grids
+
+    G = np.array(grids.grid_biadjacency((4, 4, 5), periodic_axes=(0, 1)), dtype=np.int64)
+    assert _variant(native, G, [2147483647, 2147483629]) == 2.0
+    print("skip grid 40", flush=True)
+    assert perm_binary.binary_permanent(G.tolist()) == 2551276235535120

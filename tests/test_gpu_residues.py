@@ -275,7 +275,7 @@ def test_paired_classes_are_planned(native):
     low = native.Batch([A], [_primes_from(1 << 30, 8)])
     high = native.Batch([A], [_primes_below((1 << 31) - 1, 8)])
     try:
-        assert low.stats()["paired"] == 1.0 and high.stats()["paired"] == 0.0
+        assert low.stats()["variant"] == 1.0 and high.stats()["variant"] == 0.0
     finally:
         low.close()
         high.close()
