@@ -392,6 +392,7 @@ def main() -> int:
     ap.add_argument("--no-published", action="store_true",
                     help="skip the workloads the reference publishes timings for")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="mean Poisson arrival rate (req/s) of the served leg")
     ap.add_argument("--kernel-only", action="store_true", help="headline launches only (for profilers)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="run one plain device copy of BYTES (HBM counter calibration) and exit")
@@ -627,7 +628,7 @@ def main() -> int:
         except ImportError:
             serve = None
         if serve is not None:
-            result["serve"] = serve.benchmark(seconds=args.serve_seconds)
+            result["serve"] = serve.benchmark(seconds=args.serve_seconds, rate=args.serve_rate, progress=log)
             log(f"serve: {result['serve']}")
 
     if not args.no_published:

@@ -284,10 +284,12 @@ def latency_floor(count: int = 40, seed: int = 6) -> dict:
     return out
 
 
-def benchmark(seconds: float = 2.0, rate: float = 200.0, seed: int = 5, verify: int = 24) -> dict:
+def benchmark(seconds: float = 2.0, rate: float = 200.0, seed: int = 5, verify: int = 24, progress=None) -> dict:
     """Replay the seeded Poisson stream against a live server; report
     latency percentiles, throughput and backlog, and check a sample of
-    answers against the single-request path."""
+    answers against the single-request path.  ``progress(text)``, when
+    given, is called every 1000 submitted and every 1000 answered
+    requests."""
     from . import workloads
 
     stream = workloads.serve_stream(rate=rate, seconds=seconds, seed=seed)
@@ -299,8 +301,14 @@ def benchmark(seconds: float = 2.0, rate: float = 200.0, seed: int = 5, verify: 
         if delay > 0:
             time.sleep(delay)
         futures.append((kind, matrix, server.submit(matrix, kind)))
+        if progress and len(futures) % 1000 == 0:
+            progress(f"serve: {len(futures)} requests submitted at {time.perf_counter() - t0:.1f} s")
     submit_end = time.perf_counter()
-    results = [f.result() for _, _, f in futures]
+    results = []
+    for _, _, f in futures:
+        results.append(f.result())
+        if progress and len(results) % 1000 == 0:
+            progress(f"serve: {len(results)} answered at {time.perf_counter() - t0:.1f} s")
     t_end = time.perf_counter()
     server.close()
     lat = sorted(r.finished - r.arrival for r in server.completed)
