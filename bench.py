@@ -408,6 +408,12 @@ def main() -> int:
     ap.add_argument("--wide-residue", type=int, default=0, metavar="BITS",
                     help="perm(A) of the seeded 30x30 wide-moduli matrix modulo 2^61 - 1 (BITS=61) or "
                          "2^32 - 5 (BITS=32), once, and exit (one wide Glynn launch, for profilers)")
+    ap.add_argument("--binary36", type=int, default=0, metavar="REPS",
+                    help="only the binary36 workload's 4 residues, REPS times (for profilers); "
+                         "--skip-zeros-permille sets the zero-skipping threshold")
+    ap.add_argument("--skip-zeros-permille", type=int, default=None)
+    ap.add_argument("--frontier", type=int, default=0, metavar="REPS",
+                    help="only the frontier DP on the open 12x16 grid's biadjacency (order 96), REPS times")
     ap.add_argument("--schur-residue", type=int, default=0, metavar="N",
                     help="compute perm(S_N) modulo the first prime = 1 (mod N) above 2^30 and exit "
                          "(one necklace or Glynn launch, for profilers)")
@@ -444,6 +450,28 @@ def main() -> int:
         t0 = time.perf_counter()
         r = _native.permanent_residues(A, [q])[0]
         log(f"perm(A) mod {q} = {r} in {time.perf_counter() - t0:.3f} s")
+        return 0
+    if args.binary36:
+        from permhip import binary_prep, workloads
+
+        if args.skip_zeros_permille is not None:
+            _native.set_tuning(_native.TUNE_SKIP_ZEROS_PERMILLE, args.skip_zeros_permille)
+        B = np.array(workloads.binary36(), dtype=np.int64)
+        primes = [binary_prep.descending_prime(i) for i in range(4)]
+        for rep in range(args.binary36):
+            t0 = time.perf_counter()
+            r = _native.permanent_residues(B, primes)
+            log(f"binary36 residues {r} in {(time.perf_counter() - t0) * 1e3:.1f} ms (rep {rep})")
+        return 0
+    if args.frontier:
+        import perm_binary
+        from permhip import grids
+
+        G = grids.grid_biadjacency((12, 16))
+        for rep in range(args.frontier):
+            t0 = time.perf_counter()
+            v = perm_binary.binary_permanent_frontier(G)
+            log(f"frontier 12x16 grid: {v} in {(time.perf_counter() - t0) * 1e3:.1f} ms (rep {rep})")
         return 0
     if args.schur_residue:
         n = args.schur_residue
@@ -517,7 +545,8 @@ def main() -> int:
         1.0 / (1.0 / rates["mont31_product"] + 1.0 / rates["v_add_u32"]) / 1e12)
     roof["instruction_rates"] = rates
     roof["kernel"] = (
-        f"batch_glynn_kernel<{group}, {kset}, {str(paired).lower()}> (n={HEADLINE_N}, {len(primes)} primes in "
+        f"batch_glynn_kernel<{group}, {kset}, {str(paired).lower()}, "
+        f"{str(int(st.get('variant', 0)) == 2).lower()}> (n={HEADLINE_N}, {len(primes)} primes in "
         f"{int(st['jobs'])} walk(s) of the Gray code)"
     )
     roof["kernel_ms"] = k_ms

@@ -272,3 +272,16 @@ def test_perm_binary_cuda_counts(native):
         d = oracle.subset_permanent(B)
         assert perm_binary_cuda.binary_permanent_cuda(B) == d
         assert perm_binary_cuda.binary_permanent_mod_p_cuda(B, 2147483629) == d % 2147483629
+
+
+def test_mod3_demo_functions(native, capsys):
+    """perm_mod3's demo entry points (perm_mod3.py:386-457) run the mod-3
+    kernel against the general residue kernel modulo 3 and report agreement."""
+    import perm_mod3
+
+    perm_mod3.test_random_matrices()
+    out = capsys.readouterr().out
+    assert out.count("match=True") == 4 and "match=False" not in out
+    perm_mod3.benchmark_mod3(n=12, num_trials=3)
+    out = capsys.readouterr().out
+    assert "Correctness verified." in out and "MISMATCH" not in out
