@@ -21,6 +21,16 @@
 //                           sources' residues per prime (runs are at most
 //                           the row degree long).
 //
+// Small layers take a faster path: frontier_small_kernel, one workgroup of
+// 1024 threads, runs consecutive layers with the current layer, a hash
+// table of successor keys and the successor sums all in LDS, and returns
+// when a layer would not fit (more than kSmallStates states or kSmallPairs
+// successor pairs, or more than kSmallPrimes moduli), when a layer passes
+// the cap, or at the end.  The host then continues with the kernels above
+// for one layer, and hands back to the small kernel when the layers fit
+// again.  A layer costs a few barriers instead of eight launches and two
+// host synchronisations.
+//
 // The state count of every layer is checked against the caller's cap
 // before the layer's residues are formed.
 #include <hip/hip_runtime.h>
@@ -99,6 +109,161 @@ __global__ void frontier_sum_kernel(const uint64_t* __restrict__ keys, const uin
 
 inline unsigned blocks_for(uint64_t n) { return (unsigned)((n + kFrontierBlock - 1) / kFrontierBlock); }
 
+constexpr int kSmallThreads = 1024;
+constexpr int kSmallStates = 1024;  // states of a layer held in LDS
+constexpr int kSmallPairs = 4096;   // successor pairs of one layer
+constexpr int kSmallHash = 8192;    // hash slots (load at most one half)
+constexpr int kSmallPrimes = 4;
+constexpr uint64_t kEmptyKey = ~UINT64_C(0);  // no state has every slot bit set with 64 slots in use
+constexpr uint32_t kSmallDone = 0;   // every row processed
+constexpr uint32_t kSmallCap = 1;    // the layer of row `pos` passes the cap
+constexpr uint32_t kSmallLarge = 2;  // the layer of row `pos` does not fit; current layer returned
+constexpr uint32_t kSmallEmpty = 3;  // no successor: the count is zero
+
+// Exclusive prefix sum over the workgroup; *total receives the sum.
+__device__ __forceinline__ uint32_t small_block_scan(uint32_t v, uint32_t* wave_sums, uint32_t* total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wave_sums[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < kSmallThreads / 64; ++w) {
+        const uint32_t t = wave_sums[w];
+        if (w < wave) before += t;
+        all += t;
+    }
+    *total = all;
+    __syncthreads();
+    return before + x - v;
+}
+
+// Layers pos0, pos0 + 1, ... of the DP in one workgroup.  state[0] holds
+// the current layer's size on entry (at most kSmallStates, with its keys and
+// residues in keys / vals, k per state); on exit state[0..2] = (size of the
+// returned layer, first unprocessed row, status) and keys / vals hold that
+// layer.  k <= kSmallPrimes.
+__global__ __launch_bounds__(kSmallThreads) void frontier_small_kernel(
+    const uint64_t* __restrict__ rows, const uint64_t* __restrict__ expiring, int n, int pos0, int k,
+    const uint32_t* __restrict__ primes, uint64_t max_states, uint64_t* __restrict__ keys,
+    uint32_t* __restrict__ vals, uint32_t* __restrict__ state) {
+    __shared__ uint64_t sk[kSmallStates];
+    __shared__ uint32_t sv[kSmallStates * kSmallPrimes];
+    __shared__ unsigned long long hk[kSmallHash];
+    __shared__ unsigned long long acc[kSmallStates * kSmallPrimes];
+    __shared__ uint16_t psrc[kSmallPairs];
+    __shared__ uint16_t pslot[kSmallPairs];
+    __shared__ uint32_t wave_sums[kSmallThreads / 64];
+    __shared__ uint32_t npairs, overflow;
+    const uint32_t tid = threadIdx.x;
+    uint32_t count = state[0];
+    for (uint32_t i = tid; i < count; i += kSmallThreads) {
+        sk[i] = keys[i];
+        for (int q = 0; q < k; ++q) sv[i * kSmallPrimes + q] = vals[(size_t)i * k + q];
+    }
+    uint32_t p[kSmallPrimes];
+#pragma unroll
+    for (int q = 0; q < kSmallPrimes; ++q) p[q] = q < k ? primes[q] : 1u;
+    __syncthreads();
+    int pos = pos0;
+    uint32_t status = kSmallDone;
+    for (; pos < n; ++pos) {
+        const uint64_t row = rows[pos], exp = expiring[pos];
+        if ((uint64_t)count * (uint64_t)__popcll(row) > (uint64_t)kSmallPairs) {
+            status = kSmallLarge;
+            break;
+        }
+        for (int s = tid; s < kSmallHash; s += kSmallThreads) hk[s] = kEmptyKey;
+        for (int s = tid; s < kSmallStates * kSmallPrimes; s += kSmallThreads) acc[s] = 0;
+        if (tid == 0) npairs = overflow = 0;
+        __syncthreads();
+        // Successor pairs: insert each successor key (linear probing) and
+        // record (source, slot).
+        for (uint32_t i = tid; i < count; i += kSmallThreads) {
+            const uint64_t used = sk[i];
+            for (uint64_t m = row & ~used; m; m &= m - 1) {
+                const uint64_t target = used | (m & (0 - m));
+                if ((target & exp) != exp) continue;
+                const unsigned long long key = target & ~exp;
+                uint32_t h = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 51);  // 13 bits
+                int probe = 0;
+                for (; probe < kSmallHash; ++probe) {
+                    const unsigned long long old = atomicCAS(&hk[h], kEmptyKey, key);
+                    if (old == kEmptyKey || old == key) break;
+                    h = (h + 1) & (kSmallHash - 1);
+                }
+                if (probe == kSmallHash) overflow = 1;  // not reachable: at most kSmallPairs keys
+                const uint32_t at = atomicAdd(&npairs, 1u);
+                psrc[at] = (uint16_t)i;
+                pslot[at] = (uint16_t)h;
+            }
+        }
+        __syncthreads();
+        // Dense numbering of the occupied slots (8 consecutive per thread).
+        constexpr int kPer = kSmallHash / kSmallThreads;
+        uint32_t local = 0;
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) local += hk[tid * kPer + j] != kEmptyKey;
+        uint32_t distinct = 0;
+        const uint32_t base = small_block_scan(local, wave_sums, &distinct);
+        if (distinct > max_states) {
+            status = kSmallCap;
+            break;
+        }
+        if (distinct == 0) {
+            status = kSmallEmpty;
+            count = 0;
+            break;
+        }
+        if (distinct > (uint32_t)kSmallStates || overflow) {
+            status = kSmallLarge;
+            break;
+        }
+        // The current keys are no longer read: the successors take their
+        // places, and each occupied slot now holds its dense index.
+        uint32_t d = base;
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int s = tid * kPer + j;
+            const unsigned long long key = hk[s];
+            if (key != kEmptyKey) {
+                sk[d] = key;
+                hk[s] = d;
+                ++d;
+            }
+        }
+        __syncthreads();
+        const uint32_t np = npairs;
+        for (uint32_t a = tid; a < np; a += kSmallThreads) {
+            const uint32_t dst = (uint32_t)hk[pslot[a]];
+            const uint32_t src = psrc[a];
+            for (int q = 0; q < k; ++q)
+                atomicAdd(&acc[dst * kSmallPrimes + q], (unsigned long long)sv[src * kSmallPrimes + q]);
+        }
+        __syncthreads();
+        for (uint32_t e = tid; e < distinct; e += kSmallThreads) {
+            for (int q = 0; q < k; ++q)
+                sv[e * kSmallPrimes + q] = (uint32_t)(acc[e * kSmallPrimes + q] % p[q]);
+        }
+        count = distinct;
+        __syncthreads();
+    }
+    for (uint32_t i = tid; i < count; i += kSmallThreads) {
+        keys[i] = sk[i];
+        for (int q = 0; q < k; ++q) vals[(size_t)i * k + q] = sv[i * kSmallPrimes + q];
+    }
+    if (tid == 0) {
+        state[0] = count;
+        state[1] = (uint32_t)pos;
+        state[2] = status;
+    }
+}
+
 }  // namespace
 
 }  // namespace phip
@@ -141,27 +306,70 @@ extern "C" int phip_binary_frontier_residues(const uint64_t* row_masks, const ui
         hp[q] = (uint32_t)primes[q];
         one[q] = 1;
     }
-    // Device slots 40..49: state keys and residues (40, 41 and 48, 49 in
-    // turn), counts/heads (42), offsets/slots (43), pairs and sorted pairs
-    // (44, 45), primes (46), hipcub scratch (47).
+    // Device slots 40..52: state keys and residues (40, 41 and 48, 49 in
+    // turn), counts/heads (42), offsets/output slots (43), pairs and sorted
+    // pairs (44, 45), primes (46), hipcub scratch (47), rows and expiring
+    // masks (50, 51), small-kernel state (52).
     uint64_t states = 1;
     auto grow = [&](int slot, size_t bytes) { return ctx->buffer(slot, std::max<size_t>(bytes, 16)); };
     auto* d_primes = (uint32_t*)grow(46, k * 4);
     auto* h_small = (uint32_t*)ctx->host_buffer(8, 64);
-    if (!d_primes || !h_small) return -1;
+    auto* d_rows = (uint64_t*)grow(50, (size_t)n * 8);
+    auto* d_exp = (uint64_t*)grow(51, (size_t)n * 8);
+    auto* d_state = (uint32_t*)grow(52, 16);
+    if (!d_primes || !h_small || !d_rows || !d_exp || !d_state) return -1;
     PHIP_HIP_TRY(hipMemcpyAsync(d_primes, hp.data(), k * 4, hipMemcpyHostToDevice, st));
-    uint64_t* keys = (uint64_t*)grow(40, 8);
-    uint32_t* vals = (uint32_t*)grow(41, (size_t)k * 4);
+    PHIP_HIP_TRY(hipMemcpyAsync(d_rows, row_masks, (size_t)n * 8, hipMemcpyHostToDevice, st));
+    PHIP_HIP_TRY(hipMemcpyAsync(d_exp, expiring, (size_t)n * 8, hipMemcpyHostToDevice, st));
+    // Both layer slots hold at least one small-kernel layer, so the small
+    // kernel, which works in place, never needs its slot grown.
+    const size_t small_keys = (size_t)phip::kSmallStates * 8, small_vals = (size_t)phip::kSmallStates * k * 4;
+    if (!grow(48, small_keys) || !grow(49, small_vals)) return -1;
+    int cur = 40;  // slot of the current layer's keys; its residues are in cur + 1
+    uint64_t* keys = (uint64_t*)grow(40, small_keys);
+    uint32_t* vals = (uint32_t*)grow(41, small_vals);
     if (!keys || !vals) return -1;
     const uint64_t zero = 0;
     PHIP_HIP_TRY(hipMemcpyAsync(keys, &zero, 8, hipMemcpyHostToDevice, st));
     PHIP_HIP_TRY(hipMemcpyAsync(vals, one.data(), (size_t)k * 4, hipMemcpyHostToDevice, st));
 
-    for (int pos = 0; pos < n; ++pos) {
+    // The small kernel needs k <= kSmallPrimes and fewer than 64 slots (an
+    // all-ones key marks an empty hash slot).
+    const bool small_ok = k <= phip::kSmallPrimes && slots < 64;
+    bool force_large = false;
+    int pos = 0;
+    while (pos < n) {
         const uint64_t row = row_masks[pos], exp = expiring[pos];
+        const uint64_t degree = (uint64_t)__builtin_popcountll(row);
+        if (small_ok && !force_large && states <= (uint64_t)phip::kSmallStates &&
+            states * degree <= (uint64_t)phip::kSmallPairs) {
+            h_small[0] = (uint32_t)states;
+            PHIP_HIP_TRY(hipMemcpyAsync(d_state, h_small, 4, hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(phip::frontier_small_kernel, dim3(1), dim3(phip::kSmallThreads), 0, st, d_rows, d_exp,
+                               n, pos, k, d_primes, (uint64_t)max_states, keys, vals, d_state);
+            PHIP_HIP_TRY(hipGetLastError());
+            PHIP_HIP_TRY(hipMemcpyAsync(h_small, d_state, 12, hipMemcpyDeviceToHost, st));
+            PHIP_HIP_TRY(hipStreamSynchronize(st));
+            const uint32_t reached = h_small[1], status = h_small[2];
+            states = h_small[0];
+            if (status == phip::kSmallCap) {
+                info[0] = (int64_t)reached + 1;
+                info[1] = max_states + 1;
+                return 1;
+            }
+            if (status == phip::kSmallEmpty) {
+                for (int q = 0; q < k; ++q) out[q] = 0;
+                return 0;
+            }
+            // A layer that does not fit goes through the kernels below once.
+            force_large = status == phip::kSmallLarge;
+            pos = (int)reached;
+            continue;
+        }
+        force_large = false;
         // Successor pairs are at most states x row degree; counts and
         // offsets are 32-bit and hipcub takes int sizes.
-        if (states * (uint64_t)__builtin_popcountll(row) >= 0x7fffffffull) {
+        if (states * degree >= 0x7fffffffull) {
             phip::set_error("frontier layer has too many successor pairs");
             return -1;
         }
@@ -199,19 +407,20 @@ extern "C" int phip_binary_frontier_residues(const uint64_t* row_masks, const ui
         if (!tmp) return -1;
         PHIP_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, pk, pk_sorted, ps, ps_sorted, (int)pairs, 0,
                                                        std::max(slots, 1), st));
-        // Heads and slots reuse the count/offset buffers, grown to the pairs.
+        // Heads and output slots reuse the count/offset buffers, grown to
+        // the pairs.
         auto* heads = (uint32_t*)grow(42, pairs * 4);
-        auto* slots = (uint32_t*)grow(43, pairs * 4);
-        if (!heads || !slots) return -1;
+        auto* out_slot = (uint32_t*)grow(43, pairs * 4);
+        if (!heads || !out_slot) return -1;
         hipLaunchKernelGGL(phip::frontier_head_kernel, dim3(phip::blocks_for(pairs)), dim3(phip::kFrontierBlock), 0, st,
                            pk_sorted, (uint32_t)pairs, heads);
         PHIP_HIP_TRY(hipGetLastError());
         tmp_bytes = 0;
-        PHIP_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, heads, slots, (int)pairs, st));
+        PHIP_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, heads, out_slot, (int)pairs, st));
         tmp = grow(47, tmp_bytes);
         if (!tmp) return -1;
-        PHIP_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, heads, slots, (int)pairs, st));
-        PHIP_HIP_TRY(hipMemcpyAsync(h_small, slots + pairs - 1, 4, hipMemcpyDeviceToHost, st));
+        PHIP_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, heads, out_slot, (int)pairs, st));
+        PHIP_HIP_TRY(hipMemcpyAsync(h_small, out_slot + pairs - 1, 4, hipMemcpyDeviceToHost, st));
         PHIP_HIP_TRY(hipMemcpyAsync(h_small + 1, heads + pairs - 1, 4, hipMemcpyDeviceToHost, st));
         PHIP_HIP_TRY(hipStreamSynchronize(st));
         const uint64_t next = (uint64_t)h_small[0] + h_small[1];
@@ -220,17 +429,21 @@ extern "C" int phip_binary_frontier_residues(const uint64_t* row_masks, const ui
             info[1] = max_states + 1;
             return 1;
         }
-        // The next layer's states alternate between slots (40, 41) and
-        // (48, 49), so the current layer stays readable while it is summed.
-        auto* new_keys = (uint64_t*)ctx->buffer((pos & 1) ? 40 : 48, std::max<size_t>(next * 8, 16));
-        auto* new_vals = (uint32_t*)ctx->buffer((pos & 1) ? 41 : 49, std::max<size_t>(next * k * 4, 16));
+        // The next layer goes to the other pair of slots, so the current
+        // layer stays readable while it is summed.
+        const int other = cur == 40 ? 48 : 40;
+        auto* new_keys = (uint64_t*)ctx->buffer(other, std::max<size_t>(next * 8, small_keys));
+        auto* new_vals = (uint32_t*)ctx->buffer(other + 1, std::max<size_t>(next * k * 4, small_vals));
         if (!new_keys || !new_vals) return -1;
         hipLaunchKernelGGL(phip::frontier_sum_kernel, dim3(phip::blocks_for(pairs)), dim3(phip::kFrontierBlock), 0, st,
-                           pk_sorted, ps_sorted, (uint32_t)pairs, heads, slots, vals, k, d_primes, new_keys, new_vals);
+                           pk_sorted, ps_sorted, (uint32_t)pairs, heads, out_slot, vals, k, d_primes, new_keys,
+                           new_vals);
         PHIP_HIP_TRY(hipGetLastError());
+        cur = other;
         keys = new_keys;
         vals = new_vals;
         states = next;
+        ++pos;
     }
     // Every column has expired after the last row: the only state is 0.
     std::vector<uint32_t> res((size_t)states * k);

@@ -270,3 +270,23 @@ def test_grid_domino_tilings_through_the_frontier(native, shape):
     print(f"frontier grid {shape} -> {expected}", flush=True)
     assert perm_binary.binary_permanent(matrix) == expected
     assert perm_binary.binary_permanent_frontier(matrix) == expected
+
+
+def test_frontier_switches_between_small_and_large_layers(native):
+    """All-ones blocks of order 14 reach C(14, 7) = 3432 states (beyond the
+    single-workgroup kernel's 1024) in the middle of each block and one
+    state between blocks, so the DP moves between the two paths several
+    times; the count is 14!^2 * 3!."""
+    import perm_binary
+
+    n = 14 + 3 + 14
+    M = [[0] * n for _ in range(n)]
+    for lo, size in ((0, 14), (14, 3), (17, 14)):
+        for r in range(lo, lo + size):
+            for c in range(lo, lo + size):
+                M[r][c] = 1
+    assert perm_binary.binary_permanent_frontier(M, row_order=range(n), max_states=10_000) == (
+        math.factorial(14) ** 2 * 6)
+    with pytest.raises(perm_binary.FrontierStateLimitExceeded) as caught:
+        perm_binary.binary_permanent_frontier(M, row_order=range(n), max_states=3431)
+    assert caught.value.processed_rows == 7 and caught.value.observed_states == 3432
