@@ -65,3 +65,26 @@ def test_cached_prime_sequences_match_scans():
             down.append(c)
         c -= 2
     assert [binary_prep.descending_prime(i) for i in range(40)] == down
+
+
+def test_normalize_ndarray_fast_path_matches_the_entry_loop():
+    """Integer and boolean arrays are validated vectorised; the rows and the
+    first offending entry's message equal those of the entry-by-entry loop
+    (perm_binary.py:173-215) on the same data as nested lists."""
+    import numpy as np
+
+    from permhip import binary_prep as prep
+
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        n = int(rng.integers(0, 12))
+        A = rng.integers(-2, 3, (n, n))
+        for strict in (True, False):
+            for arr in (A, A.astype(np.int8), A != 0, np.abs(A).astype(np.uint8)):
+                outcomes = []
+                for form in (arr.tolist(), arr):
+                    try:
+                        outcomes.append(("ok", prep.normalize_binary_matrix(form, strict=strict)))
+                    except (TypeError, ValueError) as exc:
+                        outcomes.append((type(exc), str(exc)))
+                assert outcomes[0] == outcomes[1]
