@@ -11,6 +11,10 @@ their own streams, so requests that arrive while others are on the GPU start
 as soon as the device drains the earlier plans' work queues.  All residues of
 a request are produced inside one plan and stay on the device until that
 plan finishes; the host then reconstructs each request's integer by CRT.
+On an idle server (nothing waiting or in flight) a request below
+``direct_cost`` runs at once through the single-matrix device path instead,
+which skips creating a plan.  The scheduler sleeps until a request arrives
+when idle, and polls in-flight plans every 0.2 ms otherwise.
 
 Latency (completion minus arrival) and throughput are recorded per request.
 """
@@ -117,9 +121,10 @@ class PermanentServer:
     """Background scheduler; ``submit`` returns a Future of the exact integer."""
 
     def __init__(self, *, max_inflight: int = 3, plan_cost: float = 4e11,
-                 plan_memory: int = 1 << 30, blocks_per_cu: int = 3):
+                 plan_memory: int = 1 << 30, blocks_per_cu: int = 3, direct_cost: float = 2.0 ** 30):
         _native.require_device()
         self.max_inflight = max_inflight
+        self.direct_cost = direct_cost
         self.plan_cost = plan_cost
         self.plan_memory = plan_memory
         self.blocks_per_cu = blocks_per_cu
@@ -128,6 +133,7 @@ class PermanentServer:
         self._inflight = []  # (Batch, [(request, part)])
         self._next_id = 0
         self._stop = threading.Event()
+        self._wake = threading.Event()  # set by submit and close
         self._lock = threading.Lock()
         self.completed: List[_Request] = []
         self.plans_launched = 0
@@ -150,10 +156,12 @@ class PermanentServer:
             return fut
         req.parts, req.constant = parts, constant
         self._incoming.put(req)
+        self._wake.set()
         return fut
 
     def close(self, wait: bool = True) -> None:
         self._stop.set()
+        self._wake.set()
         if wait:
             self._thread.join()
 
@@ -186,6 +194,13 @@ class PermanentServer:
             except queue.Empty:
                 break
             self._run_outside_plans(req)
+            if (req.done_parts < len(req.parts) and not self._inflight and not self._waiting
+                    and req.cost <= self.direct_cost):
+                # Idle device and a small request: no plan to build.
+                for part in req.parts:
+                    if part.residues is None:
+                        part.residues = _native.permanent_residues(part.matrix, part.primes)
+                        req.done_parts += 1
             if req.done_parts == len(req.parts):
                 self._complete(req)
             else:
@@ -242,13 +257,18 @@ class PermanentServer:
 
     def _loop(self) -> None:
         while True:
+            # Cleared before the queue is drained: a request that arrives
+            # after the drain sets the event again, so no wake-up is lost.
+            self._wake.clear()
             self._admit()
             progressed = self._poll()
             idle = not self._inflight and not self._waiting and self._incoming.empty()
             if self._stop.is_set() and idle:
                 return
-            if not progressed:
-                time.sleep(0.0002)
+            if idle:
+                self._wake.wait()
+            elif not progressed:
+                self._wake.wait(0.0002)
 
 
 def latency_floor(count: int = 40, seed: int = 6) -> dict:

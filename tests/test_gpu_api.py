@@ -203,7 +203,10 @@ def test_adapter_objects_as_reference_seams(native):
     assert perm_mod3_omp.permanent_mod3(np.array([[1, 1], [1, 1]])) == 2
 
 
-def test_served_results_equal_alone(native):
+@pytest.mark.parametrize("direct_cost", [0.0, 2.0 ** 30])
+def test_served_results_equal_alone(native, direct_cost):
+    """Every served answer is exact, with plans only (direct_cost 0) and with
+    small requests on an idle server taking the single-matrix path."""
     from permhip.serve import PermanentServer
 
     rng = np.random.default_rng(55)
@@ -215,10 +218,13 @@ def test_served_results_equal_alone(native):
         else:
             n = int(rng.integers(3, 18))
             reqs.append(("binary", (rng.random((n, n)) < 0.5).astype(np.int64)))
-    server = PermanentServer(max_inflight=2, plan_cost=1e6)
+    server = PermanentServer(max_inflight=2, plan_cost=1e6, direct_cost=direct_cost)
     futures = [server.submit(m, k) for k, m in reqs]
     mixed = [f.result(timeout=60) for f in futures]
+    # One at a time on the idle server: direct when small enough.
+    alone = [server.submit(m, k).result(timeout=60) for k, m in reqs[:6]]
     server.close()
+    assert alone == mixed[:6]
     for (kind, m), v in zip(reqs, mixed):
         assert v == oracle.subset_permanent(m.tolist()), kind
 
