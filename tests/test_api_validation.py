@@ -112,3 +112,59 @@ def test_binary_host_preprocessing_blocks_and_bounds():
     assert not binary_prep.is_prime_u64(341_550_071_728_321)
     assert binary_prep.ceil_nth_root(10 ** 12, 3) == 10 ** 4
     assert binary_prep.ceil_nth_root(10 ** 12 + 1, 3) == 10 ** 4 + 1
+
+
+INT64_MIN = -(1 << 63)
+
+
+@pytest.mark.parametrize(
+    "matrix,modulus,threads,error,message",
+    [
+        (([1],), 101, 1, TypeError, "list of lists"),
+        (1, 101, 1, TypeError, "list of lists"),
+        ([(1,)], 101, 1, ValueError, "square"),
+        ([[1, 2]], 101, 1, ValueError, "square"),
+        ([[1], [2, 3]], 101, 1, ValueError, "square"),
+        ([[0] * 64 for _ in range(64)], 101, 1, ValueError, "less than 64"),
+        ([[1]], 0, 1, ValueError, "positive"),
+        ([[1]], -3, 1, ValueError, "positive"),
+        ([[1]], 1 << 63, 1, OverflowError, None),
+        ([[1]], 101.0, 1, TypeError, None),
+        ([[1.0]], 101, 1, TypeError, None),
+        ([[1 << 63]], 101, 1, OverflowError, None),
+        ([[INT64_MIN - 1]], 101, 1, OverflowError, None),
+        ([[1]], 101, 1 << 31, OverflowError, None),
+        ([[1]], 101, 1.0, TypeError, None),
+    ],
+)
+def test_perm_c_ext_object_rejects_like_the_reference(matrix, modulus, threads, error, message):
+    """The perm_c_ext-shaped object validates as the reference's extension
+    (tests/test_generic_native.py:186-211), before any device call."""
+    from permhip.backends import perm_c_ext
+
+    for implementation in (perm_c_ext.permanent_mod_p, perm_c_ext.ryser_permanent_mod_p):
+        with pytest.raises(error, match=message):
+            implementation(matrix, modulus, threads)
+
+
+@pytest.mark.parametrize(
+    "matrix,prime,threads,error,message",
+    [
+        ([[1, 0]], 101, 0, ValueError, "square"),
+        ([[1, 2], [0, 1]], 101, 0, ValueError, "0 or 1"),
+        ([[1, 0], [0, 1]], 9, 0, ValueError, "prime"),
+        ([[1, 0, 0], [0, 1, 0], [0, 0, 1]], 3, 0, ValueError, "greater than"),
+        ([[1]], 2, 0, ValueError, "odd"),
+        ([[1, 0], [0, 1]], 101, -1, ValueError, "num_threads"),
+        ([[1]], True, 0, TypeError, "p must be an integer"),
+        ([[1]], 101, True, TypeError, "num_threads must be an integer"),
+        ([[0] * 64 for _ in range(64)], 101, 0, ValueError, "at most 63"),
+    ],
+)
+def test_perm_binary_ext_object_rejects_like_the_reference(matrix, prime, threads, error, message):
+    """The _binary_ext-shaped object validates as the reference's extension
+    (tests/test_binary_native.py:95-122), before any device call."""
+    from permhip.backends import perm_binary_ext
+
+    with pytest.raises(error, match=message):
+        perm_binary_ext.binary_permanent_mod_p(matrix, prime, threads)
