@@ -291,3 +291,38 @@ def test_oracle_frontier_counts_domino_tilings(shape):
     plan = prep.auto_frontier_plan(masks)
     assert plan is not None
     assert oracle.frontier_count(plan[0], plan[1]) == _kasteleyn(*shape)
+
+
+@pytest.mark.parametrize("row_order", [[0, 1], [0, 1, 1], [0, 1, 3], [0, -1, 2], [0, True, 2],
+                                       [0, np.bool_(True), 2], [0, 1.0, 2], 7])
+def test_frontier_rejects_the_reference_row_orders(row_order):
+    """The row orders of tests/test_binary_frontier.py:104-122."""
+    with pytest.raises((TypeError, ValueError), match="row_order"):
+        perm_binary.binary_permanent_frontier([[1, 0, 0], [0, 1, 0], [0, 0, 1]], row_order=row_order)
+
+
+def test_frontier_hostile_matrix_semantics():
+    """Rows and matrices whose iteration disagrees with their length
+    (tests/test_binary_frontier.py:144-172)."""
+
+    class ShortRow:
+        def __len__(self):
+            return 2
+
+        def __iter__(self):
+            return iter([1])
+
+    with pytest.raises(ValueError, match="row width changed"):
+        perm_binary.binary_permanent_frontier([ShortRow(), ShortRow()])
+
+    class ShortMatrix:
+        def __len__(self):
+            return 2
+
+        def __iter__(self):
+            return iter([[1, 0]])
+
+    with pytest.raises(ValueError, match="row count changed"):
+        perm_binary.binary_permanent_frontier(ShortMatrix())
+    with pytest.raises(ValueError, match="0 or 1"):
+        perm_binary.binary_permanent_frontier([[-7, 0], [4, 9]])

@@ -290,3 +290,12 @@ def test_frontier_switches_between_small_and_large_layers(native):
     with pytest.raises(perm_binary.FrontierStateLimitExceeded) as caught:
         perm_binary.binary_permanent_frontier(M, row_order=range(n), max_states=3431)
     assert caught.value.processed_rows == 7 and caught.value.observed_states == 3432
+
+
+def test_frontier_integral_row_order_and_support_mode(native):
+    """tests/test_binary_frontier.py:125-150 on the device."""
+    import numpy as np
+    import perm_binary
+
+    assert perm_binary.binary_permanent_frontier([[1, 0], [0, 1]], row_order=np.array([np.int64(1), np.int32(0)])) == 1
+    assert perm_binary.binary_permanent_frontier([[-7, 0], [4, 9]], strict=False) == 1
