@@ -21,6 +21,13 @@ BATCH_MAX_ORDER = 46
 
 
 def _int64_rows(matrix):
+    """(rows as lists of Python ints, int64 array or None when an entry
+    does not fit int64)."""
+    if (isinstance(matrix, np.ndarray) and matrix.ndim == 2 and matrix.shape[0] == matrix.shape[1]
+            and matrix.dtype.kind in "iu" and matrix.dtype.itemsize <= 8
+            and not (matrix.dtype == np.uint64 and matrix.size and int(matrix.max()) > (1 << 63) - 1)):
+        arr = np.ascontiguousarray(matrix, dtype=np.int64)
+        return arr.tolist(), arr
     rows = [[int(x) for x in row] for row in matrix]
     n = len(rows)
     if any(len(r) != n for r in rows):
@@ -54,7 +61,7 @@ class PermanentBatch:
         for i, m in enumerate(matrices):
             rows, arr = _int64_rows(m)
             n = len(rows)
-            bound = planner.matrix_permanent_bound(rows) if n else 1
+            bound = planner.matrix_permanent_bound(arr if arr is not None else rows) if n else 1
             primes, product = planner.generic_crt_primes(bound) if n > 1 else ([], 1)
             self.rows.append(rows)
             self.bounds.append(bound)
