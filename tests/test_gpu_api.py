@@ -291,3 +291,43 @@ def test_mod3_demo_functions(native, capsys):
     perm_mod3.benchmark_mod3(n=12, num_trials=3)
     out = capsys.readouterr().out
     assert "Correctness verified." in out and "MISMATCH" not in out
+
+
+def test_full_size_invariants_of_the_headline_matrix(native):
+    """At the headline size (32 x 32, 8 primes, 2^31 Gray states each):
+    perm(A) = perm(A^T) = perm(P A Q) for permutation matrices P, Q; the
+    permanent is linear in each row (row 5 split as u + v) and scales with
+    a row (x3).  (Its residues are pinned to the reference's golden values
+    in test_golden_reference_residues.)"""
+    import perm
+    from permhip import workloads
+
+    A = workloads.headline()
+    base = perm.matrix_permanent(A)
+    assert perm.matrix_permanent(A.T.copy()) == base
+    rng = np.random.default_rng(1)
+    P, Q = rng.permutation(32), rng.permutation(32)
+    assert perm.matrix_permanent(A[P][:, Q].copy()) == base
+    u = rng.integers(-10, 11, 32)
+    U, V = A.copy(), A.copy()
+    U[5], V[5] = u, A[5] - u
+    assert perm.matrix_permanent(U) + perm.matrix_permanent(V) == base
+    S = A.copy()
+    S[7] *= 3
+    assert perm.matrix_permanent(S) == 3 * base
+
+
+def test_full_size_invariants_of_binary36(native):
+    """binary36 (36 x 36, 4 primes, 2^35 Gray states each): the count is
+    invariant under transposition and row/column permutations, and its
+    residue mod 3 equals the mod-3 kernel's."""
+    import perm_binary
+    import perm_mod3
+    from permhip import workloads
+
+    B = np.array(workloads.binary36(), dtype=np.int64)
+    base = perm_binary.binary_permanent(B)
+    assert perm_binary.binary_permanent(B.T.copy()) == base
+    rng = np.random.default_rng(2)
+    assert perm_binary.binary_permanent(B[rng.permutation(36)][:, rng.permutation(36)].copy()) == base
+    assert perm_mod3.permanent_mod3_fast(B) == base % 3
