@@ -39,12 +39,23 @@ __global__ __launch_bounds__(256) void batch_finalize_kernel(const BatchJob* __r
     const BatchSlot sl = slots[blockIdx.x];
     const BatchJob jb = jobs[sl.job];
     const uint64_t p = sl.p;
-    // Item sums are signed (two's complement) sums of lane residues.
+    // Item sums are signed (two's complement) sums of 64 lane residues,
+    // |sum| < 2^37, so 2^16 of them add exactly in int64 (< 2^53); each such
+    // run is reduced once instead of every item.
     uint64_t s = 0;
+    int64_t run = 0;
+    uint32_t in_run = 0;
     for (uint32_t i = threadIdx.x; i < jb.n_items; i += 256) {
-        const int64_t v = (int64_t)item_partials[(size_t)(jb.first_item + i) * kMaxSet + sl.k] % (int64_t)p;
-        s += (uint64_t)(v < 0 ? v + (int64_t)p : v);
+        run += (int64_t)item_partials[(size_t)(jb.first_item + i) * kMaxSet + sl.k];
+        if (++in_run == 65536u) {
+            const int64_t v = run % (int64_t)p;
+            s += (uint64_t)(v < 0 ? v + (int64_t)p : v);
+            run = 0;
+            in_run = 0;
+        }
     }
+    const int64_t v = run % (int64_t)p;
+    s += (uint64_t)(v < 0 ? v + (int64_t)p : v);
     s = block_reduce_sum_u64<256>(s);
     if (threadIdx.x == 0) residues[blockIdx.x] = (int64_t)(((s % p) * (uint64_t)sl.scale) % p);
 }
