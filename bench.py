@@ -176,6 +176,83 @@ def valu_int32_utilisation(kernel_ms: float, compute_units: int, rates: dict) ->
     }
 
 
+def binary36_leg() -> dict:
+    """The binary configuration: the exact count of binary36 end to end
+    (matching, bound, its descending 31-bit primes in one zero-skipping
+    launch, CRT) and the mod-3 kernel on the same matrix, whose residue
+    must equal the count mod 3."""
+    import perm_binary
+    import perm_mod3
+    from permhip import workloads
+
+    B = np.array(workloads.binary36(), dtype=np.int64)
+    t0 = time.perf_counter()
+    count = perm_binary.binary_permanent(B)
+    dt = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    m3 = perm_mod3.permanent_mod3_fast(B)
+    dt3 = time.perf_counter() - t0
+    if m3 != count % 3:
+        raise RuntimeError("binary36 count and mod-3 residue disagree")
+    return {
+        "workload": "binary36: 36x36 Bernoulli(0.5), default_rng(3), resampled until no empty row or column",
+        "s_exact": dt,
+        "value": str(count),
+        "s_mod3": dt3,
+        "mod3": m3,
+    }
+
+
+def binary36_cpu_baseline(threads: int) -> dict:
+    """The reference's perm_binary_ext.binary_permanent_mod_p and
+    perm_mod3_omp.permanent_mod3 (compiled from their sources into
+    oracle/_ref; the oracle's restatements when those are not built) on
+    leading blocks of binary36 -- 30x30 for one binary prime, 32x32 mod 3 --
+    with the device's time on the same blocks beside them."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle  # checker/baseline only
+    import perm_mod3
+    from permhip import _native, workloads
+
+    exts = oracle.reference_extensions()
+    B = np.array(workloads.binary36(), dtype=np.int64)
+    p = 2_147_483_647
+    b30 = B[:30, :30].copy()
+    b32 = B[:32, :32].copy()
+    out = {}
+    if "perm_binary_ext" in exts:
+        kind, fn = "reference", (lambda: exts["perm_binary_ext"].binary_permanent_mod_p(b30.tolist(), p, threads))
+    else:
+        kind, fn = "port", (lambda: oracle.C.binary_mod_p(b30.astype(np.uint8), p, threads))
+    t0 = time.perf_counter()
+    cpu_r = fn()
+    cpu_s = time.perf_counter() - t0
+    _native.binary_permanent_residues(b30.astype(np.uint8), [p])
+    t0 = time.perf_counter()
+    gpu_r = _native.binary_permanent_residues(b30.astype(np.uint8), [p])[0]
+    gpu_s = time.perf_counter() - t0
+    if gpu_r != cpu_r:
+        raise RuntimeError("binary residue differs from the CPU baseline's")
+    out["binary_mod_p"] = {"kind": kind, "cores": threads, "sample": f"leading 30x30 block, p = {p}",
+                           "cpu_s": cpu_s, "gpu_s": gpu_s, "speedup": cpu_s / gpu_s}
+    if "perm_mod3_omp" in exts:
+        kind, fn = "reference", (lambda: exts["perm_mod3_omp"].permanent_mod3(b32 % 3, threads))
+    else:
+        kind, fn = "port", (lambda: oracle.C.mod3(b32, threads))
+    t0 = time.perf_counter()
+    cpu_m = fn()
+    cpu_s = time.perf_counter() - t0
+    perm_mod3.permanent_mod3_fast(b32)
+    t0 = time.perf_counter()
+    gpu_m = perm_mod3.permanent_mod3_fast(b32)
+    gpu_s = time.perf_counter() - t0
+    if gpu_m != cpu_m:
+        raise RuntimeError("mod-3 residue differs from the CPU baseline's")
+    out["mod3"] = {"kind": kind, "cores": threads, "sample": "leading 32x32 block",
+                   "cpu_s": cpu_s, "gpu_s": gpu_s, "speedup": cpu_s / gpu_s}
+    return out
+
+
 def cpu_baseline(matrix: np.ndarray, primes, threads: int) -> dict:
     """Reference OpenMP Glynn (perm_c_ext.permanent_mod_p) on the leading
     30 x 30 block of the headline matrix for one prime: 2^29 states."""
@@ -661,6 +738,8 @@ def main() -> int:
             log(f"serve: {result['serve']}")
 
     if not args.no_published:
+        result["binary36"] = binary36_leg()
+        log(f"binary36: {result['binary36']}")
         result["polynomial"] = polynomial_leg()
         log(f"polynomial: {result['polynomial']}")
         result["wide_moduli"] = wide_moduli()
@@ -675,6 +754,9 @@ def main() -> int:
         if "polynomial" in result:
             result["polynomial"]["cpu_baseline"] = polynomial_cpu_baseline(threads)
             log(f"polynomial cpu baseline: {result['polynomial']['cpu_baseline']}")
+        if "binary36" in result:
+            result["binary36"]["cpu_baseline"] = binary36_cpu_baseline(threads)
+            log(f"binary36 cpu baseline: {result['binary36']['cpu_baseline']}")
 
     if rank == 0:
         print(json.dumps(result), flush=True)

@@ -92,6 +92,62 @@ __device__ __forceinline__ void load_col(const uint32_t* __restrict__ cols, int 
     }
 }
 
+// Column words through the constant address space: the flipped column is
+// the same for every lane, so its words come in by scalar loads; the empty
+// asm keeps the compiler from hoisting every column into registers.
+typedef const __attribute__((address_space(4))) uint32_t* ConstWords;
+__device__ __forceinline__ ConstWords const_words(const uint32_t* p) {
+    asm volatile("" : "+s"(p));
+    return (ConstWords)p;
+}
+
+template <int W>
+__device__ __forceinline__ void load_col_uniform(const uint32_t* __restrict__ cols, int j, uint32_t (&m)[W],
+                                                 uint32_t (&s)[W]) {
+    const ConstWords c = const_words(cols + (size_t)j * 2 * W);
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        m[w] = c[w];
+        s[w] = c[W + w];
+    }
+}
+
+template <int W, bool ADD>
+__device__ __forceinline__ void f3_flip(Vec3<W>& a, const uint32_t (&m)[W], const uint32_t (&s)[W]) {
+    if constexpr (ADD) {
+        f3_add<W>(a, m, s);
+    } else {
+        f3_sub<W>(a, m, s);
+    }
+}
+
+// A flip whose direction differs between lanes: both results, then a select.
+template <int W>
+__device__ __forceinline__ void f3_flip_lane(Vec3<W>& a, const uint32_t (&m)[W], const uint32_t (&s)[W], bool add) {
+    Vec3<W> x = a, y = a;
+    f3_add<W>(x, m, s);
+    f3_sub<W>(y, m, s);
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        a.m[w] = add ? x.m[w] : y.m[w];
+        a.s[w] = add ? x.s[w] : y.s[w];
+    }
+}
+
+// f3_product for row sums whose bits beyond the order are zero (sign bits
+// lie inside the magnitude bits, so no mask is needed for the parity).
+template <int W>
+__device__ __forceinline__ int f3_term(const Vec3<W>& a, const uint32_t (&mask)[W]) {
+    bool nz = true;
+    uint32_t bits = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        nz = nz && (a.m[w] == mask[w]);
+        bits += __popc(a.s[w]);
+    }
+    return nz ? 1 - 2 * (int)(bits & 1u) : 0;
+}
+
 // cols[j] = {mag words..., sgn words...}; rs0 = column sum with all signs +.
 template <int W>
 __global__ __launch_bounds__(kMod3Block) void mod3_glynn_kernel(const uint32_t* __restrict__ cols,
@@ -122,31 +178,75 @@ __global__ __launch_bounds__(kMod3Block) void mod3_glynn_kernel(const uint32_t* 
             f3_add<W>(rs, m, s);
         }
     }
-    int acc_e = f3_product<W>(rs, mask);
-    int acc_o = 0;
     const uint64_t T = 1ull << cbits;
-    for (uint64_t t = 1; t < T; ++t) {
-        const int j = __builtin_ctzll(t);
-        const uint64_t g = base + t;
-        const bool now_negative = ((g ^ (g >> 1)) >> j) & 1u;
-        uint32_t m[W], s[W];
-        load_col<W>(cols, j, m, s);
-        if (now_negative) {
-            f3_add<W>(rs, m, s);
-        } else {
-            f3_sub<W>(rs, m, s);
+    // Terms of even local index add, odd ones subtract (the chunk's first
+    // state's sign is applied at the end).  |total| <= 2^cbits may pass
+    // 2^31 at large orders, so each step's int sum is folded into int64.
+    int64_t total = f3_term<W>(rs, mask);
+    if (cbits >= 4) {
+        // Sixteen states per step with the flip directions of the Gray code
+        // fixed at compile time: moving to local index t flips column
+        // j = ctz(t) to the new Gray bit 1 ^ bit j+1 of t (bits below cbits
+        // of base + t are those of t).  Column 0 (every other state) and
+        // columns 1 and 2 stay in registers; column 3 (t = 8 mod 16) and the
+        // column at each multiple of 16 are loaded, with the direction read
+        // from base + t, which may differ between lanes.
+        uint32_t c0m[W], c0s[W], c1m[W], c1s[W], c2m[W], c2s[W];
+        load_col_uniform<W>(cols, 0, c0m, c0s);
+        load_col_uniform<W>(cols, 1, c1m, c1s);
+        load_col_uniform<W>(cols, 2, c2m, c2s);
+        for (uint64_t t0 = 0; t0 < T; t0 += 16) {
+            int acc = 0;
+            f3_flip<W, true>(rs, c0m, c0s);  acc -= f3_term<W>(rs, mask);   // t0+1
+            f3_flip<W, true>(rs, c1m, c1s);  acc += f3_term<W>(rs, mask);   // t0+2
+            f3_flip<W, false>(rs, c0m, c0s); acc -= f3_term<W>(rs, mask);   // t0+3
+            f3_flip<W, true>(rs, c2m, c2s);  acc += f3_term<W>(rs, mask);   // t0+4
+            f3_flip<W, true>(rs, c0m, c0s);  acc -= f3_term<W>(rs, mask);   // t0+5
+            f3_flip<W, false>(rs, c1m, c1s); acc += f3_term<W>(rs, mask);   // t0+6
+            f3_flip<W, false>(rs, c0m, c0s); acc -= f3_term<W>(rs, mask);   // t0+7
+            {
+                uint32_t m[W], s[W];
+                load_col_uniform<W>(cols, 3, m, s);
+                f3_flip_lane<W>(rs, m, s, !(((base + t0 + 8) >> 4) & 1u));
+                acc += f3_term<W>(rs, mask);                                  // t0+8
+            }
+            f3_flip<W, true>(rs, c0m, c0s);  acc -= f3_term<W>(rs, mask);   // t0+9
+            f3_flip<W, true>(rs, c1m, c1s);  acc += f3_term<W>(rs, mask);   // t0+10
+            f3_flip<W, false>(rs, c0m, c0s); acc -= f3_term<W>(rs, mask);   // t0+11
+            f3_flip<W, false>(rs, c2m, c2s); acc += f3_term<W>(rs, mask);   // t0+12
+            f3_flip<W, true>(rs, c0m, c0s);  acc -= f3_term<W>(rs, mask);   // t0+13
+            f3_flip<W, false>(rs, c1m, c1s); acc += f3_term<W>(rs, mask);   // t0+14
+            f3_flip<W, false>(rs, c0m, c0s); acc -= f3_term<W>(rs, mask);   // t0+15
+            const uint64_t tn = t0 + 16;
+            if (tn < T) {
+                const int j = __builtin_ctzll(tn);  // 4 <= j < cbits, the same in every lane
+                uint32_t m[W], s[W];
+                load_col_uniform<W>(cols, j, m, s);
+                f3_flip_lane<W>(rs, m, s, !(((base + tn) >> (j + 1)) & 1u));
+                acc += f3_term<W>(rs, mask);                                  // tn (even)
+            }
+            total += acc;
         }
-        const int v = f3_product<W>(rs, mask);
-        if (t & 1) {
-            acc_o += v;
-        } else {
-            acc_e += v;
+    } else {
+        for (uint64_t t = 1; t < T; ++t) {
+            const int j = __builtin_ctzll(t);
+            const uint64_t g = base + t;
+            const bool now_negative = ((g ^ (g >> 1)) >> j) & 1u;
+            uint32_t m[W], s[W];
+            load_col<W>(cols, j, m, s);
+            if (now_negative) {
+                f3_add<W>(rs, m, s);
+            } else {
+                f3_sub<W>(rs, m, s);
+            }
+            const int v = f3_term<W>(rs, mask);
+            total += (t & 1) ? -v : v;
         }
     }
-    int64_t acc = (int64_t)acc_e - (int64_t)acc_o;
-    if (__popcll(gray) & 1) acc = -acc;
+    int64_t signed_acc = total;
+    if (__popcll(gray) & 1) signed_acc = -signed_acc;
     // Bias into [0, 2^62) so the unsigned block sum is exact, undone on the host.
-    const uint64_t biased = active ? (uint64_t)(acc + (int64_t)T) : 0u;
+    const uint64_t biased = active ? (uint64_t)(signed_acc + (int64_t)T) : 0u;
     const uint64_t s = block_reduce_sum_u64<kMod3Block>(biased);
     if (threadIdx.x == 0) partials[blockIdx.x] = (int64_t)s;
 }

@@ -157,11 +157,16 @@ def test_binary_residues(native, oracle_c, n):
         assert r == oracle_c.binary_mod_p(B, p, 8), (n, p)
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 8, 16, 21, 24, 31, 32, 33])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 16, 21, 24, 31, 32, 33, 34])
 def test_mod3(native, oracle_c, n):
-    rng = np.random.default_rng(3 + n)
-    A = rng.integers(-5, 6, size=(n, n), dtype=np.int64)
-    assert native.permanent_mod3(A % 3) == oracle_c.mod3(A, 8), n
+    """Orders below 5 take the per-state loop, the others the 16-state
+    unrolled one (with one and two 32-bit words per row vector)."""
+    for seed in range(3):
+        rng = np.random.default_rng(100 * seed + 3 + n)
+        A = rng.integers(-5, 6, size=(n, n), dtype=np.int64)
+        if seed == 2:
+            A = (rng.random((n, n)) < 0.5).astype(np.int64)
+        assert native.permanent_mod3(A % 3) == oracle_c.mod3(A, 8), (n, seed)
 
 
 def test_abi_rejections_do_not_poison_state(native):
