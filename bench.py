@@ -65,10 +65,10 @@ def instruction_mix(n: int, group: int, k: int = 1, paired: bool = False) -> dic
     serving a set of k primes (packed rows; the unrolled 16-state step
     amortised, as counted in the compiled loop).
 
-    Shared by the set: row-sum flips (one packed op per two rows, plus the
-    two lane-dependent flips per 16 states that issue both directions and a
-    select), the exact group products, and for k > 1 the exact 64-bit
-    products of adjacent groups.  Per prime: the Montgomery tree (two
+    Shared by the set: row-sum flips (one packed op per two rows; only the
+    chunk's middle state flips in a lane-dependent direction, which is
+    negligible per state), the exact group products, and for k > 1 the exact
+    64-bit products of adjacent groups.  Per prime: the Montgomery tree (two
     v_mad_i64_i32 and one v_mul_lo_u32 per product; one of each per reduced
     pair product) and the in-order signed accumulation (one v_mad_i64_i32).
     Paired sets replace, per two states, the tree's two last products and two
@@ -79,9 +79,9 @@ def instruction_mix(n: int, group: int, k: int = 1, paired: bool = False) -> dic
     groups = -(-n // group)
     words = -(-n // 2)
     mix = {
-        "v_pk_add_u16": words + words / 8,
+        "v_pk_add_u16": words,
         "v_mul_i32_i24_sdwa": n - groups,
-        "v_add_u32": words / 8 + 2,  # selects of the lane-dependent flips, register moves
+        "v_add_u32": 2,  # loop control and register moves
     }
     if k == 1:
         mix["v_mad_i64_i32"] = 2 * (groups - 1) + 1

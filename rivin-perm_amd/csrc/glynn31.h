@@ -534,9 +534,19 @@ __device__ __forceinline__ bool gray_walk(const int32_t* __restrict__ tables, in
         rs.template flip_mem<false>(d1, p);    sink.template state<false>(rs);  // s=6
         rs.template flip<false>(c0, p);        sink.template state<true>(rs);   // s=7
         {
-            // s=8 flips column 3; its new Gray bit is 1 ^ bit 4 of (base+t).
-            const uint64_t t = base + t0 + 8;
-            rs.flip_lane(D + 3 * W, !((t >> 4) & 1u), p);
+            // s=8 flips column 3; its new Gray bit is 1 ^ bit 4 of (base+t),
+            // which is bit 4 of t, the same in every lane, when the chunk
+            // has more than 4 bits.
+            if (cbits > 4) {
+                if (((t0 + 8) >> 4) & 1u) {
+                    rs.template flip_mem<false>(D + 3 * W, p);
+                } else {
+                    rs.template flip_mem<true>(D + 3 * W, p);
+                }
+            } else {
+                const uint64_t t = base + t0 + 8;
+                rs.flip_lane(D + 3 * W, !((t >> 4) & 1u), p);
+            }
             sink.template state<false>(rs);
         }
         rs.template flip<true>(c0, p);         sink.template state<true>(rs);   // s=9
@@ -549,8 +559,19 @@ __device__ __forceinline__ bool gray_walk(const int32_t* __restrict__ tables, in
         const uint32_t tn = t0 + kGlynnUnroll;
         if (tn < T) {
             const int j = __builtin_ctz(tn);  // 4 <= j <= cbits-1, uniform
-            const uint64_t t = base + tn;
-            rs.flip_lane(D + j * W, !((t >> (j + 1)) & 1u), p);
+            // The new Gray bit is 1 ^ bit j+1 of (base+tn): bit j+1 of tn in
+            // every lane except at the chunk's middle state (j = cbits-1),
+            // where it is the lane's own base bit.
+            if (j + 1 < cbits) {
+                if ((tn >> (j + 1)) & 1u) {
+                    rs.template flip_mem<false>(D + j * W, p);
+                } else {
+                    rs.template flip_mem<true>(D + j * W, p);
+                }
+            } else {
+                const uint64_t t = base + tn;
+                rs.flip_lane(D + j * W, !((t >> (j + 1)) & 1u), p);
+            }
             sink.template state<false>(rs);
         }
     }

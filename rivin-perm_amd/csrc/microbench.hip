@@ -179,9 +179,9 @@ struct MixEntry {
     MixKernel kernel;
 };
 const MixEntry kMixes[] = {
-    {{92, 56, 18, 24, 4}, mix_kernel<92, 56, 18, 24, 4>},
-    {{84, 52, 18, 24, 4}, mix_kernel<84, 52, 18, 24, 4>},
-    {{15, 7, 18, 24, 4}, mix_kernel<15, 7, 18, 24, 4>},
+    {{92, 56, 16, 24, 2}, mix_kernel<92, 56, 16, 24, 2>},
+    {{84, 52, 16, 24, 2}, mix_kernel<84, 52, 16, 24, 2>},
+    {{15, 7, 16, 24, 2}, mix_kernel<15, 7, 16, 24, 2>},
 };
 
 __global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,

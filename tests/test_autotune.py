@@ -151,3 +151,25 @@ def test_tune_single_sets_and_restores_tuning(monkeypatch):
     with pytest.raises(RuntimeError, match="residues differ"):
         autotune.tune_single(12, reps=1, clock=lambda: clock[0])
     assert state == {_native.TUNE_LANES_LOG2: 20, _native.TUNE_BATCH_BLOCKS: 2}
+
+
+def test_headline_instruction_mix_is_compiled_into_the_microbench():
+    """bench.py's roofline times the headline kernel's instruction mix on a
+    synthetic kernel compiled for fixed mixes; the mix the bench derives for
+    the headline (paired and per-state sets of 8, and single primes) must be
+    one of them, or the bench cannot report its roofline."""
+    import re
+    from pathlib import Path
+
+    import bench
+
+    root = Path(__file__).resolve().parent.parent
+    src = (root / "rivin-perm_amd" / "csrc" / "microbench.hip").read_text()
+    compiled = {tuple(int(v) for v in m.groups()) for m in re.finditer(
+        r"mix_kernel<(\d+), (\d+), (\d+), (\d+), (\d+)>", src)}
+    from permhip import _native
+
+    for k, paired in ((8, True), (8, False), (1, False)):
+        mix = bench.instruction_mix(bench.HEADLINE_N, 4, k, paired)
+        rounds = tuple(round(mix[t]) for t in _native.MIX_TYPES)
+        assert rounds in compiled, (k, paired, rounds)

@@ -331,3 +331,12 @@ def test_full_size_invariants_of_binary36(native):
     rng = np.random.default_rng(2)
     assert perm_binary.binary_permanent(B[rng.permutation(36)][:, rng.permutation(36)].copy()) == base
     assert perm_mod3.permanent_mod3_fast(B) == base % 3
+
+
+def test_bench_roofline_mix_runs(native):
+    """The headline roofline's synthetic mix kernel exists and runs."""
+    import bench
+
+    mix = bench.instruction_mix(bench.HEADLINE_N, 4, 8, True)
+    rounds = [round(mix[t]) for t in native.MIX_TYPES]
+    assert native.mix_microbench(rounds, iters=16) > 0
