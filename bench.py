@@ -110,8 +110,8 @@ def roofline(n: int, group: int, k: int, mulmods_per_s: float, paired: bool = Fa
 
     mix = instruction_mix(n, group, k, paired)
     rounds = [round(mix[t]) for t in _native.MIX_TYPES]  # 8 states x count / 8 chains
-    # Best of three timed launches: the bound is the mix's top issue rate.
-    states_per_s = 8.0 * max(_native.mix_microbench(rounds, iters=256) for _ in range(3))
+    # Best of five timed launches: the bound is the mix's top issue rate.
+    states_per_s = 8.0 * max(_native.mix_microbench(rounds, iters=512) for _ in range(5))
     peak = states_per_s * k * n
     return {
         "bound": "valu",
