@@ -299,3 +299,21 @@ def test_frontier_integral_row_order_and_support_mode(native):
 
     assert perm_binary.binary_permanent_frontier([[1, 0], [0, 1]], row_order=np.array([np.int64(1), np.int32(0)])) == 1
     assert perm_binary.binary_permanent_frontier([[-7, 0], [4, 9]], strict=False) == 1
+
+
+def test_frontier_with_64_live_columns_and_beyond(native):
+    """The upper-triangular all-ones matrix (permanent 1) keeps every column
+    live from the first row: at order 64 that is all 64 slots (the
+    multi-kernel path, whose keys may have every bit set); at order 70 the
+    device state mask cannot hold it and the call raises."""
+    import perm_binary
+    from permhip import binary_prep as prep
+
+    def upper(n):
+        return [[int(c >= r) for c in range(n)] for r in range(n)]
+
+    masks = prep.row_masks(upper(64))
+    assert prep.frontier_layers(masks, range(64))[2] == 64
+    assert perm_binary.binary_permanent_frontier(upper(64), row_order=range(64)) == 1
+    with pytest.raises(ValueError, match="live"):
+        perm_binary.binary_permanent_frontier(upper(70), row_order=range(70))
