@@ -497,8 +497,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(
-            "bench.py runs on one GPU only: this build has no multi-GPU path "
-            "(each residue would shard by prime or Gray range; see DESIGN.md)",
+            "bench.py runs on one GPU only: there is no multi-GPU path "
+            "(the residues would shard by prime, or each residue by Gray range, with a host gather)",
             file=sys.stderr,
         )
         return 2
