@@ -21,13 +21,14 @@ BATCH_MAX_ORDER = 46
 
 
 def _int64_rows(matrix):
-    """(rows as lists of Python ints, int64 array or None when an entry
-    does not fit int64)."""
+    """(rows, int64 array or None when an entry does not fit int64).  Rows
+    are lists of Python ints, or the int64 array itself for integer array
+    input (indexable the same way)."""
     if (isinstance(matrix, np.ndarray) and matrix.ndim == 2 and matrix.shape[0] == matrix.shape[1]
             and matrix.dtype.kind in "iu" and matrix.dtype.itemsize <= 8
             and not (matrix.dtype == np.uint64 and matrix.size and int(matrix.max()) > (1 << 63) - 1)):
         arr = np.ascontiguousarray(matrix, dtype=np.int64)
-        return arr.tolist(), arr
+        return arr, arr
     rows = [[int(x) for x in row] for row in matrix]
     n = len(rows)
     if any(len(r) != n for r in rows):
@@ -121,7 +122,7 @@ class PermanentBatch:
             if n == 0:
                 results.append(1)
             elif n == 1:
-                results.append(rows[0][0])
+                results.append(int(rows[0][0]))
             else:
                 value = planner.chinese_remainder_theorem(res, primes)
                 results.append(planner.signed_lift(value, bound, product))
