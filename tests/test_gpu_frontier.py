@@ -317,3 +317,14 @@ def test_frontier_with_64_live_columns_and_beyond(native):
     assert perm_binary.binary_permanent_frontier(upper(64), row_order=range(64)) == 1
     with pytest.raises(ValueError, match="live"):
         perm_binary.binary_permanent_frontier(upper(70), row_order=range(70))
+
+
+def test_frontier_entry_point_edges(native):
+    """Order 0 gives the empty product for every modulus; no modulus, too
+    many, or an even one are rejected; a row with no slots gives zero."""
+    assert native.binary_frontier_residues([], [], 0, [2147483647, 2147483629], 10)[0] == [1, 1]
+    with pytest.raises(RuntimeError, match="frontier"):
+        native.binary_frontier_residues([1], [1], 1, [], 10)
+    with pytest.raises(RuntimeError, match="odd"):
+        native.binary_frontier_residues([1], [1], 1, [2147483646], 10)
+    assert native.binary_frontier_residues([0b1, 0b0], [0b1, 0b0], 1, [2147483647], 10) == ([0], None)
