@@ -57,6 +57,9 @@ constexpr int kNecklaceBlock = 256;
 // Representatives per lane between accumulator folds: |term * orbit| <
 // 2^31 * 92 < 2^38, so 2^22 of them stay below 2^60.
 constexpr uint32_t kFoldReps = 1u << 22;
+// Eighths of the lanes with work left that must hold a representative
+// before a wavefront evaluates its terms.
+constexpr int kReadyEighths = 7;
 
 // The first L bits of prefix, prefix, prefix, ... (prefix has i bits).
 template <int L>
@@ -148,58 +151,71 @@ __global__ __launch_bounds__(kNecklaceBlock) void necklace_kernel(const uint64_t
     for (int r = 0; r < L; ++r) rs[r] = base_sum;
     uint64_t prev = 0, w = 0;
     int period = 1;
-    bool valid = false, done = false;
+    bool valid = false, done = false, have = false;
     int64_t acc = 0, total = 0;
     uint32_t reps = 0;
 
     for (;;) {
-        // Advance to the next representative of weight <= L/2 (per lane).
-        while (!done) {
-            if (!valid) {
-                const uint32_t t = atomicAdd(counter, 1u);
-                if (t >= n_tasks) {
-                    done = true;
-                    break;
+        // Advance.  Each iteration, every lane that holds no representative
+        // and has work left takes one step of its walk: a new task, or the
+        // successor of its last word.  The wavefront moves on to the terms
+        // once 7/8 of the lanes with work left hold a representative, so a
+        // lane in a long run of skipped words (periods that do not divide L,
+        // over-weight prefixes, task changes) does not hold the others back;
+        // it keeps its place and goes on in the next round.  Every
+        // iteration moves each waiting lane one step, so the loop ends.
+        for (;;) {
+            if (!have && !done) {
+                bool candidate = false;
+                if (!valid) {
+                    const uint32_t t = atomicAdd(counter, 1u);
+                    if (t >= n_tasks) {
+                        done = true;
+                    } else {
+                        const uint64_t task = tasks[t];
+                        period = (int)(task >> 56);
+                        const uint64_t bits = task & ((1ull << 56) - 1);
+                        w = periodic_extend<L>(bits >> (depth - period), period);
+                        valid = true;
+                        candidate = true;
+                    }
+                } else {
+                    uint64_t x = w;
+                    const int wt = __popcll(x);
+                    if (2 * wt > L) {
+                        // Every word sharing the prefix up to the (L/2+1)-th
+                        // set bit is over weight: fill the rest with ones so
+                        // the successor skips that whole subtree.
+                        uint64_t y = x;
+                        for (int k = L / 2 + 1; k < wt; ++k) y &= y - 1;
+                        x |= (1ull << __builtin_ctzll(y)) - 1;
+                    }
+                    const uint64_t z = ~x & kMask;
+                    const int b = z ? __builtin_ctzll(z) : 0;
+                    const int i = L - b;
+                    if (z == 0 || i <= depth) {
+                        // The walk leaves this task's subtree.
+                        valid = false;
+                    } else {
+                        const uint64_t pre = (x >> b) | 1u;
+                        // depth >= L/2, so i > L/2 and one shift-or extends the prefix.
+                        w = (2 * i >= L) ? ((pre << (L - i)) | (pre >> (2 * i - L))) : periodic_extend<L>(pre, i);
+                        period = i;
+                        candidate = true;
+                    }
                 }
-                const uint64_t task = tasks[t];
-                period = (int)(task >> 56);
-                const uint64_t bits = task & ((1ull << 56) - 1);
-                w = periodic_extend<L>(bits >> (depth - period), period);
-                valid = true;
-            } else {
-                uint64_t x = w;
-                const int wt = __popcll(x);
-                if (2 * wt > L) {
-                    // Every word sharing the prefix up to the (L/2+1)-th set
-                    // bit is over weight: fill the rest with ones so the
-                    // successor skips that whole subtree.
-                    uint64_t y = x;
-                    for (int k = L / 2 + 1; k < wt; ++k) y &= y - 1;
-                    x |= (1ull << __builtin_ctzll(y)) - 1;
-                }
-                const uint64_t z = ~x & kMask;
-                if (z == 0) {
-                    valid = false;
-                    continue;
-                }
-                const int b = __builtin_ctzll(z);
-                const int i = L - b;
-                if (i <= depth) {  // the successor leaves this task's subtree
-                    valid = false;
-                    continue;
-                }
-                const uint64_t pre = (x >> b) | 1u;
-                // depth >= L/2, so i > L/2 and one shift-or extends the prefix.
-                w = (2 * i >= L) ? ((pre << (L - i)) | (pre >> (2 * i - L))) : periodic_extend<L>(pre, i);
-                period = i;
+                if (candidate && ((kDivisors >> period) & 1u) && 2 * __popcll(w) <= L) have = true;
             }
-            if (((kDivisors >> period) & 1u) && 2 * __popcll(w) <= L) break;
+            // Lanes with work left, and those among them holding a term.
+            const int active = __popcll(__ballot(!done));
+            const int holding = __popcll(__ballot(have));
+            if (kReadyEighths * active <= 8 * holding) break;
         }
         if (__ballot(!done) == 0) break;
 
         // Row sums of the new representative: one uniform pass per column
         // that changed in any lane.
-        const uint64_t diff = done ? 0 : (prev ^ w);
+        const uint64_t diff = have ? (prev ^ w) : 0;
         uint64_t u = wave_or_u64(diff);
         while (u) {
             const int b = __builtin_ctzll(u);
@@ -210,7 +226,7 @@ __global__ __launch_bounds__(kNecklaceBlock) void necklace_kernel(const uint64_t
 #pragma unroll
             for (int r = 0; r < L; ++r) rs[r] += (int64_t)e[r] * (int64_t)sgn;
         }
-        if (!done) prev = w;
+        if (have) prev = w;
 
         // Term: reduce each row, multiply in a balanced tree.
         T v[L];
@@ -223,14 +239,14 @@ __global__ __launch_bounds__(kNecklaceBlock) void necklace_kernel(const uint64_t
         }
         const int wt = __popcll(w);
         const int orbit = (2 * wt < L) ? 2 * period : period;
-        const int32_t weight = done ? 0 : ((wt & 1) ? -orbit : orbit);
+        const int32_t weight = have ? ((wt & 1) ? -orbit : orbit) : 0;
         acc += (int64_t)ar.centred(v[0]) * (int64_t)weight;
         if (++reps == kFoldReps) {
             total += ar.fold(acc);
             acc = 0;
             reps = 0;
         }
-        valid = valid && !done;
+        have = false;
     }
     total += ar.fold(acc);
     int64_t r = total % p;
