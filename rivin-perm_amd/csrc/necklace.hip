@@ -59,7 +59,7 @@ constexpr int kNecklaceBlock = 256;
 constexpr uint32_t kFoldReps = 1u << 22;
 // Eighths of the lanes with work left that must hold a representative
 // before a wavefront evaluates its terms.
-constexpr int kReadyEighths = 7;
+constexpr int kReadyEighths = 6;
 
 // The first L bits of prefix, prefix, prefix, ... (prefix has i bits).
 template <int L>
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(kNecklaceBlock) void necklace_kernel(const uint64_t
         // Advance.  Each iteration, every lane that holds no representative
         // and has work left takes one step of its walk: a new task, or the
         // successor of its last word.  The wavefront moves on to the terms
-        // once 7/8 of the lanes with work left hold a representative, so a
+        // once 6/8 of the lanes with work left hold a representative, so a
         // lane in a long run of skipped words (periods that do not divide L,
         // over-weight prefixes, task changes) does not hold the others back;
         // it keeps its place and goes on in the next round.  Every
