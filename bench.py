@@ -447,7 +447,9 @@ def _launch_shape(plan) -> dict:
         "items_log2": int(st["items_log2"]),
         "chunk_bits": [int(st["chunk_bits_min"]), int(st["chunk_bits_max"])],
         "blocks_per_cu": int(st["blocks_per_cu"]),
-        "waves_per_cu": 4 * int(st["blocks_per_cu"]),
+        "resident_blocks_per_cu": int(st.get("resident_blocks_per_cu", -1)),
+        "waves_per_cu": 4 * (min(int(st["blocks_per_cu"]), int(st["resident_blocks_per_cu"]))
+                             if int(st.get("resident_blocks_per_cu", -1)) > 0 else int(st["blocks_per_cu"])),
         "chosen_by": "defaults",
     }
     if plan.tuning is not None:

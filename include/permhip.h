@@ -177,7 +177,9 @@ int phip_batch_set_option(void *plan, int option, int64_t value);
  * sized for, [13] workgroups per CU, [14] kernel variant of the first
  * launch: 0 plain, 1 pairs consecutive states (prime sets below
  * 1.25 * 2^30), 2 skips zero terms (0/1 tables whose states are mostly
- * zero).  Returns the number of values written or -1. */
+ * zero), [15] fewest workgroups per CU that any of the plan's kernels keeps
+ * resident (register and LDS use; -1 if unknown).  Returns the number of
+ * values written or -1. */
 int phip_batch_stats(void *plan, double *out, int len);
 
 int phip_batch_destroy(void *plan);

@@ -169,4 +169,24 @@ PHIP_DECLARE_BATCH_LAUNCH_SKIP(6, 4)
 PHIP_DECLARE_BATCH_LAUNCH_SKIP(6, 8)
 #undef PHIP_DECLARE_BATCH_LAUNCH_SKIP
 
+// Workgroups per CU each class kernel can keep resident (batch.hip), or -1.
+#define PHIP_DECLARE_BATCH_OCC(NAME) int NAME();
+PHIP_DECLARE_BATCH_OCC(occupancy_batch_g0_k1)
+PHIP_DECLARE_BATCH_OCC(occupancy_batch_g2_k1) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g2_k2)
+PHIP_DECLARE_BATCH_OCC(occupancy_batch_g2_k4) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g2_k8)
+PHIP_DECLARE_BATCH_OCC(occupancy_batch_g3_k1) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g3_k2)
+PHIP_DECLARE_BATCH_OCC(occupancy_batch_g3_k4) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g3_k8)
+PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k1) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k2)
+PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k4) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k8)
+PHIP_DECLARE_BATCH_OCC(occupancy_batch_g6_k1) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g6_k2)
+PHIP_DECLARE_BATCH_OCC(occupancy_batch_g6_k4) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g6_k8)
+PHIP_DECLARE_BATCH_OCC(occupancy_batch_g2_k4_paired) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g2_k8_paired)
+PHIP_DECLARE_BATCH_OCC(occupancy_batch_g3_k4_paired) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g3_k8_paired)
+PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k4_paired) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k8_paired)
+PHIP_DECLARE_BATCH_OCC(occupancy_batch_g6_k4_paired) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g6_k8_paired)
+PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k2_skip) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k4_skip)
+PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k8_skip) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g6_k2_skip)
+PHIP_DECLARE_BATCH_OCC(occupancy_batch_g6_k4_skip) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g6_k8_skip)
+#undef PHIP_DECLARE_BATCH_OCC
+
 }  // namespace phip

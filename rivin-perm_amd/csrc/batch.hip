@@ -152,4 +152,18 @@ hipError_t PHIP_LAUNCH_NAME(PHIP_BATCH_GROUP, PHIP_BATCH_SET, PHIP_PAIRED_SUFFIX
     return hipGetLastError();
 }
 
+#define PHIP_OCC_NAME(G, K, P) PHIP_CAT5(occupancy_batch_g, G, _k, K, P)
+
+// Workgroups of this class's kernel that one CU holds at once (its register
+// and LDS use), or -1.
+int PHIP_OCC_NAME(PHIP_BATCH_GROUP, PHIP_BATCH_SET, PHIP_PAIRED_SUFFIX)() {
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &blocks,
+            batch_glynn_kernel<PHIP_BATCH_GROUP, PHIP_BATCH_SET, PHIP_BATCH_PAIRED != 0, PHIP_BATCH_SKIP_ZEROS != 0>,
+            256, 0) != hipSuccess)
+        return -1;
+    return blocks;
+}
+
 }  // namespace phip

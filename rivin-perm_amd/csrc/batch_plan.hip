@@ -82,6 +82,28 @@ BatchLaunchFn launcher_for(int group, int set, int variant) {
     return nullptr;
 }
 
+int occupancy_for(int group, int set, int variant) {
+#define PHIP_OCC(G, K) \
+    if (group == G && set == K && variant == kVariantPlain) return occupancy_batch_g##G##_k##K();
+#define PHIP_OCC_PAIRED(G, K) \
+    if (group == G && set == K && variant == kVariantPaired) return occupancy_batch_g##G##_k##K##_paired();
+#define PHIP_OCC_SKIP(G, K) \
+    if (group == G && set == K && variant == kVariantSkipZeros) return occupancy_batch_g##G##_k##K##_skip();
+    PHIP_OCC(0, 1)
+    PHIP_OCC(2, 1) PHIP_OCC(2, 2) PHIP_OCC(2, 4) PHIP_OCC(2, 8)
+    PHIP_OCC(3, 1) PHIP_OCC(3, 2) PHIP_OCC(3, 4) PHIP_OCC(3, 8)
+    PHIP_OCC(4, 1) PHIP_OCC(4, 2) PHIP_OCC(4, 4) PHIP_OCC(4, 8)
+    PHIP_OCC(6, 1) PHIP_OCC(6, 2) PHIP_OCC(6, 4) PHIP_OCC(6, 8)
+    PHIP_OCC_PAIRED(2, 4) PHIP_OCC_PAIRED(2, 8) PHIP_OCC_PAIRED(3, 4) PHIP_OCC_PAIRED(3, 8)
+    PHIP_OCC_PAIRED(4, 4) PHIP_OCC_PAIRED(4, 8) PHIP_OCC_PAIRED(6, 4) PHIP_OCC_PAIRED(6, 8)
+    PHIP_OCC_SKIP(4, 2) PHIP_OCC_SKIP(4, 4) PHIP_OCC_SKIP(4, 8)
+    PHIP_OCC_SKIP(6, 2) PHIP_OCC_SKIP(6, 4) PHIP_OCC_SKIP(6, 8)
+#undef PHIP_OCC
+#undef PHIP_OCC_PAIRED
+#undef PHIP_OCC_SKIP
+    return -1;
+}
+
 // Sets of 4 or 8 primes all below kPairedPrimeLimit run on the paired-state
 // kernels.
 bool pairs_states(int set, const uint32_t* primes) {
@@ -699,6 +721,12 @@ int phip_batch_stats(void* handle, double* out, int len) {
     auto* plan = static_cast<BatchPlan*>(handle);
     const phip::PlanHost& h = plan->host;
     int cbits_min = h.jobs.empty() ? -1 : 64, cbits_max = -1;
+    // Fewest workgroups per CU any of the plan's class kernels keeps resident.
+    int resident = -1;
+    for (const phip::BatchClass& cl : h.classes) {
+        const int r = phip::occupancy_for(cl.group, cl.set, cl.variant);
+        if (r > 0 && (resident < 0 || r < resident)) resident = r;
+    }
     for (const phip::BatchJob& jb : h.jobs) {
         cbits_min = std::min(cbits_min, jb.cbits);
         cbits_max = std::max(cbits_max, jb.cbits);
@@ -717,7 +745,8 @@ int phip_batch_stats(void* handle, double* out, int len) {
                         (double)cbits_max,
                         (double)h.items_log2,
                         (double)plan->blocks_per_cu,
-                        h.classes.empty() ? -1.0 : (double)h.classes[0].variant};
+                        h.classes.empty() ? -1.0 : (double)h.classes[0].variant,
+                        (double)resident};
     const int m = std::min<int>(len, (int)(sizeof(v) / sizeof(v[0])));
     for (int i = 0; i < m; ++i) out[i] = v[i];
     return m;

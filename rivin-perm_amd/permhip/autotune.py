@@ -250,6 +250,14 @@ def tune_plan(
         )
         return t
 
+    # More workgroups per CU than the kernels keep resident only queue
+    # behind the others; those counts are not tried.
+    resident = int(st0.get("resident_blocks_per_cu", -1))
+    if resident > 0:
+        fitting = tuple(b for b in blocks if b <= resident)
+        blocks = fitting or (min(blocks),)
+        if start[1] > resident:
+            start = (start[0], max(blocks))
     best, best_t, seen = _descend(
         measure, start, items, blocks, tolerance, lambda: clock() - t_begin > max_seconds
     )

@@ -163,3 +163,24 @@ def test_autotuned_single_matrix_path(native):
     finally:
         native.set_tuning(native.TUNE_LANES_LOG2, old[0])
         native.set_tuning(native.TUNE_BATCH_BLOCKS, old[1])
+
+
+def test_resident_workgroups_are_reported(native):
+    """The plan reports how many workgroups per CU its kernels keep
+    resident: the headline class (139 VGPRs, 3 waves per SIMD) keeps 3, and
+    the autotuner does not try more than that."""
+    import numpy as np
+
+    from permhip import autotune
+
+    A = np.random.default_rng(20261203).integers(-10, 11, (32, 32)).astype(np.int64)
+    primes = [1073741827, 1073741831, 1073741833, 1073741839, 1073741843, 1073741857, 1073741891, 1073741909]
+    plan = native.Batch([A], [primes])
+    try:
+        resident = plan.stats()["resident_blocks_per_cu"]
+        assert 1 <= resident <= 8
+        result = autotune.tune_plan(plan, items=(14, 15), blocks=(1, 2, 3, 4), reps=1, max_seconds=10)
+        assert result.blocks_per_cu <= resident
+        assert all(t.blocks_per_cu <= resident for t in result.trials)
+    finally:
+        plan.close()
