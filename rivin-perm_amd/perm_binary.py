@@ -123,7 +123,7 @@ def _frontier_permanent_masks(row_masks, row_order, max_states: int) -> int:
             f"this row order keeps {slots} columns live at once; the device frontier DP holds at most "
             f"{_prep.FRONTIER_MAX_SLOTS}"
         )
-    bound = _prep.normalized_bound(_prep.unpack_masks(row_masks))
+    bound = _prep.normalized_bound_masks(row_masks)
     primes = _prep.crt_primes_above_bound(bound)
     residues = []
     step = _native.FRONTIER_MAX_PRIMES
@@ -179,7 +179,7 @@ def binary_permanent_bound(matrix, *, strict: bool = True) -> int:
         return 0
     if not blocks:
         return 1
-    return math.prod(_prep.normalized_bound(_prep.unpack_masks(b)) for b in blocks)
+    return math.prod(_prep.normalized_bound_masks(b) for b in blocks)
 
 
 def binary_permanent_mod_p(matrix, p, *, backend: str = "auto", parallel: bool = True,

@@ -88,3 +88,59 @@ def test_normalize_ndarray_fast_path_matches_the_entry_loop():
                     except (TypeError, ValueError) as exc:
                         outcomes.append((type(exc), str(exc)))
                 assert outcomes[0] == outcomes[1]
+
+
+def test_bound_from_masks_equals_bound_of_unpacked_rows():
+    rng = random.Random(4242)
+    for n in (0, 1, 2, 7, 8, 9, 31, 64, 65, 130):
+        for density in (0.0, 0.1, 0.5, 1.0):
+            masks = _random_masks(rng, n, density)
+            rows = binary_prep.unpack_masks(masks)
+            assert binary_prep.column_degrees(masks) == [sum(c) for c in zip(*rows)]
+            assert binary_prep.normalized_bound_masks(masks) == binary_prep.normalized_bound(rows), (n, density)
+
+
+def test_normalize_list_rows_keep_entry_loop_semantics():
+    """Rows of plain 0/1 ints are taken as they are; any other row is
+    validated entry by entry, so the first offending entry in row-major
+    order names the error and non-strict mode maps nonzero to 1."""
+    rng = random.Random(8)
+    plants = [True, False, np.int64(1), np.int8(0), 2, -1, 1.5, "1", None]
+    for _ in range(400):
+        n = rng.randrange(1, 7)
+        rows = [[rng.randrange(2) for _ in range(n)] for _ in range(n)]
+        for _ in range(rng.randrange(3)):
+            rows[rng.randrange(n)][rng.randrange(n)] = rng.choice(plants)
+        if rng.random() < 0.3:
+            rows = [tuple(r) for r in rows]
+        for strict in (True, False):
+            expected = None
+            for r, row in enumerate(rows):
+                for c, v in enumerate(row):
+                    if isinstance(v, (float, str)) or v is None:
+                        expected = (TypeError, f"entry ({r}, {c}) is {type(v).__name__}")
+                    elif strict and int(v) not in (0, 1):
+                        expected = (ValueError, f"entry ({r}, {c}) is {int(v)}")
+                    if expected:
+                        break
+                if expected:
+                    break
+            if expected:
+                with pytest.raises(expected[0]) as info:
+                    binary_prep.normalize_binary_matrix(rows, strict=strict)
+                assert str(info.value).endswith(expected[1])
+            else:
+                got = binary_prep.normalize_binary_matrix(rows, strict=strict)
+                assert got == tuple(tuple(1 if int(v) else 0 for v in row) for row in rows)
+                assert all(type(v) is int for row in got for v in row)
+
+
+def test_row_masks_set_the_bits_of_nonzero_entries():
+    rng = random.Random(31)
+    for n in (0, 1, 63, 64, 65, 100, 257):
+        for values in ((0, 1), (0, 1, 2, 255), (0, 1, -1, 300), (False, True)):
+            rows = [tuple(rng.choice(values) for _ in range(n)) for _ in range(n)]
+            expected = tuple(sum(1 << c for c in range(n) if rows[r][c]) for r in range(n))
+            assert binary_prep.row_masks(rows) == expected, (n, values)
+    ragged = [(1, 0) * 40, (1,) * 81]
+    assert binary_prep.row_masks(ragged) == (sum(1 << c for c in range(0, 80, 2)), (1 << 81) - 1)
