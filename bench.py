@@ -493,6 +493,9 @@ def main() -> int:
     ap.add_argument("--skip-zeros-permille", type=int, default=None)
     ap.add_argument("--frontier", type=int, default=0, metavar="REPS",
                     help="only the frontier DP on the open 12x16 grid's biadjacency (order 96), REPS times")
+    ap.add_argument("--batch-only", type=int, default=0, metavar="REPS",
+                    help="only the 4096-matrix batch at the default launch shape: one untimed run, then "
+                         "REPS timed runs (for profilers)")
     ap.add_argument("--schur-residue", type=int, default=0, metavar="N",
                     help="compute perm(S_N) modulo the first prime = 1 (mod N) above 2^30 and exit "
                          "(one necklace or Glynn launch, for profilers)")
@@ -541,6 +544,24 @@ def main() -> int:
             t0 = time.perf_counter()
             r = _native.permanent_residues(B, primes)
             log(f"binary36 residues {r} in {(time.perf_counter() - t0) * 1e3:.1f} ms (rep {rep})")
+        return 0
+    if args.batch_only:
+        from permhip.batch import PermanentBatch
+
+        batch = PermanentBatch(batch_matrices())
+        try:
+            warm = batch.run()
+            for rep in range(args.batch_only):
+                t0 = time.perf_counter()
+                got = batch.run()
+                dt = time.perf_counter() - t0
+                if got != warm:
+                    raise RuntimeError("batch results changed between runs")
+                st = batch.stats()
+                log(f"batch of {len(warm)} matrices in {dt * 1e3:.1f} ms (kernels {st['last_ms']:.1f} ms, "
+                    f"{st['mulmods'] / dt:.3e} mulmod/s, rep {rep})")
+        finally:
+            batch.close()
         return 0
     if args.frontier:
         import perm_binary

@@ -299,14 +299,26 @@ int assign_items(PlanHost& out, int items_log2, int min_cbits) {
     out.classes.clear();
     out.items_log2 = items_log2;
     out.min_cbits = min_cbits;
-    // Chunk bits: spread the job states over about 2^items_log2 items.
+    // Chunk bits: spread all job states over about 2^items_log2 items, and
+    // each class's states over at least about 2^(items_log2 - 2).  Classes
+    // run one after another, so a class with few states would otherwise be
+    // a handful of long items, each wave alone on its SIMD while the rest
+    // of the device idles.  The second rule never lengthens a chunk.
+    auto spread = [](double states, double items) {
+        const int s = (int)std::lround(std::log2(std::max(1.0, states / items)));
+        return std::min(std::max(s, 10), 30);
+    };
+    auto class_key = [](const BatchJob& jb) { return (jb.group * 16 + jb.set) * 4 + jb.variant; };
     double job_states = 0;
-    for (const BatchJob& jb : out.jobs) job_states += std::ldexp(1.0, jb.n - 1);
-    const double target = std::ldexp(1.0, items_log2);
-    int s = (int)std::lround(std::log2(std::max(1.0, job_states / target)));
-    s = std::min(std::max(s, 10), 30);
+    std::map<int, double> class_states;
+    for (const BatchJob& jb : out.jobs) {
+        job_states += std::ldexp(1.0, jb.n - 1);
+        class_states[class_key(jb)] += std::ldexp(1.0, jb.n - 1);
+    }
+    const int s_all = spread(job_states, std::ldexp(1.0, items_log2));
     for (BatchJob& jb : out.jobs) {
         const int L = jb.n - 1;
+        const int s = std::min(s_all, spread(class_states[class_key(jb)], std::ldexp(1.0, items_log2 - 2)));
         int c = (L >= 10) ? std::max(4, std::min(s - 6, L - 6)) : 4;
         c = std::min(std::max(c, min_cbits), L);
         jb.cbits = c;
