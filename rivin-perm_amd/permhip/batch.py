@@ -10,7 +10,7 @@ int64) go through the single-matrix device path.
 
 from __future__ import annotations
 
-from typing import List, Sequence
+from typing import List, Sequence, Tuple
 
 import numpy as np
 
@@ -55,7 +55,7 @@ class PermanentBatch:
         plan and keeps the fastest; the choice is in ``self.tuning``."""
         self.rows: List[List[List[int]]] = []
         self.bounds: List[int] = []
-        self.primes: List[List[int]] = []
+        self.primes: List[Tuple[int, ...]] = []
         self.products: List[int] = []
         device_mats, device_primes, self.device_index = [], [], []
         self.other_index = []
@@ -64,6 +64,7 @@ class PermanentBatch:
             n = len(rows)
             bound = planner.matrix_permanent_bound(arr if arr is not None else rows) if n else 1
             primes, product = planner.generic_crt_primes(bound) if n > 1 else ([], 1)
+            primes = tuple(primes)
             self.rows.append(rows)
             self.bounds.append(bound)
             self.primes.append(primes)
@@ -124,7 +125,7 @@ class PermanentBatch:
             elif n == 1:
                 results.append(int(rows[0][0]))
             else:
-                value = planner.chinese_remainder_theorem(res, primes)
+                value = planner.crt_known_moduli(res, primes)
                 results.append(planner.signed_lift(value, bound, product))
         return results
 

@@ -8,6 +8,7 @@ lift.  Each function cites the reference function it restates.
 
 from __future__ import annotations
 
+import functools
 import math
 import threading
 from numbers import Integral
@@ -166,6 +167,35 @@ def chinese_remainder_theorem(residues: Sequence[int], moduli: Sequence[int]) ->
         t = ((r_i - result) % m_i) * inverse % m_i
         result += product * t
         product *= m_i
+    return result
+
+
+@functools.lru_cache(maxsize=256)
+def _garner_inverses(moduli: Tuple[int, ...]) -> Tuple[int, ...]:
+    """(m_0 ... m_{i-1})^-1 mod m_i for i >= 1, the constants of the
+    incremental solution above for this modulus sequence."""
+    out, product = [], moduli[0]
+    for m in moduli[1:]:
+        g, inverse, _ = extended_gcd(product, m)
+        if g != 1:
+            raise ValueError("Moduli must be pairwise coprime")
+        out.append(inverse % m)
+        product *= m
+    return tuple(out)
+
+
+def crt_known_moduli(residues: Sequence[int], moduli: Tuple[int, ...]) -> int:
+    """chinese_remainder_theorem for a tuple of validated moduli (integers
+    greater than 1) that recurs, such as a batch's CRT prime lists: the
+    inverses are computed once per modulus tuple."""
+    if len(residues) != len(moduli):
+        raise ValueError("residues and moduli must have the same length")
+    if not moduli:
+        return 0
+    result, product = residues[0], moduli[0]
+    for r, m, inverse in zip(residues[1:], moduli[1:], _garner_inverses(moduli)):
+        result += product * (((r - result) % m) * inverse % m)
+        product *= m
     return result
 
 

@@ -109,7 +109,7 @@ def _finish_part(part: _Part) -> int:
         product = 1
         for p in part.primes:
             product *= p
-        value = planner.chinese_remainder_theorem(part.residues, part.primes)
+        value = planner.crt_known_moduli(part.residues, tuple(part.primes))
         return planner.signed_lift(value, part.bound, product)
     value = binary_prep.crt_nonnegative(part.residues, part.primes)
     if value > part.bound:

@@ -155,3 +155,26 @@ def test_reference_device_golden_residues_match_known_permanents():
         assert planner.is_prime(p) and p % n == 1
         assert pow(omega, n, p) == 1 and omega != 1
         assert exact[n] % p == expected, n
+
+
+def test_crt_known_moduli_solves_the_congruences():
+    """The cached-constant CRT returns the unique solution in [0, prod):
+    it satisfies every congruence and equals the incremental solution."""
+    import math
+    import random
+
+    rng = random.Random(5)
+    primes = [planner._ascending_prime(i) for i in range(12)]
+    for k in range(0, 13):
+        moduli = tuple(primes[:k])
+        for _ in range(20):
+            res = [rng.randrange(m) for m in moduli]
+            got = planner.crt_known_moduli(res, moduli)
+            assert 0 <= got < math.prod(moduli)
+            assert all(got % m == r for r, m in zip(res, moduli))
+            assert got == planner.chinese_remainder_theorem(res, moduli)
+    assert planner.crt_known_moduli([3, 4], (5, 7)) == 18
+    with pytest.raises(ValueError, match="pairwise coprime"):
+        planner.crt_known_moduli([1, 2], (6, 9))
+    with pytest.raises(ValueError, match="same length"):
+        planner.crt_known_moduli([1], (5, 7))
