@@ -164,8 +164,10 @@ double nonzero_share(const int64_t* a, int n) {
 template <typename T>
 struct DevBuf {
     T* ptr = nullptr;
-    ~DevBuf() {
+    ~DevBuf() { release(); }
+    void release() {
         if (ptr) hipFree(ptr);
+        ptr = nullptr;
     }
     int alloc(size_t count) {
         if (ptr) {
@@ -452,6 +454,17 @@ struct BatchPlan {
         return 0;
     }
 
+    void drop_graph() {
+        if (exec) {
+            hipGraphExecDestroy(exec);
+            exec = nullptr;
+        }
+        if (graph) {
+            hipGraphDestroy(graph);
+            graph = nullptr;
+        }
+    }
+
     // New chunk sizes for about 2^items_log2 work items.
     int resize_items(int items_log2) {
         PHIP_HIP_TRY(hipSetDevice(device));
@@ -465,7 +478,17 @@ struct BatchPlan {
             set_error(why);
             return -1;
         }
-        return upload_items();
+        if (upload_items() != 0) {
+            // A half-written work list must never run: without it every
+            // later launch reports the failed re-sizing.
+            const std::string why = last_error();
+            d_jobs.release();
+            d_items.release();
+            d_partials.release();
+            set_error(why);
+            return -1;
+        }
+        return 0;
     }
 
     int enqueue(bool timed) {
@@ -692,6 +715,11 @@ int phip_batch_set_option(void* handle, int option, int64_t value) {
         phip::set_error("cannot change options while the plan is running");
         return -1;
     }
+    // The captured graph holds the old launch shape and device pointers
+    // (re-sizing frees and reallocates the work list), so drop it before
+    // anything changes: the next launch captures afresh, also after an
+    // option change that fails half-way.
+    plan->drop_graph();
     switch (option) {
         case PHIP_BATCH_BLOCKS_PER_CU:
             if (value < 1 || value > 64) {
@@ -713,14 +741,6 @@ int phip_batch_set_option(void* handle, int option, int64_t value) {
         default:
             phip::set_error("unknown batch option");
             return -1;
-    }
-    if (plan->exec) {
-        hipGraphExecDestroy(plan->exec);
-        plan->exec = nullptr;
-    }
-    if (plan->graph) {
-        hipGraphDestroy(plan->graph);
-        plan->graph = nullptr;
     }
     return 0;
 }

@@ -31,9 +31,10 @@ class _BinaryLibrary:
         self._lib = lib
 
     def cuda_runtime_available(self) -> int:
-        return int(self._lib.phip_available())
+        return int(_native._touch_device(self._lib))
 
     def binary_permanent_modp_cuda(self, pointer, order, modulus) -> int:
+        _native._check_fork()
         return int(self._lib.phip_binary_permanent_modp(ctypes.cast(pointer, _U8P), int(order), int(modulus)))
 
 
@@ -63,7 +64,7 @@ def is_binary_cuda_available() -> bool:
         return False
     try:
         return bool(probe())
-    except (OSError, TypeError, ValueError):
+    except (OSError, TypeError, ValueError, _native.ForkedProcessError):
         return False
 
 

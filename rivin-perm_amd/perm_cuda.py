@@ -45,12 +45,14 @@ class _ModpLibrary:
         self._lib = lib
 
     def cuda_runtime_available(self) -> int:
-        return int(self._lib.phip_available())
+        return int(_native._touch_device(self._lib))
 
     def permanent_modp_cuda(self, pointer, n, p) -> int:
+        _native._check_fork()
         return int(self._lib.phip_permanent_modp(ctypes.cast(pointer, _I64P), int(n), int(p)))
 
     def schur_permanent_modp_cuda(self, n, omega, p) -> int:
+        _native._check_fork()
         return int(self._lib.phip_schur_permanent_modp(int(n), int(omega), int(p)))
 
 
@@ -61,9 +63,10 @@ class _Mod3Library:
         self._lib = lib
 
     def cuda_runtime_available(self) -> int:
-        return int(self._lib.phip_available())
+        return int(_native._touch_device(self._lib))
 
     def permanent_mod3_matrix(self, pointer, n) -> int:
+        _native._check_fork()
         return int(self._lib.phip_permanent_mod3_matrix(ctypes.cast(pointer, _I32P), int(n)))
 
 
@@ -96,7 +99,7 @@ def is_cuda_available() -> bool:
         return False
     try:
         return bool(probe())
-    except (OSError, TypeError, ValueError):
+    except (OSError, TypeError, ValueError, _native.ForkedProcessError):
         return False
 
 

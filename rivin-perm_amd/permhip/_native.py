@@ -20,6 +20,11 @@ LIBRARY_PATH = Path(os.environ.get("PERMHIP_LIBRARY", Path(__file__).resolve().p
 
 _lock = threading.Lock()
 _lib = None
+# Process that first touched the HIP runtime through this module.  A child
+# forked after that point inherits a runtime it cannot use (HIP does not
+# support fork after initialisation), so every device call in such a child
+# raises instead of entering the library.
+_device_pid = None
 
 _I64P = ctypes.POINTER(ctypes.c_int64)
 _U64P = ctypes.POINTER(ctypes.c_uint64)
@@ -152,17 +157,41 @@ def last_error() -> str:
     return buf.value.decode("utf-8", "replace")
 
 
+class ForkedProcessError(RuntimeError):
+    """A device call in a process forked after the parent initialised HIP."""
+
+
+def _check_fork() -> None:
+    if _device_pid is not None and os.getpid() != _device_pid:
+        raise ForkedProcessError(
+            f"libpermhip was initialised in process {_device_pid}; this process ({os.getpid()}) is a fork of it "
+            "and cannot use the HIP runtime. Call the permanent functions with parallel=False, or start "
+            "worker processes with the 'spawn' method"
+        )
+
+
+def _touch_device(lib) -> int:
+    """phip_available(), recording the first process that initialised HIP."""
+    global _device_pid
+    _check_fork()
+    status = lib.phip_available()
+    if _device_pid is None:
+        _device_pid = os.getpid()
+    return status
+
+
 def available() -> bool:
-    """True when the library loads and a HIP device is usable."""
+    """True when the library loads and a HIP device is usable in this
+    process (False in a fork of a process that initialised HIP)."""
     try:
-        return bool(load().phip_available())
+        return bool(_touch_device(load()))
     except (OSError, RuntimeError):
         return False
 
 
 def require_device():
     lib = load()
-    if not lib.phip_available():
+    if not _touch_device(lib):
         raise RuntimeError("no HIP device is available to libpermhip (there is no CPU fallback)")
     return lib
 

@@ -16,6 +16,13 @@ On an idle server (nothing waiting or in flight) a request below
 which skips creating a plan.  The scheduler sleeps until a request arrives
 when idle, and polls in-flight plans every 0.2 ms otherwise.
 
+Failures stay with the requests they belong to: a request the device path
+cannot take (an integer matrix of order 64 or more) is refused in
+``submit``; an error while a request runs outside the plans fails that
+request's future; an error while a plan is created, launched or polled
+fails the futures of every request in that plan.  The scheduler thread
+keeps serving the other requests in every case.
+
 Latency (completion minus arrival) and throughput are recorded per request.
 """
 
@@ -63,15 +70,27 @@ class _Request:
     constant: int = 1  # product of parts computed outside the device plans
     done_parts: int = 0
     finished: float = 0.0
+    failed: bool = False
 
     @property
     def cost(self) -> float:
         return sum(p.cost for p in self.parts)
 
 
+# Largest order the device residue kernels take (the limit of
+# perm_c_omp.c:885-888 and perm_modp_cuda.cu:744-747).
+MAX_ORDER = 63
+# Bytes of partial sums one work item holds (kMaxSet 8-byte slots).
+_ITEM_PARTIAL_BYTES = 8 * 8
+
+
 def _plan_integer(matrix) -> tuple:
     rows = [[int(x) for x in r] for r in matrix]
     n = len(rows)
+    if any(len(r) != n for r in rows):
+        raise ValueError("Matrix must be square")
+    if n > MAX_ORDER:
+        raise ValueError("matrix dimension must be less than 64")
     if n == 0:
         return [], 1
     if n == 1:
@@ -128,6 +147,14 @@ class PermanentServer:
         self.plan_cost = plan_cost
         self.plan_memory = plan_memory
         self.blocks_per_cu = blocks_per_cu
+        # Device bytes of one plan that do not depend on its requests: the
+        # work items' partial sums.  The planner spreads a plan over about
+        # 2^items_log2 items, plus up to 2^(items_log2 - 2) more per extra
+        # (group, set, variant) class; 4 * 2^items_log2 items covers the
+        # classes a served mix uses.
+        items_log2 = _native.get_tuning(_native.TUNE_BATCH_ITEMS_LOG2)
+        self.plan_overhead_bytes = 4 * (1 << items_log2) * _ITEM_PARTIAL_BYTES
+        self.errors = 0  # requests failed by device or scheduler errors
         self._incoming: "queue.Queue[_Request]" = queue.Queue()
         self._waiting: List[_Request] = []
         self._inflight = []  # (Batch, [(request, part)])
@@ -166,7 +193,19 @@ class PermanentServer:
             self._thread.join()
 
     # -- scheduler -----------------------------------------------------
+    def _fail(self, req: _Request, exc: BaseException) -> None:
+        """Fail one request's future (once); the scheduler carries on."""
+        if req.failed or req.future.done():
+            return
+        req.failed = True
+        req.finished = time.perf_counter()
+        self.errors += 1
+        req.future.set_exception(exc)
+        self.completed.append(req)
+
     def _complete(self, req: _Request) -> None:
+        if req.failed:
+            return
         try:
             value = req.constant
             for part in req.parts:
@@ -187,12 +226,11 @@ class PermanentServer:
                 part.residues = _native.permanent_residues(part.matrix, part.primes)
                 req.done_parts += 1
 
-    def _admit(self) -> None:
-        while True:
-            try:
-                req = self._incoming.get_nowait()
-            except queue.Empty:
-                break
+    def _take_incoming(self, req: _Request) -> None:
+        """Parts outside the plans' orders, and a small request on an idle
+        device, run at once through the single-matrix path; everything else
+        waits for a plan.  An error fails this request only."""
+        try:
             self._run_outside_plans(req)
             if (req.done_parts < len(req.parts) and not self._inflight and not self._waiting
                     and req.cost <= self.direct_cost):
@@ -201,14 +239,53 @@ class PermanentServer:
                     if part.residues is None:
                         part.residues = _native.permanent_residues(part.matrix, part.primes)
                         req.done_parts += 1
-            if req.done_parts == len(req.parts):
-                self._complete(req)
-            else:
-                self._waiting.append(req)
+        except Exception as exc:
+            self._fail(req, exc)
+            return
+        if req.done_parts == len(req.parts):
+            self._complete(req)
+        else:
+            self._waiting.append(req)
+
+    def _launch_plan(self, chosen: List[_Request]) -> None:
+        """One device plan for the chosen requests' remaining parts; an
+        error while it is created or launched fails those requests."""
+        mats, primes, owners = [], [], []
+        for req in chosen:
+            for part in req.parts:
+                if part.residues is None:
+                    mats.append(part.matrix)
+                    primes.append(part.primes)
+                    owners.append((req, part))
+        plan = None
+        try:
+            plan = _native.Batch(mats, primes)
+            plan.set_option(_native.BATCH_USE_GRAPH, 0)
+            plan.set_option(_native.BATCH_BLOCKS_PER_CU, self.blocks_per_cu)
+            plan.launch()
+        except Exception as exc:
+            if plan is not None:
+                try:
+                    plan.close()
+                except Exception:
+                    pass
+            for req in chosen:
+                self._fail(req, exc)
+            return
+        self.plans_launched += 1
+        self._inflight.append((plan, owners))
+
+    def _admit(self) -> None:
+        while True:
+            try:
+                req = self._incoming.get_nowait()
+            except queue.Empty:
+                break
+            self._take_incoming(req)
         while self._waiting and len(self._inflight) < self.max_inflight:
             chosen = [self._waiting.pop(0)]  # oldest first
             cost = chosen[0].cost
-            mem = sum(p.device_bytes for p in chosen[0].parts)
+            mem = self.plan_overhead_bytes + sum(p.device_bytes for p in chosen[0].parts)
             for req in sorted(self._waiting, key=lambda r: r.cost):
                 if cost + req.cost > self.plan_cost:
                     break
@@ -220,25 +297,24 @@ class PermanentServer:
                 mem += rmem
             for req in chosen[1:]:
                 self._waiting.remove(req)
-            mats, primes, owners = [], [], []
-            for req in chosen:
-                for part in req.parts:
-                    if part.residues is None:
-                        mats.append(part.matrix)
-                        primes.append(part.primes)
-                        owners.append((req, part))
-            plan = _native.Batch(mats, primes)
-            plan.set_option(_native.BATCH_USE_GRAPH, 0)
-            plan.set_option(_native.BATCH_BLOCKS_PER_CU, self.blocks_per_cu)
-            plan.launch()
-            self.plans_launched += 1
-            self._inflight.append((plan, owners))
+            self._launch_plan(chosen)
 
     def _poll(self) -> bool:
         progressed = False
         still = []
         for plan, owners in self._inflight:
-            flat = plan.poll(block=False)
+            try:
+                flat = plan.poll(block=False)
+            except Exception as exc:
+                # The plan's results are lost: fail its requests, drop it.
+                progressed = True
+                for req, _ in owners:
+                    self._fail(req, exc)
+                try:
+                    plan.close()
+                except Exception:
+                    pass
+                continue
             if flat is None:
                 still.append((plan, owners))
                 continue
@@ -255,13 +331,32 @@ class PermanentServer:
         self._inflight = still
         return progressed
 
+    def _fail_all(self, exc: BaseException) -> None:
+        """An error outside any one request or plan: fail every request the
+        scheduler holds, so no future is left unresolved."""
+        for plan, owners in self._inflight:
+            for req, _ in owners:
+                self._fail(req, exc)
+            try:
+                plan.close()
+            except Exception:
+                pass
+        self._inflight = []
+        for req in self._waiting:
+            self._fail(req, exc)
+        self._waiting = []
+
     def _loop(self) -> None:
         while True:
             # Cleared before the queue is drained: a request that arrives
             # after the drain sets the event again, so no wake-up is lost.
             self._wake.clear()
-            self._admit()
-            progressed = self._poll()
+            try:
+                self._admit()
+                progressed = self._poll()
+            except Exception as exc:
+                self._fail_all(exc)
+                progressed = True
             idle = not self._inflight and not self._waiting and self._incoming.empty()
             if self._stop.is_set() and idle:
                 return

@@ -189,3 +189,66 @@ def test_serve_scheduler(fake_device):
     assert got == [oracle.subset_permanent(m.tolist()) for _, m in reqs]
     assert len(server.completed) == len(reqs)
     assert math.isfinite(sum(r.finished - r.arrival for r in server.completed))
+
+
+def test_serve_scheduler_survives_failures(fake_device, monkeypatch):
+    """A request or plan that fails resolves its own futures with the error;
+    requests before and after it still complete and the scheduler thread
+    stays alive."""
+    from permhip import serve
+    from permhip.serve import PermanentServer
+
+    rng = np.random.default_rng(10)
+    server = PermanentServer(max_inflight=2, plan_cost=1e4, direct_cost=0.0)
+    first = rng.integers(-9, 10, (7, 7))
+    ok1 = server.submit(first, "integer")
+    assert ok1.result(timeout=30) == oracle.subset_permanent(first.tolist())
+
+    # Orders the device path does not take are refused at submit.
+    huge = server.submit(np.ones((64, 64), dtype=np.int64), "integer")
+    with pytest.raises(ValueError, match="less than 64"):
+        huge.result(timeout=30)
+
+    # A plan whose creation fails fails only its own requests.
+    class Boom(FakeBatch):
+        def __init__(self, *a, **k):
+            raise RuntimeError("phip_batch_create failed: hipMalloc failed")
+
+    monkeypatch.setattr(fake_device, "Batch", Boom)
+    doomed = server.submit(rng.integers(-9, 10, (8, 8)), "integer")
+    with pytest.raises(RuntimeError, match="hipMalloc"):
+        doomed.result(timeout=30)
+
+    # A plan whose poll fails, likewise.
+    class BadPoll(FakeBatch):
+        def poll(self, block=False):
+            raise RuntimeError("phip_batch_query failed: device lost")
+
+    monkeypatch.setattr(fake_device, "Batch", BadPoll)
+    lost = server.submit(rng.integers(-9, 10, (9, 9)), "integer")
+    with pytest.raises(RuntimeError, match="device lost"):
+        lost.result(timeout=30)
+
+    # A request whose outside-plan part fails (order < 5 runs outside the
+    # plans through the single-matrix path).
+    monkeypatch.setattr(fake_device, "Batch", FakeBatch)
+    real = fake_device.permanent_residues
+
+    def flaky(matrix, moduli):
+        if np.asarray(matrix).shape[-1] == 3:
+            raise RuntimeError("phip_permanent_residues failed: injected")
+        return real(matrix, moduli)
+
+    monkeypatch.setattr(fake_device, "permanent_residues", flaky)
+    small = server.submit(rng.integers(-9, 10, (3, 3)), "integer")
+    with pytest.raises(RuntimeError, match="injected"):
+        small.result(timeout=30)
+
+    # Later requests complete normally.
+    later = [rng.integers(-9, 10, (n, n)) for n in (6, 10, 4)]
+    futs = [server.submit(m, "integer") for m in later]
+    assert [f.result(timeout=30) for f in futs] == [oracle.subset_permanent(m.tolist()) for m in later]
+    assert server._thread.is_alive()
+    assert server.errors == 3
+    server.close()
+    assert serve.MAX_ORDER == 63
