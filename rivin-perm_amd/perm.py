@@ -321,12 +321,16 @@ def matrix_permanent(
     """Exact permanent of an integer matrix (perm.py:1076-1212).
 
     All CRT residues of the generic path are computed in one device launch.
-    ``method='low_rank'`` runs the reference's exact low-rank coefficient
-    identity on the host and raises LowRankLimitExceeded beyond rank 4 or
-    50,000 coefficient states; 'auto' and 'generic' use the device Glynn
-    path (the low-rank branch is only cheaper than the device for orders the
-    device finishes in microseconds).  ``parallel`` and ``num_threads`` are
-    validated and otherwise ignored: the device is always used whole.
+    ``method='low_rank'`` runs the exact low-rank coefficient identity on
+    the host and raises LowRankLimitExceeded beyond rank 4 or 50,000
+    coefficient states.  ``method='auto'`` first probes the rank modulo
+    every CRT prime, up to the largest rank whose host cost beats the
+    device's estimated Glynn time (``permhip.lowrank.rank_limit``; orders
+    below 20 are never probed), as the reference's auto mode does
+    (perm.py:1158-1169); a matrix over that rank at any prime goes to the
+    device.  'generic' always uses the device.  ``parallel`` and
+    ``num_threads`` are validated and otherwise ignored: the device is
+    always used whole.
     """
     n = _validate_integer_matrix(matrix)
     _validate_num_threads(num_threads)
@@ -345,11 +349,19 @@ def matrix_permanent(
     if verbose:
         print(f"Using {len(primes)} primes")
         print(f"Prime product: {prime_product:.2e}, bound: {bound:.2e}")
+    residues = None
     if method == "low_rank":
         residues = _lowrank.low_rank_residues(rows, primes, _lowrank.LOW_RANK_MAX_RANK, required=True)
+    elif method == "auto":
+        limit = _lowrank.rank_limit(n, len(primes))
+        if limit:
+            residues = _lowrank.low_rank_residues(rows, primes, limit, required=False)
+    if residues is not None:
         if verbose:
-            print("Using exact low-rank coefficient computation (serial residues)")
+            print("Using exact low-rank coefficient computation (host residues)")
     else:
+        if n >= 64:
+            raise ValueError("matrix dimension must be less than 64")
         residues = _device_residues(rows, primes)
     result = chinese_remainder_theorem(residues, primes)
     return _planner.signed_lift(result, bound, prime_product)
