@@ -70,13 +70,26 @@ int phip_affine_permanent_residues(const int64_t *C, const int64_t *L, int64_t p
 
 /* Host preprocessing (no device work): a perfect matching of the 0/1 matrix
  * given as n <= 63 row masks (bit j = column j) and the irreducible blocks
- * of its matching digraph, in the order and local column order of
- * perm_binary.py's _matching_blocks (perm_binary.py:709-752).  Returns 1
- * with *block_count blocks (block_sizes[b] rows each; block_masks holds
- * their rows' local masks, block after block, n words in all), 0 when no
- * perfect matching exists, -1 on invalid input. */
+ * of its matching digraph (the strongly connected components), which
+ * replaces _matching_blocks (perm_binary.py:709-752).  Blocks are numbered
+ * by their smallest row; a block's rows and columns keep ascending order,
+ * which fixes the local column labels.  Returns 1 with *block_count blocks
+ * (block_sizes[b] rows each; block_masks holds their rows' local masks,
+ * block after block, n words in all), 0 when no perfect matching exists,
+ * -1 on invalid input. */
 int phip_binary_blocks(const uint64_t *masks, int n, int *block_count, int *block_sizes,
                        uint64_t *block_masks);
+
+/* The same decomposition for a 0/1 matrix of any order in CSR form (row r's
+ * columns are col_idx[row_ptr[r] .. row_ptr[r+1]), no repeats): a perfect
+ * matching (row_match[r] = its column; row_match may be null), and for
+ * every row and column the block it belongs to (blocks numbered by their
+ * smallest row).  Returns 1, 0 when no perfect matching exists, -1 on
+ * invalid input.  Host code only.  Replaces _matching_blocks and
+ * _perfect_matching (perm_binary.py:609-655, 709-752) for orders beyond
+ * the 64-bit masks of phip_binary_blocks. */
+int phip_binary_blocks_csr(const int32_t *row_ptr, const int32_t *col_idx, int n, int32_t *row_match,
+                           int32_t *row_block, int32_t *col_block, int *block_count);
 
 /* Live-column frontier DP of a 0/1 matrix (perm_binary.py:389-441) on the
  * device, counts modulo k <= 64 odd moduli below 2^31.  The reference's DP
@@ -94,12 +107,14 @@ int phip_binary_frontier_residues(const uint64_t *row_masks, const uint64_t *exp
                                   const int64_t *primes, int k, int64_t max_states, int64_t *out,
                                   int64_t *info);
 
-/* Greedy processing order of a 0/1 matrix's rows for the frontier DP
- * (perm_binary.py:282-386), replacing _frontier_greedy_order.  The matrix
- * is in CSR form: row r's columns are col_idx[row_ptr[r] .. row_ptr[r+1])
- * (distinct, in [0, n), any order).  order_out receives n row indices,
- * *max_width the largest live-column frontier and *area the sum of the
- * widths.  Host code only.  Returns 0 or -1. */
+/* Processing order of a 0/1 matrix's rows for the frontier DP, in place of
+ * _frontier_greedy_order (perm_binary.py:282-386): Cuthill-McKee level
+ * structures refined by sifting, the order with the least (maximum width,
+ * frontier area) among those tried.  The matrix is in CSR form: row r's
+ * columns are col_idx[row_ptr[r] .. row_ptr[r+1]) (distinct, in [0, n),
+ * any order).  order_out receives n row indices, *max_width the largest
+ * live-column frontier and *area the sum of the widths.  Host code only.
+ * Returns 0 or -1. */
 int phip_frontier_order(const int32_t *row_ptr, const int32_t *col_idx, int n, int32_t *order_out, int *max_width,
                         int64_t *area);
 

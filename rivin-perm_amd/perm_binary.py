@@ -6,16 +6,24 @@ return values.  Each matching-induced irreducible block (perm_binary.py:709-752)
 is sent to the device with all of its CRT primes in one launch; bounds, prime
 plans and CRT stay on the host in exact integers.
 
-Backends: 'auto' and 'native' both select the device (``_binary_ext`` is the
-GPU object shaped like the reference's native extension).  As in the
-reference, 'auto' first tries the live-column frontier DP on sparse blocks
-of order >= 20 with a narrow frontier (perm_binary.py:475-516, 975-990) and
-falls back to Glynn when a layer exceeds the state cap; the frontier DP
-runs on the device too (phip_binary_frontier_residues), with counts kept
-modulo the same descending CRT primes.  The reference's portable Python
-subset DP is not part of this build (backend='python' raises), so blocks of
-order 64 or more that the frontier does not take are rejected, as the
-reference's native backend does.
+Backends: every backend name computes on the device (``_binary_ext`` is the
+GPU object shaped like the reference's native extension); the names keep
+the reference's block rules (perm_binary.py:893-1005):
+
+* 'auto' first tries the live-column frontier DP on sparse blocks of order
+  >= 20 with a narrow frontier (perm_binary.py:475-516, 975-990) under the
+  auto state cap, then Glynn for blocks below order 64 and the uncapped DP
+  for larger ones;
+* 'native' runs Glynn and refuses blocks of order 64 or more;
+* 'python' (the reference's portable subset DP, perm_binary.py:230-265)
+  runs Glynn below order 64 and the uncapped DP above it.
+
+The DP runs on the device too (phip_binary_frontier_residues), with counts
+kept modulo the same descending CRT primes.  For the large blocks it plays
+the part of the reference's subset DP, which tracks the set of used
+columns; the frontier DP tracks only the live ones and returns the same
+count.  It is limited by its 64 live-column slots and by device memory
+(``DP_MAX_STATES`` states per layer).
 """
 
 from __future__ import annotations
@@ -38,7 +46,7 @@ def get_binary_backend() -> str:
 
 
 def available_binary_backends() -> Tuple[str, ...]:
-    return ("native",)
+    return ("native", "python")
 
 
 def _select_backend(backend) -> str:
@@ -48,8 +56,24 @@ def _select_backend(backend) -> str:
     if backend in ("auto", "native"):
         return "native"
     if backend == "python":
-        raise ValueError("the portable Python backend is not part of this build; use backend='auto'")
+        return "python"
     raise ValueError("backend must be one of: 'auto', 'native', 'python'")
+
+
+# Layer limit of the uncapped DP for blocks of order 64 or more: a layer
+# holds 8 + 4k bytes per state for k CRT primes, so 2^27 states stay within
+# a few GiB of device memory.
+DP_MAX_STATES = 1 << 27
+
+
+def _large_block_permanent(block) -> int:
+    """Exact count of a block of order >= 64 by the frontier DP in the
+    library's row order, as the reference's subset DP would give it."""
+    order, _, _ = _prep.frontier_order(block)
+    try:
+        return _frontier_permanent_masks(block, order, DP_MAX_STATES)
+    except _FrontierTooWide as exc:
+        raise ValueError(f"block of order {len(block)}: {exc}") from exc
 
 
 class FrontierStateLimitExceeded(RuntimeError):
@@ -70,7 +94,6 @@ class _FrontierTooWide(ValueError):
     """More columns live during one row than the device state mask holds."""
 
 
-_FRONTIER_ORDERING_WORK_BUDGET = _prep.FRONTIER_ORDERING_WORK_BUDGET
 _AUTO_FRONTIER_MIN_ORDER = _prep.AUTO_FRONTIER_MIN_ORDER
 _AUTO_FRONTIER_MAX_AVERAGE_DEGREE = _prep.AUTO_FRONTIER_MAX_AVERAGE_DEGREE
 _AUTO_FRONTIER_MAX_WIDTH = _prep.AUTO_FRONTIER_MAX_WIDTH
@@ -190,7 +213,8 @@ def binary_permanent_mod_p(matrix, p, *, backend: str = "auto", parallel: bool =
     _prep.validate_num_threads(num_threads)
     rows = _prep.normalize_binary_matrix(matrix, strict=strict)
     modulus = _prep.validate_residue_prime(p, len(rows))
-    _select_backend(backend)
+    selected = _select_backend(backend)
+    explicit_native = selected == "native" and backend.lower() == "native"
     if not rows:
         return 1 % modulus
     if len(rows) == 1:
@@ -203,8 +227,11 @@ def binary_permanent_mod_p(matrix, p, *, backend: str = "auto", parallel: bool =
         if len(block) == 1:
             continue
         if len(block) >= 64:
-            raise ValueError("native binary-permanent blocks must have order less than 64")
-        residue = _checked_residues(_block_array(block), [modulus])[0]
+            if explicit_native:
+                raise ValueError("native binary-permanent blocks must have order less than 64")
+            residue = _large_block_permanent(block) % modulus
+        else:
+            residue = _checked_residues(_block_array(block), [modulus])[0]
         result = result * residue % modulus
     return result
 
@@ -217,8 +244,9 @@ def binary_permanent(matrix, *, backend: str = "auto", parallel: bool = True,
     num_threads = _prep.validate_num_threads(num_threads)
     threads = num_threads if parallel else 1
     rows = _prep.normalize_binary_matrix(matrix, strict=strict)
-    _select_backend(backend)
+    selected = _select_backend(backend)
     automatic = backend.lower() == "auto"
+    explicit_native = selected == "native" and not automatic
     if not rows:
         return 1
     if len(rows) == 1:
@@ -239,7 +267,10 @@ def binary_permanent(matrix, *, backend: str = "auto", parallel: bool = True,
                 except (FrontierStateLimitExceeded, _FrontierTooWide):
                     pass
         if len(block) >= 64:
-            raise ValueError("native binary-permanent blocks must have order less than 64")
+            if explicit_native:
+                raise ValueError("native binary-permanent blocks must have order less than 64")
+            result *= _large_block_permanent(block)
+            continue
         block_matrix = _unpack_masks(block)
         result *= _native_exact_permanent(block_matrix, _normalized_bound(block_matrix), threads)
     return result
@@ -248,7 +279,7 @@ def binary_permanent(matrix, *, backend: str = "auto", parallel: bool = True,
 # Private helper names of the reference module (perm_binary.py:173-227,
 # 538-814), used by perm_binary_cuda as they are in the reference.
 _crt_primes_above_bound = _prep.crt_primes_above_bound
-_frontier_greedy_order = _prep.frontier_greedy_order
+_frontier_greedy_order = _prep.frontier_order
 _is_prime = _prep.is_prime_u64
 _matching_blocks = _prep.matching_blocks
 _normalize_binary_matrix = _prep.normalize_binary_matrix

@@ -58,6 +58,10 @@ _SIGNATURES = {
         [ctypes.POINTER(ctypes.c_uint64), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint64)],
     ),
+    "phip_binary_blocks_csr": (
+        ctypes.c_int,
+        [_I32P, _I32P, ctypes.c_int, _I32P, _I32P, _I32P, ctypes.POINTER(ctypes.c_int)],
+    ),
     "phip_affine_permanent_residues": (
         ctypes.c_int,
         [_I64P, _I64P, ctypes.c_int64, ctypes.c_int, _I64P, ctypes.c_int, _I64P],
@@ -260,10 +264,32 @@ def binary_blocks(masks):
     return True, tuple(blocks)
 
 
+def binary_blocks_csr(row_ptr: np.ndarray, col_idx: np.ndarray, n: int):
+    """(row matching, row block ids, column block ids, block count) of a 0/1
+    matrix of any order in CSR form, or None without a perfect matching:
+    host code in the library, no device needed."""
+    lib = load()
+    ptr = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    idx = np.ascontiguousarray(col_idx, dtype=np.int32)
+    if ptr.shape != (n + 1,):
+        raise ValueError("row_ptr must hold n + 1 offsets")
+    match = np.zeros(max(1, n), dtype=np.int32)
+    rows = np.zeros(max(1, n), dtype=np.int32)
+    cols = np.zeros(max(1, n), dtype=np.int32)
+    count = ctypes.c_int()
+    status = lib.phip_binary_blocks_csr(ptr.ctypes.data_as(_I32P), idx.ctypes.data_as(_I32P) if idx.size else None,
+                                        int(n), match.ctypes.data_as(_I32P), rows.ctypes.data_as(_I32P),
+                                        cols.ctypes.data_as(_I32P), ctypes.byref(count))
+    _check(status, "phip_binary_blocks_csr")
+    if status == 0:
+        return None
+    return match[:n].tolist(), rows[:n].tolist(), cols[:n].tolist(), int(count.value)
+
+
 def frontier_order(columns):
-    """(row order, maximum frontier width, frontier area) of the greedy
-    ordering for the frontier DP, for a 0/1 matrix given as the column
-    lists of its rows: host code in the library."""
+    """(row order, maximum frontier width, frontier area) of the frontier
+    DP's row order for a 0/1 matrix given as the column lists of its rows:
+    host code in the library."""
     lib = load()
     n = len(columns)
     ptr = np.zeros(n + 1, dtype=np.int32)

@@ -72,7 +72,7 @@ def test_frontier_matches_oracle_and_glynn_on_sparse_blocks(native, n, density):
     checked = 0
     for _ in range(4):
         masks = _random_masks(rng, n, density)
-        order, _, _ = prep.frontier_greedy_order(masks)
+        order, _, _ = prep.frontier_order(masks)
         try:
             expected = oracle.frontier_count(masks, order, max_states=200_000)
         except oracle.FrontierCapExceeded:
@@ -96,7 +96,7 @@ def test_frontier_layer_sizes_match_the_oracle_through_the_cap(native):
     rng = random.Random(4242)
     for n in (12, 24, 36):
         masks = _random_masks(rng, n, 0.12)
-        order, _, _ = prep.frontier_greedy_order(masks)
+        order, _, _ = prep.frontier_order(masks)
         sizes = []
         expected = oracle.frontier_count(masks, order, layer_sizes=sizes)
         matrix = prep.unpack_masks(masks)
@@ -328,3 +328,44 @@ def test_frontier_entry_point_edges(native):
     with pytest.raises(RuntimeError, match="odd"):
         native.binary_frontier_residues([1], [1], 1, [2147483646], 10)
     assert native.binary_frontier_residues([0b1, 0b0], [0b1, 0b0], 1, [2147483647], 10) == ([0], None)
+
+
+def _band(n, half_width):
+    return [[int(abs(r - c) <= half_width) for c in range(n)] for r in range(n)]
+
+
+def test_dense_large_blocks_take_the_uncapped_dp(native):
+    """A band of half-width 3 and order 70 has about 7 entries a row, so
+    auto does not try the capped frontier; the block (order >= 64) goes to
+    the uncapped DP, the part the reference's subset DP plays
+    (perm_binary.py:230-265, 997-1003).  Checked against the oracle's
+    Python frontier DP; backend='python' gives the same, explicit native
+    refuses the block."""
+    import perm_binary
+    from permhip import binary_prep as prep
+
+    band = _band(70, 3)
+    masks = prep.row_masks(band)
+    assert prep.auto_frontier_plan(masks) is None
+    expect = oracle.frontier_count(masks, range(70), max_states=10 ** 7)
+    assert perm_binary.binary_permanent(band) == expect
+    assert perm_binary.binary_permanent(band, backend="python") == expect
+    for p in (2147483647, (1 << 61) - 1):
+        assert perm_binary.binary_permanent_mod_p(band, p) == expect % p
+        assert perm_binary.binary_permanent_mod_p(band, p, backend="python") == expect % p
+    with pytest.raises(ValueError, match="less than 64"):
+        perm_binary.binary_permanent(band, backend="native")
+    with pytest.raises(ValueError, match="less than 64"):
+        perm_binary.binary_permanent_mod_p(band, 2147483647, backend="native")
+
+
+def test_python_backend_name_matches_the_definition(native):
+    import perm_binary
+
+    rng = random.Random(31)
+    for n in range(0, 13):
+        B = [[int(rng.random() < 0.5) for _ in range(n)] for _ in range(n)]
+        d = oracle.subset_permanent(B) if n else 1
+        assert perm_binary.binary_permanent(B, backend="python") == d
+        if n:
+            assert perm_binary.binary_permanent_mod_p(B, 1000003, backend="python") == d % 1000003

@@ -1,12 +1,19 @@
-"""Host preprocessing in the library (phip_binary_blocks, no device): the
-perfect matching and irreducible blocks must equal the Python restatement
-of perm_binary.py:709-752 block for block, in the same order; and the cached
-prime sequences must equal a fresh scan."""
+"""Host preprocessing in the library (no device): the perfect matching and
+the irreducible blocks against a brute-force decomposition, and the cached
+prime sequences against a fresh scan.
+
+The brute force is independent of the library: an entry (r, c) lies on a
+perfect matching exactly when the matrix without row r and column c has
+one (scipy's maximum_bipartite_matching decides it), and the irreducible
+blocks are the connected components of the bipartite graph of those
+entries (Dulmage-Mendelsohn fine decomposition)."""
 
 import random
 
 import numpy as np
 import pytest
+from scipy.sparse import csr_matrix
+from scipy.sparse.csgraph import connected_components, maximum_bipartite_matching
 
 from permhip import _native, binary_prep, planner
 
@@ -15,18 +22,134 @@ def _random_masks(rng, n, density):
     return [sum(1 << j for j in range(n) if rng.random() < density) for _ in range(n)]
 
 
-@pytest.mark.parametrize("n", [0, 1, 2, 3, 5, 8, 13, 21, 34, 47, 63])
-def test_native_blocks_equal_python(n):
+def _has_perfect_matching(A):
+    n = A.shape[0]
+    if n == 0:
+        return True
+    m = maximum_bipartite_matching(csr_matrix(A), perm_type="column")
+    return bool((m >= 0).all())
+
+
+def _brute_blocks(A):
+    """None without a perfect matching, else the set of blocks, each a
+    (frozenset of rows, frozenset of columns)."""
+    n = A.shape[0]
+    if not _has_perfect_matching(A):
+        return None
+    allowed = []
+    for r in range(n):
+        for c in range(n):
+            if A[r, c]:
+                sub = np.delete(np.delete(A, r, axis=0), c, axis=1)
+                if _has_perfect_matching(sub):
+                    allowed.append((r, c))
+    # Nodes: rows 0..n-1, columns n..2n-1.
+    data = np.ones(len(allowed))
+    rows = [r for r, _ in allowed]
+    cols = [n + c for _, c in allowed]
+    graph = csr_matrix((data, (rows, cols)), shape=(2 * n, 2 * n))
+    _, label = connected_components(graph, directed=False)
+    out = {}
+    for v in range(2 * n):
+        out.setdefault(label[v], (set(), set()))[0 if v < n else 1].add(v % n)
+    return {(frozenset(r), frozenset(c)) for r, c in out.values()}
+
+
+def _library_blocks(masks):
+    n = len(masks)
+    ptr, idx = binary_prep._csr(masks)
+    got = _native.binary_blocks_csr(ptr, idx, n)
+    if got is None:
+        return None, None
+    match, row_block, col_block, count = got
+    blocks = {}
+    for r in range(n):
+        blocks.setdefault(row_block[r], (set(), set()))[0].add(r)
+    for c in range(n):
+        blocks.setdefault(col_block[c], (set(), set()))[1].add(c)
+    assert sorted(blocks) == list(range(count))
+    return match, {(frozenset(r), frozenset(c)) for r, c in blocks.values()}
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 5, 8, 13, 21, 34])
+def test_blocks_equal_brute_force(n):
     rng = random.Random(900 + n)
     for density in (0.05, 0.15, 0.3, 0.6, 1.0):
-        for _ in range(20):
+        for _ in range(6):
             masks = _random_masks(rng, n, density)
-            if rng.random() < 0.5 and n:
+            if rng.random() < 0.6 and n:
                 # Force a perfect matching through a random permutation.
                 perm = list(range(n))
                 rng.shuffle(perm)
                 masks = [m | (1 << perm[i]) for i, m in enumerate(masks)]
-            assert binary_prep.matching_blocks(masks) == binary_prep.matching_blocks_reference(masks), (n, density)
+            A = np.array([[(m >> j) & 1 for j in range(n)] for m in masks], dtype=np.int8).reshape(n, n)
+            expect = _brute_blocks(A)
+            match, got = _library_blocks(masks)
+            assert got == expect, (n, density)
+            feasible, mask_blocks = binary_prep.matching_blocks(masks)
+            assert feasible == (expect is not None)
+            if expect is None:
+                assert binary_prep.perfect_matching(masks) is None
+                continue
+            # A valid perfect matching.
+            assert sorted(match) == list(range(n)) and all(A[r, c] for r, c in enumerate(match))
+            assert binary_prep.perfect_matching(masks) == tuple(match)
+            # The mask entry point: blocks by smallest row, local columns in
+            # ascending order, each block the submatrix on its rows/columns.
+            ordered = sorted(expect, key=lambda b: min(b[0]))
+            assert len(mask_blocks) == len(ordered)
+            for (rows, cols), local in zip(ordered, mask_blocks):
+                cs = sorted(cols)
+                want = tuple(sum(1 << k for k, c in enumerate(cs) if A[r, c]) for r in sorted(rows))
+                assert local == want
+
+
+@pytest.mark.parametrize("n", [47, 63, 64, 90, 200])
+def test_large_orders_and_both_entry_points(n):
+    """Mask and CSR entry points agree up to order 63; above it the CSR
+    path is checked against a planted block structure."""
+    rng = random.Random(n)
+    sizes = []
+    left = n
+    while left:
+        s = min(left, rng.randint(1, 12))
+        sizes.append(s)
+        left -= s
+    perm_r = list(range(n))
+    perm_c = list(range(n))
+    rng.shuffle(perm_r)
+    rng.shuffle(perm_c)
+    A = np.zeros((n, n), dtype=np.int8)
+    start = 0
+    for s in sizes:
+        # A full s x s square is fully indecomposable; entries above the
+        # diagonal blocks keep the structure block triangular.
+        A[start:start + s, start:start + s] = 1
+        A[start:start + s, start + s:] = np.array([[rng.random() < 0.1 for _ in range(n - start - s)]
+                                                 for _ in range(s)], dtype=np.int8).reshape(s, n - start - s)
+        start += s
+    B = A[np.ix_(perm_r, perm_c)]
+    masks = [int(sum(1 << j for j in range(n) if B[i, j])) for i in range(n)]
+    feasible, blocks = binary_prep.matching_blocks(masks)
+    assert feasible
+    assert sorted(len(b) for b in blocks) == sorted(sizes)
+    assert all(all(m == (1 << len(b)) - 1 for m in b) for b in blocks)
+    _, got = _library_blocks(masks)
+    inv_r = {r: i for i, r in enumerate(perm_r)}
+    inv_c = {c: i for i, c in enumerate(perm_c)}
+    expect, start = set(), 0
+    for s in sizes:
+        rows = frozenset(i for i in range(n) if start <= perm_r[i] < start + s)
+        cols = frozenset(j for j in range(n) if start <= perm_c[j] < start + s)
+        expect.add((rows, cols))
+        start += s
+    assert got == expect
+    assert inv_r and inv_c
+    # Removing one entry of a 1x1 block (if any) or any diagonal square
+    # entry count keeps it consistent; a zero row has no matching.
+    masks[0] = 0
+    assert binary_prep.matching_blocks(masks) == (False, ())
+    assert binary_prep.perfect_matching(masks) is None
 
 
 def test_block_structure_invariants():
@@ -50,6 +173,13 @@ def test_rejections():
     with pytest.raises(RuntimeError, match="beyond the order"):
         _native.binary_blocks([0b100, 0b1])
     assert _native.binary_blocks([0b01, 0b01]) == (False, ())
+    ptr = np.array([0, 1, 2], dtype=np.int32)
+    with pytest.raises(RuntimeError, match="outside the matrix"):
+        _native.binary_blocks_csr(ptr, np.array([0, 2], dtype=np.int32), 2)
+    with pytest.raises(RuntimeError, match="repeated column"):
+        _native.binary_blocks_csr(np.array([0, 2, 3], dtype=np.int32), np.array([1, 1, 0], dtype=np.int32), 2)
+    assert _native.binary_blocks_csr(ptr, np.array([0, 0], dtype=np.int32), 2) is None
+    assert _native.binary_blocks_csr(np.zeros(1, dtype=np.int32), np.zeros(0, dtype=np.int32), 0) == ([], [], [], 0)
 
 
 def test_cached_prime_sequences_match_scans():
