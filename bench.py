@@ -9,13 +9,19 @@ timed region); a step replays the captured launch and reconstructs the
 integer.
 
 Printed as one JSON line (rank 0 only; this build runs on one GPU):
-  value        algorithmic Gray-code mulmods per second, n 2^(n-1) per
-               residue (the reference's n-factor row product per state)
-  roofline     the dominant kernel against the VALU issue bound of its own
-               instruction mix (a synthetic kernel issuing the same
-               per-state instruction multiset, measured live)
+  value        reference-equivalent Gray-code mulmods per second: n 2^(n-1)
+               per residue, the reference's n-factor row product per state
+               (perm_c_omp.c:40-47).  The kernel executes fewer modular
+               reductions than that (exact 16-bit row sums, exact group
+               products shared by the prime set); `executed` reports the
+               Montgomery reductions it does run per state and per second
+  roofline     the dominant kernel's executed VALU lane operations per
+               second (rocprof SQ_INSTS_VALU per launch x 64 lanes, over
+               the kernel time measured live with HIP events) against the
+               nominal INT32 VALU peak, 256 CUs x 64 lanes x 2.4 GHz
   cpu_baseline the reference's own OpenMP kernel (perm_c_omp.c, compiled
-               from its sources into oracle/_ref) on a bounded sample
+               from its sources into oracle/_ref) on every host core: one
+               full residue of the headline matrix, scaled by the prime count
   plain / batch / serve   the per-matrix plain launch, the 4096-matrix
                uneven batch in one persistent launch, and the served stream
 """
@@ -99,33 +105,46 @@ def instruction_mix(n: int, group: int, k: int = 1, paired: bool = False) -> dic
     return mix
 
 
-def roofline(n: int, group: int, k: int, mulmods_per_s: float, paired: bool = False) -> dict:
-    """The dominant kernel against the issue bound of its own instruction
-    stream: a synthetic kernel issues the same per-state instruction
-    multiset (8 states per iteration, every type on independent chains, no
-    memory, no branches) at full occupancy, and its state rate times the
-    k n algorithmic mulmods per state is the peak.  HBM traffic is a few MB
-    per launch, so the path is VALU-issue bound."""
-    from permhip import _native
+# Nominal INT32 VALU peak of one MI355X: 256 CUs, 4 SIMDs of 16 lanes each
+# completing one 32-bit operation per lane per cycle, 2.4 GHz peak clock.
+NOMINAL_INT32_LANE_OPS = 256 * 64 * 2.4e9
 
-    mix = instruction_mix(n, group, k, paired)
-    rounds = [round(mix[t]) for t in _native.MIX_TYPES]  # 8 states x count / 8 chains
-    # Best of five timed launches: the bound is the mix's top issue rate.
-    states_per_s = 8.0 * max(_native.mix_microbench(rounds, iters=512) for _ in range(5))
-    peak = states_per_s * k * n
-    return {
+
+def roofline(kernel_ms: float, states_per_launch: float, compute_units: int, rates: dict, mix: dict) -> dict:
+    """Executed VALU lane operations per second of the dominant kernel
+    against the nominal INT32 peak.  The instruction count per launch is
+    rocprof's SQ_INSTS_VALU from the committed counter profile of this
+    kernel at this shape (profiles/glynn_headline_n32_pmc.txt); the time is
+    this run's HIP-event kernel time.  The fastest 32-bit instruction
+    measured here is reported beside the nominal peak: adds and shifts
+    issue faster than one per lane per cycle on gfx950, so a stream rich in
+    them can pass the nominal figure."""
+    insts = _profile_value("SQ_INSTS_VALU")
+    nominal = compute_units * 64 * 2.4e9
+    fastest = max(rates.get("v_add_u32", 0.0), rates.get("v_ashrrev_i32", 0.0))
+    out = {
         "bound": "valu",
-        "achieved": mulmods_per_s / 1e12,
-        "peak": peak / 1e12,
-        "unit": "Tmulmod/s",
-        "frac": mulmods_per_s / peak,
+        "achieved": None,
+        "peak": nominal / 1e12,
+        "unit": "Tlane-op/s (INT32 VALU)",
+        "frac": None,
         "traffic": traffic_from_profiles(),
-        "group": group,
-        "prime_set": k,
-        "paired_states": paired,
-        "instructions_per_state": mix,
-        "mix_rounds_per_8_states": dict(zip(_native.MIX_TYPES, rounds)),
+        "kernel_ms": kernel_ms,
+        "peak_measured_fastest_int32": fastest / 1e12,
+        "valu_insts_per_state_model": sum(mix.values()),
+        "instructions_per_state_model": mix,
+        "source": "profiles/glynn_headline_n32_pmc.txt (SQ_INSTS_VALU per launch)",
     }
+    if insts is not None:
+        achieved = insts * 64 / (kernel_ms * 1e-3)
+        out.update({
+            "achieved": achieved / 1e12,
+            "frac": achieved / nominal,
+            "frac_of_measured_fastest": achieved / fastest if fastest else None,
+            "valu_wave_insts_per_launch": insts,
+            "valu_insts_per_state_measured": insts * 64 / states_per_launch,
+        })
+    return out
 
 
 def _profile_value(key: str):
@@ -147,33 +166,6 @@ def traffic_from_profiles():
     """HBM bytes per launch of the headline kernel (FETCH_SIZE + WRITE_SIZE,
     copy-calibrated), else None."""
     return _profile_value("hbm_bytes_per_launch")
-
-
-def valu_int32_utilisation(kernel_ms: float, compute_units: int, rates: dict) -> dict:
-    """Achieved VALU lane operations per second of the headline kernel
-    (SQ_INSTS_VALU per launch from the committed counter profile, 64 lanes
-    per wavefront instruction, over this run's kernel time) against two
-    INT32 peaks of the chip: the nominal one (4 SIMDs per CU, each completing
-    a 64-lane instruction every 4 cycles at the 2.4 GHz peak engine clock)
-    and the fastest 32-bit VALU instruction measured here (v_add_u32 and
-    v_ashrrev_i32 issue above the nominal rate on gfx950).  The kernel's
-    stream is mostly 64-bit and 32-bit multiplies, which issue at about half
-    the add rate, so its tight bound is the instruction-mix roofline."""
-    insts = _profile_value("SQ_INSTS_VALU")
-    if insts is None:
-        return None
-    achieved = insts * 64 / (kernel_ms * 1e-3)
-    nominal = compute_units * 4 * 16 * 2.4e9
-    measured = max(rates.get("v_add_u32", 0.0), rates.get("v_ashrrev_i32", 0.0))
-    return {
-        "achieved_lane_ops_per_s": achieved,
-        "peak_nominal_lane_ops_per_s": nominal,
-        "frac_of_nominal": achieved / nominal,
-        "peak_measured_int32_lane_ops_per_s": measured,
-        "frac_of_measured": achieved / measured if measured else None,
-        "valu_wave_instructions_per_launch": insts,
-        "source": "profiles/glynn_headline_n32_pmc.txt (SQ_INSTS_VALU)",
-    }
 
 
 def binary36_leg() -> dict:
@@ -253,39 +245,59 @@ def binary36_cpu_baseline(threads: int) -> dict:
     return out
 
 
+def host_cpu() -> dict:
+    """Host cores this process may use and the CPU model (/proc/cpuinfo)."""
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cores": cores, "model": model, "os_cpu_count": os.cpu_count()}
+
+
 def cpu_baseline(matrix: np.ndarray, primes, threads: int) -> dict:
-    """Reference OpenMP Glynn (perm_c_ext.permanent_mod_p) on the leading
-    30 x 30 block of the headline matrix for one prime: 2^29 states."""
+    """The reference's OpenMP Glynn (perm_c_ext.permanent_mod_p, compiled
+    from perm_c_omp.c) for one full residue of the headline matrix: 32 x 32,
+    the first CRT prime, 2^31 Gray states, on every host core.  The exact
+    permanent needs the same walk for each of the primes, so its CPU time is
+    this residue's time times the prime count."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle  # checker/baseline only
 
     exts = oracle.reference_extensions()
-    sub = 30
     p = primes[0]
-    block = [[int(v) % p for v in row[:sub]] for row in matrix[:sub]]
+    reduced = [[int(v) % p for v in row] for row in matrix]
     if "perm_c_ext" in exts:
         ext = exts["perm_c_ext"]
         kind = "reference"
-        fn = lambda: ext.permanent_mod_p(block, p, threads)  # noqa: E731
+        fn = lambda: ext.permanent_mod_p(reduced, p, threads)  # noqa: E731
     else:
         kind = "port"
-        fn = lambda: oracle.C.permanent_mod_p(block, p, threads)  # noqa: E731
+        fn = lambda: oracle.C.permanent_mod_p(reduced, p, threads)  # noqa: E731
     t0 = time.perf_counter()
-    fn()
+    residue = fn()
     dt = time.perf_counter() - t0
-    mulmods = sub * 2.0 ** (sub - 1)
-    rate = mulmods / dt
-    full = HEADLINE_N * 2.0 ** (HEADLINE_N - 1) * len(primes)
+    n = len(reduced)
+    mulmods = n * 2.0 ** (n - 1)
+    cpu = host_cpu()
     return {
-        "value": rate,
+        "value": mulmods / dt,
         "unit": "mulmod/s",
         "cores": threads,
         "kind": kind,
         "sample": (
-            f"perm_c_ext.permanent_mod_p on the leading {sub}x{sub} block of the headline matrix, "
-            f"one prime ({p}), 2^{sub - 1} Glynn states, {dt:.2f} s"
+            f"perm_c_ext.permanent_mod_p on the {n}x{n} headline matrix modulo its first CRT prime ({p}): "
+            f"one full residue, 2^{n - 1} Glynn states, {dt:.2f} s on {threads} threads"
         ),
-        "extrapolated_exact_n32_s": full / rate,
+        "residue": int(residue),
+        "residue_s": dt,
+        "exact_n32_s": dt * len(primes),
+        "cpu_model": cpu["model"],
+        "host_cores": cpu["cores"],
     }
 
 
@@ -470,7 +482,7 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
     ap.add_argument("--no-published", action="store_true",
                     help="skip the workloads the reference publishes timings for")
-    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=200.0, help="mean Poisson arrival rate (req/s) of the served leg")
     ap.add_argument("--kernel-only", action="store_true", help="headline launches only (for profilers)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -638,20 +650,39 @@ def main() -> int:
     log("VALU lane-ops/s: " + ", ".join(f"{k}={v:.3e}" for k, v in rates.items()))
     st = plan.stats()
     group, kset, paired = int(st["group"]), int(st["set"]), int(st.get("variant", 0)) == 1
-    roof = roofline(HEADLINE_N, group, kset, kernel_rate, paired)
-    # Every mulmod as one signed Montgomery product plus one row-sum add: the
-    # ceiling of a kernel that multiplies each row sum in modular arithmetic.
-    roof["one_montgomery_per_mulmod_peak"] = (
-        1.0 / (1.0 / rates["mont31_product"] + 1.0 / rates["v_add_u32"]) / 1e12)
-    roof["instruction_rates"] = rates
+    cus = int(re.search(r"CUs=(\d+)", info).group(1)) if re.search(r"CUs=(\d+)", info) else 256
+    mix = instruction_mix(HEADLINE_N, group, kset, paired)
+    states_per_launch = float(st["states"]) / len(primes)  # one walk serves the whole prime set
+    roof = roofline(k_ms, states_per_launch, cus, rates, mix)
     roof["kernel"] = (
         f"batch_glynn_kernel<{group}, {kset}, {str(paired).lower()}, "
         f"{str(int(st.get('variant', 0)) == 2).lower()}> (n={HEADLINE_N}, {len(primes)} primes in "
         f"{int(st['jobs'])} walk(s) of the Gray code)"
     )
-    roof["kernel_ms"] = k_ms
-    cus = int(re.search(r"CUs=(\d+)", info).group(1)) if re.search(r"CUs=(\d+)", info) else 256
-    roof["valu_int32"] = valu_int32_utilisation(k_ms, cus, rates)
+    roof["instruction_rates"] = rates
+    # What the kernel executes against what `value` counts: one Montgomery
+    # reduction is one v_mul_lo_u32 of the mix.
+    reductions_per_state = mix["v_mul_lo_u32"]
+    executed = {
+        "reference_equivalent_mulmods_per_state": HEADLINE_N * len(primes),
+        "montgomery_reductions_per_state": reductions_per_state,
+        "montgomery_reductions_per_s": reductions_per_state * states_per_launch / (k_ms * 1e-3),
+        "gray_states_per_s": states_per_launch / (k_ms * 1e-3),
+        "kernel_reference_equivalent_mulmods_per_s": kernel_rate,
+    }
+
+    # The drop-in call: perm.matrix_permanent(A) end to end (validation,
+    # bound, primes, a one-matrix plan in the device context's buffers, the
+    # launch, CRT), as a caller of the reference's API sees it.
+    import perm
+
+    drop_in = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        v = perm.matrix_permanent(A.tolist())
+        drop_in.append(time.perf_counter() - t0)
+        if v != exact:
+            raise RuntimeError("perm.matrix_permanent disagrees with the resident plan")
 
     # Plain launch: one synchronous C-ABI call per (matrix, prime) -- host
     # tables, copy, launch, reduce, copy back -- as the reference wrappers
@@ -687,8 +718,11 @@ def main() -> int:
             "primes": len(primes),
             "parallelism": "single GPU",
         },
+        "value_counts": "reference-equivalent mulmods: n 2^(n-1) per residue (perm_c_omp.c:40-47)",
         "perms_per_s": args.steps / total,
         "exact_permanent": str(exact),
+        "executed": executed,
+        "drop_in_call_s": statistics.median(drop_in),
         "plain_launch": {"s_per_perm": plain_s, "perms_per_s": 1.0 / plain_s},
         "roofline": roof,
         "launch_shape": _launch_shape(plan),
@@ -738,6 +772,9 @@ def main() -> int:
             "matrices_per_s": len(mats) / b_s,
             "s_per_batch": b_s,
             "plan_create_s": t_create,
+            "one_shot_matrices_per_s": len(mats) / (t_create + b_s),
+            "one_shot_speedup_vs_plain": t_plain / (t_create + b_s),
+            "one_shot_speedup_vs_plain_all_primes_per_call": t_plain_matrix / (t_create + b_s),
             "mulmod_per_s": st.get("mulmods", 0) / b_s,
             "jobs": st.get("jobs"),
             "items": st.get("items"),
@@ -771,8 +808,10 @@ def main() -> int:
         log(f"published workloads: {result['published_workloads']}")
 
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_cpu()["cores"]
         result["cpu_baseline"] = cpu_baseline(A, primes, threads)
+        if result["cpu_baseline"]["residue"] != exact % primes[0]:
+            raise RuntimeError("the CPU baseline's residue differs from the device's")
         log(f"cpu baseline: {result['cpu_baseline']}")
         if "polynomial" in result:
             result["polynomial"]["cpu_baseline"] = polynomial_cpu_baseline(threads)

@@ -132,6 +132,16 @@ int64_t phip_permanent_modp(const int64_t *A, int n, int64_t p);
  * perm_c_ext.schur_permanent_mod_p. */
 int64_t phip_schur_permanent_modp(int n, int64_t omega, int64_t p);
 
+/* The necklace walk of the Schur path for an odd prime order n, with no
+ * terms formed: *representatives = orbits of sign words of length n - 1
+ * under rotation and complement that the walk's representatives make up
+ * (the walk visits a weight-(n-1)/2 necklace and its complement's necklace
+ * separately; such a pair counts once), *words = sign words they stand for
+ * (2^(n-1) when the walk covers every word once).  Replaces perm_c_ext.schur_necklace_orbit_stats
+ * (perm_c_omp.c:1062-1080, 314-318).  The device walks orders up to 47;
+ * larger prime orders (2^52 words and more) are refused.  0 or -1. */
+int phip_schur_necklace_orbit_stats(int n, uint64_t *representatives, uint64_t *words);
+
 /* Permanent of a 0/1 matrix (row-major bytes) modulo k odd primes p with
  * n < p < 2^63.  A may be NULL only for n == 0.  Entries other than 0/1,
  * composite or too small moduli and n > 63 are rejected with -1 before any
@@ -196,6 +206,16 @@ int phip_batch_set_option(void *plan, int option, int64_t value);
  * resident (register and LDS use; -1 if unknown).  Returns the number of
  * values written or -1. */
 int phip_batch_stats(void *plan, double *out, int len);
+
+/* The kernel class of every residue of a batch, in input order, as
+ * phip_batch_create would plan it: (group * 16 + prime-set size) * 4 +
+ * variant (0 plain, 1 paired states, 2 zero-skipping).  Same description
+ * arguments as phip_batch_create; host code only (no device).  Writes
+ * min(len, residues) entries and returns the residue count, or -1.  No
+ * counterpart in the reference (plan introspection for tests and
+ * profiles). */
+int phip_batch_plan_classes(const int64_t *entries, const int32_t *orders, int count, const int64_t *moduli,
+                            const int32_t *moduli_counts, int32_t *out, int len);
 
 int phip_batch_destroy(void *plan);
 

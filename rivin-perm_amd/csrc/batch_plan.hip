@@ -605,13 +605,14 @@ int run_plan_in_context(DeviceContext* ctx, const std::vector<PlanInput>& in, in
 
 namespace {
 
-int build_plan(BatchPlan& plan, const int64_t* entries, const int32_t* orders, int count, const int64_t* moduli,
-               const int32_t* moduli_counts) {
+// The plan inputs of a batch description (phip_batch_create's arguments).
+int parse_batch(const int64_t* entries, const int32_t* orders, int count, const int64_t* moduli,
+                const int32_t* moduli_counts, std::vector<PlanInput>& in) {
     if (count < 0 || (count > 0 && (!entries || !orders || !moduli || !moduli_counts))) {
         set_error("invalid batch description");
         return -1;
     }
-    std::vector<PlanInput> in;
+    in.clear();
     uint64_t entry_off = 0;
     size_t mod_off = 0;
     for (int m = 0; m < count; ++m) {
@@ -635,6 +636,13 @@ int build_plan(BatchPlan& plan, const int64_t* entries, const int32_t* orders, i
         entry_off += (uint64_t)n * n;
         mod_off += moduli_counts[m];
     }
+    return 0;
+}
+
+int build_plan(BatchPlan& plan, const int64_t* entries, const int32_t* orders, int count, const int64_t* moduli,
+               const int32_t* moduli_counts) {
+    std::vector<PlanInput> in;
+    if (parse_batch(entries, orders, count, moduli, moduli_counts, in) != 0) return -1;
     if (build_plan_host(in, tuning().batch_items_log2, 4, plan.host) != 0) return -1;
     return plan.upload();
 }
@@ -782,6 +790,24 @@ int phip_batch_stats(void* handle, double* out, int len) {
     const int m = std::min<int>(len, (int)(sizeof(v) / sizeof(v[0])));
     for (int i = 0; i < m; ++i) out[i] = v[i];
     return m;
+}
+
+int phip_batch_plan_classes(const int64_t* entries, const int32_t* orders, int count, const int64_t* moduli,
+                            const int32_t* moduli_counts, int32_t* out, int len) {
+    if (!out && len > 0) {
+        phip::set_error("null argument");
+        return -1;
+    }
+    std::vector<phip::PlanInput> in;
+    if (phip::parse_batch(entries, orders, count, moduli, moduli_counts, in) != 0) return -1;
+    phip::PlanHost h;
+    if (phip::build_plan_host(in, phip::tuning().batch_items_log2, 4, h) != 0) return -1;
+    const int m = std::min<int>(len, (int)h.slots.size());
+    for (int i = 0; i < m; ++i) {
+        const phip::BatchJob& jb = h.jobs[h.slots[i].job];
+        out[i] = (jb.group * 16 + jb.set) * 4 + jb.variant;
+    }
+    return (int)h.slots.size();
 }
 
 int phip_batch_destroy(void* handle) {

@@ -256,6 +256,21 @@ int64_t phip_schur_permanent_modp(int n, int64_t omega, int64_t p) {
     return r;
 }
 
+int phip_schur_necklace_orbit_stats(int n, uint64_t* representatives, uint64_t* words) {
+    if (!representatives || !words) {
+        phip::set_error("null argument");
+        return -1;
+    }
+    if (n < 3 || n >= 64 || !(n & 1) || !phip::is_prime_u64((uint64_t)n)) {
+        phip::set_error("n must be an odd prime satisfying 3 <= n < 64");
+        return -1;
+    }
+    DeviceContext* ctx = DeviceContext::current();
+    if (!ctx) return -1;
+    std::lock_guard<std::mutex> lock(ctx->mutex());
+    return phip::necklace_orbit_stats(ctx, n, representatives, words);
+}
+
 int phip_binary_permanent_residues(const uint8_t* A, int n, const int64_t* primes, int k, int64_t* out) {
     if (!validate_binary(A, n, primes, k)) return -1;
     if (k > 0 && !out) {

@@ -132,11 +132,24 @@ struct NeckUnsigned32 {
     __device__ __forceinline__ int64_t fold(int64_t acc) const { return (int64_t)redc((uint64_t)(acc + acc_bias)); }
 };
 
+// Stand-in arithmetic of the counting walk: the walk returns before any
+// term is formed, so these are never executed.
+struct NeckCount {
+    typedef int32_t T;
+    __device__ __forceinline__ T row(int64_t) const { return 0; }
+    __device__ __forceinline__ T mul(T, T) const { return 0; }
+    __device__ __forceinline__ int32_t centred(T v) const { return v; }
+    __device__ __forceinline__ int64_t fold(int64_t) const { return 0; }
+};
+
 // tasks[t] = prefix bits (depth bits, a[1] on top) | period << 56.
 // E[j] = w[(-j) mod L] centred, 0 <= j < 2L, so row r of column c reads
 // E[L - c + r].  block_partials[b] receives the block's sum of lane
-// residues in [0, p).
-template <int L, class Ar>
+// residues in [0, p).  With kCount the walk forms no terms: block_partials[b]
+// receives the number of orbits under rotation and complement the block's
+// representatives make up and block_partials[gridDim.x + b] the number of
+// words they stand for.
+template <int L, class Ar, bool kCount>
 __global__ __launch_bounds__(kNecklaceBlock) void necklace_kernel(const uint64_t* __restrict__ tasks,
                                                                   uint32_t n_tasks, int depth,
                                                                   const int32_t* __restrict__ E, int64_t base_sum,
@@ -213,6 +226,34 @@ __global__ __launch_bounds__(kNecklaceBlock) void necklace_kernel(const uint64_t
         }
         if (__ballot(!done) == 0) break;
 
+        if constexpr (kCount) {
+            if (have) {
+                const int wt = __popcll(w);
+                if (2 * wt < L) {
+                    acc += 1;
+                    total += 2 * period;
+                } else {
+                    // Weight L/2: the walk visits a necklace and the necklace
+                    // of its complement separately, each for its rotations.
+                    // They form one orbit under rotation and complement,
+                    // counted once, at the smaller of the two (numeric order
+                    // of the L-bit words is lexicographic order); a necklace
+                    // whose complement is one of its rotations is its own
+                    // pair.
+                    const uint64_t c = ~w & kMask;
+                    uint64_t least = c;
+                    for (int r = 1; r < L; ++r) {
+                        const uint64_t rot = ((c << r) | (c >> (L - r))) & kMask;
+                        least = rot < least ? rot : least;
+                    }
+                    acc += (w <= least) ? 1 : 0;
+                    total += period;
+                }
+            }
+            have = false;
+            continue;
+        }
+
         // Row sums of the new representative: one uniform pass per column
         // that changed in any lane.
         const uint64_t diff = have ? (prev ^ w) : 0;
@@ -248,11 +289,20 @@ __global__ __launch_bounds__(kNecklaceBlock) void necklace_kernel(const uint64_t
         }
         have = false;
     }
-    total += ar.fold(acc);
-    int64_t r = total % p;
-    if (r < 0) r += p;
-    const uint64_t s = block_reduce_sum_u64<kNecklaceBlock>((uint64_t)r);
-    if (threadIdx.x == 0) block_partials[blockIdx.x] = s;
+    if constexpr (kCount) {
+        const uint64_t reps_sum = block_reduce_sum_u64<kNecklaceBlock>((uint64_t)acc);
+        const uint64_t words_sum = block_reduce_sum_u64<kNecklaceBlock>((uint64_t)total);
+        if (threadIdx.x == 0) {
+            block_partials[blockIdx.x] = reps_sum;
+            block_partials[gridDim.x + blockIdx.x] = words_sum;
+        }
+    } else {
+        total += ar.fold(acc);
+        int64_t r = total % p;
+        if (r < 0) r += p;
+        const uint64_t s = block_reduce_sum_u64<kNecklaceBlock>((uint64_t)r);
+        if (threadIdx.x == 0) block_partials[blockIdx.x] = s;
+    }
 }
 
 // FKM prefix tasks of the given depth in lexicographic order (subtrees with
@@ -295,12 +345,12 @@ template <class Ar>
 using NecklaceKernel = void (*)(const uint64_t*, uint32_t, int, const int32_t*, int64_t, Ar, int64_t, uint32_t*,
                                 uint64_t*);
 
-template <class Ar>
+template <class Ar, bool kCount = false>
 NecklaceKernel<Ar> kernel_for(int L) {
     switch (L) {
 #define PHIP_NECKLACE_CASE(X) \
-    case X: return necklace_kernel<X, Ar>;
-        PHIP_NECKLACE_CASE(4) PHIP_NECKLACE_CASE(6) PHIP_NECKLACE_CASE(10) PHIP_NECKLACE_CASE(12)
+    case X: return necklace_kernel<X, Ar, kCount>;
+        PHIP_NECKLACE_CASE(2) PHIP_NECKLACE_CASE(4) PHIP_NECKLACE_CASE(6) PHIP_NECKLACE_CASE(10) PHIP_NECKLACE_CASE(12)
         PHIP_NECKLACE_CASE(16) PHIP_NECKLACE_CASE(18) PHIP_NECKLACE_CASE(22) PHIP_NECKLACE_CASE(28)
         PHIP_NECKLACE_CASE(30) PHIP_NECKLACE_CASE(36) PHIP_NECKLACE_CASE(40) PHIP_NECKLACE_CASE(42)
         PHIP_NECKLACE_CASE(46)
@@ -334,6 +384,61 @@ bool necklace_supported(int n, uint64_t p) {
            p < (1ull << 32) && kernel_for<NeckSigned31>(n - 1) != nullptr;
 }
 
+namespace {
+
+// FKM task depth for word length L: at least L/2 (the kernel extends a
+// prefix in one step), 14 levels above the leaves where the task list
+// allows (fewer successor steps per representative in the slowest lane of
+// a wave), at most the tuning limit (task list size).
+int necklace_depth(int L) { return std::max(L / 2, std::min(L - 14, tuning().necklace_depth)); }
+
+}  // namespace
+
+int necklace_orbit_stats(DeviceContext* ctx, int n, uint64_t* representatives, uint64_t* words) {
+    if (n < 3 || n > kNecklaceMaxOrder || !is_prime_u64((uint64_t)n) || !(n & 1)) {
+        set_error("n must be an odd prime satisfying 3 <= n <= 47 on the device");
+        return -1;
+    }
+    const int L = n - 1;
+    const NecklaceKernel<NeckCount> kernel = kernel_for<NeckCount, true>(L);
+    if (!kernel) {
+        set_error("internal: no necklace kernel for this order");
+        return -1;
+    }
+    const int depth = necklace_depth(L);
+    std::vector<uint64_t> tasks;
+    collect_tasks(L, depth, tasks);
+    if (tasks.empty() || tasks.size() >= 0xffffffffull) {
+        set_error("internal: necklace task list");
+        return -1;
+    }
+    const int cus = ctx->compute_units() > 0 ? ctx->compute_units() : 256;
+    const unsigned grid = (unsigned)(cus * 2);
+    hipStream_t st = ctx->stream();
+    auto* d_tasks = (uint64_t*)ctx->buffer(25, tasks.size() * 8);
+    auto* d_E = (int32_t*)ctx->buffer(26, 2 * (size_t)L * 4);
+    auto* d_cnt = (uint32_t*)ctx->buffer(27, 16);
+    auto* d_part = (uint64_t*)ctx->buffer(28, (size_t)grid * 16);
+    auto* h_part = (uint64_t*)ctx->host_buffer(5, (size_t)grid * 16);
+    if (!d_tasks || !d_E || !d_cnt || !d_part || !h_part) return -1;
+    const uint32_t zero = 0;
+    PHIP_HIP_TRY(hipMemcpyAsync(d_tasks, tasks.data(), tasks.size() * 8, hipMemcpyHostToDevice, st));
+    PHIP_HIP_TRY(hipMemcpyAsync(d_cnt, &zero, 4, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(kNecklaceBlock), 0, st, d_tasks, (uint32_t)tasks.size(), depth, d_E,
+                       (int64_t)0, NeckCount{}, (int64_t)1, d_cnt, d_part);
+    PHIP_HIP_TRY(hipGetLastError());
+    PHIP_HIP_TRY(hipMemcpyAsync(h_part, d_part, (size_t)grid * 16, hipMemcpyDeviceToHost, st));
+    PHIP_HIP_TRY(hipStreamSynchronize(st));
+    uint64_t reps = 0, covered = 0;
+    for (unsigned b = 0; b < grid; ++b) {
+        reps += h_part[b];
+        covered += h_part[grid + b];
+    }
+    *representatives = reps;
+    *words = covered;
+    return 0;
+}
+
 int necklace_minor_modp(DeviceContext* ctx, int n, uint64_t omega, uint64_t p, int64_t* minor_out) {
     if (!necklace_supported(n, p)) {
         set_error("necklace path needs a prime order n <= 47 and an odd prime p < 2^32");
@@ -359,11 +464,7 @@ int necklace_minor_modp(DeviceContext* ctx, int n, uint64_t omega, uint64_t p, i
     std::vector<int32_t> E(2 * L);
     for (int j = 0; j < 2 * L; ++j) E[j] = (int32_t)first[((-j) % L + L) % L];
 
-    // Task depth: at least L/2 (one-step prefix extension in the kernel),
-    // 14 levels above the leaves where the task list allows (fewer
-    // successor steps per representative in the slowest lane of a wave), at
-    // most the tuning limit (task list size).
-    const int depth = std::max(L / 2, std::min(L - 14, tuning().necklace_depth));
+    const int depth = necklace_depth(L);
     std::vector<uint64_t> tasks;
     collect_tasks(L, depth, tasks);
     if (tasks.empty() || tasks.size() >= 0xffffffffull) {

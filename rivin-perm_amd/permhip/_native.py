@@ -67,6 +67,7 @@ _SIGNATURES = {
         [_I64P, _I64P, ctypes.c_int64, ctypes.c_int, _I64P, ctypes.c_int, _I64P],
     ),
     "phip_schur_permanent_modp": (ctypes.c_int64, [ctypes.c_int, ctypes.c_int64, ctypes.c_int64]),
+    "phip_schur_necklace_orbit_stats": (ctypes.c_int, [ctypes.c_int, _U64P, _U64P]),
     "phip_binary_permanent_residues": (
         ctypes.c_int,
         [_U8P, ctypes.c_int, _I64P, ctypes.c_int, _I64P],
@@ -83,6 +84,7 @@ _SIGNATURES = {
     "phip_batch_query": (ctypes.c_int, [ctypes.c_void_p, _I64P, ctypes.c_int]),
     "phip_batch_set_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64]),
     "phip_batch_stats": (ctypes.c_int, [ctypes.c_void_p, _F64P, ctypes.c_int]),
+    "phip_batch_plan_classes": (ctypes.c_int, [_I64P, _I32P, ctypes.c_int, _I64P, _I32P, _I32P, ctypes.c_int]),
     "phip_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "phip_set_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64]),
     "phip_get_tuning": (ctypes.c_int64, [ctypes.c_int]),
@@ -361,6 +363,16 @@ def schur_permanent_modp(n: int, omega: int, p: int) -> int:
     return int(result)
 
 
+def schur_necklace_orbit_stats(n: int):
+    """(representatives, words) of the device's necklace walk for S_n."""
+    lib = require_device()
+    reps = ctypes.c_uint64()
+    words = ctypes.c_uint64()
+    _check(lib.phip_schur_necklace_orbit_stats(int(n), ctypes.byref(reps), ctypes.byref(words)),
+           "phip_schur_necklace_orbit_stats")
+    return int(reps.value), int(words.value)
+
+
 def binary_permanent_residues(matrix: np.ndarray, primes) -> list:
     lib = require_device()
     a = np.ascontiguousarray(matrix, dtype=np.uint8)
@@ -429,24 +441,41 @@ def copy_calibration(nbytes: int, reps: int = 1) -> None:
     _check(require_device().phip_copy_calibration(int(nbytes), int(reps)), "phip_copy_calibration")
 
 
+def _batch_description(matrices, moduli_lists):
+    mats = [np.ascontiguousarray(m, dtype=np.int64) for m in matrices]
+    if len(mats) != len(moduli_lists):
+        raise ValueError("one modulus list per matrix is required")
+    orders = np.array([m.shape[0] for m in mats], dtype=np.int32)
+    entries = (np.concatenate([m.ravel() for m in mats]) if mats else np.zeros(1, dtype=np.int64)).astype(np.int64)
+    counts = np.array([len(ms) for ms in moduli_lists], dtype=np.int32)
+    flat = [int(q) for ms in moduli_lists for q in ms]
+    moduli = np.array(flat if flat else [0], dtype=np.int64)
+    return mats, entries, orders, counts, moduli, len(flat)
+
+
+def batch_plan_classes(matrices, moduli_lists) -> list:
+    """(group, prime-set size, variant) of the kernel class that would
+    compute each residue of this batch, in input order: host code, no
+    device."""
+    lib = load()
+    mats, entries, orders, counts, moduli, total = _batch_description(matrices, moduli_lists)
+    out = np.zeros(max(1, total), dtype=np.int32)
+    count = lib.phip_batch_plan_classes(entries.ctypes.data_as(_I64P), orders.ctypes.data_as(_I32P), len(mats),
+                                        moduli.ctypes.data_as(_I64P), counts.ctypes.data_as(_I32P),
+                                        out.ctypes.data_as(_I32P), total)
+    _check(count, "phip_batch_plan_classes")
+    return [(int(c) // 64, (int(c) // 4) % 16, int(c) % 4) for c in out[:count]]
+
+
 class Batch:
     """Device-resident batch plan (phip_batch_*)."""
 
     def __init__(self, matrices, moduli_lists):
         lib = require_device()
-        mats = [np.ascontiguousarray(m, dtype=np.int64) for m in matrices]
-        if len(mats) != len(moduli_lists):
-            raise ValueError("one modulus list per matrix is required")
-        orders = np.array([m.shape[0] for m in mats], dtype=np.int32)
-        entries = (
-            np.concatenate([m.ravel() for m in mats]) if mats else np.zeros(1, dtype=np.int64)
-        ).astype(np.int64)
-        counts = np.array([len(ms) for ms in moduli_lists], dtype=np.int32)
-        flat = [int(q) for ms in moduli_lists for q in ms]
-        moduli = np.array(flat if flat else [0], dtype=np.int64)
+        mats, entries, orders, counts, moduli, total = _batch_description(matrices, moduli_lists)
         self._keep = (entries, orders, counts, moduli)
         self.counts = [int(c) for c in counts]
-        self.total = len(flat)
+        self.total = total
         handle = ctypes.c_void_p()
         status = lib.phip_batch_create(
             entries.ctypes.data_as(_I64P),

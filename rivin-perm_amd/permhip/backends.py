@@ -116,6 +116,17 @@ class HipPermanentExtension:
             return 0
         return _native.schur_permanent_modp(n, omega, p)
 
+    def schur_necklace_orbit_stats(self, *args):
+        """(representatives, words) of the necklace walk the Schur path uses
+        for prime order n (perm_c_omp.c:1062-1080): the device runs the walk
+        with no terms formed.  Orders above 47 are refused by the device."""
+        if len(args) != 1:
+            raise TypeError("schur_necklace_orbit_stats() takes exactly one argument")
+        n = _parse_int(args[0], "n")
+        if n < 3 or n >= 64 or n % 2 == 0 or not _is_prime_u64(n):
+            raise ValueError("n must be an odd prime satisfying 3 <= n < 64")
+        return _native.schur_necklace_orbit_stats(n)
+
     def get_max_threads(self) -> int:
         """Parallel width reported to thread-count heuristics: the device's
         compute units, so callers never fan residues out over processes."""

@@ -9,6 +9,8 @@ identified by their generator (permhip.workloads) plus a SHA-256 of the
 matrix bytes, with the moduli and the residues the reference returned.
 
 Usage: python tests/golden/make_golden.py [--threads 8] [--only NAME ...]
+NAME is one of config1, schur31, batch_sample, batch_classes, serve_slice,
+binary36, headline.
 """
 
 from __future__ import annotations
@@ -46,13 +48,33 @@ def descending_primes(bound: int):
     return primes
 
 
+SERVE_SLICE = 200
+
+
+def binary_bound(rows) -> int:
+    """min(n!, product of row sums, product of column sums): an upper bound
+    on the permanent of a 0/1 matrix."""
+    import math
+
+    n = len(rows)
+    return min(math.factorial(n), math.prod(sum(r) for r in rows), math.prod(sum(c) for c in zip(*rows)))
+
+
+def crt(residues, primes) -> int:
+    x, m = 0, 1
+    for r, p in zip(residues, primes):
+        x += m * ((r - x) * pow(m, -1, p) % p)
+        m *= p
+    return x
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--threads", type=int, default=8)
     ap.add_argument("--only", nargs="*", default=None)
     args = ap.parse_args()
     ext = oracle.reference_extensions()
-    if len(ext) != 3:
+    if not {"perm_c_ext", "perm_binary_ext", "perm_mod3_omp"} <= set(ext):
         print("build the reference extensions first: make -C oracle", file=sys.stderr)
         return 1
     data = json.loads(OUT.read_text()) if OUT.exists() else {}
@@ -110,6 +132,66 @@ def main() -> int:
         picks = [i for i, m in enumerate(mats) if m.shape[0] >= 23][:6]
         for i in picks:
             integer_case(f"batch4096[{i}]", mats[i], f"permhip.workloads.batch4096()[{i}]")
+    if want("batch_classes"):
+        # Two matrices of every order 12..28 of the 4096-matrix batch: their
+        # prime counts (4..10) and row bounds cover every (group, prime set,
+        # variant) class the full batch plan uses.
+        mats = workloads.batch4096()
+        for n in range(12, 29):
+            for i in [i for i, m in enumerate(mats) if m.shape[0] == n][:2]:
+                name = f"batch4096[{i}]"
+                if name not in data:
+                    integer_case(name, mats[i], f"permhip.workloads.batch4096()[{i}]")
+        # Then two matrices of every kernel class of the full plan that the
+        # picks above leave uncovered (the plan's classes come from the
+        # library's host planner, no device needed).
+        from permhip import _native
+
+        primes = [planner.generic_crt_primes(planner.matrix_permanent_bound(m.tolist()))[0] for m in mats]
+        classes = _native.batch_plan_classes(mats, primes)
+        of_matrix, pos = [], 0
+        for ps in primes:
+            of_matrix.append(set(classes[pos:pos + len(ps)]))
+            pos += len(ps)
+        covered = set()
+        for key in data:
+            if key.startswith("batch4096["):
+                covered |= of_matrix[int(key[len("batch4096["):-1])]
+        for cls in sorted(set(classes) - covered):
+            for i in [i for i in range(len(mats)) if cls in of_matrix[i]][:2]:
+                integer_case(f"batch4096[{i}]", mats[i], f"permhip.workloads.batch4096()[{i}]")
+    if want("serve_slice"):
+        # The first SERVE_SLICE requests of the served stream: integer
+        # requests up to order 28 and 0/1 requests up to order 30 (the rest
+        # take the reference's CPU path too long to pin here).
+        stream = workloads.serve_stream()[:SERVE_SLICE]
+        out = {}
+        t0 = time.time()
+        for i, (arrival, kind, A) in enumerate(stream):
+            n = A.shape[0]
+            if kind == "integer" and n <= 28:
+                rows = A.tolist()
+                bound = planner.matrix_permanent_bound(rows)
+                primes, product = planner.generic_crt_primes(bound)
+                res = [int(ext["perm_c_ext"].permanent_mod_p([[int(v) % p for v in r] for r in rows], p, T))
+                       for p in primes]
+                exact = planner.signed_lift(planner.chinese_remainder_theorem(res, primes), bound, product)
+            elif kind == "binary" and n <= 30:
+                rows = A.astype(np.int64).tolist()
+                primes = descending_primes(binary_bound(rows))
+                res = [int(ext["perm_binary_ext"].binary_permanent_mod_p(rows, p, T)) for p in primes]
+                exact = crt(res, primes)
+            else:
+                continue
+            out[str(i)] = {"kind": kind, "n": n, "sha256_int64": digest(A), "moduli": primes,
+                           "residues": res, "exact": str(exact)}
+            print(f"  serve[{i}] {kind} n={n} ({time.time() - t0:.0f}s)", flush=True)
+        data["serve_slice"] = {
+            "generator": f"permhip.workloads.serve_stream()[:{SERVE_SLICE}]",
+            "requests": out,
+            "source": "perm_c_ext.permanent_mod_p / perm_binary_ext.binary_permanent_mod_p (reference)",
+        }
+        save()
     if want("binary36"):
         B = workloads.binary36()
         rows = B.tolist()

@@ -168,3 +168,15 @@ def test_perm_binary_ext_object_rejects_like_the_reference(matrix, prime, thread
 
     with pytest.raises(error, match=message):
         perm_binary_ext.binary_permanent_mod_p(matrix, prime, threads)
+
+
+@pytest.mark.parametrize("order", [0, 2, 9, 25, 64])
+def test_necklace_stats_reject_unsupported_orders(order):
+    """perm_c_ext.schur_necklace_orbit_stats refuses orders that are not odd
+    primes in [3, 64) before any device work (perm_c_omp.c:1066-1071)."""
+    from permhip.backends import perm_c_ext
+
+    with pytest.raises(ValueError, match="odd prime"):
+        perm_c_ext.schur_necklace_orbit_stats(order)
+    with pytest.raises(TypeError):
+        perm_c_ext.schur_necklace_orbit_stats(5.0)

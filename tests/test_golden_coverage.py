@@ -1,0 +1,32 @@
+"""The golden fixtures of the 4096-matrix batch cover every kernel class
+(row-sum group, prime-set size, variant) the batch's plan uses, so the GPU
+test that runs the whole plan checks every class against residues written
+by the reference.  Host planning only (no device)."""
+
+import json
+from pathlib import Path
+
+from permhip import _native, planner, workloads
+
+GOLDEN = Path(__file__).resolve().parent / "golden" / "reference_residues.json"
+
+
+def test_batch_golden_covers_every_plan_class():
+    golden = json.loads(GOLDEN.read_text())
+    mats = workloads.batch4096()
+    primes = [planner.generic_crt_primes(planner.matrix_permanent_bound(m.tolist()))[0] for m in mats]
+    classes = _native.batch_plan_classes(mats, primes)
+    assert len(classes) == sum(len(p) for p in primes)
+    covered, pos, starts = set(), 0, []
+    for p in primes:
+        starts.append(pos)
+        pos += len(p)
+    for key in golden:
+        if key.startswith("batch4096["):
+            i = int(key[len("batch4096["):-1])
+            assert golden[key]["moduli"] == primes[i]
+            covered |= set(classes[starts[i]:starts[i] + len(primes[i])])
+    assert set(classes) == covered, sorted(set(classes) - covered)
+    # The classes a 4096-matrix plan of entries U[-100, 100] uses: packed row
+    # sums in groups of 2 or 3, sets of 1, 2, 4 and 8 primes.
+    assert {c[0] for c in classes} <= {2, 3} and {c[1] for c in classes} <= {1, 2, 4, 8}

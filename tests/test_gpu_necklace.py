@@ -95,3 +95,38 @@ def test_reference_device_golden_residues(native, n, omega, p, expected):
     two orders its necklace kernel serves."""
     assert pow(omega, n, p) == 1
     assert native.schur_permanent_modp(n, omega, p) == expected
+
+
+def _rotation_complement_orbits(length):
+    """Orbits of {0,1}^length under rotation and complement, by Burnside's
+    lemma over the dihedral-free group Z_length x Z_2 (rotation by k fixes
+    2^gcd(k, length) words; rotation by k composed with complement fixes
+    2^gcd(k, length) words when length / gcd(k, length) is even, else none)."""
+    import math
+
+    fixed = 0
+    for k in range(length):
+        g = math.gcd(k, length)
+        fixed += 2 ** g
+        if (length // g) % 2 == 0:
+            fixed += 2 ** g
+    return fixed // (2 * length)
+
+
+@pytest.mark.parametrize("n", [3, 5, 7, 11, 13, 17, 19, 23, 29, 31])
+def test_necklace_orbit_stats_cover_every_sign_word(native, n):
+    """perm_c_ext.schur_necklace_orbit_stats (perm_c_omp.c:1062-1080;
+    the reference's test tests/test_schur_cpu_glynn.py:101-130): one
+    representative per orbit, and the orbit weights cover all 2^(n-1)
+    sign words exactly once."""
+    from permhip.backends import perm_c_ext
+
+    reps, words = perm_c_ext.schur_necklace_orbit_stats(n)
+    assert reps == _rotation_complement_orbits(n - 1)
+    assert words == 1 << (n - 1)
+
+
+def test_necklace_orbit_stats_at_order_41(native):
+    reps, words = native.schur_necklace_orbit_stats(41)
+    assert reps == _rotation_complement_orbits(40)
+    assert words == 1 << 40
