@@ -56,6 +56,18 @@ int phip_device_info(char *buf, size_t len);
 int phip_permanent_residues(const int64_t *A, int64_t prime_stride, int n, const int64_t *moduli,
                             int k, int64_t *out);
 
+/* phip_permanent_residues without any device allocation: no counterpart in
+ * the reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results, restricted to the persistent engine's inputs:
+ * orders 5..46, 1..64 odd moduli in [3, 2^31).  The plan's device image
+ * lives in a fixed 8 MiB device array per GPU (held until the process
+ * ends); a plan that does not fit is refused with an error.  Calls on one
+ * GPU share that array and are serialised by the library; callers outside
+ * the library that run it concurrently on one GPU would race.  0 or -1. */
+int phip_permanent_residues_noalloc(const int64_t *A, int64_t prime_stride, int n, const int64_t *moduli,
+                                    int k, int64_t *out);
+
 /* Coefficients of per(C + x L) modulo each of k odd moduli 3 <= m < 2^63
  * (not necessarily prime), 0 <= n <= 32.  C and L are row-major int64;
  * prime_stride as in phip_permanent_residues (0: one pair for every modulus;

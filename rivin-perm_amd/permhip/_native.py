@@ -42,6 +42,10 @@ _SIGNATURES = {
         ctypes.c_int,
         [_I64P, ctypes.c_int64, ctypes.c_int, _I64P, ctypes.c_int, _I64P],
     ),
+    "phip_permanent_residues_noalloc": (
+        ctypes.c_int,
+        [_I64P, ctypes.c_int64, ctypes.c_int, _I64P, ctypes.c_int, _I64P],
+    ),
     "phip_permanent_modp": (ctypes.c_int64, [_I64P, ctypes.c_int, ctypes.c_int64]),
     "phip_binary_frontier_residues": (
         ctypes.c_int,

@@ -49,6 +49,7 @@ PLAIN_SOURCES = (
     "affine.hip",
     "host_prep.cpp",
     "frontier.hip",
+    "noalloc.hip",
 )
 
 
