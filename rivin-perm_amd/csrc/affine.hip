@@ -11,8 +11,8 @@
 // One launch per residue arithmetic (S30 below 2^30: signed lazy Montgomery
 // with exact int64 accumulators; wide_arith.h: Mont31L below 2^31 with one
 // lazy reduction per coefficient, Mont32, M61, PM61, Mont64) and
-// register tier (every 4 rows from 8 to 32 for 32-bit residues, every 8
-// for 64-bit ones); blockIdx.y selects the modulus.
+// register tier (every 4 rows from 8 to 40 for 32-bit residues, every 8 to
+// 32 for 64-bit ones); blockIdx.y selects the modulus.
 // As in wide.hip the flipped column and its direction are wave-uniform
 // except at a chunk's middle state, so deltas and negated deltas come by
 // scalar loads.  Each wavefront folds its lanes' accumulators with
@@ -71,8 +71,10 @@ struct S30 {
 };
 
 // How a term is built and summed for arithmetic A: by default the product
-// coefficients and the accumulators live in A's residues.
-template <class A>
+// coefficients and the accumulators live in A's residues.  kCompact marks the
+// register tiers above 32 rows, where the accumulators must take one
+// register per coefficient.
+template <class A, bool kCompact = false>
 struct AffineOps {
     typedef typename A::T T;
     typedef typename A::T V;
@@ -88,7 +90,7 @@ struct AffineOps {
 };
 
 template <>
-struct AffineOps<S30> {
+struct AffineOps<S30, false> {
     typedef uint32_t T;
     typedef int32_t V;
     typedef int64_t Acc;
@@ -107,6 +109,32 @@ struct AffineOps<S30> {
     }
 };
 
+// S30 above 32 rows: the same signed lazy products, but each accumulator is
+// one int32 kept in (-p, p) (|acc +- v| < 2p < 2^31, then one correction),
+// so 41 coefficients fit beside 80 row sums.  The accumulators hold plain
+// residue sums of the products, so no extra R^-1 is removed at the end.
+template <>
+struct AffineOps<S30, true> {
+    typedef uint32_t T;
+    typedef int32_t V;
+    typedef int32_t Acc;
+    static constexpr int kExtraR = 0;
+    __device__ __forceinline__ static V from_row(T x) { return (int32_t)x; }
+    __device__ __forceinline__ static V mul(const S30& ar, V a, T b) { return ar.redc((int64_t)a * (int32_t)b); }
+    __device__ __forceinline__ static V fma2(const S30& ar, V a, T b, V c, T d) {
+        return ar.redc((int64_t)a * (int32_t)b + (int64_t)c * (int32_t)d);
+    }
+    __device__ __forceinline__ static Acc acc_add(const S30& ar, Acc acc, V v, bool negative) {
+        const int32_t p = (int32_t)ar.p;
+        int32_t s = negative ? acc - v : acc + v;
+        s = s >= p ? s - p : s;
+        return s <= -p ? s + p : s;
+    }
+    __device__ __forceinline__ static T finish(const S30& ar, Acc acc) {
+        return (T)(acc < 0 ? acc + (int32_t)ar.p : acc);
+    }
+};
+
 // Tables hold NMAX rows per column: c0, l0: [q][NMAX] state-0 row sums;
 // dc, ndc, dl, ndl: [q][n-1][NMAX] deltas (2 C, 2 L by column) and their
 // negations.  Rows at or beyond n are the identity factor (constant part
@@ -120,7 +148,7 @@ __global__ __launch_bounds__(kAffineBlock) void affine_glynn_kernel(
     const typename A::T* __restrict__ dl, const typename A::T* __restrict__ ndl, const A* __restrict__ arith, int n,
     int cbits, uint64_t lanes, uint64_t waves, uint64_t* __restrict__ partials) {
     typedef typename A::T T;
-    typedef AffineOps<A> Ops;
+    typedef AffineOps<A, (NMAX > 32)> Ops;
     typedef typename Ops::V V;
     const int q = blockIdx.y;
     const A ar = arith[q];
@@ -279,6 +307,8 @@ int launch_affine(int tier, dim3 grid, hipStream_t st, const typename A::T* c0, 
             case 24: PHIP_AFFINE_LAUNCH(24); break;
             case 28: PHIP_AFFINE_LAUNCH(28); break;
             case 32: PHIP_AFFINE_LAUNCH(32); break;
+            case 36: PHIP_AFFINE_LAUNCH(36); break;
+            case 40: PHIP_AFFINE_LAUNCH(40); break;
             default: set_error("internal: no affine kernel for this register tier"); return -1;
         }
     } else {
@@ -304,10 +334,16 @@ int run_affine_class(DeviceContext* ctx, const int64_t* C, const int64_t* L, int
     const int kk = (int)jobs.size();
     if (kk == 0) return 0;
     // Register tier: the smallest that holds n rows (32-bit residues have
-    // tiers every 4 rows from 8; 64-bit ones every 8), or the tuning floor.
+    // tiers every 4 rows from 8 to 40; 64-bit ones every 8 to 32), or the
+    // tuning floor.
     const int step = sizeof(T) == 4 ? 4 : 8;
+    const int top = sizeof(T) == 4 ? kAffineMaxOrder : kAffineMaxOrderWide;
+    if (n > top) {
+        set_error("affine permanents of order above 32 need moduli below 2^32 on the device");
+        return -1;
+    }
     const int rows = std::max({8, n, min_tier});
-    const int W = std::min(32, (rows + step - 1) / step * step);  // rows per table column
+    const int W = std::min(top, (rows + step - 1) / step * step);  // rows per table column
     const size_t cols = (size_t)(n - 1) * W;
     // One host image: c0, l0 ([kk][W] each), then dc, ndc, dl, ndl
     // ([kk][n-1][W] each); rows n..W-1 are the identity factor.
@@ -402,7 +438,8 @@ int run_affine_class(DeviceContext* ctx, const int64_t* C, const int64_t* L, int
         uint64_t scale = host_powmod((q + 1) / 2, (uint64_t)Lbits, q);
         if (A::kRBits > 0) {
             const uint64_t r_mod_q = (uint64_t)((((unsigned __int128)1) << A::kRBits) % q);
-            scale = host_mulmod(scale, host_powmod(r_mod_q, (uint64_t)(Lbits + AffineOps<A>::kExtraR), q), q);
+            const int extra = W > 32 ? AffineOps<A, true>::kExtraR : AffineOps<A, false>::kExtraR;
+            scale = host_mulmod(scale, host_powmod(r_mod_q, (uint64_t)(Lbits + extra), q), q);
         }
         for (int d = 0; d < width; ++d)
             out[(size_t)jobs[s].index * width + d] = (int64_t)host_mulmod(h_out[(size_t)s * width + d], scale, q);
@@ -415,7 +452,7 @@ int run_affine_class(DeviceContext* ctx, const int64_t* C, const int64_t* L, int
 int affine_permanent_residues(DeviceContext* ctx, const int64_t* C, const int64_t* L, int64_t stride, int n,
                               const int64_t* moduli, int k, int min_tier, int64_t* out) {
     if (n < 0 || n > kAffineMaxOrder) {
-        set_error("affine permanent orders must lie in [0, 32] on the device");
+        set_error("affine permanent orders must lie in [0, 40] on the device");
         return -1;
     }
     if (k < 0 || (k > 0 && (!moduli || !out))) {

@@ -413,7 +413,9 @@ int necklace_orbit_stats(DeviceContext* ctx, int n, uint64_t* representatives, u
         return -1;
     }
     const int cus = ctx->compute_units() > 0 ? ctx->compute_units() : 256;
-    const unsigned grid = (unsigned)(cus * 2);
+    int per_cu = 0;
+    PHIP_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kNecklaceBlock, 0));
+    const unsigned grid = (unsigned)(cus * std::max(1, per_cu));
     hipStream_t st = ctx->stream();
     auto* d_tasks = (uint64_t*)ctx->buffer(25, tasks.size() * 8);
     auto* d_E = (int32_t*)ctx->buffer(26, 2 * (size_t)L * 4);
@@ -472,7 +474,18 @@ int necklace_minor_modp(DeviceContext* ctx, int n, uint64_t omega, uint64_t p, i
         return -1;
     }
     const int cus = ctx->compute_units() > 0 ? ctx->compute_units() : 256;
-    const unsigned grid = (unsigned)(cus * 2);
+    // As many workgroups per CU as the kernel's registers allow (3 at minor
+    // order 40 and 46, 4 below): the waves hide each other's scalar-load
+    // and cross-lane latency, and the task counter balances them.
+    int per_cu = 0;
+    if (p < (1ull << 31)) {
+        PHIP_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_for<NeckSigned31>(L),
+                                                                  kNecklaceBlock, 0));
+    } else {
+        PHIP_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_for<NeckUnsigned32>(L),
+                                                                  kNecklaceBlock, 0));
+    }
+    const unsigned grid = (unsigned)(cus * std::max(1, per_cu));
     hipStream_t st = ctx->stream();
     auto* d_tasks = (uint64_t*)ctx->buffer(25, tasks.size() * 8);
     auto* d_E = (int32_t*)ctx->buffer(26, E.size() * 4);

@@ -22,7 +22,7 @@ struct Tuning {
     int necklace_min_order = 23;  // smallest prime Schur order sent to the necklace kernel
     int necklace_depth = 28;      // largest FKM prefix depth of the necklace tasks
     int wide_min_tier = 8;        // smallest register tier (8..64, multiple of 4) of the wide Glynn kernels
-    int affine_min_tier = 8;      // smallest register tier (8..32, multiple of 4) of the affine Glynn kernels
+    int affine_min_tier = 8;      // smallest register tier (8..40, multiple of 4) of the affine Glynn kernels
     int skip_zeros_permille = 250;
 };
 Tuning& tuning();

@@ -18,8 +18,11 @@ Every residue comes from the coefficient-valued Glynn kernel on the device
 (``_polynomial_ext`` is the GPU object shaped like the reference's native
 extension).  Bounds, prime plans and CRT stay on the host in exact
 integers.  The reference's portable Python backend is not part of this
-build (backend='python' raises), and orders above 32 are rejected by the
-device kernel (2^32 Gray states of degree-n products per modulus).
+build (backend='python' raises).  The device kernel takes orders up to 40
+(2^39 Gray states of degree-n products per modulus); orders 41-63, which the
+reference accepts, raise ValueError.  Moduli of 2^32 and above run on the
+device up to order 32; above it their coefficients come from the exact
+coefficients (30-bit primes and CRT) reduced.
 """
 
 from __future__ import annotations
@@ -269,10 +272,24 @@ def _crt_vectors(rows: Sequence[Sequence[int]], primes: Sequence[int]) -> tuple:
 # Device residues.
 
 
+# Largest order of the device kernel; moduli of 2^32 and above stop at 32.
+DEVICE_MAX_ORDER = 40
+DEVICE_MAX_ORDER_WIDE = 32
+
+
+def _check_device_order(n: int) -> None:
+    if n > DEVICE_MAX_ORDER:
+        raise ValueError(
+            f"polynomial permanents of order above {DEVICE_MAX_ORDER} are beyond the device kernel "
+            f"(2^(n-1) Gray states of degree-n products per modulus)"
+        )
+
+
 def _device_affine_residues(constant, linear, primes) -> list:
     """Coefficient vectors modulo each prime, one device pass.  Entries that
     do not fit int64 are reduced per prime on the host first."""
     n = len(constant)
+    _check_device_order(n)
     flat = [v for m in (constant, linear) for row in m for v in row]
     if all(-(1 << 63) <= v <= _INT64_MAX for v in flat):
         C = np.array(constant, dtype=np.int64).reshape(n, n)
@@ -288,6 +305,11 @@ def _device_affine_residues(constant, linear, primes) -> list:
 
 
 def _affine_permanent_mod_validated(constant, linear, modulus: int) -> tuple:
+    n = len(constant)
+    _check_device_order(n)
+    if n > DEVICE_MAX_ORDER_WIDE and modulus >= (1 << 32):
+        # Beyond the 64-bit kernels' orders: the exact coefficients reduced.
+        return tuple(v % modulus for v in _affine_permanent_validated(constant, linear))
     reduced_c = tuple(tuple(v % modulus for v in row) for row in constant)
     reduced_l = tuple(tuple(v % modulus for v in row) for row in linear)
     if not reduced_c:

@@ -327,6 +327,14 @@ class HipPolynomialExtension:
             return (1 % m,)
         if n == 1:
             return (int(C[0, 0]) % m, int(L[0, 0]) % m)
+        if n > 32 and m >= (1 << 32):
+            # The 64-bit kernels stop at order 32: exact coefficients
+            # (30-bit primes, CRT), reduced.
+            import perm_polynomial
+
+            exact = perm_polynomial._affine_permanent_validated(tuple(map(tuple, C.tolist())),
+                                                                tuple(map(tuple, L.tolist())))
+            return tuple(v % m for v in exact)
         return _native.affine_permanent_residues(C, L, [m])[0]
 
     def _rook_many(self, matrix, moduli, num_threads, binary: bool):

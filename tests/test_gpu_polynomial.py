@@ -23,7 +23,7 @@ def _pair(rng, n, lo, hi):
     return C, L
 
 
-@pytest.mark.parametrize("tier", [8, 12, 16, 20, 24, 28, 32])
+@pytest.mark.parametrize("tier", [8, 12, 16, 20, 24, 28, 32, 36, 40])
 def test_affine_register_tiers(native, oracle_c, tier):
     old = native.get_tuning(native.TUNE_AFFINE_MIN_TIER)
     try:
@@ -63,6 +63,42 @@ def test_affine_evaluates_to_plain_permanents(native, n):
             assert sum(c * pow(t, d, p) for d, c in enumerate(row)) % p == want, (n, t, p)
 
 
+@pytest.mark.parametrize("n,moduli", [(33, [1_000_000_007, 2_147_483_647, 4_294_967_291]),
+                                      (36, [1_073_741_789])])
+def test_affine_above_order_32_evaluates_to_plain_permanents(native, n, moduli):
+    """Orders 33-40 run the register tiers 36 and 40 (32-bit residues): the
+    coefficient vector at t equals the plain permanent of C + t L from the
+    integer kernels, for several t."""
+    rng = random.Random(7250 + n)
+    C, L = _pair(rng, n, -2, 2)
+    rows = native.affine_permanent_residues(np.array(C, dtype=np.int64), np.array(L, dtype=np.int64), moduli)
+    for t in (1, -1, 3):
+        M = np.array([[C[i][j] + t * L[i][j] for j in range(n)] for i in range(n)], dtype=np.int64)
+        plain = native.permanent_residues(M, moduli)
+        for p, row, want in zip(moduli, rows, plain):
+            assert len(row) == n + 1
+            assert sum(c * pow(t, d, p) for d, c in enumerate(row)) % p == want, (n, t, p)
+
+
+def test_affine_wide_moduli_above_order_32(native):
+    """Moduli of 2^32 and above stop at order 32 on the device; above it
+    the mirror and the seam object reduce the exact coefficients."""
+    import perm_polynomial as pp
+    from permhip.backends import perm_polynomial_ext
+
+    n = 33
+    rng = random.Random(7260)
+    A = [[rng.randrange(2) for _ in range(n)] for _ in range(n)]
+    J = [[1] * n for _ in range(n)]
+    m = (1 << 61) - 1
+    exact = pp.affine_permanent(J, A)
+    assert pp.affine_permanent_mod(J, A, m) == tuple(v % m for v in exact)
+    assert tuple(perm_polynomial_ext.affine_permanent_mod(J, A, m)) == tuple(v % m for v in exact)
+    with pytest.raises(ValueError, match="above 40"):
+        big = [[1] * 41 for _ in range(41)]
+        pp.affine_permanent_mod(big, big, 101)
+
+
 def test_affine_per_modulus_stride(native, oracle_c):
     """prime_stride = n*n: one pre-reduced pair per modulus."""
     rng = random.Random(7300)
@@ -80,9 +116,12 @@ def test_affine_rejections(native):
     A = np.eye(3, dtype=np.int64)
     with pytest.raises(RuntimeError, match="odd"):
         native.affine_permanent_residues(A, A, [4])
-    with pytest.raises(RuntimeError, match="32"):
-        big = np.zeros((33, 33), dtype=np.int64)
+    with pytest.raises(RuntimeError, match="40"):
+        big = np.zeros((41, 41), dtype=np.int64)
         native.affine_permanent_residues(big, big, [101])
+    with pytest.raises(RuntimeError, match="2\\^32"):
+        big = np.zeros((33, 33), dtype=np.int64)
+        native.affine_permanent_residues(big, big, [(1 << 61) - 1])
     assert native.affine_permanent_residues(A, A, [101]) == [(1, 3, 3, 1)]
 
 
