@@ -247,6 +247,9 @@ int phip_batch_destroy(void *plan);
 #define PHIP_TUNE_SKIP_ZEROS_PERMILLE 9 /* 0/1 tables: largest estimated share of nonzero Gray terms, in
                                            permille, that runs on the zero-skipping kernels (0..1000, 0 off;
                                            read when a plan is built) */
+#define PHIP_TUNE_EXACT_MIN_PRIMES 10   /* fewest primes of one table (of at most 8 row groups) that run on the
+                                           exact-accumulation kernels, one walk for all of them (0 off, 1..1024;
+                                           read when a plan is built) */
 int phip_set_tuning(int key, int64_t value);
 int64_t phip_get_tuning(int key);
 

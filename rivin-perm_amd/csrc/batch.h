@@ -22,6 +22,11 @@ constexpr int kMaxSet = 8;
 constexpr int kVariantPlain = 0;
 constexpr int kVariantPaired = 1;     // consecutive states paired (primes below kPairedPrimeLimit)
 constexpr int kVariantSkipZeros = 2;  // zero terms skipped through a per-wavefront queue
+constexpr int kVariantExact = 3;      // exact accumulation: one walk for any number of primes
+// The exact variant's class set size: its items hold eight limb columns.
+constexpr int kExactColumns = 8;
+// Largest group count of the exact variant.
+constexpr int kExactMaxGroups = 8;
 
 // One job: one table (a matrix under one row order) and a set of K odd
 // moduli below 2^31 that share it (K = 1 for residue rows).  56 bytes,
@@ -37,7 +42,7 @@ struct BatchJob {
     int32_t group;
     int32_t nprimes;     // real primes (<= K); padded entries repeat the last one
     int32_t set;         // K of the job's launch class
-    int32_t variant;     // kVariantPlain, kVariantPaired or kVariantSkipZeros
+    int32_t variant;     // kVariantPlain, kVariantPaired, kVariantSkipZeros or kVariantExact
     int32_t pad;
 };
 
@@ -46,7 +51,7 @@ struct BatchSlot {
     uint32_t job;
     uint32_t k;
     uint32_t p;
-    uint32_t scale;  // R^groups * 2^-(n-1) mod p
+    uint32_t scale;  // R^groups * 2^-(n-1) mod p (exact jobs: 2^-(n-1) mod p)
 };
 
 // One persistent launch: the jobs of one (row-sum group, set size,
@@ -168,6 +173,16 @@ PHIP_DECLARE_BATCH_LAUNCH_SKIP(6, 2)
 PHIP_DECLARE_BATCH_LAUNCH_SKIP(6, 4)
 PHIP_DECLARE_BATCH_LAUNCH_SKIP(6, 8)
 #undef PHIP_DECLARE_BATCH_LAUNCH_SKIP
+// Exact-accumulation variants (at most eight groups, any number of primes).
+#define PHIP_DECLARE_BATCH_LAUNCH_EXACT(G)                                                                    \
+    hipError_t launch_batch_g##G##_k8_exact(dim3, hipStream_t, const BatchJob*, const uint2*, uint32_t,     \
+                                            uint32_t*, const int32_t*, const int32_t*, const uint32_t*,     \
+                                            uint64_t*);
+PHIP_DECLARE_BATCH_LAUNCH_EXACT(2)
+PHIP_DECLARE_BATCH_LAUNCH_EXACT(3)
+PHIP_DECLARE_BATCH_LAUNCH_EXACT(4)
+PHIP_DECLARE_BATCH_LAUNCH_EXACT(6)
+#undef PHIP_DECLARE_BATCH_LAUNCH_EXACT
 
 // Workgroups per CU each class kernel can keep resident (batch.hip), or -1.
 #define PHIP_DECLARE_BATCH_OCC(NAME) int NAME();
@@ -187,6 +202,8 @@ PHIP_DECLARE_BATCH_OCC(occupancy_batch_g6_k4_paired) PHIP_DECLARE_BATCH_OCC(occu
 PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k2_skip) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k4_skip)
 PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k8_skip) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g6_k2_skip)
 PHIP_DECLARE_BATCH_OCC(occupancy_batch_g6_k4_skip) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g6_k8_skip)
+PHIP_DECLARE_BATCH_OCC(occupancy_batch_g2_k8_exact) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g3_k8_exact)
+PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k8_exact) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g6_k8_exact)
 #undef PHIP_DECLARE_BATCH_OCC
 
 }  // namespace phip

@@ -15,7 +15,9 @@
 // PHIP_BATCH_SET select it, PHIP_BATCH_PAIRED=1 the variant for prime
 // sets below kPairedPrimeLimit that pairs consecutive states, and
 // PHIP_BATCH_SKIP_ZEROS=1 the variant for tables whose states are mostly
-// zero terms (ZeroSkipSink), whose item sums cover every lane.
+// zero terms (ZeroSkipSink), whose item sums cover every lane, and
+// PHIP_BATCH_EXACT=1 the exact-accumulation variant (ExactSink), whose items
+// hold eight limb columns of the exact sums for all of a job's primes.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -33,6 +35,9 @@
 #endif
 #ifndef PHIP_BATCH_SKIP_ZEROS
 #define PHIP_BATCH_SKIP_ZEROS 0
+#endif
+#ifndef PHIP_BATCH_EXACT
+#define PHIP_BATCH_EXACT 0
 #endif
 
 namespace phip {
@@ -54,11 +59,22 @@ __device__ __forceinline__ void run_item(const int32_t* __restrict__ tables, con
     }
 }
 
+// One wavefront's chunk of an exact job: the lanes' exact signed sums as
+// eight limb columns each (orders with more than kExactMaxGroups groups are
+// never planned on the exact variant and compile to nothing).
+template <int N, int G>
+__device__ __forceinline__ void run_item_exact(const int32_t* __restrict__ tables, const BatchJob& jb, uint64_t base,
+                                               int64_t (&cols)[kExactColumns]) {
+    if constexpr (G > 0 && (N + G - 1) / G <= kExactMaxGroups) {
+        glynn_chunk_exact<N, G>(tables, N, jb.cbits, base, cols);
+    }
+}
+
 // Largest group count over the engine's orders, for the zero-skip queue.
 template <int G>
 constexpr int kMaxGroups = G > 0 ? (kBatchMaxOrder + G - 1) / G : 1;
 
-template <int G, int K, bool P, bool Z>
+template <int G, int K, bool P, bool Z, bool E>
 __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __restrict__ jobs,
                                                          const uint2* __restrict__ items, uint32_t n_items,
                                                          uint32_t* __restrict__ counter,
@@ -85,6 +101,40 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
         const bool active = lane < jb.lanes;
         const uint64_t base = (active ? lane : 0) << jb.cbits;
         const int32_t* __restrict__ tables = blob + jb.data_off;
+        if constexpr (E) {
+            // Exact jobs: eight limb columns per item, for all of the job's
+            // primes (the finalisation reduces them per prime).
+            int64_t cols[kExactColumns];
+#pragma unroll
+            for (int c = 0; c < kExactColumns; ++c) cols[c] = 0;
+            switch (jb.n) {
+#define PHIP_EXACT_CASE(N) \
+    case N:                \
+        run_item_exact<N, G>(tables, jb, base, cols); \
+        break;
+                PHIP_EXACT_CASE(5) PHIP_EXACT_CASE(6) PHIP_EXACT_CASE(7) PHIP_EXACT_CASE(8)
+                PHIP_EXACT_CASE(9) PHIP_EXACT_CASE(10) PHIP_EXACT_CASE(11) PHIP_EXACT_CASE(12)
+                PHIP_EXACT_CASE(13) PHIP_EXACT_CASE(14) PHIP_EXACT_CASE(15) PHIP_EXACT_CASE(16)
+                PHIP_EXACT_CASE(17) PHIP_EXACT_CASE(18) PHIP_EXACT_CASE(19) PHIP_EXACT_CASE(20)
+                PHIP_EXACT_CASE(21) PHIP_EXACT_CASE(22) PHIP_EXACT_CASE(23) PHIP_EXACT_CASE(24)
+                PHIP_EXACT_CASE(25) PHIP_EXACT_CASE(26) PHIP_EXACT_CASE(27) PHIP_EXACT_CASE(28)
+                PHIP_EXACT_CASE(29) PHIP_EXACT_CASE(30) PHIP_EXACT_CASE(31) PHIP_EXACT_CASE(32)
+                PHIP_EXACT_CASE(33) PHIP_EXACT_CASE(34) PHIP_EXACT_CASE(35) PHIP_EXACT_CASE(36)
+                PHIP_EXACT_CASE(37) PHIP_EXACT_CASE(38) PHIP_EXACT_CASE(39) PHIP_EXACT_CASE(40)
+                PHIP_EXACT_CASE(41) PHIP_EXACT_CASE(42) PHIP_EXACT_CASE(43) PHIP_EXACT_CASE(44)
+                PHIP_EXACT_CASE(45) PHIP_EXACT_CASE(46)
+#undef PHIP_EXACT_CASE
+                default:
+                    break;
+            }
+#pragma unroll
+            for (int c = 0; c < kExactColumns; ++c) {
+                // Column sums of 64 lanes in two's complement: |sum| <
+                // 2^(37 + cbits).
+                const uint64_t s = wave_reduce_sum_u64(active ? (uint64_t)cols[c] : 0u);
+                if (lane_id == 0) item_partials[(size_t)idx * kMaxSet + c] = s;
+            }
+        } else {
         int32_t p[K];
         uint32_t pinv[K];
 #pragma unroll
@@ -123,6 +173,7 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
             const uint64_t s = wave_reduce_sum_u64((Z || active) ? (uint64_t)(int64_t)r[k] : 0u);
             if (lane_id == 0) item_partials[(size_t)idx * kMaxSet + k] = s;
         }
+        }
     }
 }
 
@@ -130,8 +181,13 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
 
 #define PHIP_CAT5(a, b, c, d, e) a##b##c##d##e
 #define PHIP_LAUNCH_NAME(G, K, P) PHIP_CAT5(launch_batch_g, G, _k, K, P)
-#if PHIP_BATCH_PAIRED && PHIP_BATCH_SKIP_ZEROS
-#error "a class either pairs states or skips zero terms"
+#if (PHIP_BATCH_PAIRED + PHIP_BATCH_SKIP_ZEROS + PHIP_BATCH_EXACT) > 1
+#error "a class either pairs states, skips zero terms or accumulates exactly"
+#elif PHIP_BATCH_EXACT
+#if PHIP_BATCH_SET != 8
+#error "exact classes hold eight limb columns per item"
+#endif
+#define PHIP_PAIRED_SUFFIX _exact
 #elif PHIP_BATCH_PAIRED
 #define PHIP_PAIRED_SUFFIX _paired
 #elif PHIP_BATCH_SKIP_ZEROS
@@ -146,7 +202,7 @@ hipError_t PHIP_LAUNCH_NAME(PHIP_BATCH_GROUP, PHIP_BATCH_SET, PHIP_PAIRED_SUFFIX
                                                               const int32_t* set_p, const uint32_t* set_pinv,
                                                               uint64_t* item_partials) {
     hipLaunchKernelGGL((batch_glynn_kernel<PHIP_BATCH_GROUP, PHIP_BATCH_SET, PHIP_BATCH_PAIRED != 0,
-                                           PHIP_BATCH_SKIP_ZEROS != 0>),
+                                           PHIP_BATCH_SKIP_ZEROS != 0, PHIP_BATCH_EXACT != 0>),
                        grid, dim3(256), 0, stream, jobs,
                        items, n_items, counter, blob, set_p, set_pinv, item_partials);
     return hipGetLastError();
@@ -160,7 +216,8 @@ int PHIP_OCC_NAME(PHIP_BATCH_GROUP, PHIP_BATCH_SET, PHIP_PAIRED_SUFFIX)() {
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &blocks,
-            batch_glynn_kernel<PHIP_BATCH_GROUP, PHIP_BATCH_SET, PHIP_BATCH_PAIRED != 0, PHIP_BATCH_SKIP_ZEROS != 0>,
+            batch_glynn_kernel<PHIP_BATCH_GROUP, PHIP_BATCH_SET, PHIP_BATCH_PAIRED != 0, PHIP_BATCH_SKIP_ZEROS != 0,
+                               PHIP_BATCH_EXACT != 0>,
             256, 0) != hipSuccess)
         return -1;
     return blocks;

@@ -39,6 +39,42 @@ __global__ __launch_bounds__(256) void batch_finalize_kernel(const BatchJob* __r
     const BatchSlot sl = slots[blockIdx.x];
     const BatchJob jb = jobs[sl.job];
     const uint64_t p = sl.p;
+    if (jb.variant == kVariantExact) {
+        // The job's items hold eight limb columns of exact signed sums,
+        // |column| < 2^(37 + cbits) per item (cbits <= 24), so runs of
+        // 2^(24 - cbits) items add exactly in int64 (< 2^61) before each
+        // reduction.  Column c weighs 2^(31 c).
+        const uint32_t run_len = 1u << (jb.cbits < 24 ? 24 - jb.cbits : 0);
+        int64_t run[kExactColumns] = {};
+        uint64_t col[kExactColumns] = {};
+        uint32_t in_run = 0;
+        for (uint32_t i = threadIdx.x; i < jb.n_items; i += 256) {
+            const uint64_t* part = item_partials + (size_t)(jb.first_item + i) * kMaxSet;
+#pragma unroll
+            for (int c = 0; c < kExactColumns; ++c) run[c] += (int64_t)part[c];
+            if (++in_run == run_len) {
+#pragma unroll
+                for (int c = 0; c < kExactColumns; ++c) {
+                    const int64_t v = run[c] % (int64_t)p;
+                    col[c] += (uint64_t)(v < 0 ? v + (int64_t)p : v);
+                    run[c] = 0;
+                }
+                in_run = 0;
+            }
+        }
+        uint64_t s = 0, w = 1 % p;
+        const uint64_t step = (1ull << 31) % p;
+#pragma unroll
+        for (int c = 0; c < kExactColumns; ++c) {
+            const int64_t v = run[c] % (int64_t)p;
+            const uint64_t r = (col[c] + (uint64_t)(v < 0 ? v + (int64_t)p : v)) % p;
+            s += (r * w) % p;
+            w = (w * step) % p;
+        }
+        s = block_reduce_sum_u64<256>(s % p);
+        if (threadIdx.x == 0) residues[blockIdx.x] = (int64_t)(((s % p) * (uint64_t)sl.scale) % p);
+        return;
+    }
     // Item sums are signed (two's complement) sums of 64 lane residues,
     // |sum| < 2^37, so 2^16 of them add exactly in int64 (< 2^53); each such
     // run is reduced once instead of every item.
@@ -76,6 +112,12 @@ BatchLaunchFn launcher_for(int group, int set, int variant) {
     PHIP_LAUNCHER_PAIRED(4, 4) PHIP_LAUNCHER_PAIRED(4, 8) PHIP_LAUNCHER_PAIRED(6, 4) PHIP_LAUNCHER_PAIRED(6, 8)
     PHIP_LAUNCHER_SKIP(4, 2) PHIP_LAUNCHER_SKIP(4, 4) PHIP_LAUNCHER_SKIP(4, 8)
     PHIP_LAUNCHER_SKIP(6, 2) PHIP_LAUNCHER_SKIP(6, 4) PHIP_LAUNCHER_SKIP(6, 8)
+    if (set == kExactColumns && variant == kVariantExact) {
+        if (group == 2) return launch_batch_g2_k8_exact;
+        if (group == 3) return launch_batch_g3_k8_exact;
+        if (group == 4) return launch_batch_g4_k8_exact;
+        if (group == 6) return launch_batch_g6_k8_exact;
+    }
 #undef PHIP_LAUNCHER
 #undef PHIP_LAUNCHER_PAIRED
 #undef PHIP_LAUNCHER_SKIP
@@ -98,6 +140,12 @@ int occupancy_for(int group, int set, int variant) {
     PHIP_OCC_PAIRED(4, 4) PHIP_OCC_PAIRED(4, 8) PHIP_OCC_PAIRED(6, 4) PHIP_OCC_PAIRED(6, 8)
     PHIP_OCC_SKIP(4, 2) PHIP_OCC_SKIP(4, 4) PHIP_OCC_SKIP(4, 8)
     PHIP_OCC_SKIP(6, 2) PHIP_OCC_SKIP(6, 4) PHIP_OCC_SKIP(6, 8)
+    if (set == kExactColumns && variant == kVariantExact) {
+        if (group == 2) return occupancy_batch_g2_k8_exact();
+        if (group == 3) return occupancy_batch_g3_k8_exact();
+        if (group == 4) return occupancy_batch_g4_k8_exact();
+        if (group == 6) return occupancy_batch_g6_k8_exact();
+    }
 #undef PHIP_OCC
 #undef PHIP_OCC_PAIRED
 #undef PHIP_OCC_SKIP
@@ -242,8 +290,41 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
 
     // Pass 2: jobs (one per set chunk) and slots.
     out.slots.resize(in.size());
+    const int exact_min = tuning().exact_min_primes;
     for (const Set& st : sets) {
         size_t next = 0;
+        // A packed table of at most eight groups with enough primes runs
+        // one exact-accumulation walk for all of them: the walk costs
+        // about as much as the modular walk of a set of six primes, and
+        // the residues come from its exact sums.
+        if (exact_min > 0 && st.group > 0 && !st.skip_zeros && groups_of(st.n, st.group) <= kExactMaxGroups &&
+            st.inputs.size() >= (size_t)exact_min) {
+            BatchJob jb{};
+            jb.data_off = st.data_off;
+            jb.n = st.n;
+            jb.group = st.group;
+            jb.set = kExactColumns;
+            jb.nprimes = (int32_t)st.inputs.size();
+            jb.prime_off = (uint32_t)out.set_p.size();
+            jb.variant = kVariantExact;
+            const uint32_t job = (uint32_t)out.jobs.size();
+            const int L = st.n - 1;
+            for (size_t k = 0; k < st.inputs.size(); ++k) {
+                const size_t idx = st.inputs[k];
+                const uint32_t q = in[idx].q;
+                BatchSlot sl;
+                sl.job = job;
+                sl.k = (uint32_t)k;
+                sl.p = q;
+                // The exact sums carry only Glynn's 2^(n-1).
+                sl.scale = (uint32_t)host_powmod((q + 1) / 2, (uint64_t)L, q);
+                out.slots[idx] = sl;
+                out.total_states += std::ldexp(1.0, L);
+                out.total_mulmods += (double)st.n * std::ldexp(1.0, L);
+            }
+            out.jobs.push_back(jb);
+            continue;
+        }
         std::vector<int> set_sizes =
             st.group == 0 ? std::vector<int>(st.inputs.size(), 1) : split_set((int)st.inputs.size());
         // The zero-skipping kernels take sets of 2, 4 or 8 (one prime is
@@ -323,13 +404,16 @@ int assign_items(PlanHost& out, int items_log2, int min_cbits) {
         const int s = std::min(s_all, spread(class_states[class_key(jb)], std::ldexp(1.0, items_log2 - 2)));
         int c = (L >= 10) ? std::max(4, std::min(s - 6, L - 6)) : 4;
         c = std::min(std::max(c, min_cbits), L);
+        // Exact jobs keep a lane's columns below 2^55 and an item's below
+        // 2^61 (the finalisation's run length relies on it).
+        if (jb.variant == kVariantExact) c = std::min(c, 24);
         jb.cbits = c;
         jb.lanes = 1ull << (L - c);
     }
 
     // Items: per (group, set, variant) class, jobs by descending cost, each
     // job's items contiguous.
-    static const int kClasses[][3] = {{0, 1, 0},
+    static const int kClasses[][3] = {{2, 8, 3}, {3, 8, 3}, {4, 8, 3}, {6, 8, 3}, {0, 1, 0},
                                       {2, 8, 1}, {2, 8, 0}, {2, 4, 1}, {2, 4, 0}, {2, 2, 0}, {2, 1, 0},
                                       {3, 8, 1}, {3, 8, 0}, {3, 4, 1}, {3, 4, 0}, {3, 2, 0}, {3, 1, 0},
                                       {4, 8, 1}, {4, 8, 2}, {4, 8, 0}, {4, 4, 1}, {4, 4, 2}, {4, 4, 0},

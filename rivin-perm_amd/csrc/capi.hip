@@ -141,6 +141,10 @@ int phip_set_tuning(int key, int64_t value) {
             if (value < 0 || value > 1000) break;
             t.skip_zeros_permille = (int)value;
             return 0;
+        case PHIP_TUNE_EXACT_MIN_PRIMES:
+            if (value < 0 || value > 1024) break;
+            t.exact_min_primes = (int)value;
+            return 0;
         default:
             phip::set_error("unknown tuning key");
             return -1;
@@ -161,6 +165,7 @@ int64_t phip_get_tuning(int key) {
         case PHIP_TUNE_WIDE_MIN_TIER: return t.wide_min_tier;
         case PHIP_TUNE_AFFINE_MIN_TIER: return t.affine_min_tier;
         case PHIP_TUNE_SKIP_ZEROS_PERMILLE: return t.skip_zeros_permille;
+        case PHIP_TUNE_EXACT_MIN_PRIMES: return t.exact_min_primes;
         default:
             phip::set_error("unknown tuning key");
             return -1;

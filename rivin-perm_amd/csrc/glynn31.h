@@ -44,6 +44,7 @@
 
 #include <utility>
 
+#include "exact_limbs.h"
 #include "phip_arith.h"
 #include "launchers.h"
 #include "phip_reduce.h"
@@ -389,6 +390,48 @@ struct PrimeSetSink {
     }
 };
 
+// Exact accumulation (any number of primes): each state's product of at
+// most eight exact group products (|g| < 2^31) is formed exactly, as a
+// magnitude in eight radix-2^31 limbs (exact_limbs.h) and a sign, and every
+// limb is added with that sign to its own int64 column accumulator.  No
+// modular arithmetic runs in the walk: the lane's columns hold its exact
+// signed sum of terms, sum_c acc[c] 2^(31 c), and any prime's residue
+// follows from them at the end.  Per state that is the exact pair products
+// (4 multiplies), the two 2 x 2 and one 4 x 4 limb products (24 multiplies
+// and 13 limb splits), the magnitudes and the sign, and 8 accumulations,
+// for every prime at once.  Every limb is below 2^31, so after 2^cbits
+// states |acc[c]| < 2^(31 + cbits).
+template <int NR, int G>
+struct ExactSink {
+    static_assert(G > 0, "exact accumulation needs exact packed row sums");
+    using Rows = PackedRows<NR, G>;
+    static constexpr int kGroups = Rows::kGroups;
+    static_assert(kGroups <= 8, "exact accumulation takes at most eight groups");
+    int64_t acc[8];
+
+    template <bool ODD>
+    __device__ __forceinline__ void state(const Rows& rs) {
+        int32_t g[kGroups];
+        rs.groups(g, std::make_integer_sequence<int, kGroups>{});
+        uint32_t m[8];
+        uint32_t sign = ODD ? 0x80000000u : 0u;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (k < kGroups) {
+                sign ^= (uint32_t)g[k];
+                m[k] = (uint32_t)(g[k] < 0 ? -g[k] : g[k]);
+            } else {
+                m[k] = 1;
+            }
+        }
+        uint32_t z[8];
+        limb_product8(m, z);
+        const int32_t e = ((int32_t)sign >> 31) | 1;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[c] = mad_exact(acc[c], (int32_t)z[c], e);
+    }
+};
+
 // K primes sharing one packed table whose states mostly have a zero row sum
 // (0/1 matrices: a row of even degree d sums to zero in C(d, d/2) / 2^d of
 // the states, and one zero row zeroes the term).  A term is zero exactly
@@ -614,6 +657,19 @@ __device__ __forceinline__ void glynn_chunk_set(const int32_t* __restrict__ tabl
     const bool neg = gray_walk<NR, G>(tables, n, cbits, base, 0, sink);
 #pragma unroll
     for (int k = 0; k < K; ++k) out[k] = lane_value(sink.acc[k], neg, sink.negp[k], sink.pinv[k]);
+}
+
+// The lane's exact signed sum of terms as eight int64 columns (value
+// sum_c out[c] 2^(31 c)), for tables of at most eight groups.
+template <int NR, int G>
+__device__ __forceinline__ void glynn_chunk_exact(const int32_t* __restrict__ tables, int n, int cbits,
+                                                  uint64_t base, int64_t (&out)[8]) {
+    ExactSink<NR, G> sink;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) sink.acc[c] = 0;
+    const bool neg = gray_walk<NR, G>(tables, n, cbits, base, 0, sink);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) out[c] = neg ? -sink.acc[c] : sink.acc[c];
 }
 
 // glynn_chunk_set through ZeroSkipSink.  Every lane of the wavefront must

@@ -28,14 +28,16 @@ LIBRARY = PACKAGE_DIR / "libpermhip.so"
 ARCH = "gfx950"
 # (row-sum group, prime-set size, variant) classes of the persistent Glynn
 # kernel.  Variant 0 is the plain kernel; sets of 4 and 8 primes below
-# 1.25 * 2^30 also have the paired-state variant (1), and 0/1 tables in
-# groups of 4 or 6 the zero-skipping variant (2) for sets of 2, 4 and 8.
-BATCH_VARIANT_SUFFIX = {0: "", 1: "_paired", 2: "_skip"}
+# 1.25 * 2^30 also have the paired-state variant (1), 0/1 tables in groups
+# of 4 or 6 the zero-skipping variant (2) for sets of 2, 4 and 8, and every
+# group the exact-accumulation variant (3, eight limb columns per item).
+BATCH_VARIANT_SUFFIX = {0: "", 1: "_paired", 2: "_skip", 3: "_exact"}
 BATCH_CLASSES = (
     ((0, 1, 0),)
     + tuple((g, k, 0) for g in (2, 3, 4, 6) for k in (1, 2, 4, 8))
     + tuple((g, k, 1) for g in (2, 3, 4, 6) for k in (4, 8))
     + tuple((g, k, 2) for g in (4, 6) for k in (2, 4, 8))
+    + tuple((g, 8, 3) for g in (2, 3, 4, 6))
 )
 PLAIN_SOURCES = (
     "device_context.hip",
@@ -98,7 +100,7 @@ def build(jobs: int | None = None, verbose: bool = False) -> Path:
                 CSRC / "batch.hip",
                 BUILD_DIR / f"batch_g{group}_k{k}{BATCH_VARIANT_SUFFIX[variant]}.o",
                 [f"-DPHIP_BATCH_GROUP={group}", f"-DPHIP_BATCH_SET={k}", f"-DPHIP_BATCH_PAIRED={int(variant == 1)}",
-                 f"-DPHIP_BATCH_SKIP_ZEROS={int(variant == 2)}"],
+                 f"-DPHIP_BATCH_SKIP_ZEROS={int(variant == 2)}", f"-DPHIP_BATCH_EXACT={int(variant == 3)}"],
             )
         )
     for name in PLAIN_SOURCES:

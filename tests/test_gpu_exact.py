@@ -1,0 +1,121 @@
+"""The exact-accumulation kernels (ExactSink: each Gray state's product of
+at most eight exact group products formed exactly in radix-2^31 limbs, one
+walk for every prime of a table) against the oracle, bit for bit, and
+against the modular kernels they replace (PHIP_TUNE_EXACT_MIN_PRIMES = 0).
+Every case names its seed, order, entry range and prime count."""
+
+import random
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+PRIMES = [1073741827, 1073741831, 1073741833, 1073741839, 1073741843, 1073741857, 1073741891, 1073741909,
+          1073741939, 1073741953, 1073741969, 1073741971, 2147483647, 2147483629, 2147483587, 2147483579]
+
+
+def _with_tuning(native, key, value, fn):
+    old = native.get_tuning(key)
+    native.set_tuning(key, value)
+    try:
+        return fn()
+    finally:
+        native.set_tuning(key, old)
+
+
+def _classes(native, A, primes):
+    return set(native.batch_plan_classes([A], [primes]))
+
+
+# (order, entry bound): row bounds that make the planner pick groups of 6, 4,
+# 3 and 2 rows, with at most eight groups (the exact variant's limit).
+SHAPES = [(5, 1), (8, 3), (12, 1), (13, 40), (16, 100), (18, 4), (20, 9), (23, 12), (24, 1), (27, 2),
+          (28, 6), (30, 1), (31, 3), (32, 10)]
+
+
+@pytest.mark.parametrize("n,bound", SHAPES)
+def test_exact_matches_oracle_and_modular(native, oracle_c, n, bound):
+    rng = np.random.default_rng(900 + 31 * n + bound)
+    A = rng.integers(-bound, bound + 1, (n, n)).astype(np.int64)
+    for k in (6, 8, 10, 16):
+        primes = PRIMES[:k]
+        assert any(c[2] == 3 for c in _classes(native, A, primes)), (n, bound, k)
+        got = native.permanent_residues(A, primes)
+        modular = _with_tuning(native, native.TUNE_EXACT_MIN_PRIMES, 0, lambda: native.permanent_residues(A, primes))
+        assert got == modular, (n, bound, k)
+        if n <= 20:
+            assert got == [oracle_c.permanent_mod_p(A.tolist(), p, 8) for p in primes], (n, bound, k)
+
+
+def test_exact_extreme_group_products(native, oracle_c):
+    """Rows whose sums stay at their bounds: every group product at the
+    largest magnitude the planner admits, with both signs."""
+    for seed, n in ((1, 16), (2, 20), (3, 24)):
+        rng = random.Random(seed)
+        # Rows of equal entries: every state's row sums are +-(n-2)*a ... +-n*a.
+        A = np.array([[rng.choice((-1, 1)) * 35 for _ in range(n)] for _ in range(n)], dtype=np.int64)
+        primes = PRIMES[:8]
+        assert any(c[2] == 3 for c in _classes(native, A, primes))
+        got = native.permanent_residues(A, primes)
+        if n <= 20:
+            assert got == [oracle_c.permanent_mod_p(A.tolist(), p, 8) for p in primes]
+        assert got == _with_tuning(native, native.TUNE_EXACT_MIN_PRIMES, 0,
+                                   lambda: native.permanent_residues(A, primes))
+
+
+def test_exact_zero_rows_and_identity(native):
+    n = 24
+    Z = np.zeros((n, n), dtype=np.int64)
+    Z[3] = 0
+    Z[np.arange(n), np.arange(n)] = 1
+    assert native.permanent_residues(Z, PRIMES[:8]) == [1] * 8
+    Z[5] = 0
+    assert native.permanent_residues(Z, PRIMES[:8]) == [0] * 8
+
+
+def test_exact_long_chunks(native, oracle_c):
+    """A large minimum chunk size (the whole 2^17-state walk in one lane's
+    chunk when 2^30 states per lane are requested): the lane columns grow to
+    their largest; results are unchanged."""
+    rng = np.random.default_rng(77)
+    A = rng.integers(-3, 4, (18, 18)).astype(np.int64)
+    primes = PRIMES[:8]
+    want = [oracle_c.permanent_mod_p(A.tolist(), p, 8) for p in primes]
+    got = _with_tuning(native, native.TUNE_MIN_CBITS, 30, lambda: native.permanent_residues(A, primes))
+    assert got == want
+
+
+def test_exact_in_batches_with_modular_jobs(native, oracle_c):
+    """One plan mixing exact jobs (many primes) with modular ones (few
+    primes, more than eight groups, 0/1 tables) in every class."""
+    rng = np.random.default_rng(5)
+    mats, moduli = [], []
+    for i in range(40):
+        n = int(rng.integers(5, 21))
+        bound = int(rng.choice((1, 5, 60, 3000)))
+        mats.append(rng.integers(-bound, bound + 1, (n, n)).astype(np.int64))
+        moduli.append(PRIMES[: int(rng.integers(1, 13))])
+    plan = native.Batch(mats, moduli)
+    try:
+        flat = plan.run()
+    finally:
+        plan.close()
+    classes = native.batch_plan_classes(mats, moduli)
+    assert any(c[2] == 3 for c in classes) and any(c[2] != 3 for c in classes)
+    pos = 0
+    for A, ps in zip(mats, moduli):
+        want = [oracle_c.permanent_mod_p(A.tolist(), p, 8) for p in ps]
+        assert flat[pos:pos + len(ps)] == want
+        pos += len(ps)
+
+
+def test_headline_goes_exact(native):
+    """The 32 x 32 headline matrix with its 8 primes runs on one exact walk."""
+    from permhip import planner, workloads
+
+    A = workloads.headline()
+    primes, _ = planner.generic_crt_primes(planner.matrix_permanent_bound(A.tolist()))
+    assert _classes(native, A, primes) == {(4, 8, 3)}
+    assert native.permanent_residues(A, primes) == _with_tuning(
+        native, native.TUNE_EXACT_MIN_PRIMES, 0, lambda: native.permanent_residues(A, primes))

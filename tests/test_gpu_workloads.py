@@ -42,7 +42,9 @@ def test_batch4096_one_plan_matches_reference_residues(native, golden):
     batch = PermanentBatch(mats)
     try:
         stats = batch.stats()
-        assert stats["jobs"] > 6000 and stats["residues"] == sum(len(p) for p in batch.primes)
+        # At least one job per matrix (a table's primes split into sets, or
+        # one exact job for all of them).
+        assert stats["jobs"] >= len(mats) and stats["residues"] == sum(len(p) for p in batch.primes)
         residues = batch.residues()
         exact = batch.run()
     finally:
