@@ -11,10 +11,11 @@ integer.
 Printed as one JSON line (rank 0 only; this build runs on one GPU):
   value        reference-equivalent Gray-code mulmods per second: n 2^(n-1)
                per residue, the reference's n-factor row product per state
-               (perm_c_omp.c:40-47).  The kernel executes fewer modular
-               reductions than that (exact 16-bit row sums, exact group
-               products shared by the prime set); `executed` reports the
-               Montgomery reductions it does run per state and per second
+               (perm_c_omp.c:40-47).  The kernel executes no modular
+               reduction in its walk (exact 16-bit row sums, exact group
+               products, each state's product exact in radix-2^31 limbs,
+               one walk for all eight primes); `executed` reports what it
+               does run per state and per second
   roofline     the dominant kernel's executed VALU lane operations per
                second (rocprof SQ_INSTS_VALU per launch x 64 lanes, over
                the kernel time measured live with HIP events) against the
@@ -64,6 +65,25 @@ def batch_matrices(count: int = 4096):
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+# VALU instructions per Gray state of the exact-accumulation kernel at
+# n = 32 in groups of 4 (eight groups), counted in the compiled loop of
+# batch_glynn_kernel<4, 8, false, false, true> (an eight-state block):
+# flips, group products, magnitudes and sign, the limb products and splits,
+# the column accumulations.
+EXACT_MIX_N32_G4 = {
+    "v_pk_add_u16/v_pk_sub_i16": 18,
+    "v_mul_i32_i24(_sdwa)": 24,
+    "v_sub_u32+v_max_i32": 16,
+    "v_xor_b32": 7,
+    "v_mad_u64_u32": 28,
+    "v_and_b32": 17.1,
+    "v_lshrrev_b64": 10.1,
+    "v_alignbit_b32": 7,
+    "v_mad_i64_i32": 8,
+    "other": 5.3,
+}
 
 
 def instruction_mix(n: int, group: int, k: int = 1, paired: bool = False) -> dict:
@@ -652,27 +672,44 @@ def main() -> int:
     rates = _native.valu_microbench()
     log("VALU lane-ops/s: " + ", ".join(f"{k}={v:.3e}" for k, v in rates.items()))
     st = plan.stats()
-    group, kset, paired = int(st["group"]), int(st["set"]), int(st.get("variant", 0)) == 1
+    group, kset, variant = int(st["group"]), int(st["set"]), int(st.get("variant", 0))
     cus = int(re.search(r"CUs=(\d+)", info).group(1)) if re.search(r"CUs=(\d+)", info) else 256
-    mix = instruction_mix(HEADLINE_N, group, kset, paired)
-    states_per_launch = float(st["states"]) / len(primes)  # one walk serves the whole prime set
+    exact_variant = variant == 3
+    if exact_variant:
+        mix = dict(EXACT_MIX_N32_G4)
+    else:
+        mix = instruction_mix(HEADLINE_N, group, kset, variant == 1)
+    states_per_launch = float(st["states"]) / len(primes)  # one walk serves every prime of the table
     roof = roofline(k_ms, states_per_launch, cus, rates, mix)
     roof["kernel"] = (
-        f"batch_glynn_kernel<{group}, {kset}, {str(paired).lower()}, "
-        f"{str(int(st.get('variant', 0)) == 2).lower()}> (n={HEADLINE_N}, {len(primes)} primes in "
+        f"batch_glynn_kernel<{group}, {kset}, {str(variant == 1).lower()}, {str(variant == 2).lower()}, "
+        f"{str(exact_variant).lower()}> (n={HEADLINE_N}, {len(primes)} primes in "
         f"{int(st['jobs'])} walk(s) of the Gray code)"
     )
     roof["instruction_rates"] = rates
-    # What the kernel executes against what `value` counts: one Montgomery
-    # reduction is one v_mul_lo_u32 of the mix.
-    reductions_per_state = mix["v_mul_lo_u32"]
+    # What the kernel executes against what `value` counts.
     executed = {
         "reference_equivalent_mulmods_per_state": HEADLINE_N * len(primes),
-        "montgomery_reductions_per_state": reductions_per_state,
-        "montgomery_reductions_per_s": reductions_per_state * states_per_launch / (k_ms * 1e-3),
         "gray_states_per_s": states_per_launch / (k_ms * 1e-3),
         "kernel_reference_equivalent_mulmods_per_s": kernel_rate,
     }
+    if exact_variant:
+        # Exact accumulation: no modular arithmetic in the walk; each state's
+        # product of 8 exact group products is formed in radix-2^31 limbs
+        # (28 multiply-adds) and added to 8 int64 columns, for all primes.
+        executed.update({
+            "variant": "exact accumulation",
+            "montgomery_reductions_per_state": 0,
+            "limb_multiply_adds_per_state": mix["v_mad_u64_u32"],
+            "limb_multiply_adds_per_s": mix["v_mad_u64_u32"] * states_per_launch / (k_ms * 1e-3),
+        })
+    else:
+        # One Montgomery reduction is one v_mul_lo_u32 of the mix.
+        executed.update({
+            "variant": "paired prime set" if variant == 1 else "prime set",
+            "montgomery_reductions_per_state": mix["v_mul_lo_u32"],
+            "montgomery_reductions_per_s": mix["v_mul_lo_u32"] * states_per_launch / (k_ms * 1e-3),
+        })
 
     # The drop-in call: perm.matrix_permanent(A) end to end (validation,
     # bound, primes, a one-matrix plan in the device context's buffers, the
