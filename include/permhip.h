@@ -250,6 +250,10 @@ int phip_batch_destroy(void *plan);
 #define PHIP_TUNE_EXACT_MIN_PRIMES 10   /* fewest primes of one table (of at most 8 row groups) that run on the
                                            exact-accumulation kernels, one walk for all of them (0 off, 1..1024;
                                            read when a plan is built) */
+#define PHIP_TUNE_EXACT_WIDE_MIN_PRIMES 11 /* the same for tables of 9..16 row groups whose row-bound product is
+                                              below 2^372 (0 off, 1..1024; read when a plan is built; the
+                                              exact variant is off for every table while
+                                              PHIP_TUNE_EXACT_MIN_PRIMES is 0) */
 int phip_set_tuning(int key, int64_t value);
 int64_t phip_get_tuning(int key);
 

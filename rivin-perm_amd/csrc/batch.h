@@ -14,19 +14,25 @@ class DeviceContext;
 
 constexpr int kBatchMinOrder = 5;
 constexpr int kBatchMaxOrder = 46;
-// Most primes one job evaluates together, and the stride of the per-item
-// partial sums.
+// Most primes one job evaluates together.
 constexpr int kMaxSet = 8;
+// Per-item partial sums: one 64-bit slot per prime of the job's set, or per
+// limb column of an exact job (at most kExactMaxColumns).
+constexpr int kItemSlots = 12;
 
 // Kernel variants of a (group, set) class.
 constexpr int kVariantPlain = 0;
 constexpr int kVariantPaired = 1;     // consecutive states paired (primes below kPairedPrimeLimit)
 constexpr int kVariantSkipZeros = 2;  // zero terms skipped through a per-wavefront queue
 constexpr int kVariantExact = 3;      // exact accumulation: one walk for any number of primes
-// The exact variant's class set size: its items hold eight limb columns.
+// The exact variant's classes: a class's set size is its number of limb
+// columns L.  L = 8 takes tables of at most eight groups (full products);
+// L = 10 and 12 take up to sixteen groups when the product of the table's
+// row bounds is below 2^(31 L) (products kept modulo 2^(31 L)).
 constexpr int kExactColumns = 8;
-// Largest group count of the exact variant.
 constexpr int kExactMaxGroups = 8;
+constexpr int kExactWideMaxGroups = 16;
+constexpr int kExactMaxColumns = 12;
 
 // One job: one table (a matrix under one row order) and a set of K odd
 // moduli below 2^31 that share it (K = 1 for residue rows).  56 bytes,
@@ -97,7 +103,7 @@ struct PlanDevice {
     int32_t* set_p;
     uint32_t* set_pinv;
     BatchSlot* slots;
-    uint64_t* partials;  // kMaxSet per item
+    uint64_t* partials;  // kItemSlots per item
     uint32_t* counters;  // one per class
     int64_t* residues;   // one per slot
 };
@@ -173,15 +179,21 @@ PHIP_DECLARE_BATCH_LAUNCH_SKIP(6, 2)
 PHIP_DECLARE_BATCH_LAUNCH_SKIP(6, 4)
 PHIP_DECLARE_BATCH_LAUNCH_SKIP(6, 8)
 #undef PHIP_DECLARE_BATCH_LAUNCH_SKIP
-// Exact-accumulation variants (at most eight groups, any number of primes).
-#define PHIP_DECLARE_BATCH_LAUNCH_EXACT(G)                                                                    \
-    hipError_t launch_batch_g##G##_k8_exact(dim3, hipStream_t, const BatchJob*, const uint2*, uint32_t,     \
-                                            uint32_t*, const int32_t*, const int32_t*, const uint32_t*,     \
-                                            uint64_t*);
-PHIP_DECLARE_BATCH_LAUNCH_EXACT(2)
-PHIP_DECLARE_BATCH_LAUNCH_EXACT(3)
-PHIP_DECLARE_BATCH_LAUNCH_EXACT(4)
-PHIP_DECLARE_BATCH_LAUNCH_EXACT(6)
+// Exact-accumulation variants (any number of primes; L = K limb columns).
+#define PHIP_DECLARE_BATCH_LAUNCH_EXACT(G, K)                                                                 \
+    hipError_t launch_batch_g##G##_k##K##_exact(dim3, hipStream_t, const BatchJob*, const uint2*, uint32_t,  \
+                                              uint32_t*, const int32_t*, const int32_t*, const uint32_t*,  \
+                                              uint64_t*);
+PHIP_DECLARE_BATCH_LAUNCH_EXACT(2, 8)
+PHIP_DECLARE_BATCH_LAUNCH_EXACT(3, 8)
+PHIP_DECLARE_BATCH_LAUNCH_EXACT(4, 8)
+PHIP_DECLARE_BATCH_LAUNCH_EXACT(6, 8)
+PHIP_DECLARE_BATCH_LAUNCH_EXACT(2, 10)
+PHIP_DECLARE_BATCH_LAUNCH_EXACT(2, 12)
+PHIP_DECLARE_BATCH_LAUNCH_EXACT(3, 10)
+PHIP_DECLARE_BATCH_LAUNCH_EXACT(3, 12)
+PHIP_DECLARE_BATCH_LAUNCH_EXACT(4, 10)
+PHIP_DECLARE_BATCH_LAUNCH_EXACT(4, 12)
 #undef PHIP_DECLARE_BATCH_LAUNCH_EXACT
 
 // Workgroups per CU each class kernel can keep resident (batch.hip), or -1.
@@ -204,6 +216,9 @@ PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k8_skip) PHIP_DECLARE_BATCH_OCC(occupa
 PHIP_DECLARE_BATCH_OCC(occupancy_batch_g6_k4_skip) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g6_k8_skip)
 PHIP_DECLARE_BATCH_OCC(occupancy_batch_g2_k8_exact) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g3_k8_exact)
 PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k8_exact) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g6_k8_exact)
+PHIP_DECLARE_BATCH_OCC(occupancy_batch_g2_k10_exact) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g2_k12_exact)
+PHIP_DECLARE_BATCH_OCC(occupancy_batch_g3_k10_exact) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g3_k12_exact)
+PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k10_exact) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k12_exact)
 #undef PHIP_DECLARE_BATCH_OCC
 
 }  // namespace phip

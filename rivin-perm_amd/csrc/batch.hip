@@ -17,7 +17,8 @@
 // PHIP_BATCH_SKIP_ZEROS=1 the variant for tables whose states are mostly
 // zero terms (ZeroSkipSink), whose item sums cover every lane, and
 // PHIP_BATCH_EXACT=1 the exact-accumulation variant (ExactSink), whose items
-// hold eight limb columns of the exact sums for all of a job's primes.
+// hold PHIP_BATCH_SET limb columns of the exact sums for all of a job's
+// primes.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -59,14 +60,18 @@ __device__ __forceinline__ void run_item(const int32_t* __restrict__ tables, con
     }
 }
 
-// One wavefront's chunk of an exact job: the lanes' exact signed sums as
-// eight limb columns each (orders with more than kExactMaxGroups groups are
-// never planned on the exact variant and compile to nothing).
-template <int N, int G>
+// One wavefront's chunk of an exact job: the lanes' exact signed sums as L
+// limb columns each.  The L = 8 class takes tables of at most
+// kExactMaxGroups groups, the wider ones tables of more groups, up to
+// kExactWideMaxGroups; other orders are never planned on the class and
+// compile to nothing.
+template <int N, int G, int L>
 __device__ __forceinline__ void run_item_exact(const int32_t* __restrict__ tables, const BatchJob& jb, uint64_t base,
-                                               int64_t (&cols)[kExactColumns]) {
-    if constexpr (G > 0 && (N + G - 1) / G <= kExactMaxGroups) {
-        glynn_chunk_exact<N, G>(tables, N, jb.cbits, base, cols);
+                                               int64_t (&cols)[L]) {
+    constexpr int kGroups = G > 0 ? (N + G - 1) / G : 0;
+    if constexpr (G > 0 && (L == kExactColumns ? kGroups <= kExactMaxGroups
+                                               : (kGroups > kExactMaxGroups && kGroups <= kExactWideMaxGroups))) {
+        glynn_chunk_exact<N, G, L>(tables, N, jb.cbits, base, cols);
     }
 }
 
@@ -102,15 +107,15 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
         const uint64_t base = (active ? lane : 0) << jb.cbits;
         const int32_t* __restrict__ tables = blob + jb.data_off;
         if constexpr (E) {
-            // Exact jobs: eight limb columns per item, for all of the job's
+            // Exact jobs: K limb columns per item, for all of the job's
             // primes (the finalisation reduces them per prime).
-            int64_t cols[kExactColumns];
+            int64_t cols[K];
 #pragma unroll
-            for (int c = 0; c < kExactColumns; ++c) cols[c] = 0;
+            for (int c = 0; c < K; ++c) cols[c] = 0;
             switch (jb.n) {
 #define PHIP_EXACT_CASE(N) \
     case N:                \
-        run_item_exact<N, G>(tables, jb, base, cols); \
+        run_item_exact<N, G, K>(tables, jb, base, cols); \
         break;
                 PHIP_EXACT_CASE(5) PHIP_EXACT_CASE(6) PHIP_EXACT_CASE(7) PHIP_EXACT_CASE(8)
                 PHIP_EXACT_CASE(9) PHIP_EXACT_CASE(10) PHIP_EXACT_CASE(11) PHIP_EXACT_CASE(12)
@@ -128,11 +133,11 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
                     break;
             }
 #pragma unroll
-            for (int c = 0; c < kExactColumns; ++c) {
+            for (int c = 0; c < K; ++c) {
                 // Column sums of 64 lanes in two's complement: |sum| <
                 // 2^(37 + cbits).
                 const uint64_t s = wave_reduce_sum_u64(active ? (uint64_t)cols[c] : 0u);
-                if (lane_id == 0) item_partials[(size_t)idx * kMaxSet + c] = s;
+                if (lane_id == 0) item_partials[(size_t)idx * kItemSlots + c] = s;
             }
         } else {
         int32_t p[K];
@@ -171,7 +176,7 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
             // Zero-skip lanes hold terms of any lane's chunk (inactive
             // lanes queue nothing but may take queued terms).
             const uint64_t s = wave_reduce_sum_u64((Z || active) ? (uint64_t)(int64_t)r[k] : 0u);
-            if (lane_id == 0) item_partials[(size_t)idx * kMaxSet + k] = s;
+            if (lane_id == 0) item_partials[(size_t)idx * kItemSlots + k] = s;
         }
         }
     }
@@ -184,8 +189,8 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
 #if (PHIP_BATCH_PAIRED + PHIP_BATCH_SKIP_ZEROS + PHIP_BATCH_EXACT) > 1
 #error "a class either pairs states, skips zero terms or accumulates exactly"
 #elif PHIP_BATCH_EXACT
-#if PHIP_BATCH_SET != 8
-#error "exact classes hold eight limb columns per item"
+#if PHIP_BATCH_SET != 8 && PHIP_BATCH_SET != 10 && PHIP_BATCH_SET != 12
+#error "exact classes hold 8, 10 or 12 limb columns per item"
 #endif
 #define PHIP_PAIRED_SUFFIX _exact
 #elif PHIP_BATCH_PAIRED

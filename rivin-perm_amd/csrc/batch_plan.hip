@@ -10,6 +10,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "batch.h"
@@ -40,21 +41,23 @@ __global__ __launch_bounds__(256) void batch_finalize_kernel(const BatchJob* __r
     const BatchJob jb = jobs[sl.job];
     const uint64_t p = sl.p;
     if (jb.variant == kVariantExact) {
-        // The job's items hold eight limb columns of exact signed sums,
+        // The job's items hold jb.set limb columns of exact signed sums,
         // |column| < 2^(37 + cbits) per item (cbits <= 24), so runs of
         // 2^(24 - cbits) items add exactly in int64 (< 2^61) before each
         // reduction.  Column c weighs 2^(31 c).
+        const int ncols = jb.set;
         const uint32_t run_len = 1u << (jb.cbits < 24 ? 24 - jb.cbits : 0);
-        int64_t run[kExactColumns] = {};
-        uint64_t col[kExactColumns] = {};
+        int64_t run[kExactMaxColumns] = {};
+        uint64_t col[kExactMaxColumns] = {};
         uint32_t in_run = 0;
         for (uint32_t i = threadIdx.x; i < jb.n_items; i += 256) {
-            const uint64_t* part = item_partials + (size_t)(jb.first_item + i) * kMaxSet;
+            const uint64_t* part = item_partials + (size_t)(jb.first_item + i) * kItemSlots;
 #pragma unroll
-            for (int c = 0; c < kExactColumns; ++c) run[c] += (int64_t)part[c];
+            for (int c = 0; c < kExactMaxColumns; ++c)
+                if (c < ncols) run[c] += (int64_t)part[c];
             if (++in_run == run_len) {
 #pragma unroll
-                for (int c = 0; c < kExactColumns; ++c) {
+                for (int c = 0; c < kExactMaxColumns; ++c) {
                     const int64_t v = run[c] % (int64_t)p;
                     col[c] += (uint64_t)(v < 0 ? v + (int64_t)p : v);
                     run[c] = 0;
@@ -65,7 +68,7 @@ __global__ __launch_bounds__(256) void batch_finalize_kernel(const BatchJob* __r
         uint64_t s = 0, w = 1 % p;
         const uint64_t step = (1ull << 31) % p;
 #pragma unroll
-        for (int c = 0; c < kExactColumns; ++c) {
+        for (int c = 0; c < kExactMaxColumns; ++c) {
             const int64_t v = run[c] % (int64_t)p;
             const uint64_t r = (col[c] + (uint64_t)(v < 0 ? v + (int64_t)p : v)) % p;
             s += (r * w) % p;
@@ -82,7 +85,7 @@ __global__ __launch_bounds__(256) void batch_finalize_kernel(const BatchJob* __r
     int64_t run = 0;
     uint32_t in_run = 0;
     for (uint32_t i = threadIdx.x; i < jb.n_items; i += 256) {
-        run += (int64_t)item_partials[(size_t)(jb.first_item + i) * kMaxSet + sl.k];
+        run += (int64_t)item_partials[(size_t)(jb.first_item + i) * kItemSlots + sl.k];
         if (++in_run == 65536u) {
             const int64_t v = run % (int64_t)p;
             s += (uint64_t)(v < 0 ? v + (int64_t)p : v);
@@ -112,11 +115,17 @@ BatchLaunchFn launcher_for(int group, int set, int variant) {
     PHIP_LAUNCHER_PAIRED(4, 4) PHIP_LAUNCHER_PAIRED(4, 8) PHIP_LAUNCHER_PAIRED(6, 4) PHIP_LAUNCHER_PAIRED(6, 8)
     PHIP_LAUNCHER_SKIP(4, 2) PHIP_LAUNCHER_SKIP(4, 4) PHIP_LAUNCHER_SKIP(4, 8)
     PHIP_LAUNCHER_SKIP(6, 2) PHIP_LAUNCHER_SKIP(6, 4) PHIP_LAUNCHER_SKIP(6, 8)
-    if (set == kExactColumns && variant == kVariantExact) {
-        if (group == 2) return launch_batch_g2_k8_exact;
-        if (group == 3) return launch_batch_g3_k8_exact;
-        if (group == 4) return launch_batch_g4_k8_exact;
-        if (group == 6) return launch_batch_g6_k8_exact;
+    if (variant == kVariantExact) {
+        if (group == 2 && set == 8) return launch_batch_g2_k8_exact;
+        if (group == 3 && set == 8) return launch_batch_g3_k8_exact;
+        if (group == 4 && set == 8) return launch_batch_g4_k8_exact;
+        if (group == 6 && set == 8) return launch_batch_g6_k8_exact;
+        if (group == 2 && set == 10) return launch_batch_g2_k10_exact;
+        if (group == 2 && set == 12) return launch_batch_g2_k12_exact;
+        if (group == 3 && set == 10) return launch_batch_g3_k10_exact;
+        if (group == 3 && set == 12) return launch_batch_g3_k12_exact;
+        if (group == 4 && set == 10) return launch_batch_g4_k10_exact;
+        if (group == 4 && set == 12) return launch_batch_g4_k12_exact;
     }
 #undef PHIP_LAUNCHER
 #undef PHIP_LAUNCHER_PAIRED
@@ -140,11 +149,17 @@ int occupancy_for(int group, int set, int variant) {
     PHIP_OCC_PAIRED(4, 4) PHIP_OCC_PAIRED(4, 8) PHIP_OCC_PAIRED(6, 4) PHIP_OCC_PAIRED(6, 8)
     PHIP_OCC_SKIP(4, 2) PHIP_OCC_SKIP(4, 4) PHIP_OCC_SKIP(4, 8)
     PHIP_OCC_SKIP(6, 2) PHIP_OCC_SKIP(6, 4) PHIP_OCC_SKIP(6, 8)
-    if (set == kExactColumns && variant == kVariantExact) {
-        if (group == 2) return occupancy_batch_g2_k8_exact();
-        if (group == 3) return occupancy_batch_g3_k8_exact();
-        if (group == 4) return occupancy_batch_g4_k8_exact();
-        if (group == 6) return occupancy_batch_g6_k8_exact();
+    if (variant == kVariantExact) {
+        if (group == 2 && set == 8) return occupancy_batch_g2_k8_exact();
+        if (group == 3 && set == 8) return occupancy_batch_g3_k8_exact();
+        if (group == 4 && set == 8) return occupancy_batch_g4_k8_exact();
+        if (group == 6 && set == 8) return occupancy_batch_g6_k8_exact();
+        if (group == 2 && set == 10) return occupancy_batch_g2_k10_exact();
+        if (group == 2 && set == 12) return occupancy_batch_g2_k12_exact();
+        if (group == 3 && set == 10) return occupancy_batch_g3_k10_exact();
+        if (group == 3 && set == 12) return occupancy_batch_g3_k12_exact();
+        if (group == 4 && set == 10) return occupancy_batch_g4_k10_exact();
+        if (group == 4 && set == 12) return occupancy_batch_g4_k12_exact();
     }
 #undef PHIP_OCC
 #undef PHIP_OCC_PAIRED
@@ -184,6 +199,55 @@ std::vector<int> split_set(int r) {
 }
 
 int groups_of(int n, int group) { return group == 0 ? n : (n + group - 1) / group; }
+
+// Bit length of the product of the row bounds (0 when a bound is 0): every
+// state's product of row sums, and so of group products, is at most that
+// product in magnitude.
+int bound_bits(const std::vector<uint64_t>& S) {
+    std::vector<uint32_t> x(1, 1);  // little-endian 32-bit words
+    for (uint64_t v : S) {
+        if (v == 0) return 0;
+        const uint64_t lo = v & 0xffffffffu, hi = v >> 32;
+        std::vector<uint32_t> y(x.size() + 2, 0);
+        for (int part = 0; part < 2; ++part) {
+            const uint64_t f = part ? hi : lo;
+            if (!f) continue;
+            uint64_t carry = 0;
+            size_t i = 0;
+            for (; i < x.size(); ++i) {
+                const unsigned __int128 t = (unsigned __int128)x[i] * f + y[i + part] + carry;
+                y[i + part] = (uint32_t)t;
+                carry = (uint64_t)(t >> 32);
+            }
+            for (size_t j = i + part; carry; ++j) {
+                const uint64_t t = (uint64_t)y[j] + carry;
+                y[j] = (uint32_t)t;
+                carry = t >> 32;
+            }
+        }
+        while (y.size() > 1 && y.back() == 0) y.pop_back();
+        x.swap(y);
+    }
+    int bits = 32 * (int)(x.size() - 1);
+    for (uint32_t top = x.back(); top; top >>= 1) ++bits;
+    return bits;
+}
+
+// Limb columns of the exact variant for a packed table, or 0 when the table
+// stays on the modular kernels: 8 for at most eight groups (full products),
+// 10 or 12 for 9..16 groups whose row-bound product is below 2^310 or 2^372
+// (products kept modulo 2^(31 L)).  `primes` is the number of moduli the
+// walk would serve.
+int exact_columns(int groups, int bits, size_t primes) {
+    const Tuning& t = tuning();
+    if (t.exact_min_primes <= 0) return 0;
+    if (groups <= kExactMaxGroups) return primes >= (size_t)t.exact_min_primes ? kExactColumns : 0;
+    if (groups > kExactWideMaxGroups || t.exact_wide_min_primes <= 0 || primes < (size_t)t.exact_wide_min_primes)
+        return 0;
+    if (bits <= 31 * 10) return 10;
+    if (bits <= 31 * kExactMaxColumns) return kExactMaxColumns;
+    return 0;
+}
 
 // Estimated share of the Gray states whose term is nonzero, for a 0/1
 // matrix: the product over rows of 1 - C(d, d/2) / 2^d, the share of sign
@@ -244,11 +308,20 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
         int group;
         uint64_t data_off;
         bool skip_zeros;
+        int bits;  // bound_bits of the table's row bounds
         std::vector<size_t> inputs;
     };
     std::vector<Set> sets;
     std::map<int, std::vector<size_t>> sets_of_key;
     std::vector<int32_t> table;
+    // Every modulus above twice a matrix's largest |entry| sees the matrix
+    // itself as its centred table.  The first such input of a matrix plans
+    // the table; later ones join its set without planning it again.
+    struct Plain {
+        uint64_t maxabs;
+        size_t set;
+    };
+    std::map<std::tuple<int, const int64_t*, int>, Plain> plain_of;
     for (size_t i = 0; i < in.size(); ++i) {
         const PlanInput& x = in[i];
         if (x.n < kBatchMinOrder || x.n > kBatchMaxOrder) {
@@ -259,8 +332,24 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
             set_error("batch moduli must be odd and lie in [3, 2^31)");
             return -1;
         }
+        const auto pkey = std::make_tuple(x.key, x.a, x.n);
+        auto pit = plain_of.find(pkey);
+        if (pit == plain_of.end()) {
+            uint64_t m = 0;
+            for (size_t e = 0; e < (size_t)x.n * x.n; ++e) {
+                const int64_t v = x.a[e];
+                m = std::max(m, v < 0 ? 0 - (uint64_t)v : (uint64_t)v);
+            }
+            pit = plain_of.emplace(pkey, Plain{m, SIZE_MAX}).first;
+        }
+        const bool plain = pit->second.maxabs <= (uint64_t)(x.q - 1) / 2;
+        if (plain && pit->second.set != SIZE_MAX) {
+            sets[pit->second.set].inputs.push_back(i);
+            continue;
+        }
         std::vector<int> order;
-        const int group = plan_rows(centred_row_bounds(x.a, x.n, (int64_t)x.q), order);
+        const std::vector<uint64_t> bounds = centred_row_bounds(x.a, x.n, (int64_t)x.q);
+        const int group = plan_rows(bounds, order);
         const size_t words = (size_t)x.n * glynn_stride(x.n, group);
         table.assign(words, 0);
         build_glynn_tables(x.a, x.n, x.q, group, order, table.data());
@@ -281,29 +370,31 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
             const int permille = tuning().skip_zeros_permille;
             const bool skip = permille > 0 && (group == 4 || group == 6) &&
                               nonzero_share(x.a, x.n) * 1000.0 <= (double)permille;
-            sets.push_back({x.key, x.n, group, (uint64_t)out.blob.size(), skip, {}});
+            sets.push_back({x.key, x.n, group, (uint64_t)out.blob.size(), skip, group > 0 ? bound_bits(bounds) : 0, {}});
             sets_of_key[x.key].push_back(found);
             out.blob.insert(out.blob.end(), table.begin(), table.end());
         }
         sets[found].inputs.push_back(i);
+        if (plain && group > 0) pit->second.set = found;
     }
 
     // Pass 2: jobs (one per set chunk) and slots.
     out.slots.resize(in.size());
-    const int exact_min = tuning().exact_min_primes;
     for (const Set& st : sets) {
         size_t next = 0;
-        // A packed table of at most eight groups with enough primes runs
-        // one exact-accumulation walk for all of them: the walk costs
-        // about as much as the modular walk of a set of six primes, and
-        // the residues come from its exact sums.
-        if (exact_min > 0 && st.group > 0 && !st.skip_zeros && groups_of(st.n, st.group) <= kExactMaxGroups &&
-            st.inputs.size() >= (size_t)exact_min) {
+        // A packed table with enough primes runs one exact-accumulation
+        // walk for all of them (exact_columns): with eight groups the walk
+        // costs about as much as the modular walk of a set of six primes,
+        // and the residues come from its exact sums.
+        const int columns = (st.group > 0 && !st.skip_zeros)
+                                ? exact_columns(groups_of(st.n, st.group), st.bits, st.inputs.size())
+                                : 0;
+        if (columns > 0) {
             BatchJob jb{};
             jb.data_off = st.data_off;
             jb.n = st.n;
             jb.group = st.group;
-            jb.set = kExactColumns;
+            jb.set = columns;
             jb.nprimes = (int32_t)st.inputs.size();
             jb.prime_off = (uint32_t)out.set_p.size();
             jb.variant = kVariantExact;
@@ -413,7 +504,9 @@ int assign_items(PlanHost& out, int items_log2, int min_cbits) {
 
     // Items: per (group, set, variant) class, jobs by descending cost, each
     // job's items contiguous.
-    static const int kClasses[][3] = {{2, 8, 3}, {3, 8, 3}, {4, 8, 3}, {6, 8, 3}, {0, 1, 0},
+    static const int kClasses[][3] = {{2, 8, 3}, {3, 8, 3}, {4, 8, 3}, {6, 8, 3},
+                                      {2, 10, 3}, {2, 12, 3}, {3, 10, 3}, {3, 12, 3}, {4, 10, 3}, {4, 12, 3},
+                                      {0, 1, 0},
                                       {2, 8, 1}, {2, 8, 0}, {2, 4, 1}, {2, 4, 0}, {2, 2, 0}, {2, 1, 0},
                                       {3, 8, 1}, {3, 8, 0}, {3, 4, 1}, {3, 4, 0}, {3, 2, 0}, {3, 1, 0},
                                       {4, 8, 1}, {4, 8, 2}, {4, 8, 0}, {4, 4, 1}, {4, 4, 2}, {4, 4, 0},
@@ -464,7 +557,7 @@ int enqueue_plan(const PlanHost& h, const PlanDevice& d, hipStream_t stream, uns
             return -1;
         }
         PHIP_HIP_TRY(fn(dim3(grid), stream, d.jobs, d.items + cl.first_item, cl.n_items, d.counters + c, d.blob,
-                        d.set_p, d.set_pinv, d.partials + (size_t)cl.first_item * kMaxSet));
+                        d.set_p, d.set_pinv, d.partials + (size_t)cl.first_item * kItemSlots));
     }
     if (ev1) PHIP_HIP_TRY(hipEventRecord(ev1, stream));
     hipLaunchKernelGGL(batch_finalize_kernel, dim3((unsigned)h.slots.size()), dim3(256), 0, stream, d.jobs,
@@ -531,7 +624,7 @@ struct BatchPlan {
     // Jobs, items and partials: the parts that depend on the chunk sizes.
     int upload_items() {
         const PlanHost& h = host;
-        if (d_jobs.alloc(h.jobs.size()) || d_items.alloc(h.items.size()) || d_partials.alloc(h.items.size() * kMaxSet))
+        if (d_jobs.alloc(h.jobs.size()) || d_items.alloc(h.items.size()) || d_partials.alloc(h.items.size() * kItemSlots))
             return -1;
         PHIP_HIP_TRY(hipMemcpy(d_jobs.ptr, h.jobs.data(), h.jobs.size() * sizeof(BatchJob), hipMemcpyHostToDevice));
         PHIP_HIP_TRY(hipMemcpy(d_items.ptr, h.items.data(), h.items.size() * sizeof(uint2), hipMemcpyHostToDevice));
@@ -653,7 +746,7 @@ int run_plan_in_context(DeviceContext* ctx, const std::vector<PlanInput>& in, in
     d.set_p = (int32_t*)ctx->buffer(19, h.set_p.size() * sizeof(int32_t));
     d.set_pinv = (uint32_t*)ctx->buffer(20, h.set_pinv.size() * sizeof(uint32_t));
     d.slots = (BatchSlot*)ctx->buffer(21, h.slots.size() * sizeof(BatchSlot));
-    d.partials = (uint64_t*)ctx->buffer(22, h.items.size() * kMaxSet * sizeof(uint64_t));
+    d.partials = (uint64_t*)ctx->buffer(22, h.items.size() * kItemSlots * sizeof(uint64_t));
     d.counters = (uint32_t*)ctx->buffer(23, 32 * sizeof(uint32_t));
     d.residues = (int64_t*)ctx->buffer(24, h.slots.size() * sizeof(int64_t));
     auto* h_res = (int64_t*)ctx->host_buffer(4, h.slots.size() * sizeof(int64_t));

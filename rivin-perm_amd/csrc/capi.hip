@@ -145,6 +145,10 @@ int phip_set_tuning(int key, int64_t value) {
             if (value < 0 || value > 1024) break;
             t.exact_min_primes = (int)value;
             return 0;
+        case PHIP_TUNE_EXACT_WIDE_MIN_PRIMES:
+            if (value < 0 || value > 1024) break;
+            t.exact_wide_min_primes = (int)value;
+            return 0;
         default:
             phip::set_error("unknown tuning key");
             return -1;
@@ -166,6 +170,7 @@ int64_t phip_get_tuning(int key) {
         case PHIP_TUNE_AFFINE_MIN_TIER: return t.affine_min_tier;
         case PHIP_TUNE_SKIP_ZEROS_PERMILLE: return t.skip_zeros_permille;
         case PHIP_TUNE_EXACT_MIN_PRIMES: return t.exact_min_primes;
+        case PHIP_TUNE_EXACT_WIDE_MIN_PRIMES: return t.exact_wide_min_primes;
         default:
             phip::set_error("unknown tuning key");
             return -1;

@@ -1,5 +1,6 @@
-// Exact products of up to eight 31-bit magnitudes in radix-2^31 limbs, for
-// the exact-accumulation Glynn sink (glynn31.h, ExactSink).
+// Exact products of 31-bit magnitudes in radix-2^31 limbs, for the
+// exact-accumulation Glynn sink (glynn31.h, ExactSink): eight values in
+// eight limbs, and up to sixteen values modulo 2^(31 L).
 //
 // A value is an array of limbs l[0..k), value = sum l[i] 2^(31 i), every
 // limb in [0, 2^31).  Multiplying two such arrays is schoolbook with 64-bit
@@ -19,8 +20,10 @@
 
 #ifndef __HIPCC__
 #define PHIP_LIMB_FN static inline
+#define PHIP_LIMB_MEMBER static inline
 #else
 #define PHIP_LIMB_FN __host__ __device__ __forceinline__
+#define PHIP_LIMB_MEMBER __host__ __device__ __forceinline__ static
 #endif
 
 namespace phip {
@@ -111,5 +114,89 @@ PHIP_LIMB_FN void limb_product8(const uint32_t (&m)[8], uint32_t (&out)[8]) {
     limb_mul_2x2(p2, p3, y);
     limb_mul_4x4(x, y, out);
 }
+
+// (a * b) mod 2^(31 LO) in LO limbs, for a of LA limbs and b of LB limbs,
+// LO <= LA + LB.  Column k is the sum of the limb products a_i b_j with
+// i + j = k plus the carry out of column k - 1, in one unsigned 64-bit
+// word t.  A limb product is at most (2^31 - 1)^2 = 2^62 - 2^32 + 1, so
+// four of them fit on top of a word below 2^33 and three on top of one
+// below 2^62.  A column whose products do not all fit is split: the low 31
+// bits of t stay in t (four more products fit) and t >> 31 goes to the
+// column's carry-out.  At most eight products per column (the shorter
+// operand has at most eight limbs) means at most two splits, so a carry-out
+// is a sum of at most three words shifted right by 31, below 3 * 2^33; a
+// column entered with such a carry takes three products before its first
+// split, one entered with the carry of an unsplit column (below 2^33) four.
+// Every branch below depends on loop indices only and folds away once the
+// loops are unrolled.  The top limb of a full product (LO = LA + LB) is its
+// carry, below 2^31 because the product is below 2^(31 LO).
+template <int LA, int LB, int LO>
+PHIP_LIMB_FN void limb_mul_trunc(const uint32_t (&a)[LA], const uint32_t (&b)[LB], uint32_t (&out)[LO]) {
+    static_assert(LO <= LA + LB, "more output limbs than the product has");
+    static_assert(LA <= 8 || LB <= 8, "at most eight limb products per column");
+    uint64_t carry = 0;
+    bool carry_small = true;
+#pragma unroll
+    for (int k = 0; k < LO; ++k) {
+        if (k == LA + LB - 1) {
+            // No products in the top column of a full product.
+            out[k] = (uint32_t)carry;
+            break;
+        }
+        uint64_t t = carry;
+        uint64_t spill = 0;
+        int room = carry_small ? 4 : 3;
+        bool split = false;
+        bool empty = k == 0;
+#pragma unroll
+        for (int i = 0; i < LA; ++i) {
+            const int j = k - i;
+            if (j < 0 || j >= LB) continue;
+            if (room == 0) {
+                spill += t >> 31;
+                t &= kLimbMask;
+                room = 4;
+                split = true;
+            }
+            t = empty ? limb_mul(a[i], b[j]) : limb_mad(a[i], b[j], t);
+            empty = false;
+            --room;
+        }
+        out[k] = (uint32_t)t & kLimbMask;
+        carry = (t >> 31) + spill;
+        carry_small = !split;
+    }
+}
+
+// Product of m[Lo..Hi) (each below 2^31) modulo 2^(31 L), in
+// min(Hi - Lo, L) limbs: the product of n values below 2^31 is below
+// 2^(31 n).  Split at the largest power of two below the count, so eight
+// values multiply as limb_product8 does.
+constexpr int limb_split(int count) {
+    int s = 1;
+    while (2 * s < count) s *= 2;
+    return s;
+}
+
+template <int Lo, int Hi, int L>
+struct LimbProduct {
+    static constexpr int kCount = Hi - Lo;
+    static constexpr int kLimbs = kCount < L ? kCount : L;
+    template <int M>
+    PHIP_LIMB_MEMBER void run(const uint32_t (&m)[M], uint32_t (&out)[kLimbs]) {
+        static_assert(Hi <= M && kCount >= 1, "range outside the values");
+        if constexpr (kCount == 1) {
+            out[0] = m[Lo];
+        } else {
+            constexpr int kMid = Lo + limb_split(kCount);
+            using A = LimbProduct<Lo, kMid, L>;
+            using B = LimbProduct<kMid, Hi, L>;
+            uint32_t x[A::kLimbs], y[B::kLimbs];
+            A::run(m, x);
+            B::run(m, y);
+            limb_mul_trunc(x, y, out);
+        }
+    }
+};
 
 }  // namespace phip

@@ -390,33 +390,44 @@ struct PrimeSetSink {
     }
 };
 
-// Exact accumulation (any number of primes): each state's product of at
-// most eight exact group products (|g| < 2^31) is formed exactly, as a
-// magnitude in eight radix-2^31 limbs (exact_limbs.h) and a sign, and every
-// limb is added with that sign to its own int64 column accumulator.  No
-// modular arithmetic runs in the walk: the lane's columns hold its exact
-// signed sum of terms, sum_c acc[c] 2^(31 c), and any prime's residue
-// follows from them at the end.  Per state that is the exact pair products
-// (4 multiplies), the two 2 x 2 and one 4 x 4 limb products (24 multiplies
-// and 13 limb splits), the magnitudes and the sign, and 8 accumulations,
-// for every prime at once.  Every limb is below 2^31, so after 2^cbits
-// states |acc[c]| < 2^(31 + cbits).
-template <int NR, int G>
+// Exact accumulation (any number of primes): each state's product of the
+// exact group products (|g| < 2^31) is formed exactly, as a magnitude in
+// radix-2^31 limbs (exact_limbs.h) and a sign, and every limb is added with
+// that sign to its own int64 column accumulator.  No modular arithmetic
+// runs in the walk: the lane's columns hold its exact signed sum of terms,
+// sum_c acc[c] 2^(31 c), and any prime's residue follows from them at the
+// end.
+//
+// L = 8 takes at most eight groups: the product of eight values below 2^31
+// is below 2^248, eight limbs, with no further condition.  Per state that
+// is the exact pair products (4 multiplies), the two 2 x 2 and one 4 x 4
+// limb products (24 multiplies and 13 limb splits), the magnitudes and the
+// sign, and 8 accumulations, for every prime at once.
+//
+// L = 10 or 12 takes up to sixteen groups, whose product can need sixteen
+// limbs; the walk keeps it modulo 2^(31 L) (LimbProduct), which is the
+// exact product whenever the product of the table's row bounds (sum_j
+// |a_ij|, each bounding |row sum| in every state) is below 2^(31 L).  The
+// planner sends a table here only then.
+//
+// Every limb is below 2^31, so after 2^cbits states |acc[c]| < 2^(31 + cbits).
+template <int NR, int G, int L>
 struct ExactSink {
     static_assert(G > 0, "exact accumulation needs exact packed row sums");
     using Rows = PackedRows<NR, G>;
     static constexpr int kGroups = Rows::kGroups;
-    static_assert(kGroups <= 8, "exact accumulation takes at most eight groups");
-    int64_t acc[8];
+    static_assert(L == 8 ? kGroups <= 8 : kGroups <= 16, "too many groups for the exact sink");
+    int64_t acc[L];
 
     template <bool ODD>
     __device__ __forceinline__ void state(const Rows& rs) {
         int32_t g[kGroups];
         rs.groups(g, std::make_integer_sequence<int, kGroups>{});
-        uint32_t m[8];
+        constexpr int kValues = L == 8 ? 8 : kGroups;
+        uint32_t m[kValues];
         uint32_t sign = ODD ? 0x80000000u : 0u;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
+        for (int k = 0; k < kValues; ++k) {
             if (k < kGroups) {
                 sign ^= (uint32_t)g[k];
                 m[k] = (uint32_t)(g[k] < 0 ? -g[k] : g[k]);
@@ -424,11 +435,19 @@ struct ExactSink {
                 m[k] = 1;
             }
         }
-        uint32_t z[8];
-        limb_product8(m, z);
         const int32_t e = ((int32_t)sign >> 31) | 1;
+        if constexpr (L == 8) {
+            uint32_t z[8];
+            limb_product8(m, z);
 #pragma unroll
-        for (int c = 0; c < 8; ++c) acc[c] = mad_exact(acc[c], (int32_t)z[c], e);
+            for (int c = 0; c < 8; ++c) acc[c] = mad_exact(acc[c], (int32_t)z[c], e);
+        } else {
+            using P = LimbProduct<0, kGroups, L>;
+            uint32_t z[P::kLimbs];
+            P::run(m, z);
+#pragma unroll
+            for (int c = 0; c < P::kLimbs; ++c) acc[c] = mad_exact(acc[c], (int32_t)z[c], e);
+        }
     }
 };
 
@@ -659,17 +678,17 @@ __device__ __forceinline__ void glynn_chunk_set(const int32_t* __restrict__ tabl
     for (int k = 0; k < K; ++k) out[k] = lane_value(sink.acc[k], neg, sink.negp[k], sink.pinv[k]);
 }
 
-// The lane's exact signed sum of terms as eight int64 columns (value
-// sum_c out[c] 2^(31 c)), for tables of at most eight groups.
-template <int NR, int G>
+// The lane's exact signed sum of terms as L int64 columns (value
+// sum_c out[c] 2^(31 c)); ExactSink states which tables each L takes.
+template <int NR, int G, int L>
 __device__ __forceinline__ void glynn_chunk_exact(const int32_t* __restrict__ tables, int n, int cbits,
-                                                  uint64_t base, int64_t (&out)[8]) {
-    ExactSink<NR, G> sink;
+                                                  uint64_t base, int64_t (&out)[L]) {
+    ExactSink<NR, G, L> sink;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) sink.acc[c] = 0;
+    for (int c = 0; c < L; ++c) sink.acc[c] = 0;
     const bool neg = gray_walk<NR, G>(tables, n, cbits, base, 0, sink);
 #pragma unroll
-    for (int c = 0; c < 8; ++c) out[c] = neg ? -sink.acc[c] : sink.acc[c];
+    for (int c = 0; c < L; ++c) out[c] = neg ? -sink.acc[c] : sink.acc[c];
 }
 
 // glynn_chunk_set through ZeroSkipSink.  Every lane of the wavefront must

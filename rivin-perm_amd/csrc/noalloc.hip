@@ -34,7 +34,7 @@
 #include "residues.h"
 
 // 8 MiB: the default plan of one matrix with up to 64 moduli needs about
-// 1 MiB (2^14 work items of 64 bytes of partials, 6.4 KB tables per
+// 1.6 MiB (2^14 work items of 96 bytes of partials, 6.4 KB tables per
 // distinct table).
 constexpr size_t kScratchBytes = size_t(8) << 20;
 
@@ -126,7 +126,7 @@ extern "C" int phip_permanent_residues_noalloc(const int64_t* A, int64_t prime_s
     const size_t o_p = carve(off, h.set_p.size() * sizeof(int32_t));
     const size_t o_pinv = carve(off, h.set_pinv.size() * sizeof(uint32_t));
     const size_t o_slots = carve(off, h.slots.size() * sizeof(BatchSlot));
-    const size_t o_part = carve(off, h.items.size() * kMaxSet * sizeof(uint64_t));
+    const size_t o_part = carve(off, h.items.size() * kItemSlots * sizeof(uint64_t));
     const size_t o_cnt = carve(off, 32 * sizeof(uint32_t));
     const size_t o_res = carve(off, h.slots.size() * sizeof(int64_t));
     if (off > kScratchBytes) {

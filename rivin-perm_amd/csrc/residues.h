@@ -25,6 +25,7 @@ struct Tuning {
     int affine_min_tier = 8;      // smallest register tier (8..40, multiple of 4) of the affine Glynn kernels
     int skip_zeros_permille = 250;
     int exact_min_primes = 6;     // fewest primes of a table of <= 8 groups that take the exact variant (0 off)
+    int exact_wide_min_primes = 6;  // the same for 9..16 groups (row-bound product below 2^372)
 };
 Tuning& tuning();
 

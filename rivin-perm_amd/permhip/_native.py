@@ -111,6 +111,7 @@ TUNE_WIDE_MIN_TIER = 7
 TUNE_AFFINE_MIN_TIER = 8
 TUNE_SKIP_ZEROS_PERMILLE = 9
 TUNE_EXACT_MIN_PRIMES = 10
+TUNE_EXACT_WIDE_MIN_PRIMES = 11
 BATCH_BLOCKS_PER_CU = 1
 BATCH_USE_GRAPH = 2
 BATCH_ITEMS_LOG2 = 3

@@ -38,6 +38,7 @@ BATCH_CLASSES = (
     + tuple((g, k, 1) for g in (2, 3, 4, 6) for k in (4, 8))
     + tuple((g, k, 2) for g in (4, 6) for k in (2, 4, 8))
     + tuple((g, 8, 3) for g in (2, 3, 4, 6))
+    + tuple((g, k, 3) for g in (2, 3, 4) for k in (10, 12))
 )
 PLAIN_SOURCES = (
     "device_context.hip",

@@ -80,8 +80,8 @@ class _Request:
 # Largest order the device residue kernels take (the limit of
 # perm_c_omp.c:885-888 and perm_modp_cuda.cu:744-747).
 MAX_ORDER = 63
-# Bytes of partial sums one work item holds (kMaxSet 8-byte slots).
-_ITEM_PARTIAL_BYTES = 8 * 8
+# Bytes of partial sums one work item holds (kItemSlots 8-byte slots).
+_ITEM_PARTIAL_BYTES = 12 * 8
 
 
 def _plan_integer(matrix) -> tuple:

@@ -1,8 +1,11 @@
 """The exact-accumulation kernels (ExactSink: each Gray state's product of
-at most eight exact group products formed exactly in radix-2^31 limbs, one
-walk for every prime of a table) against the oracle, bit for bit, and
-against the modular kernels they replace (PHIP_TUNE_EXACT_MIN_PRIMES = 0).
-Every case names its seed, order, entry range and prime count."""
+the exact group products formed in radix-2^31 limbs, one walk for every
+prime of a table) against the oracle, bit for bit, and against the modular
+kernels they replace (PHIP_TUNE_EXACT_MIN_PRIMES = 0): tables of at most
+eight groups (full products in 8 limbs) and of 9..16 groups (products kept
+modulo 2^(31 L), L = 10 or 12, exact while the row-bound product is below
+2^(31 L)).  Every case names its seed, order, entry range and prime
+count."""
 
 import random
 
@@ -119,3 +122,72 @@ def test_headline_goes_exact(native):
     assert _classes(native, A, primes) == {(4, 8, 3)}
     assert native.permanent_residues(A, primes) == _with_tuning(
         native, native.TUNE_EXACT_MIN_PRIMES, 0, lambda: native.permanent_residues(A, primes))
+
+
+def _row_bound_bits(A):
+    prod = 1
+    for v in np.abs(A).sum(axis=1).tolist():
+        prod *= int(v)
+    return prod.bit_length()
+
+
+# (order, entry bound, group size, limb columns): 9..16 groups of 2, 3 or 4
+# rows, on both sides of the 2^310 limit of the 10-column class.
+WIDE_SHAPES = [(17, 1500, 2, 10), (20, 1000, 2, 10), (26, 100, 3, 10), (28, 100, 2, 10), (30, 300, 2, 12),
+               (32, 20, 3, 10), (33, 6, 4, 10), (34, 50, 3, 12)]
+
+
+@pytest.mark.parametrize("n,bound,group,columns", WIDE_SHAPES)
+def test_wide_exact_matches_oracle_and_modular(native, oracle_c, n, bound, group, columns):
+    rng = np.random.default_rng(1900 + 31 * n + bound)
+    A = rng.integers(-bound, bound + 1, (n, n)).astype(np.int64)
+    for k in (8, 10, 16):
+        primes = PRIMES[:k]
+        assert _classes(native, A, primes) == {(group, columns, 3)}, (n, bound, k, _row_bound_bits(A))
+        got = native.permanent_residues(A, primes)
+        modular = _with_tuning(native, native.TUNE_EXACT_MIN_PRIMES, 0, lambda: native.permanent_residues(A, primes))
+        assert got == modular, (n, bound, k)
+        if n <= 20:
+            assert got == [oracle_c.permanent_mod_p(A.tolist(), p, 8) for p in primes], (n, bound, k)
+
+
+@pytest.mark.parametrize("entry,columns", [(76, 10), (77, 12), (350, 12)])
+def test_wide_exact_at_the_truncation_limit(native, entry, columns):
+    """28 x 28 tables of entries +-entry: every row bound is 28 * entry and
+    the largest state product equals the row-bound product, which sits just
+    below 2^310 (76), just above it (77: 12 columns) and just below 2^372
+    (350).  The product kept modulo 2^(31 L) must still be exact."""
+    rng = np.random.default_rng(entry)
+    A = (rng.choice((-1, 1), (28, 28)) * entry).astype(np.int64)
+    bits = _row_bound_bits(A)
+    assert bits <= 31 * columns and (columns == 10 or bits > 310)
+    primes = PRIMES[:10]
+    assert _classes(native, A, primes) == {(2, columns, 3)}
+    assert native.permanent_residues(A, primes) == _with_tuning(
+        native, native.TUNE_EXACT_MIN_PRIMES, 0, lambda: native.permanent_residues(A, primes))
+
+
+def test_wide_exact_declines_large_bounds_and_few_primes(native):
+    """Row-bound products above 2^372 and tables with fewer primes than
+    PHIP_TUNE_EXACT_WIDE_MIN_PRIMES stay on the modular kernels (host
+    planning only)."""
+    rng = np.random.default_rng(3)
+    big = rng.integers(-1000, 1001, (32, 32)).astype(np.int64)
+    assert _row_bound_bits(big) > 372
+    assert all(c[2] != 3 for c in _classes(native, big, PRIMES[:16]))
+    A = rng.integers(-100, 101, (28, 28)).astype(np.int64)
+    assert all(c[2] != 3 for c in _classes(native, A, PRIMES[:5]))
+    assert _classes(native, A, PRIMES[:6]) == {(2, 10, 3)}
+    assert _with_tuning(native, native.TUNE_EXACT_WIDE_MIN_PRIMES, 0,
+                        lambda: all(c[2] != 3 for c in _classes(native, A, PRIMES[:10])))
+
+
+def test_wide_exact_long_chunks(native):
+    """Chunks of 2^24 states per lane (the exact variant's cap) on a 14-group
+    table: the lane columns grow to their largest."""
+    rng = np.random.default_rng(78)
+    A = rng.integers(-100, 101, (26, 26)).astype(np.int64)
+    primes = PRIMES[:10]
+    want = _with_tuning(native, native.TUNE_EXACT_MIN_PRIMES, 0, lambda: native.permanent_residues(A, primes))
+    got = _with_tuning(native, native.TUNE_MIN_CBITS, 30, lambda: native.permanent_residues(A, primes))
+    assert got == want

@@ -25,6 +25,22 @@ def _random_matrix(rng, n, lo, hi):
     return [[rng.randrange(lo, hi + 1) for _ in range(n)] for _ in range(n)]
 
 
+@pytest.fixture(params=["default", "modular"])
+def engine(request, native):
+    """Runs a test twice: with the default planning (tables with enough
+    primes take the exact-accumulation walk) and with the exact variant off,
+    so the modular prime-set kernels stay covered."""
+    if request.param == "default":
+        yield request.param
+        return
+    old = native.get_tuning(native.TUNE_EXACT_MIN_PRIMES)
+    native.set_tuning(native.TUNE_EXACT_MIN_PRIMES, 0)
+    try:
+        yield request.param
+    finally:
+        native.set_tuning(native.TUNE_EXACT_MIN_PRIMES, old)
+
+
 @pytest.mark.parametrize("n", list(range(0, 15)))
 def test_all_moduli_small_orders(native, oracle_c, n):
     rng = random.Random(20261203 + n)
@@ -91,7 +107,7 @@ def _primes_from(start, count):
         (7, -10, 10),
     ],
 )
-def test_prime_sets_against_oracle(native, oracle_c, n, lo, hi, k):
+def test_prime_sets_against_oracle(native, oracle_c, engine, n, lo, hi, k):
     """Primes whose centred tables coincide share one Gray walk (sets of 2, 4
     or 8, padded); every residue must equal the oracle's."""
     rng = random.Random(4000 + 31 * n + k)
@@ -102,7 +118,7 @@ def test_prime_sets_against_oracle(native, oracle_c, n, lo, hi, k):
         assert r == oracle_c.permanent_mod_p(A, p, 8), (n, lo, hi, k, p)
 
 
-def test_prime_sets_mixed_tables(native, oracle_c):
+def test_prime_sets_mixed_tables(native, oracle_c, engine):
     """Moduli below twice the largest entry get their own tables; the rest
     share one.  Order of the answers follows the caller's modulus order."""
     rng = random.Random(77)
@@ -258,7 +274,7 @@ def _primes_below(limit, count):
 
 
 @pytest.mark.parametrize("n,lo,hi", [(16, -1, 1), (20, -10, 10), (23, -100, 100), (21, -3, 3), (25, -10, 10)])
-def test_paired_state_sets_against_oracle(native, oracle_c, n, lo, hi):
+def test_paired_state_sets_against_oracle(native, oracle_c, engine, n, lo, hi):
     """Sets of 8 and 4 primes just below the paired bound (paired kernels)
     and just above it (per-state kernels) agree with the oracle; groups of
     every row-sum representation and odd group counts are covered."""
@@ -273,8 +289,21 @@ def test_paired_state_sets_against_oracle(native, oracle_c, n, lo, hi):
 
 
 def test_paired_classes_are_planned(native):
-    """The plan routes a set of 8 primes from 2^30 to a paired class and a
-    set from 2^31 - 1 down to a per-state class."""
+    """With the exact variant off, the plan routes a set of 8 primes from
+    2^30 to a paired class and a set from 2^31 - 1 down to a per-state
+    class; with it on, both run as one exact walk."""
+    rng = np.random.default_rng(3)
+    A = rng.integers(-10, 11, (20, 20)).astype(np.int64)
+    assert set(native.batch_plan_classes([A], [_primes_from(1 << 30, 8)])) == {(4, 8, 3)}
+    old = native.get_tuning(native.TUNE_EXACT_MIN_PRIMES)
+    native.set_tuning(native.TUNE_EXACT_MIN_PRIMES, 0)
+    try:
+        _paired_classes(native)
+    finally:
+        native.set_tuning(native.TUNE_EXACT_MIN_PRIMES, old)
+
+
+def _paired_classes(native):
     rng = np.random.default_rng(3)
     A = rng.integers(-10, 11, (20, 20)).astype(np.int64)
     low = native.Batch([A], [_primes_from(1 << 30, 8)])
