@@ -215,8 +215,9 @@ int phip_batch_set_option(void *plan, int option, int64_t value);
  * sized for, [13] workgroups per CU, [14] kernel variant of the first
  * launch: 0 plain, 1 pairs consecutive states (prime sets below
  * 1.25 * 2^30), 2 skips zero terms (0/1 tables whose states are mostly
- * zero), [15] fewest workgroups per CU that any of the plan's kernels keeps
- * resident (register and LDS use; -1 if unknown).  Returns the number of
+ * zero), [15] most workgroups per CU that any of the plan's kernels keeps
+ * resident (register and LDS use; each class's grid is capped at its own
+ * count; -1 if unknown).  Returns the number of
  * values written or -1. */
 int phip_batch_stats(void *plan, double *out, int len);
 
@@ -239,7 +240,8 @@ int phip_batch_destroy(void *plan);
 #define PHIP_TUNE_LANES_LOG2 1       /* single-matrix launch: target lanes (log2) */
 #define PHIP_TUNE_MIN_CBITS 2        /* single-matrix launch: minimum chunk bits */
 #define PHIP_TUNE_BATCH_ITEMS_LOG2 3 /* batch: target work items (log2), read at create */
-#define PHIP_TUNE_BATCH_BLOCKS 4     /* batch: default persistent workgroups per CU */
+#define PHIP_TUNE_BATCH_BLOCKS 4     /* batch: default persistent workgroups per CU (default 4; each class's
+                                        grid is capped at what its kernel keeps resident) */
 #define PHIP_TUNE_NECKLACE_MIN_ORDER 5 /* Schur: smallest prime order using necklace orbits (5..64) */
 #define PHIP_TUNE_NECKLACE_DEPTH 6   /* Schur necklaces: largest FKM prefix depth of the tasks (2..30) */
 #define PHIP_TUNE_WIDE_MIN_TIER 7    /* wide-modulus Glynn: smallest row-register tier (8..64, a multiple of 4) */

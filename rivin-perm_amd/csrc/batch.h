@@ -68,7 +68,18 @@ struct BatchClass {
     int variant;
     uint32_t first_item;
     uint32_t n_items;
+    int max_n;  // largest order of the class's jobs (sizes the zero-skip queues)
 };
+
+// Rows of one wavefront's zero-skip queue for tables of order n: the packed
+// row words and the sign (ZeroSkipSink), kZeroQueueEntries words each.
+constexpr int kZeroQueueEntries = 128;
+constexpr int skip_queue_rows(int n) { return (n + 1) / 2 + 1; }
+// Dynamic LDS of one workgroup (4 wavefronts) of a class's kernel: the
+// zero-skip queues, nothing for the other variants.
+inline unsigned class_lds_bytes(const BatchClass& cl) {
+    return cl.variant == 2 ? 4u * (unsigned)skip_queue_rows(cl.max_n) * kZeroQueueEntries * 4u : 0u;
+}
 
 // One requested residue before planning: a row-major n x n matrix and an odd
 // modulus in [3, 2^31).  Inputs with equal `key` may share a job when their
@@ -119,21 +130,28 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
 int assign_items(PlanHost& out, int items_log2, int min_cbits);
 
 // Enqueues one run: counter reset, one persistent launch per class (between
-// ev0 and ev1 when given), finalisation into d.residues.  Returns 0 or -1.
-int enqueue_plan(const PlanHost& h, const PlanDevice& d, hipStream_t stream, unsigned grid, hipEvent_t ev0,
-                 hipEvent_t ev1);
+// ev0 and ev1 when given; cus x min(blocks_per_cu, what the class's kernel
+// keeps resident) workgroups), finalisation into d.residues.  Returns 0 or -1.
+int enqueue_plan(const PlanHost& h, const PlanDevice& d, hipStream_t stream, int cus, int blocks_per_cu,
+                 hipEvent_t ev0, hipEvent_t ev1);
+
+// Workgroups per CU a class's kernel keeps resident with its dynamic LDS,
+// or -1.
+int class_resident(const BatchClass& cl);
 
 // Builds and runs a plan for `in` in the device context's stream and
 // buffers (the single-matrix entry points); out[i] receives the residue of
 // in[i].  Chunk sizes follow the context tuning.  Returns 0 or -1.
 int run_plan_in_context(DeviceContext* ctx, const std::vector<PlanInput>& in, int64_t* out);
 
-using BatchLaunchFn = hipError_t (*)(dim3 grid, hipStream_t stream, const BatchJob* jobs, const uint2* items,
-                                     uint32_t n_items, uint32_t* counter, const int32_t* blob,
+// Launches a class's persistent kernel; lds_bytes of dynamic LDS per
+// workgroup (class_lds_bytes).
+using BatchLaunchFn = hipError_t (*)(dim3 grid, unsigned lds_bytes, hipStream_t stream, const BatchJob* jobs,
+                                     const uint2* items, uint32_t n_items, uint32_t* counter, const int32_t* blob,
                                      const int32_t* set_p, const uint32_t* set_pinv, uint64_t* item_partials);
 
 #define PHIP_DECLARE_BATCH_LAUNCH(G, K)                                                                    \
-    hipError_t launch_batch_g##G##_k##K(dim3, hipStream_t, const BatchJob*, const uint2*, uint32_t, uint32_t*, \
+    hipError_t launch_batch_g##G##_k##K(dim3, unsigned, hipStream_t, const BatchJob*, const uint2*, uint32_t, uint32_t*, \
                                         const int32_t*, const int32_t*, const uint32_t*, uint64_t*);
 PHIP_DECLARE_BATCH_LAUNCH(0, 1)
 PHIP_DECLARE_BATCH_LAUNCH(2, 1)
@@ -155,7 +173,7 @@ PHIP_DECLARE_BATCH_LAUNCH(6, 8)
 #undef PHIP_DECLARE_BATCH_LAUNCH
 // Paired-state variants (every prime of the set below kPairedPrimeLimit).
 #define PHIP_DECLARE_BATCH_LAUNCH_PAIRED(G, K)                                                                   \
-    hipError_t launch_batch_g##G##_k##K##_paired(dim3, hipStream_t, const BatchJob*, const uint2*, uint32_t,   \
+    hipError_t launch_batch_g##G##_k##K##_paired(dim3, unsigned, hipStream_t, const BatchJob*, const uint2*, uint32_t,   \
                                                uint32_t*, const int32_t*, const int32_t*, const uint32_t*,   \
                                                uint64_t*);
 PHIP_DECLARE_BATCH_LAUNCH_PAIRED(2, 4)
@@ -169,7 +187,7 @@ PHIP_DECLARE_BATCH_LAUNCH_PAIRED(6, 8)
 #undef PHIP_DECLARE_BATCH_LAUNCH_PAIRED
 // Zero-skipping variants (0/1 tables whose states are mostly zero terms).
 #define PHIP_DECLARE_BATCH_LAUNCH_SKIP(G, K)                                                                   \
-    hipError_t launch_batch_g##G##_k##K##_skip(dim3, hipStream_t, const BatchJob*, const uint2*, uint32_t,   \
+    hipError_t launch_batch_g##G##_k##K##_skip(dim3, unsigned, hipStream_t, const BatchJob*, const uint2*, uint32_t,   \
                                              uint32_t*, const int32_t*, const int32_t*, const uint32_t*,   \
                                              uint64_t*);
 PHIP_DECLARE_BATCH_LAUNCH_SKIP(4, 2)
@@ -181,7 +199,7 @@ PHIP_DECLARE_BATCH_LAUNCH_SKIP(6, 8)
 #undef PHIP_DECLARE_BATCH_LAUNCH_SKIP
 // Exact-accumulation variants (any number of primes; L = K limb columns).
 #define PHIP_DECLARE_BATCH_LAUNCH_EXACT(G, K)                                                                 \
-    hipError_t launch_batch_g##G##_k##K##_exact(dim3, hipStream_t, const BatchJob*, const uint2*, uint32_t,  \
+    hipError_t launch_batch_g##G##_k##K##_exact(dim3, unsigned, hipStream_t, const BatchJob*, const uint2*, uint32_t,  \
                                               uint32_t*, const int32_t*, const int32_t*, const uint32_t*,  \
                                               uint64_t*);
 PHIP_DECLARE_BATCH_LAUNCH_EXACT(2, 8)
@@ -196,8 +214,9 @@ PHIP_DECLARE_BATCH_LAUNCH_EXACT(4, 10)
 PHIP_DECLARE_BATCH_LAUNCH_EXACT(4, 12)
 #undef PHIP_DECLARE_BATCH_LAUNCH_EXACT
 
-// Workgroups per CU each class kernel can keep resident (batch.hip), or -1.
-#define PHIP_DECLARE_BATCH_OCC(NAME) int NAME();
+// Workgroups per CU each class kernel can keep resident with lds_bytes of
+// dynamic LDS per workgroup (batch.hip), or -1.
+#define PHIP_DECLARE_BATCH_OCC(NAME) int NAME(unsigned lds_bytes);
 PHIP_DECLARE_BATCH_OCC(occupancy_batch_g0_k1)
 PHIP_DECLARE_BATCH_OCC(occupancy_batch_g2_k1) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g2_k2)
 PHIP_DECLARE_BATCH_OCC(occupancy_batch_g2_k4) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g2_k8)

@@ -75,9 +75,7 @@ __device__ __forceinline__ void run_item_exact(const int32_t* __restrict__ table
     }
 }
 
-// Largest group count over the engine's orders, for the zero-skip queue.
-template <int G>
-constexpr int kMaxGroups = G > 0 ? (kBatchMaxOrder + G - 1) / G : 1;
+static_assert(kZeroQueueEntries == kZeroQueue, "zero-skip queue sizes disagree");
 
 template <int G, int K, bool P, bool Z, bool E>
 __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __restrict__ jobs,
@@ -86,12 +84,15 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
                                                          const int32_t* __restrict__ blob,
                                                          const int32_t* __restrict__ set_p,
                                                          const uint32_t* __restrict__ set_pinv,
-                                                         uint64_t* __restrict__ item_partials) {
+                                                         uint64_t* __restrict__ item_partials,
+                                                         uint32_t queue_words) {
     const int lane_id = threadIdx.x & 63;
     const uint32_t claim = lane_id == 0 ? 1u : 0u;
-    // One zero-skip queue per wavefront (4 per workgroup).
-    __shared__ int32_t queues[Z ? 4 * kMaxGroups<G> * kZeroQueue : 1];
-    int32_t* const queue = queues + (Z ? (threadIdx.x >> 6) * kMaxGroups<G> * kZeroQueue : 0);
+    // One zero-skip queue of queue_words words per wavefront (4 per
+    // workgroup) in dynamic LDS: skip_queue_rows(n) * kZeroQueue words for
+    // the class's largest order n, set by the host.
+    extern __shared__ int32_t queues[];
+    int32_t* const queue = queues + (Z ? (threadIdx.x >> 6) * queue_words : 0);
     for (;;) {
         // Every lane executes the same atomic (lane 0 adds 1, the others 0)
         // so no lane-dependent branch sits in front of the broadcast: with a
@@ -201,15 +202,16 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
 #define PHIP_PAIRED_SUFFIX
 #endif
 
-hipError_t PHIP_LAUNCH_NAME(PHIP_BATCH_GROUP, PHIP_BATCH_SET, PHIP_PAIRED_SUFFIX)(dim3 grid, hipStream_t stream, const BatchJob* jobs,
+hipError_t PHIP_LAUNCH_NAME(PHIP_BATCH_GROUP, PHIP_BATCH_SET, PHIP_PAIRED_SUFFIX)(dim3 grid, unsigned lds_bytes,
+                                                              hipStream_t stream, const BatchJob* jobs,
                                                               const uint2* items, uint32_t n_items,
                                                               uint32_t* counter, const int32_t* blob,
                                                               const int32_t* set_p, const uint32_t* set_pinv,
                                                               uint64_t* item_partials) {
     hipLaunchKernelGGL((batch_glynn_kernel<PHIP_BATCH_GROUP, PHIP_BATCH_SET, PHIP_BATCH_PAIRED != 0,
                                            PHIP_BATCH_SKIP_ZEROS != 0, PHIP_BATCH_EXACT != 0>),
-                       grid, dim3(256), 0, stream, jobs,
-                       items, n_items, counter, blob, set_p, set_pinv, item_partials);
+                       grid, dim3(256), lds_bytes, stream, jobs,
+                       items, n_items, counter, blob, set_p, set_pinv, item_partials, lds_bytes / 16u);
     return hipGetLastError();
 }
 
@@ -217,13 +219,13 @@ hipError_t PHIP_LAUNCH_NAME(PHIP_BATCH_GROUP, PHIP_BATCH_SET, PHIP_PAIRED_SUFFIX
 
 // Workgroups of this class's kernel that one CU holds at once (its register
 // and LDS use), or -1.
-int PHIP_OCC_NAME(PHIP_BATCH_GROUP, PHIP_BATCH_SET, PHIP_PAIRED_SUFFIX)() {
+int PHIP_OCC_NAME(PHIP_BATCH_GROUP, PHIP_BATCH_SET, PHIP_PAIRED_SUFFIX)(unsigned lds_bytes) {
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &blocks,
             batch_glynn_kernel<PHIP_BATCH_GROUP, PHIP_BATCH_SET, PHIP_BATCH_PAIRED != 0, PHIP_BATCH_SKIP_ZEROS != 0,
                                PHIP_BATCH_EXACT != 0>,
-            256, 0) != hipSuccess)
+            256, lds_bytes) != hipSuccess)
         return -1;
     return blocks;
 }

@@ -133,13 +133,13 @@ BatchLaunchFn launcher_for(int group, int set, int variant) {
     return nullptr;
 }
 
-int occupancy_for(int group, int set, int variant) {
+int occupancy_for(int group, int set, int variant, unsigned lds) {
 #define PHIP_OCC(G, K) \
-    if (group == G && set == K && variant == kVariantPlain) return occupancy_batch_g##G##_k##K();
+    if (group == G && set == K && variant == kVariantPlain) return occupancy_batch_g##G##_k##K(lds);
 #define PHIP_OCC_PAIRED(G, K) \
-    if (group == G && set == K && variant == kVariantPaired) return occupancy_batch_g##G##_k##K##_paired();
+    if (group == G && set == K && variant == kVariantPaired) return occupancy_batch_g##G##_k##K##_paired(lds);
 #define PHIP_OCC_SKIP(G, K) \
-    if (group == G && set == K && variant == kVariantSkipZeros) return occupancy_batch_g##G##_k##K##_skip();
+    if (group == G && set == K && variant == kVariantSkipZeros) return occupancy_batch_g##G##_k##K##_skip(lds);
     PHIP_OCC(0, 1)
     PHIP_OCC(2, 1) PHIP_OCC(2, 2) PHIP_OCC(2, 4) PHIP_OCC(2, 8)
     PHIP_OCC(3, 1) PHIP_OCC(3, 2) PHIP_OCC(3, 4) PHIP_OCC(3, 8)
@@ -150,16 +150,16 @@ int occupancy_for(int group, int set, int variant) {
     PHIP_OCC_SKIP(4, 2) PHIP_OCC_SKIP(4, 4) PHIP_OCC_SKIP(4, 8)
     PHIP_OCC_SKIP(6, 2) PHIP_OCC_SKIP(6, 4) PHIP_OCC_SKIP(6, 8)
     if (variant == kVariantExact) {
-        if (group == 2 && set == 8) return occupancy_batch_g2_k8_exact();
-        if (group == 3 && set == 8) return occupancy_batch_g3_k8_exact();
-        if (group == 4 && set == 8) return occupancy_batch_g4_k8_exact();
-        if (group == 6 && set == 8) return occupancy_batch_g6_k8_exact();
-        if (group == 2 && set == 10) return occupancy_batch_g2_k10_exact();
-        if (group == 2 && set == 12) return occupancy_batch_g2_k12_exact();
-        if (group == 3 && set == 10) return occupancy_batch_g3_k10_exact();
-        if (group == 3 && set == 12) return occupancy_batch_g3_k12_exact();
-        if (group == 4 && set == 10) return occupancy_batch_g4_k10_exact();
-        if (group == 4 && set == 12) return occupancy_batch_g4_k12_exact();
+        if (group == 2 && set == 8) return occupancy_batch_g2_k8_exact(lds);
+        if (group == 3 && set == 8) return occupancy_batch_g3_k8_exact(lds);
+        if (group == 4 && set == 8) return occupancy_batch_g4_k8_exact(lds);
+        if (group == 6 && set == 8) return occupancy_batch_g6_k8_exact(lds);
+        if (group == 2 && set == 10) return occupancy_batch_g2_k10_exact(lds);
+        if (group == 2 && set == 12) return occupancy_batch_g2_k12_exact(lds);
+        if (group == 3 && set == 10) return occupancy_batch_g3_k10_exact(lds);
+        if (group == 3 && set == 12) return occupancy_batch_g3_k12_exact(lds);
+        if (group == 4 && set == 10) return occupancy_batch_g4_k10_exact(lds);
+        if (group == 4 && set == 12) return occupancy_batch_g4_k12_exact(lds);
     }
 #undef PHIP_OCC
 #undef PHIP_OCC_PAIRED
@@ -525,9 +525,10 @@ int assign_items(PlanHost& out, int items_log2, int min_cbits) {
                    (b.n + 3.0 * groups_of(b.n, b.group));
         };
         std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost(a) > cost(b); });
-        BatchClass cl{gk[0], gk[1], gk[2], (uint32_t)out.items.size(), 0};
+        BatchClass cl{gk[0], gk[1], gk[2], (uint32_t)out.items.size(), 0, 0};
         for (uint32_t j : order) {
             BatchJob& b = out.jobs[j];
+            cl.max_n = std::max(cl.max_n, b.n);
             const uint64_t waves = (b.lanes + 63) / 64;
             if (out.items.size() + waves >= 0xffffffffull) {
                 set_error("batch has too many work items");
@@ -543,8 +544,14 @@ int assign_items(PlanHost& out, int items_log2, int min_cbits) {
     return 0;
 }
 
-int enqueue_plan(const PlanHost& h, const PlanDevice& d, hipStream_t stream, unsigned grid, hipEvent_t ev0,
-                 hipEvent_t ev1) {
+// Workgroups per CU that a class's kernel keeps resident with its dynamic
+// LDS, or -1 when the runtime cannot say.
+int class_resident(const BatchClass& cl) {
+    return occupancy_for(cl.group, cl.set, cl.variant, class_lds_bytes(cl));
+}
+
+int enqueue_plan(const PlanHost& h, const PlanDevice& d, hipStream_t stream, int cus, int blocks_per_cu,
+                 hipEvent_t ev0, hipEvent_t ev1) {
     if (h.slots.empty()) return 0;
     hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, stream, d.counters, (int)h.classes.size());
     PHIP_HIP_TRY(hipGetLastError());
@@ -556,8 +563,14 @@ int enqueue_plan(const PlanHost& h, const PlanDevice& d, hipStream_t stream, uns
             set_error("internal: no kernel for a batch class");
             return -1;
         }
-        PHIP_HIP_TRY(fn(dim3(grid), stream, d.jobs, d.items + cl.first_item, cl.n_items, d.counters + c, d.blob,
-                        d.set_p, d.set_pinv, d.partials + (size_t)cl.first_item * kItemSlots));
+        // The persistent grid holds at most as many workgroups per CU as
+        // the class's kernel keeps resident: more would only queue behind
+        // the resident ones and start once the work is gone.
+        const int resident = class_resident(cl);
+        const int per_cu = resident > 0 ? std::min(blocks_per_cu, resident) : blocks_per_cu;
+        PHIP_HIP_TRY(fn(dim3((unsigned)(cus * std::max(per_cu, 1))), class_lds_bytes(cl), stream, d.jobs,
+                        d.items + cl.first_item, cl.n_items, d.counters + c, d.blob, d.set_p, d.set_pinv,
+                        d.partials + (size_t)cl.first_item * kItemSlots));
     }
     if (ev1) PHIP_HIP_TRY(hipEventRecord(ev1, stream));
     hipLaunchKernelGGL(batch_finalize_kernel, dim3((unsigned)h.slots.size()), dim3(256), 0, stream, d.jobs,
@@ -669,7 +682,7 @@ struct BatchPlan {
     }
 
     int enqueue(bool timed) {
-        if (enqueue_plan(host, device_image(), stream, (unsigned)(cus * blocks_per_cu), timed ? ev0 : nullptr,
+        if (enqueue_plan(host, device_image(), stream, cus, blocks_per_cu, timed ? ev0 : nullptr,
                          timed ? ev1 : nullptr) != 0)
             return -1;
         PHIP_HIP_TRY(hipMemcpyAsync(h_res, d_res.ptr, host.slots.size() * sizeof(int64_t), hipMemcpyDeviceToHost,
@@ -764,8 +777,7 @@ int run_plan_in_context(DeviceContext* ctx, const std::vector<PlanInput>& in, in
     PHIP_HIP_TRY(
         hipMemcpyAsync(d.slots, h.slots.data(), h.slots.size() * sizeof(BatchSlot), hipMemcpyHostToDevice, st));
     const int cus = ctx->compute_units() > 0 ? ctx->compute_units() : 256;
-    const unsigned grid = (unsigned)(cus * t.batch_blocks_per_cu);
-    if (enqueue_plan(h, d, st, grid, ctx->profiling ? ctx->prof_start : nullptr,
+    if (enqueue_plan(h, d, st, cus, t.batch_blocks_per_cu, ctx->profiling ? ctx->prof_start : nullptr,
                      ctx->profiling ? ctx->prof_stop : nullptr) != 0)
         return -1;
     PHIP_HIP_TRY(hipMemcpyAsync(h_res, d.residues, h.slots.size() * sizeof(int64_t), hipMemcpyDeviceToHost, st));
@@ -938,12 +950,11 @@ int phip_batch_stats(void* handle, double* out, int len) {
     auto* plan = static_cast<BatchPlan*>(handle);
     const phip::PlanHost& h = plan->host;
     int cbits_min = h.jobs.empty() ? -1 : 64, cbits_max = -1;
-    // Fewest workgroups per CU any of the plan's class kernels keeps resident.
+    // Most workgroups per CU any of the plan's class kernels keeps resident:
+    // each class's grid is capped at its own count, so a larger setting
+    // changes nothing.
     int resident = -1;
-    for (const phip::BatchClass& cl : h.classes) {
-        const int r = phip::occupancy_for(cl.group, cl.set, cl.variant);
-        if (r > 0 && (resident < 0 || r < resident)) resident = r;
-    }
+    for (const phip::BatchClass& cl : h.classes) resident = std::max(resident, phip::class_resident(cl));
     for (const phip::BatchJob& jb : h.jobs) {
         cbits_min = std::min(cbits_min, jb.cbits);
         cbits_max = std::max(cbits_max, jb.cbits);

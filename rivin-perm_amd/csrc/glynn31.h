@@ -454,16 +454,20 @@ struct ExactSink {
 // K primes sharing one packed table whose states mostly have a zero row sum
 // (0/1 matrices: a row of even degree d sums to zero in C(d, d/2) / 2^d of
 // the states, and one zero row zeroes the term).  A term is zero exactly
-// when one of the state's exact group products is zero, so only the other
-// states need the per-prime trees.  Each wavefront queues those states in
-// LDS (their group products, with the state's sign folded into the first
-// one) and, once 64 are queued, every lane takes one and runs the trees of
-// PrimeSetSink's per-state form: the trees run on nonzero terms only, at
-// full lane occupancy.  A lane adds the terms of whichever states it takes,
-// so the sign is absolute (the chunk's first-state parity is known before
-// the walk) and the lane's sum is not negated at the end.  The queue holds
-// kZeroQueue entries per group, column-major: between states it holds
-// fewer than 64, and one state adds at most 64.
+// when one of the state's row sums is zero.  Read as unsigned 16-bit halves,
+// a zero row sum is the smallest value a half can hold, so the halfwise
+// minimum of the packed row words (a tree of v_pk_min_u16, one instruction
+// per word) has a zero half exactly then; the padding half of an odd order
+// holds 1 in every state (build_glynn_tables).  The group products and the
+// per-prime trees run only for the other states: each wavefront queues them
+// in LDS (their packed row words and the state's sign) and, once 64 are
+// queued, every lane takes one, forms its group products and runs the trees
+// of PrimeSetSink's per-state form at full lane occupancy.  A lane adds the
+// terms of whichever states it takes, so the sign is absolute (the chunk's
+// first-state parity is known before the walk) and the lane's sum is not
+// negated at the end.  The queue holds kZeroQueue entries per row word plus
+// one for the sign, column-major: between states it holds fewer than 64,
+// and one state adds at most 64.
 constexpr int kZeroQueue = 128;
 
 template <int NR, int G, int K>
@@ -471,19 +475,27 @@ struct ZeroSkipSink {
     static_assert(G > 0, "prime sets need exact packed row sums");
     using Rows = PackedRows<NR, G>;
     static constexpr int kGroups = Rows::kGroups;
+    static constexpr int kWords = Rows::kStride;
     static constexpr int kPairs = kGroups / 2;
     static constexpr int kLeaves = kPairs + (kGroups & 1);
     int32_t negp[K];
     uint32_t pinv[K];
     int64_t acc[K];
-    int32_t* queue;   // this wavefront's kGroups * kZeroQueue words of LDS
+    int32_t* queue;   // this wavefront's (kWords + 1) * kZeroQueue words of LDS
     uint32_t count;   // queued states (wave-uniform)
     uint32_t lane;
     bool neg;         // the chunk's first state has sign -1
     bool active;      // the lane's chunk belongs to the job
 
-    // Adds the term of one state given by its signed group products.
-    __device__ __forceinline__ void consume(const int32_t (&g)[kGroups]) {
+    // Adds the term of queued entry `at`: its group products from its row
+    // words, the first one negated for a state of sign -1.
+    __device__ __forceinline__ void consume(uint32_t at) {
+        Rows q;
+#pragma unroll
+        for (int k = 0; k < kWords; ++k) q.r[k] = as_u16x2((uint32_t)queue[k * kZeroQueue + at]);
+        int32_t g[kGroups];
+        q.groups(g, std::make_integer_sequence<int, kGroups>{});
+        if (queue[kWords * kZeroQueue + at]) g[0] = -g[0];
         int64_t P[kPairs > 0 ? kPairs : 1];
 #pragma unroll
         for (int j = 0; j < kPairs; ++j) P[j] = (int64_t)g[2 * j] * (int64_t)g[2 * j + 1];
@@ -504,19 +516,23 @@ struct ZeroSkipSink {
 
     template <bool ODD>
     __device__ __forceinline__ void state(const Rows& rs) {
-        int32_t g[kGroups];
-        rs.groups(g, std::make_integer_sequence<int, kGroups>{});
-        bool nz = active;
+        u16x2 m[kWords];
 #pragma unroll
-        for (int i = 0; i < kGroups; ++i) nz &= g[i] != 0;
+        for (int k = 0; k < kWords; ++k) m[k] = rs.r[k];
+#pragma unroll
+        for (int w = 1; w < kWords; w *= 2) {
+#pragma unroll
+            for (int i = 0; i + w < kWords; i += 2 * w) m[i] = __builtin_elementwise_min(m[i], m[i + w]);
+        }
+        const bool nz = active && __builtin_elementwise_min(m[0].x, m[0].y) != 0;
         const uint64_t mask = __ballot(nz);
         if (mask == 0) return;
         if (nz) {
             const uint32_t pos =
                 count + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-            queue[pos] = (ODD != neg) ? -g[0] : g[0];
 #pragma unroll
-            for (int i = 1; i < kGroups; ++i) queue[i * kZeroQueue + pos] = g[i];
+            for (int k = 0; k < kWords; ++k) queue[k * kZeroQueue + pos] = (int32_t)as_u32(rs.r[k]);
+            queue[kWords * kZeroQueue + pos] = (ODD != neg) ? 1 : 0;
         }
         count += (uint32_t)__popcll(mask);
         if (count >= 64) {
@@ -525,10 +541,7 @@ struct ZeroSkipSink {
             // wavefront's accesses in order, and this keeps the compiler
             // from moving the reads up.
             __builtin_amdgcn_wave_barrier();
-            int32_t h[kGroups];
-#pragma unroll
-            for (int i = 0; i < kGroups; ++i) h[i] = queue[i * kZeroQueue + count + lane];
-            consume(h);
+            consume(count + lane);
         }
     }
 
@@ -536,12 +549,7 @@ struct ZeroSkipSink {
     __device__ __forceinline__ void flush() {
         if (count == 0) return;
         __builtin_amdgcn_wave_barrier();
-        if (lane < count) {
-            int32_t h[kGroups];
-#pragma unroll
-            for (int i = 0; i < kGroups; ++i) h[i] = queue[i * kZeroQueue + lane];
-            consume(h);
-        }
+        if (lane < count) consume(lane);
         count = 0;
     }
 };

@@ -160,8 +160,7 @@ extern "C" int phip_permanent_residues_noalloc(const int64_t* A, int64_t prime_s
     PHIP_HIP_TRY(
         hipMemcpyAsync(d.slots, h.slots.data(), h.slots.size() * sizeof(BatchSlot), hipMemcpyHostToDevice, st));
     const int cus = ctx->compute_units() > 0 ? ctx->compute_units() : 256;
-    const unsigned grid = (unsigned)(cus * t.batch_blocks_per_cu);
-    if (enqueue_plan(h, d, st, grid, nullptr, nullptr) != 0) return -1;
+    if (enqueue_plan(h, d, st, cus, t.batch_blocks_per_cu, nullptr, nullptr) != 0) return -1;
     PHIP_HIP_TRY(hipMemcpyAsync(out, d.residues, (size_t)k * sizeof(int64_t), hipMemcpyDeviceToHost, st));
     PHIP_HIP_TRY(hipStreamSynchronize(st));
     return 0;

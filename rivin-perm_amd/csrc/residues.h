@@ -18,7 +18,7 @@ struct Tuning {
     int lanes_log2 = 20;
     int min_cbits = 4;
     int batch_items_log2 = 15;
-    int batch_blocks_per_cu = 3;
+    int batch_blocks_per_cu = 4;  // capped per class at what its kernel keeps resident
     int necklace_min_order = 23;  // smallest prime Schur order sent to the necklace kernel
     int necklace_depth = 28;      // largest FKM prefix depth of the necklace tasks
     int wide_min_tier = 8;        // smallest register tier (8..64, multiple of 4) of the wide Glynn kernels

@@ -142,6 +142,10 @@ inline void build_glynn_tables(const int64_t* M, int n, uint32_t q, int group, c
             put(1 + j, i, group == 0 ? (2 * a) % (int64_t)q : 2 * a);
         }
     }
+    // The padding half of an odd packed order holds 1 in every state (its
+    // flip deltas are 0), so a halfwise minimum over the row words sees a
+    // zero only for a zero row sum.  No product reads it.
+    if (group > 0 && (n & 1)) put(0, n, 1);
 }
 
 }  // namespace phip

@@ -10,7 +10,8 @@ changes the arithmetic, only the schedule:
   tail better over the compute units.
 * workgroups per CU: 256-lane workgroups (4 wavefronts) per compute unit in
   the persistent grid, i.e. the waves per CU that fetch items.  More waves
-  hide latency as long as their registers let them stay resident.
+  hide latency as long as their registers and LDS let them stay resident;
+  each class's grid is capped at what its kernel keeps resident.
 
 ``tune_plan`` times candidate settings on the caller's own plan by replaying
 it, checks that every candidate returns the same residues, and leaves the plan

@@ -166,9 +166,9 @@ def test_autotuned_single_matrix_path(native):
 
 
 def test_resident_workgroups_are_reported(native):
-    """The plan reports how many workgroups per CU its kernels keep
-    resident: the headline class (139 VGPRs, 3 waves per SIMD) keeps 3, and
-    the autotuner does not try more than that."""
+    """The plan reports the most workgroups per CU any of its class kernels
+    keeps resident (each class's grid is capped at its own count), and the
+    autotuner does not try more than that."""
     import numpy as np
 
     from permhip import autotune
