@@ -38,7 +38,7 @@ constexpr int kExactMaxColumns = 12;
 // moduli below 2^31 that share it (K = 1 for residue rows).  56 bytes,
 // read with scalar loads.
 struct BatchJob {
-    uint64_t data_off;   // int32 offset of the job's tables (n * glynn_stride words)
+    uint64_t data_off;   // int32 offset of the job's tables (n * glynn_stride words; n * byte_stride for kVariantSkipZeros)
     uint64_t lanes;      // lanes of this job (2^(n-1-cbits))
     int32_t n;
     int32_t cbits;
@@ -71,10 +71,10 @@ struct BatchClass {
     int max_n;  // largest order of the class's jobs (sizes the zero-skip queues)
 };
 
-// Rows of one wavefront's zero-skip queue for tables of order n: the packed
+// Rows of one wavefront's zero-skip queue for tables of order n: the byte
 // row words and the sign (ZeroSkipSink), kZeroQueueEntries words each.
 constexpr int kZeroQueueEntries = 128;
-constexpr int skip_queue_rows(int n) { return (n + 1) / 2 + 1; }
+constexpr int skip_queue_rows(int n) { return (n + 3) / 4 + 1; }
 // Dynamic LDS of one workgroup (4 wavefronts) of a class's kernel: the
 // zero-skip queues, nothing for the other variants.
 inline unsigned class_lds_bytes(const BatchClass& cl) {

@@ -350,14 +350,25 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
         std::vector<int> order;
         const std::vector<uint64_t> bounds = centred_row_bounds(x.a, x.n, (int64_t)x.q);
         const int group = plan_rows(bounds, order);
-        const size_t words = (size_t)x.n * glynn_stride(x.n, group);
+        // Tables of 0/1 matrices (nonzero_share is 1 for any other entry) in
+        // groups of 4 or 6 with a small share of nonzero terms run on the
+        // zero-skipping kernels, whose walk reads byte tables in the same
+        // row order.
+        const int permille = tuning().skip_zeros_permille;
+        const bool skip = permille > 0 && (group == 4 || group == 6) &&
+                          nonzero_share(x.a, x.n) * 1000.0 <= (double)permille;
+        const size_t words = (size_t)x.n * (skip ? byte_stride(x.n) : glynn_stride(x.n, group));
         table.assign(words, 0);
-        build_glynn_tables(x.a, x.n, x.q, group, order, table.data());
+        if (skip) {
+            build_byte_tables(x.a, x.n, order, table.data());
+        } else {
+            build_glynn_tables(x.a, x.n, x.q, group, order, table.data());
+        }
         size_t found = sets.size();
         if (group > 0) {
             for (size_t si : sets_of_key[x.key]) {
                 const Set& st = sets[si];
-                if (st.n == x.n && st.group == group &&
+                if (st.n == x.n && st.group == group && st.skip_zeros == skip &&
                     memcmp(out.blob.data() + st.data_off, table.data(), words * sizeof(int32_t)) == 0) {
                     found = si;
                     break;
@@ -365,11 +376,6 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
             }
         }
         if (found == sets.size()) {
-            // Tables of 0/1 matrices in groups of 4 or 6 with a small share
-            // of nonzero terms run on the zero-skipping kernels.
-            const int permille = tuning().skip_zeros_permille;
-            const bool skip = permille > 0 && (group == 4 || group == 6) &&
-                              nonzero_share(x.a, x.n) * 1000.0 <= (double)permille;
             sets.push_back({x.key, x.n, group, (uint64_t)out.blob.size(), skip, group > 0 ? bound_bits(bounds) : 0, {}});
             sets_of_key[x.key].push_back(found);
             out.blob.insert(out.blob.end(), table.begin(), table.end());

@@ -116,6 +116,8 @@ struct WideRows {
     using Word = int32_t;
     int32_t r[NR];
 
+    __device__ __forceinline__ static Word word_of(int32_t v) { return v; }
+
     __device__ __forceinline__ static int32_t add(int32_t x, int32_t d, int32_t p) {
         uint32_t a = (uint32_t)x + (uint32_t)d;
         uint32_t b = a - (uint32_t)p;
@@ -174,6 +176,8 @@ struct PackedRows {
     static constexpr int kGroups = (NR + G - 1) / G;
     using Word = u16x2;
     u16x2 r[kStride];
+
+    __device__ __forceinline__ static Word word_of(int32_t v) { return as_u16x2((uint32_t)v); }
 
     __device__ __forceinline__ void load(const int32_t* __restrict__ t) {
 #pragma unroll
@@ -263,6 +267,144 @@ struct PackedRows {
 
 template <int NR, int G>
 using GlynnRows = typename std::conditional<G == 0, WideRows<NR>, PackedRows<NR, G>>::type;
+
+// v_mul_i32_i24 on sign-extended bytes (S = 0..3) or on a full dword
+// (S = 4), through SDWA source selects.
+template <int S0, int S1>
+__device__ __forceinline__ int32_t mul_bsel(uint32_t a, uint32_t b);
+
+#define PHIP_MUL_BSEL(S0, S1, OPS, SELS)                                                            \
+    template <>                                                                                     \
+    __device__ __forceinline__ int32_t mul_bsel<S0, S1>(uint32_t a, uint32_t b) {                   \
+        int32_t r;                                                                                  \
+        asm("v_mul_i32_i24_sdwa %0, " OPS " dst_sel:DWORD dst_unused:UNUSED_PAD " SELS              \
+            : "=v"(r)                                                                               \
+            : "v"(a), "v"(b));                                                                      \
+        return r;                                                                                   \
+    }
+#define PHIP_MUL_BSEL_ROW(S0)                                                                        \
+    PHIP_MUL_BSEL(S0, 0, "sext(%1), sext(%2)", "src0_sel:BYTE_" #S0 " src1_sel:BYTE_0")              \
+    PHIP_MUL_BSEL(S0, 1, "sext(%1), sext(%2)", "src0_sel:BYTE_" #S0 " src1_sel:BYTE_1")              \
+    PHIP_MUL_BSEL(S0, 2, "sext(%1), sext(%2)", "src0_sel:BYTE_" #S0 " src1_sel:BYTE_2")              \
+    PHIP_MUL_BSEL(S0, 3, "sext(%1), sext(%2)", "src0_sel:BYTE_" #S0 " src1_sel:BYTE_3")
+PHIP_MUL_BSEL_ROW(0)
+PHIP_MUL_BSEL_ROW(1)
+PHIP_MUL_BSEL_ROW(2)
+PHIP_MUL_BSEL_ROW(3)
+PHIP_MUL_BSEL(4, 0, "%1, sext(%2)", "src0_sel:DWORD src1_sel:BYTE_0")
+PHIP_MUL_BSEL(4, 1, "%1, sext(%2)", "src0_sel:DWORD src1_sel:BYTE_1")
+PHIP_MUL_BSEL(4, 2, "%1, sext(%2)", "src0_sel:DWORD src1_sel:BYTE_2")
+PHIP_MUL_BSEL(4, 3, "%1, sext(%2)", "src0_sel:DWORD src1_sel:BYTE_3")
+#undef PHIP_MUL_BSEL_ROW
+#undef PHIP_MUL_BSEL
+
+// Row sums of tables with entries in {0, 1} (the zero-skipping walk).  A
+// row sum lies in [-d, d] for the row's degree d <= 63, so it is kept as the
+// byte s + 64, in [1, 127], four rows per 32-bit word (row 4k + b in byte b
+// of word k).  The flip deltas 2 a_ij are bytes 0 or 2 and every true row
+// sum stays in range, so a flip is one 32-bit add or subtract per four rows
+// with no carry or borrow between bytes (a full-rate v_add_u32 /
+// v_sub_u32, against one v_pk_add_u16 per two rows for PackedRows).  The
+// padding bytes of an order that is not a multiple of four hold 65 (a row
+// sum of 1) with zero deltas.  Tables: build_byte_tables.
+template <int NR>
+struct ByteRows {
+    static constexpr int kStride = (NR + 3) / 4;
+    using Word = uint32_t;
+    uint32_t r[kStride];
+
+    __device__ __forceinline__ static Word word_of(int32_t v) { return (uint32_t)v; }
+    __device__ __forceinline__ void load(const int32_t* __restrict__ t) {
+#pragma unroll
+        for (int k = 0; k < kStride; ++k) r[k] = (uint32_t)t[k];
+    }
+    __device__ __forceinline__ void sub_masked(const int32_t* __restrict__ col, int32_t mask, int32_t) {
+#pragma unroll
+        for (int k = 0; k < kStride; ++k) r[k] -= (uint32_t)(col[k] & mask);
+    }
+    template <bool SUB>
+    __device__ __forceinline__ void flip(const Word (&d)[kStride], int32_t) {
+#pragma unroll
+        for (int k = 0; k < kStride; ++k) r[k] = SUB ? r[k] - d[k] : r[k] + d[k];
+    }
+    template <bool SUB>
+    __device__ __forceinline__ void flip_mem(const int32_t* __restrict__ col, int32_t) {
+        const ConstCol c = opaque_ptr(col);
+#pragma unroll
+        for (int k = 0; k < kStride; ++k) {
+            const uint32_t d = (uint32_t)c[k];
+            r[k] = SUB ? r[k] - d : r[k] + d;
+        }
+    }
+    // Subtracting d from a word is adding its 32-bit negation: the word's
+    // value changes by -d in either form, and no byte leaves [1, 127].
+    __device__ __forceinline__ void flip_lane(const int32_t* __restrict__ col, bool sub_dir, int32_t) {
+        const ConstCol c = opaque_ptr(col);
+#pragma unroll
+        for (int k = 0; k < kStride; ++k) {
+            const uint32_t d = (uint32_t)c[k];
+            r[k] += sub_dir ? 0u - d : d;
+        }
+    }
+    // Nonzero in some bit 7 exactly when a row sum of the word is zero:
+    // every byte of v = w ^ 0x40404040 is (s + 64) ^ 64, in [0, 127] and
+    // zero only for s = 0.  v - 0x01010101 leaves every byte of a word
+    // without zero bytes in [0, 126] (no borrows), and turns the lowest zero
+    // byte of any other word into 0xff (no borrow reaches it).  One
+    // v_xad_u32 per word (x1010 = 0xfefefeff, the 32-bit -0x01010101, in a
+    // register: a gfx9 VALU instruction reads at most one scalar operand);
+    // the callers OR the results and test 0x80808080.
+    __device__ __forceinline__ static uint32_t zero_bits(uint32_t w, uint32_t x1010) {
+        uint32_t r;
+        asm("v_xad_u32 %0, %1, %2, %3" : "=v"(r) : "v"(w), "s"(0x40404040u), "v"(x1010));
+        return r;
+    }
+};
+
+// The signed row sums of a ByteRows word as two's-complement bytes: byte
+// b + 64 stays below 256 (b <= 127), and flipping bit 7 of it gives
+// b - 64 modulo 256.
+__device__ __forceinline__ uint32_t byte_rows_signed(uint32_t w) { return (w + 0x40404040u) ^ 0x80808080u; }
+
+// Exact products of signed byte row sums (words from byte_rows_signed),
+// grouped as PackedRows groups them: `C` consecutive rows from FIRST as
+//   1: r0   2: r0 r1   3: (r0 r1) r2   4: (r0 r1)(r2 r3)
+//   5: ((r0 r1) r2)(r3 r4)   6: ((r0 r1) r2)((r3 r4) r5)
+// under the same operand limits (plan_rows checks them on the host).
+template <int NR, int G>
+struct ByteGroups {
+    static constexpr int kWords = (NR + 3) / 4;
+    static constexpr int kGroups = (NR + G - 1) / G;
+    const uint32_t (&w)[kWords];
+
+    template <int ROW>
+    __device__ __forceinline__ int32_t row_value() const {
+        return (int32_t)(int8_t)(uint8_t)(w[ROW / 4] >> (8 * (ROW & 3)));
+    }
+    template <int FIRST, int C>
+    __device__ __forceinline__ int32_t run_product() const {
+        if constexpr (C == 1) {
+            return row_value<FIRST>();
+        } else if constexpr (C == 2) {
+            return mul_bsel<FIRST & 3, (FIRST + 1) & 3>(w[FIRST / 4], w[(FIRST + 1) / 4]);
+        } else if constexpr (C == 3) {
+            return mul_bsel<4, (FIRST + 2) & 3>((uint32_t)run_product<FIRST, 2>(), w[(FIRST + 2) / 4]);
+        } else {
+            constexpr int A = (C + 1) / 2;
+            return mul_i24(run_product<FIRST, A>(), run_product<FIRST + A, C - A>());
+        }
+    }
+    template <int GI>
+    __device__ __forceinline__ int32_t group_product() const {
+        constexpr int first = GI * G;
+        constexpr int count = (NR - first < G) ? NR - first : G;
+        return run_product<first, count>();
+    }
+    template <int... GI>
+    __device__ __forceinline__ void groups(int32_t (&v)[kGroups], std::integer_sequence<int, GI...>) const {
+        ((v[GI] = group_product<GI>()), ...);
+    }
+};
 
 // acc + S * v (S = +1 or -1) as one v_mad_i64_i32 with an inline constant.
 // Written as asm so the accumulation stays in state order: left to itself
@@ -451,18 +593,16 @@ struct ExactSink {
     }
 };
 
-// K primes sharing one packed table whose states mostly have a zero row sum
-// (0/1 matrices: a row of even degree d sums to zero in C(d, d/2) / 2^d of
-// the states, and one zero row zeroes the term).  A term is zero exactly
-// when one of the state's row sums is zero.  Read as unsigned 16-bit halves,
-// a zero row sum is the smallest value a half can hold, so the halfwise
-// minimum of the packed row words (a tree of v_pk_min_u16, one instruction
-// per word) has a zero half exactly then; the padding half of an odd order
-// holds 1 in every state (build_glynn_tables).  The group products and the
-// per-prime trees run only for the other states: each wavefront queues them
-// in LDS (their packed row words and the state's sign) and, once 64 are
-// queued, every lane takes one, forms its group products and runs the trees
-// of PrimeSetSink's per-state form at full lane occupancy.  A lane adds the
+// K primes sharing one table of a 0/1 matrix, whose states mostly have a
+// zero row sum (a row of even degree d sums to zero in C(d, d/2) / 2^d of
+// the states, and one zero row zeroes the term).  The walk keeps the row
+// sums as bytes (ByteRows) and tests each state with one v_xad_u32 per four
+// rows and an AND over the words.  The group products and the per-prime
+// trees run only for the other states: each wavefront queues them in LDS
+// (their row words and the state's sign) and, once 64 are queued, every
+// lane takes one, turns its words into signed bytes, forms its group
+// products (ByteGroups, SDWA byte selects) and runs the trees of
+// PrimeSetSink's per-state form at full lane occupancy.  A lane adds the
 // terms of whichever states it takes, so the sign is absolute (the chunk's
 // first-state parity is known before the walk) and the lane's sum is not
 // negated at the end.  The queue holds kZeroQueue entries per row word plus
@@ -472,9 +612,10 @@ constexpr int kZeroQueue = 128;
 
 template <int NR, int G, int K>
 struct ZeroSkipSink {
-    static_assert(G > 0, "prime sets need exact packed row sums");
-    using Rows = PackedRows<NR, G>;
-    static constexpr int kGroups = Rows::kGroups;
+    static_assert(G > 0, "prime sets need exact row sums");
+    using Rows = ByteRows<NR>;
+    using Groups = ByteGroups<NR, G>;
+    static constexpr int kGroups = Groups::kGroups;
     static constexpr int kWords = Rows::kStride;
     static constexpr int kPairs = kGroups / 2;
     static constexpr int kLeaves = kPairs + (kGroups & 1);
@@ -486,14 +627,16 @@ struct ZeroSkipSink {
     uint32_t lane;
     bool neg;         // the chunk's first state has sign -1
     bool active;      // the lane's chunk belongs to the job
+    uint32_t x1010;   // 0xfefefeff (ByteRows::zero_bits)
 
     // Adds the term of queued entry `at`: its group products from its row
     // words, the first one negated for a state of sign -1.
     __device__ __forceinline__ void consume(uint32_t at) {
-        Rows q;
+        uint32_t w[kWords];
 #pragma unroll
-        for (int k = 0; k < kWords; ++k) q.r[k] = as_u16x2((uint32_t)queue[k * kZeroQueue + at]);
+        for (int k = 0; k < kWords; ++k) w[k] = byte_rows_signed((uint32_t)queue[k * kZeroQueue + at]);
         int32_t g[kGroups];
+        const Groups q{w};
         q.groups(g, std::make_integer_sequence<int, kGroups>{});
         if (queue[kWords * kZeroQueue + at]) g[0] = -g[0];
         int64_t P[kPairs > 0 ? kPairs : 1];
@@ -506,9 +649,9 @@ struct ZeroSkipSink {
             for (int j = 0; j < kPairs; ++j) v[j] = redc31(P[j], negp[k], pinv[k]);
             if constexpr (kGroups & 1) v[kLeaves - 1] = g[kGroups - 1];
 #pragma unroll
-            for (int w = 1; w < kLeaves; w *= 2) {
+            for (int wd = 1; wd < kLeaves; wd *= 2) {
 #pragma unroll
-                for (int i = 0; i + w < kLeaves; i += 2 * w) v[i] = mont31(v[i], v[i + w], negp[k], pinv[k]);
+                for (int i = 0; i + wd < kLeaves; i += 2 * wd) v[i] = mont31(v[i], v[i + wd], negp[k], pinv[k]);
             }
             acc[k] = mac_sign<1>(acc[k], v[0]);
         }
@@ -516,22 +659,17 @@ struct ZeroSkipSink {
 
     template <bool ODD>
     __device__ __forceinline__ void state(const Rows& rs) {
-        u16x2 m[kWords];
+        uint32_t m = Rows::zero_bits(rs.r[0], x1010);
 #pragma unroll
-        for (int k = 0; k < kWords; ++k) m[k] = rs.r[k];
-#pragma unroll
-        for (int w = 1; w < kWords; w *= 2) {
-#pragma unroll
-            for (int i = 0; i + w < kWords; i += 2 * w) m[i] = __builtin_elementwise_min(m[i], m[i + w]);
-        }
-        const bool nz = active && __builtin_elementwise_min(m[0].x, m[0].y) != 0;
+        for (int k = 1; k < kWords; ++k) m |= Rows::zero_bits(rs.r[k], x1010);
+        const bool nz = active && (m & 0x80808080u) == 0u;
         const uint64_t mask = __ballot(nz);
         if (mask == 0) return;
         if (nz) {
             const uint32_t pos =
                 count + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 #pragma unroll
-            for (int k = 0; k < kWords; ++k) queue[k * kZeroQueue + pos] = (int32_t)as_u32(rs.r[k]);
+            for (int k = 0; k < kWords; ++k) queue[k * kZeroQueue + pos] = (int32_t)rs.r[k];
             queue[kWords * kZeroQueue + pos] = (ODD != neg) ? 1 : 0;
         }
         count += (uint32_t)__popcll(mask);
@@ -556,16 +694,15 @@ struct ZeroSkipSink {
 
 // Walks one lane's chunk: the 2^cbits Gray states starting at `base` (a
 // multiple of 2^cbits), handing the row sums of each state to the sink.
-// `tables` holds the row sums of state 0 (all column signs +1: exact centred
-// sums for G > 0, packed two per word, or residues for G = 0) followed by
-// the flip deltas of the n-1 Gray columns, column-major with
-// glynn_stride(NR, G) words per column (2a_ij centred and packed, or
-// 2a_ij mod p; p is read only for G = 0).  Returns the parity of gray(base),
-// the sign of the chunk's first state.
-template <int NR, int G, class Sink>
-__device__ __forceinline__ bool gray_walk(const int32_t* __restrict__ tables, int n, int cbits, uint64_t base,
-                                          int32_t p, Sink& sink) {
-    using Rows = GlynnRows<NR, G>;
+// `tables` holds the row sums of state 0 (all column signs +1, in the Rows
+// representation: exact centred sums packed two per word, biased bytes four
+// per word, or residues) followed by the flip deltas of the n-1 Gray
+// columns, column-major with Rows::kStride words per column (2a_ij in the
+// same representation, or 2a_ij mod p; p is read only by WideRows).
+// Returns the parity of gray(base), the sign of the chunk's first state.
+template <int NR, class Rows, class Sink>
+__device__ __forceinline__ bool gray_walk_rows(const int32_t* __restrict__ tables, int n, int cbits, uint64_t base,
+                                               int32_t p, Sink& sink) {
     constexpr int W = Rows::kStride;
     const int32_t* __restrict__ D = tables + W;
     const uint64_t gray = base ^ (base >> 1);
@@ -581,13 +718,7 @@ __device__ __forceinline__ bool gray_walk(const int32_t* __restrict__ tables, in
     // Column 0 flips every other state; keep it resident.
     typename Rows::Word c0[W];
 #pragma unroll
-    for (int k = 0; k < W; ++k) {
-        if constexpr (G == 0) {
-            c0[k] = D[k];
-        } else {
-            c0[k] = as_u16x2((uint32_t)D[k]);
-        }
-    }
+    for (int k = 0; k < W; ++k) c0[k] = Rows::word_of(D[k]);
     const int32_t* __restrict__ d1 = D + W;
     const int32_t* __restrict__ d2 = D + 2 * W;
 
@@ -648,6 +779,14 @@ __device__ __forceinline__ bool gray_walk(const int32_t* __restrict__ tables, in
     return __popcll(gray) & 1;
 }
 
+// The walk over GlynnRows<NR, G> tables (glynn_stride(NR, G) words per
+// column).
+template <int NR, int G, class Sink>
+__device__ __forceinline__ bool gray_walk(const int32_t* __restrict__ tables, int n, int cbits, uint64_t base,
+                                          int32_t p, Sink& sink) {
+    return gray_walk_rows<NR, GlynnRows<NR, G>>(tables, n, cbits, base, p, sink);
+}
+
 // The lane's signed sum of terms as one Montgomery residue: |acc| <
 // 2^(31+cbits) <= 2^61, so redc31 applies and the result is congruent to
 // acc * R^-1 with |result| < 2^31 (no 64-bit division in the epilogue).
@@ -699,7 +838,8 @@ __device__ __forceinline__ void glynn_chunk_exact(const int32_t* __restrict__ ta
     for (int c = 0; c < L; ++c) out[c] = neg ? -sink.acc[c] : sink.acc[c];
 }
 
-// glynn_chunk_set through ZeroSkipSink.  Every lane of the wavefront must
+// glynn_chunk_set through ZeroSkipSink, on byte tables (build_byte_tables,
+// byte_stride(NR) words per column).  Every lane of the wavefront must
 // call it (the queue is shared); lanes with active == false walk but queue
 // nothing.  A lane's value is its sum of the absolutely signed terms it
 // took, which may come from any lane's chunk, so only the wavefront's total
@@ -723,7 +863,8 @@ __device__ __forceinline__ void glynn_chunk_set_skip(const int32_t* __restrict__
     sink.lane = __lane_id();
     sink.neg = __popcll(base ^ (base >> 1)) & 1;
     sink.active = active;
-    gray_walk<NR, G>(tables, n, cbits, base, 0, sink);
+    sink.x1010 = 0xfefefeffu;
+    gray_walk_rows<NR, ByteRows<NR>>(tables, n, cbits, base, 0, sink);
     sink.flush();
 #pragma unroll
     for (int k = 0; k < K; ++k) out[k] = lane_value(sink.acc[k], false, sink.negp[k], sink.pinv[k]);

@@ -21,5 +21,8 @@ constexpr uint32_t kPairedPrimeLimit = 1342177280u;  // 1.25 * 2^30
 // Words per table column: residues one per word (group 0) or exact row sums
 // packed two 16-bit halves per word (groups 2..4).
 __host__ __device__ constexpr int glynn_stride(int n, int group) { return group == 0 ? n : (n + 1) / 2; }
+// Words per column of the zero-skipping walk's tables: biased byte row sums
+// of a 0/1 matrix, four per word.
+__host__ __device__ constexpr int byte_stride(int n) { return (n + 3) / 4; }
 
 }  // namespace phip

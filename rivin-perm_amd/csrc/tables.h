@@ -148,4 +148,30 @@ inline void build_glynn_tables(const int64_t* M, int n, uint32_t q, int group, c
     if (group > 0 && (n & 1)) put(0, n, 1);
 }
 
+// Tables of the zero-skipping walk (glynn31.h, ByteRows) for a matrix with
+// entries in {0, 1}: n * byte_stride(n) int32 words, the biased row sums
+// 64 + sum_j a_ij of state 0 (every column sign +1), then the flip deltas
+// 2 a_ij of the n-1 Gray columns, one byte per row (kernel row i from
+// matrix row order[i], in byte i % 4 of word i / 4).  Padding rows hold 65
+// (a row sum of 1, never zero) with zero deltas.  Every row degree must be
+// at most 63, so that each row sum stays in [-63, 63].
+inline void build_byte_tables(const int64_t* M, int n, const std::vector<int>& order, int32_t* out) {
+    const int W = byte_stride(n);
+    std::vector<uint32_t> words((size_t)n * W, 0u);
+    auto put = [&](int col, int row, uint32_t byte) { words[(size_t)col * W + row / 4] |= byte << (8 * (row % 4)); };
+    for (int i = 0; i < 4 * W; ++i) {
+        if (i >= n) {
+            put(0, i, 65u);
+            continue;
+        }
+        const int src = order.empty() ? i : order[i];
+        uint32_t degree = 0;
+        for (int j = 0; j < n; ++j) degree += M[(size_t)src * n + j] != 0 ? 1u : 0u;
+        put(0, i, 64u + degree);
+        for (int j = 0; j < n - 1; ++j)
+            if (M[(size_t)src * n + j] != 0) put(1 + j, i, 2u);
+    }
+    for (size_t k = 0; k < words.size(); ++k) out[k] = (int32_t)words[k];
+}
+
 }  // namespace phip
