@@ -49,7 +49,7 @@ struct BatchJob {
     int32_t nprimes;     // real primes (<= K); padded entries repeat the last one
     int32_t set;         // K of the job's launch class
     int32_t variant;     // kVariantPlain, kVariantPaired, kVariantSkipZeros or kVariantExact
-    int32_t pad;
+    int32_t test_words;  // kVariantSkipZeros: leading table words the zero test reads (skip_test_words)
 };
 
 // One requested residue: prime slot `k` of job `job`.
@@ -75,6 +75,14 @@ struct BatchClass {
 // row words and the sign (ZeroSkipSink), kZeroQueueEntries words each.
 constexpr int kZeroQueueEntries = 128;
 constexpr int skip_queue_rows(int n) { return (n + 3) / 4 + 1; }
+// Word counts the zero-skipping walk's test may read for tables of order n:
+// every word, or the leading half of them when every row of even degree
+// lies there (a row of odd degree never sums to zero).
+// The half-width test is compiled for orders from kSkipHalfMinOrder on
+// (below it a residue takes microseconds either way).
+constexpr int kSkipHalfMinOrder = 20;
+constexpr int skip_full_words(int n) { return (n + 3) / 4; }
+constexpr int skip_half_words(int n) { return n < kSkipHalfMinOrder ? (n + 3) / 4 : ((n + 3) / 4 + 1) / 2; }
 // Dynamic LDS of one workgroup (4 wavefronts) of a class's kernel: the
 // zero-skip queues, nothing for the other variants.
 inline unsigned class_lds_bytes(const BatchClass& cl) {

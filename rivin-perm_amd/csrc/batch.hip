@@ -52,7 +52,13 @@ __device__ __forceinline__ void run_item(const int32_t* __restrict__ tables, con
                                          bool active, int32_t* queue, const int32_t (&p)[K],
                                          const uint32_t (&pinv)[K], int32_t (&r)[K]) {
     if constexpr (Z) {
-        glynn_chunk_set_skip<N, G, K>(tables, N, jb.cbits, base, active, queue, p, pinv, r);
+        if constexpr (N >= kSkipHalfMinOrder) {
+            if (jb.test_words < skip_full_words(N)) {
+                glynn_chunk_set_skip<N, G, K, true>(tables, N, jb.cbits, base, active, queue, p, pinv, r);
+                return;
+            }
+        }
+        glynn_chunk_set_skip<N, G, K, false>(tables, N, jb.cbits, base, active, queue, p, pinv, r);
     } else if constexpr (K == 1) {
         r[0] = glynn_chunk<N, G>(tables, N, jb.cbits, base, p[0], pinv[0]);
     } else {

@@ -272,6 +272,53 @@ double nonzero_share(const int64_t* a, int n) {
     return share;
 }
 
+// Row order and zero-test width of a 0/1 table on the zero-skipping walk.
+// A row of odd degree sums to an odd number in every state, so only rows of
+// even degree can zero a term.  Rows of even degree go first, smallest
+// degree first (the likeliest to sum to zero), then the rows of odd degree;
+// when every even row lies in the leading skip_half_words(n) words and each
+// group of `group` consecutive rows of that order can be arranged to meet
+// the kernel's operand limits (group_fits on the degrees, which are the
+// row bounds of a 0/1 matrix), `order` becomes that order and the test
+// reads only those words.  Otherwise `order` is left as it is (plan_rows'
+// order) and the test reads every word.  Returns the words tested.
+int skip_plan(const int64_t* a, int n, int group, std::vector<int>& order) {
+    std::vector<uint64_t> degree(n, 0);
+    std::vector<int> even, odd;
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < n; ++j) degree[i] += a[(size_t)i * n + j] != 0 ? 1u : 0u;
+        (degree[i] % 2 == 0 ? even : odd).push_back(i);
+    }
+    const int half_rows = 4 * skip_half_words(n);
+    if ((int)even.size() > half_rows || skip_half_words(n) == skip_full_words(n)) return skip_full_words(n);
+    std::stable_sort(even.begin(), even.end(), [&](int x, int y) { return degree[x] < degree[y]; });
+    std::vector<int> cand(even);
+    cand.insert(cand.end(), odd.begin(), odd.end());
+    // Arrangements within a group are tried from the even-first one on;
+    // rows of equal degree are interchangeable for group_fits.
+    auto key = [&](int x, int y) {
+        return std::make_pair(degree[x] % 2, degree[x]) < std::make_pair(degree[y] % 2, degree[y]);
+    };
+    for (int first = 0; first < n; first += group) {
+        const auto lo = cand.begin() + first;
+        const auto hi = cand.begin() + std::min(n, first + group);
+        std::stable_sort(lo, hi, key);
+        bool fits = false;
+        do {
+            uint64_t b[6];
+            for (auto it = lo; it != hi; ++it) b[it - lo] = degree[*it];
+            if (group_fits(b, (int)(hi - lo))) {
+                fits = true;
+                break;
+            }
+        } while (std::next_permutation(lo, hi, key));
+        if (!fits) return skip_full_words(n);
+    }
+    for (int pos = half_rows; pos < n; ++pos)
+        if (degree[cand[pos]] % 2 == 0) return skip_full_words(n);
+    order = cand;
+    return skip_half_words(n);
+}
 
 template <typename T>
 struct DevBuf {
@@ -308,6 +355,7 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
         int group;
         uint64_t data_off;
         bool skip_zeros;
+        int test_words;  // skip_zeros: the walk's zero-test width (skip_plan)
         int bits;  // bound_bits of the table's row bounds
         std::vector<size_t> inputs;
     };
@@ -358,6 +406,7 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
         const bool skip = permille > 0 && (group == 4 || group == 6) &&
                           nonzero_share(x.a, x.n) * 1000.0 <= (double)permille;
         const size_t words = (size_t)x.n * (skip ? byte_stride(x.n) : glynn_stride(x.n, group));
+        const int test_words = skip ? skip_plan(x.a, x.n, group, order) : 0;
         table.assign(words, 0);
         if (skip) {
             build_byte_tables(x.a, x.n, order, table.data());
@@ -368,7 +417,7 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
         if (group > 0) {
             for (size_t si : sets_of_key[x.key]) {
                 const Set& st = sets[si];
-                if (st.n == x.n && st.group == group && st.skip_zeros == skip &&
+                if (st.n == x.n && st.group == group && st.skip_zeros == skip && st.test_words == test_words &&
                     memcmp(out.blob.data() + st.data_off, table.data(), words * sizeof(int32_t)) == 0) {
                     found = si;
                     break;
@@ -376,7 +425,8 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
             }
         }
         if (found == sets.size()) {
-            sets.push_back({x.key, x.n, group, (uint64_t)out.blob.size(), skip, group > 0 ? bound_bits(bounds) : 0, {}});
+            sets.push_back({x.key, x.n, group, (uint64_t)out.blob.size(), skip, test_words,
+                            group > 0 ? bound_bits(bounds) : 0, {}});
             sets_of_key[x.key].push_back(found);
             out.blob.insert(out.blob.end(), table.begin(), table.end());
         }
@@ -462,6 +512,7 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
             next += take;
             if (st.skip_zeros) {
                 jb.variant = kVariantSkipZeros;
+                jb.test_words = st.test_words;
             } else if (st.group > 0 && pairs_states(K, (const uint32_t*)out.set_p.data() + jb.prime_off)) {
                 jb.variant = kVariantPaired;
             } else {
@@ -571,9 +622,14 @@ int enqueue_plan(const PlanHost& h, const PlanDevice& d, hipStream_t stream, int
         }
         // The persistent grid holds at most as many workgroups per CU as
         // the class's kernel keeps resident: more would only queue behind
-        // the resident ones and start once the work is gone.
+        // the resident ones and start once the work is gone.  The
+        // zero-skipping kernels always run every resident workgroup: their
+        // walk is short dependent chains of full-rate adds and logic, and
+        // more waves per SIMD hide them (binary36 40 -> 36 ms from 4 to 7
+        // workgroups per CU).
         const int resident = class_resident(cl);
-        const int per_cu = resident > 0 ? std::min(blocks_per_cu, resident) : blocks_per_cu;
+        const int want = (cl.variant == kVariantSkipZeros && resident > 0) ? resident : blocks_per_cu;
+        const int per_cu = resident > 0 ? std::min(want, resident) : want;
         PHIP_HIP_TRY(fn(dim3((unsigned)(cus * std::max(per_cu, 1))), class_lds_bytes(cl), stream, d.jobs,
                         d.items + cl.first_item, cl.n_items, d.counters + c, d.blob, d.set_p, d.set_pinv,
                         d.partials + (size_t)cl.first_item * kItemSlots));
