@@ -217,7 +217,9 @@ int phip_batch_set_option(void *plan, int option, int64_t value);
  * 1.25 * 2^30), 2 skips zero terms (0/1 tables whose states are mostly
  * zero), [15] most workgroups per CU that any of the plan's kernels keeps
  * resident (register and LDS use; each class's grid is capped at its own
- * count; -1 if unknown).  Returns the number of
+ * count; -1 if unknown), [16] table words the zero test of the first job
+ * reads (zero-skipping jobs: all of them, or the leading half when every
+ * row of even degree lies there; 0 for other jobs).  Returns the number of
  * values written or -1. */
 int phip_batch_stats(void *plan, double *out, int len);
 

@@ -198,6 +198,19 @@ std::vector<int> split_set(int r) {
     return out;
 }
 
+// Sets of a 0/1 table on the zero-skipping kernels: its walk costs far more
+// than the per-prime trees, which run on the few nonzero states only, so
+// up to eight primes share one walk as one set of 2, 4 or 8 (padded).
+std::vector<int> split_skip_set(int r) {
+    std::vector<int> out;
+    while (r > 0) {
+        const int k = r <= 2 ? 2 : (r <= 4 ? 4 : 8);
+        out.push_back(k);
+        r -= std::min(r, k);
+    }
+    return out;
+}
+
 int groups_of(int n, int group) { return group == 0 ? n : (n + group - 1) / group; }
 
 // Bit length of the product of the row bounds (0 when a bound is 0): every
@@ -473,11 +486,8 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
             continue;
         }
         std::vector<int> set_sizes =
-            st.group == 0 ? std::vector<int>(st.inputs.size(), 1) : split_set((int)st.inputs.size());
-        // The zero-skipping kernels take sets of 2, 4 or 8 (one prime is
-        // padded to two).
-        if (st.skip_zeros)
-            for (int& K : set_sizes) K = std::max(K, 2);
+            st.group == 0 ? std::vector<int>(st.inputs.size(), 1)
+                          : (st.skip_zeros ? split_skip_set((int)st.inputs.size()) : split_set((int)st.inputs.size()));
         for (int K : set_sizes) {
             const int take = (int)std::min<size_t>((size_t)K, st.inputs.size() - next);
             BatchJob jb{};
@@ -1036,7 +1046,8 @@ int phip_batch_stats(void* handle, double* out, int len) {
                         (double)h.items_log2,
                         (double)plan->blocks_per_cu,
                         h.classes.empty() ? -1.0 : (double)h.classes[0].variant,
-                        (double)resident};
+                        (double)resident,
+                        h.jobs.empty() ? -1.0 : (double)h.jobs[0].test_words};
     const int m = std::min<int>(len, (int)(sizeof(v) / sizeof(v[0])));
     for (int i = 0; i < m; ++i) out[i] = v[i];
     return m;

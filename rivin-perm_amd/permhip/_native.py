@@ -522,7 +522,8 @@ class Batch:
 
     def stats(self) -> dict:
         keys = ("last_ms", "states", "mulmods", "jobs", "items", "launches", "waves", "group", "residues", "set",
-                "chunk_bits_min", "chunk_bits_max", "items_log2", "blocks_per_cu", "variant", "resident_blocks_per_cu")
+                "chunk_bits_min", "chunk_bits_max", "items_log2", "blocks_per_cu", "variant", "resident_blocks_per_cu",
+                "skip_test_words")
         vals = (ctypes.c_double * len(keys))()
         count = self._lib.phip_batch_stats(self._handle, vals, len(keys))
         _check(count, "phip_batch_stats")

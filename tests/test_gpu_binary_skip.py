@@ -145,3 +145,75 @@ def test_grid_and_binary36_through_skip(native):
     assert _variant(native, G, [2147483647, 2147483629]) == 2.0
     print("skip grid 40", flush=True)
     assert perm_binary.binary_permanent(G.tolist()) == 2551276235535120
+
+
+def _odd_heavy_matrix(rng, n, density, even_rows):
+    """A 0/1 matrix with exactly `even_rows` rows of even degree (the rest
+    odd), rows shuffled, so the zero test may read only half the words."""
+    A = (rng.random((n, n)) < density).astype(np.int64)
+    for i in range(n):
+        want_even = i < even_rows
+        if (A[i].sum() % 2 == 0) != want_even:
+            j = int(rng.integers(n))
+            A[i, j] ^= 1
+    return A[rng.permutation(n)]
+
+
+@pytest.mark.parametrize("n", [20, 23, 26, 29, 33])
+def test_half_width_zero_test(native, oracle_c, n):
+    """Tables whose rows of even degree fit in the leading half of the byte
+    words run the half-width test; residues equal the plain kernels' and,
+    up to order 26, the oracle's."""
+    rng = np.random.default_rng(300 + n)
+    words = (n + 3) // 4
+    half = (words + 1) // 2
+    for density, even_rows in ((0.5, 4 * half), (0.35, 4 * half - 3), (0.65, 3)):
+        A = _odd_heavy_matrix(rng, n, density, min(even_rows, n))
+        primes = BINARY_PRIMES[:4]
+        plan = native.Batch([A], [primes])
+        try:
+            st = plan.stats()
+        finally:
+            plan.close()
+        print(f"half test n={n} density={density} even={even_rows} stats={st['variant']},{st['skip_test_words']}",
+              flush=True)
+        if st["variant"] != 2.0:
+            continue
+        assert st["skip_test_words"] == half
+        skip = native.permanent_residues(A, primes)
+        plain = _with_skip(native, 0, lambda: native.permanent_residues(A, primes))
+        assert skip == plain, (n, density)
+        if n <= 26:
+            for p, r in zip(primes, skip):
+                assert r == oracle_c.binary_mod_p(A.astype(np.uint8), p, 16), (n, p)
+
+
+def test_full_width_when_even_rows_spill(native):
+    rng = np.random.default_rng(17)
+    n = 30
+    A = _even_degree_matrix(rng, n, 0.5)
+    plan = native.Batch([A], [BINARY_PRIMES[:4]])
+    try:
+        st = plan.stats()
+    finally:
+        plan.close()
+    assert st["variant"] == 2.0 and st["skip_test_words"] == (n + 3) // 4
+
+
+@pytest.mark.parametrize("k", [3, 5, 6, 7])
+def test_one_walk_for_up_to_eight_primes(native, k):
+    """A 0/1 table's primes share one zero-skipping walk (a padded set of 4
+    or 8), with the plain kernels' residues."""
+    rng = np.random.default_rng(40 + k)
+    A = _even_degree_matrix(rng, 24, 0.5)
+    primes = BINARY_PRIMES[:k]
+    plan = native.Batch([A], [primes])
+    try:
+        st = plan.stats()
+    finally:
+        plan.close()
+    assert st["variant"] == 2.0 and st["jobs"] == 1.0 and st["set"] == (4.0 if k <= 4 else 8.0)
+    print(f"one walk k={k}", flush=True)
+    skip = native.permanent_residues(A, primes)
+    plain = _with_skip(native, 0, lambda: native.permanent_residues(A, primes))
+    assert skip == plain
