@@ -674,7 +674,7 @@ def main() -> int:
     st = plan.stats()
     group, kset, variant = int(st["group"]), int(st["set"]), int(st.get("variant", 0))
     cus = int(re.search(r"CUs=(\d+)", info).group(1)) if re.search(r"CUs=(\d+)", info) else 256
-    exact_variant = variant == 3
+    exact_variant = variant in (3, 4)
     if exact_variant:
         mix = dict(EXACT_MIX_N32_G4)
     else:

@@ -215,7 +215,7 @@ int phip_batch_set_option(void *plan, int option, int64_t value);
  * sized for, [13] workgroups per CU, [14] kernel variant of the first
  * launch: 0 plain, 1 pairs consecutive states (prime sets below
  * 1.25 * 2^30), 2 skips zero terms (0/1 tables whose states are mostly
- * zero), [15] most workgroups per CU that any of the plan's kernels keeps
+ * zero), 3 exact accumulation, 4 compact exact accumulation, [15] most workgroups per CU that any of the plan's kernels keeps
  * resident (register and LDS use; each class's grid is capped at its own
  * count; -1 if unknown), [16] table words the zero test of the first job
  * reads (zero-skipping jobs: all of them, or the leading half when every
@@ -224,8 +224,10 @@ int phip_batch_set_option(void *plan, int option, int64_t value);
 int phip_batch_stats(void *plan, double *out, int len);
 
 /* The kernel class of every residue of a batch, in input order, as
- * phip_batch_create would plan it: (group * 16 + prime-set size) * 4 +
- * variant (0 plain, 1 paired states, 2 zero-skipping).  Same description
+ * phip_batch_create would plan it: (group * 32 + prime-set size) * 8 +
+ * variant (0 plain, 1 paired states, 2 zero-skipping, 3 exact accumulation,
+ * 4 compact exact accumulation; the set size of an exact class is its limb
+ * column count).  Same description
  * arguments as phip_batch_create; host code only (no device).  Writes
  * min(len, residues) entries and returns the residue count, or -1.  No
  * counterpart in the reference (plan introspection for tests and
@@ -258,6 +260,10 @@ int phip_batch_destroy(void *plan);
                                               below 2^372 (0 off, 1..1024; read when a plan is built; the
                                               exact variant is off for every table while
                                               PHIP_TUNE_EXACT_MIN_PRIMES is 0) */
+#define PHIP_TUNE_EXACT_COMPACT 12 /* 1 (default): tables of 9..16 groups of 2 rows whose partial bound
+                                      products fit take the compact exact classes (product tree sized for
+                                      22-bit group values); 0: the 31-bit exact classes.  Read when a plan is
+                                      built */
 int phip_set_tuning(int key, int64_t value);
 int64_t phip_get_tuning(int key);
 

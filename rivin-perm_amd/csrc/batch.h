@@ -25,6 +25,7 @@ constexpr int kVariantPlain = 0;
 constexpr int kVariantPaired = 1;     // consecutive states paired (primes below kPairedPrimeLimit)
 constexpr int kVariantSkipZeros = 2;  // zero terms skipped through a per-wavefront queue
 constexpr int kVariantExact = 3;      // exact accumulation: one walk for any number of primes
+constexpr int kVariantExactCompact = 4;  // the same with the product tree sized for kCompactValueBits-bit values
 // The exact variant's classes: a class's set size is its number of limb
 // columns L.  L = 8 takes tables of at most eight groups (full products);
 // L = 10 and 12 take up to sixteen groups when the product of the table's
@@ -33,6 +34,12 @@ constexpr int kExactColumns = 8;
 constexpr int kExactMaxGroups = 8;
 constexpr int kExactWideMaxGroups = 16;
 constexpr int kExactMaxColumns = 12;
+// The compact exact classes (groups of 2 rows, 10 limb columns): a product
+// of c >= 2 group values keeps ceil(22 c / 31) limbs (LimbProduct with
+// V = 22), for tables whose partial bound products fit (the products of two
+// row sums of entries up to about 100 at orders 24..28 are near 2^21).
+constexpr int kCompactValueBits = 22;
+constexpr bool is_exact_variant(int variant) { return variant == kVariantExact || variant == kVariantExactCompact; }
 
 // One job: one table (a matrix under one row order) and a set of K odd
 // moduli below 2^31 that share it (K = 1 for residue rows).  56 bytes,
@@ -48,7 +55,7 @@ struct BatchJob {
     int32_t group;
     int32_t nprimes;     // real primes (<= K); padded entries repeat the last one
     int32_t set;         // K of the job's launch class
-    int32_t variant;     // kVariantPlain, kVariantPaired, kVariantSkipZeros or kVariantExact
+    int32_t variant;     // kVariantPlain, kVariantPaired, kVariantSkipZeros, kVariantExact or kVariantExactCompact
     int32_t test_words;  // kVariantSkipZeros: leading table words the zero test reads (skip_test_words)
 };
 
@@ -221,6 +228,9 @@ PHIP_DECLARE_BATCH_LAUNCH_EXACT(3, 12)
 PHIP_DECLARE_BATCH_LAUNCH_EXACT(4, 10)
 PHIP_DECLARE_BATCH_LAUNCH_EXACT(4, 12)
 #undef PHIP_DECLARE_BATCH_LAUNCH_EXACT
+// The compact exact class (groups of 2 rows, 10 limb columns).
+hipError_t launch_batch_g2_k10_compact(dim3, unsigned, hipStream_t, const BatchJob*, const uint2*, uint32_t, uint32_t*,
+                                       const int32_t*, const int32_t*, const uint32_t*, uint64_t*);
 
 // Workgroups per CU each class kernel can keep resident with lds_bytes of
 // dynamic LDS per workgroup (batch.hip), or -1.
@@ -246,6 +256,7 @@ PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k8_exact) PHIP_DECLARE_BATCH_OCC(occup
 PHIP_DECLARE_BATCH_OCC(occupancy_batch_g2_k10_exact) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g2_k12_exact)
 PHIP_DECLARE_BATCH_OCC(occupancy_batch_g3_k10_exact) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g3_k12_exact)
 PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k10_exact) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k12_exact)
+PHIP_DECLARE_BATCH_OCC(occupancy_batch_g2_k10_compact)
 #undef PHIP_DECLARE_BATCH_OCC
 
 }  // namespace phip

@@ -18,7 +18,8 @@
 // zero terms (ZeroSkipSink), whose item sums cover every lane, and
 // PHIP_BATCH_EXACT=1 the exact-accumulation variant (ExactSink), whose items
 // hold PHIP_BATCH_SET limb columns of the exact sums for all of a job's
-// primes.
+// primes, and PHIP_BATCH_COMPACT=1 (with it) the form whose product tree is
+// sized for kCompactValueBits-bit group values (LimbProduct).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -40,6 +41,14 @@
 #ifndef PHIP_BATCH_EXACT
 #define PHIP_BATCH_EXACT 0
 #endif
+#ifndef PHIP_BATCH_COMPACT
+#define PHIP_BATCH_COMPACT 0
+#endif
+#if PHIP_BATCH_COMPACT && !PHIP_BATCH_EXACT
+#error "the compact classes are exact-accumulation classes"
+#endif
+// Width of the values the exact classes' product trees are sized for.
+constexpr int kBatchValueBits = PHIP_BATCH_COMPACT ? phip::kCompactValueBits : 31;
 
 namespace phip {
 
@@ -71,19 +80,19 @@ __device__ __forceinline__ void run_item(const int32_t* __restrict__ tables, con
 // kExactMaxGroups groups, the wider ones tables of more groups, up to
 // kExactWideMaxGroups; other orders are never planned on the class and
 // compile to nothing.
-template <int N, int G, int L>
+template <int N, int G, int L, int V>
 __device__ __forceinline__ void run_item_exact(const int32_t* __restrict__ tables, const BatchJob& jb, uint64_t base,
                                                int64_t (&cols)[L]) {
     constexpr int kGroups = G > 0 ? (N + G - 1) / G : 0;
     if constexpr (G > 0 && (L == kExactColumns ? kGroups <= kExactMaxGroups
                                                : (kGroups > kExactMaxGroups && kGroups <= kExactWideMaxGroups))) {
-        glynn_chunk_exact<N, G, L>(tables, N, jb.cbits, base, cols);
+        glynn_chunk_exact<N, G, L, V>(tables, N, jb.cbits, base, cols);
     }
 }
 
 static_assert(kZeroQueueEntries == kZeroQueue, "zero-skip queue sizes disagree");
 
-template <int G, int K, bool P, bool Z, bool E>
+template <int G, int K, bool P, bool Z, bool E, int V>
 __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __restrict__ jobs,
                                                          const uint2* __restrict__ items, uint32_t n_items,
                                                          uint32_t* __restrict__ counter,
@@ -122,7 +131,7 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
             switch (jb.n) {
 #define PHIP_EXACT_CASE(N) \
     case N:                \
-        run_item_exact<N, G, K>(tables, jb, base, cols); \
+        run_item_exact<N, G, K, V>(tables, jb, base, cols); \
         break;
                 PHIP_EXACT_CASE(5) PHIP_EXACT_CASE(6) PHIP_EXACT_CASE(7) PHIP_EXACT_CASE(8)
                 PHIP_EXACT_CASE(9) PHIP_EXACT_CASE(10) PHIP_EXACT_CASE(11) PHIP_EXACT_CASE(12)
@@ -199,7 +208,11 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
 #if PHIP_BATCH_SET != 8 && PHIP_BATCH_SET != 10 && PHIP_BATCH_SET != 12
 #error "exact classes hold 8, 10 or 12 limb columns per item"
 #endif
+#if PHIP_BATCH_COMPACT
+#define PHIP_PAIRED_SUFFIX _compact
+#else
 #define PHIP_PAIRED_SUFFIX _exact
+#endif
 #elif PHIP_BATCH_PAIRED
 #define PHIP_PAIRED_SUFFIX _paired
 #elif PHIP_BATCH_SKIP_ZEROS
@@ -215,7 +228,7 @@ hipError_t PHIP_LAUNCH_NAME(PHIP_BATCH_GROUP, PHIP_BATCH_SET, PHIP_PAIRED_SUFFIX
                                                               const int32_t* set_p, const uint32_t* set_pinv,
                                                               uint64_t* item_partials) {
     hipLaunchKernelGGL((batch_glynn_kernel<PHIP_BATCH_GROUP, PHIP_BATCH_SET, PHIP_BATCH_PAIRED != 0,
-                                           PHIP_BATCH_SKIP_ZEROS != 0, PHIP_BATCH_EXACT != 0>),
+                                           PHIP_BATCH_SKIP_ZEROS != 0, PHIP_BATCH_EXACT != 0, kBatchValueBits>),
                        grid, dim3(256), lds_bytes, stream, jobs,
                        items, n_items, counter, blob, set_p, set_pinv, item_partials, lds_bytes / 16u);
     return hipGetLastError();
@@ -230,7 +243,7 @@ int PHIP_OCC_NAME(PHIP_BATCH_GROUP, PHIP_BATCH_SET, PHIP_PAIRED_SUFFIX)(unsigned
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &blocks,
             batch_glynn_kernel<PHIP_BATCH_GROUP, PHIP_BATCH_SET, PHIP_BATCH_PAIRED != 0, PHIP_BATCH_SKIP_ZEROS != 0,
-                               PHIP_BATCH_EXACT != 0>,
+                               PHIP_BATCH_EXACT != 0, kBatchValueBits>,
             256, lds_bytes) != hipSuccess)
         return -1;
     return blocks;

@@ -15,6 +15,7 @@
 
 #include "batch.h"
 #include "device_context.h"
+#include "exact_limbs.h"
 #include "launchers.h"
 #include "permhip.h"
 #include "phip_arith.h"
@@ -40,7 +41,7 @@ __global__ __launch_bounds__(256) void batch_finalize_kernel(const BatchJob* __r
     const BatchSlot sl = slots[blockIdx.x];
     const BatchJob jb = jobs[sl.job];
     const uint64_t p = sl.p;
-    if (jb.variant == kVariantExact) {
+    if (is_exact_variant(jb.variant)) {
         // The job's items hold jb.set limb columns of exact signed sums,
         // |column| < 2^(37 + cbits) per item (cbits <= 24), so runs of
         // 2^(24 - cbits) items add exactly in int64 (< 2^61) before each
@@ -127,6 +128,7 @@ BatchLaunchFn launcher_for(int group, int set, int variant) {
         if (group == 4 && set == 10) return launch_batch_g4_k10_exact;
         if (group == 4 && set == 12) return launch_batch_g4_k12_exact;
     }
+    if (variant == kVariantExactCompact && group == 2 && set == 10) return launch_batch_g2_k10_compact;
 #undef PHIP_LAUNCHER
 #undef PHIP_LAUNCHER_PAIRED
 #undef PHIP_LAUNCHER_SKIP
@@ -161,6 +163,7 @@ int occupancy_for(int group, int set, int variant, unsigned lds) {
         if (group == 4 && set == 10) return occupancy_batch_g4_k10_exact(lds);
         if (group == 4 && set == 12) return occupancy_batch_g4_k12_exact(lds);
     }
+    if (variant == kVariantExactCompact && group == 2 && set == 10) return occupancy_batch_g2_k10_compact(lds);
 #undef PHIP_OCC
 #undef PHIP_OCC_PAIRED
 #undef PHIP_OCC_SKIP
@@ -260,6 +263,25 @@ int exact_columns(int groups, int bits, size_t primes) {
     if (bits <= 31 * 10) return 10;
     if (bits <= 31 * kExactMaxColumns) return kExactMaxColumns;
     return 0;
+}
+
+// Whether a table of groups of G rows (row bounds `S` in kernel row order)
+// fits the compact exact classes: every node [lo, hi) of the product tree
+// (LimbProduct<0, groups, L, kCompactValueBits>: split at limb_split) whose
+// limb count is below L holds its exact product, i.e. the product of its
+// rows' bounds has at most 31 limbs_for(hi - lo, V) bits.  Nodes of L limbs
+// are products modulo 2^(31 L), exact at the root by the table's bound.
+bool compact_nodes_fit(const std::vector<uint64_t>& S, int G, int lo, int hi, int L) {
+    const int count = hi - lo;
+    if (count <= 1) return true;
+    const int limbs = limbs_for(count, kCompactValueBits);
+    if (limbs < L) {
+        const int n = (int)S.size();
+        const std::vector<uint64_t> rows(S.begin() + lo * G, S.begin() + std::min(n, hi * G));
+        if (bound_bits(rows) > 31 * limbs) return false;
+    }
+    const int mid = lo + limb_split(count);
+    return compact_nodes_fit(S, G, lo, mid, L) && compact_nodes_fit(S, G, mid, hi, L);
 }
 
 // Estimated share of the Gray states whose term is nonzero, for a 0/1
@@ -368,6 +390,7 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
         int group;
         uint64_t data_off;
         bool skip_zeros;
+        bool compact;    // groups of 2 rows whose partial bound products fit the compact exact classes
         int test_words;  // skip_zeros: the walk's zero-test width (skip_plan)
         int bits;  // bound_bits of the table's row bounds
         std::vector<size_t> inputs;
@@ -438,7 +461,16 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
             }
         }
         if (found == sets.size()) {
-            sets.push_back({x.key, x.n, group, (uint64_t)out.blob.size(), skip, test_words,
+            bool compact = false;
+            if (group == 2 && !skip && tuning().exact_compact) {
+                const int groups = groups_of(x.n, group);
+                if (groups > kExactMaxGroups && groups <= kExactWideMaxGroups) {
+                    std::vector<uint64_t> kernel_bounds(x.n);
+                    for (int r = 0; r < x.n; ++r) kernel_bounds[r] = bounds[order.empty() ? r : order[r]];
+                    compact = compact_nodes_fit(kernel_bounds, group, 0, groups, 10);
+                }
+            }
+            sets.push_back({x.key, x.n, group, (uint64_t)out.blob.size(), skip, compact, test_words,
                             group > 0 ? bound_bits(bounds) : 0, {}});
             sets_of_key[x.key].push_back(found);
             out.blob.insert(out.blob.end(), table.begin(), table.end());
@@ -466,7 +498,7 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
             jb.set = columns;
             jb.nprimes = (int32_t)st.inputs.size();
             jb.prime_off = (uint32_t)out.set_p.size();
-            jb.variant = kVariantExact;
+            jb.variant = (columns == 10 && st.compact) ? kVariantExactCompact : kVariantExact;
             const uint32_t job = (uint32_t)out.jobs.size();
             const int L = st.n - 1;
             for (size_t k = 0; k < st.inputs.size(); ++k) {
@@ -549,7 +581,7 @@ int assign_items(PlanHost& out, int items_log2, int min_cbits) {
         const int s = (int)std::lround(std::log2(std::max(1.0, states / items)));
         return std::min(std::max(s, 10), 30);
     };
-    auto class_key = [](const BatchJob& jb) { return (jb.group * 16 + jb.set) * 4 + jb.variant; };
+    auto class_key = [](const BatchJob& jb) { return (jb.group * 32 + jb.set) * 8 + jb.variant; };
     double job_states = 0;
     std::map<int, double> class_states;
     for (const BatchJob& jb : out.jobs) {
@@ -564,14 +596,15 @@ int assign_items(PlanHost& out, int items_log2, int min_cbits) {
         c = std::min(std::max(c, min_cbits), L);
         // Exact jobs keep a lane's columns below 2^55 and an item's below
         // 2^61 (the finalisation's run length relies on it).
-        if (jb.variant == kVariantExact) c = std::min(c, 24);
+        if (is_exact_variant(jb.variant)) c = std::min(c, 24);
         jb.cbits = c;
         jb.lanes = 1ull << (L - c);
     }
 
     // Items: per (group, set, variant) class, jobs by descending cost, each
     // job's items contiguous.
-    static const int kClasses[][3] = {{2, 8, 3}, {3, 8, 3}, {4, 8, 3}, {6, 8, 3},
+    std::vector<bool> placed(out.jobs.size(), false);
+    static const int kClasses[][3] = {{2, 8, 3}, {3, 8, 3}, {4, 8, 3}, {6, 8, 3}, {2, 10, 4},
                                       {2, 10, 3}, {2, 12, 3}, {3, 10, 3}, {3, 12, 3}, {4, 10, 3}, {4, 12, 3},
                                       {0, 1, 0},
                                       {2, 8, 1}, {2, 8, 0}, {2, 4, 1}, {2, 4, 0}, {2, 2, 0}, {2, 1, 0},
@@ -603,11 +636,19 @@ int assign_items(PlanHost& out, int items_log2, int min_cbits) {
             }
             b.first_item = (uint32_t)out.items.size();
             b.n_items = (uint32_t)waves;
+            placed[j] = true;
             for (uint64_t w = 0; w < waves; ++w) out.items.push_back(make_uint2(j, (uint32_t)w));
         }
         cl.n_items = (uint32_t)out.items.size() - cl.first_item;
         out.classes.push_back(cl);
     }
+    // A job of a class missing from the list above would get no items and
+    // its residues would read as zero.
+    for (bool p : placed)
+        if (!p) {
+            set_error("internal: a batch job has no kernel class");
+            return -1;
+        }
     return 0;
 }
 
@@ -1066,7 +1107,7 @@ int phip_batch_plan_classes(const int64_t* entries, const int32_t* orders, int c
     const int m = std::min<int>(len, (int)h.slots.size());
     for (int i = 0; i < m; ++i) {
         const phip::BatchJob& jb = h.jobs[h.slots[i].job];
-        out[i] = (jb.group * 16 + jb.set) * 4 + jb.variant;
+        out[i] = (jb.group * 32 + jb.set) * 8 + jb.variant;
     }
     return (int)h.slots.size();
 }

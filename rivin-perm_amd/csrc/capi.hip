@@ -149,6 +149,10 @@ int phip_set_tuning(int key, int64_t value) {
             if (value < 0 || value > 1024) break;
             t.exact_wide_min_primes = (int)value;
             return 0;
+        case PHIP_TUNE_EXACT_COMPACT:
+            if (value < 0 || value > 1) break;
+            t.exact_compact = (int)value;
+            return 0;
         default:
             phip::set_error("unknown tuning key");
             return -1;
@@ -171,6 +175,7 @@ int64_t phip_get_tuning(int key) {
         case PHIP_TUNE_SKIP_ZEROS_PERMILLE: return t.skip_zeros_permille;
         case PHIP_TUNE_EXACT_MIN_PRIMES: return t.exact_min_primes;
         case PHIP_TUNE_EXACT_WIDE_MIN_PRIMES: return t.exact_wide_min_primes;
+        case PHIP_TUNE_EXACT_COMPACT: return t.exact_compact;
         default:
             phip::set_error("unknown tuning key");
             return -1;

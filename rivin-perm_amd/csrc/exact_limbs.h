@@ -168,29 +168,41 @@ PHIP_LIMB_FN void limb_mul_trunc(const uint32_t (&a)[LA], const uint32_t (&b)[LB
     }
 }
 
-// Product of m[Lo..Hi) (each below 2^31) modulo 2^(31 L), in
-// min(Hi - Lo, L) limbs: the product of n values below 2^31 is below
-// 2^(31 n).  Split at the largest power of two below the count, so eight
-// values multiply as limb_product8 does.
+// Product of m[Lo..Hi) (each below 2^31) modulo 2^(31 L).  Split at the
+// largest power of two below the count, so eight values multiply as
+// limb_product8 does.
+//
+// V is the width the caller guarantees for the partial products: a node of
+// c >= 2 values keeps limbs_for(c, V) = ceil(c V / 31) limbs (at most L), so
+// its truncated product is its exact product only when that product is
+// below 2^(31 limbs_for(c, V)).  With V = 31 that holds for any values below
+// 2^31 (c values, c limbs).  A smaller V (values of about V bits, such as
+// the products of two row sums of a large-entry table) drops the limb
+// columns that must be zero; the host checks every internal node's bound
+// product against its limbs before it picks such a kernel
+// (compact_nodes_fit in the batch planner).
 constexpr int limb_split(int count) {
     int s = 1;
     while (2 * s < count) s *= 2;
     return s;
 }
 
-template <int Lo, int Hi, int L>
+constexpr int limbs_for(int count, int V) { return count <= 1 ? 1 : (count * V + 30) / 31; }
+
+template <int Lo, int Hi, int L, int V = 31>
 struct LimbProduct {
     static constexpr int kCount = Hi - Lo;
-    static constexpr int kLimbs = kCount < L ? kCount : L;
+    static constexpr int kLimbs = limbs_for(kCount, V) < L ? limbs_for(kCount, V) : L;
     template <int M>
     PHIP_LIMB_MEMBER void run(const uint32_t (&m)[M], uint32_t (&out)[kLimbs]) {
         static_assert(Hi <= M && kCount >= 1, "range outside the values");
+        static_assert(V >= 1 && V <= 31, "values are below 2^31");
         if constexpr (kCount == 1) {
             out[0] = m[Lo];
         } else {
             constexpr int kMid = Lo + limb_split(kCount);
-            using A = LimbProduct<Lo, kMid, L>;
-            using B = LimbProduct<kMid, Hi, L>;
+            using A = LimbProduct<Lo, kMid, L, V>;
+            using B = LimbProduct<kMid, Hi, L, V>;
             uint32_t x[A::kLimbs], y[B::kLimbs];
             A::run(m, x);
             B::run(m, y);

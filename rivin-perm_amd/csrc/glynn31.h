@@ -560,8 +560,12 @@ struct PrimeSetSink {
 // |a_ij|, each bounding |row sum| in every state) is below 2^(31 L).  The
 // planner sends a table here only then.
 //
+// V < 31 (the compact classes) sizes the product tree's internal nodes for
+// values of about V bits (LimbProduct); the planner sends a table there only
+// when every node's bound product fits its limbs.
+//
 // Every limb is below 2^31, so after 2^cbits states |acc[c]| < 2^(31 + cbits).
-template <int NR, int G, int L>
+template <int NR, int G, int L, int V = 31>
 struct ExactSink {
     static_assert(G > 0, "exact accumulation needs exact packed row sums");
     using Rows = PackedRows<NR, G>;
@@ -592,7 +596,7 @@ struct ExactSink {
 #pragma unroll
             for (int c = 0; c < 8; ++c) acc[c] = mad_exact(acc[c], (int32_t)z[c], e);
         } else {
-            using P = LimbProduct<0, kGroups, L>;
+            using P = LimbProduct<0, kGroups, L, V>;
             uint32_t z[P::kLimbs];
             P::run(m, z);
 #pragma unroll
@@ -850,10 +854,10 @@ __device__ __forceinline__ void glynn_chunk_set(const int32_t* __restrict__ tabl
 
 // The lane's exact signed sum of terms as L int64 columns (value
 // sum_c out[c] 2^(31 c)); ExactSink states which tables each L takes.
-template <int NR, int G, int L>
+template <int NR, int G, int L, int V = 31>
 __device__ __forceinline__ void glynn_chunk_exact(const int32_t* __restrict__ tables, int n, int cbits,
                                                   uint64_t base, int64_t (&out)[L]) {
-    ExactSink<NR, G, L> sink;
+    ExactSink<NR, G, L, V> sink;
 #pragma unroll
     for (int c = 0; c < L; ++c) sink.acc[c] = 0;
     const bool neg = gray_walk<NR, G>(tables, n, cbits, base, 0, sink);

@@ -26,6 +26,7 @@ struct Tuning {
     int skip_zeros_permille = 250;
     int exact_min_primes = 6;     // fewest primes of a table of <= 8 groups that take the exact variant (0 off)
     int exact_wide_min_primes = 6;  // the same for 9..16 groups (row-bound product below 2^372)
+    int exact_compact = 1;          // groups of 2 rows may take the compact exact classes (0 off)
 };
 Tuning& tuning();
 

@@ -112,6 +112,7 @@ TUNE_AFFINE_MIN_TIER = 8
 TUNE_SKIP_ZEROS_PERMILLE = 9
 TUNE_EXACT_MIN_PRIMES = 10
 TUNE_EXACT_WIDE_MIN_PRIMES = 11
+TUNE_EXACT_COMPACT = 12
 BATCH_BLOCKS_PER_CU = 1
 BATCH_USE_GRAPH = 2
 BATCH_ITEMS_LOG2 = 3
@@ -470,7 +471,7 @@ def batch_plan_classes(matrices, moduli_lists) -> list:
                                         moduli.ctypes.data_as(_I64P), counts.ctypes.data_as(_I32P),
                                         out.ctypes.data_as(_I32P), total)
     _check(count, "phip_batch_plan_classes")
-    return [(int(c) // 64, (int(c) // 4) % 16, int(c) % 4) for c in out[:count]]
+    return [(int(c) // 256, (int(c) // 8) % 32, int(c) % 8) for c in out[:count]]
 
 
 class Batch:

@@ -30,8 +30,10 @@ ARCH = "gfx950"
 # kernel.  Variant 0 is the plain kernel; sets of 4 and 8 primes below
 # 1.25 * 2^30 also have the paired-state variant (1), 0/1 tables in groups
 # of 4 or 6 the zero-skipping variant (2) for sets of 2, 4 and 8, and every
-# group the exact-accumulation variant (3, eight limb columns per item).
-BATCH_VARIANT_SUFFIX = {0: "", 1: "_paired", 2: "_skip", 3: "_exact"}
+# group the exact-accumulation variant (3, eight limb columns per item), and
+# groups of 2 rows its compact form (4: the product tree sized for 22-bit
+# group values, ten limb columns).
+BATCH_VARIANT_SUFFIX = {0: "", 1: "_paired", 2: "_skip", 3: "_exact", 4: "_compact"}
 BATCH_CLASSES = (
     ((0, 1, 0),)
     + tuple((g, k, 0) for g in (2, 3, 4, 6) for k in (1, 2, 4, 8))
@@ -39,6 +41,7 @@ BATCH_CLASSES = (
     + tuple((g, k, 2) for g in (4, 6) for k in (2, 4, 8))
     + tuple((g, 8, 3) for g in (2, 3, 4, 6))
     + tuple((g, k, 3) for g in (2, 3, 4) for k in (10, 12))
+    + ((2, 10, 4),)
 )
 PLAIN_SOURCES = (
     "device_context.hip",
@@ -101,7 +104,8 @@ def build(jobs: int | None = None, verbose: bool = False) -> Path:
                 CSRC / "batch.hip",
                 BUILD_DIR / f"batch_g{group}_k{k}{BATCH_VARIANT_SUFFIX[variant]}.o",
                 [f"-DPHIP_BATCH_GROUP={group}", f"-DPHIP_BATCH_SET={k}", f"-DPHIP_BATCH_PAIRED={int(variant == 1)}",
-                 f"-DPHIP_BATCH_SKIP_ZEROS={int(variant == 2)}", f"-DPHIP_BATCH_EXACT={int(variant == 3)}"],
+                 f"-DPHIP_BATCH_SKIP_ZEROS={int(variant == 2)}", f"-DPHIP_BATCH_EXACT={int(variant in (3, 4))}",
+                 f"-DPHIP_BATCH_COMPACT={int(variant == 4)}"],
             )
         )
     for name in PLAIN_SOURCES:

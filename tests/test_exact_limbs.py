@@ -140,3 +140,104 @@ def test_truncated_products_match_python(truncated_harness):
             prod *= v
         want = prod % (1 << (31 * len(got)))
         assert sum(x << (31 * i) for i, x in enumerate(got)) == want, (limbs, vals)
+
+
+def _compact_limbs(count, value_bits=22):
+    return 1 if count <= 1 else (count * value_bits + 30) // 31
+
+
+def _compact_fits(vals, lo, hi, limbs_cap, value_bits=22):
+    """The compact tree's condition, written from its definition: every node
+    (split at the largest power of two below its count) whose limb count is
+    below the cap holds a product of at most 31 * limbs bits."""
+    count = hi - lo
+    if count <= 1:
+        return True
+    limbs = _compact_limbs(count, value_bits)
+    if limbs < limbs_cap:
+        prod = 1
+        for v in vals[lo:hi]:
+            prod *= v
+        if prod.bit_length() > 31 * limbs:
+            return False
+    split = 1
+    while 2 * split < count:
+        split *= 2
+    return (_compact_fits(vals, lo, lo + split, limbs_cap, value_bits)
+            and _compact_fits(vals, lo + split, hi, limbs_cap, value_bits))
+
+
+def _compact_harness_source():
+    cases = []
+    for count in range(1, 17):
+        cases.append(
+            f"        if (c == {count}) {{\n"
+            f"            using P = LimbProduct<0, {count}, 10, 22>;\n"
+            f"            uint32_t out[P::kLimbs];\n"
+            f"            P::run(in, out);\n"
+            f"            for (int i = 0; i < P::kLimbs; ++i) std::printf(\"%u \", out[i]);\n"
+            f"        }}\n"
+        )
+    return (
+        "#include <cstdio>\n#include <cstdint>\n#include \"exact_limbs.h\"\nusing namespace phip;\n"
+        "int main() {\n    int c;\n"
+        "    while (std::scanf(\"%d\", &c) == 1) {\n"
+        "        uint32_t in[16] = {};\n"
+        "        for (int i = 0; i < c; ++i) { unsigned v; std::scanf(\"%u\", &v); in[i] = v; }\n"
+        + "".join(cases)
+        + "        std::printf(\"\\n\");\n    }\n    return 0;\n}\n"
+    )
+
+
+@pytest.fixture(scope="module")
+def compact_harness(tmp_path_factory):
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    d = tmp_path_factory.mktemp("limbs_compact")
+    src = d / "c.cpp"
+    src.write_text(_compact_harness_source())
+    exe = d / "c"
+    subprocess.run([gxx, "-O2", "-std=c++17", "-I", str(CSRC), str(src), "-o", str(exe)], check=True)
+    return exe
+
+
+def test_compact_products_match_python(compact_harness):
+    """LimbProduct<0, c, 10, 22> (the compact exact classes' tree): nodes of
+    c >= 2 values keep ceil(22 c / 31) limbs.  Whenever every node's product
+    fits its limbs (the planner's condition, restated above) the result is
+    the product modulo 2^(31 min(10, ceil(22 c / 31))), in limbs below 2^31;
+    values of up to 22 bits always fit, and wider values that still fit put
+    the nodes at their limits.  A value set that breaks a node's limit gives
+    a different number, so the condition is not vacuous."""
+    rng = random.Random(2222)
+    fitting, broken = [], []
+    for count in range(1, 17):
+        fitting.append([(1 << 22) - 1] * count)
+        fitting.append([1] * count)
+        for _ in range(400):
+            if rng.random() < 0.5:
+                vals = [rng.randrange(1 << 22) for _ in range(count)]
+            else:
+                vals = [rng.randrange(1 << rng.randrange(1, 32)) | 1 for _ in range(count)]
+            (fitting if _compact_fits(vals, 0, count, 10) else broken).append(vals)
+    assert len(broken) > 100
+    cases = fitting + broken
+    text = "".join(f"{len(v)} {' '.join(map(str, v))}\n" for v in cases)
+    out = subprocess.run([str(compact_harness)], input=text, capture_output=True, text=True,
+                         check=True).stdout.split("\n")
+    mismatches = 0
+    for i, (vals, line) in enumerate(zip(cases, out)):
+        got = [int(x) for x in line.split()]
+        limbs = min(10, _compact_limbs(len(vals)))
+        assert len(got) == limbs
+        assert all(0 <= x < (1 << 31) for x in got)
+        prod = 1
+        for v in vals:
+            prod *= v
+        value = sum(x << (31 * k) for k, x in enumerate(got))
+        if i < len(fitting):
+            assert value == prod % (1 << (31 * limbs)), vals
+        elif value != prod % (1 << (31 * limbs)):
+            mismatches += 1
+    assert mismatches > 0

@@ -30,7 +30,10 @@ def test_batch_golden_covers_every_plan_class():
     # The classes a 4096-matrix plan of entries U[-100, 100] uses: packed row
     # sums in groups of 2 or 3; modular sets of 1, 2, 4 and 8 primes, exact
     # walks with 8 limb columns (at most eight groups, orders 16..24) and
-    # 10 (the orders 24..28, whose row-bound products stay below 2^310).
+    # 10 (the orders 24..28, whose row-bound products stay below 2^310),
+    # the tables of 2-row groups among them on the compact classes (their
+    # partial bound products fit 22-bit group values).
     assert {c[0] for c in classes} <= {2, 3}
-    assert {c[1] for c in classes if c[2] != 3} <= {1, 2, 4, 8}
+    assert {c[1] for c in classes if c[2] not in (3, 4)} <= {1, 2, 4, 8}
     assert {c[1] for c in classes if c[2] == 3} == {8, 10}
+    assert {(c[0], c[1]) for c in classes if c[2] == 4} == {(2, 10)}

@@ -131,19 +131,26 @@ def _row_bound_bits(A):
     return prod.bit_length()
 
 
-# (order, entry bound, group size, limb columns): 9..16 groups of 2, 3 or 4
-# rows, on both sides of the 2^310 limit of the 10-column class.
-WIDE_SHAPES = [(17, 1500, 2, 10), (20, 1000, 2, 10), (26, 100, 3, 10), (28, 100, 2, 10), (30, 300, 2, 12),
-               (32, 20, 3, 10), (33, 6, 4, 10), (34, 50, 3, 12)]
+# (order, entry bound, group size, limb columns, variant): 9..16 groups of
+# 2, 3 or 4 rows, on both sides of the 2^310 limit of the 10-column class;
+# groups of 2 rows whose partial bound products fit 22-bit values take the
+# compact classes (variant 4).
+WIDE_SHAPES = [(17, 1500, 2, 10, 3), (20, 1000, 2, 10, 3), (26, 100, 3, 10, 3), (28, 100, 2, 10, 4),
+               (19, 250, 2, 10, 4), (25, 120, 2, 10, 4), (30, 300, 2, 12, 3), (32, 20, 3, 10, 3), (33, 6, 4, 10, 3),
+               (34, 50, 3, 12, 3)]
 
 
-@pytest.mark.parametrize("n,bound,group,columns", WIDE_SHAPES)
-def test_wide_exact_matches_oracle_and_modular(native, oracle_c, n, bound, group, columns):
+def _exact(c):
+    return c[2] in (3, 4)
+
+
+@pytest.mark.parametrize("n,bound,group,columns,variant", WIDE_SHAPES)
+def test_wide_exact_matches_oracle_and_modular(native, oracle_c, n, bound, group, columns, variant):
     rng = np.random.default_rng(1900 + 31 * n + bound)
     A = rng.integers(-bound, bound + 1, (n, n)).astype(np.int64)
     for k in (8, 10, 16):
         primes = PRIMES[:k]
-        assert _classes(native, A, primes) == {(group, columns, 3)}, (n, bound, k, _row_bound_bits(A))
+        assert _classes(native, A, primes) == {(group, columns, variant)}, (n, bound, k, _row_bound_bits(A))
         got = native.permanent_residues(A, primes)
         modular = _with_tuning(native, native.TUNE_EXACT_MIN_PRIMES, 0, lambda: native.permanent_residues(A, primes))
         assert got == modular, (n, bound, k)
@@ -151,20 +158,26 @@ def test_wide_exact_matches_oracle_and_modular(native, oracle_c, n, bound, group
             assert got == [oracle_c.permanent_mod_p(A.tolist(), p, 8) for p in primes], (n, bound, k)
 
 
-@pytest.mark.parametrize("entry,columns", [(76, 10), (77, 12), (350, 12)])
-def test_wide_exact_at_the_truncation_limit(native, entry, columns):
+@pytest.mark.parametrize("entry,columns,variant", [(76, 10, 4), (77, 12, 3), (350, 12, 3)])
+def test_wide_exact_at_the_truncation_limit(native, entry, columns, variant):
     """28 x 28 tables of entries +-entry: every row bound is 28 * entry and
     the largest state product equals the row-bound product, which sits just
     below 2^310 (76), just above it (77: 12 columns) and just below 2^372
-    (350).  The product kept modulo 2^(31 L) must still be exact."""
+    (350).  The product kept modulo 2^(31 L) must still be exact.  At 76
+    the group values reach 2^22.1, and the compact tree's nodes of 4, 6 and
+    8 groups are within 5, 22 and 9 bits of their 3, 5 and 6 limbs."""
     rng = np.random.default_rng(entry)
     A = (rng.choice((-1, 1), (28, 28)) * entry).astype(np.int64)
     bits = _row_bound_bits(A)
     assert bits <= 31 * columns and (columns == 10 or bits > 310)
     primes = PRIMES[:10]
-    assert _classes(native, A, primes) == {(2, columns, 3)}
-    assert native.permanent_residues(A, primes) == _with_tuning(
-        native, native.TUNE_EXACT_MIN_PRIMES, 0, lambda: native.permanent_residues(A, primes))
+    assert _classes(native, A, primes) == {(2, columns, variant)}
+    want = _with_tuning(native, native.TUNE_EXACT_MIN_PRIMES, 0, lambda: native.permanent_residues(A, primes))
+    assert native.permanent_residues(A, primes) == want
+    if variant == 4:
+        assert _with_tuning(native, native.TUNE_EXACT_COMPACT, 0,
+                            lambda: _classes(native, A, primes)) == {(2, columns, 3)}
+        assert _with_tuning(native, native.TUNE_EXACT_COMPACT, 0, lambda: native.permanent_residues(A, primes)) == want
 
 
 def test_wide_exact_declines_large_bounds_and_few_primes(native):
@@ -174,12 +187,12 @@ def test_wide_exact_declines_large_bounds_and_few_primes(native):
     rng = np.random.default_rng(3)
     big = rng.integers(-1000, 1001, (32, 32)).astype(np.int64)
     assert _row_bound_bits(big) > 372
-    assert all(c[2] != 3 for c in _classes(native, big, PRIMES[:16]))
+    assert not any(_exact(c) for c in _classes(native, big, PRIMES[:16]))
     A = rng.integers(-100, 101, (28, 28)).astype(np.int64)
-    assert all(c[2] != 3 for c in _classes(native, A, PRIMES[:5]))
-    assert _classes(native, A, PRIMES[:6]) == {(2, 10, 3)}
+    assert not any(_exact(c) for c in _classes(native, A, PRIMES[:5]))
+    assert _classes(native, A, PRIMES[:6]) == {(2, 10, 4)}
     assert _with_tuning(native, native.TUNE_EXACT_WIDE_MIN_PRIMES, 0,
-                        lambda: all(c[2] != 3 for c in _classes(native, A, PRIMES[:10])))
+                        lambda: not any(_exact(c) for c in _classes(native, A, PRIMES[:10])))
 
 
 def test_wide_exact_long_chunks(native):
@@ -191,3 +204,46 @@ def test_wide_exact_long_chunks(native):
     want = _with_tuning(native, native.TUNE_EXACT_MIN_PRIMES, 0, lambda: native.permanent_residues(A, primes))
     got = _with_tuning(native, native.TUNE_MIN_CBITS, 30, lambda: native.permanent_residues(A, primes))
     assert got == want
+
+
+@pytest.mark.parametrize("entry,variant", [(157, 4), (158, 3)])
+def test_compact_exact_at_its_node_limits(native, oracle_c, entry, variant):
+    """20 x 20 tables whose rows are constant (+-entry, random signs): the
+    first Gray state attains every row bound at once, so its group products
+    (2 rows, 23.2 bits) and the tree's nodes of 4 and 8 groups reach their
+    bounds.  At 157 those nodes need exactly 93 and 186 bits, the most that
+    3 and 6 limbs hold (compact class); at 158 they need 94 and 187 and the
+    planner keeps the 31-bit exact class.  Both agree with the 31-bit exact
+    class, the modular kernels and the oracle."""
+    rng = np.random.default_rng(entry)
+    A = np.outer(rng.choice((-1, 1), 20), np.ones(20, dtype=np.int64)) * entry
+    A = A.astype(np.int64)
+    primes = PRIMES[:10]
+    assert _classes(native, A, primes) == {(2, 10, variant)}
+    got = native.permanent_residues(A, primes)
+    assert got == _with_tuning(native, native.TUNE_EXACT_COMPACT, 0, lambda: native.permanent_residues(A, primes))
+    assert got == _with_tuning(native, native.TUNE_EXACT_MIN_PRIMES, 0, lambda: native.permanent_residues(A, primes))
+    assert got == [oracle_c.permanent_mod_p(A.tolist(), p, 8) for p in primes]
+
+
+def test_compact_exact_random_tables(native, oracle_c):
+    """Random tables of 2-row groups on the compact class at orders 18..28
+    (entries up to a few hundred), against the 31-bit exact class and, up
+    to order 20, the oracle."""
+    rng = np.random.default_rng(4040)
+    seen = 0
+    for _ in range(24):
+        n = int(rng.integers(18, 29))
+        bound = int(rng.integers(90, 400))
+        A = rng.integers(-bound, bound + 1, (n, n)).astype(np.int64)
+        primes = PRIMES[:int(rng.integers(6, 17))]
+        if _classes(native, A, primes) != {(2, 10, 4)}:
+            continue
+        seen += 1
+        print(f"compact n={n} bound={bound} k={len(primes)}", flush=True)
+        got = native.permanent_residues(A, primes)
+        assert got == _with_tuning(native, native.TUNE_EXACT_COMPACT, 0,
+                                   lambda: native.permanent_residues(A, primes)), (n, bound)
+        if n <= 20:
+            assert got == [oracle_c.permanent_mod_p(A.tolist(), p, 8) for p in primes], (n, bound)
+    assert seen >= 8
