@@ -76,14 +76,14 @@ EXACT_MIX_N32_G4 = {
     "v_pk_add_u16/v_pk_sub_i16": 18,
     "v_mul_i32_i24(_sdwa)": 24,
     "v_sub_u32+v_max_i32": 16,
-    "v_xor_b32": 7,
+    "v_bitop3_b32 (sign, three-input XOR)": 4,
     "v_mad_u64_u32": 28,
     "v_and_b32": 17.1,
     "v_lshrrev_b64": 10.1,
     "v_alignbit_b32": 7,
     "v_mad_i64_i32": 8,
     "other": 5.3,
-}
+}  # 137.5 in all; rocprof counts 133.1 per state (SQ_INSTS_VALU)
 
 
 def instruction_mix(n: int, group: int, k: int = 1, paired: bool = False) -> dict:

@@ -51,19 +51,34 @@ __global__ __launch_bounds__(256) void batch_finalize_kernel(const BatchJob* __r
         int64_t run[kExactMaxColumns] = {};
         uint64_t col[kExactMaxColumns] = {};
         uint32_t in_run = 0;
-        for (uint32_t i = threadIdx.x; i < jb.n_items; i += 256) {
-            const uint64_t* part = item_partials + (size_t)(jb.first_item + i) * kItemSlots;
+        const uint64_t* parts = item_partials + (size_t)jb.first_item * kItemSlots;
+        // Four items per pass with every load issued first: the item sums
+        // come from L2 and one item per pass left each thread waiting on a
+        // load per item.
+        constexpr int kAhead = 4;
+        for (uint32_t i0 = threadIdx.x; i0 < jb.n_items; i0 += 256 * kAhead) {
+            int64_t v[kAhead][kExactMaxColumns];
 #pragma unroll
-            for (int c = 0; c < kExactMaxColumns; ++c)
-                if (c < ncols) run[c] += (int64_t)part[c];
-            if (++in_run == run_len) {
+            for (int u = 0; u < kAhead; ++u) {
+                const uint32_t i = i0 + 256u * u;
 #pragma unroll
-                for (int c = 0; c < kExactMaxColumns; ++c) {
-                    const int64_t v = run[c] % (int64_t)p;
-                    col[c] += (uint64_t)(v < 0 ? v + (int64_t)p : v);
-                    run[c] = 0;
+                for (int c = 0; c < kExactMaxColumns; ++c)
+                    v[u][c] = (i < jb.n_items && c < ncols) ? (int64_t)parts[(size_t)i * kItemSlots + c] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < kAhead; ++u) {
+                if (i0 + 256u * u >= jb.n_items) break;
+#pragma unroll
+                for (int c = 0; c < kExactMaxColumns; ++c) run[c] += v[u][c];
+                if (++in_run == run_len) {
+#pragma unroll
+                    for (int c = 0; c < kExactMaxColumns; ++c) {
+                        const int64_t r = run[c] % (int64_t)p;
+                        col[c] += (uint64_t)(r < 0 ? r + (int64_t)p : r);
+                        run[c] = 0;
+                    }
+                    in_run = 0;
                 }
-                in_run = 0;
             }
         }
         uint64_t s = 0, w = 1 % p;

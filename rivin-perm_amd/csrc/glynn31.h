@@ -361,6 +361,14 @@ struct ByteRows {
     }
 };
 
+// a ^ b ^ c as v_bitop3_b32 (truth table 0x96), at the rate of one plain
+// 32-bit logic instruction.
+__device__ __forceinline__ uint32_t xor3_fast(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t r;
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
 // a | b | c as v_bitop3_b32 (truth table 0xfe), which issues at the rate of
 // the plain 32-bit logic instructions, about 1.7x that of v_or3_b32.
 __device__ __forceinline__ uint32_t or3_fast(uint32_t a, uint32_t b, uint32_t c) {
@@ -579,16 +587,18 @@ struct ExactSink {
         rs.groups(g, std::make_integer_sequence<int, kGroups>{});
         constexpr int kValues = L == 8 ? 8 : kGroups;
         uint32_t m[kValues];
-        uint32_t sign = ODD ? 0x80000000u : 0u;
 #pragma unroll
-        for (int k = 0; k < kValues; ++k) {
-            if (k < kGroups) {
-                sign ^= (uint32_t)g[k];
-                m[k] = (uint32_t)(g[k] < 0 ? -g[k] : g[k]);
-            } else {
-                m[k] = 1;
-            }
-        }
+        for (int k = 0; k < kValues; ++k) m[k] = k < kGroups ? (uint32_t)(g[k] < 0 ? -g[k] : g[k]) : 1u;
+        // The term's sign: bit 31 of the XOR of the group products (and of
+        // 0x80000000 for an odd state), folded three at a time.
+        constexpr int kTerms = kGroups + (ODD ? 1 : 0);
+        uint32_t t[kTerms];
+#pragma unroll
+        for (int k = 0; k < kGroups; ++k) t[k] = (uint32_t)g[k];
+        if constexpr (ODD) t[kGroups] = 0x80000000u;
+        uint32_t sign = t[0];
+#pragma unroll
+        for (int k = 1; k < kTerms; k += 2) sign = k + 1 < kTerms ? xor3_fast(sign, t[k], t[k + 1]) : sign ^ t[k];
         const int32_t e = ((int32_t)sign >> 31) | 1;
         if constexpr (L == 8) {
             uint32_t z[8];
