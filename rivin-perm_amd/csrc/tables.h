@@ -11,13 +11,21 @@
 
 namespace phip {
 
+// The representative of v modulo an odd q in [-(q-1)/2, (q-1)/2]: v itself
+// when it already lies there (the common case, no division).
+inline int64_t centred_mod(int64_t v, int64_t q) {
+    const int64_t h = (q - 1) / 2;
+    if (v >= -h && v <= h) return v;
+    const int64_t r = host_floor_mod(v, q);
+    return r > h ? r - q : r;
+}
+
 // Row bounds S_i = sum_j |centred a_ij mod q|.
 inline std::vector<uint64_t> centred_row_bounds(const int64_t* M, int n, int64_t q) {
     std::vector<uint64_t> S(n, 0);
     for (int r = 0; r < n; ++r)
         for (int c = 0; c < n; ++c) {
-            int64_t v = host_floor_mod(M[(size_t)r * n + c], q);
-            if (v > q / 2) v -= q;
+            const int64_t v = centred_mod(M[(size_t)r * n + c], q);
             S[r] += (uint64_t)(v < 0 ? -v : v);
         }
     return S;
@@ -82,6 +90,14 @@ inline int plan_rows(const std::vector<uint64_t>& S, std::vector<int>& order) {
         std::vector<int> out;
         for (auto& grp : groups) {
             std::sort(grp.begin(), grp.end());
+            // The group product's bound does not depend on the arrangement:
+            // a group above it fails every one of them.
+            uint64_t all[6];
+            for (size_t i = 0; i < grp.size(); ++i) all[i] = S[grp[i]];
+            if (bound_product(all, (int)grp.size()) >= (1ull << 31)) {
+                ok = false;
+                break;
+            }
             bool found = false;
             do {
                 uint64_t b[6];
@@ -118,8 +134,12 @@ inline void build_glynn_tables(const int64_t* M, int n, uint32_t q, int group, c
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j) {
             const int src = order.empty() ? i : order[i];
-            const int64_t r = host_floor_mod(M[(size_t)src * n + j], (int64_t)q);
-            v[(size_t)i * n + j] = (group == 0 || r <= (int64_t)(q / 2)) ? r : r - (int64_t)q;
+            const int64_t a = M[(size_t)src * n + j];
+            if (group == 0) {
+                v[(size_t)i * n + j] = (a >= 0 && a < (int64_t)q) ? a : host_floor_mod(a, (int64_t)q);
+            } else {
+                v[(size_t)i * n + j] = centred_mod(a, (int64_t)q);
+            }
         }
     auto put = [&](int col, int row, int64_t value) {
         int32_t* c = out + (size_t)col * W;
