@@ -59,10 +59,18 @@ class PermanentBatch:
         self.products: List[int] = []
         device_mats, device_primes, self.device_index = [], [], []
         self.other_index = []
-        for i, m in enumerate(matrices):
-            rows, arr = _int64_rows(m)
+        converted = [_int64_rows(m) for m in matrices]
+        # Bounds of the int64 matrices in one pass per order.
+        fitting = [i for i, (_, arr) in enumerate(converted) if arr is not None]
+        bulk = dict(zip(fitting, planner.matrix_permanent_bounds([converted[i][1] for i in fitting])))
+        for i, (rows, arr) in enumerate(converted):
             n = len(rows)
-            bound = planner.matrix_permanent_bound(arr if arr is not None else rows) if n else 1
+            if not n:
+                bound = 1
+            elif i in bulk:
+                bound = bulk[i]
+            else:
+                bound = planner.matrix_permanent_bound(rows)
             primes, product = planner.generic_crt_primes(bound) if n > 1 else ([], 1)
             primes = tuple(primes)
             self.rows.append(rows)

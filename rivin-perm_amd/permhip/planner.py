@@ -227,6 +227,32 @@ def matrix_permanent_bound(rows: Sequence[Sequence[int]]) -> int:
     return bound
 
 
+def matrix_permanent_bounds(arrays: Sequence[np.ndarray]) -> List[int]:
+    """matrix_permanent_bound of many square int64 arrays: the arrays of
+    each order are stacked and their row sums of |a| taken at once (exact
+    while n * max|a| < 2^63, as in the single version; other arrays go
+    through it)."""
+    out: List[int] = [0] * len(arrays)
+    by_order: dict = {}
+    for i, a in enumerate(arrays):
+        by_order.setdefault(a.shape[0], []).append(i)
+    for n, idx in by_order.items():
+        if n == 0:
+            for i in idx:
+                out[i] = 1
+            continue
+        stack = np.stack([arrays[i] for i in idx])
+        limit = (1 << 62) // n
+        safe = (stack.max(axis=(1, 2)) < limit) & (stack.min(axis=(1, 2)) > -limit)
+        if not safe.all():
+            stack[~safe] = 0
+        np.abs(stack, out=stack)
+        sums = np.maximum(stack.sum(axis=2), 1).tolist()
+        for j, i in enumerate(idx):
+            out[i] = math.prod(sums[j]) if safe[j] else matrix_permanent_bound(arrays[i])
+    return out
+
+
 _ASCENDING_PRIMES: List[int] = []
 _ASCENDING_PRIMES_LOCK = threading.Lock()
 

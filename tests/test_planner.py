@@ -178,3 +178,26 @@ def test_crt_known_moduli_solves_the_congruences():
         planner.crt_known_moduli([1, 2], (6, 9))
     with pytest.raises(ValueError, match="same length"):
         planner.crt_known_moduli([1], (5, 7))
+
+
+def test_bulk_bounds_equal_single_bounds():
+    """matrix_permanent_bounds (stacked per order) equals
+    matrix_permanent_bound on every array: mixed orders, zero rows
+    (max(1, 0) = 1), entries near int64's ends (the single version's exact
+    fallback) and the empty matrix."""
+    import numpy as np
+
+    rng = np.random.default_rng(8)
+    arrays = []
+    for _ in range(300):
+        n = int(rng.integers(0, 12))
+        bound = int(rng.choice((1, 10, 1000, 1 << 40)))
+        a = rng.integers(-bound, bound + 1, (n, n)).astype(np.int64)
+        if n and rng.random() < 0.2:
+            a[int(rng.integers(n))] = 0
+        arrays.append(a)
+    big = np.full((3, 3), -(1 << 63), dtype=np.int64)
+    big[1, 1] = (1 << 63) - 1
+    arrays.append(big)
+    assert planner.matrix_permanent_bounds(arrays) == [planner.matrix_permanent_bound(a) for a in arrays]
+    assert planner.matrix_permanent_bound(big) == (3 << 63) * ((3 << 63) - 1) * (3 << 63)
