@@ -88,6 +88,42 @@ def test_planner_picks_the_variant(native):
     assert _variant(native, B, BINARY_PRIMES[:4]) != 2.0
 
 
+def _block_ones(sizes, rng):
+    """A block-diagonal 0/1 matrix of all-ones blocks (permanent the
+    product of the blocks' factorials), rows and columns shuffled."""
+    n = sum(sizes)
+    A = np.zeros((n, n), dtype=np.int64)
+    at = 0
+    for k in sizes:
+        A[at:at + k, at:at + k] = 1
+        at += k
+    return A[rng.permutation(n)][:, rng.permutation(n)]
+
+
+@pytest.mark.parametrize("sizes,half", [((4,) * 11, False), ((3,) * 9 + (4,) * 4 + (2,), True)])
+def test_skip_largest_orders_closed_form(native, sizes, half):
+    """Orders 44 and 45 (near the byte walk's largest, 46): eleven J_4
+    blocks (every row of even degree, full-width test) and nine J_3, four
+    J_4 and one J_2 block (18 even rows of 45, half-width test; an order
+    that leaves padding bytes).  2^43 and 2^44 Gray states; the residues
+    equal the product of the blocks' factorials."""
+    rng = np.random.default_rng(len(sizes))
+    A = _block_ones(sizes, rng)
+    n = A.shape[0]
+    primes = BINARY_PRIMES[:4]
+    plan = native.Batch([A], [primes])
+    try:
+        st = plan.stats()
+    finally:
+        plan.close()
+    words = (n + 3) // 4
+    assert st["variant"] == 2.0
+    assert st["skip_test_words"] == ((words + 1) // 2 if half else words)
+    exact = math.prod(math.factorial(k) for k in sizes)
+    print(f"skip largest n={n} half={half}", flush=True)
+    assert native.permanent_residues(A, primes) == [exact % p for p in primes]
+
+
 def test_skip_known_counts(native):
     """Closed forms through the zero-skipping kernels: all-ones J_n
     (n!, rows of even degree for even n), J_n - I (derangements), a zero
