@@ -531,6 +531,9 @@ def main() -> int:
                     help="only the binary36 workload's 4 residues, REPS times (for profilers); "
                          "--skip-zeros-permille sets the zero-skipping threshold")
     ap.add_argument("--skip-zeros-permille", type=int, default=None)
+    ap.add_argument("--skip-blocks", type=int, default=None, choices=(0, 1),
+                    help="0/1 tables on the zero-skipping kernels: 1 walks in blocks of 32 states (the default), "
+                         "0 tests every state")
     ap.add_argument("--frontier", type=int, default=0, metavar="REPS",
                     help="only the frontier DP on the open 12x16 grid's biadjacency (order 96), REPS times")
     ap.add_argument("--batch-only", type=int, default=0, metavar="REPS",
@@ -551,6 +554,9 @@ def main() -> int:
 
     from permhip import _native, planner
     from permhip.batch import PermanentBatch
+
+    if args.skip_blocks is not None:
+        _native.set_tuning(_native.TUNE_SKIP_BLOCKS, args.skip_blocks)
 
     if args.copy_calibration:
         _native.copy_calibration(args.copy_calibration, 1)

@@ -219,8 +219,9 @@ int phip_batch_set_option(void *plan, int option, int64_t value);
  * resident (register and LDS use; each class's grid is capped at its own
  * count; -1 if unknown), [16] table words the zero test of the first job
  * reads (zero-skipping jobs: all of them, or the leading half when every
- * row of even degree lies there; 0 for other jobs).  Returns the number of
- * values written or -1. */
+ * row of even degree lies there; 0 for other jobs), [17] rows the first
+ * job's 32-state block test reads (zero-skipping jobs walked in blocks;
+ * -1 otherwise).  Returns the number of values written or -1. */
 int phip_batch_stats(void *plan, double *out, int len);
 
 /* The kernel class of every residue of a batch, in input order, as
@@ -234,6 +235,14 @@ int phip_batch_stats(void *plan, double *out, int len);
  * profiles). */
 int phip_batch_plan_classes(const int64_t *entries, const int32_t *orders, int count, const int64_t *moduli,
                             const int32_t *moduli_counts, int32_t *out, int len);
+
+/* The tables the block zero-skipping walk reads for one 0/1 matrix of
+ * order 8..46 (row-major int64 entries), as a plan builds them: meta[0]
+ * rows the block test reads, [1] rows of even degree (first in the table's
+ * row order), [2] row-sum group, [3] n.  Writes the table words to out when
+ * cap holds them and returns their count, or -1.  Host code only.  No
+ * counterpart in the reference (table introspection for tests). */
+int64_t phip_block_tables(const int64_t *a, int n, int32_t *out, int64_t cap, int32_t *meta);
 
 int phip_batch_destroy(void *plan);
 
@@ -264,6 +273,9 @@ int phip_batch_destroy(void *plan);
                                       products fit take the compact exact classes (product tree sized for
                                       22-bit group values); 0: the 31-bit exact classes.  Read when a plan is
                                       built */
+#define PHIP_TUNE_SKIP_BLOCKS 13   /* 1 (default): tables on the zero-skipping kernels of order 8 and above
+                                      walk in blocks of 32 Gray states, one mask lookup per row and block;
+                                      0: one zero test per state.  Read when a plan is built */
 int phip_set_tuning(int key, int64_t value);
 int64_t phip_get_tuning(int key);
 

@@ -57,6 +57,9 @@ struct BatchJob {
     int32_t set;         // K of the job's launch class
     int32_t variant;     // kVariantPlain, kVariantPaired, kVariantSkipZeros, kVariantExact or kVariantExactCompact
     int32_t test_words;  // kVariantSkipZeros: leading table words the zero test reads (skip_test_words)
+    int32_t block_rows;  // kVariantSkipZeros: rows the 32-state block test reads (block_test_rows: a multiple
+                         // of 8 or every row word; the rows of even degree lead the table), or -1 for the
+                         // per-state test
 };
 
 // One requested residue: prime slot `k` of job `job`.
@@ -76,6 +79,7 @@ struct BatchClass {
     uint32_t first_item;
     uint32_t n_items;
     int max_n;  // largest order of the class's jobs (sizes the zero-skip queues)
+    int max_block_words;  // largest block-walk LDS region of the class's jobs (block_region_words), or 0
 };
 
 // Rows of one wavefront's zero-skip queue for tables of order n: the byte
@@ -90,10 +94,28 @@ constexpr int skip_queue_rows(int n) { return (n + 3) / 4 + 1; }
 constexpr int kSkipHalfMinOrder = 20;
 constexpr int skip_full_words(int n) { return (n + 3) / 4; }
 constexpr int skip_half_words(int n) { return n < kSkipHalfMinOrder ? (n + 3) / 4 : ((n + 3) / 4 + 1) / 2; }
+// The block form of the zero-skipping walk (glynn_chunk_set_blocks): the
+// kBlockInnerBits fastest Gray columns are enumerated per outer state, and
+// one 32-bit LDS lookup per row marks the inner states that zero the row.
+// Its per-wavefront LDS region (words): the rows' mask tables (32 words a
+// row, plus 32), the inner row-sum table (32 words per row word), 64
+// words of owners and kBlockRing entries of (row words, survivor mask,
+// first state index).
+constexpr int kBlockInnerBits = 5;
+constexpr int kBlockRing = 128;
+constexpr int kBlockMinChunkBits = kBlockInnerBits + 1;
+// Smallest order planned on the block walk (smaller tables take the
+// per-state test).
+constexpr int kBlockMinOrder = 8;
+constexpr int block_region_words(int n, int rows) {
+    return (rows + 1) * 32 + 32 * ((n + 3) / 4) + 64 + ((n + 3) / 4 + 2) * kBlockRing;
+}
 // Dynamic LDS of one workgroup (4 wavefronts) of a class's kernel: the
-// zero-skip queues, nothing for the other variants.
+// zero-skip queues or block-walk regions, nothing for the other variants.
 inline unsigned class_lds_bytes(const BatchClass& cl) {
-    return cl.variant == 2 ? 4u * (unsigned)skip_queue_rows(cl.max_n) * kZeroQueueEntries * 4u : 0u;
+    if (cl.variant != 2) return 0u;
+    const int words = skip_queue_rows(cl.max_n) * kZeroQueueEntries;
+    return 4u * 4u * (unsigned)(words > cl.max_block_words ? words : cl.max_block_words);
 }
 
 // One requested residue before planning: a row-major n x n matrix and an odd

@@ -58,9 +58,16 @@ namespace {
 // lane values (signed Montgomery residues) in r.
 template <int N, int G, int K, bool P, bool Z>
 __device__ __forceinline__ void run_item(const int32_t* __restrict__ tables, const BatchJob& jb, uint64_t base,
-                                         bool active, int32_t* queue, const int32_t (&p)[K],
+                                         bool active, int32_t* queue, bool reload, const int32_t (&p)[K],
                                          const uint32_t (&pinv)[K], int32_t (&r)[K]) {
     if constexpr (Z) {
+        if constexpr (N >= kBlockMinOrder) {
+            if (jb.block_rows >= 0) {
+                glynn_chunk_set_blocks<N, G, K>(tables, N, jb.cbits, base, active, queue, jb.block_rows, reload, p,
+                                                pinv, r);
+                return;
+            }
+        }
         if constexpr (N >= kSkipHalfMinOrder) {
             if (jb.test_words < skip_full_words(N)) {
                 glynn_chunk_set_skip<N, G, K, true>(tables, N, jb.cbits, base, active, queue, p, pinv, r);
@@ -108,6 +115,8 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
     // the class's largest order n, set by the host.
     extern __shared__ int32_t queues[];
     int32_t* const queue = queues + (Z ? (threadIdx.x >> 6) * queue_words : 0);
+    // The job whose block-walk tables this wavefront's LDS region holds.
+    uint32_t loaded_job = 0xffffffffu;
     for (;;) {
         // Every lane executes the same atomic (lane 0 adds 1, the others 0)
         // so no lane-dependent branch sits in front of the broadcast: with a
@@ -166,10 +175,13 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
         int32_t r[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) r[k] = 0;
+        const bool block = Z && jb.block_rows >= 0;
+        const bool reload = block && it.x != loaded_job;
+        loaded_job = block ? it.x : 0xffffffffu;
         switch (jb.n) {
 #define PHIP_BATCH_CASE(N) \
     case N:                \
-        run_item<N, G, K, P, Z>(tables, jb, base, active, queue, p, pinv, r); \
+        run_item<N, G, K, P, Z>(tables, jb, base, active, queue, reload, p, pinv, r); \
         break;
             PHIP_BATCH_CASE(5) PHIP_BATCH_CASE(6) PHIP_BATCH_CASE(7) PHIP_BATCH_CASE(8)
             PHIP_BATCH_CASE(9) PHIP_BATCH_CASE(10) PHIP_BATCH_CASE(11) PHIP_BATCH_CASE(12)

@@ -370,6 +370,44 @@ int skip_plan(const int64_t* a, int n, int group, std::vector<int>& order) {
     return skip_half_words(n);
 }
 
+// Row order and row-sum group of a 0/1 table on the block zero-skipping
+// walk (build_block_tables): the rows of even degree first, since the
+// block test reads the leading rows, then the rows of odd degree, each by
+// ascending degree.  The kernel multiplies the doubled row sums 2 s, so the
+// operand limits apply to the bounds 2 d: `group` becomes 6 when every
+// group of six consecutive rows in this order meets them, else 4, which
+// every order meets for degrees up to 46 (92^4 < 2^31, 92^2 < 2^23).
+// Returns the number of rows of even degree, or -1 when neither fits.
+int block_plan(const int64_t* a, int n, std::vector<int>& order, int& group) {
+    std::vector<uint64_t> degree(n, 0);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) degree[i] += a[(size_t)i * n + j] != 0 ? 1u : 0u;
+    order.resize(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return std::make_pair(degree[x] % 2, degree[x]) < std::make_pair(degree[y] % 2, degree[y]);
+    });
+    int even = 0;
+    while (even < n && degree[order[even]] % 2 == 0) ++even;
+    auto fits = [&](int g) {
+        for (int first = 0; first < n; first += g) {
+            uint64_t b[6];
+            const int c = std::min(g, n - first);
+            for (int k = 0; k < c; ++k) b[k] = 2 * degree[order[first + k]];
+            if (!group_fits(b, c)) return false;
+        }
+        return true;
+    };
+    if (fits(6)) {
+        group = 6;
+    } else if (fits(4)) {
+        group = 4;
+    } else {
+        return -1;
+    }
+    return even;
+}
+
 template <typename T>
 struct DevBuf {
     T* ptr = nullptr;
@@ -407,6 +445,7 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
         bool skip_zeros;
         bool compact;    // groups of 2 rows whose partial bound products fit the compact exact classes
         int test_words;  // skip_zeros: the walk's zero-test width (skip_plan)
+        int block_rows;  // skip_zeros: rows the block walk's test reads, or -1 (per-state test)
         int bits;  // bound_bits of the table's row bounds
         std::vector<size_t> inputs;
     };
@@ -448,7 +487,7 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
         }
         std::vector<int> order;
         const std::vector<uint64_t> bounds = centred_row_bounds(x.a, x.n, (int64_t)x.q);
-        const int group = plan_rows(bounds, order);
+        int group = plan_rows(bounds, order);
         // Tables of 0/1 matrices (nonzero_share is 1 for any other entry) in
         // groups of 4 or 6 with a small share of nonzero terms run on the
         // zero-skipping kernels, whose walk reads byte tables in the same
@@ -456,10 +495,30 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
         const int permille = tuning().skip_zeros_permille;
         const bool skip = permille > 0 && (group == 4 || group == 6) &&
                           nonzero_share(x.a, x.n) * 1000.0 <= (double)permille;
-        const size_t words = (size_t)x.n * (skip ? byte_stride(x.n) : glynn_stride(x.n, group));
-        const int test_words = skip ? skip_plan(x.a, x.n, group, order) : 0;
+        // Those of order kBlockMinOrder and above walk in blocks of 32 states
+        // (build_block_tables, block_plan's row order and group).
+        int test_words = 0, block_rows = -1, even = -1;
+        if (skip && tuning().skip_blocks && x.n >= kBlockMinOrder) {
+            std::vector<int> block_order;
+            int block_group = group;
+            even = block_plan(x.a, x.n, block_order, block_group);
+            if (even >= 0) {
+                order = block_order;
+                group = block_group;
+                block_rows = block_test_rows(x.n, even);
+            }
+        }
+        size_t words;
+        if (block_rows >= 0) {
+            words = block_table_words(x.n, block_rows);
+        } else {
+            words = (size_t)x.n * (skip ? byte_stride(x.n) : glynn_stride(x.n, group));
+            test_words = skip ? skip_plan(x.a, x.n, group, order) : 0;
+        }
         table.assign(words, 0);
-        if (skip) {
+        if (block_rows >= 0) {
+            build_block_tables(x.a, x.n, order, even, block_rows, table.data());
+        } else if (skip) {
             build_byte_tables(x.a, x.n, order, table.data());
         } else {
             build_glynn_tables(x.a, x.n, x.q, group, order, table.data());
@@ -469,6 +528,7 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
             for (size_t si : sets_of_key[x.key]) {
                 const Set& st = sets[si];
                 if (st.n == x.n && st.group == group && st.skip_zeros == skip && st.test_words == test_words &&
+                    st.block_rows == block_rows &&
                     memcmp(out.blob.data() + st.data_off, table.data(), words * sizeof(int32_t)) == 0) {
                     found = si;
                     break;
@@ -485,7 +545,7 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
                     compact = compact_nodes_fit(kernel_bounds, group, 0, groups, 10);
                 }
             }
-            sets.push_back({x.key, x.n, group, (uint64_t)out.blob.size(), skip, compact, test_words,
+            sets.push_back({x.key, x.n, group, (uint64_t)out.blob.size(), skip, compact, test_words, block_rows,
                             group > 0 ? bound_bits(bounds) : 0, {}});
             sets_of_key[x.key].push_back(found);
             out.blob.insert(out.blob.end(), table.begin(), table.end());
@@ -558,18 +618,22 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
                     sl.k = (uint32_t)k;
                     sl.p = q;
                     // The tree leaves R^-(groups-1) and the lane reduction one
-                    // more R^-1.
+                    // more R^-1; the block walk multiplies doubled row sums
+                    // (2^n more).
+                    const uint64_t halvings = (uint64_t)L + (st.block_rows >= 0 ? (uint64_t)st.n : 0u);
                     sl.scale = (uint32_t)host_mulmod(host_powmod((1ull << 32) % q, (uint64_t)groups, q),
-                                                     host_powmod((q + 1) / 2, (uint64_t)L, q), q);
+                                                     host_powmod((q + 1) / 2, halvings, q), q);
                     out.slots[idx] = sl;
                     out.total_states += std::ldexp(1.0, L);
                     out.total_mulmods += (double)st.n * std::ldexp(1.0, L);
                 }
             }
             next += take;
+            jb.block_rows = -1;
             if (st.skip_zeros) {
                 jb.variant = kVariantSkipZeros;
                 jb.test_words = st.test_words;
+                jb.block_rows = st.block_rows;
             } else if (st.group > 0 && pairs_states(K, (const uint32_t*)out.set_p.data() + jb.prime_off)) {
                 jb.variant = kVariantPaired;
             } else {
@@ -612,6 +676,9 @@ int assign_items(PlanHost& out, int items_log2, int min_cbits) {
         // Exact jobs keep a lane's columns below 2^55 and an item's below
         // 2^61 (the finalisation's run length relies on it).
         if (is_exact_variant(jb.variant)) c = std::min(c, 24);
+        // The block walk enumerates the first kBlockInnerBits Gray columns
+        // inside a chunk (n >= kBlockMinOrder leaves L >= 7).
+        if (jb.variant == kVariantSkipZeros && jb.block_rows >= 0) c = std::max(c, kBlockMinChunkBits);
         jb.cbits = c;
         jb.lanes = 1ull << (L - c);
     }
@@ -640,10 +707,12 @@ int assign_items(PlanHost& out, int items_log2, int min_cbits) {
                    (b.n + 3.0 * groups_of(b.n, b.group));
         };
         std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost(a) > cost(b); });
-        BatchClass cl{gk[0], gk[1], gk[2], (uint32_t)out.items.size(), 0, 0};
+        BatchClass cl{gk[0], gk[1], gk[2], (uint32_t)out.items.size(), 0, 0, 0};
         for (uint32_t j : order) {
             BatchJob& b = out.jobs[j];
             cl.max_n = std::max(cl.max_n, b.n);
+            if (b.variant == kVariantSkipZeros && b.block_rows >= 0)
+                cl.max_block_words = std::max(cl.max_block_words, block_region_words(b.n, b.block_rows));
             const uint64_t waves = (b.lanes + 63) / 64;
             if (out.items.size() + waves >= 0xffffffffull) {
                 set_error("batch has too many work items");
@@ -1103,7 +1172,8 @@ int phip_batch_stats(void* handle, double* out, int len) {
                         (double)plan->blocks_per_cu,
                         h.classes.empty() ? -1.0 : (double)h.classes[0].variant,
                         (double)resident,
-                        h.jobs.empty() ? -1.0 : (double)h.jobs[0].test_words};
+                        h.jobs.empty() ? -1.0 : (double)h.jobs[0].test_words,
+                        h.jobs.empty() ? -1.0 : (double)h.jobs[0].block_rows};
     const int m = std::min<int>(len, (int)(sizeof(v) / sizeof(v[0])));
     for (int i = 0; i < m; ++i) out[i] = v[i];
     return m;
@@ -1125,6 +1195,37 @@ int phip_batch_plan_classes(const int64_t* entries, const int32_t* orders, int c
         out[i] = (jb.group * 32 + jb.set) * 8 + jb.variant;
     }
     return (int)h.slots.size();
+}
+
+int64_t phip_block_tables(const int64_t* a, int n, int32_t* out, int64_t cap, int32_t* meta) {
+    if (!a || !meta || (!out && cap > 0)) {
+        phip::set_error("null argument");
+        return -1;
+    }
+    if (n < phip::kBlockMinOrder || n > phip::kBatchMaxOrder) {
+        phip::set_error("block tables take orders 8..46");
+        return -1;
+    }
+    for (size_t e = 0; e < (size_t)n * n; ++e)
+        if (a[e] != 0 && a[e] != 1) {
+            phip::set_error("block tables take 0/1 matrices");
+            return -1;
+        }
+    std::vector<int> order;
+    int group = 0;
+    const int even = phip::block_plan(a, n, order, group);
+    if (even < 0) {
+        phip::set_error("no row-sum group fits the doubled row sums");
+        return -1;
+    }
+    const int rows = phip::block_test_rows(n, even);
+    const size_t words = phip::block_table_words(n, rows);
+    meta[0] = rows;
+    meta[1] = even;
+    meta[2] = group;
+    meta[3] = n;
+    if ((int64_t)words <= cap) phip::build_block_tables(a, n, order, even, rows, out);
+    return (int64_t)words;
 }
 
 int phip_batch_destroy(void* handle) {

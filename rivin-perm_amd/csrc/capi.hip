@@ -153,6 +153,10 @@ int phip_set_tuning(int key, int64_t value) {
             if (value < 0 || value > 1) break;
             t.exact_compact = (int)value;
             return 0;
+        case PHIP_TUNE_SKIP_BLOCKS:
+            if (value < 0 || value > 1) break;
+            t.skip_blocks = (int)value;
+            return 0;
         default:
             phip::set_error("unknown tuning key");
             return -1;
@@ -176,6 +180,7 @@ int64_t phip_get_tuning(int key) {
         case PHIP_TUNE_EXACT_MIN_PRIMES: return t.exact_min_primes;
         case PHIP_TUNE_EXACT_WIDE_MIN_PRIMES: return t.exact_wide_min_primes;
         case PHIP_TUNE_EXACT_COMPACT: return t.exact_compact;
+        case PHIP_TUNE_SKIP_BLOCKS: return t.skip_blocks;
         default:
             phip::set_error("unknown tuning key");
             return -1;

@@ -44,6 +44,7 @@
 
 #include <utility>
 
+#include "batch.h"
 #include "exact_limbs.h"
 #include "phip_arith.h"
 #include "launchers.h"
@@ -906,6 +907,302 @@ __device__ __forceinline__ void glynn_chunk_set_skip(const int32_t* __restrict__
     sink.flush();
 #pragma unroll
     for (int k = 0; k < K; ++k) out[k] = lane_value(sink.acc[k], false, sink.negp[k], sink.pinv[k]);
+}
+
+
+// ---------------------------------------------------------------------------
+// Block zero-skipping walk for 0/1 tables (build_block_tables).
+//
+// The chunk's Gray states are visited as 2^(c-5) outer states (the Gray
+// code over columns 5..n-2, one flip per outer step) times all 32 inner
+// states y of the first kBlockInnerBits Gray columns: state (u, y) has row
+// sums O_i(u) - 2 * (ones of row i among the columns selected by y) and
+// sign (-1)^(parity of u's Gray bits + popcount y).  A row of even degree
+// is zero at the inner states whose count of ones equals O_i / 2, a set
+// that depends only on O_i: the host tabulates it per row as 32-bit masks
+// over y, indexed by the row's byte, so one LDS lookup per row (an SDWA
+// byte add and a ds_read_b32) tests 32 states.  The states left (every
+// tested row nonzero) are queued as entries of a per-wavefront LDS ring:
+// the lane's outer row words with the sign in bit 0, its survivor mask and
+// the index of its first state in the wavefront's running count.  Whenever
+// 64 states are queued, each lane takes one: it finds its entry by the
+// entry's first index (entries starting inside the batch mark their first
+// lane, a wave-wide running maximum spreads them), picks the y of its rank
+// in the survivor mask, rebuilds the row sums from the inner table and adds
+// the term as ZeroSkipSink does.  Row bytes are doubled sums 2 s + beta
+// (beta = 128 even degree, 126 odd), all multiples of 4, so the bytes index
+// the mask tables directly; the terms are products of the doubled sums
+// (2^n times the true term, removed by the finalisation's scale).
+
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
+// Inclusive prefix sum over the wavefront in lane order: DPP row shifts
+// inside rows of 16 lanes, then the gfx9 row broadcasts of lanes 15 and 31.
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);
+    return v;
+}
+
+// Inclusive running maximum over the wavefront (unsigned, lanes without a
+// source contribute 0).
+__device__ __forceinline__ uint32_t wave_inclusive_max(uint32_t v) {
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));
+    return v;
+}
+
+// Position of set bit k (counting from 0 at the low end) of m, by halving
+// popcounts; always in [0, 31], meaningful for k < popcount(m).
+__device__ __forceinline__ uint32_t kth_set_bit(uint32_t m, uint32_t k) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (int level = 0; level < 5; ++level) {
+        const int h = 16 >> level;
+        const uint32_t c = (uint32_t)__popc((m >> pos) & ((1u << h) - 1u));
+        const bool up = k >= c;
+        k = up ? k - c : k;
+        pos = up ? pos + h : pos;
+    }
+    return pos;
+}
+
+template <int NR, int G, int K>
+struct BlockSkipWalk {
+    static_assert(G > 0, "prime sets need exact row sums");
+    using Rows = ByteRows<NR>;
+    using Groups = ByteGroups<NR, G>;
+    static constexpr int W = Rows::kStride;
+    static constexpr int kGroups = Groups::kGroups;
+    static constexpr int kPairs = kGroups / 2;
+    static constexpr int kLeaves = kPairs + (kGroups & 1);
+    static constexpr uint32_t kRing = kBlockRing;
+    int32_t negp[K];
+    uint32_t pinv[K];
+    int64_t acc[K];
+    uint32_t delta[W];    // per-byte corrections 128 - beta (wave-uniform)
+    lds_u32* inner;       // 32 W words: word k of inner state y at k * 32 + y
+    lds_u32* owner;       // 64 words
+    lds_u32* ring;        // (W + 2) * kRing words
+    uint32_t mask_base;   // LDS byte address of row 0's mask table (row i at + 128 i)
+    int test_words;       // rows tested / 4 (wave-uniform)
+    uint32_t ehead, etail, thead, ttail;  // entries and states queued / taken (wave-uniform)
+    uint32_t lane;
+    bool active;
+
+    // Mask entry of row `row` (byte B of its row word w): the byte plus the
+    // table base is the entry's LDS byte address (one SDWA add, which the
+    // compiler forms from the shift and mask) and the row's table offset
+    // goes into the ds_read_b32 offset field.
+    template <int B>
+    __device__ __forceinline__ uint32_t lookup(uint32_t w, int row) const {
+        const uint32_t addr = ((w >> (8 * B)) & 0xffu) + mask_base;
+        return ((const lds_u32*)(uintptr_t)addr)[row * 32];
+    }
+
+    // Bit y set: some tested row is zero at inner state y.  Rows are tested
+    // eight at a time (two row words), whose eight loads are in flight
+    // together; test_words is even or the table's last word.
+    __device__ __forceinline__ uint32_t zero_mask(const Rows& rs) const {
+        uint32_t m = 0;
+#pragma unroll
+        for (int k = 0; k < W; k += 2) {
+            if (k < test_words) {
+                uint32_t a[8];
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                    if (k + b / 4 < W) {
+                        const uint32_t w = rs.r[k + b / 4];
+                        switch (b % 4) {
+                            case 0: a[b] = lookup<0>(w, 4 * k + b); break;
+                            case 1: a[b] = lookup<1>(w, 4 * k + b); break;
+                            case 2: a[b] = lookup<2>(w, 4 * k + b); break;
+                            default: a[b] = lookup<3>(w, 4 * k + b); break;
+                        }
+                    } else {
+                        a[b] = 0u;
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < 8; b += 2) m = __builtin_amdgcn_bitop3_b32(m, a[b], a[b + 1], 0xfe);
+            }
+        }
+        return m;
+    }
+
+    // Takes `cnt` (<= 64) queued states, one per lane from lane 0.
+    __device__ __forceinline__ void consume(uint32_t cnt) {
+        __builtin_amdgcn_wave_barrier();
+        owner[lane] = 0u;
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t live = ehead - etail;
+#pragma unroll
+        for (uint32_t pass = 0; pass < 2; ++pass) {
+            const uint32_t rel_e = pass * 64u + lane;
+            if (rel_e < live) {
+                const uint32_t slot = (etail + rel_e) & (kRing - 1);
+                const uint32_t rel = ring[(W + 1) * kRing + slot] - ttail;
+                if (rel < 64u) owner[rel] = rel_e;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t rel_e = wave_inclusive_max(owner[lane]);
+        const uint32_t slot = (etail + rel_e) & (kRing - 1);
+        const uint32_t k = ttail + lane - ring[(W + 1) * kRing + slot];
+        const uint32_t mask = ring[W * kRing + slot];
+        const uint32_t y = kth_set_bit(mask, k);
+        uint32_t w[W];
+        uint32_t sign = 0;
+#pragma unroll
+        for (int kk = 0; kk < W; ++kk) {
+            uint32_t o = ring[kk * kRing + slot];
+            if (kk == 0) {
+                sign = o & 1u;
+                o ^= sign;
+            }
+            w[kk] = ((o - inner[kk * 32 + y]) + delta[kk]) ^ 0x80808080u;
+        }
+        sign ^= (uint32_t)__popc(y) & 1u;
+        if (lane < cnt) {
+            int32_t g[kGroups];
+            const Groups q{w};
+            q.groups(g, std::make_integer_sequence<int, kGroups>{});
+            if (sign) g[0] = -g[0];
+            int64_t P[kPairs > 0 ? kPairs : 1];
+#pragma unroll
+            for (int j = 0; j < kPairs; ++j) P[j] = (int64_t)g[2 * j] * (int64_t)g[2 * j + 1];
+#pragma unroll
+            for (int p = 0; p < K; ++p) {
+                int32_t v[kLeaves];
+#pragma unroll
+                for (int j = 0; j < kPairs; ++j) v[j] = redc31(P[j], negp[p], pinv[p]);
+                if constexpr (kGroups & 1) v[kLeaves - 1] = g[kGroups - 1];
+#pragma unroll
+                for (int wd = 1; wd < kLeaves; wd *= 2) {
+#pragma unroll
+                    for (int i = 0; i + wd < kLeaves; i += 2 * wd) v[i] = mont31(v[i], v[i + wd], negp[p], pinv[p]);
+                }
+                acc[p] = mac_sign<1>(acc[p], v[0]);
+            }
+        }
+        if (cnt == 64u) {
+            // The next state to take lies in lane 63's entry unless lane 63
+            // took that entry's last state.
+            const uint32_t next = rel_e + ((k + 1u == (uint32_t)__popc(mask)) ? 1u : 0u);
+            etail += (uint32_t)__builtin_amdgcn_readlane((int)next, 63);
+        }
+        ttail += cnt;
+    }
+
+    // One outer state: tests its 32 inner states and queues the survivors.
+    __device__ __forceinline__ void block(const Rows& rs, uint32_t par) {
+        const uint32_t surv = ~zero_mask(rs);
+        const bool has = active && surv != 0u;
+        const uint64_t bal = __ballot(has);
+        if (bal == 0) return;
+        const uint32_t c = has ? (uint32_t)__popc(surv) : 0u;
+        const uint32_t incl = wave_inclusive_sum(c);
+        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+        if (has) {
+            const uint32_t rank =
+                __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+            const uint32_t slot = (ehead + rank) & (kRing - 1);
+#pragma unroll
+            for (int kk = 0; kk < W; ++kk) ring[kk * kRing + slot] = kk == 0 ? (rs.r[0] | par) : rs.r[kk];
+            ring[W * kRing + slot] = surv;
+            ring[(W + 1) * kRing + slot] = thead + incl - c;
+        }
+        ehead += (uint32_t)__popcll(bal);
+        thead += total;
+        while (thead - ttail >= 64u) consume(64u);
+    }
+};
+
+// glynn_chunk_set_skip's block form on block tables (build_block_tables):
+// the lane's 2^cbits states (cbits >= kBlockMinChunkBits), the residues of
+// the K primes in out (only the wavefront's total is meaningful, as there).
+// `region` is the wavefront's LDS region (block_region_words(n, rows)
+// words); with `reload` the mask and inner tables are copied into it first
+// (a wavefront keeps them while it stays on one job).  Every lane of the
+// wavefront must call it.
+template <int NR, int G, int K>
+__device__ __forceinline__ void glynn_chunk_set_blocks(const int32_t* __restrict__ tables, int n, int cbits,
+                                                       uint64_t base, bool active, int32_t* region, int rows,
+                                                       bool reload, const int32_t (&primes)[K],
+                                                       const uint32_t (&pinvs)[K], int32_t (&out)[K]) {
+    using Walk = BlockSkipWalk<NR, G, K>;
+    constexpr int W = Walk::W;
+    Walk walk;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        walk.negp[k] = -primes[k];
+        walk.pinv[k] = pinvs[k];
+        walk.acc[k] = 0;
+    }
+    const uint32_t lane = __lane_id();
+    lds_u32* const lds = (lds_u32*)region;
+    walk.inner = lds + (rows + 1) * 32;
+    walk.owner = walk.inner + 32 * W;
+    walk.ring = walk.owner + 64;
+    walk.mask_base = (uint32_t)(uintptr_t)lds;
+    walk.test_words = rows / 4;
+    walk.ehead = walk.etail = walk.thead = walk.ttail = 0;
+    walk.lane = lane;
+    walk.active = active;
+    const int32_t* __restrict__ D = tables + W;
+    {
+        const ConstCol c = opaque_ptr(tables + (size_t)n * W);
+#pragma unroll
+        for (int k = 0; k < W; ++k) walk.delta[k] = (uint32_t)c[k];
+    }
+    if (reload) {
+        const int32_t* __restrict__ src = tables + (size_t)n * W + W;
+        for (int i = (int)lane; i < 32 * W; i += 64) walk.inner[i] = (uint32_t)src[i];
+        src += 32 * W;
+        for (int i = (int)lane; i < (rows + 1) * 32; i += 64) lds[i] = (uint32_t)src[i];
+    }
+    __builtin_amdgcn_wave_barrier();
+
+    // Outer row sums of the chunk's first state: the Gray bits of base from
+    // column kBlockInnerBits on (bits below cbits-1 are zero).
+    const uint64_t gray = base ^ (base >> 1);
+    ByteRows<NR> rs;
+    rs.load(tables);
+    for (int j = cbits - 1; j < n - 1; ++j) rs.sub_masked(D + j * W, -(int32_t)((gray >> j) & 1u), 0);
+    uint32_t par = (uint32_t)__popcll(gray) & 1u;
+    const uint64_t ub = base >> kBlockInnerBits;
+    const int ubits = cbits - kBlockInnerBits;
+    const uint32_t U = 1u << ubits;
+    walk.block(rs, par);
+    for (uint32_t u = 1; u < U; ++u) {
+        const int jj = __builtin_ctz(u);  // uniform
+        const int32_t* col = D + (kBlockInnerBits + jj) * W;
+        // The new Gray bit is 1 ^ bit jj+1 of (ub + u): bit jj+1 of u except
+        // at the chunk's middle outer state, where it is the lane's own.
+        if (jj + 1 < ubits) {
+            if ((u >> (jj + 1)) & 1u) {
+                rs.template flip_mem<false>(col, 0);
+            } else {
+                rs.template flip_mem<true>(col, 0);
+            }
+        } else {
+            rs.flip_lane(col, !(((ub + u) >> (jj + 1)) & 1u), 0);
+        }
+        par ^= 1u;
+        walk.block(rs, par);
+    }
+    const uint32_t pend = walk.thead - walk.ttail;
+    if (pend) walk.consume(pend);
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = lane_value(walk.acc[k], false, walk.negp[k], walk.pinv[k]);
 }
 
 }  // namespace phip

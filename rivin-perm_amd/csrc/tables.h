@@ -6,6 +6,7 @@
 #include <cmath>
 #include <vector>
 
+#include "batch.h"
 #include "launchers.h"
 #include "phip_arith.h"
 
@@ -190,6 +191,82 @@ inline void build_byte_tables(const int64_t* M, int n, const std::vector<int>& o
         put(0, i, 64u + degree);
         for (int j = 0; j < n - 1; ++j)
             if (M[(size_t)src * n + j] != 0) put(1 + j, i, 2u);
+    }
+    for (size_t k = 0; k < words.size(); ++k) out[k] = (int32_t)words[k];
+}
+
+}  // namespace phip
+
+namespace phip {
+
+// Rows the block walk tests for a table of order n whose `even` leading
+// rows have even degree: whole pairs of row words (the kernel tests eight
+// rows at a time), at most every row word.
+inline int block_test_rows(int n, int even) {
+    const int words = byte_stride(n);
+    return 4 * std::min(words, 2 * ((even + 7) / 8));
+}
+
+// Words of a block-walk table (build_block_tables) of order n whose `rows`
+// leading rows are tested (block_test_rows).
+inline size_t block_table_words(int n, int rows) {
+    const size_t W = (size_t)byte_stride(n);
+    return (size_t)n * W + W + 32 * W + ((size_t)rows + 1) * 32;
+}
+
+// Tables of the block zero-skipping walk (glynn31.h,
+// glynn_chunk_set_blocks) for a matrix with entries in {0, 1} whose rows,
+// in kernel order order[i], put the `even` rows of even degree first.  Row
+// sums are kept doubled, as the byte 2 s + beta_i with beta_i = 128 for a
+// row of even degree and 126 for one of odd degree, so every byte is a
+// multiple of 4 (an aligned LDS word address) and lies in [0, 254] for
+// |s| <= 63; padding rows hold 132.  Layout, in int32 words (W =
+// byte_stride(n)):
+//   [0, n W)          the bytes of state 0 (every column sign +1), then the
+//                     flip deltas 4 a_ij of the n-1 Gray columns;
+//   [n W, n W + W)    per-byte corrections 128 - beta_i: adding them and
+//                     flipping bit 7 gives the signed doubled sums 2 s;
+//   then 32 W         the inner table: word k of the deltas summed over the
+//                     set bits y_j of y (the first kBlockInnerBits Gray
+//                     columns), at k * 32 + y;
+//   then (rows+1) 32  the masks: entry e of row i (i < rows), at word
+//                     32 i + e, has bit y set when the row's sum at inner
+//                     state y is zero for an outer byte 4 e, i.e. when e =
+//                     32 + (the row's ones among the inner columns selected
+//                     by y), so only e in [32, 38) is ever nonzero.  Rows
+//                     of odd degree never sum to zero: their entries are 0.
+//                     Rows overlap (a stride of 32 words, not 64): a row
+//                     byte of order n <= 46 lies in [36, 220], so e lies in
+//                     [9, 55] and a read of row i meets only its own
+//                     nonzero words or words no row sets (row i-1's nonzero
+//                     words are its e in [0, 6), row i+1's its e in
+//                     [64, 70)).
+inline void build_block_tables(const int64_t* M, int n, const std::vector<int>& order, int even, int rows,
+                               int32_t* out) {
+    const int W = byte_stride(n);
+    std::vector<uint32_t> words(block_table_words(n, rows), 0u);
+    auto put = [&](size_t word_base, int row, uint32_t byte) { words[word_base + row / 4] |= byte << (8 * (row % 4)); };
+    auto entry = [&](int i, int j) { return M[(size_t)order[i] * n + j] != 0 ? 1u : 0u; };
+    const size_t delta_at = (size_t)n * W, inner_at = delta_at + W, masks_at = inner_at + 32 * (size_t)W;
+    for (int i = 0; i < 4 * W; ++i) {
+        if (i >= n) {
+            put(0, i, 132u);
+            continue;
+        }
+        uint32_t degree = 0;
+        for (int j = 0; j < n; ++j) degree += entry(i, j);
+        const uint32_t beta = degree % 2 == 0 ? 128u : 126u;
+        put(0, i, 2u * degree + beta);
+        for (int j = 0; j < n - 1; ++j)
+            if (entry(i, j)) put((size_t)(1 + j) * W, i, 4u);
+        put(delta_at, i, 128u - beta);
+        for (int y = 0; y < 32; ++y) {
+            uint32_t ones = 0;
+            for (int j = 0; j < kBlockInnerBits; ++j)
+                if ((y >> j) & 1) ones += entry(i, j);
+            words[inner_at + (size_t)(i / 4) * 32 + y] |= (4u * ones) << (8 * (i % 4));
+            if (i < even && i < rows) words[masks_at + (size_t)i * 32 + 32 + ones] |= 1u << y;
+        }
     }
     for (size_t k = 0; k < words.size(); ++k) out[k] = (int32_t)words[k];
 }

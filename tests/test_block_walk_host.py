@@ -1,0 +1,209 @@
+"""Host side of the block zero-skipping walk: the library's block tables
+(phip_block_tables, the tables a plan uploads) driven through a lane-level
+model of the device walk (64 lanes, the LDS ring of survivor entries, the
+owner marks and running maximum that hand each lane its state, the k-th
+set bit of a survivor mask) give the permanent of random 0/1 matrices, as
+the oracle's subset DP and permutation sum compute it.  CPU only: the GPU
+tests run the kernel itself against the oracle."""
+
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+
+RING = 128
+MASK32 = 0xFFFFFFFF
+
+
+def _byte(word, b):
+    return (int(word) >> (8 * b)) & 0xFF
+
+
+def _signed_byte(v):
+    return v - 256 if v >= 128 else v
+
+
+class _Tables:
+    def __init__(self, A):
+        from permhip import _native
+
+        words, self.rows, self.even, self.group = _native.block_tables(A)
+        self.n = n = A.shape[0]
+        self.W = W = (n + 3) // 4
+        w = [int(x) for x in words]
+        self.state0 = w[:W]
+        self.deltas = [w[W * (1 + j):W * (2 + j)] for j in range(n - 1)]
+        self.correction = w[n * W:n * W + W]
+        inner = w[n * W + W:n * W + W + 32 * W]
+        self.inner = [[inner[k * 32 + y] for y in range(32)] for k in range(W)]
+        self.masks = w[n * W + W + 32 * W:]
+        assert len(self.masks) == (self.rows + 1) * 32
+
+
+def _walk_item(T, cbits, first_lane, lanes_total):
+    """One wavefront's item: the sum of its 64 lanes' accumulated terms
+    (doubled row sums, so 2^n times the true terms)."""
+    n, W = T.n, T.W
+    lanes = range(64)
+    active = [first_lane + l < lanes_total for l in lanes]
+    base = [((first_lane + l) if active[l] else 0) << cbits for l in lanes]
+    outer, par = [], []
+    for l in lanes:
+        gray = base[l] ^ (base[l] >> 1)
+        o = list(T.state0)
+        for j in range(cbits - 1, n - 1):
+            if (gray >> j) & 1:
+                o = [(o[k] - T.deltas[j][k]) & MASK32 for k in range(W)]
+        outer.append(o)
+        par.append(bin(gray).count("1") & 1)
+    ring = [[0] * RING for _ in range(W + 2)]
+    state = {"ehead": 0, "etail": 0, "thead": 0, "ttail": 0}
+    acc = [0] * 64
+
+    def consume(cnt):
+        etail, ttail, ehead = state["etail"], state["ttail"], state["ehead"]
+        owner = [0] * 64
+        for rel_e in range(min(ehead - etail, 2 * 64)):
+            rel = (ring[W + 1][(etail + rel_e) % RING] - ttail) & MASK32
+            if rel < 64:
+                owner[rel] = rel_e
+        running = list(itertools.accumulate(owner, max))
+        last = None
+        for l in lanes:
+            slot = (etail + running[l]) % RING
+            k = (ttail + l - ring[W + 1][slot]) & MASK32
+            mask = ring[W][slot]
+            m = mask
+            for _ in range(min(k, 32)):
+                m &= m - 1
+            y = (m & -m).bit_length() - 1 if m else 31
+            words = [ring[kk][slot] for kk in range(W)]
+            sign = words[0] & 1
+            words[0] ^= sign
+            sign ^= bin(y).count("1") & 1
+            if l == 63:
+                last = running[l] + (1 if k + 1 == bin(mask).count("1") else 0)
+            if l < cnt:
+                signed = [(((words[kk] - T.inner[kk][y]) + T.correction[kk]) & MASK32) ^ 0x80808080
+                          for kk in range(W)]
+                term = math.prod(_signed_byte(_byte(signed[i // 4], i % 4)) for i in range(n))
+                acc[l] += -term if sign else term
+        if cnt == 64:
+            state["etail"] = etail + last
+        state["ttail"] = (ttail + cnt) & MASK32
+
+    def block():
+        surv = []
+        for l in lanes:
+            m = 0
+            for i in range(T.rows):
+                b = _byte(outer[l][i // 4], i % 4)
+                assert b % 4 == 0, "row bytes must be aligned table offsets"
+                m |= T.masks[32 * i + b // 4]
+            surv.append(~m & MASK32)
+        has = [active[l] and surv[l] != 0 for l in lanes]
+        if not any(has):
+            return
+        counts = [bin(surv[l]).count("1") if has[l] else 0 for l in lanes]
+        incl = list(itertools.accumulate(counts))
+        rank = 0
+        for l in lanes:
+            if has[l]:
+                slot = (state["ehead"] + rank) % RING
+                rank += 1
+                for kk in range(W):
+                    ring[kk][slot] = outer[l][kk] | (par[l] if kk == 0 else 0)
+                ring[W][slot] = surv[l]
+                ring[W + 1][slot] = (state["thead"] + incl[l] - counts[l]) & MASK32
+        state["ehead"] += rank
+        assert state["ehead"] - state["etail"] < RING
+        state["thead"] = (state["thead"] + incl[63]) & MASK32
+        while (state["thead"] - state["ttail"]) & MASK32 >= 64:
+            consume(64)
+
+    ub = [b >> 5 for b in base]
+    block()
+    for u in range(1, 1 << (cbits - 5)):
+        jj = (u & -u).bit_length() - 1
+        col = T.deltas[5 + jj]
+        for l in lanes:
+            subtract = not (((ub[l] + u) >> (jj + 1)) & 1)
+            outer[l] = [((outer[l][k] - col[k]) if subtract else (outer[l][k] + col[k])) & MASK32 for k in range(W)]
+            par[l] ^= 1
+        block()
+    pending = (state["thead"] - state["ttail"]) & MASK32
+    if pending:
+        consume(pending)
+    return sum(acc)
+
+
+def _model_permanent(A, cbits):
+    T = _Tables(A)
+    lanes_total = 1 << (T.n - 1 - cbits)
+    total = sum(_walk_item(T, cbits, first, lanes_total) for first in range(0, lanes_total, 64))
+    scale = 1 << (2 * T.n - 1)  # doubled row sums (2^n) and Glynn's 2^(n-1)
+    assert total % scale == 0
+    return total // scale
+
+
+def _random_binary(rng, n, density):
+    return (rng.random((n, n)) < density).astype(np.int64)
+
+
+@pytest.mark.parametrize("n,cbits", [(8, 6), (9, 7), (10, 6), (11, 8), (12, 6)])
+def test_block_walk_model_matches_oracle(n, cbits):
+    rng = np.random.default_rng(900 + n)
+    for density in (0.35, 0.6):
+        A = _random_binary(rng, n, density)
+        want = oracle.subset_permanent(A.tolist())
+        assert _model_permanent(A, cbits) == want, (n, cbits, density)
+
+
+def test_block_walk_model_many_entries():
+    """Order 14 with chunks of 2^9 states: 16 outer states per lane, so the
+    ring wraps several times and batches span entries of many lanes."""
+    rng = np.random.default_rng(77)
+    A = _random_binary(rng, 14, 0.65)
+    assert _model_permanent(A, 9) == oracle.subset_permanent(A.tolist())
+
+
+def test_block_tables_layout():
+    """Rows of even degree lead, the tested rows cover them in whole pairs
+    of row words, every row byte is a multiple of 4, and each tested row's
+    masks partition the 32 inner states over its possible zero sums."""
+    from permhip import _native
+
+    rng = np.random.default_rng(5)
+    for n in (8, 13, 21, 30, 37, 46):
+        A = _random_binary(rng, n, 0.5)
+        T = _Tables(A)
+        degrees = sorted(int(d) for d in A.sum(axis=1))
+        assert T.even == sum(1 for d in degrees if d % 2 == 0)
+        words = (n + 3) // 4
+        assert T.rows == 4 * min(words, 2 * ((T.even + 7) // 8))
+        assert T.group in (4, 6)
+        for i in range(4 * words):
+            assert _byte(T.state0[i // 4], i % 4) % 4 == 0
+        for i in range(T.rows):
+            union = 0
+            for c in range(6):
+                assert union & T.masks[32 * (i + 1) + c] == 0
+                union |= T.masks[32 * (i + 1) + c]
+            assert union == (MASK32 if i < T.even else 0), (n, i)
+        # Only the words a row's zero sums select are set.
+        used = {32 * (i + 1) + c for i in range(T.even) for c in range(6)}
+        assert all(T.masks[k] == 0 for k in range(len(T.masks)) if k not in used)
+    with pytest.raises(RuntimeError):
+        _native.block_tables(np.full((9, 9), 2, dtype=np.int64))
+    with pytest.raises(RuntimeError):
+        _native.block_tables(np.ones((7, 7), dtype=np.int64))
+
+
+def test_block_tables_definition_small():
+    """A matrix small enough for the permutation sum: the model agrees."""
+    rng = np.random.default_rng(3)
+    A = _random_binary(rng, 8, 0.7)
+    assert _model_permanent(A, 7) == oracle.definition_permanent(A.tolist())

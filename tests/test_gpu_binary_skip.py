@@ -1,7 +1,9 @@
 """Zero-skipping Glynn kernels (0/1 tables whose Gray states are mostly
 zero terms): the planner picks them, and their residues equal the plain
 kernels' and the oracle's, bit for bit, including chunks with inactive
-lanes, padded single primes, zero rows and mixed plans."""
+lanes, padded single primes, zero rows and mixed plans.  Both walks are
+covered: the block walk (32 states per row lookup, the default from order
+8 on) and the per-state zero test."""
 
 import math
 import random
@@ -26,6 +28,15 @@ def _with_skip(native, permille, fn):
         return fn()
     finally:
         native.set_tuning(native.TUNE_SKIP_ZEROS_PERMILLE, old)
+
+
+def _with_blocks(native, on, fn):
+    old = native.get_tuning(native.TUNE_SKIP_BLOCKS)
+    native.set_tuning(native.TUNE_SKIP_BLOCKS, 1 if on else 0)
+    try:
+        return fn()
+    finally:
+        native.set_tuning(native.TUNE_SKIP_BLOCKS, old)
 
 
 def _even_degree_matrix(rng, n, density):
@@ -55,8 +66,9 @@ def test_skip_matches_plain_and_oracle_small(native, oracle_c, n):
         for primes in (BINARY_PRIMES[:2], BINARY_PRIMES[:4], BINARY_PRIMES[:8], BINARY_PRIMES[:1]):
             print(f"skip n={n} density={density} k={len(primes)}", flush=True)
             skip = native.permanent_residues(A, primes)
+            per_state = _with_blocks(native, False, lambda: native.permanent_residues(A, primes))
             plain = _with_skip(native, 0, lambda: native.permanent_residues(A, primes))
-            assert skip == plain, (n, density, primes)
+            assert skip == plain == per_state, (n, density, primes)
             for p, r in zip(primes, skip):
                 assert r == oracle_c.binary_mod_p(A.astype(np.uint8), p, 16), (n, p)
 
@@ -68,8 +80,9 @@ def test_skip_matches_plain_large(native, n):
     assert _variant(native, A, BINARY_PRIMES[:4]) == 2.0
     print(f"skip large n={n}", flush=True)
     skip = native.permanent_residues(A, BINARY_PRIMES[:4])
+    per_state = _with_blocks(native, False, lambda: native.permanent_residues(A, BINARY_PRIMES[:4]))
     plain = _with_skip(native, 0, lambda: native.permanent_residues(A, BINARY_PRIMES[:4]))
-    assert skip == plain
+    assert skip == plain == per_state
 
 
 def test_planner_picks_the_variant(native):
@@ -111,16 +124,22 @@ def test_skip_largest_orders_closed_form(native, sizes, half):
     A = _block_ones(sizes, rng)
     n = A.shape[0]
     primes = BINARY_PRIMES[:4]
-    plan = native.Batch([A], [primes])
-    try:
-        st = plan.stats()
-    finally:
-        plan.close()
-    words = (n + 3) // 4
-    assert st["variant"] == 2.0
-    assert st["skip_test_words"] == ((words + 1) // 2 if half else words)
     exact = math.prod(math.factorial(k) for k in sizes)
-    print(f"skip largest n={n} half={half}", flush=True)
+
+    def per_state():
+        plan = native.Batch([A], [primes])
+        try:
+            st = plan.stats()
+        finally:
+            plan.close()
+        words = (n + 3) // 4
+        assert st["variant"] == 2.0 and st["skip_block_rows"] == -1.0
+        assert st["skip_test_words"] == ((words + 1) // 2 if half else words)
+        print(f"skip largest n={n} half={half} per state", flush=True)
+        return native.permanent_residues(A, primes)
+
+    assert _with_blocks(native, False, per_state) == [exact % p for p in primes]
+    print(f"skip largest n={n} half={half} blocks", flush=True)
     assert native.permanent_residues(A, primes) == [exact % p for p in primes]
 
 
@@ -197,6 +216,10 @@ def _odd_heavy_matrix(rng, n, density, even_rows):
 
 @pytest.mark.parametrize("n", [20, 23, 26, 29, 33])
 def test_half_width_zero_test(native, oracle_c, n):
+    _with_blocks(native, False, lambda: _half_width_case(native, oracle_c, n))
+
+
+def _half_width_case(native, oracle_c, n):
     """Tables whose rows of even degree fit in the leading half of the byte
     words run the half-width test; residues equal the plain kernels' and,
     up to order 26, the oracle's."""
@@ -228,11 +251,15 @@ def test_full_width_when_even_rows_spill(native):
     rng = np.random.default_rng(17)
     n = 30
     A = _even_degree_matrix(rng, n, 0.5)
-    plan = native.Batch([A], [BINARY_PRIMES[:4]])
-    try:
-        st = plan.stats()
-    finally:
-        plan.close()
+
+    def stats():
+        plan = native.Batch([A], [BINARY_PRIMES[:4]])
+        try:
+            return plan.stats()
+        finally:
+            plan.close()
+
+    st = _with_blocks(native, False, stats)
     assert st["variant"] == 2.0 and st["skip_test_words"] == (n + 3) // 4
 
 
@@ -251,5 +278,88 @@ def test_one_walk_for_up_to_eight_primes(native, k):
     assert st["variant"] == 2.0 and st["jobs"] == 1.0 and st["set"] == (4.0 if k <= 4 else 8.0)
     print(f"one walk k={k}", flush=True)
     skip = native.permanent_residues(A, primes)
+    per_state = _with_blocks(native, False, lambda: native.permanent_residues(A, primes))
     plain = _with_skip(native, 0, lambda: native.permanent_residues(A, primes))
-    assert skip == plain
+    assert skip == plain == per_state
+
+
+def _block_rows(n, even):
+    words = (n + 3) // 4
+    return 4 * min(words, 2 * ((even + 7) // 8))
+
+
+@pytest.mark.parametrize("n,even_rows", [(8, 8), (13, 5), (24, 24), (31, 9), (38, 20), (38, 38)])
+def test_block_walk_stats_and_parity(native, oracle_c, n, even_rows):
+    """The block walk tests whole pairs of row words covering the rows of
+    even degree; its residues equal the per-state walk's, and up to order
+    24 the oracle's."""
+    rng = np.random.default_rng(500 + n + even_rows)
+    A = _odd_heavy_matrix(rng, n, 0.5, even_rows)
+    even = int(sum(1 for d in A.sum(axis=1) if d % 2 == 0))
+    primes = BINARY_PRIMES[:4]
+    plan = native.Batch([A], [primes])
+    try:
+        st = plan.stats()
+    finally:
+        plan.close()
+    print(f"block n={n} even={even} stats={st['variant']},{st['skip_block_rows']},{st['chunk_bits_min']}", flush=True)
+    if st["variant"] != 2.0:
+        assert even < 4
+        return
+    assert st["skip_block_rows"] == _block_rows(n, even)
+    assert st["chunk_bits_min"] >= 6
+    blocks = native.permanent_residues(A, primes)
+    per_state = _with_blocks(native, False, lambda: native.permanent_residues(A, primes))
+    assert blocks == per_state, (n, even)
+    if n <= 24:
+        for p, r in zip(primes, blocks):
+            assert r == oracle_c.binary_mod_p(A.astype(np.uint8), p, 16), (n, p)
+
+
+def test_block_walk_many_jobs_one_class(native, oracle_c):
+    """One plan of 0/1 matrices of orders 8..22 on one block class: the
+    wavefronts move between jobs of different table sizes (the LDS region
+    is sized for the largest; tables reload on a job change), and single
+    primes, sets of 2 and 4 mix."""
+    rng = np.random.default_rng(61)
+    mats, mods = [], []
+    for i, n in enumerate(list(range(8, 23)) * 2):
+        mats.append(_even_degree_matrix(rng, n, 0.4 + 0.02 * (i % 10)))
+        mods.append(BINARY_PRIMES[: (1, 2, 4)[i % 3]])
+    plan = native.Batch(mats, mods)
+    try:
+        flat = plan.run()
+        flat_again = plan.run()
+    finally:
+        plan.close()
+    assert flat == flat_again
+    at = 0
+    for A, ps in zip(mats, mods):
+        for p in ps:
+            assert flat[at] == oracle_c.binary_mod_p(A.astype(np.uint8), p, 16), (A.shape[0], p)
+            at += 1
+
+
+@pytest.mark.parametrize("n", [40, 42])
+def test_block_walk_large_closed_forms(native, n):
+    """Orders 40 and 42 on the block walk: block-diagonal all-ones blocks of
+    sizes 2..5 (rows of both parities), residues the product of the blocks'
+    factorials."""
+    rng = np.random.default_rng(n)
+    sizes = []
+    while sum(sizes) < n:
+        sizes.append(min(int(rng.integers(2, 6)), n - sum(sizes)))
+    if sizes[-1] == 1:
+        sizes[-2] -= 1
+        sizes[-1] = 2
+    A = _block_ones(sizes, rng)
+    primes = BINARY_PRIMES[:2]
+    plan = native.Batch([A], [primes])
+    try:
+        st = plan.stats()
+    finally:
+        plan.close()
+    assert st["variant"] == 2.0 and st["skip_block_rows"] > 0
+    exact = math.prod(math.factorial(k) for k in sizes)
+    print(f"block closed form n={n} sizes={sizes}", flush=True)
+    assert native.permanent_residues(A, primes) == [exact % p for p in primes]
