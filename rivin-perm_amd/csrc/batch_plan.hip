@@ -373,11 +373,11 @@ int skip_plan(const int64_t* a, int n, int group, std::vector<int>& order) {
 // Row order and row-sum group of a 0/1 table on the block zero-skipping
 // walk (build_block_tables): the rows of even degree first, since the
 // block test reads the leading rows, then the rows of odd degree, each by
-// ascending degree.  The kernel multiplies the doubled row sums 2 s, so the
-// operand limits apply to the bounds 2 d: `group` becomes 6 when every
-// group of six consecutive rows in this order meets them, else 4, which
-// every order meets for degrees up to 46 (92^4 < 2^31, 92^2 < 2^23).
-// Returns the number of rows of even degree, or -1 when neither fits.
+// ascending degree.  The row bounds are the degrees d: `group` becomes 6
+// when every group of six consecutive rows in this order meets the
+// operand limits, else 4, which every order meets for degrees up to 46
+// (46^4 < 2^31, 46^2 < 2^23).  Returns the number of rows of even degree,
+// or -1 when neither fits.
 int block_plan(const int64_t* a, int n, std::vector<int>& order, int& group) {
     std::vector<uint64_t> degree(n, 0);
     for (int i = 0; i < n; ++i)
@@ -393,7 +393,7 @@ int block_plan(const int64_t* a, int n, std::vector<int>& order, int& group) {
         for (int first = 0; first < n; first += g) {
             uint64_t b[6];
             const int c = std::min(g, n - first);
-            for (int k = 0; k < c; ++k) b[k] = 2 * degree[order[first + k]];
+            for (int k = 0; k < c; ++k) b[k] = degree[order[first + k]];
             if (!group_fits(b, c)) return false;
         }
         return true;
@@ -618,11 +618,9 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
                     sl.k = (uint32_t)k;
                     sl.p = q;
                     // The tree leaves R^-(groups-1) and the lane reduction one
-                    // more R^-1; the block walk multiplies doubled row sums
-                    // (2^n more).
-                    const uint64_t halvings = (uint64_t)L + (st.block_rows >= 0 ? (uint64_t)st.n : 0u);
+                    // more R^-1.
                     sl.scale = (uint32_t)host_mulmod(host_powmod((1ull << 32) % q, (uint64_t)groups, q),
-                                                     host_powmod((q + 1) / 2, halvings, q), q);
+                                                     host_powmod((q + 1) / 2, (uint64_t)L, q), q);
                     out.slots[idx] = sl;
                     out.total_states += std::ldexp(1.0, L);
                     out.total_mulmods += (double)st.n * std::ldexp(1.0, L);

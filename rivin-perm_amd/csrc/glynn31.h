@@ -931,8 +931,7 @@ __device__ __forceinline__ void glynn_chunk_set_skip(const int32_t* __restrict__
 // in the survivor mask, rebuilds the row sums from the inner table and adds
 // the term as ZeroSkipSink does.  Row bytes are doubled sums 2 s + beta
 // (beta = 128 even degree, 126 odd), all multiples of 4, so the bytes index
-// the mask tables directly; the terms are products of the doubled sums
-// (2^n times the true term, removed by the finalisation's scale).
+// the mask tables directly; the taker halves them back to s.
 
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 
@@ -988,7 +987,6 @@ struct BlockSkipWalk {
     int32_t negp[K];
     uint32_t pinv[K];
     int64_t acc[K];
-    uint32_t delta[W];    // per-byte corrections 128 - beta (wave-uniform)
     lds_u32* inner;       // 32 W words: word k of inner state y at k * 32 + y
     lds_u32* owner;       // 64 words
     lds_u32* ring;        // (W + 2) * kRing words
@@ -1043,10 +1041,12 @@ struct BlockSkipWalk {
         __builtin_amdgcn_wave_barrier();
         owner[lane] = 0u;
         __builtin_amdgcn_wave_barrier();
+        // Every entry holds at least one state, so the entries that start
+        // inside the batch are among the 64 after the oldest live one
+        // (etail, which holds state ttail and is every lane's default).
         const uint32_t live = ehead - etail;
-#pragma unroll
-        for (uint32_t pass = 0; pass < 2; ++pass) {
-            const uint32_t rel_e = pass * 64u + lane;
+        {
+            const uint32_t rel_e = lane + 1u;
             if (rel_e < live) {
                 const uint32_t slot = (etail + rel_e) & (kRing - 1);
                 const uint32_t rel = ring[(W + 1) * kRing + slot] - ttail;
@@ -1061,6 +1061,10 @@ struct BlockSkipWalk {
         const uint32_t y = kth_set_bit(mask, k);
         uint32_t w[W];
         uint32_t sign = 0;
+        // The outer word less the inner one has the bytes 2 s + 128 (the
+        // inner table carries the odd rows' bias correction); halved, s + 64
+        // (bit 0 of every byte is 0, so no bit crosses a byte), then signed
+        // as in ZeroSkipSink.
 #pragma unroll
         for (int kk = 0; kk < W; ++kk) {
             uint32_t o = ring[kk * kRing + slot];
@@ -1068,7 +1072,7 @@ struct BlockSkipWalk {
                 sign = o & 1u;
                 o ^= sign;
             }
-            w[kk] = ((o - inner[kk * 32 + y]) + delta[kk]) ^ 0x80808080u;
+            w[kk] = byte_rows_signed((o - inner[kk * 32 + y]) >> 1);
         }
         sign ^= (uint32_t)__popc(y) & 1u;
         if (lane < cnt) {
@@ -1158,13 +1162,8 @@ __device__ __forceinline__ void glynn_chunk_set_blocks(const int32_t* __restrict
     walk.lane = lane;
     walk.active = active;
     const int32_t* __restrict__ D = tables + W;
-    {
-        const ConstCol c = opaque_ptr(tables + (size_t)n * W);
-#pragma unroll
-        for (int k = 0; k < W; ++k) walk.delta[k] = (uint32_t)c[k];
-    }
     if (reload) {
-        const int32_t* __restrict__ src = tables + (size_t)n * W + W;
+        const int32_t* __restrict__ src = tables + (size_t)n * W;
         for (int i = (int)lane; i < 32 * W; i += 64) walk.inner[i] = (uint32_t)src[i];
         src += 32 * W;
         for (int i = (int)lane; i < (rows + 1) * 32; i += 64) lds[i] = (uint32_t)src[i];

@@ -211,7 +211,7 @@ inline int block_test_rows(int n, int even) {
 // leading rows are tested (block_test_rows).
 inline size_t block_table_words(int n, int rows) {
     const size_t W = (size_t)byte_stride(n);
-    return (size_t)n * W + W + 32 * W + ((size_t)rows + 1) * 32;
+    return (size_t)n * W + 32 * W + ((size_t)rows + 1) * 32;
 }
 
 // Tables of the block zero-skipping walk (glynn31.h,
@@ -224,11 +224,14 @@ inline size_t block_table_words(int n, int rows) {
 // byte_stride(n)):
 //   [0, n W)          the bytes of state 0 (every column sign +1), then the
 //                     flip deltas 4 a_ij of the n-1 Gray columns;
-//   [n W, n W + W)    per-byte corrections 128 - beta_i: adding them and
-//                     flipping bit 7 gives the signed doubled sums 2 s;
 //   then 32 W         the inner table: word k of the deltas summed over the
 //                     set bits y_j of y (the first kBlockInnerBits Gray
-//                     columns), at k * 32 + y;
+//                     columns) less the per-byte corrections 128 - beta_i,
+//                     at k * 32 + y, as a 32-bit difference: an outer word
+//                     minus it has the bytes 2 s + 128 of the row sums at
+//                     (outer state, y), each in [0, 254], so the word
+//                     difference is exact whatever the borrows inside the
+//                     table word;
 //   then (rows+1) 32  the masks: entry e of row i (i < rows), at word
 //                     32 i + e, has bit y set when the row's sum at inner
 //                     state y is zero for an outer byte 4 e, i.e. when e =
@@ -247,7 +250,8 @@ inline void build_block_tables(const int64_t* M, int n, const std::vector<int>& 
     std::vector<uint32_t> words(block_table_words(n, rows), 0u);
     auto put = [&](size_t word_base, int row, uint32_t byte) { words[word_base + row / 4] |= byte << (8 * (row % 4)); };
     auto entry = [&](int i, int j) { return M[(size_t)order[i] * n + j] != 0 ? 1u : 0u; };
-    const size_t delta_at = (size_t)n * W, inner_at = delta_at + W, masks_at = inner_at + 32 * (size_t)W;
+    const size_t inner_at = (size_t)n * W, masks_at = inner_at + 32 * (size_t)W;
+    std::vector<uint32_t> correction(W, 0u);
     for (int i = 0; i < 4 * W; ++i) {
         if (i >= n) {
             put(0, i, 132u);
@@ -259,7 +263,7 @@ inline void build_block_tables(const int64_t* M, int n, const std::vector<int>& 
         put(0, i, 2u * degree + beta);
         for (int j = 0; j < n - 1; ++j)
             if (entry(i, j)) put((size_t)(1 + j) * W, i, 4u);
-        put(delta_at, i, 128u - beta);
+        correction[i / 4] |= (128u - beta) << (8 * (i % 4));
         for (int y = 0; y < 32; ++y) {
             uint32_t ones = 0;
             for (int j = 0; j < kBlockInnerBits; ++j)
@@ -268,6 +272,8 @@ inline void build_block_tables(const int64_t* M, int n, const std::vector<int>& 
             if (i < even && i < rows) words[masks_at + (size_t)i * 32 + 32 + ones] |= 1u << y;
         }
     }
+    for (int k = 0; k < W; ++k)
+        for (int y = 0; y < 32; ++y) words[inner_at + (size_t)k * 32 + y] -= correction[k];
     for (size_t k = 0; k < words.size(); ++k) out[k] = (int32_t)words[k];
 }
 

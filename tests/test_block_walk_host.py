@@ -36,16 +36,14 @@ class _Tables:
         w = [int(x) for x in words]
         self.state0 = w[:W]
         self.deltas = [w[W * (1 + j):W * (2 + j)] for j in range(n - 1)]
-        self.correction = w[n * W:n * W + W]
-        inner = w[n * W + W:n * W + W + 32 * W]
+        inner = w[n * W:n * W + 32 * W]
         self.inner = [[inner[k * 32 + y] for y in range(32)] for k in range(W)]
-        self.masks = w[n * W + W + 32 * W:]
+        self.masks = w[n * W + 32 * W:]
         assert len(self.masks) == (self.rows + 1) * 32
 
 
 def _walk_item(T, cbits, first_lane, lanes_total):
-    """One wavefront's item: the sum of its 64 lanes' accumulated terms
-    (doubled row sums, so 2^n times the true terms)."""
+    """One wavefront's item: the sum of its 64 lanes' accumulated terms."""
     n, W = T.n, T.W
     lanes = range(64)
     active = [first_lane + l < lanes_total for l in lanes]
@@ -66,7 +64,7 @@ def _walk_item(T, cbits, first_lane, lanes_total):
     def consume(cnt):
         etail, ttail, ehead = state["etail"], state["ttail"], state["ehead"]
         owner = [0] * 64
-        for rel_e in range(min(ehead - etail, 2 * 64)):
+        for rel_e in range(1, min(ehead - etail, 65)):
             rel = (ring[W + 1][(etail + rel_e) % RING] - ttail) & MASK32
             if rel < 64:
                 owner[rel] = rel_e
@@ -87,8 +85,8 @@ def _walk_item(T, cbits, first_lane, lanes_total):
             if l == 63:
                 last = running[l] + (1 if k + 1 == bin(mask).count("1") else 0)
             if l < cnt:
-                signed = [(((words[kk] - T.inner[kk][y]) + T.correction[kk]) & MASK32) ^ 0x80808080
-                          for kk in range(W)]
+                halved = [((words[kk] - T.inner[kk][y]) & MASK32) >> 1 for kk in range(W)]
+                signed = [((h + 0x40404040) & MASK32) ^ 0x80808080 for h in halved]
                 term = math.prod(_signed_byte(_byte(signed[i // 4], i % 4)) for i in range(n))
                 acc[l] += -term if sign else term
         if cnt == 64:
@@ -144,7 +142,7 @@ def _model_permanent(A, cbits):
     T = _Tables(A)
     lanes_total = 1 << (T.n - 1 - cbits)
     total = sum(_walk_item(T, cbits, first, lanes_total) for first in range(0, lanes_total, 64))
-    scale = 1 << (2 * T.n - 1)  # doubled row sums (2^n) and Glynn's 2^(n-1)
+    scale = 1 << (T.n - 1)  # Glynn's 2^(n-1)
     assert total % scale == 0
     return total // scale
 
