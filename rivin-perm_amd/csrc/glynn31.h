@@ -923,7 +923,7 @@ __device__ __forceinline__ void glynn_chunk_set_skip(const int32_t* __restrict__
 // over y, indexed by the row's byte, so one LDS lookup per row (an SDWA
 // byte add and a ds_read_b32) tests 32 states.  The states left (every
 // tested row nonzero) are queued as entries of a per-wavefront LDS ring:
-// the lane's outer row words with the sign in bit 0, its survivor mask and
+// the lane's outer row words with the sign, its survivor mask and
 // the index of its first state in the wavefront's running count.  Whenever
 // 64 states are queued, each lane takes one: it finds its entry by the
 // entry's first index (entries starting inside the batch mark their first
@@ -931,7 +931,9 @@ __device__ __forceinline__ void glynn_chunk_set_skip(const int32_t* __restrict__
 // in the survivor mask, rebuilds the row sums from the inner table and adds
 // the term as ZeroSkipSink does.  Row bytes are doubled sums 2 s + beta
 // (beta = 128 even degree, 126 odd), all multiples of 4, so the bytes index
-// the mask tables directly; the taker halves them back to s.
+// the mask tables directly.  Entries hold the halved outer words (the sign
+// in bit 31), and the inner table the halved, re-biased inner words, so a
+// word of a taken state's signed sums is one subtraction and one XOR.
 
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 
@@ -1061,18 +1063,16 @@ struct BlockSkipWalk {
         const uint32_t y = kth_set_bit(mask, k);
         uint32_t w[W];
         uint32_t sign = 0;
-        // The outer word less the inner one has the bytes 2 s + 128 (the
-        // inner table carries the odd rows' bias correction); halved, s + 64
-        // (bit 0 of every byte is 0, so no bit crosses a byte), then signed
-        // as in ZeroSkipSink.
+        // The halved outer word less the inner word has the bytes s + 128
+        // (build_block_tables); flipping bit 7 gives the signed sums s.
 #pragma unroll
         for (int kk = 0; kk < W; ++kk) {
             uint32_t o = ring[kk * kRing + slot];
             if (kk == 0) {
-                sign = o & 1u;
-                o ^= sign;
+                sign = o >> 31;
+                o &= 0x7fffffffu;
             }
-            w[kk] = byte_rows_signed((o - inner[kk * 32 + y]) >> 1);
+            w[kk] = (o - inner[kk * 32 + y]) ^ 0x80808080u;
         }
         sign ^= (uint32_t)__popc(y) & 1u;
         if (lane < cnt) {
@@ -1119,8 +1119,11 @@ struct BlockSkipWalk {
             const uint32_t rank =
                 __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
             const uint32_t slot = (ehead + rank) & (kRing - 1);
+            // Halved outer words (every byte is even); word 0 carries the
+            // outer parity in bit 31 (free: a halved top byte is below 128).
 #pragma unroll
-            for (int kk = 0; kk < W; ++kk) ring[kk * kRing + slot] = kk == 0 ? (rs.r[0] | par) : rs.r[kk];
+            for (int kk = 0; kk < W; ++kk)
+                ring[kk * kRing + slot] = kk == 0 ? __builtin_amdgcn_alignbit(par, rs.r[0], 1) : rs.r[kk] >> 1;
             ring[W * kRing + slot] = surv;
             ring[(W + 1) * kRing + slot] = thead + incl - c;
         }

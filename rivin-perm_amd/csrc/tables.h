@@ -224,14 +224,16 @@ inline size_t block_table_words(int n, int rows) {
 // byte_stride(n)):
 //   [0, n W)          the bytes of state 0 (every column sign +1), then the
 //                     flip deltas 4 a_ij of the n-1 Gray columns;
-//   then 32 W         the inner table: word k of the deltas summed over the
-//                     set bits y_j of y (the first kBlockInnerBits Gray
-//                     columns) less the per-byte corrections 128 - beta_i,
-//                     at k * 32 + y, as a 32-bit difference: an outer word
-//                     minus it has the bytes 2 s + 128 of the row sums at
-//                     (outer state, y), each in [0, 254], so the word
-//                     difference is exact whatever the borrows inside the
-//                     table word;
+//   then 32 W         the inner table, word k of inner state y at k * 32 + y:
+//                     with I the deltas summed over the set bits y_j of y
+//                     (the first kBlockInnerBits Gray columns) and C the
+//                     per-byte corrections 128 - beta_i, the word
+//                     (I - C) / 2 - 0x40404040 (modulo 2^32; I - C is even
+//                     and lies in (-2^31, 2^31)).  Half an outer word minus
+//                     it has the bytes s + 128 of the row sums at (outer
+//                     state, y): (O - I + C) / 2 has the bytes s + 64 in
+//                     [1, 127] (O, I and C have even bytes), so no byte
+//                     carries into the next;
 //   then (rows+1) 32  the masks: entry e of row i (i < rows), at word
 //                     32 i + e, has bit y set when the row's sum at inner
 //                     state y is zero for an outer byte 4 e, i.e. when e =
@@ -273,7 +275,10 @@ inline void build_block_tables(const int64_t* M, int n, const std::vector<int>& 
         }
     }
     for (int k = 0; k < W; ++k)
-        for (int y = 0; y < 32; ++y) words[inner_at + (size_t)k * 32 + y] -= correction[k];
+        for (int y = 0; y < 32; ++y) {
+            const int64_t d = (int64_t)words[inner_at + (size_t)k * 32 + y] - (int64_t)correction[k];
+            words[inner_at + (size_t)k * 32 + y] = (uint32_t)(d / 2 - 0x40404040);
+        }
     for (size_t k = 0; k < words.size(); ++k) out[k] = (int32_t)words[k];
 }
 

@@ -79,14 +79,13 @@ def _walk_item(T, cbits, first_lane, lanes_total):
                 m &= m - 1
             y = (m & -m).bit_length() - 1 if m else 31
             words = [ring[kk][slot] for kk in range(W)]
-            sign = words[0] & 1
-            words[0] ^= sign
+            sign = words[0] >> 31
+            words[0] &= 0x7FFFFFFF
             sign ^= bin(y).count("1") & 1
             if l == 63:
                 last = running[l] + (1 if k + 1 == bin(mask).count("1") else 0)
             if l < cnt:
-                halved = [((words[kk] - T.inner[kk][y]) & MASK32) >> 1 for kk in range(W)]
-                signed = [((h + 0x40404040) & MASK32) ^ 0x80808080 for h in halved]
+                signed = [((words[kk] - T.inner[kk][y]) & MASK32) ^ 0x80808080 for kk in range(W)]
                 term = math.prod(_signed_byte(_byte(signed[i // 4], i % 4)) for i in range(n))
                 acc[l] += -term if sign else term
         if cnt == 64:
@@ -113,7 +112,7 @@ def _walk_item(T, cbits, first_lane, lanes_total):
                 slot = (state["ehead"] + rank) % RING
                 rank += 1
                 for kk in range(W):
-                    ring[kk][slot] = outer[l][kk] | (par[l] if kk == 0 else 0)
+                    ring[kk][slot] = (outer[l][kk] >> 1) | ((par[l] << 31) if kk == 0 else 0)
                 ring[W][slot] = surv[l]
                 ring[W + 1][slot] = (state["thead"] + incl[l] - counts[l]) & MASK32
         state["ehead"] += rank
