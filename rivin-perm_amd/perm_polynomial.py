@@ -17,8 +17,10 @@ options, validation messages and return values.
 Every residue comes from the coefficient-valued Glynn kernel on the device
 (``_polynomial_ext`` is the GPU object shaped like the reference's native
 extension).  Bounds, prime plans and CRT stay on the host in exact
-integers.  The reference's portable Python backend is not part of this
-build (backend='python' raises).  The device kernel takes orders up to 40
+integers.  backend='python' (the reference's portable backend) is accepted
+and computes on the device like 'native': the coefficients are the same
+integers, and this build has no host arithmetic path.  The device kernel
+takes orders up to 40
 (2^39 Gray states of degree-n products per modulus); orders 41-63, which the
 reference accepts, raise ValueError.  Moduli of 2^32 and above run on the
 device up to order 32; above it their coefficients come from the exact
@@ -56,7 +58,7 @@ def polynomial_backend_capabilities() -> dict:
     """Available backends and native hooks (perm_polynomial.py:70-95)."""
     hooks = tuple(name for name in _POLYNOMIAL_NATIVE_HOOK_NAMES if hasattr(_polynomial_ext, name))
     return {
-        "available_backends": ("native",),
+        "available_backends": ("native", "python"),
         "default_backend": "native",
         "native_extension_available": True,
         "native_hooks": hooks,
@@ -104,10 +106,8 @@ def _select_backend(backend) -> str:
     if not isinstance(backend, str):
         raise TypeError("backend must be a string")
     chosen = backend.lower()
-    if chosen in ("auto", "native"):
+    if chosen in ("auto", "native", "python"):
         return "native"
-    if chosen == "python":
-        raise ValueError("the portable Python backend is not part of this build; use backend='auto'")
     raise ValueError("backend must be one of: 'auto', 'native', 'python'")
 
 

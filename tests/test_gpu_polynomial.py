@@ -169,3 +169,19 @@ def test_exact_public_functions(native, oracle_c):
     W = [[rng.randrange(0, 70000) for _ in range(n)] for _ in range(n)]
     wr = pp.weighted_rook_polynomial(W)
     assert wr[1] == sum(map(sum, W)) and wr[n] == oracle.subset_permanent(W)
+
+
+def test_python_backend_name_runs_on_the_device():
+    """backend='python' (the reference's portable backend) is accepted and
+    gives the native coefficients: the rook polynomial of J_9 is
+    C(9, k)^2 k!, and a random board's equals the 'auto' result."""
+    import math
+
+    import numpy as np
+    import perm_polynomial as poly
+
+    J = [[1] * 9 for _ in range(9)]
+    want = tuple(math.comb(9, k) ** 2 * math.factorial(k) for k in range(10))
+    assert poly.rook_polynomial(J, backend="python") == want
+    board = np.random.default_rng(12).integers(0, 2, (11, 11)).tolist()
+    assert poly.rook_polynomial(board, backend="python") == poly.rook_polynomial(board, backend="auto")

@@ -223,10 +223,12 @@ def test_matrix_and_option_validation(poly):
         poly.rook_polynomial([[1]], num_threads=-1)
     with pytest.raises(ValueError, match="backend must be one of"):
         poly.rook_polynomial([[1]], backend="gpu")
-    with pytest.raises(ValueError, match="portable Python backend"):
-        poly.rook_polynomial([[1]], backend="python")
+    # The portable backend's name is accepted and computes like the native
+    # one (a one-cell board's rook polynomial is closed form on the host).
+    assert poly.rook_polynomial([[1]], backend="python") == poly.rook_polynomial([[1]]) == (1, 1)
     caps = poly.polynomial_backend_capabilities()
     assert caps["default_backend"] == "native" and "affine_permanent_mod" in caps["native_hooks"]
+    assert caps["available_backends"] == ("native", "python")
 
 
 def test_native_hooks_validate_like_the_extension(poly):
