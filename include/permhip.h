@@ -69,15 +69,15 @@ int phip_permanent_residues_noalloc(const int64_t *A, int64_t prime_stride, int 
                                     int k, int64_t *out);
 
 /* Coefficients of per(C + x L) modulo each of k odd moduli 3 <= m < 2^63
- * (not necessarily prime), 0 <= n <= 40 for moduli below 2^32 and n <= 32
- * for larger ones.  C and L are row-major int64;
+ * (not necessarily prime), 0 <= n <= 63 for moduli below 2^30, n <= 40
+ * below 2^32 and n <= 32 for larger ones.  C and L are row-major int64;
  * prime_stride as in phip_permanent_residues (0: one pair for every modulus;
  * n*n: k pairs already reduced by the caller).  out[i*(n+1) + d] receives
  * the coefficient of x^d modulo moduli[i] in [0, m).  Returns 0 or -1.
  * Device form of perm_polynomial_ext.affine_permanent_mod
  * (perm_polynomial_omp.c:1207-1260; perm_polynomial.py:402-443), all
- * moduli in one pass.  The reference accepts orders up to 63; orders above
- * 40 (2^40 Gray states of degree-n products per modulus) are rejected here. */
+ * moduli in one pass.  Orders 41..63 take 2^40 and more Gray states of
+ * degree-n products per modulus (hours at order 44). */
 int phip_affine_permanent_residues(const int64_t *C, const int64_t *L, int64_t prime_stride, int n,
                                    const int64_t *moduli, int k, int64_t *out);
 
@@ -258,7 +258,8 @@ int phip_batch_destroy(void *plan);
 #define PHIP_TUNE_NECKLACE_MIN_ORDER 5 /* Schur: smallest prime order using necklace orbits (5..64) */
 #define PHIP_TUNE_NECKLACE_DEPTH 6   /* Schur necklaces: largest FKM prefix depth of the tasks (2..30) */
 #define PHIP_TUNE_WIDE_MIN_TIER 7    /* wide-modulus Glynn: smallest row-register tier (8..64, a multiple of 4) */
-#define PHIP_TUNE_AFFINE_MIN_TIER 8  /* affine Glynn: smallest row-register tier (8..40, a multiple of 4) */
+#define PHIP_TUNE_AFFINE_MIN_TIER 8  /* affine Glynn: smallest row-register tier (8..40 a multiple of 4, or 48,
+                                        56, 64: taken by moduli below 2^30, others use their largest tier) */
 #define PHIP_TUNE_SKIP_ZEROS_PERMILLE 9 /* 0/1 tables: largest estimated share of nonzero Gray terms, in
                                            permille, that runs on the zero-skipping kernels (0..1000, 0 off;
                                            read when a plan is built) */

@@ -11,8 +11,11 @@
 // One launch per residue arithmetic (S30 below 2^30: signed lazy Montgomery
 // with exact int64 accumulators; wide_arith.h: Mont31L below 2^31 with one
 // lazy reduction per coefficient, Mont32, M61, PM61, Mont64) and
-// register tier (every 4 rows from 8 to 40 for 32-bit residues, every 8 to
-// 32 for 64-bit ones); blockIdx.y selects the modulus.
+// register tier (every 4 rows from 8 to 40 for 32-bit residues, then 48, 56
+// and 64 for moduli below 2^30, every 8 to 32 for 64-bit ones); blockIdx.y
+// selects the modulus.  The tiers above 40 hold more values than the 256
+// VGPRs: the compiler keeps the excess in AGPRs (no scratch; one wavefront
+// per SIMD at tier 64).
 // As in wide.hip the flipped column and its direction are wave-uniform
 // except at a chunk's middle state, so deltas and negated deltas come by
 // scalar loads.  Each wavefront folds its lanes' accumulators with
@@ -22,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "affine.h"
@@ -309,6 +313,18 @@ int launch_affine(int tier, dim3 grid, hipStream_t st, const typename A::T* c0, 
             case 32: PHIP_AFFINE_LAUNCH(32); break;
             case 36: PHIP_AFFINE_LAUNCH(36); break;
             case 40: PHIP_AFFINE_LAUNCH(40); break;
+            case 48:
+            case 56:
+            case 64:
+                if constexpr (std::is_same<A, S30>::value) {
+                    if (tier == 48) PHIP_AFFINE_LAUNCH(48);
+                    else if (tier == 56) PHIP_AFFINE_LAUNCH(56);
+                    else PHIP_AFFINE_LAUNCH(64);
+                    break;
+                } else {
+                    set_error("internal: no affine kernel for this register tier");
+                    return -1;
+                }
             default: set_error("internal: no affine kernel for this register tier"); return -1;
         }
     } else {
@@ -334,15 +350,17 @@ int run_affine_class(DeviceContext* ctx, const int64_t* C, const int64_t* L, int
     const int kk = (int)jobs.size();
     if (kk == 0) return 0;
     // Register tier: the smallest that holds n rows (32-bit residues have
-    // tiers every 4 rows from 8 to 40; 64-bit ones every 8 to 32), or the
-    // tuning floor.
-    const int step = sizeof(T) == 4 ? 4 : 8;
-    const int top = sizeof(T) == 4 ? kAffineMaxOrder : kAffineMaxOrderWide;
+    // tiers every 4 rows from 8 to 40, and moduli below 2^30 every 8 from
+    // 48 to 64; 64-bit ones every 8 to 32), or the tuning floor.
+    constexpr bool kS30 = std::is_same<A, S30>::value;
+    const int top = sizeof(T) == 4 ? (kS30 ? kAffineMaxOrderS30 + 1 : kAffineMaxOrder) : kAffineMaxOrderWide;
     if (n > top) {
-        set_error("affine permanents of order above 32 need moduli below 2^32 on the device");
+        set_error(sizeof(T) == 4 ? "affine permanents of order above 40 need moduli below 2^30 on the device"
+                                 : "affine permanents of order above 32 need moduli below 2^32 on the device");
         return -1;
     }
-    const int rows = std::max({8, n, min_tier});
+    const int rows = std::min(top, std::max({8, n, min_tier}));
+    const int step = (sizeof(T) == 4 && rows <= kAffineMaxOrder) ? 4 : 8;
     const int W = std::min(top, (rows + step - 1) / step * step);  // rows per table column
     const size_t cols = (size_t)(n - 1) * W;
     // One host image: c0, l0 ([kk][W] each), then dc, ndc, dl, ndl
@@ -451,8 +469,8 @@ int run_affine_class(DeviceContext* ctx, const int64_t* C, const int64_t* L, int
 
 int affine_permanent_residues(DeviceContext* ctx, const int64_t* C, const int64_t* L, int64_t stride, int n,
                               const int64_t* moduli, int k, int min_tier, int64_t* out) {
-    if (n < 0 || n > kAffineMaxOrder) {
-        set_error("affine permanent orders must lie in [0, 40] on the device");
+    if (n < 0 || n > kAffineMaxOrderS30) {
+        set_error("affine permanent orders must lie in [0, 63] on the device");
         return -1;
     }
     if (k < 0 || (k > 0 && (!moduli || !out))) {

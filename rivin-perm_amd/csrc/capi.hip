@@ -134,7 +134,7 @@ int phip_set_tuning(int key, int64_t value) {
             t.wide_min_tier = (int)value;
             return 0;
         case PHIP_TUNE_AFFINE_MIN_TIER:
-            if (value < 8 || value > 40 || value % 4 != 0) break;
+            if (value < 8 || value > 64 || value % 4 != 0 || (value > 40 && value % 8 != 0)) break;
             t.affine_min_tier = (int)value;
             return 0;
         case PHIP_TUNE_SKIP_ZEROS_PERMILLE:

@@ -20,11 +20,10 @@ extension).  Bounds, prime plans and CRT stay on the host in exact
 integers.  backend='python' (the reference's portable backend) is accepted
 and computes on the device like 'native': the coefficients are the same
 integers, and this build has no host arithmetic path.  The device kernel
-takes orders up to 40
-(2^39 Gray states of degree-n products per modulus); orders 41-63, which the
-reference accepts, raise ValueError.  Moduli of 2^32 and above run on the
-device up to order 32; above it their coefficients come from the exact
-coefficients (30-bit primes and CRT) reduced.
+takes every order the reference accepts (below 64) for moduli below 2^30,
+up to order 40 below 2^32 and up to 32 above; beyond those orders a
+modulus gets the exact coefficients (30-bit primes and CRT) reduced.  Orders
+above 40 walk 2^40 and more Gray states of degree-n products per modulus.
 """
 
 from __future__ import annotations
@@ -272,17 +271,26 @@ def _crt_vectors(rows: Sequence[Sequence[int]], primes: Sequence[int]) -> tuple:
 # Device residues.
 
 
-# Largest order of the device kernel; moduli of 2^32 and above stop at 32.
-DEVICE_MAX_ORDER = 40
+# Largest orders of the device kernel: every order below 64 for moduli below
+# 2^30, 40 for moduli below 2^32, 32 for larger ones.
+DEVICE_MAX_ORDER = 63
+DEVICE_MAX_ORDER_32 = 40
 DEVICE_MAX_ORDER_WIDE = 32
+
+
+def _device_takes(n: int, modulus: int) -> bool:
+    """Whether the device kernel computes order n modulo `modulus` itself
+    (otherwise the exact coefficients are reduced)."""
+    if modulus < (1 << 30):
+        return n <= DEVICE_MAX_ORDER
+    if modulus < (1 << 32):
+        return n <= DEVICE_MAX_ORDER_32
+    return n <= DEVICE_MAX_ORDER_WIDE
 
 
 def _check_device_order(n: int) -> None:
     if n > DEVICE_MAX_ORDER:
-        raise ValueError(
-            f"polynomial permanents of order above {DEVICE_MAX_ORDER} are beyond the device kernel "
-            f"(2^(n-1) Gray states of degree-n products per modulus)"
-        )
+        raise ValueError("matrix order must be less than 64")
 
 
 def _device_affine_residues(constant, linear, primes) -> list:
@@ -307,8 +315,9 @@ def _device_affine_residues(constant, linear, primes) -> list:
 def _affine_permanent_mod_validated(constant, linear, modulus: int) -> tuple:
     n = len(constant)
     _check_device_order(n)
-    if n > DEVICE_MAX_ORDER_WIDE and modulus >= (1 << 32):
-        # Beyond the 64-bit kernels' orders: the exact coefficients reduced.
+    if not _device_takes(n, modulus):
+        # Beyond the kernels' orders for this modulus: the exact
+        # coefficients (30-bit primes) reduced.
         return tuple(v % modulus for v in _affine_permanent_validated(constant, linear))
     reduced_c = tuple(tuple(v % modulus for v in row) for row in constant)
     reduced_l = tuple(tuple(v % modulus for v in row) for row in linear)

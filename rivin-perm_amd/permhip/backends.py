@@ -327,11 +327,11 @@ class HipPolynomialExtension:
             return (1 % m,)
         if n == 1:
             return (int(C[0, 0]) % m, int(L[0, 0]) % m)
-        if n > 32 and m >= (1 << 32):
-            # The 64-bit kernels stop at order 32: exact coefficients
-            # (30-bit primes, CRT), reduced.
-            import perm_polynomial
+        import perm_polynomial
 
+        if not perm_polynomial._device_takes(n, m):
+            # Beyond the kernels' orders for this modulus: exact
+            # coefficients (30-bit primes, CRT), reduced.
             exact = perm_polynomial._affine_permanent_validated(tuple(map(tuple, C.tolist())),
                                                                 tuple(map(tuple, L.tolist())))
             return tuple(v % m for v in exact)
@@ -360,7 +360,16 @@ class HipPolynomialExtension:
             return tuple((1, int(A[0, 0]) % q) for q in mods)
         # r_k = [x^k] per(J + x A) / (n - k)!
         J = np.ones((n, n), dtype=np.int64)
-        rows = _native.affine_permanent_residues(J, A, mods)
+        import perm_polynomial
+
+        if all(perm_polynomial._device_takes(n, q) for q in mods):
+            rows = _native.affine_permanent_residues(J, A, mods)
+        else:
+            # Orders the kernels take only below 2^30 for these primes: the
+            # exact coefficients (30-bit primes, CRT), reduced per prime.
+            exact = perm_polynomial._affine_permanent_validated(tuple(map(tuple, J.tolist())),
+                                                                tuple(map(tuple, A.tolist())))
+            rows = [[v % q for v in exact] for q in mods]
         out = []
         for q, row in zip(mods, rows):
             fact = [1] * (n + 1)

@@ -23,8 +23,11 @@ def _pair(rng, n, lo, hi):
     return C, L
 
 
-@pytest.mark.parametrize("tier", [8, 12, 16, 20, 24, 28, 32, 36, 40])
+@pytest.mark.parametrize("tier", [8, 12, 16, 20, 24, 28, 32, 36, 40, 48, 56, 64])
 def test_affine_register_tiers(native, oracle_c, tier):
+    """Every register tier against the oracle at small orders (rows beyond
+    the order are identity factors).  Tiers 48, 56 and 64 are taken by the
+    moduli below 2^30; the others run their own largest tier."""
     old = native.get_tuning(native.TUNE_AFFINE_MIN_TIER)
     try:
         native.set_tuning(native.TUNE_AFFINE_MIN_TIER, tier)
@@ -94,9 +97,25 @@ def test_affine_wide_moduli_above_order_32(native):
     exact = pp.affine_permanent(J, A)
     assert pp.affine_permanent_mod(J, A, m) == tuple(v % m for v in exact)
     assert tuple(perm_polynomial_ext.affine_permanent_mod(J, A, m)) == tuple(v % m for v in exact)
-    with pytest.raises(ValueError, match="above 40"):
-        big = [[1] * 41 for _ in range(41)]
-        pp.affine_permanent_mod(big, big, 101)
+
+
+def test_affine_largest_tier_at_order_24(native):
+    """The tier-64 kernel (its excess values in AGPRs) at order 24 gives
+    the default tier's coefficient vectors, for moduli below 2^30."""
+    rng = random.Random(7290)
+    n = 24
+    C, L = _pair(rng, n, -5, 5)
+    C = np.array(C, dtype=np.int64)
+    L = np.array(L, dtype=np.int64)
+    moduli = [1_000_003, 536_870_909, 1_073_741_789]
+    want = native.affine_permanent_residues(C, L, moduli)
+    old = native.get_tuning(native.TUNE_AFFINE_MIN_TIER)
+    try:
+        native.set_tuning(native.TUNE_AFFINE_MIN_TIER, 64)
+        print("affine tier 64 n=24", flush=True)
+        assert native.affine_permanent_residues(C, L, moduli) == want
+    finally:
+        native.set_tuning(native.TUNE_AFFINE_MIN_TIER, old)
 
 
 def test_affine_per_modulus_stride(native, oracle_c):
@@ -116,8 +135,11 @@ def test_affine_rejections(native):
     A = np.eye(3, dtype=np.int64)
     with pytest.raises(RuntimeError, match="odd"):
         native.affine_permanent_residues(A, A, [4])
-    with pytest.raises(RuntimeError, match="40"):
+    with pytest.raises(RuntimeError, match="2\\^30"):
         big = np.zeros((41, 41), dtype=np.int64)
+        native.affine_permanent_residues(big, big, [2_147_483_647])
+    with pytest.raises(RuntimeError, match="63"):
+        big = np.zeros((64, 64), dtype=np.int64)
         native.affine_permanent_residues(big, big, [101])
     with pytest.raises(RuntimeError, match="2\\^32"):
         big = np.zeros((33, 33), dtype=np.int64)

@@ -242,3 +242,14 @@ def test_native_hooks_validate_like_the_extension(poly):
     with pytest.raises(ValueError, match="between 0 and 65535"):
         ext.weighted_rook_polynomial_mod_many([[70000]], [101])
     assert ext.rook_polynomial_mod_many([[1, 1], [1, 1]], [101, 103]) == ((1, 4, 2), (1, 4, 2))
+
+
+def test_device_order_routing(poly):
+    """Which (order, modulus) pairs the device kernel takes itself: every
+    order below 64 for moduli below 2^30, up to 40 below 2^32, up to 32
+    above; other pairs reduce the exact coefficients."""
+    takes = poly._device_takes
+    assert takes(63, (1 << 30) - 35) and takes(41, 101)
+    assert takes(40, (1 << 31) - 1) and not takes(41, (1 << 31) - 1)
+    assert takes(40, (1 << 32) - 5) and not takes(41, (1 << 32) - 5)
+    assert takes(32, (1 << 61) - 1) and not takes(33, (1 << 61) - 1)
