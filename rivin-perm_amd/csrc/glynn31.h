@@ -926,8 +926,8 @@ __device__ __forceinline__ void glynn_chunk_set_skip(const int32_t* __restrict__
 // the lane's outer row words with the sign, its survivor mask and
 // the index of its first state in the wavefront's running count.  Whenever
 // 64 states are queued, each lane takes one: it finds its entry by the
-// entry's first index (entries starting inside the batch mark their first
-// lane, a wave-wide running maximum spreads them), picks the y of its rank
+// entry's first index (entries starting inside the batch flag their first
+// lane; a lane's entry is the number of flags up to it), picks the y of its rank
 // in the survivor mask, rebuilds the row sums from the inner table and adds
 // the term as ZeroSkipSink does.  Row bytes are doubled sums 2 s + beta
 // (beta = 128 even degree, 126 odd), all multiples of 4, so the bytes index
@@ -946,18 +946,6 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
     v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);
     v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);
     v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);
-    return v;
-}
-
-// Inclusive running maximum over the wavefront (unsigned, lanes without a
-// source contribute 0).
-__device__ __forceinline__ uint32_t wave_inclusive_max(uint32_t v) {
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false));
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false));
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));
     return v;
 }
 
@@ -990,7 +978,7 @@ struct BlockSkipWalk {
     uint32_t pinv[K];
     int64_t acc[K];
     lds_u32* inner;       // 32 W words: word k of inner state y at k * 32 + y
-    lds_u32* owner;       // 64 words
+    lds_u32* owner;       // 64 words: flags of the lanes where an entry starts
     lds_u32* ring;        // (W + 2) * kRing words
     uint32_t mask_base;   // LDS byte address of row 0's mask table (row i at + 128 i)
     int test_words;       // rows tested / 4 (wave-uniform)
@@ -1047,16 +1035,19 @@ struct BlockSkipWalk {
         // inside the batch are among the 64 after the oldest live one
         // (etail, which holds state ttail and is every lane's default).
         const uint32_t live = ehead - etail;
-        {
-            const uint32_t rel_e = lane + 1u;
-            if (rel_e < live) {
-                const uint32_t slot = (etail + rel_e) & (kRing - 1);
-                const uint32_t rel = ring[(W + 1) * kRing + slot] - ttail;
-                if (rel < 64u) owner[rel] = rel_e;
-            }
+        if (lane + 1u < live) {
+            const uint32_t slot = (etail + lane + 1u) & (kRing - 1);
+            const uint32_t rel = ring[(W + 1) * kRing + slot] - ttail;
+            if (rel < 64u) owner[rel] = 1u;
         }
         __builtin_amdgcn_wave_barrier();
-        const uint32_t rel_e = wave_inclusive_max(owner[lane]);
+        // Entries start in order, so a lane's entry (relative to etail) is
+        // the number of entries that start at or before it.
+        const uint32_t flag = owner[lane];
+        const uint64_t starts = __ballot(flag != 0u);
+        const uint32_t rel_e =
+            __builtin_amdgcn_mbcnt_hi((uint32_t)(starts >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)starts, 0u)) +
+            flag;
         const uint32_t slot = (etail + rel_e) & (kRing - 1);
         const uint32_t k = ttail + lane - ring[(W + 1) * kRing + slot];
         const uint32_t mask = ring[W * kRing + slot];

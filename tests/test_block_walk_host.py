@@ -1,7 +1,7 @@
 """Host side of the block zero-skipping walk: the library's block tables
 (phip_block_tables, the tables a plan uploads) driven through a lane-level
 model of the device walk (64 lanes, the LDS ring of survivor entries, the
-owner marks and running maximum that hand each lane its state, the k-th
+owner flags whose running count hands each lane its entry, the k-th
 set bit of a survivor mask) give the permanent of random 0/1 matrices, as
 the oracle's subset DP and permutation sum compute it.  CPU only: the GPU
 tests run the kernel itself against the oracle."""
@@ -63,12 +63,12 @@ def _walk_item(T, cbits, first_lane, lanes_total):
 
     def consume(cnt):
         etail, ttail, ehead = state["etail"], state["ttail"], state["ehead"]
-        owner = [0] * 64
+        flags = [0] * 64
         for rel_e in range(1, min(ehead - etail, 65)):
             rel = (ring[W + 1][(etail + rel_e) % RING] - ttail) & MASK32
             if rel < 64:
-                owner[rel] = rel_e
-        running = list(itertools.accumulate(owner, max))
+                flags[rel] = 1
+        running = list(itertools.accumulate(flags))
         last = None
         for l in lanes:
             slot = (etail + running[l]) % RING
