@@ -98,9 +98,10 @@ constexpr int skip_half_words(int n) { return n < kSkipHalfMinOrder ? (n + 3) / 
 // kBlockInnerBits fastest Gray columns are enumerated per outer state, and
 // one 32-bit LDS lookup per row marks the inner states that zero the row.
 // Its per-wavefront LDS region (words): the rows' mask tables (32 words a
-// row, plus 32), the inner row-sum table (32 words per row word), 64
-// words of owners and kBlockRing entries of (row words, survivor mask,
-// first state index).
+// row, plus 32), the inner row-sum tables (12 words per row word: 4 for
+// the first two inner columns, 8 for the other three), 64 words of owner
+// flags and kBlockRing entries of (row words, survivor mask, first state
+// index).
 constexpr int kBlockInnerBits = 5;
 constexpr int kBlockRing = 128;
 constexpr int kBlockMinChunkBits = kBlockInnerBits + 1;
@@ -108,7 +109,7 @@ constexpr int kBlockMinChunkBits = kBlockInnerBits + 1;
 // per-state test).
 constexpr int kBlockMinOrder = 8;
 constexpr int block_region_words(int n, int rows) {
-    return (rows + 1) * 32 + 32 * ((n + 3) / 4) + 64 + ((n + 3) / 4 + 2) * kBlockRing;
+    return (rows + 1) * 32 + 12 * ((n + 3) / 4) + 64 + ((n + 3) / 4 + 2) * kBlockRing;
 }
 // Dynamic LDS of one workgroup (4 wavefronts) of a class's kernel: the
 // zero-skip queues or block-walk regions, nothing for the other variants.

@@ -977,7 +977,7 @@ struct BlockSkipWalk {
     int32_t negp[K];
     uint32_t pinv[K];
     int64_t acc[K];
-    lds_u32* inner;       // 32 W words: word k of inner state y at k * 32 + y
+    lds_u32* inner;       // 12 W words: row word k's two inner tables at 12 k (build_block_tables)
     lds_u32* owner;       // 64 words: flags of the lanes where an entry starts
     lds_u32* ring;        // (W + 2) * kRing words
     uint32_t mask_base;   // LDS byte address of row 0's mask table (row i at + 128 i)
@@ -1054,8 +1054,9 @@ struct BlockSkipWalk {
         const uint32_t y = kth_set_bit(mask, k);
         uint32_t w[W];
         uint32_t sign = 0;
-        // The halved outer word less the inner word has the bytes s + 128
-        // (build_block_tables); flipping bit 7 gives the signed sums s.
+        // The halved outer word plus the two inner words has the bytes
+        // s + 128 (build_block_tables); flipping bit 7 gives the signed sums.
+        const uint32_t ya = y & 3u, yb = 4u + (y >> 2);
 #pragma unroll
         for (int kk = 0; kk < W; ++kk) {
             uint32_t o = ring[kk * kRing + slot];
@@ -1063,7 +1064,7 @@ struct BlockSkipWalk {
                 sign = o >> 31;
                 o &= 0x7fffffffu;
             }
-            w[kk] = (o - inner[kk * 32 + y]) ^ 0x80808080u;
+            w[kk] = (o + inner[kk * 12 + ya] + inner[kk * 12 + yb]) ^ 0x80808080u;
         }
         sign ^= (uint32_t)__popc(y) & 1u;
         if (lane < cnt) {
@@ -1148,7 +1149,7 @@ __device__ __forceinline__ void glynn_chunk_set_blocks(const int32_t* __restrict
     const uint32_t lane = __lane_id();
     lds_u32* const lds = (lds_u32*)region;
     walk.inner = lds + (rows + 1) * 32;
-    walk.owner = walk.inner + 32 * W;
+    walk.owner = walk.inner + 12 * W;
     walk.ring = walk.owner + 64;
     walk.mask_base = (uint32_t)(uintptr_t)lds;
     walk.test_words = rows / 4;
@@ -1158,8 +1159,8 @@ __device__ __forceinline__ void glynn_chunk_set_blocks(const int32_t* __restrict
     const int32_t* __restrict__ D = tables + W;
     if (reload) {
         const int32_t* __restrict__ src = tables + (size_t)n * W;
-        for (int i = (int)lane; i < 32 * W; i += 64) walk.inner[i] = (uint32_t)src[i];
-        src += 32 * W;
+        for (int i = (int)lane; i < 12 * W; i += 64) walk.inner[i] = (uint32_t)src[i];
+        src += 12 * W;
         for (int i = (int)lane; i < (rows + 1) * 32; i += 64) lds[i] = (uint32_t)src[i];
     }
     __builtin_amdgcn_wave_barrier();

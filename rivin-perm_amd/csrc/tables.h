@@ -211,7 +211,7 @@ inline int block_test_rows(int n, int even) {
 // leading rows are tested (block_test_rows).
 inline size_t block_table_words(int n, int rows) {
     const size_t W = (size_t)byte_stride(n);
-    return (size_t)n * W + 32 * W + ((size_t)rows + 1) * 32;
+    return (size_t)n * W + 12 * W + ((size_t)rows + 1) * 32;
 }
 
 // Tables of the block zero-skipping walk (glynn31.h,
@@ -224,16 +224,18 @@ inline size_t block_table_words(int n, int rows) {
 // byte_stride(n)):
 //   [0, n W)          the bytes of state 0 (every column sign +1), then the
 //                     flip deltas 4 a_ij of the n-1 Gray columns;
-//   then 32 W         the inner table, word k of inner state y at k * 32 + y:
-//                     with I the deltas summed over the set bits y_j of y
-//                     (the first kBlockInnerBits Gray columns) and C the
-//                     per-byte corrections 128 - beta_i, the word
-//                     (I - C) / 2 - 0x40404040 (modulo 2^32; I - C is even
-//                     and lies in (-2^31, 2^31)).  Half an outer word minus
-//                     it has the bytes s + 128 of the row sums at (outer
-//                     state, y): (O - I + C) / 2 has the bytes s + 64 in
-//                     [1, 127] (O, I and C have even bytes), so no byte
-//                     carries into the next;
+//   then 12 W         the inner tables, for row word k at 12 k: with I the
+//                     deltas summed over the set bits y_j of y (the first
+//                     kBlockInnerBits Gray columns), split as I = A(y & 3)
+//                     + B(y >> 2) over columns 0-1 and 2-4, and C the
+//                     per-byte corrections 128 - beta_i, the 4 words
+//                     -A(a) / 2 and the 8 words -((B(b) - C) / 2 - 0x40404040)
+//                     (modulo 2^32; A and B - C are even and lie in
+//                     (-2^31, 2^31)).  Half an outer word plus both has the
+//                     bytes s + 128 of the row sums at (outer state, y):
+//                     (O - I + C) / 2 has the bytes s + 64 in [1, 127] (O,
+//                     I and C have even bytes), so no byte carries into
+//                     the next;
 //   then (rows+1) 32  the masks: entry e of row i (i < rows), at word
 //                     32 i + e, has bit y set when the row's sum at inner
 //                     state y is zero for an outer byte 4 e, i.e. when e =
@@ -252,8 +254,9 @@ inline void build_block_tables(const int64_t* M, int n, const std::vector<int>& 
     std::vector<uint32_t> words(block_table_words(n, rows), 0u);
     auto put = [&](size_t word_base, int row, uint32_t byte) { words[word_base + row / 4] |= byte << (8 * (row % 4)); };
     auto entry = [&](int i, int j) { return M[(size_t)order[i] * n + j] != 0 ? 1u : 0u; };
-    const size_t inner_at = (size_t)n * W, masks_at = inner_at + 32 * (size_t)W;
+    const size_t inner_at = (size_t)n * W, masks_at = inner_at + 12 * (size_t)W;
     std::vector<uint32_t> correction(W, 0u);
+    std::vector<uint32_t> low((size_t)W * 4, 0u), high((size_t)W * 8, 0u);  // A(a), B(b) per row word
     for (int i = 0; i < 4 * W; ++i) {
         if (i >= n) {
             put(0, i, 132u);
@@ -266,19 +269,28 @@ inline void build_block_tables(const int64_t* M, int n, const std::vector<int>& 
         for (int j = 0; j < n - 1; ++j)
             if (entry(i, j)) put((size_t)(1 + j) * W, i, 4u);
         correction[i / 4] |= (128u - beta) << (8 * (i % 4));
+        for (int a = 0; a < 4; ++a)
+            low[(size_t)(i / 4) * 4 + a] |= (4u * ((a & 1 ? entry(i, 0) : 0u) + (a & 2 ? entry(i, 1) : 0u)))
+                                            << (8 * (i % 4));
+        for (int b = 0; b < 8; ++b)
+            high[(size_t)(i / 4) * 8 + b] |=
+                (4u * ((b & 1 ? entry(i, 2) : 0u) + (b & 2 ? entry(i, 3) : 0u) + (b & 4 ? entry(i, 4) : 0u)))
+                << (8 * (i % 4));
         for (int y = 0; y < 32; ++y) {
             uint32_t ones = 0;
             for (int j = 0; j < kBlockInnerBits; ++j)
                 if ((y >> j) & 1) ones += entry(i, j);
-            words[inner_at + (size_t)(i / 4) * 32 + y] |= (4u * ones) << (8 * (i % 4));
             if (i < even && i < rows) words[masks_at + (size_t)i * 32 + 32 + ones] |= 1u << y;
         }
     }
-    for (int k = 0; k < W; ++k)
-        for (int y = 0; y < 32; ++y) {
-            const int64_t d = (int64_t)words[inner_at + (size_t)k * 32 + y] - (int64_t)correction[k];
-            words[inner_at + (size_t)k * 32 + y] = (uint32_t)(d / 2 - 0x40404040);
+    for (int k = 0; k < W; ++k) {
+        for (int a = 0; a < 4; ++a)
+            words[inner_at + (size_t)k * 12 + a] = (uint32_t)(-((int64_t)low[(size_t)k * 4 + a] / 2));
+        for (int b = 0; b < 8; ++b) {
+            const int64_t d = (int64_t)high[(size_t)k * 8 + b] - (int64_t)correction[k];
+            words[inner_at + (size_t)k * 12 + 4 + b] = (uint32_t)(-(d / 2 - 0x40404040));
         }
+    }
     for (size_t k = 0; k < words.size(); ++k) out[k] = (int32_t)words[k];
 }
 

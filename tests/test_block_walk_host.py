@@ -36,9 +36,10 @@ class _Tables:
         w = [int(x) for x in words]
         self.state0 = w[:W]
         self.deltas = [w[W * (1 + j):W * (2 + j)] for j in range(n - 1)]
-        inner = w[n * W:n * W + 32 * W]
-        self.inner = [[inner[k * 32 + y] for y in range(32)] for k in range(W)]
-        self.masks = w[n * W + 32 * W:]
+        inner = w[n * W:n * W + 12 * W]
+        self.inner = [[(inner[k * 12 + (y & 3)] + inner[k * 12 + 4 + (y >> 2)]) & MASK32 for y in range(32)]
+                      for k in range(W)]
+        self.masks = w[n * W + 12 * W:]
         assert len(self.masks) == (self.rows + 1) * 32
 
 
@@ -85,7 +86,7 @@ def _walk_item(T, cbits, first_lane, lanes_total):
             if l == 63:
                 last = running[l] + (1 if k + 1 == bin(mask).count("1") else 0)
             if l < cnt:
-                signed = [((words[kk] - T.inner[kk][y]) & MASK32) ^ 0x80808080 for kk in range(W)]
+                signed = [((words[kk] + T.inner[kk][y]) & MASK32) ^ 0x80808080 for kk in range(W)]
                 term = math.prod(_signed_byte(_byte(signed[i // 4], i % 4)) for i in range(n))
                 acc[l] += -term if sign else term
         if cnt == 64:
