@@ -419,6 +419,91 @@ def _rook_bounds(rows) -> tuple:
     return tuple(b // factorial(n - k) for k, b in enumerate(_affine_bounds(ones, rows)))
 
 
+# Fixed-point scale (bits) of the partial-matching bounds below.
+_ROOK_SCALE_BITS = 32
+
+
+def _ceil_scaled_power_root(base: int, power: int, degree: int, shift: int) -> int:
+    """ceil(2^shift * base^(power/degree)), exactly (an integer root of an
+    exact integer)."""
+    return _prep.ceil_nth_root(base ** power << (shift * degree), degree)
+
+
+def _matching_bounds(rows) -> tuple:
+    """Upper bounds of the k-matching counts r_k of a 0/1 matrix from
+    Bregman-Minc (the reference's partial-matching bound,
+    perm_polynomial.py:643-692, restated).  For a set S of k rows, append
+    n - k all-one rows: every k-matching of S extends to exactly (n - k)!
+    permutations of that n x n matrix, whose permanent is at most
+    prod_{i in S} (d_i!)^(1/d_i) (n!)^((n-k)/n).  Summing over S gives
+    r_k <= e_k(w) (n!)^((n-k)/n) / (n-k)! with w_i = (d_i!)^(1/d_i) and e_k
+    the elementary symmetric sum; columns give a second bound.  Every
+    factor is rounded up at a 2^32 fixed-point scale, so the floors are
+    rigorous."""
+    n = len(rows)
+    shift = _ROOK_SCALE_BITS
+    roots = [0] + [_ceil_scaled_power_root(factorial(d), 1, d, shift) for d in range(1, n + 1)]
+
+    def symmetric(weights):
+        e = [1] + [0] * n
+        for count, w in enumerate(weights, start=1):
+            for k in range(count, 0, -1):
+                e[k] += e[k - 1] * w
+        return e
+
+    e_rows = symmetric([roots[sum(row)] for row in rows])
+    e_cols = symmetric([roots[sum(rows[i][j] for i in range(n))] for j in range(n)])
+    n_fact = factorial(n)
+    out = [1]
+    for k in range(1, n + 1):
+        dummy = -(-_ceil_scaled_power_root(n_fact, n - k, n, shift) // factorial(n - k))
+        out.append(min(e_rows[k], e_cols[k]) * dummy >> (shift * (k + 1)))
+    return tuple(out)
+
+
+def _rook_parities(rows) -> tuple:
+    """r_k mod 2 for every k, without a Glynn walk: over GF(2),
+    det(I + x A A^T) = sum_k x^k sum_{|R| = |C| = k} det(A[R, C])^2 (Cauchy-
+    Binet), and det = per mod 2, so its coefficients are the matching
+    counts' parities (the reference's certificate, perm_polynomial.py:
+    700-760, by another route: Hessenberg reduction of A A^T mod 2 by
+    similarity, then the characteristic-polynomial recurrence of a
+    Hessenberg matrix, polynomials as bit masks).  For a weighted matrix
+    pass the entries mod 2."""
+    n = len(rows)
+    masks = [sum(1 << j for j, v in enumerate(row) if v & 1) for row in rows]
+    H = [[bin(masks[i] & masks[j]).count("1") & 1 for j in range(n)] for i in range(n)]
+    for k in range(n - 2):
+        pivot = next((i for i in range(k + 1, n) if H[i][k]), None)
+        if pivot is None:
+            continue
+        if pivot != k + 1:
+            H[k + 1], H[pivot] = H[pivot], H[k + 1]
+            for row in H:
+                row[k + 1], row[pivot] = row[pivot], row[k + 1]
+        for r in range(k + 2, n):
+            if H[r][k]:
+                H[r] = [a ^ b for a, b in zip(H[r], H[k + 1])]
+                for row in H:
+                    row[k + 1] ^= row[r]
+    # p_m: characteristic polynomial of the leading m x m block (bit d is
+    # the coefficient of t^d); over GF(2) every sign is +.
+    p = [1]
+    for m in range(1, n + 1):
+        acc = (p[m - 1] << 1) ^ (p[m - 1] if H[m - 1][m - 1] else 0)
+        chain = 1
+        for i in range(m - 1, 0, -1):
+            chain &= H[i][i - 1]
+            if not chain:
+                break
+            if H[i - 1][m - 1]:
+                acc ^= p[i - 1]
+        p.append(acc)
+    char = p[n]
+    # det(I + x G) = x^n char(1/x): the coefficient of x^k is that of t^(n-k).
+    return tuple((char >> (n - k)) & 1 for k in range(n + 1))
+
+
 _WEIGHTED_ROOK_NATIVE_MAX_ENTRY = (1 << 16) - 1
 
 
@@ -450,11 +535,20 @@ def _rook_residue_rows(rows, primes, weighted: bool) -> list:
 
 
 def _rook_exact(rows, weighted: bool) -> tuple:
+    """Exact (weighted) matching counts: bounds, primes whose product
+    exceeds half the largest bound, CRT, and the parity certificate to pick
+    between the residue and its one possible lift (perm_polynomial.py:
+    1324-1409)."""
     n = len(rows)
     bounds = _rook_bounds(rows)
-    primes = _descending_primes(max(bounds), minimum=n)
-    values, _ = _crt_vectors(_rook_residue_rows(rows, primes, weighted), primes)
-    exact = tuple(values)
+    if not weighted:
+        bounds = tuple(min(a, b) for a, b in zip(bounds, _matching_bounds(rows)))
+    parities = _rook_parities(rows)
+    primes = _descending_primes(max(bounds) // 2, minimum=n)
+    values, modulus = _crt_vectors(_rook_residue_rows(rows, primes, weighted), primes)
+    # The modulus is odd and above half of every bound: a count is its
+    # residue or the residue plus the modulus, whichever has its parity.
+    exact = tuple(v if v % 2 == parity else v + modulus for v, parity in zip(values, parities))
     if not all(0 <= v <= b for v, b in zip(exact, bounds)):
         what = "weighted-rook" if weighted else "rook"
         raise RuntimeError(f"CRT reconstruction lies outside its certified {what} bound")

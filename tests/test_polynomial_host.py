@@ -253,3 +253,55 @@ def test_device_order_routing(poly):
     assert takes(40, (1 << 31) - 1) and not takes(41, (1 << 31) - 1)
     assert takes(40, (1 << 32) - 5) and not takes(41, (1 << 32) - 5)
     assert takes(32, (1 << 61) - 1) and not takes(33, (1 << 61) - 1)
+
+
+def _matching_sums(rows):
+    """Weighted k-matching sums by a row-by-row DP over used-column masks
+    (plain counts for a 0/1 matrix): independent of the bounds, the
+    GF(2) determinant and the device."""
+    from collections import defaultdict
+
+    n = len(rows)
+    states = defaultdict(int)
+    states[(0, 0)] = 1
+    for i in range(n):
+        nxt = defaultdict(int)
+        for (mask, k), c in states.items():
+            nxt[(mask, k)] += c
+            for j in range(n):
+                if rows[i][j] and not (mask >> j) & 1:
+                    nxt[(mask | 1 << j, k + 1)] += c * rows[i][j]
+        states = nxt
+    out = [0] * (n + 1)
+    for (_, k), c in states.items():
+        out[k] += c
+    return out
+
+
+def test_rook_parity_certificate_and_matching_bounds(poly):
+    """The GF(2) determinant gives every matching count's parity, and the
+    partial-matching (Bregman-Minc) bounds hold, on random boards of orders
+    1-9 (weighted boards through their entries mod 2); the closed forms of
+    J_n (C(n,k)^2 k!) and I_n (C(n,k)) at order 24."""
+    import math
+    import random
+
+    rng = random.Random(4401)
+    for _ in range(200):
+        n = rng.randint(1, 9)
+        density = rng.random()
+        A = tuple(tuple(int(rng.random() < density) for _ in range(n)) for _ in range(n))
+        counts = _matching_sums(A)
+        assert poly._rook_parities(A) == tuple(c % 2 for c in counts), A
+        assert all(b >= c for b, c in zip(poly._matching_bounds(A), counts)), A
+        W = tuple(tuple(rng.randrange(0, 5) for _ in range(n)) for _ in range(n))
+        assert poly._rook_parities(W) == tuple(c % 2 for c in _matching_sums(W)), W
+    n = 24
+    J = tuple((1,) * n for _ in range(n))
+    I = tuple(tuple(int(i == j) for j in range(n)) for i in range(n))
+    rJ = [math.comb(n, k) ** 2 * math.factorial(k) for k in range(n + 1)]
+    rI = [math.comb(n, k) for k in range(n + 1)]
+    assert poly._rook_parities(J) == tuple(v % 2 for v in rJ)
+    assert poly._rook_parities(I) == tuple(v % 2 for v in rI)
+    assert all(b >= v for b, v in zip(poly._matching_bounds(J), rJ))
+    assert all(b >= v for b, v in zip(poly._matching_bounds(I), rI))
