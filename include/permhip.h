@@ -236,13 +236,15 @@ int phip_batch_stats(void *plan, double *out, int len);
 int phip_batch_plan_classes(const int64_t *entries, const int32_t *orders, int count, const int64_t *moduli,
                             const int32_t *moduli_counts, int32_t *out, int len);
 
-/* The tables the block zero-skipping walk reads for one 0/1 matrix of
- * order 8..46 (row-major int64 entries), as a plan builds them: meta[0]
- * rows the block test reads, [1] rows of even degree (first in the table's
- * row order), [2] row-sum group, [3] n.  Writes the table words to out when
- * cap holds them and returns their count, or -1.  Host code only.  No
- * counterpart in the reference (table introspection for tests). */
-int64_t phip_block_tables(const int64_t *a, int n, int32_t *out, int64_t cap, int32_t *meta);
+/* The tables the block zero-skipping walk reads for one matrix of order
+ * 8..46 (row-major int64 entries), as a plan builds them: with mod3 == 0 a
+ * 0/1 matrix's, with mod3 != 0 the modulo-3 walk's for entries in
+ * {-1, 0, 1}.  meta[0] rows the block test reads, [1] rows of even degree
+ * (first in the table's row order), [2] row-sum group, [3] n.  Writes the
+ * table words to out when cap holds them and returns their count, or -1.
+ * Host code only.  No counterpart in the reference (table introspection
+ * for tests). */
+int64_t phip_block_tables(const int64_t *a, int n, int mod3, int32_t *out, int64_t cap, int32_t *meta);
 
 int phip_batch_destroy(void *plan);
 
@@ -277,6 +279,9 @@ int phip_batch_destroy(void *plan);
 #define PHIP_TUNE_SKIP_BLOCKS 13   /* 1 (default): tables on the zero-skipping kernels of order 8 and above
                                       walk in blocks of 32 Gray states, one mask lookup per row and block;
                                       0: one zero test per state.  Read when a plan is built */
+#define PHIP_TUNE_MOD3_BLOCKS 14   /* 1 (default): permanents modulo 3 of orders 8..46 run on the modulo-3
+                                      block walk (32 states per row lookup, a term is zero when a row sum
+                                      is a multiple of 3); 0: the bit-sliced kernel for every order */
 int phip_set_tuning(int key, int64_t value);
 int64_t phip_get_tuning(int key);
 

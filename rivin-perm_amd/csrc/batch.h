@@ -60,6 +60,8 @@ struct BatchJob {
     int32_t block_rows;  // kVariantSkipZeros: rows the 32-state block test reads (block_test_rows: a multiple
                          // of 8 or every row word; the rows of even degree lead the table), or -1 for the
                          // per-state test
+    int32_t block_mod3;  // kVariantSkipZeros with block_rows >= 0: 1 for the modulo-3 block walk (row sums that
+                         // are multiples of 3 zero a term; build_block_tables_mod3), else 0
 };
 
 // One requested residue: prime slot `k` of job `job`.
@@ -111,6 +113,10 @@ constexpr int kBlockMinOrder = 8;
 constexpr int block_region_words(int n, int rows) {
     return (rows + 1) * 32 + 12 * ((n + 3) / 4) + 64 + ((n + 3) / 4 + 2) * kBlockRing;
 }
+// The modulo-3 block walk's region: 64-word mask tables per tested row and
+// the inner tables; its rare nonzero states are taken in place, with no
+// ring.
+constexpr int block_region_words_mod3(int n, int rows) { return rows * 64 + 12 * ((n + 3) / 4); }
 // Dynamic LDS of one workgroup (4 wavefronts) of a class's kernel: the
 // zero-skip queues or block-walk regions, nothing for the other variants.
 inline unsigned class_lds_bytes(const BatchClass& cl) {
@@ -127,6 +133,9 @@ struct PlanInput {
     int n;
     uint32_t q;
     int key;
+    // The modulo-3 block walk: q == 3, entries in {-1, 0, 1}, order
+    // kBlockMinOrder..kBatchMaxOrder.
+    bool mod3 = false;
 };
 
 // Host image of a plan.

@@ -63,6 +63,14 @@ __device__ __forceinline__ void run_item(const int32_t* __restrict__ tables, con
     if constexpr (Z) {
         if constexpr (N >= kBlockMinOrder) {
             if (jb.block_rows >= 0) {
+                // Modulo-3 jobs have one modulus, padded to a set of 2.
+                if constexpr (K == 2) {
+                    if (jb.block_mod3) {
+                        glynn_chunk_set_blocks<N, G, K, true>(tables, N, jb.cbits, base, active, queue,
+                                                              jb.block_rows, reload, p, pinv, r);
+                        return;
+                    }
+                }
                 glynn_chunk_set_blocks<N, G, K>(tables, N, jb.cbits, base, active, queue, jb.block_rows, reload, p,
                                                 pinv, r);
                 return;

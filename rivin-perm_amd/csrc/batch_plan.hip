@@ -446,6 +446,7 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
         bool compact;    // groups of 2 rows whose partial bound products fit the compact exact classes
         int test_words;  // skip_zeros: the walk's zero-test width (skip_plan)
         int block_rows;  // skip_zeros: rows the block walk's test reads, or -1 (per-state test)
+        bool mod3;       // the modulo-3 block walk
         int bits;  // bound_bits of the table's row bounds
         std::vector<size_t> inputs;
     };
@@ -480,8 +481,12 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
             }
             pit = plain_of.emplace(pkey, Plain{m, SIZE_MAX}).first;
         }
+        if (x.mod3 && (x.q != 3 || pit->second.maxabs > 1 || x.n < kBlockMinOrder)) {
+            set_error("internal: a modulo-3 block walk needs entries in {-1, 0, 1}, modulus 3 and order >= 8");
+            return -1;
+        }
         const bool plain = pit->second.maxabs <= (uint64_t)(x.q - 1) / 2;
-        if (plain && pit->second.set != SIZE_MAX) {
+        if (plain && pit->second.set != SIZE_MAX && sets[pit->second.set].mod3 == x.mod3) {
             sets[pit->second.set].inputs.push_back(i);
             continue;
         }
@@ -493,12 +498,20 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
         // zero-skipping kernels, whose walk reads byte tables in the same
         // row order.
         const int permille = tuning().skip_zeros_permille;
-        const bool skip = permille > 0 && (group == 4 || group == 6) &&
-                          nonzero_share(x.a, x.n) * 1000.0 <= (double)permille;
+        const bool skip = x.mod3 || (permille > 0 && (group == 4 || group == 6) &&
+                                     nonzero_share(x.a, x.n) * 1000.0 <= (double)permille);
         // Those of order kBlockMinOrder and above walk in blocks of 32 states
         // (build_block_tables, block_plan's row order and group).
         int test_words = 0, block_rows = -1, even = -1;
-        if (skip && tuning().skip_blocks && x.n >= kBlockMinOrder) {
+        if (x.mod3) {
+            // Every row may sum to a multiple of 3: all of them are tested.
+            even = block_plan(x.a, x.n, order, group);
+            if (even < 0) {
+                set_error("internal: no row-sum group fits a modulo-3 table");
+                return -1;
+            }
+            block_rows = block_test_rows(x.n, x.n);
+        } else if (skip && tuning().skip_blocks && x.n >= kBlockMinOrder) {
             std::vector<int> block_order;
             int block_group = group;
             even = block_plan(x.a, x.n, block_order, block_group);
@@ -509,14 +522,18 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
             }
         }
         size_t words;
-        if (block_rows >= 0) {
+        if (x.mod3) {
+            words = block_table_words_mod3(x.n, block_rows);
+        } else if (block_rows >= 0) {
             words = block_table_words(x.n, block_rows);
         } else {
             words = (size_t)x.n * (skip ? byte_stride(x.n) : glynn_stride(x.n, group));
             test_words = skip ? skip_plan(x.a, x.n, group, order) : 0;
         }
         table.assign(words, 0);
-        if (block_rows >= 0) {
+        if (x.mod3) {
+            build_block_tables_mod3(x.a, x.n, order, block_rows, table.data());
+        } else if (block_rows >= 0) {
             build_block_tables(x.a, x.n, order, even, block_rows, table.data());
         } else if (skip) {
             build_byte_tables(x.a, x.n, order, table.data());
@@ -528,7 +545,7 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
             for (size_t si : sets_of_key[x.key]) {
                 const Set& st = sets[si];
                 if (st.n == x.n && st.group == group && st.skip_zeros == skip && st.test_words == test_words &&
-                    st.block_rows == block_rows &&
+                    st.block_rows == block_rows && st.mod3 == x.mod3 &&
                     memcmp(out.blob.data() + st.data_off, table.data(), words * sizeof(int32_t)) == 0) {
                     found = si;
                     break;
@@ -546,7 +563,7 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
                 }
             }
             sets.push_back({x.key, x.n, group, (uint64_t)out.blob.size(), skip, compact, test_words, block_rows,
-                            group > 0 ? bound_bits(bounds) : 0, {}});
+                            x.mod3, group > 0 ? bound_bits(bounds) : 0, {}});
             sets_of_key[x.key].push_back(found);
             out.blob.insert(out.blob.end(), table.begin(), table.end());
         }
@@ -628,10 +645,12 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
             }
             next += take;
             jb.block_rows = -1;
+            jb.block_mod3 = 0;
             if (st.skip_zeros) {
                 jb.variant = kVariantSkipZeros;
                 jb.test_words = st.test_words;
                 jb.block_rows = st.block_rows;
+                jb.block_mod3 = st.mod3 ? 1 : 0;
             } else if (st.group > 0 && pairs_states(K, (const uint32_t*)out.set_p.data() + jb.prime_off)) {
                 jb.variant = kVariantPaired;
             } else {
@@ -710,7 +729,9 @@ int assign_items(PlanHost& out, int items_log2, int min_cbits) {
             BatchJob& b = out.jobs[j];
             cl.max_n = std::max(cl.max_n, b.n);
             if (b.variant == kVariantSkipZeros && b.block_rows >= 0)
-                cl.max_block_words = std::max(cl.max_block_words, block_region_words(b.n, b.block_rows));
+                cl.max_block_words =
+                    std::max(cl.max_block_words, b.block_mod3 ? block_region_words_mod3(b.n, b.block_rows)
+                                                              : block_region_words(b.n, b.block_rows));
             const uint64_t waves = (b.lanes + 63) / 64;
             if (out.items.size() + waves >= 0xffffffffull) {
                 set_error("batch has too many work items");
@@ -1195,7 +1216,7 @@ int phip_batch_plan_classes(const int64_t* entries, const int32_t* orders, int c
     return (int)h.slots.size();
 }
 
-int64_t phip_block_tables(const int64_t* a, int n, int32_t* out, int64_t cap, int32_t* meta) {
+int64_t phip_block_tables(const int64_t* a, int n, int mod3, int32_t* out, int64_t cap, int32_t* meta) {
     if (!a || !meta || (!out && cap > 0)) {
         phip::set_error("null argument");
         return -1;
@@ -1205,24 +1226,30 @@ int64_t phip_block_tables(const int64_t* a, int n, int32_t* out, int64_t cap, in
         return -1;
     }
     for (size_t e = 0; e < (size_t)n * n; ++e)
-        if (a[e] != 0 && a[e] != 1) {
-            phip::set_error("block tables take 0/1 matrices");
+        if (a[e] != 0 && a[e] != 1 && !(mod3 && a[e] == -1)) {
+            phip::set_error(mod3 ? "modulo-3 block tables take entries -1, 0 and 1" : "block tables take 0/1 matrices");
             return -1;
         }
     std::vector<int> order;
     int group = 0;
     const int even = phip::block_plan(a, n, order, group);
     if (even < 0) {
-        phip::set_error("no row-sum group fits the doubled row sums");
+        phip::set_error("no row-sum group fits the row sums");
         return -1;
     }
-    const int rows = phip::block_test_rows(n, even);
-    const size_t words = phip::block_table_words(n, rows);
+    const int rows = mod3 ? phip::block_test_rows(n, n) : phip::block_test_rows(n, even);
+    const size_t words = mod3 ? phip::block_table_words_mod3(n, rows) : phip::block_table_words(n, rows);
     meta[0] = rows;
     meta[1] = even;
     meta[2] = group;
     meta[3] = n;
-    if ((int64_t)words <= cap) phip::build_block_tables(a, n, order, even, rows, out);
+    if ((int64_t)words <= cap) {
+        if (mod3) {
+            phip::build_block_tables_mod3(a, n, order, rows, out);
+        } else {
+            phip::build_block_tables(a, n, order, even, rows, out);
+        }
+    }
     return (int64_t)words;
 }
 

@@ -13,6 +13,7 @@
 #include "necklace.h"
 #include "affine.h"
 #include "residues.h"
+#include "batch.h"
 
 namespace phip {
 int mod3_from_columns(DeviceContext* ctx, const uint64_t* mags, const uint64_t* sgns, int n);
@@ -157,6 +158,10 @@ int phip_set_tuning(int key, int64_t value) {
             if (value < 0 || value > 1) break;
             t.skip_blocks = (int)value;
             return 0;
+        case PHIP_TUNE_MOD3_BLOCKS:
+            if (value < 0 || value > 1) break;
+            t.mod3_blocks = (int)value;
+            return 0;
         default:
             phip::set_error("unknown tuning key");
             return -1;
@@ -181,6 +186,7 @@ int64_t phip_get_tuning(int key) {
         case PHIP_TUNE_EXACT_WIDE_MIN_PRIMES: return t.exact_wide_min_primes;
         case PHIP_TUNE_EXACT_COMPACT: return t.exact_compact;
         case PHIP_TUNE_SKIP_BLOCKS: return t.skip_blocks;
+        case PHIP_TUNE_MOD3_BLOCKS: return t.mod3_blocks;
         default:
             phip::set_error("unknown tuning key");
             return -1;
@@ -312,6 +318,21 @@ int phip_permanent_mod3(const uint64_t* mags, const uint64_t* sgns, int n) {
     DeviceContext* ctx = DeviceContext::current();
     if (!ctx) return -1;
     std::lock_guard<std::mutex> lock(ctx->mutex());
+    if (phip::tuning().mod3_blocks && n >= phip::kBlockMinOrder && n <= phip::kBatchMaxOrder && mags && sgns) {
+        // The modulo-3 block walk: the residues of the entries as -1, 0, 1
+        // and the persistent engine's block walk with modulus 3 (row sums
+        // that are multiples of 3 zero a term, so almost every state is
+        // skipped 32 at a time).
+        std::vector<int64_t> a((size_t)n * n, 0);
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                if ((mags[j] >> i) & 1u) a[(size_t)i * n + j] = ((sgns[j] >> i) & 1u) ? -1 : 1;
+        phip::PlanInput in{a.data(), n, 3u, 0};
+        in.mod3 = true;
+        int64_t r = -1;
+        if (phip::run_plan_in_context(ctx, {in}, &r) != 0) return -1;
+        return (int)r;
+    }
     return phip::mod3_from_columns(ctx, mags, sgns, n);
 }
 

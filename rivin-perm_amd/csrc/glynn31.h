@@ -964,7 +964,10 @@ __device__ __forceinline__ uint32_t kth_set_bit(uint32_t m, uint32_t k) {
     return pos;
 }
 
-template <int NR, int G, int K>
+// kMod3: the modulo-3 walk (build_block_tables_mod3): 64-word mask tables
+// per row, and the nonzero states, which are rare there (every row must
+// avoid a multiple of 3), taken in place by the lanes that hold them.
+template <int NR, int G, int K, bool kMod3 = false>
 struct BlockSkipWalk {
     static_assert(G > 0, "prime sets need exact row sums");
     using Rows = ByteRows<NR>;
@@ -980,7 +983,7 @@ struct BlockSkipWalk {
     lds_u32* inner;       // 12 W words: row word k's two inner tables at 12 k (build_block_tables)
     lds_u32* owner;       // 64 words: flags of the lanes where an entry starts
     lds_u32* ring;        // (W + 2) * kRing words
-    uint32_t mask_base;   // LDS byte address of row 0's mask table (row i at + 128 i)
+    uint32_t mask_base;   // LDS byte address of row 0's mask table (row i at + 128 i; + 256 i modulo 3)
     int test_words;       // rows tested / 4 (wave-uniform)
     uint32_t ehead, etail, thead, ttail;  // entries and states queued / taken (wave-uniform)
     uint32_t lane;
@@ -993,7 +996,33 @@ struct BlockSkipWalk {
     template <int B>
     __device__ __forceinline__ uint32_t lookup(uint32_t w, int row) const {
         const uint32_t addr = ((w >> (8 * B)) & 0xffu) + mask_base;
-        return ((const lds_u32*)(uintptr_t)addr)[row * 32];
+        return ((const lds_u32*)(uintptr_t)addr)[row * (kMod3 ? 64 : 32)];
+    }
+
+    // Adds the term of one state from its signed row-sum bytes w and its
+    // sign (1: negative): the group products, the pair products and each
+    // prime's Montgomery tree, as ZeroSkipSink does.
+    __device__ __forceinline__ void add_term(const uint32_t (&w)[W], uint32_t sign) {
+        int32_t g[kGroups];
+        const Groups q{w};
+        q.groups(g, std::make_integer_sequence<int, kGroups>{});
+        if (sign) g[0] = -g[0];
+        int64_t P[kPairs > 0 ? kPairs : 1];
+#pragma unroll
+        for (int j = 0; j < kPairs; ++j) P[j] = (int64_t)g[2 * j] * (int64_t)g[2 * j + 1];
+#pragma unroll
+        for (int p = 0; p < K; ++p) {
+            int32_t v[kLeaves];
+#pragma unroll
+            for (int j = 0; j < kPairs; ++j) v[j] = redc31(P[j], negp[p], pinv[p]);
+            if constexpr (kGroups & 1) v[kLeaves - 1] = g[kGroups - 1];
+#pragma unroll
+            for (int wd = 1; wd < kLeaves; wd *= 2) {
+#pragma unroll
+                for (int i = 0; i + wd < kLeaves; i += 2 * wd) v[i] = mont31(v[i], v[i + wd], negp[p], pinv[p]);
+            }
+            acc[p] = mac_sign<1>(acc[p], v[0]);
+        }
     }
 
     // Bit y set: some tested row is zero at inner state y.  Rows are tested
@@ -1067,28 +1096,7 @@ struct BlockSkipWalk {
             w[kk] = (o + inner[kk * 12 + ya] + inner[kk * 12 + yb]) ^ 0x80808080u;
         }
         sign ^= (uint32_t)__popc(y) & 1u;
-        if (lane < cnt) {
-            int32_t g[kGroups];
-            const Groups q{w};
-            q.groups(g, std::make_integer_sequence<int, kGroups>{});
-            if (sign) g[0] = -g[0];
-            int64_t P[kPairs > 0 ? kPairs : 1];
-#pragma unroll
-            for (int j = 0; j < kPairs; ++j) P[j] = (int64_t)g[2 * j] * (int64_t)g[2 * j + 1];
-#pragma unroll
-            for (int p = 0; p < K; ++p) {
-                int32_t v[kLeaves];
-#pragma unroll
-                for (int j = 0; j < kPairs; ++j) v[j] = redc31(P[j], negp[p], pinv[p]);
-                if constexpr (kGroups & 1) v[kLeaves - 1] = g[kGroups - 1];
-#pragma unroll
-                for (int wd = 1; wd < kLeaves; wd *= 2) {
-#pragma unroll
-                    for (int i = 0; i + wd < kLeaves; i += 2 * wd) v[i] = mont31(v[i], v[i + wd], negp[p], pinv[p]);
-                }
-                acc[p] = mac_sign<1>(acc[p], v[0]);
-            }
-        }
+        if (lane < cnt) add_term(w, sign);
         if (cnt == 64u) {
             // The next state to take lies in lane 63's entry unless lane 63
             // took that entry's last state.
@@ -1098,9 +1106,26 @@ struct BlockSkipWalk {
         ttail += cnt;
     }
 
-    // One outer state: tests its 32 inner states and queues the survivors.
+    // One outer state: tests its 32 inner states and queues the survivors
+    // (modulo 3: takes them in place).
     __device__ __forceinline__ void block(const Rows& rs, uint32_t par) {
         const uint32_t surv = ~zero_mask(rs);
+        if constexpr (kMod3) {
+            uint32_t left = active ? surv : 0u;
+            while (__ballot(left != 0u)) {
+                if (left != 0u) {
+                    const uint32_t y = (uint32_t)__builtin_ctz(left);
+                    left &= left - 1u;
+                    const uint32_t ya = y & 3u, yb = 4u + (y >> 2);
+                    uint32_t w[W];
+#pragma unroll
+                    for (int kk = 0; kk < W; ++kk)
+                        w[kk] = ((rs.r[kk] >> 1) + inner[kk * 12 + ya] + inner[kk * 12 + yb]) ^ 0x80808080u;
+                    add_term(w, par ^ ((uint32_t)__popc(y) & 1u));
+                }
+            }
+            return;
+        }
         const bool has = active && surv != 0u;
         const uint64_t bal = __ballot(has);
         if (bal == 0) return;
@@ -1132,12 +1157,12 @@ struct BlockSkipWalk {
 // words); with `reload` the mask and inner tables are copied into it first
 // (a wavefront keeps them while it stays on one job).  Every lane of the
 // wavefront must call it.
-template <int NR, int G, int K>
+template <int NR, int G, int K, bool kMod3 = false>
 __device__ __forceinline__ void glynn_chunk_set_blocks(const int32_t* __restrict__ tables, int n, int cbits,
                                                        uint64_t base, bool active, int32_t* region, int rows,
                                                        bool reload, const int32_t (&primes)[K],
                                                        const uint32_t (&pinvs)[K], int32_t (&out)[K]) {
-    using Walk = BlockSkipWalk<NR, G, K>;
+    using Walk = BlockSkipWalk<NR, G, K, kMod3>;
     constexpr int W = Walk::W;
     Walk walk;
 #pragma unroll
@@ -1148,7 +1173,10 @@ __device__ __forceinline__ void glynn_chunk_set_blocks(const int32_t* __restrict
     }
     const uint32_t lane = __lane_id();
     lds_u32* const lds = (lds_u32*)region;
-    walk.inner = lds + (rows + 1) * 32;
+    // Mask tables (rows + 1 overlapping tables of 32 words, or rows of 64
+    // modulo 3), then the inner tables, the owner flags and the ring.
+    const int mask_words = kMod3 ? rows * 64 : (rows + 1) * 32;
+    walk.inner = lds + mask_words;
     walk.owner = walk.inner + 12 * W;
     walk.ring = walk.owner + 64;
     walk.mask_base = (uint32_t)(uintptr_t)lds;
@@ -1161,7 +1189,7 @@ __device__ __forceinline__ void glynn_chunk_set_blocks(const int32_t* __restrict
         const int32_t* __restrict__ src = tables + (size_t)n * W;
         for (int i = (int)lane; i < 12 * W; i += 64) walk.inner[i] = (uint32_t)src[i];
         src += 12 * W;
-        for (int i = (int)lane; i < (rows + 1) * 32; i += 64) lds[i] = (uint32_t)src[i];
+        for (int i = (int)lane; i < mask_words; i += 64) lds[i] = (uint32_t)src[i];
     }
     __builtin_amdgcn_wave_barrier();
 
@@ -1193,8 +1221,10 @@ __device__ __forceinline__ void glynn_chunk_set_blocks(const int32_t* __restrict
         par ^= 1u;
         walk.block(rs, par);
     }
-    const uint32_t pend = walk.thead - walk.ttail;
-    if (pend) walk.consume(pend);
+    if constexpr (!kMod3) {
+        const uint32_t pend = walk.thead - walk.ttail;
+        if (pend) walk.consume(pend);
+    }
 #pragma unroll
     for (int k = 0; k < K; ++k) out[k] = lane_value(walk.acc[k], false, walk.negp[k], walk.pinv[k]);
 }

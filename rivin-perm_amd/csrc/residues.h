@@ -28,6 +28,7 @@ struct Tuning {
     int exact_wide_min_primes = 6;  // the same for 9..16 groups (row-bound product below 2^372)
     int exact_compact = 1;          // groups of 2 rows may take the compact exact classes (0 off)
     int skip_blocks = 1;            // zero-skipping tables test 32 Gray states per row lookup (0: per state)
+    int mod3_blocks = 1;            // permanents modulo 3 of orders 8..46 on the modulo-3 block walk (0: bit-sliced)
 };
 Tuning& tuning();
 

@@ -294,4 +294,72 @@ inline void build_block_tables(const int64_t* M, int n, const std::vector<int>& 
     for (size_t k = 0; k < words.size(); ++k) out[k] = (int32_t)words[k];
 }
 
+
+// Words of a modulo-3 block-walk table (build_block_tables_mod3).
+inline size_t block_table_words_mod3(int n, int rows) {
+    const size_t W = (size_t)byte_stride(n);
+    return (size_t)n * W + 12 * W + (size_t)rows * 64;
+}
+
+// Tables of the modulo-3 block walk for a matrix with entries in {-1, 0, 1}
+// (the residues of a matrix modulo 3, 2 taken as -1), rows in kernel order
+// order[i].  Row sums s are kept as in build_block_tables: the byte
+// 2 s + beta_i with beta_i = 128 or 126 by the parity of the row's number
+// of nonzero entries d_i (s = d_i modulo 2), flip deltas 4 a_ij as signed
+// bytes of one 32-bit word (a word sum or difference is exact because every
+// resulting byte lies in [36, 220] for d_i <= 46), and the inner tables of
+// the same form.  A row sum that is a multiple of 3 zeroes the term modulo
+// 3, so every row is tested: the masks are 64 words per row (no overlap),
+// entry e of row i (i < rows, i < n) having bit y set when the row sum at
+// inner state y, O - 2 (sum of the row's entries in the inner columns y
+// selects) for the outer sum O = (4 e - beta_i) / 2, is a multiple of 3.
+inline void build_block_tables_mod3(const int64_t* M, int n, const std::vector<int>& order, int rows, int32_t* out) {
+    const int W = byte_stride(n);
+    std::vector<int64_t> words(block_table_words_mod3(n, rows), 0);
+    auto put = [&](size_t word, int row, int64_t byte_value) { words[word] += byte_value << (8 * (row % 4)); };
+    auto entry = [&](int i, int j) { return M[(size_t)order[i] * n + j]; };
+    const size_t inner_at = (size_t)n * W, masks_at = inner_at + 12 * (size_t)W;
+    std::vector<int64_t> correction(W, 0), low((size_t)W * 4, 0), high((size_t)W * 8, 0);
+    for (int i = 0; i < 4 * W; ++i) {
+        if (i >= n) {
+            put((size_t)i / 4, i, 132);
+            continue;
+        }
+        int64_t sum = 0, degree = 0;
+        for (int j = 0; j < n; ++j) {
+            sum += entry(i, j);
+            degree += entry(i, j) != 0 ? 1 : 0;
+        }
+        const int64_t beta = degree % 2 == 0 ? 128 : 126;
+        put((size_t)i / 4, i, 2 * sum + beta);
+        for (int j = 0; j < n - 1; ++j) put((size_t)(1 + j) * W + i / 4, i, 4 * entry(i, j));
+        correction[i / 4] += (128 - beta) << (8 * (i % 4));
+        for (int a = 0; a < 4; ++a)
+            low[(size_t)(i / 4) * 4 + a] += (4 * ((a & 1 ? entry(i, 0) : 0) + (a & 2 ? entry(i, 1) : 0))) << (8 * (i % 4));
+        for (int b = 0; b < 8; ++b)
+            high[(size_t)(i / 4) * 8 + b] +=
+                (4 * ((b & 1 ? entry(i, 2) : 0) + (b & 2 ? entry(i, 3) : 0) + (b & 4 ? entry(i, 4) : 0)))
+                << (8 * (i % 4));
+        if (i < rows) {
+            for (int e = 0; e < 64; ++e) {
+                const int64_t outer = (4 * e - beta) / 2;
+                uint32_t mask = 0;
+                for (int y = 0; y < 32; ++y) {
+                    int64_t inner = 0;
+                    for (int j = 0; j < kBlockInnerBits; ++j)
+                        if ((y >> j) & 1) inner += entry(i, j);
+                    if ((outer - 2 * inner) % 3 == 0) mask |= 1u << y;
+                }
+                words[masks_at + (size_t)i * 64 + e] = mask;
+            }
+        }
+    }
+    for (int k = 0; k < W; ++k) {
+        for (int a = 0; a < 4; ++a) words[inner_at + (size_t)k * 12 + a] = -(low[(size_t)k * 4 + a] / 2);
+        for (int b = 0; b < 8; ++b)
+            words[inner_at + (size_t)k * 12 + 4 + b] = -((high[(size_t)k * 8 + b] - correction[k]) / 2 - 0x40404040);
+    }
+    for (size_t k = 0; k < words.size(); ++k) out[k] = (int32_t)(uint32_t)(uint64_t)words[k];
+}
+
 }  // namespace phip

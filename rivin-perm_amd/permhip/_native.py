@@ -89,7 +89,7 @@ _SIGNATURES = {
     "phip_batch_set_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64]),
     "phip_batch_stats": (ctypes.c_int, [ctypes.c_void_p, _F64P, ctypes.c_int]),
     "phip_batch_plan_classes": (ctypes.c_int, [_I64P, _I32P, ctypes.c_int, _I64P, _I32P, _I32P, ctypes.c_int]),
-    "phip_block_tables": (ctypes.c_int64, [_I64P, ctypes.c_int, _I32P, ctypes.c_int64, _I32P]),
+    "phip_block_tables": (ctypes.c_int64, [_I64P, ctypes.c_int, ctypes.c_int, _I32P, ctypes.c_int64, _I32P]),
     "phip_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "phip_set_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64]),
     "phip_get_tuning": (ctypes.c_int64, [ctypes.c_int]),
@@ -115,6 +115,7 @@ TUNE_EXACT_MIN_PRIMES = 10
 TUNE_EXACT_WIDE_MIN_PRIMES = 11
 TUNE_EXACT_COMPACT = 12
 TUNE_SKIP_BLOCKS = 13
+TUNE_MOD3_BLOCKS = 14
 BATCH_BLOCKS_PER_CU = 1
 BATCH_USE_GRAPH = 2
 BATCH_ITEMS_LOG2 = 3
@@ -476,18 +477,19 @@ def batch_plan_classes(matrices, moduli_lists) -> list:
     return [(int(c) // 256, (int(c) // 8) % 32, int(c) % 8) for c in out[:count]]
 
 
-def block_tables(matrix: np.ndarray):
-    """The block zero-skipping walk's tables for one 0/1 matrix (host
-    code, no device): (uint32 words, rows tested, rows of even degree,
-    row-sum group)."""
+def block_tables(matrix: np.ndarray, mod3: bool = False):
+    """The block zero-skipping walk's tables for one 0/1 matrix, or with
+    mod3 the modulo-3 walk's for entries in {-1, 0, 1} (host code, no
+    device): (uint32 words, rows tested, rows of even degree, row-sum
+    group)."""
     lib = load()
     a = np.ascontiguousarray(matrix, dtype=np.int64)
     n = int(a.shape[0])
     meta = np.zeros(4, dtype=np.int32)
-    words = lib.phip_block_tables(a.ctypes.data_as(_I64P), n, None, 0, meta.ctypes.data_as(_I32P))
+    words = lib.phip_block_tables(a.ctypes.data_as(_I64P), n, int(mod3), None, 0, meta.ctypes.data_as(_I32P))
     _check(words, "phip_block_tables")
     out = np.zeros(int(words), dtype=np.int32)
-    got = lib.phip_block_tables(a.ctypes.data_as(_I64P), n, out.ctypes.data_as(_I32P), int(words),
+    got = lib.phip_block_tables(a.ctypes.data_as(_I64P), n, int(mod3), out.ctypes.data_as(_I32P), int(words),
                                 meta.ctypes.data_as(_I32P))
     _check(got, "phip_block_tables")
     return out.view(np.uint32), int(meta[0]), int(meta[1]), int(meta[2])

@@ -27,10 +27,10 @@ def _signed_byte(v):
 
 
 class _Tables:
-    def __init__(self, A):
+    def __init__(self, A, mod3=False):
         from permhip import _native
 
-        words, self.rows, self.even, self.group = _native.block_tables(A)
+        words, self.rows, self.even, self.group = _native.block_tables(A, mod3)
         self.n = n = A.shape[0]
         self.W = W = (n + 3) // 4
         w = [int(x) for x in words]
@@ -40,7 +40,7 @@ class _Tables:
         self.inner = [[(inner[k * 12 + (y & 3)] + inner[k * 12 + 4 + (y >> 2)]) & MASK32 for y in range(32)]
                       for k in range(W)]
         self.masks = w[n * W + 12 * W:]
-        assert len(self.masks) == (self.rows + 1) * 32
+        assert len(self.masks) == (self.rows * 64 if mod3 else (self.rows + 1) * 32)
 
 
 def _walk_item(T, cbits, first_lane, lanes_total):
@@ -205,3 +205,58 @@ def test_block_tables_definition_small():
     rng = np.random.default_rng(3)
     A = _random_binary(rng, 8, 0.7)
     assert _model_permanent(A, 7) == oracle.definition_permanent(A.tolist())
+
+
+def _mod3_permanent(A):
+    """perm(A) mod 3 through the library's modulo-3 block tables and a lane
+    model of that walk: masks of 64 words per row, the nonzero states (no
+    tested row a multiple of 3) taken in place."""
+    B = np.where(A % 3 == 2, -1, A % 3).astype(np.int64)
+    T = _Tables(B, mod3=True)
+    n, W = T.n, T.W
+    cbits = max(6, n - 1 - 6)
+    lanes_total = 1 << (n - 1 - cbits)
+    total = 0
+    for lane in range(lanes_total):
+        base = lane << cbits
+        gray = base ^ (base >> 1)
+        outer = list(T.state0)
+        for j in range(cbits - 1, n - 1):
+            if (gray >> j) & 1:
+                outer = [(outer[k] - T.deltas[j][k]) & MASK32 for k in range(W)]
+        par = bin(gray).count("1") & 1
+        ub = base >> 5
+        for u in range(1 << (cbits - 5)):
+            if u:
+                jj = (u & -u).bit_length() - 1
+                col = T.deltas[5 + jj]
+                subtract = not (((ub + u) >> (jj + 1)) & 1)
+                outer = [((outer[k] - col[k]) if subtract else (outer[k] + col[k])) & MASK32 for k in range(W)]
+                par ^= 1
+            m = 0
+            for i in range(T.rows):
+                b = _byte(outer[i // 4], i % 4)
+                assert b % 4 == 0
+                m |= T.masks[64 * i + b // 4]
+            surv = ~m & MASK32
+            while surv:
+                y = (surv & -surv).bit_length() - 1
+                surv &= surv - 1
+                signed = [(((outer[k] >> 1) + T.inner[k][y]) & MASK32) ^ 0x80808080 for k in range(W)]
+                vals = [_signed_byte(_byte(signed[i // 4], i % 4)) for i in range(n)]
+                assert all(v % 3 for v in vals), "a taken state has a row sum that is a multiple of 3"
+                term = math.prod(vals)
+                total += -term if (par ^ (bin(y).count("1") & 1)) else term
+    # Glynn's 2^(n-1) is (-1)^(n-1) modulo 3.
+    return (total * (-1) ** (n - 1)) % 3
+
+
+@pytest.mark.parametrize("n", [8, 9, 11, 12])
+def test_mod3_block_tables_model_matches_oracle(oracle_c, n):
+    """The modulo-3 tables (entries -1, 0, 1; every row tested) give perm
+    mod 3 of random integer matrices, as the oracle's bipedal Ryser does."""
+    rng = np.random.default_rng(950 + n)
+    for _ in range(3):
+        A = rng.integers(-4, 5, (n, n)).astype(np.int64)
+        want = oracle_c.mod3(np.where(A % 3 < 0, A % 3 + 3, A % 3).astype(np.int64))
+        assert _mod3_permanent(A) == want, (n, A.tolist())

@@ -363,3 +363,65 @@ def test_block_walk_large_closed_forms(native, n):
     exact = math.prod(math.factorial(k) for k in sizes)
     print(f"block closed form n={n} sizes={sizes}", flush=True)
     assert native.permanent_residues(A, primes) == [exact % p for p in primes]
+
+
+def _with_mod3_blocks(native, on, fn):
+    old = native.get_tuning(native.TUNE_MOD3_BLOCKS)
+    native.set_tuning(native.TUNE_MOD3_BLOCKS, 1 if on else 0)
+    try:
+        return fn()
+    finally:
+        native.set_tuning(native.TUNE_MOD3_BLOCKS, old)
+
+
+@pytest.mark.parametrize("n", [8, 9, 13, 17, 22, 27, 31, 36])
+def test_mod3_block_walk_matches_bit_sliced_and_oracle(native, oracle_c, n):
+    """Permanents modulo 3 on the modulo-3 block walk (orders 8..46) equal
+    the bit-sliced kernel's and, up to order 27, the oracle's: integer
+    matrices, 0/1 matrices, matrices of -1/0/1 entries heavy in +-1, and a
+    matrix with a row of multiples of 3."""
+    rng = np.random.default_rng(600 + n)
+    cases = [rng.integers(-7, 8, (n, n)).astype(np.int64),
+             (rng.random((n, n)) < 0.5).astype(np.int64),
+             rng.choice([-1, 1, 1, 0], size=(n, n)).astype(np.int64)]
+    Z = rng.integers(-4, 5, (n, n)).astype(np.int64)
+    Z[n // 2] = 3 * rng.integers(-2, 3, n)
+    cases.append(Z)
+    for idx, A in enumerate(cases):
+        print(f"mod3 blocks n={n} case={idx}", flush=True)
+        blocks = native.permanent_mod3(A % 3)
+        sliced = _with_mod3_blocks(native, False, lambda: native.permanent_mod3(A % 3))
+        assert blocks == sliced, (n, idx)
+        if n <= 27:
+            assert blocks == oracle_c.mod3(A, 8), (n, idx)
+        if idx == 3:
+            assert blocks == 0
+
+
+@pytest.mark.parametrize("n", [40, 44])
+def test_mod3_block_walk_large_orders(native, n):
+    """Orders 40 and 44 (2^39 and 2^43 Gray states): the modulo-3 block walk
+    against the bit-sliced kernel, and a block-diagonal matrix against the
+    product of its blocks' permanents modulo 3."""
+    rng = np.random.default_rng(n)
+    A = rng.integers(-2, 3, (n, n)).astype(np.int64)
+    print(f"mod3 blocks large n={n}", flush=True)
+    blocks = native.permanent_mod3(A % 3)
+    if n == 40:
+        sliced = _with_mod3_blocks(native, False, lambda: native.permanent_mod3(A % 3))
+        assert blocks == sliced
+    sizes = [5] * (n // 5)
+    sizes[-1] += n - sum(sizes)
+    D = np.zeros((n, n), dtype=np.int64)
+    at = 0
+    want = 1
+    import itertools
+
+    for k in sizes:
+        blk = rng.integers(-1, 2, (k, k)).astype(np.int64)
+        D[at:at + k, at:at + k] = blk
+        want = want * sum(math.prod(int(blk[i, s[i]]) for i in range(k)) for s in itertools.permutations(range(k)))
+        at += k
+    perm_r = rng.permutation(n)
+    D = D[perm_r][:, rng.permutation(n)]
+    assert native.permanent_mod3(D % 3) == want % 3
