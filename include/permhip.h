@@ -237,9 +237,10 @@ int phip_batch_plan_classes(const int64_t *entries, const int32_t *orders, int c
                             const int32_t *moduli_counts, int32_t *out, int len);
 
 /* The tables the block zero-skipping walk reads for one matrix of order
- * 8..46 (row-major int64 entries), as a plan builds them: with mod3 == 0 a
- * 0/1 matrix's, with mod3 != 0 the modulo-3 walk's for entries in
- * {-1, 0, 1}.  meta[0] rows the block test reads, [1] rows of even degree
+ * 8..46 (row-major int64 entries), as a plan builds them for the matrix as
+ * given (a plan walks a 0/1 matrix's transpose instead when its estimated
+ * share of nonzero terms is smaller): with mod3 == 0 a 0/1 matrix's, with
+ * mod3 != 0 the modulo-3 walk's for entries in {-1, 0, 1}.  meta[0] rows the block test reads, [1] rows of even degree
  * (first in the table's row order), [2] row-sum group, [3] n.  Writes the
  * table words to out when cap holds them and returns their count, or -1.
  * Host code only.  No counterpart in the reference (table introspection

@@ -490,8 +490,29 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
             sets[pit->second.set].inputs.push_back(i);
             continue;
         }
+        // A 0/1 matrix walks in the orientation (it or its transpose, the
+        // same permanent) with the smaller estimated share of nonzero terms:
+        // only lines of even degree can sum to zero, and the counts of even
+        // rows and of even columns differ by chance.  The taken states are
+        // most of a 0/1 walk's work (about 12% fewer at the served orders
+        // 35-38).
+        const int64_t* a = x.a;
+        std::vector<int64_t> transposed;
+        double share = nonzero_share(x.a, x.n);
+        bool binary = !x.mod3;
+        for (size_t e = 0; binary && e < (size_t)x.n * x.n; ++e) binary = x.a[e] == 0 || x.a[e] == 1;
+        if (binary) {
+            transposed.resize((size_t)x.n * x.n);
+            for (int r = 0; r < x.n; ++r)
+                for (int c = 0; c < x.n; ++c) transposed[(size_t)c * x.n + r] = x.a[(size_t)r * x.n + c];
+            const double share_t = nonzero_share(transposed.data(), x.n);
+            if (share_t < share) {
+                a = transposed.data();
+                share = share_t;
+            }
+        }
         std::vector<int> order;
-        const std::vector<uint64_t> bounds = centred_row_bounds(x.a, x.n, (int64_t)x.q);
+        const std::vector<uint64_t> bounds = centred_row_bounds(a, x.n, (int64_t)x.q);
         int group = plan_rows(bounds, order);
         // Tables of 0/1 matrices (nonzero_share is 1 for any other entry) in
         // groups of 4 or 6 with a small share of nonzero terms run on the
@@ -499,13 +520,13 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
         // row order.
         const int permille = tuning().skip_zeros_permille;
         const bool skip = x.mod3 || (permille > 0 && (group == 4 || group == 6) &&
-                                     nonzero_share(x.a, x.n) * 1000.0 <= (double)permille);
+                                     share * 1000.0 <= (double)permille);
         // Those of order kBlockMinOrder and above walk in blocks of 32 states
         // (build_block_tables, block_plan's row order and group).
         int test_words = 0, block_rows = -1, even = -1;
         if (x.mod3) {
             // Every row may sum to a multiple of 3: all of them are tested.
-            even = block_plan(x.a, x.n, order, group);
+            even = block_plan(a, x.n, order, group);
             if (even < 0) {
                 set_error("internal: no row-sum group fits a modulo-3 table");
                 return -1;
@@ -514,7 +535,7 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
         } else if (skip && tuning().skip_blocks && x.n >= kBlockMinOrder) {
             std::vector<int> block_order;
             int block_group = group;
-            even = block_plan(x.a, x.n, block_order, block_group);
+            even = block_plan(a, x.n, block_order, block_group);
             if (even >= 0) {
                 order = block_order;
                 group = block_group;
@@ -528,17 +549,17 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
             words = block_table_words(x.n, block_rows);
         } else {
             words = (size_t)x.n * (skip ? byte_stride(x.n) : glynn_stride(x.n, group));
-            test_words = skip ? skip_plan(x.a, x.n, group, order) : 0;
+            test_words = skip ? skip_plan(a, x.n, group, order) : 0;
         }
         table.assign(words, 0);
         if (x.mod3) {
-            build_block_tables_mod3(x.a, x.n, order, block_rows, table.data());
+            build_block_tables_mod3(a, x.n, order, block_rows, table.data());
         } else if (block_rows >= 0) {
-            build_block_tables(x.a, x.n, order, even, block_rows, table.data());
+            build_block_tables(a, x.n, order, even, block_rows, table.data());
         } else if (skip) {
-            build_byte_tables(x.a, x.n, order, table.data());
+            build_byte_tables(a, x.n, order, table.data());
         } else {
-            build_glynn_tables(x.a, x.n, x.q, group, order, table.data());
+            build_glynn_tables(a, x.n, x.q, group, order, table.data());
         }
         size_t found = sets.size();
         if (group > 0) {

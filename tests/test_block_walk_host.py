@@ -260,3 +260,26 @@ def test_mod3_block_tables_model_matches_oracle(oracle_c, n):
         A = rng.integers(-4, 5, (n, n)).astype(np.int64)
         want = oracle_c.mod3(np.where(A % 3 < 0, A % 3 + 3, A % 3).astype(np.int64))
         assert _mod3_permanent(A) == want, (n, A.tolist())
+
+
+def test_planner_walks_the_better_orientation():
+    """A 0/1 matrix whose rows all have odd degree (no row can sum to zero)
+    but whose columns all have even degree: the planner walks its
+    transpose on the zero-skipping kernels (host planner, no device)."""
+    from permhip import _native
+
+    rng = np.random.default_rng(31)
+    n = 14
+    A = (rng.random((n, n)) < 0.5).astype(np.int64)
+    for i in range(n - 1):
+        if A[i].sum() % 2 == 0:
+            A[i, n - 1] ^= 1
+    for j in range(n - 1):
+        if A[:, j].sum() % 2 == 1:
+            A[n - 1, j] ^= 1
+    if A[n - 1].sum() % 2 == 0:
+        A[n - 1, n - 1] ^= 1
+    assert all(d % 2 == 1 for d in A.sum(axis=1)) and all(d % 2 == 0 for d in A.sum(axis=0))
+    primes = [2147483647, 2147483629]
+    assert {cls[2] for cls in _native.batch_plan_classes([A], [primes])} == {2}
+    assert {cls[2] for cls in _native.batch_plan_classes([A.T.copy()], [primes])} == {2}

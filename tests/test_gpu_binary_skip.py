@@ -204,13 +204,23 @@ def test_grid_and_binary36_through_skip(native):
 
 def _odd_heavy_matrix(rng, n, density, even_rows):
     """A 0/1 matrix with exactly `even_rows` rows of even degree (the rest
-    odd), rows shuffled, so the zero test may read only half the words."""
+    odd), rows shuffled, so the zero test may read only half the words.  Its
+    columns have odd degree (all but the last when the parities require it),
+    so the planner keeps this orientation rather than walk the transpose."""
     A = (rng.random((n, n)) < density).astype(np.int64)
-    for i in range(n):
-        want_even = i < even_rows
-        if (A[i].sum() % 2 == 0) != want_even:
-            j = int(rng.integers(n))
-            A[i, j] ^= 1
+    row_odd = [i >= even_rows for i in range(n)]
+    col_odd = [True] * n
+    if sum(row_odd) % 2 != n % 2:
+        col_odd[n - 1] = False
+    for i in range(n - 1):
+        if (A[i].sum() % 2 == 1) != row_odd[i]:
+            A[i, n - 1] ^= 1
+    for j in range(n - 1):
+        if (A[:, j].sum() % 2 == 1) != col_odd[j]:
+            A[n - 1, j] ^= 1
+    if (A[n - 1].sum() % 2 == 1) != row_odd[n - 1]:
+        A[n - 1, n - 1] ^= 1
+    assert [d % 2 == 1 for d in A.sum(axis=1)] == row_odd
     return A[rng.permutation(n)]
 
 
