@@ -120,6 +120,28 @@ def test_update_cost_matches_definition():
     assert oracle.C.binary_update_cost(B) == 6
 
 
+def _simulated_update_cost(matrix):
+    """Multiplies of the ratio walk by simulation: the Gray code over the
+    n - 1 sparsest columns (ascending degree, the densest fixed) flips
+    column ctz(t) at state t and pays its degree; the cheaper of the matrix
+    and its transpose."""
+    best = None
+    for m in (matrix, matrix.T):
+        n = m.shape[0]
+        deg = sorted(int(d) for d in m.sum(axis=0))[: n - 1]
+        cost = sum(deg[(t & -t).bit_length() - 1] for t in range(1, 1 << (n - 1)))
+        best = cost if best is None else min(best, cost)
+    return best
+
+
+def test_update_cost_matches_simulated_walk():
+    rng = np.random.default_rng(19)
+    for n in range(2, 12):
+        for density in (0.2, 0.5, 0.8):
+            B = (rng.random((n, n)) < density).astype(np.uint8)
+            assert oracle.C.binary_update_cost(B) == _simulated_update_cost(B), (n, density)
+
+
 def test_reference_extensions_agree_with_oracle_when_built():
     ext = oracle.reference_extensions()
     if "perm_c_ext" not in ext:
