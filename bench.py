@@ -531,6 +531,9 @@ def main() -> int:
                     help="only the binary36 workload's 4 residues, REPS times (for profilers); "
                          "--skip-zeros-permille sets the zero-skipping threshold")
     ap.add_argument("--skip-zeros-permille", type=int, default=None)
+    ap.add_argument("--serve-breakdown", type=int, default=0, metavar="SAMPLES",
+                    help="time SAMPLES requests alone per (kind, order) of the served stream, scale by the stream's "
+                         "counts, print the device-time table and exit")
     ap.add_argument("--skip-blocks", type=int, default=None, choices=(0, 1),
                     help="0/1 tables on the zero-skipping kernels: 1 walks in blocks of 32 states (the default), "
                          "0 tests every state")
@@ -590,6 +593,31 @@ def main() -> int:
             t0 = time.perf_counter()
             r = _native.permanent_residues(B, primes)
             log(f"binary36 residues {r} in {(time.perf_counter() - t0) * 1e3:.1f} ms (rep {rep})")
+        return 0
+    if args.serve_breakdown:
+        import collections
+
+        import perm
+        import perm_binary
+        from permhip import workloads
+
+        by_order = collections.defaultdict(list)
+        for _, kind, A in workloads.serve_stream(args.serve_rate, args.serve_seconds):
+            by_order[(kind, A.shape[0])].append(A)
+        totals = collections.Counter()
+        print("kind | order | requests | ms per request (mean of the first SAMPLES, alone) | estimated s")
+        for (kind, n), mats in sorted(by_order.items()):
+            fn = perm.matrix_permanent if kind == "integer" else perm_binary.binary_permanent
+            fn(mats[0])  # warm the plan shapes
+            sample = mats[: args.serve_breakdown]
+            t0 = time.perf_counter()
+            for A in sample:
+                fn(A)
+            each = (time.perf_counter() - t0) / len(sample)
+            totals[kind] += each * len(mats)
+            print(f"{kind} | {n} | {len(mats)} | {each * 1e3:.2f} | {each * len(mats):.2f}", flush=True)
+        print(f"total | | | | integer {totals['integer']:.1f} s, binary {totals['binary']:.1f} s, "
+              f"all {sum(totals.values()):.1f} s")
         return 0
     if args.batch_only:
         from permhip.batch import PermanentBatch
