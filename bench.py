@@ -537,6 +537,9 @@ def main() -> int:
     ap.add_argument("--skip-blocks", type=int, default=None, choices=(0, 1),
                     help="0/1 tables on the zero-skipping kernels: 1 walks in blocks of 32 states (the default), "
                          "0 tests every state")
+    ap.add_argument("--skip-exact-min-primes", type=int, default=None, metavar="K",
+                    help="0/1 block tables of at most eight row groups with K primes or more take the block walk "
+                         "with exact accumulation (default 5; 0 keeps every table on the per-prime walk)")
     ap.add_argument("--frontier", type=int, default=0, metavar="REPS",
                     help="only the frontier DP on the open 12x16 grid's biadjacency (order 96), REPS times")
     ap.add_argument("--batch-only", type=int, default=0, metavar="REPS",
@@ -560,6 +563,8 @@ def main() -> int:
 
     if args.skip_blocks is not None:
         _native.set_tuning(_native.TUNE_SKIP_BLOCKS, args.skip_blocks)
+    if args.skip_exact_min_primes is not None:
+        _native.set_tuning(_native.TUNE_SKIP_EXACT_MIN_PRIMES, args.skip_exact_min_primes)
 
     if args.copy_calibration:
         _native.copy_calibration(args.copy_calibration, 1)

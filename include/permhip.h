@@ -215,7 +215,9 @@ int phip_batch_set_option(void *plan, int option, int64_t value);
  * sized for, [13] workgroups per CU, [14] kernel variant of the first
  * launch: 0 plain, 1 pairs consecutive states (prime sets below
  * 1.25 * 2^30), 2 skips zero terms (0/1 tables whose states are mostly
- * zero), 3 exact accumulation, 4 compact exact accumulation, [15] most workgroups per CU that any of the plan's kernels keeps
+ * zero), 3 exact accumulation, 4 compact exact accumulation, 5 the
+ * zero-skipping block walk with exact accumulation, [15] most workgroups
+ * per CU that any of the plan's kernels keeps
  * resident (register and LDS use; each class's grid is capped at its own
  * count; -1 if unknown), [16] table words the zero test of the first job
  * reads (zero-skipping jobs: all of them, or the leading half when every
@@ -227,8 +229,8 @@ int phip_batch_stats(void *plan, double *out, int len);
 /* The kernel class of every residue of a batch, in input order, as
  * phip_batch_create would plan it: (group * 32 + prime-set size) * 8 +
  * variant (0 plain, 1 paired states, 2 zero-skipping, 3 exact accumulation,
- * 4 compact exact accumulation; the set size of an exact class is its limb
- * column count).  Same description
+ * 4 compact exact accumulation, 5 zero-skipping block walk with exact
+ * accumulation; the set size of an exact class is its limb column count).  Same description
  * arguments as phip_batch_create; host code only (no device).  Writes
  * min(len, residues) entries and returns the residue count, or -1.  No
  * counterpart in the reference (plan introspection for tests and
@@ -283,6 +285,11 @@ int phip_batch_destroy(void *plan);
 #define PHIP_TUNE_MOD3_BLOCKS 14   /* 1 (default): permanents modulo 3 of orders 8..46 run on the modulo-3
                                       block walk (32 states per row lookup, a term is zero when a row sum
                                       is a multiple of 3); 0: the bit-sliced kernel for every order */
+#define PHIP_TUNE_SKIP_EXACT_MIN_PRIMES 15  /* fewest primes (default 5; 0 off) of a 0/1 block table of at most
+                                      eight row groups that take the block walk with exact accumulation (one
+                                      limb product per taken state for all primes instead of a Montgomery
+                                      tree per prime of a padded set of 4 or 8).  Read when a plan is
+                                      built */
 int phip_set_tuning(int key, int64_t value);
 int64_t phip_get_tuning(int key);
 

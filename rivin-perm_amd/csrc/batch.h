@@ -26,6 +26,7 @@ constexpr int kVariantPaired = 1;     // consecutive states paired (primes below
 constexpr int kVariantSkipZeros = 2;  // zero terms skipped through a per-wavefront queue
 constexpr int kVariantExact = 3;      // exact accumulation: one walk for any number of primes
 constexpr int kVariantExactCompact = 4;  // the same with the product tree sized for kCompactValueBits-bit values
+constexpr int kVariantSkipExact = 5;  // the block zero-skipping walk with exact accumulation (kExactColumns columns)
 // The exact variant's classes: a class's set size is its number of limb
 // columns L.  L = 8 takes tables of at most eight groups (full products);
 // L = 10 and 12 take up to sixteen groups when the product of the table's
@@ -39,7 +40,12 @@ constexpr int kExactMaxColumns = 12;
 // V = 22), for tables whose partial bound products fit (the products of two
 // row sums of entries up to about 100 at orders 24..28 are near 2^21).
 constexpr int kCompactValueBits = 22;
-constexpr bool is_exact_variant(int variant) { return variant == kVariantExact || variant == kVariantExactCompact; }
+constexpr bool is_exact_variant(int variant) {
+    return variant == kVariantExact || variant == kVariantExactCompact || variant == kVariantSkipExact;
+}
+// Variants that walk 0/1 tables on the zero-skipping kernels (byte rows,
+// block tables, per-wavefront LDS regions).
+constexpr bool is_skip_variant(int variant) { return variant == kVariantSkipZeros || variant == kVariantSkipExact; }
 
 // One job: one table (a matrix under one row order) and a set of K odd
 // moduli below 2^31 that share it (K = 1 for residue rows).  56 bytes,
@@ -55,11 +61,12 @@ struct BatchJob {
     int32_t group;
     int32_t nprimes;     // real primes (<= K); padded entries repeat the last one
     int32_t set;         // K of the job's launch class
-    int32_t variant;     // kVariantPlain, kVariantPaired, kVariantSkipZeros, kVariantExact or kVariantExactCompact
+    int32_t variant;     // kVariantPlain, kVariantPaired, kVariantSkipZeros, kVariantExact, kVariantExactCompact
+                         // or kVariantSkipExact
     int32_t test_words;  // kVariantSkipZeros: leading table words the zero test reads (skip_test_words)
-    int32_t block_rows;  // kVariantSkipZeros: rows the 32-state block test reads (block_test_rows: a multiple
-                         // of 8 or every row word; the rows of even degree lead the table), or -1 for the
-                         // per-state test
+    int32_t block_rows;  // kVariantSkipZeros, kVariantSkipExact: rows the 32-state block test reads
+                         // (block_test_rows: a multiple of 8 or every row word; the rows of even degree lead
+                         // the table), or -1 for the per-state test
     int32_t block_mod3;  // kVariantSkipZeros with block_rows >= 0: 1 for the modulo-3 block walk (row sums that
                          // are multiples of 3 zero a term; build_block_tables_mod3), else 0
 };
@@ -118,9 +125,11 @@ constexpr int block_region_words(int n, int rows) {
 // ring.
 constexpr int block_region_words_mod3(int n, int rows) { return rows * 64 + 12 * ((n + 3) / 4); }
 // Dynamic LDS of one workgroup (4 wavefronts) of a class's kernel: the
-// zero-skip queues or block-walk regions, nothing for the other variants.
+// zero-skip queues or block-walk regions (only block regions for the exact
+// form, which runs block tables only), nothing for the other variants.
 inline unsigned class_lds_bytes(const BatchClass& cl) {
-    if (cl.variant != 2) return 0u;
+    if (cl.variant == kVariantSkipExact) return 4u * 4u * (unsigned)cl.max_block_words;
+    if (cl.variant != kVariantSkipZeros) return 0u;
     const int words = skip_queue_rows(cl.max_n) * kZeroQueueEntries;
     return 4u * 4u * (unsigned)(words > cl.max_block_words ? words : cl.max_block_words);
 }
@@ -244,6 +253,11 @@ PHIP_DECLARE_BATCH_LAUNCH_SKIP(6, 2)
 PHIP_DECLARE_BATCH_LAUNCH_SKIP(6, 4)
 PHIP_DECLARE_BATCH_LAUNCH_SKIP(6, 8)
 #undef PHIP_DECLARE_BATCH_LAUNCH_SKIP
+// Block zero-skipping walks with exact accumulation (8 limb columns).
+hipError_t launch_batch_g4_k8_skipexact(dim3, unsigned, hipStream_t, const BatchJob*, const uint2*, uint32_t,
+                                        uint32_t*, const int32_t*, const int32_t*, const uint32_t*, uint64_t*);
+hipError_t launch_batch_g6_k8_skipexact(dim3, unsigned, hipStream_t, const BatchJob*, const uint2*, uint32_t,
+                                        uint32_t*, const int32_t*, const int32_t*, const uint32_t*, uint64_t*);
 // Exact-accumulation variants (any number of primes; L = K limb columns).
 #define PHIP_DECLARE_BATCH_LAUNCH_EXACT(G, K)                                                                 \
     hipError_t launch_batch_g##G##_k##K##_exact(dim3, unsigned, hipStream_t, const BatchJob*, const uint2*, uint32_t,  \
@@ -289,6 +303,7 @@ PHIP_DECLARE_BATCH_OCC(occupancy_batch_g2_k10_exact) PHIP_DECLARE_BATCH_OCC(occu
 PHIP_DECLARE_BATCH_OCC(occupancy_batch_g3_k10_exact) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g3_k12_exact)
 PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k10_exact) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k12_exact)
 PHIP_DECLARE_BATCH_OCC(occupancy_batch_g2_k10_compact)
+PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k8_skipexact) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g6_k8_skipexact)
 #undef PHIP_DECLARE_BATCH_OCC
 
 }  // namespace phip

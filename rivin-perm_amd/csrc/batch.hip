@@ -19,7 +19,10 @@
 // PHIP_BATCH_EXACT=1 the exact-accumulation variant (ExactSink), whose items
 // hold PHIP_BATCH_SET limb columns of the exact sums for all of a job's
 // primes, and PHIP_BATCH_COMPACT=1 (with it) the form whose product tree is
-// sized for kCompactValueBits-bit group values (LimbProduct).
+// sized for kCompactValueBits-bit group values (LimbProduct).  Both
+// PHIP_BATCH_SKIP_ZEROS=1 and PHIP_BATCH_EXACT=1 (with PHIP_BATCH_SET=8)
+// select the block zero-skipping walk with exact accumulation
+// (glynn_chunk_blocks_exact): 0/1 block tables of at most eight groups.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -95,12 +98,17 @@ __device__ __forceinline__ void run_item(const int32_t* __restrict__ tables, con
 // kExactMaxGroups groups, the wider ones tables of more groups, up to
 // kExactWideMaxGroups; other orders are never planned on the class and
 // compile to nothing.
-template <int N, int G, int L, int V>
+template <int N, int G, int L, int V, bool Z>
 __device__ __forceinline__ void run_item_exact(const int32_t* __restrict__ tables, const BatchJob& jb, uint64_t base,
-                                               int64_t (&cols)[L]) {
+                                               bool active, int32_t* queue, bool reload, int64_t (&cols)[L]) {
     constexpr int kGroups = G > 0 ? (N + G - 1) / G : 0;
-    if constexpr (G > 0 && (L == kExactColumns ? kGroups <= kExactMaxGroups
-                                               : (kGroups > kExactMaxGroups && kGroups <= kExactWideMaxGroups))) {
+    if constexpr (Z) {
+        // Block tables of at most eight groups (the planner sends no others).
+        if constexpr (N >= kBlockMinOrder && L == kExactColumns && G > 0 && kGroups <= kExactMaxGroups) {
+            glynn_chunk_blocks_exact<N, G>(tables, N, jb.cbits, base, active, queue, jb.block_rows, reload, cols);
+        }
+    } else if constexpr (G > 0 && (L == kExactColumns ? kGroups <= kExactMaxGroups
+                                                      : (kGroups > kExactMaxGroups && kGroups <= kExactWideMaxGroups))) {
         glynn_chunk_exact<N, G, L, V>(tables, N, jb.cbits, base, cols);
     }
 }
@@ -139,6 +147,10 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
         const bool active = lane < jb.lanes;
         const uint64_t base = (active ? lane : 0) << jb.cbits;
         const int32_t* __restrict__ tables = blob + jb.data_off;
+        // A block-walk wavefront keeps its job's tables in LDS between items.
+        const bool block = Z && jb.block_rows >= 0;
+        const bool reload = block && it.x != loaded_job;
+        loaded_job = block ? it.x : 0xffffffffu;
         if constexpr (E) {
             // Exact jobs: K limb columns per item, for all of the job's
             // primes (the finalisation reduces them per prime).
@@ -148,7 +160,7 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
             switch (jb.n) {
 #define PHIP_EXACT_CASE(N) \
     case N:                \
-        run_item_exact<N, G, K, V>(tables, jb, base, cols); \
+        run_item_exact<N, G, K, V, Z>(tables, jb, base, active, queue, reload, cols); \
         break;
                 PHIP_EXACT_CASE(5) PHIP_EXACT_CASE(6) PHIP_EXACT_CASE(7) PHIP_EXACT_CASE(8)
                 PHIP_EXACT_CASE(9) PHIP_EXACT_CASE(10) PHIP_EXACT_CASE(11) PHIP_EXACT_CASE(12)
@@ -168,8 +180,8 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
 #pragma unroll
             for (int c = 0; c < K; ++c) {
                 // Column sums of 64 lanes in two's complement: |sum| <
-                // 2^(37 + cbits).
-                const uint64_t s = wave_reduce_sum_u64(active ? (uint64_t)cols[c] : 0u);
+                // 2^(37 + cbits).  Block-walk lanes take states of any lane.
+                const uint64_t s = wave_reduce_sum_u64((Z || active) ? (uint64_t)cols[c] : 0u);
                 if (lane_id == 0) item_partials[(size_t)idx * kItemSlots + c] = s;
             }
         } else {
@@ -183,9 +195,6 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
         int32_t r[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) r[k] = 0;
-        const bool block = Z && jb.block_rows >= 0;
-        const bool reload = block && it.x != loaded_job;
-        loaded_job = block ? it.x : 0xffffffffu;
         switch (jb.n) {
 #define PHIP_BATCH_CASE(N) \
     case N:                \
@@ -222,8 +231,13 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
 
 #define PHIP_CAT5(a, b, c, d, e) a##b##c##d##e
 #define PHIP_LAUNCH_NAME(G, K, P) PHIP_CAT5(launch_batch_g, G, _k, K, P)
-#if (PHIP_BATCH_PAIRED + PHIP_BATCH_SKIP_ZEROS + PHIP_BATCH_EXACT) > 1
-#error "a class either pairs states, skips zero terms or accumulates exactly"
+#if PHIP_BATCH_PAIRED && (PHIP_BATCH_SKIP_ZEROS + PHIP_BATCH_EXACT) > 0
+#error "a paired class neither skips zero terms nor accumulates exactly"
+#elif PHIP_BATCH_SKIP_ZEROS && PHIP_BATCH_EXACT
+#if PHIP_BATCH_SET != 8 || PHIP_BATCH_COMPACT
+#error "exact block walks hold 8 limb columns per item"
+#endif
+#define PHIP_PAIRED_SUFFIX _skipexact
 #elif PHIP_BATCH_EXACT
 #if PHIP_BATCH_SET != 8 && PHIP_BATCH_SET != 10 && PHIP_BATCH_SET != 12
 #error "exact classes hold 8, 10 or 12 limb columns per item"

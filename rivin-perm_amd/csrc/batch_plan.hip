@@ -144,6 +144,8 @@ BatchLaunchFn launcher_for(int group, int set, int variant) {
         if (group == 4 && set == 12) return launch_batch_g4_k12_exact;
     }
     if (variant == kVariantExactCompact && group == 2 && set == 10) return launch_batch_g2_k10_compact;
+    if (variant == kVariantSkipExact && group == 4 && set == 8) return launch_batch_g4_k8_skipexact;
+    if (variant == kVariantSkipExact && group == 6 && set == 8) return launch_batch_g6_k8_skipexact;
 #undef PHIP_LAUNCHER
 #undef PHIP_LAUNCHER_PAIRED
 #undef PHIP_LAUNCHER_SKIP
@@ -179,6 +181,8 @@ int occupancy_for(int group, int set, int variant, unsigned lds) {
         if (group == 4 && set == 12) return occupancy_batch_g4_k12_exact(lds);
     }
     if (variant == kVariantExactCompact && group == 2 && set == 10) return occupancy_batch_g2_k10_compact(lds);
+    if (variant == kVariantSkipExact && group == 4 && set == 8) return occupancy_batch_g4_k8_skipexact(lds);
+    if (variant == kVariantSkipExact && group == 6 && set == 8) return occupancy_batch_g6_k8_skipexact(lds);
 #undef PHIP_OCC
 #undef PHIP_OCC_PAIRED
 #undef PHIP_OCC_SKIP
@@ -600,7 +604,14 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
         // walk for all of them (exact_columns): with eight groups the walk
         // costs about as much as the modular walk of a set of six primes,
         // and the residues come from its exact sums.
-        const int columns = (st.group > 0 && !st.skip_zeros)
+        // A 0/1 block table of at most eight groups with enough primes runs
+        // the block walk once with exact accumulation (a taken state costs
+        // its limb product instead of a Montgomery tree per prime).
+        const bool skip_exact = st.skip_zeros && st.block_rows >= 0 && !st.mod3 &&
+                                groups_of(st.n, st.group) <= kExactMaxGroups && tuning().skip_exact_min_primes > 0 &&
+                                st.inputs.size() >= (size_t)tuning().skip_exact_min_primes;
+        const int columns = skip_exact ? kExactColumns
+                            : (st.group > 0 && !st.skip_zeros)
                                 ? exact_columns(groups_of(st.n, st.group), st.bits, st.inputs.size())
                                 : 0;
         if (columns > 0) {
@@ -611,7 +622,11 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
             jb.set = columns;
             jb.nprimes = (int32_t)st.inputs.size();
             jb.prime_off = (uint32_t)out.set_p.size();
-            jb.variant = (columns == 10 && st.compact) ? kVariantExactCompact : kVariantExact;
+            jb.variant = skip_exact ? kVariantSkipExact
+                                    : ((columns == 10 && st.compact) ? kVariantExactCompact : kVariantExact);
+            jb.block_rows = skip_exact ? st.block_rows : -1;
+            jb.test_words = skip_exact ? st.test_words : 0;
+            jb.block_mod3 = 0;
             const uint32_t job = (uint32_t)out.jobs.size();
             const int L = st.n - 1;
             for (size_t k = 0; k < st.inputs.size(); ++k) {
@@ -716,7 +731,7 @@ int assign_items(PlanHost& out, int items_log2, int min_cbits) {
         if (is_exact_variant(jb.variant)) c = std::min(c, 24);
         // The block walk enumerates the first kBlockInnerBits Gray columns
         // inside a chunk (n >= kBlockMinOrder leaves L >= 7).
-        if (jb.variant == kVariantSkipZeros && jb.block_rows >= 0) c = std::max(c, kBlockMinChunkBits);
+        if (is_skip_variant(jb.variant) && jb.block_rows >= 0) c = std::max(c, kBlockMinChunkBits);
         jb.cbits = c;
         jb.lanes = 1ull << (L - c);
     }
@@ -724,7 +739,8 @@ int assign_items(PlanHost& out, int items_log2, int min_cbits) {
     // Items: per (group, set, variant) class, jobs by descending cost, each
     // job's items contiguous.
     std::vector<bool> placed(out.jobs.size(), false);
-    static const int kClasses[][3] = {{2, 8, 3}, {3, 8, 3}, {4, 8, 3}, {6, 8, 3}, {2, 10, 4},
+    static const int kClasses[][3] = {{4, 8, 5}, {6, 8, 5},
+                                      {2, 8, 3}, {3, 8, 3}, {4, 8, 3}, {6, 8, 3}, {2, 10, 4},
                                       {2, 10, 3}, {2, 12, 3}, {3, 10, 3}, {3, 12, 3}, {4, 10, 3}, {4, 12, 3},
                                       {0, 1, 0},
                                       {2, 8, 1}, {2, 8, 0}, {2, 4, 1}, {2, 4, 0}, {2, 2, 0}, {2, 1, 0},
@@ -749,7 +765,7 @@ int assign_items(PlanHost& out, int items_log2, int min_cbits) {
         for (uint32_t j : order) {
             BatchJob& b = out.jobs[j];
             cl.max_n = std::max(cl.max_n, b.n);
-            if (b.variant == kVariantSkipZeros && b.block_rows >= 0)
+            if (is_skip_variant(b.variant) && b.block_rows >= 0)
                 cl.max_block_words =
                     std::max(cl.max_block_words, b.block_mod3 ? block_region_words_mod3(b.n, b.block_rows)
                                                               : block_region_words(b.n, b.block_rows));
@@ -803,7 +819,7 @@ int enqueue_plan(const PlanHost& h, const PlanDevice& d, hipStream_t stream, int
         // more waves per SIMD hide them (binary36 40 -> 36 ms from 4 to 7
         // workgroups per CU).
         const int resident = class_resident(cl);
-        const int want = (cl.variant == kVariantSkipZeros && resident > 0) ? resident : blocks_per_cu;
+        const int want = (is_skip_variant(cl.variant) && resident > 0) ? resident : blocks_per_cu;
         const int per_cu = resident > 0 ? std::min(want, resident) : want;
         PHIP_HIP_TRY(fn(dim3((unsigned)(cus * std::max(per_cu, 1))), class_lds_bytes(cl), stream, d.jobs,
                         d.items + cl.first_item, cl.n_items, d.counters + c, d.blob, d.set_p, d.set_pinv,

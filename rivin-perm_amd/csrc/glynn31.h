@@ -967,9 +967,19 @@ __device__ __forceinline__ uint32_t kth_set_bit(uint32_t m, uint32_t k) {
 // kMod3: the modulo-3 walk (build_block_tables_mod3): 64-word mask tables
 // per row, and the nonzero states, which are rare there (every row must
 // avoid a multiple of 3), taken in place by the lanes that hold them.
-template <int NR, int G, int K, bool kMod3 = false>
+//
+// kExact: the terms go to K = kExactColumns exact limb columns instead of K
+// primes' residues: each taken state's product of group products in
+// radix-2^31 limbs (LimbProduct, full products of at most eight groups of
+// magnitudes below 2^31), each limb added with the term's sign to its int64
+// column, for every prime of the job at once.  A lane takes at most one
+// state per batch of 64 and at most 2^cbits batches complete per item, so
+// |column| < 2^(31 + cbits) per lane, as in ExactSink.
+template <int NR, int G, int K, bool kMod3 = false, bool kExact = false>
 struct BlockSkipWalk {
     static_assert(G > 0, "prime sets need exact row sums");
+    static_assert(!kExact || (!kMod3 && K == kExactColumns && (NR + G - 1) / G <= kExactMaxGroups),
+                  "exact block walks take at most eight groups in kExactColumns columns");
     using Rows = ByteRows<NR>;
     using Groups = ByteGroups<NR, G>;
     static constexpr int W = Rows::kStride;
@@ -1006,6 +1016,24 @@ struct BlockSkipWalk {
         int32_t g[kGroups];
         const Groups q{w};
         q.groups(g, std::make_integer_sequence<int, kGroups>{});
+        if constexpr (kExact) {
+            uint32_t m[kGroups];
+#pragma unroll
+            for (int k = 0; k < kGroups; ++k) m[k] = (uint32_t)(g[k] < 0 ? -g[k] : g[k]);
+            // The term's sign: bit 31 of the XOR of the group products and
+            // of the state's sign, folded three at a time.
+            uint32_t s = sign << 31;
+#pragma unroll
+            for (int k = 0; k < kGroups; k += 2)
+                s = k + 1 < kGroups ? xor3_fast(s, (uint32_t)g[k], (uint32_t)g[k + 1]) : s ^ (uint32_t)g[k];
+            const int32_t e = ((int32_t)s >> 31) | 1;
+            using P = LimbProduct<0, kGroups, kExactColumns>;
+            uint32_t z[P::kLimbs];
+            P::run(m, z);
+#pragma unroll
+            for (int c = 0; c < P::kLimbs; ++c) acc[c] = mad_exact(acc[c], (int32_t)z[c], e);
+            return;
+        }
         if (sign) g[0] = -g[0];
         int64_t P[kPairs > 0 ? kPairs : 1];
 #pragma unroll
@@ -1157,20 +1185,14 @@ struct BlockSkipWalk {
 // words); with `reload` the mask and inner tables are copied into it first
 // (a wavefront keeps them while it stays on one job).  Every lane of the
 // wavefront must call it.
-template <int NR, int G, int K, bool kMod3 = false>
-__device__ __forceinline__ void glynn_chunk_set_blocks(const int32_t* __restrict__ tables, int n, int cbits,
-                                                       uint64_t base, bool active, int32_t* region, int rows,
-                                                       bool reload, const int32_t (&primes)[K],
-                                                       const uint32_t (&pinvs)[K], int32_t (&out)[K]) {
-    using Walk = BlockSkipWalk<NR, G, K, kMod3>;
+template <int NR, int G, int K, bool kMod3, bool kExact>
+__device__ __forceinline__ void block_walk_chunk(BlockSkipWalk<NR, G, K, kMod3, kExact>& walk,
+                                                 const int32_t* __restrict__ tables, int n, int cbits, uint64_t base,
+                                                 bool active, int32_t* region, int rows, bool reload) {
+    using Walk = BlockSkipWalk<NR, G, K, kMod3, kExact>;
     constexpr int W = Walk::W;
-    Walk walk;
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        walk.negp[k] = -primes[k];
-        walk.pinv[k] = pinvs[k];
-        walk.acc[k] = 0;
-    }
+    for (int k = 0; k < K; ++k) walk.acc[k] = 0;
     const uint32_t lane = __lane_id();
     lds_u32* const lds = (lds_u32*)region;
     // Mask tables (rows + 1 overlapping tables of 32 words, or rows of 64
@@ -1225,8 +1247,35 @@ __device__ __forceinline__ void glynn_chunk_set_blocks(const int32_t* __restrict
         const uint32_t pend = walk.thead - walk.ttail;
         if (pend) walk.consume(pend);
     }
+}
+
+template <int NR, int G, int K, bool kMod3 = false>
+__device__ __forceinline__ void glynn_chunk_set_blocks(const int32_t* __restrict__ tables, int n, int cbits,
+                                                       uint64_t base, bool active, int32_t* region, int rows,
+                                                       bool reload, const int32_t (&primes)[K],
+                                                       const uint32_t (&pinvs)[K], int32_t (&out)[K]) {
+    BlockSkipWalk<NR, G, K, kMod3> walk;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        walk.negp[k] = -primes[k];
+        walk.pinv[k] = pinvs[k];
+    }
+    block_walk_chunk(walk, tables, n, cbits, base, active, region, rows, reload);
 #pragma unroll
     for (int k = 0; k < K; ++k) out[k] = lane_value(walk.acc[k], false, walk.negp[k], walk.pinv[k]);
+}
+
+// The same walk with exact accumulation: the lane's kExactColumns limb
+// columns of the Glynn sum over the states it takes (only the wavefront's
+// total is meaningful).
+template <int NR, int G>
+__device__ __forceinline__ void glynn_chunk_blocks_exact(const int32_t* __restrict__ tables, int n, int cbits,
+                                                         uint64_t base, bool active, int32_t* region, int rows,
+                                                         bool reload, int64_t (&cols)[kExactColumns]) {
+    BlockSkipWalk<NR, G, kExactColumns, false, true> walk;
+    block_walk_chunk(walk, tables, n, cbits, base, active, region, rows, reload);
+#pragma unroll
+    for (int k = 0; k < kExactColumns; ++k) cols[k] = walk.acc[k];
 }
 
 }  // namespace phip

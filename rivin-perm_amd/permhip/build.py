@@ -32,8 +32,9 @@ ARCH = "gfx950"
 # of 4 or 6 the zero-skipping variant (2) for sets of 2, 4 and 8, and every
 # group the exact-accumulation variant (3, eight limb columns per item), and
 # groups of 2 rows its compact form (4: the product tree sized for 22-bit
-# group values, ten limb columns).
-BATCH_VARIANT_SUFFIX = {0: "", 1: "_paired", 2: "_skip", 3: "_exact", 4: "_compact"}
+# group values, ten limb columns), and 0/1 block tables in groups of 4 or 6
+# the block walk with exact accumulation (5, eight limb columns).
+BATCH_VARIANT_SUFFIX = {0: "", 1: "_paired", 2: "_skip", 3: "_exact", 4: "_compact", 5: "_skipexact"}
 BATCH_CLASSES = (
     ((0, 1, 0),)
     + tuple((g, k, 0) for g in (2, 3, 4, 6) for k in (1, 2, 4, 8))
@@ -42,6 +43,7 @@ BATCH_CLASSES = (
     + tuple((g, 8, 3) for g in (2, 3, 4, 6))
     + tuple((g, k, 3) for g in (2, 3, 4) for k in (10, 12))
     + ((2, 10, 4),)
+    + tuple((g, 8, 5) for g in (4, 6))
 )
 PLAIN_SOURCES = (
     "device_context.hip",
@@ -104,7 +106,7 @@ def build(jobs: int | None = None, verbose: bool = False) -> Path:
                 CSRC / "batch.hip",
                 BUILD_DIR / f"batch_g{group}_k{k}{BATCH_VARIANT_SUFFIX[variant]}.o",
                 [f"-DPHIP_BATCH_GROUP={group}", f"-DPHIP_BATCH_SET={k}", f"-DPHIP_BATCH_PAIRED={int(variant == 1)}",
-                 f"-DPHIP_BATCH_SKIP_ZEROS={int(variant == 2)}", f"-DPHIP_BATCH_EXACT={int(variant in (3, 4))}",
+                 f"-DPHIP_BATCH_SKIP_ZEROS={int(variant in (2, 5))}", f"-DPHIP_BATCH_EXACT={int(variant in (3, 4, 5))}",
                  f"-DPHIP_BATCH_COMPACT={int(variant == 4)}"],
             )
         )

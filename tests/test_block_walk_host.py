@@ -283,3 +283,32 @@ def test_planner_walks_the_better_orientation():
     primes = [2147483647, 2147483629]
     assert {cls[2] for cls in _native.batch_plan_classes([A], [primes])} == {2}
     assert {cls[2] for cls in _native.batch_plan_classes([A.T.copy()], [primes])} == {2}
+
+
+def test_planner_takes_the_exact_block_walk():
+    """0/1 block tables of at most eight row groups with five primes or
+    more take the block walk with exact accumulation (variant 5, eight limb
+    columns); fewer primes, more groups (groups of 4 rows above order 32)
+    or the threshold at 0 keep the per-prime walk (host planner, no
+    device)."""
+    from permhip import _native
+
+    primes = [2147483647, 2147483629, 2147483587, 2147483579, 2147483563]
+    rng = np.random.default_rng(36)
+    A = (rng.random((36, 36)) < 0.5).astype(np.int64)
+    classes = lambda M, k: {cls for cls in _native.batch_plan_classes([M], [primes[:k]])}
+    assert classes(A, 2) == {(6, 2, 2)}
+    for k in (3, 4):
+        assert classes(A, k) == {(6, 4, 2)}
+    assert classes(A, 5) == {(6, 8, 5)}
+    # All-ones J_40: degree 40 rows group by 4 (40^6 >= 2^31), ten groups.
+    assert classes(np.ones((40, 40), dtype=np.int64), 5) == {(4, 8, 2)}
+    old = _native.get_tuning(_native.TUNE_SKIP_EXACT_MIN_PRIMES)
+    assert old == 5
+    try:
+        _native.set_tuning(_native.TUNE_SKIP_EXACT_MIN_PRIMES, 0)
+        assert classes(A, 5) == {(6, 8, 2)}
+        _native.set_tuning(_native.TUNE_SKIP_EXACT_MIN_PRIMES, 2)
+        assert classes(A, 2) == {(6, 8, 5)}
+    finally:
+        _native.set_tuning(_native.TUNE_SKIP_EXACT_MIN_PRIMES, old)

@@ -3,7 +3,10 @@ zero terms): the planner picks them, and their residues equal the plain
 kernels' and the oracle's, bit for bit, including chunks with inactive
 lanes, padded single primes, zero rows and mixed plans.  Both walks are
 covered: the block walk (32 states per row lookup, the default from order
-8 on) and the per-state zero test."""
+8 on) and the per-state zero test; the block walk in both its forms, with
+a Montgomery tree per prime and with exact accumulation (the default for
+five primes or more; the block_exact fixture runs the tests that take it
+once with it from three primes on and once with it off)."""
 
 import math
 import random
@@ -28,6 +31,24 @@ def _with_skip(native, permille, fn):
         return fn()
     finally:
         native.set_tuning(native.TUNE_SKIP_ZEROS_PERMILLE, old)
+
+
+# Plan variants of the zero-skipping walks: per prime (2) and the block walk
+# with exact accumulation (5).
+SKIP_VARIANTS = (2.0, 5.0)
+
+
+@pytest.fixture(params=[3, 0], ids=["exact-blocks", "modular-blocks"])
+def block_exact(request, native):
+    """Runs a test with the exact block walk from three primes on (the
+    default is five) and switched off (every block table on the per-prime
+    walk)."""
+    old = native.get_tuning(native.TUNE_SKIP_EXACT_MIN_PRIMES)
+    native.set_tuning(native.TUNE_SKIP_EXACT_MIN_PRIMES, request.param)
+    try:
+        yield request.param
+    finally:
+        native.set_tuning(native.TUNE_SKIP_EXACT_MIN_PRIMES, old)
 
 
 def _with_blocks(native, on, fn):
@@ -74,10 +95,10 @@ def test_skip_matches_plain_and_oracle_small(native, oracle_c, n):
 
 
 @pytest.mark.parametrize("n", [28, 32])
-def test_skip_matches_plain_large(native, n):
+def test_skip_matches_plain_large(native, block_exact, n):
     rng = np.random.default_rng(7 + n)
     A = _even_degree_matrix(rng, n, 0.5)
-    assert _variant(native, A, BINARY_PRIMES[:4]) == 2.0
+    assert _variant(native, A, BINARY_PRIMES[:4]) == (5.0 if block_exact else 2.0)
     print(f"skip large n={n}", flush=True)
     skip = native.permanent_residues(A, BINARY_PRIMES[:4])
     per_state = _with_blocks(native, False, lambda: native.permanent_residues(A, BINARY_PRIMES[:4]))
@@ -88,7 +109,9 @@ def test_skip_matches_plain_large(native, n):
 def test_planner_picks_the_variant(native):
     rng = np.random.default_rng(11)
     A = _even_degree_matrix(rng, 20, 0.5)
+    assert _variant(native, A, BINARY_PRIMES[:5]) == 5.0
     assert _variant(native, A, BINARY_PRIMES[:4]) == 2.0
+    assert _with_blocks(native, False, lambda: _variant(native, A, BINARY_PRIMES[:4])) == 2.0
     assert _with_skip(native, 0, lambda: _variant(native, A, BINARY_PRIMES[:4])) == 0.0
     # Odd degrees everywhere: no row ever sums to zero, so no skipping.
     odd = np.zeros((20, 20), dtype=np.int64)
@@ -98,7 +121,7 @@ def test_planner_picks_the_variant(native):
     assert _variant(native, odd, BINARY_PRIMES[:4]) == 0.0
     # Integer tables never skip.
     B = rng.integers(-3, 4, (20, 20)).astype(np.int64)
-    assert _variant(native, B, BINARY_PRIMES[:4]) != 2.0
+    assert _variant(native, B, BINARY_PRIMES[:4]) not in SKIP_VARIANTS
 
 
 def _block_ones(sizes, rng):
@@ -114,7 +137,7 @@ def _block_ones(sizes, rng):
 
 
 @pytest.mark.parametrize("sizes,half", [((4,) * 11, False), ((3,) * 9 + (4,) * 4 + (2,), True)])
-def test_skip_largest_orders_closed_form(native, sizes, half):
+def test_skip_largest_orders_closed_form(native, block_exact, sizes, half):
     """Orders 44 and 45 (near the byte walk's largest, 46): eleven J_4
     blocks (every row of even degree, full-width test) and nine J_3, four
     J_4 and one J_2 block (18 even rows of 45, half-width test; an order
@@ -138,12 +161,13 @@ def test_skip_largest_orders_closed_form(native, sizes, half):
         print(f"skip largest n={n} half={half} per state", flush=True)
         return native.permanent_residues(A, primes)
 
-    assert _with_blocks(native, False, per_state) == [exact % p for p in primes]
+    if block_exact:  # the per-state walk does not depend on the block walk's form
+        assert _with_blocks(native, False, per_state) == [exact % p for p in primes]
     print(f"skip largest n={n} half={half} blocks", flush=True)
     assert native.permanent_residues(A, primes) == [exact % p for p in primes]
 
 
-def test_skip_known_counts(native):
+def test_skip_known_counts(native, block_exact):
     """Closed forms through the zero-skipping kernels: all-ones J_n
     (n!, rows of even degree for even n), J_n - I (derangements), a zero
     row, and block-diagonal ones."""
@@ -166,7 +190,7 @@ def test_skip_known_counts(native):
     assert perm_binary.binary_permanent(B) == math.factorial(4) ** 6
 
 
-def test_skip_batch_mixes_variants(native, oracle_c):
+def test_skip_batch_mixes_variants(native, oracle_c, block_exact):
     """One plan with zero-skipping, plain and paired jobs: every residue
     equals the oracle's."""
     rng = np.random.default_rng(21)
@@ -274,9 +298,10 @@ def test_full_width_when_even_rows_spill(native):
 
 
 @pytest.mark.parametrize("k", [3, 5, 6, 7])
-def test_one_walk_for_up_to_eight_primes(native, k):
+def test_one_walk_for_up_to_eight_primes(native, block_exact, k):
     """A 0/1 table's primes share one zero-skipping walk (a padded set of 4
-    or 8), with the plain kernels' residues."""
+    or 8, or the exact block walk's eight limb columns), with the plain
+    kernels' residues."""
     rng = np.random.default_rng(40 + k)
     A = _even_degree_matrix(rng, 24, 0.5)
     primes = BINARY_PRIMES[:k]
@@ -285,7 +310,10 @@ def test_one_walk_for_up_to_eight_primes(native, k):
         st = plan.stats()
     finally:
         plan.close()
-    assert st["variant"] == 2.0 and st["jobs"] == 1.0 and st["set"] == (4.0 if k <= 4 else 8.0)
+    if block_exact:
+        assert st["variant"] == 5.0 and st["jobs"] == 1.0 and st["set"] == 8.0
+    else:
+        assert st["variant"] == 2.0 and st["jobs"] == 1.0 and st["set"] == (4.0 if k <= 4 else 8.0)
     print(f"one walk k={k}", flush=True)
     skip = native.permanent_residues(A, primes)
     per_state = _with_blocks(native, False, lambda: native.permanent_residues(A, primes))
@@ -299,7 +327,7 @@ def _block_rows(n, even):
 
 
 @pytest.mark.parametrize("n,even_rows", [(8, 8), (13, 5), (24, 24), (31, 9), (38, 20), (38, 38)])
-def test_block_walk_stats_and_parity(native, oracle_c, n, even_rows):
+def test_block_walk_stats_and_parity(native, oracle_c, block_exact, n, even_rows):
     """The block walk tests whole pairs of row words covering the rows of
     even degree; its residues equal the per-state walk's, and up to order
     24 the oracle's."""
@@ -313,9 +341,10 @@ def test_block_walk_stats_and_parity(native, oracle_c, n, even_rows):
     finally:
         plan.close()
     print(f"block n={n} even={even} stats={st['variant']},{st['skip_block_rows']},{st['chunk_bits_min']}", flush=True)
-    if st["variant"] != 2.0:
+    if st["variant"] not in SKIP_VARIANTS:
         assert even < 4
         return
+    assert st["variant"] == (5.0 if block_exact else 2.0)
     assert st["skip_block_rows"] == _block_rows(n, even)
     assert st["chunk_bits_min"] >= 6
     blocks = native.permanent_residues(A, primes)
@@ -326,16 +355,17 @@ def test_block_walk_stats_and_parity(native, oracle_c, n, even_rows):
             assert r == oracle_c.binary_mod_p(A.astype(np.uint8), p, 16), (n, p)
 
 
-def test_block_walk_many_jobs_one_class(native, oracle_c):
+def test_block_walk_many_jobs_one_class(native, oracle_c, block_exact):
     """One plan of 0/1 matrices of orders 8..22 on one block class: the
     wavefronts move between jobs of different table sizes (the LDS region
     is sized for the largest; tables reload on a job change), and single
-    primes, sets of 2 and 4 mix."""
+    primes, sets of 2, 3, 4 and 5 mix (with the exact block walk on, the
+    sets of 3 or more on its class, the rest on the per-prime walk)."""
     rng = np.random.default_rng(61)
     mats, mods = [], []
     for i, n in enumerate(list(range(8, 23)) * 2):
         mats.append(_even_degree_matrix(rng, n, 0.4 + 0.02 * (i % 10)))
-        mods.append(BINARY_PRIMES[: (1, 2, 4)[i % 3]])
+        mods.append(BINARY_PRIMES[: (1, 2, 4, 3, 5)[i % 5]])
     plan = native.Batch(mats, mods)
     try:
         flat = plan.run()
@@ -351,7 +381,7 @@ def test_block_walk_many_jobs_one_class(native, oracle_c):
 
 
 @pytest.mark.parametrize("n", [40, 42])
-def test_block_walk_large_closed_forms(native, n):
+def test_block_walk_large_closed_forms(native, block_exact, n):
     """Orders 40 and 42 on the block walk: block-diagonal all-ones blocks of
     sizes 2..5 (rows of both parities), residues the product of the blocks'
     factorials."""
@@ -363,13 +393,13 @@ def test_block_walk_large_closed_forms(native, n):
         sizes[-2] -= 1
         sizes[-1] = 2
     A = _block_ones(sizes, rng)
-    primes = BINARY_PRIMES[:2]
+    primes = BINARY_PRIMES[:3]
     plan = native.Batch([A], [primes])
     try:
         st = plan.stats()
     finally:
         plan.close()
-    assert st["variant"] == 2.0 and st["skip_block_rows"] > 0
+    assert st["variant"] == (5.0 if block_exact else 2.0) and st["skip_block_rows"] > 0
     exact = math.prod(math.factorial(k) for k in sizes)
     print(f"block closed form n={n} sizes={sizes}", flush=True)
     assert native.permanent_residues(A, primes) == [exact % p for p in primes]
