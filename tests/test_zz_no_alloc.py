@@ -49,6 +49,13 @@ def test_noalloc_matches_the_allocating_entry_point(native):
     s2, got = _call(lib, "phip_permanent_residues_noalloc", D, [2147483647, 2147483629])
     assert s2 == 0
     assert got == [oracle.C.permanent_mod_p(D.tolist(), p, 4) for p in (2147483647, 2147483629)]
+    # Five primes: the block walk with exact accumulation (limb columns in
+    # the item slots), against the allocating entry point.
+    five = [2147483647, 2147483629, 2147483587, 2147483579, 2147483563]
+    s1, want = _call(lib, "phip_permanent_residues", D, five)
+    s2, got = _call(lib, "phip_permanent_residues_noalloc", D, five)
+    assert s1 == s2 == 0 and got == want
+    assert got[:2] == [oracle.C.permanent_mod_p(D.tolist(), p, 4) for p in five[:2]]
 
 
 def test_noalloc_refusals(native):
