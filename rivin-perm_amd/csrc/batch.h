@@ -253,9 +253,8 @@ PHIP_DECLARE_BATCH_LAUNCH_SKIP(6, 2)
 PHIP_DECLARE_BATCH_LAUNCH_SKIP(6, 4)
 PHIP_DECLARE_BATCH_LAUNCH_SKIP(6, 8)
 #undef PHIP_DECLARE_BATCH_LAUNCH_SKIP
-// Block zero-skipping walks with exact accumulation (8 limb columns).
-hipError_t launch_batch_g4_k8_skipexact(dim3, unsigned, hipStream_t, const BatchJob*, const uint2*, uint32_t,
-                                        uint32_t*, const int32_t*, const int32_t*, const uint32_t*, uint64_t*);
+// The block zero-skipping walk with exact accumulation (groups of 6, 8 limb
+// columns).
 hipError_t launch_batch_g6_k8_skipexact(dim3, unsigned, hipStream_t, const BatchJob*, const uint2*, uint32_t,
                                         uint32_t*, const int32_t*, const int32_t*, const uint32_t*, uint64_t*);
 // Exact-accumulation variants (any number of primes; L = K limb columns).
@@ -303,7 +302,7 @@ PHIP_DECLARE_BATCH_OCC(occupancy_batch_g2_k10_exact) PHIP_DECLARE_BATCH_OCC(occu
 PHIP_DECLARE_BATCH_OCC(occupancy_batch_g3_k10_exact) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g3_k12_exact)
 PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k10_exact) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k12_exact)
 PHIP_DECLARE_BATCH_OCC(occupancy_batch_g2_k10_compact)
-PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k8_skipexact) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g6_k8_skipexact)
+PHIP_DECLARE_BATCH_OCC(occupancy_batch_g6_k8_skipexact)
 #undef PHIP_DECLARE_BATCH_OCC
 
 }  // namespace phip

@@ -144,7 +144,6 @@ BatchLaunchFn launcher_for(int group, int set, int variant) {
         if (group == 4 && set == 12) return launch_batch_g4_k12_exact;
     }
     if (variant == kVariantExactCompact && group == 2 && set == 10) return launch_batch_g2_k10_compact;
-    if (variant == kVariantSkipExact && group == 4 && set == 8) return launch_batch_g4_k8_skipexact;
     if (variant == kVariantSkipExact && group == 6 && set == 8) return launch_batch_g6_k8_skipexact;
 #undef PHIP_LAUNCHER
 #undef PHIP_LAUNCHER_PAIRED
@@ -181,7 +180,6 @@ int occupancy_for(int group, int set, int variant, unsigned lds) {
         if (group == 4 && set == 12) return occupancy_batch_g4_k12_exact(lds);
     }
     if (variant == kVariantExactCompact && group == 2 && set == 10) return occupancy_batch_g2_k10_compact(lds);
-    if (variant == kVariantSkipExact && group == 4 && set == 8) return occupancy_batch_g4_k8_skipexact(lds);
     if (variant == kVariantSkipExact && group == 6 && set == 8) return occupancy_batch_g6_k8_skipexact(lds);
 #undef PHIP_OCC
 #undef PHIP_OCC_PAIRED
@@ -604,10 +602,13 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
         // walk for all of them (exact_columns): with eight groups the walk
         // costs about as much as the modular walk of a set of six primes,
         // and the residues come from its exact sums.
-        // A 0/1 block table of at most eight groups with enough primes runs
-        // the block walk once with exact accumulation (a taken state costs
-        // its limb product instead of a Montgomery tree per prime).
-        const bool skip_exact = st.skip_zeros && st.block_rows >= 0 && !st.mod3 &&
+        // A 0/1 block table of at most eight groups of 6 rows with enough
+        // primes runs the block walk once with exact accumulation (a taken
+        // state costs its limb product instead of a Montgomery tree per
+        // prime).  Block tables fall back to groups of 4 only above order
+        // 35 (group_fits on degrees up to 35 holds for 6), i.e. with more
+        // than eight groups.
+        const bool skip_exact = st.skip_zeros && st.block_rows >= 0 && !st.mod3 && st.group == 6 &&
                                 groups_of(st.n, st.group) <= kExactMaxGroups && tuning().skip_exact_min_primes > 0 &&
                                 st.inputs.size() >= (size_t)tuning().skip_exact_min_primes;
         const int columns = skip_exact ? kExactColumns
@@ -739,7 +740,7 @@ int assign_items(PlanHost& out, int items_log2, int min_cbits) {
     // Items: per (group, set, variant) class, jobs by descending cost, each
     // job's items contiguous.
     std::vector<bool> placed(out.jobs.size(), false);
-    static const int kClasses[][3] = {{4, 8, 5}, {6, 8, 5},
+    static const int kClasses[][3] = {{6, 8, 5},
                                       {2, 8, 3}, {3, 8, 3}, {4, 8, 3}, {6, 8, 3}, {2, 10, 4},
                                       {2, 10, 3}, {2, 12, 3}, {3, 10, 3}, {3, 12, 3}, {4, 10, 3}, {4, 12, 3},
                                       {0, 1, 0},

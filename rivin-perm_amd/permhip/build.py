@@ -32,8 +32,10 @@ ARCH = "gfx950"
 # of 4 or 6 the zero-skipping variant (2) for sets of 2, 4 and 8, and every
 # group the exact-accumulation variant (3, eight limb columns per item), and
 # groups of 2 rows its compact form (4: the product tree sized for 22-bit
-# group values, ten limb columns), and 0/1 block tables in groups of 4 or 6
-# the block walk with exact accumulation (5, eight limb columns).
+# group values, ten limb columns), and 0/1 block tables in groups of 6
+# the block walk with exact accumulation (5, eight limb columns; groups of 6
+# only: a block table falls back to groups of 4 only above order 35, where
+# it has more than eight groups).
 BATCH_VARIANT_SUFFIX = {0: "", 1: "_paired", 2: "_skip", 3: "_exact", 4: "_compact", 5: "_skipexact"}
 BATCH_CLASSES = (
     ((0, 1, 0),)
@@ -43,7 +45,7 @@ BATCH_CLASSES = (
     + tuple((g, 8, 3) for g in (2, 3, 4, 6))
     + tuple((g, k, 3) for g in (2, 3, 4) for k in (10, 12))
     + ((2, 10, 4),)
-    + tuple((g, 8, 5) for g in (4, 6))
+    + ((6, 8, 5),)
 )
 PLAIN_SOURCES = (
     "device_context.hip",
