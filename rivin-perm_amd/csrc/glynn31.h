@@ -1179,12 +1179,13 @@ struct BlockSkipWalk {
 };
 
 // glynn_chunk_set_skip's block form on block tables (build_block_tables):
-// the lane's 2^cbits states (cbits >= kBlockMinChunkBits), the residues of
-// the K primes in out (only the wavefront's total is meaningful, as there).
-// `region` is the wavefront's LDS region (block_region_words(n, rows)
-// words); with `reload` the mask and inner tables are copied into it first
-// (a wavefront keeps them while it stays on one job).  Every lane of the
-// wavefront must call it.
+// walks the lane's 2^cbits states (cbits >= kBlockMinChunkBits) into
+// walk.acc (the K primes' Montgomery sums, or the exact limb columns; only
+// the wavefront's total is meaningful, as there).  `region` is the
+// wavefront's LDS region (block_region_words(n, rows) words); with `reload`
+// the mask and inner tables are copied into it first (a wavefront keeps
+// them while it stays on one job).  Every lane of the wavefront must call
+// it.
 template <int NR, int G, int K, bool kMod3, bool kExact>
 __device__ __forceinline__ void block_walk_chunk(BlockSkipWalk<NR, G, K, kMod3, kExact>& walk,
                                                  const int32_t* __restrict__ tables, int n, int cbits, uint64_t base,
@@ -1249,6 +1250,7 @@ __device__ __forceinline__ void block_walk_chunk(BlockSkipWalk<NR, G, K, kMod3, 
     }
 }
 
+// The residues of the K primes in out (each lane's Montgomery sum reduced).
 template <int NR, int G, int K, bool kMod3 = false>
 __device__ __forceinline__ void glynn_chunk_set_blocks(const int32_t* __restrict__ tables, int n, int cbits,
                                                        uint64_t base, bool active, int32_t* region, int rows,
