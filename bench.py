@@ -85,6 +85,21 @@ EXACT_MIX_N32_G4 = {
     "other": 5.3,
 }  # 137.5 in all; rocprof counts 133.1 per state (SQ_INSTS_VALU)
 
+# The signed-limb form of the same walk (variant 6: signed group products
+# multiplied in signed limbs, no magnitudes or sign per state), as counted
+# in the compiled loop.
+EXACT_SIGNED_MIX_N32_G4 = {
+    "v_pk_add_u16/v_pk_sub_i16": 14,
+    "v_mul_i32_i24(_sdwa)": 24,
+    "v_mad_i64_i32 (20 signed limb products, 8 accumulations)": 28,
+    "v_mad_u64_u32 (unsigned limb columns)": 8,
+    "v_and_b32": 18,
+    "v_ashrrev_i64": 7,
+    "v_lshrrev_b64": 4,
+    "v_alignbit_b32": 7,
+    "other": 1,
+}  # 111 in all
+
 
 def instruction_mix(n: int, group: int, k: int = 1, paired: bool = False) -> dict:
     """VALU instructions per Gray state of one walk of gray_walk<n, group>
@@ -721,8 +736,10 @@ def main() -> int:
     st = plan.stats()
     group, kset, variant = int(st["group"]), int(st["set"]), int(st.get("variant", 0))
     cus = int(re.search(r"CUs=(\d+)", info).group(1)) if re.search(r"CUs=(\d+)", info) else 256
-    exact_variant = variant in (3, 4)
-    if exact_variant:
+    exact_variant = variant in (3, 4, 6)
+    if variant == 6:
+        mix = dict(EXACT_SIGNED_MIX_N32_G4)
+    elif exact_variant:
         mix = dict(EXACT_MIX_N32_G4)
     else:
         mix = instruction_mix(HEADLINE_N, group, kset, variant == 1)
@@ -744,11 +761,12 @@ def main() -> int:
         # Exact accumulation: no modular arithmetic in the walk; each state's
         # product of 8 exact group products is formed in radix-2^31 limbs
         # (28 multiply-adds) and added to 8 int64 columns, for all primes.
+        limb_mads = 28  # 4 pair products, two 2 x 2 and one 4 x 4 limb products, either form
         executed.update({
-            "variant": "exact accumulation",
+            "variant": "exact accumulation, signed limbs" if variant == 6 else "exact accumulation",
             "montgomery_reductions_per_state": 0,
-            "limb_multiply_adds_per_state": mix["v_mad_u64_u32"],
-            "limb_multiply_adds_per_s": mix["v_mad_u64_u32"] * states_per_launch / (k_ms * 1e-3),
+            "limb_multiply_adds_per_state": limb_mads,
+            "limb_multiply_adds_per_s": limb_mads * states_per_launch / (k_ms * 1e-3),
         })
     else:
         # One Montgomery reduction is one v_mul_lo_u32 of the mix.

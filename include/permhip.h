@@ -290,6 +290,10 @@ int phip_batch_destroy(void *plan);
                                       limb product per taken state for all primes instead of a Montgomery
                                       tree per prime of a padded set of 4 or 8).  Read when a plan is
                                       built */
+#define PHIP_TUNE_EXACT_SIGNED 16  /* 1 (default): exact-accumulation tables of at most eight groups of 3 or 4
+                                      rows whose four-group bound products are below 2^122 multiply their
+                                      signed group products in signed limbs (no magnitudes or sign per
+                                      state); 0: the magnitude classes.  Read when a plan is built */
 int phip_set_tuning(int key, int64_t value);
 int64_t phip_get_tuning(int key);
 

@@ -27,6 +27,12 @@ constexpr int kVariantSkipZeros = 2;  // zero terms skipped through a per-wavefr
 constexpr int kVariantExact = 3;      // exact accumulation: one walk for any number of primes
 constexpr int kVariantExactCompact = 4;  // the same with the product tree sized for kCompactValueBits-bit values
 constexpr int kVariantSkipExact = 5;  // the block zero-skipping walk with exact accumulation (kExactColumns columns)
+constexpr int kVariantExactSigned = 6;  // exact accumulation of signed group products in signed limbs (8 columns)
+// Value-bits parameter of the exact sink that selects signed limbs.
+constexpr int kSignedLimbs = 0;
+// Bound of a four-group product for the signed-limb class (top limbs of
+// the four-limb operands below 2^29).
+constexpr int kSignedQuadBits = 122;
 // The exact variant's classes: a class's set size is its number of limb
 // columns L.  L = 8 takes tables of at most eight groups (full products);
 // L = 10 and 12 take up to sixteen groups when the product of the table's
@@ -41,7 +47,8 @@ constexpr int kExactMaxColumns = 12;
 // row sums of entries up to about 100 at orders 24..28 are near 2^21).
 constexpr int kCompactValueBits = 22;
 constexpr bool is_exact_variant(int variant) {
-    return variant == kVariantExact || variant == kVariantExactCompact || variant == kVariantSkipExact;
+    return variant == kVariantExact || variant == kVariantExactCompact || variant == kVariantSkipExact ||
+           variant == kVariantExactSigned;
 }
 // Variants that walk 0/1 tables on the zero-skipping kernels (byte rows,
 // block tables, per-wavefront LDS regions).
@@ -62,7 +69,7 @@ struct BatchJob {
     int32_t nprimes;     // real primes (<= K); padded entries repeat the last one
     int32_t set;         // K of the job's launch class
     int32_t variant;     // kVariantPlain, kVariantPaired, kVariantSkipZeros, kVariantExact, kVariantExactCompact
-                         // or kVariantSkipExact
+                         // kVariantSkipExact or kVariantExactSigned
     int32_t test_words;  // kVariantSkipZeros: leading table words the zero test reads (skip_test_words)
     int32_t block_rows;  // kVariantSkipZeros, kVariantSkipExact: rows the 32-state block test reads
                          // (block_test_rows: a multiple of 8 or every row word; the rows of even degree lead
@@ -273,6 +280,11 @@ PHIP_DECLARE_BATCH_LAUNCH_EXACT(3, 12)
 PHIP_DECLARE_BATCH_LAUNCH_EXACT(4, 10)
 PHIP_DECLARE_BATCH_LAUNCH_EXACT(4, 12)
 #undef PHIP_DECLARE_BATCH_LAUNCH_EXACT
+// Signed-limb exact classes (groups of 3 or 4 rows, 8 limb columns).
+hipError_t launch_batch_g3_k8_signed(dim3, unsigned, hipStream_t, const BatchJob*, const uint2*, uint32_t, uint32_t*,
+                                     const int32_t*, const int32_t*, const uint32_t*, uint64_t*);
+hipError_t launch_batch_g4_k8_signed(dim3, unsigned, hipStream_t, const BatchJob*, const uint2*, uint32_t, uint32_t*,
+                                     const int32_t*, const int32_t*, const uint32_t*, uint64_t*);
 // The compact exact class (groups of 2 rows, 10 limb columns).
 hipError_t launch_batch_g2_k10_compact(dim3, unsigned, hipStream_t, const BatchJob*, const uint2*, uint32_t, uint32_t*,
                                        const int32_t*, const int32_t*, const uint32_t*, uint64_t*);
@@ -303,6 +315,7 @@ PHIP_DECLARE_BATCH_OCC(occupancy_batch_g3_k10_exact) PHIP_DECLARE_BATCH_OCC(occu
 PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k10_exact) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k12_exact)
 PHIP_DECLARE_BATCH_OCC(occupancy_batch_g2_k10_compact)
 PHIP_DECLARE_BATCH_OCC(occupancy_batch_g6_k8_skipexact)
+PHIP_DECLARE_BATCH_OCC(occupancy_batch_g3_k8_signed) PHIP_DECLARE_BATCH_OCC(occupancy_batch_g4_k8_signed)
 #undef PHIP_DECLARE_BATCH_OCC
 
 }  // namespace phip

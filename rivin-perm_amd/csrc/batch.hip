@@ -19,7 +19,9 @@
 // PHIP_BATCH_EXACT=1 the exact-accumulation variant (ExactSink), whose items
 // hold PHIP_BATCH_SET limb columns of the exact sums for all of a job's
 // primes, and PHIP_BATCH_COMPACT=1 (with it) the form whose product tree is
-// sized for kCompactValueBits-bit group values (LimbProduct).  Both
+// sized for kCompactValueBits-bit group values (LimbProduct), and
+// PHIP_BATCH_SIGNED=1 (with it, PHIP_BATCH_SET=8) the signed-limb form
+// (slimb_product8).  Both
 // PHIP_BATCH_SKIP_ZEROS=1 and PHIP_BATCH_EXACT=1 (with PHIP_BATCH_SET=8)
 // select the block zero-skipping walk with exact accumulation
 // (glynn_chunk_blocks_exact): 0/1 block tables of at most eight groups.
@@ -47,11 +49,18 @@
 #ifndef PHIP_BATCH_COMPACT
 #define PHIP_BATCH_COMPACT 0
 #endif
+#ifndef PHIP_BATCH_SIGNED
+#define PHIP_BATCH_SIGNED 0
+#endif
+#if PHIP_BATCH_SIGNED && (!PHIP_BATCH_EXACT || PHIP_BATCH_COMPACT || PHIP_BATCH_SKIP_ZEROS || PHIP_BATCH_SET != 8)
+#error "the signed-limb classes are 8-column exact-accumulation classes"
+#endif
 #if PHIP_BATCH_COMPACT && !PHIP_BATCH_EXACT
 #error "the compact classes are exact-accumulation classes"
 #endif
 // Width of the values the exact classes' product trees are sized for.
-constexpr int kBatchValueBits = PHIP_BATCH_COMPACT ? phip::kCompactValueBits : 31;
+constexpr int kBatchValueBits =
+    PHIP_BATCH_SIGNED ? phip::kSignedLimbs : (PHIP_BATCH_COMPACT ? phip::kCompactValueBits : 31);
 
 namespace phip {
 
@@ -244,6 +253,8 @@ __global__ __launch_bounds__(256, 2) void batch_glynn_kernel(const BatchJob* __r
 #endif
 #if PHIP_BATCH_COMPACT
 #define PHIP_PAIRED_SUFFIX _compact
+#elif PHIP_BATCH_SIGNED
+#define PHIP_PAIRED_SUFFIX _signed
 #else
 #define PHIP_PAIRED_SUFFIX _exact
 #endif

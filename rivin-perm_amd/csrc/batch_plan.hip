@@ -145,6 +145,8 @@ BatchLaunchFn launcher_for(int group, int set, int variant) {
     }
     if (variant == kVariantExactCompact && group == 2 && set == 10) return launch_batch_g2_k10_compact;
     if (variant == kVariantSkipExact && group == 6 && set == 8) return launch_batch_g6_k8_skipexact;
+    if (variant == kVariantExactSigned && group == 3 && set == 8) return launch_batch_g3_k8_signed;
+    if (variant == kVariantExactSigned && group == 4 && set == 8) return launch_batch_g4_k8_signed;
 #undef PHIP_LAUNCHER
 #undef PHIP_LAUNCHER_PAIRED
 #undef PHIP_LAUNCHER_SKIP
@@ -181,6 +183,8 @@ int occupancy_for(int group, int set, int variant, unsigned lds) {
     }
     if (variant == kVariantExactCompact && group == 2 && set == 10) return occupancy_batch_g2_k10_compact(lds);
     if (variant == kVariantSkipExact && group == 6 && set == 8) return occupancy_batch_g6_k8_skipexact(lds);
+    if (variant == kVariantExactSigned && group == 3 && set == 8) return occupancy_batch_g3_k8_signed(lds);
+    if (variant == kVariantExactSigned && group == 4 && set == 8) return occupancy_batch_g4_k8_signed(lds);
 #undef PHIP_OCC
 #undef PHIP_OCC_PAIRED
 #undef PHIP_OCC_SKIP
@@ -299,6 +303,18 @@ bool compact_nodes_fit(const std::vector<uint64_t>& S, int G, int lo, int hi, in
     }
     const int mid = lo + limb_split(count);
     return compact_nodes_fit(S, G, lo, mid, L) && compact_nodes_fit(S, G, mid, hi, L);
+}
+
+// Whether a table of at most eight groups of G rows (row bounds S in kernel
+// order) fits the signed-limb exact classes: the product of the row bounds
+// of groups [0, 4) and of groups [4, 8) each below 2^kSignedQuadBits.
+bool signed_quads_fit(const std::vector<uint64_t>& S, int G) {
+    const int n = (int)S.size();
+    for (int lo = 0; lo < n; lo += 4 * G) {
+        const std::vector<uint64_t> rows(S.begin() + lo, S.begin() + std::min(n, lo + 4 * G));
+        if (bound_bits(rows) > kSignedQuadBits) return false;
+    }
+    return true;
 }
 
 // Estimated share of the Gray states whose term is nonzero, for a 0/1
@@ -446,6 +462,7 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
         uint64_t data_off;
         bool skip_zeros;
         bool compact;    // groups of 2 rows whose partial bound products fit the compact exact classes
+        bool signed_ok;  // groups of 3 or 4 rows whose four-group bound products fit the signed-limb classes
         int test_words;  // skip_zeros: the walk's zero-test width (skip_plan)
         int block_rows;  // skip_zeros: rows the block walk's test reads, or -1 (per-state test)
         bool mod3;       // the modulo-3 block walk
@@ -585,7 +602,14 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
                     compact = compact_nodes_fit(kernel_bounds, group, 0, groups, 10);
                 }
             }
-            sets.push_back({x.key, x.n, group, (uint64_t)out.blob.size(), skip, compact, test_words, block_rows,
+            bool signed_ok = false;
+            if ((group == 3 || group == 4) && !skip && tuning().exact_signed &&
+                groups_of(x.n, group) <= kExactMaxGroups) {
+                std::vector<uint64_t> kernel_bounds(x.n);
+                for (int r = 0; r < x.n; ++r) kernel_bounds[r] = bounds[order.empty() ? r : order[r]];
+                signed_ok = signed_quads_fit(kernel_bounds, group);
+            }
+            sets.push_back({x.key, x.n, group, (uint64_t)out.blob.size(), skip, compact, signed_ok, test_words, block_rows,
                             x.mod3, group > 0 ? bound_bits(bounds) : 0, {}});
             sets_of_key[x.key].push_back(found);
             out.blob.insert(out.blob.end(), table.begin(), table.end());
@@ -623,8 +647,10 @@ int build_plan_host(const std::vector<PlanInput>& in, int items_log2, int min_cb
             jb.set = columns;
             jb.nprimes = (int32_t)st.inputs.size();
             jb.prime_off = (uint32_t)out.set_p.size();
-            jb.variant = skip_exact ? kVariantSkipExact
-                                    : ((columns == 10 && st.compact) ? kVariantExactCompact : kVariantExact);
+            jb.variant = skip_exact                     ? kVariantSkipExact
+                         : (columns == 10 && st.compact) ? kVariantExactCompact
+                         : (columns == 8 && st.signed_ok) ? kVariantExactSigned
+                                                          : kVariantExact;
             jb.block_rows = skip_exact ? st.block_rows : -1;
             jb.test_words = skip_exact ? st.test_words : 0;
             jb.block_mod3 = 0;
@@ -740,7 +766,7 @@ int assign_items(PlanHost& out, int items_log2, int min_cbits) {
     // Items: per (group, set, variant) class, jobs by descending cost, each
     // job's items contiguous.
     std::vector<bool> placed(out.jobs.size(), false);
-    static const int kClasses[][3] = {{6, 8, 5},
+    static const int kClasses[][3] = {{6, 8, 5}, {3, 8, 6}, {4, 8, 6},
                                       {2, 8, 3}, {3, 8, 3}, {4, 8, 3}, {6, 8, 3}, {2, 10, 4},
                                       {2, 10, 3}, {2, 12, 3}, {3, 10, 3}, {3, 12, 3}, {4, 10, 3}, {4, 12, 3},
                                       {0, 1, 0},

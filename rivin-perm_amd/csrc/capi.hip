@@ -162,6 +162,10 @@ int phip_set_tuning(int key, int64_t value) {
             if (value < 0 || value > 1) break;
             t.mod3_blocks = (int)value;
             return 0;
+        case PHIP_TUNE_EXACT_SIGNED:
+            if (value < 0 || value > 1) break;
+            t.exact_signed = (int)value;
+            return 0;
         case PHIP_TUNE_SKIP_EXACT_MIN_PRIMES:
             if (value < 0 || value > 1024) break;
             t.skip_exact_min_primes = (int)value;
@@ -192,6 +196,7 @@ int64_t phip_get_tuning(int key) {
         case PHIP_TUNE_SKIP_BLOCKS: return t.skip_blocks;
         case PHIP_TUNE_MOD3_BLOCKS: return t.mod3_blocks;
         case PHIP_TUNE_SKIP_EXACT_MIN_PRIMES: return t.skip_exact_min_primes;
+        case PHIP_TUNE_EXACT_SIGNED: return t.exact_signed;
         default:
             phip::set_error("unknown tuning key");
             return -1;

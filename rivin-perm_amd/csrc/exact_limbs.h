@@ -1,6 +1,7 @@
 // Exact products of 31-bit magnitudes in radix-2^31 limbs, for the
 // exact-accumulation Glynn sink (glynn31.h, ExactSink): eight values in
-// eight limbs, and up to sixteen values modulo 2^(31 L).
+// eight limbs, and up to sixteen values modulo 2^(31 L); and eight signed
+// values in eight signed limbs (slimb_product8).
 //
 // A value is an array of limbs l[0..k), value = sum l[i] 2^(31 i), every
 // limb in [0, 2^31).  Multiplying two such arrays is schoolbook with 64-bit
@@ -113,6 +114,105 @@ PHIP_LIMB_FN void limb_product8(const uint32_t (&m)[8], uint32_t (&out)[8]) {
     limb_mul_2x2(p0, p1, x);
     limb_mul_2x2(p2, p3, y);
     limb_mul_4x4(x, y, out);
+}
+
+// Signed limbs: value = sum l[i] 2^(31 i) with l[0..k-1) in [0, 2^31) and
+// the top limb signed (the floor of value / 2^(31 (k-1))), so a product of
+// signed values needs no magnitudes and no sign.  Each signed multiply-add
+// is one v_mad_i64_i32 on the device, into a signed 64-bit column word that
+// must stay within (-2^63, 2^63): a product of two limbs (top limbs signed)
+// is below 2^62 in magnitude, so two of them plus a carry below 2^33 fit.
+PHIP_LIMB_FN int64_t slimb_mad(int32_t a, int32_t b, int64_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    int64_t r;
+    uint64_t carry;
+    asm("v_mad_i64_i32 %0, %1, %2, %3, %4" : "=v"(r), "=s"(carry) : "v"(a), "v"(b), "v"(c));
+    return r;
+#else
+    return (int64_t)a * b + c;
+#endif
+}
+
+PHIP_LIMB_FN int64_t slimb_mul(int32_t a, int32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    int64_t r;
+    uint64_t carry;
+    asm("v_mad_i64_i32 %0, %1, %2, %3, 0" : "=v"(r), "=s"(carry) : "v"(a), "v"(b));
+    return r;
+#else
+    return (int64_t)a * b;
+#endif
+}
+
+// a * b for |a|, |b| < 2^31: two signed limbs.
+PHIP_LIMB_FN void slimb_mul_1x1(int32_t a, int32_t b, int32_t (&out)[2]) {
+    const int64_t t = slimb_mul(a, b);
+    out[0] = (int32_t)((uint32_t)t & kLimbMask);
+    out[1] = (int32_t)(t >> 31);
+}
+
+// Two-limb signed values (|value| < 2^62) to four limbs: every column holds
+// at most two products and a carry below 2^32.  The top limb is the floor of
+// the product / 2^93, in [-2^31, 2^31) since |product| < 2^124.
+PHIP_LIMB_FN void slimb_mul_2x2(const int32_t (&a)[2], const int32_t (&b)[2], int32_t (&out)[4]) {
+    int64_t t = slimb_mul(a[0], b[0]);
+    out[0] = (int32_t)((uint32_t)t & kLimbMask);
+    t = slimb_mad(a[0], b[1], t >> 31);
+    t = slimb_mad(a[1], b[0], t);
+    out[1] = (int32_t)((uint32_t)t & kLimbMask);
+    t = slimb_mad(a[1], b[1], t >> 31);
+    out[2] = (int32_t)((uint32_t)t & kLimbMask);
+    out[3] = (int32_t)(t >> 31);
+}
+
+// Four-limb signed values to eight limbs, for |a|, |b| < 2^122 (top limbs
+// below 2^29 in magnitude; the planner checks the bound of every four-group
+// product).  Columns of unsigned limbs only (0..2, and the two unsigned
+// products of column 3) run unsigned as limb_mul_4x4 does; column 3 is split
+// before its two signed products (each below 2^60), so every signed column
+// word stays below 2^62 + 2^61 + 2^33 in magnitude.
+PHIP_LIMB_FN void slimb_mul_4x4(const int32_t (&a)[4], const int32_t (&b)[4], int32_t (&out)[8]) {
+    const uint32_t a0 = (uint32_t)a[0], a1 = (uint32_t)a[1], a2 = (uint32_t)a[2];
+    const uint32_t b0 = (uint32_t)b[0], b1 = (uint32_t)b[1], b2 = (uint32_t)b[2];
+    uint64_t u = limb_mul(a0, b0);
+    out[0] = (int32_t)((uint32_t)u & kLimbMask);
+    u = limb_mad(a0, b1, u >> 31);
+    u = limb_mad(a1, b0, u);
+    out[1] = (int32_t)((uint32_t)u & kLimbMask);
+    u = limb_mad(a0, b2, u >> 31);
+    u = limb_mad(a1, b1, u);
+    u = limb_mad(a2, b0, u);
+    out[2] = (int32_t)((uint32_t)u & kLimbMask);
+    u = limb_mad(a1, b2, u >> 31);
+    u = limb_mad(a2, b1, u);
+    const int64_t spill = (int64_t)(u >> 31);
+    int64_t t = (int64_t)((uint32_t)u & kLimbMask);
+    t = slimb_mad(a[0], b[3], t);
+    t = slimb_mad(a[3], b[0], t);
+    out[3] = (int32_t)((uint32_t)t & kLimbMask);
+    t = slimb_mad(a[1], b[3], (t >> 31) + spill);
+    t = slimb_mad(a[3], b[1], t);
+    t = slimb_mad(a[2], b[2], t);
+    out[4] = (int32_t)((uint32_t)t & kLimbMask);
+    t = slimb_mad(a[2], b[3], t >> 31);
+    t = slimb_mad(a[3], b[2], t);
+    out[5] = (int32_t)((uint32_t)t & kLimbMask);
+    t = slimb_mad(a[3], b[3], t >> 31);
+    out[6] = (int32_t)((uint32_t)t & kLimbMask);
+    out[7] = (int32_t)(t >> 31);
+}
+
+// prod g[0..8) in eight signed limbs, for |g[i]| < 2^31 whose four-value
+// products g[0..4) and g[4..8) are below 2^122 in magnitude.
+PHIP_LIMB_FN void slimb_product8(const int32_t (&g)[8], int32_t (&out)[8]) {
+    int32_t p0[2], p1[2], p2[2], p3[2], x[4], y[4];
+    slimb_mul_1x1(g[0], g[1], p0);
+    slimb_mul_1x1(g[2], g[3], p1);
+    slimb_mul_1x1(g[4], g[5], p2);
+    slimb_mul_1x1(g[6], g[7], p3);
+    slimb_mul_2x2(p0, p1, x);
+    slimb_mul_2x2(p2, p3, y);
+    slimb_mul_4x4(x, y, out);
 }
 
 // (a * b) mod 2^(31 LO) in LO limbs, for a of LA limbs and b of LB limbs,

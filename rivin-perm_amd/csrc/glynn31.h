@@ -573,10 +573,17 @@ struct PrimeSetSink {
 // values of about V bits (LimbProduct); the planner sends a table there only
 // when every node's bound product fits its limbs.
 //
-// Every limb is below 2^31, so after 2^cbits states |acc[c]| < 2^(31 + cbits).
+// V = kSignedLimbs (L = 8) multiplies the signed group products in signed
+// limbs (slimb_product8): no magnitudes and no sign per state, for tables
+// whose four-group products are below 2^122 (the planner checks their bound
+// products).
+//
+// Every limb is below 2^31 in magnitude, so after 2^cbits states
+// |acc[c]| < 2^(31 + cbits).
 template <int NR, int G, int L, int V = 31>
 struct ExactSink {
     static_assert(G > 0, "exact accumulation needs exact packed row sums");
+    static_assert(V != kSignedLimbs || L == 8, "signed limbs take eight groups in eight columns");
     using Rows = PackedRows<NR, G>;
     static constexpr int kGroups = Rows::kGroups;
     static_assert(L == 8 ? kGroups <= 8 : kGroups <= 16, "too many groups for the exact sink");
@@ -586,6 +593,18 @@ struct ExactSink {
     __device__ __forceinline__ void state(const Rows& rs) {
         int32_t g[kGroups];
         rs.groups(g, std::make_integer_sequence<int, kGroups>{});
+        if constexpr (V == kSignedLimbs) {
+            // Signed limbs: the group products as they are, the term's sign
+            // only the state's.
+            int32_t gv[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) gv[k] = k < kGroups ? g[k] : 1;
+            int32_t z[8];
+            slimb_product8(gv, z);
+#pragma unroll
+            for (int c = 0; c < 8; ++c) acc[c] = mac_sign<ODD ? -1 : 1>(acc[c], z[c]);
+            return;
+        }
         constexpr int kValues = L == 8 ? 8 : kGroups;
         uint32_t m[kValues];
 #pragma unroll

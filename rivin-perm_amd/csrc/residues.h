@@ -29,6 +29,7 @@ struct Tuning {
     int exact_compact = 1;          // groups of 2 rows may take the compact exact classes (0 off)
     int skip_blocks = 1;            // zero-skipping tables test 32 Gray states per row lookup (0: per state)
     int mod3_blocks = 1;            // permanents modulo 3 of orders 8..46 on the modulo-3 block walk (0: bit-sliced)
+    int exact_signed = 1;           // groups of 3 or 4 rows may take the signed-limb exact classes (0 off)
     int skip_exact_min_primes = 5;  // fewest primes of a 0/1 block table of <= 8 groups on the exact block walk (0 off)
 };
 Tuning& tuning();

@@ -117,6 +117,7 @@ TUNE_EXACT_COMPACT = 12
 TUNE_SKIP_BLOCKS = 13
 TUNE_MOD3_BLOCKS = 14
 TUNE_SKIP_EXACT_MIN_PRIMES = 15
+TUNE_EXACT_SIGNED = 16
 BATCH_BLOCKS_PER_CU = 1
 BATCH_USE_GRAPH = 2
 BATCH_ITEMS_LOG2 = 3

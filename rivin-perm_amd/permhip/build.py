@@ -35,8 +35,9 @@ ARCH = "gfx950"
 # group values, ten limb columns), and 0/1 block tables in groups of 6
 # the block walk with exact accumulation (5, eight limb columns; groups of 6
 # only: a block table falls back to groups of 4 only above order 35, where
-# it has more than eight groups).
-BATCH_VARIANT_SUFFIX = {0: "", 1: "_paired", 2: "_skip", 3: "_exact", 4: "_compact", 5: "_skipexact"}
+# it has more than eight groups), and groups of 3 or 4 rows the signed-limb
+# exact form (6, eight limb columns).
+BATCH_VARIANT_SUFFIX = {0: "", 1: "_paired", 2: "_skip", 3: "_exact", 4: "_compact", 5: "_skipexact", 6: "_signed"}
 BATCH_CLASSES = (
     ((0, 1, 0),)
     + tuple((g, k, 0) for g in (2, 3, 4, 6) for k in (1, 2, 4, 8))
@@ -46,6 +47,7 @@ BATCH_CLASSES = (
     + tuple((g, k, 3) for g in (2, 3, 4) for k in (10, 12))
     + ((2, 10, 4),)
     + ((6, 8, 5),)
+    + tuple((g, 8, 6) for g in (3, 4))
 )
 PLAIN_SOURCES = (
     "device_context.hip",
@@ -108,8 +110,8 @@ def build(jobs: int | None = None, verbose: bool = False) -> Path:
                 CSRC / "batch.hip",
                 BUILD_DIR / f"batch_g{group}_k{k}{BATCH_VARIANT_SUFFIX[variant]}.o",
                 [f"-DPHIP_BATCH_GROUP={group}", f"-DPHIP_BATCH_SET={k}", f"-DPHIP_BATCH_PAIRED={int(variant == 1)}",
-                 f"-DPHIP_BATCH_SKIP_ZEROS={int(variant in (2, 5))}", f"-DPHIP_BATCH_EXACT={int(variant in (3, 4, 5))}",
-                 f"-DPHIP_BATCH_COMPACT={int(variant == 4)}"],
+                 f"-DPHIP_BATCH_SKIP_ZEROS={int(variant in (2, 5))}", f"-DPHIP_BATCH_EXACT={int(variant in (3, 4, 5, 6))}",
+                 f"-DPHIP_BATCH_COMPACT={int(variant == 4)}", f"-DPHIP_BATCH_SIGNED={int(variant == 6)}"],
             )
         )
     for name in PLAIN_SOURCES:

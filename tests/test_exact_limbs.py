@@ -241,3 +241,97 @@ def test_compact_products_match_python(compact_harness):
         elif value != prod % (1 << (31 * limbs)):
             mismatches += 1
     assert mismatches > 0
+
+
+SIGNED_HARNESS = r"""
+#include <cstdio>
+#include <cstdint>
+#include "exact_limbs.h"
+using namespace phip;
+int main() {
+    long long v[8];
+    while (std::scanf("%lld %lld %lld %lld %lld %lld %lld %lld", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6],
+                      &v[7]) == 8) {
+        int32_t g[8], out[8], p0[2], p1[2], x[4];
+        for (int i = 0; i < 8; ++i) g[i] = (int32_t)v[i];
+        slimb_product8(g, out);
+        slimb_mul_1x1(g[0], g[1], p0);
+        slimb_mul_1x1(g[2], g[3], p1);
+        slimb_mul_2x2(p0, p1, x);
+        for (int i = 0; i < 8; ++i) std::printf("%d ", out[i]);
+        for (int i = 0; i < 4; ++i) std::printf("%d ", x[i]);
+        std::printf("\n");
+    }
+    return 0;
+}
+"""
+
+
+def _signed_value(limbs):
+    return sum(v << (31 * i) for i, v in enumerate(limbs))
+
+
+def test_signed_limb_products_match_python(tmp_path):
+    """slimb_product8 (signed limbs, the signed-limb exact classes) against
+    Python integers: values below 2^31 in magnitude whose four-value
+    products stay below 2^122 (the planner's condition), at the edges
+    (every value +-(2^30.5), four-value products just below 2^122) and at
+    random; limbs 0..6 in [0, 2^31), the top limb in [-2^31, 2^31)."""
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    src = tmp_path / "s.cpp"
+    src.write_text(SIGNED_HARNESS)
+    exe = tmp_path / "s"
+    subprocess.run([gxx, "-O2", "-std=c++17", "-I", str(CSRC), str(src), "-o", str(exe)], check=True)
+    rng = random.Random(122)
+    edge = int(2 ** 30.5) - 1  # four of them: just below 2^122
+    cases = [[0] * 8, [1] * 8, [-1] * 8, [edge] * 8, [-edge] * 8, [edge, -edge] * 4, [(1 << 31) - 1] + [1] * 7,
+             [-(1 << 31) + 1] * 2 + [1] * 6]
+    for _ in range(3000):
+        bits = rng.randrange(1, 31)
+        row = [rng.randrange(-(1 << bits) + 1, 1 << bits) for _ in range(8)]
+        if rng.random() < 0.3:
+            row = [rng.choice((-1, 1)) * rng.choice((edge, edge - 1, 1 << 30, 1, 0)) for _ in range(8)]
+        cases.append(row)
+    for c in cases:
+        for q in (c[:4], c[4:]):
+            assert abs(q[0] * q[1] * q[2] * q[3]) < 1 << 122
+    text = "\n".join(" ".join(map(str, c)) for c in cases) + "\n"
+    out = subprocess.run([str(exe)], input=text, capture_output=True, text=True, check=True).stdout.split("\n")
+    for c, line in zip(cases, out):
+        vals = [int(v) for v in line.split()]
+        z, x = vals[:8], vals[8:]
+        assert all(0 <= v < (1 << 31) for v in z[:7] + x[:3]), c
+        assert -(1 << 31) <= z[7] < (1 << 31) and -(1 << 31) <= x[3] < (1 << 31), c
+        prod8 = 1
+        for v in c:
+            prod8 *= v
+        assert _signed_value(z) == prod8, c
+        assert _signed_value(x) == c[0] * c[1] * c[2] * c[3], c
+
+
+def test_planner_takes_signed_limbs_where_quads_fit():
+    """Host planner (no device): the headline table (groups of 4, eight
+    primes) takes the signed-limb class; tuning 0, or four-group bound
+    products of 2^123 (row bounds 208), keep the magnitude class."""
+    import numpy as np
+
+    from permhip import _native, planner, workloads
+
+    A = workloads.headline()
+    primes, _ = planner.generic_crt_primes(planner.matrix_permanent_bound(A.tolist()))
+    assert set(_native.batch_plan_classes([A], [primes])) == {(4, 8, 6)}
+    old = _native.get_tuning(_native.TUNE_EXACT_SIGNED)
+    assert old == 1
+    try:
+        _native.set_tuning(_native.TUNE_EXACT_SIGNED, 0)
+        assert set(_native.batch_plan_classes([A], [primes])) == {(4, 8, 3)}
+    finally:
+        _native.set_tuning(_native.TUNE_EXACT_SIGNED, old)
+    rng = np.random.default_rng(208)
+    mags = np.array([6] * 16 + [7] * 16)
+    B = np.array([rng.permutation(mags) * rng.choice((-1, 1), 32) for _ in range(32)], dtype=np.int64)
+    assert set(_native.batch_plan_classes([B], [primes])) == {(4, 8, 3)}
+    C = np.full((32, 32), 6, dtype=np.int64)
+    assert set(_native.batch_plan_classes([C], [primes])) == {(4, 8, 6)}

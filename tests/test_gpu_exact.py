@@ -2,7 +2,9 @@
 the exact group products formed in radix-2^31 limbs, one walk for every
 prime of a table) against the oracle, bit for bit, and against the modular
 kernels they replace (PHIP_TUNE_EXACT_MIN_PRIMES = 0): tables of at most
-eight groups (full products in 8 limbs) and of 9..16 groups (products kept
+eight groups (full products in 8 limbs; groups of 3 or 4 rows whose
+four-group bound products are below 2^122 in signed limbs, variant 6, else
+in magnitudes and a sign, variant 3) and of 9..16 groups (products kept
 modulo 2^(31 L), L = 10 or 12, exact while the row-bound product is below
 2^(31 L)).  Every case names its seed, order, entry range and prime
 count."""
@@ -43,10 +45,11 @@ def test_exact_matches_oracle_and_modular(native, oracle_c, n, bound):
     A = rng.integers(-bound, bound + 1, (n, n)).astype(np.int64)
     for k in (6, 8, 10, 16):
         primes = PRIMES[:k]
-        assert any(c[2] == 3 for c in _classes(native, A, primes)), (n, bound, k)
+        assert any(c[2] in (3, 6) for c in _classes(native, A, primes)), (n, bound, k)
         got = native.permanent_residues(A, primes)
         modular = _with_tuning(native, native.TUNE_EXACT_MIN_PRIMES, 0, lambda: native.permanent_residues(A, primes))
-        assert got == modular, (n, bound, k)
+        magnitudes = _with_tuning(native, native.TUNE_EXACT_SIGNED, 0, lambda: native.permanent_residues(A, primes))
+        assert got == modular == magnitudes, (n, bound, k)
         if n <= 20:
             assert got == [oracle_c.permanent_mod_p(A.tolist(), p, 8) for p in primes], (n, bound, k)
 
@@ -59,7 +62,7 @@ def test_exact_extreme_group_products(native, oracle_c):
         # Rows of equal entries: every state's row sums are +-(n-2)*a ... +-n*a.
         A = np.array([[rng.choice((-1, 1)) * 35 for _ in range(n)] for _ in range(n)], dtype=np.int64)
         primes = PRIMES[:8]
-        assert any(c[2] == 3 for c in _classes(native, A, primes))
+        assert any(c[2] in (3, 6) for c in _classes(native, A, primes))
         got = native.permanent_residues(A, primes)
         if n <= 20:
             assert got == [oracle_c.permanent_mod_p(A.tolist(), p, 8) for p in primes]
@@ -105,7 +108,7 @@ def test_exact_in_batches_with_modular_jobs(native, oracle_c):
     finally:
         plan.close()
     classes = native.batch_plan_classes(mats, moduli)
-    assert any(c[2] == 3 for c in classes) and any(c[2] != 3 for c in classes)
+    assert any(c[2] in (3, 6) for c in classes) and any(c[2] not in (3, 6) for c in classes)
     pos = 0
     for A, ps in zip(mats, moduli):
         want = [oracle_c.permanent_mod_p(A.tolist(), p, 8) for p in ps]
@@ -119,6 +122,44 @@ def test_headline_goes_exact(native):
 
     A = workloads.headline()
     primes, _ = planner.generic_crt_primes(planner.matrix_permanent_bound(A.tolist()))
+    assert _classes(native, A, primes) == {(4, 8, 6)}
+    got = native.permanent_residues(A, primes)
+    assert got == _with_tuning(native, native.TUNE_EXACT_MIN_PRIMES, 0, lambda: native.permanent_residues(A, primes))
+    assert got == _with_tuning(native, native.TUNE_EXACT_SIGNED, 0, lambda: native.permanent_residues(A, primes))
+
+
+@pytest.mark.parametrize("n,entry,variant", [(32, 6, 6), (24, 47, 6), (24, 48, 3)])
+def test_signed_limbs_at_the_quad_limit(native, n, entry, variant):
+    """Constant matrices c J_n with rows negated at random (perm = +-c^n n!),
+    whose first Gray state puts every row sum at its bound.  Order 32 in
+    groups of 4: 16 rows of bound 192 give four-group products up to
+    2^121.4, signed limbs.  Order 24 in groups of 3: 12 rows of bound
+    24 * 47 = 1128 give 2^121.7 (signed limbs), of bound 1152 give 2^122.04
+    (the magnitude class)."""
+    import math
+
+    primes = PRIMES[:8]
+    rng = np.random.default_rng(entry)
+    signs = rng.choice((-1, 1), n)
+    for flip in (False, True):
+        M = np.full((n, n), entry, dtype=np.int64)
+        if flip:
+            M = M * signs[:, None]
+        print(f"signed limit n={n} entry={entry} flip={flip}", flush=True)
+        assert _classes(native, M, primes) == {(4 if n == 32 else 3, 8, variant)}
+        exact = (int(np.prod(signs)) if flip else 1) * entry ** n * math.factorial(n)
+        assert native.permanent_residues(M, primes) == [exact % p for p in primes], (n, entry, flip)
+
+
+def test_signed_limbs_decline_large_quads(native):
+    """Order 32 in groups of 4 with row bounds 208 (sixteen entries of
+    magnitude 6 and sixteen of 7): groups below 2^31 but four-group bound
+    products of 2^123.2, so the magnitude class; its residues equal the
+    modular kernels'."""
+    rng = np.random.default_rng(208)
+    mags = np.array([6] * 16 + [7] * 16)
+    A = np.array([rng.permutation(mags) * rng.choice((-1, 1), 32) for _ in range(32)], dtype=np.int64)
+    primes = PRIMES[:8]
     assert _classes(native, A, primes) == {(4, 8, 3)}
     assert native.permanent_residues(A, primes) == _with_tuning(
         native, native.TUNE_EXACT_MIN_PRIMES, 0, lambda: native.permanent_residues(A, primes))

@@ -294,7 +294,7 @@ def test_paired_classes_are_planned(native):
     class; with it on, both run as one exact walk."""
     rng = np.random.default_rng(3)
     A = rng.integers(-10, 11, (20, 20)).astype(np.int64)
-    assert set(native.batch_plan_classes([A], [_primes_from(1 << 30, 8)])) == {(4, 8, 3)}
+    assert set(native.batch_plan_classes([A], [_primes_from(1 << 30, 8)])) == {(4, 8, 6)}
     old = native.get_tuning(native.TUNE_EXACT_MIN_PRIMES)
     native.set_tuning(native.TUNE_EXACT_MIN_PRIMES, 0)
     try:
