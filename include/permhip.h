@@ -216,7 +216,8 @@ int phip_batch_set_option(void *plan, int option, int64_t value);
  * launch: 0 plain, 1 pairs consecutive states (prime sets below
  * 1.25 * 2^30), 2 skips zero terms (0/1 tables whose states are mostly
  * zero), 3 exact accumulation, 4 compact exact accumulation, 5 the
- * zero-skipping block walk with exact accumulation, [15] most workgroups
+ * zero-skipping block walk with exact accumulation, 6 exact accumulation in
+ * signed limbs, [15] most workgroups
  * per CU that any of the plan's kernels keeps
  * resident (register and LDS use; each class's grid is capped at its own
  * count; -1 if unknown), [16] table words the zero test of the first job
@@ -230,7 +231,7 @@ int phip_batch_stats(void *plan, double *out, int len);
  * phip_batch_create would plan it: (group * 32 + prime-set size) * 8 +
  * variant (0 plain, 1 paired states, 2 zero-skipping, 3 exact accumulation,
  * 4 compact exact accumulation, 5 zero-skipping block walk with exact
- * accumulation; the set size of an exact class is its limb column count).  Same description
+ * accumulation, 6 exact accumulation in signed limbs; the set size of an exact class is its limb column count).  Same description
  * arguments as phip_batch_create; host code only (no device).  Writes
  * min(len, residues) entries and returns the residue count, or -1.  No
  * counterpart in the reference (plan introspection for tests and
